@@ -1,0 +1,175 @@
+/*
+ * liggghts_amd.h -- C ABI of the MI355X-native DEM step library (libdem_amd.so).
+ *
+ * One handle owns one granular system resident in GPU memory (SoA arrays,
+ * neighbor list, contact histories) and one HIP stream. Calls on one handle
+ * must come from one host thread at a time. Every entry point returns a
+ * dem_status (0 = success, negative = error) instead of exiting the process;
+ * dem_last_error() gives the message. Arrays passed in or out are host
+ * pointers in the caller's particle order (the order of dem_set_particles).
+ *
+ * Each entry point replaces a piece of LIGGGHTS-PUBLIC 3.8.0's plugin seam
+ * for the per-timestep path; the reference interface is cited per function.
+ */
+#ifndef LIGGGHTS_AMD_H
+#define LIGGGHTS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dem_handle dem_handle;
+
+enum dem_status {
+  DEM_OK = 0,
+  DEM_ERR_ARG = -1,      /* invalid argument (error->all in the reference) */
+  DEM_ERR_HIP = -2,      /* a HIP runtime call failed */
+  DEM_ERR_OVERFLOW = -3, /* neighbor list overflow ("boost neigh_modify one") */
+  DEM_ERR_STATE = -4,    /* call out of order (e.g. run before particles) */
+  DEM_ERR_PHYSICS = -5,  /* non-finite state or negative overlap */
+  DEM_ERR_NODEV = -6     /* no usable gfx950 device */
+};
+
+enum dem_precision {
+  DEM_FP64 = 0,  /* every quantity in FP64 */
+  DEM_MIXED = 1  /* FP64 x, v, x_i-x_j and overlap; FP32 contact law, history, omega */
+};
+
+/* Run parameters gathered from the input script:
+ *   timestep                          -> dt            (update.cpp)
+ *   neighbor <skin> bin               -> skin          (neighbor.cpp:1903-1930)
+ *   neigh_modify contact_distance_factor -> cdf
+ *   neigh_modify one <n>              -> max_neighbors (per-particle list width)
+ *   region/create_box + boundary p p f -> boxlo, boxhi, periodic (domain.cpp)
+ *   pair_style gran model hertz tangential history [rolling_friction cdt]
+ *     [tangential_damping on|off] [limitForce on|off] [torsionTorque on|off]
+ *                                     -> rolling_cdt, tangential_damping,
+ *                                        limit_force, torsion_torque
+ *                                        (pair_gran_proxy.cpp:63-79, settings.h:237-242)
+ *   fix gravity <g> vector ax ay az   -> grav_acc = g * a/|a| (fix_gravity.cpp:361-398)
+ */
+typedef struct dem_params {
+  double dt;
+  double skin;
+  double cdf;
+  double boxlo[3];
+  double boxhi[3];
+  int periodic[3];
+  int rolling_cdt;
+  int tangential_damping;
+  int limit_force;
+  int torsion_torque;
+  int has_gravity;
+  double grav_acc[3];
+  int gravity_first;  /* 1: fix gravity defined before fix wall/gran (post_force order) */
+  int precision;      /* enum dem_precision */
+  int max_neighbors;  /* 0 selects 32 */
+  int device;         /* HIP device ordinal */
+} dem_params;
+
+typedef struct dem_stats {
+  int64_t ntimestep;       /* steps run since creation (update->ntimestep) */
+  int64_t nrebuilds;       /* neighbor list builds, setup builds included */
+  int64_t npairs;          /* half-list pairs at the last build */
+  int64_t ncontacts;       /* touching half-list pairs at the last force pass */
+  int64_t nwall_contacts;  /* touching particle-wall contacts at the last force pass */
+  int max_neighbors_seen;  /* widest per-particle list at the last build */
+  double step_kernel_ms;   /* average duration of the fused step kernel (HIP events) */
+  int64_t step_kernel_launches;
+  double rebuild_ms_total; /* wall time inside neighbor rebuilds */
+} dem_stats;
+
+/* Creates a system. Replaces the construction done by input.cpp:560-650 for
+ * atom_style granular + pair_style gran + neighbor + fix nve/sphere. */
+int dem_create(const dem_params *params, dem_handle **out);
+
+/* Frees every device allocation of the handle. */
+int dem_destroy(dem_handle *h);
+
+/* Message of the last failed call on this handle (or of dem_create when h is NULL). */
+const char *dem_last_error(const dem_handle *h);
+
+/* Material properties of fix property/global youngsModulus / poissonsRatio
+ * (peratomtype, ntypes values) and coefficientRestitution / coefficientFriction /
+ * coefficientRollingFriction (peratomtypepair, ntypes*ntypes row-major, mur may
+ * be NULL). Mixed on the host into Y*, G*, beta per type pair exactly as
+ * MODEL_PARAMS::createYeff/createGeff/createBetaEff (global_properties.cpp:411-558),
+ * with the reference's range checks (5e6 <= Y <= 1e9, 0 <= nu <= 0.5,
+ * 0.05 < e <= 1). Replaces PropertyRegistry::connect of Yeff/Geff/betaeff/
+ * coeffFrict/coeffRollFrict (normal_model_hertz.h:146-160). */
+int dem_set_materials(dem_handle *h, int ntypes, const double *youngs, const double *poisson,
+                      const double *restitution, const double *friction,
+                      const double *rolling_friction);
+
+/* Uploads n particles (read_data Atoms + Velocities, atom_vec_sphere.cpp:1055-1140).
+ * x, v, omega are n*3 row-major; rmass is the mass of each sphere; type is 1-based;
+ * tag is the unique positive atom ID. Replaces Atom::x/v/omega/radius/rmass/type/tag. */
+int dem_set_particles(dem_handle *h, int64_t n, const double *x, const double *v,
+                      const double *omega, const double *radius, const double *rmass,
+                      const int *type, const int *tag);
+
+/* Adds a planar primitive wall: fix wall/gran ... primitive type <t> {x,y,z}plane <pos>
+ * (fix_wall_gran.cpp:188-227, primitive_wall_definitions.h:130-150). axis 0/1/2. */
+int dem_add_plane_wall(dem_handle *h, int axis, double position, int wall_type);
+
+/* Verlet::setup (verlet.cpp:134-205): carries contact history out, wraps periodic
+ * coordinates, reorders particles by Morton cell key, rebuilds the neighbor list
+ * with history carry-in, and computes forces with shearupdate = 0. */
+int dem_setup(dem_handle *h);
+
+/* Verlet::run (verlet.cpp:264-392) for nsteps timesteps: nve/sphere initial
+ * integrate, Neighbor::decide (every step, delay 0), rebuild if any particle
+ * moved more than skin/2, pair force (PairGran::compute, pair_gran.cpp:641-670),
+ * post_force fixes (gravity, wall/gran) and nve/sphere final integrate.
+ * Runs dem_setup first when the state changed since the last setup. */
+int dem_run(dem_handle *h, int64_t nsteps);
+
+/* `run N pre no` (run.cpp:116-119, :209): continues integration from the last
+ * force pass without a new setup (no forced rebuild, no shearupdate-0 pass), so
+ * dem_run(a) followed by dem_run_continue(b) equals one run of a+b steps.
+ * Needs a preceding dem_run, dem_run_continue or dem_setup on this state. */
+int dem_run_continue(dem_handle *h, int64_t nsteps);
+
+/* One force evaluation on the current state with the given shearupdate flag,
+ * no integration: Pair::compute + post_force of the wall and gravity fixes.
+ * f and torque become readable with dem_get_state. Histories and contact flags
+ * are updated exactly as in a step. */
+int dem_compute_forces(dem_handle *h, int shearupdate);
+
+/* Copies the state back, in caller order; any pointer may be NULL. f and torque
+ * are those of the last force pass (setup, step or dem_compute_forces). */
+int dem_get_state(dem_handle *h, double *x, double *v, double *omega, double *f,
+                  double *torque);
+
+/* Pairs of the current half neighbor list (NeighList + listgranhistory,
+ * neigh_gran.cpp:485-639): tags[2k] < tags[2k+1] are atom tags, flags[k] the
+ * contact flag, hist[3k..3k+2] the tangential history oriented from the lower
+ * tag to the higher. *npairs receives the number of pairs; min(cap, *npairs)
+ * are written. */
+int dem_get_pairs(dem_handle *h, int64_t cap, int *tags, int *flags, double *hist,
+                  int64_t *npairs);
+
+/* Primitive-wall shear history, n*nwalls*3 values, wall-major inside each particle
+ * (FixWallGran's property/atom history, fix_wall_gran.cpp:498-522). */
+int dem_get_wall_history(dem_handle *h, double *hist);
+
+/* Counters and timing of the handle. */
+int dem_get_stats(dem_handle *h, dem_stats *out);
+
+/* Brackets every step-kernel launch of dem_run with HIP events on the handle's
+ * stream and accumulates their durations into dem_stats (on != 0), or stops
+ * doing so. Resets the accumulated timing either way. No reference counterpart:
+ * the reference's Timer buckets (timer.h:51-52) play this role on the CPU. */
+int dem_set_kernel_timing(dem_handle *h, int on);
+
+/* Runs an input script with the reference's command subset for this path
+ * (Input::file, input.cpp:183, and Input::execute_command, input.cpp:560-650).
+ * log_path may be NULL (log to stdout). Returns a dem_status. */
+int dem_run_script(const char *script_path, const char *log_path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,858 @@
+// HIP kernels of the DEM step for gfx950 (MI355X).
+//
+// Layout: particles live in Morton-cell order (re-sorted at every neighbor
+// rebuild), as packed double4 rows (pos+radius, vel+mass, omega+type) so a
+// neighbor gather is one or two 32-byte loads. The neighbor list is a full
+// list in column-major ELL form (entry k of particle i at k*cap+i), so the 64
+// lanes of a wavefront read consecutive words; bit 31 of an entry is the
+// contact flag. Each direction of a pair keeps its own copy of the tangential
+// history, in the same ELL layout; the two copies are exact mirror images
+// (x_i - x_j is formed so that swapping i and j negates it bit for bit), so the
+// list needs no atomics and every particle's force is summed by one thread.
+//
+// The file is compiled with -ffp-contract=off: in FP64 mode every product and
+// sum rounds as in the CPU reference, so one pair's contribution is bitwise
+// equal to the reference's; only the order of the per-particle sums differs.
+#include "dem_device.h"
+
+namespace dem {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b%8).
+// Remap so that each XCD walks one contiguous range of Morton-ordered
+// particles: neighbors of a particle then sit in the same XCD's L2.
+__device__ __forceinline__ int xcd_block(int nblocks) {
+  const int b = blockIdx.x;
+  const int xcd = b & 7, slot = b >> 3;
+  const int q = nblocks >> 3, r = nblocks & 7;
+  return xcd * q + (xcd < r ? xcd : r) + slot;
+}
+
+__device__ __forceinline__ int cell_coord(const DevParams &p, int d, double x) {
+  int c = (int)floor((x - p.lo[d]) * p.cinv[d]);
+  c = c < 0 ? 0 : c;
+  c = c >= p.nc[d] ? p.nc[d] - 1 : c;
+  return c;
+}
+
+// x_i - x_j with the minimum-image shift applied to the difference; exactly
+// antisymmetric under i <-> j.
+__device__ __forceinline__ void min_image(const DevParams &p, double xi, double yi, double zi,
+                                          double xj, double yj, double zj, double &dx,
+                                          double &dy, double &dz) {
+  dx = xi - xj;
+  dy = yi - yj;
+  dz = zi - zj;
+  if (p.per[0]) {
+    if (dx > p.halfL[0]) dx -= p.L[0];
+    else if (dx < -p.halfL[0]) dx += p.L[0];
+  }
+  if (p.per[1]) {
+    if (dy > p.halfL[1]) dy -= p.L[1];
+    else if (dy < -p.halfL[1]) dy += p.L[1];
+  }
+  if (p.per[2]) {
+    if (dz > p.halfL[2]) dz -= p.L[2];
+    else if (dz < -p.halfL[2]) dz += p.L[2];
+  }
+}
+
+template <typename T> struct OmgT;
+template <> struct OmgT<double> { typedef double4 type; };
+template <> struct OmgT<float> { typedef float4 type; };
+
+// ---------------------------------------------------------------------------
+// contact law: SurfaceModel<DEFAULT> (surface_model_default.h:140-205) ->
+// NormalModel<HERTZ> (normal_model_hertz.h:202-405) -> TangentialModel<HISTORY>
+// (tangential_model_history.h:130-356) -> RollingModel<CDT>
+// (rolling_model_cdt.h:89-167). deltan is formed by the caller in FP64.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct CIn {
+  T radi, radj, rinv, meff, deltan, deltan_in;
+  T en[3], del[3];
+  T vr[3];
+  T wi[3], wj[3];
+  T Y, G, beta, mu, mur;
+};
+
+template <typename T, bool WALL>
+__device__ __forceinline__ void contact_chain(const DevParams &p, const CIn<T> &c, T dt,
+                                              int shearupdate, T sh[3], T dF[3], T dTi[3],
+                                              T dTj[3]) {
+  const T enx = c.en[0], eny = c.en[1], enz = c.en[2];
+  const T vr1 = c.vr[0], vr2 = c.vr[1], vr3 = c.vr[2];
+  const T vn = vr1 * enx + vr2 * eny + vr3 * enz;
+  const T vn1 = vn * enx, vn2 = vn * eny, vn3 = vn * enz;
+  const T vt1 = vr1 - vn1, vt2 = vr2 - vn2, vt3 = vr3 - vn3;
+  const T deltan = c.deltan;
+  const T dx = c.del[0], dy = c.del[1], dz = c.del[2];
+  const T rinv = c.rinv;
+  T wr1, wr2, wr3, cri, crj = T(0);
+  if (WALL) {
+    const T cr = c.radi - T(0.5) * c.deltan_in;
+    wr1 = cr * c.wi[0] * rinv;
+    wr2 = cr * c.wi[1] * rinv;
+    wr3 = cr * c.wi[2] * rinv;
+    cri = cr;
+  } else {
+    cri = c.radi - T(0.5) * deltan;
+    crj = c.radj - T(0.5) * deltan;
+    wr1 = (cri * c.wi[0] + crj * c.wj[0]) * rinv;
+    wr2 = (cri * c.wi[1] + crj * c.wj[1]) * rinv;
+    wr3 = (cri * c.wi[2] + crj * c.wj[2]) * rinv;
+  }
+  const T vtr1 = vt1 - (dz * wr2 - dy * wr3);
+  const T vtr2 = vt2 - (dx * wr3 - dz * wr1);
+  const T vtr3 = vt3 - (dy * wr1 - dx * wr2);
+
+  // Hertz
+  const T reff = WALL ? c.radi : (c.radi * c.radj / (c.radi + c.radj));
+  const T sqrtval = sqrt(reff * deltan);
+  const T Sn = T(2.) * c.Y * sqrtval;
+  const T St = T(8.) * c.G * sqrtval;
+  const T kn = T(4.) / T(3.) * c.Y * sqrtval;
+  const T kt = St;
+  const T s56 = T(0.91287092917527685576161630466800355658790782499663875);
+  const T gamman = T(-2.) * s56 * c.beta * sqrt(Sn * c.meff);
+  const T gammat = p.tdamp ? T(-2.) * s56 * c.beta * sqrt(St * c.meff) : T(0);
+  const T Fn_damping = -gamman * vn;
+  const T Fn_contact = kn * deltan;
+  T Fn = Fn_damping + Fn_contact;
+  if (p.limitf && Fn < T(0)) Fn = T(0);
+  dF[0] = Fn * enx;
+  dF[1] = Fn * eny;
+  dF[2] = Fn * enz;
+
+  // tangential history
+  if (shearupdate) {
+    sh[0] += vtr1 * dt;
+    sh[1] += vtr2 * dt;
+    sh[2] += vtr3 * dt;
+    const T rsht = sh[0] * enx + sh[1] * eny + sh[2] * enz;
+    sh[0] -= rsht * enx;
+    sh[1] -= rsht * eny;
+    sh[2] -= rsht * enz;
+  }
+  const T shrmag = sqrt(sh[0] * sh[0] + sh[1] * sh[1] + sh[2] * sh[2]);
+  T Ft1 = -(kt * sh[0]);
+  T Ft2 = -(kt * sh[1]);
+  T Ft3 = -(kt * sh[2]);
+  const T Ft_shear = kt * shrmag;
+  const T Ft_friction = c.mu * fabs(Fn);
+  if (Ft_shear > Ft_friction) {
+    if (shrmag != T(0)) {
+      const T ratio = Ft_friction / Ft_shear;
+      Ft1 *= ratio;
+      Ft2 *= ratio;
+      Ft3 *= ratio;
+      if (shearupdate) {
+        sh[0] = -Ft1 / kt;
+        sh[1] = -Ft2 / kt;
+        sh[2] = -Ft3 / kt;
+      }
+    } else {
+      Ft1 = Ft2 = Ft3 = T(0);
+    }
+  } else {
+    Ft1 -= (gammat * vtr1);
+    Ft2 -= (gammat * vtr2);
+    Ft3 -= (gammat * vtr3);
+  }
+  const T tor1 = eny * Ft3 - enz * Ft2;
+  const T tor2 = enz * Ft1 - enx * Ft3;
+  const T tor3 = enx * Ft2 - eny * Ft1;
+  dF[0] += Ft1;
+  dF[1] += Ft2;
+  dF[2] += Ft3;
+  dTi[0] = T(0) + -cri * tor1;
+  dTi[1] = T(0) + -cri * tor2;
+  dTi[2] = T(0) + -cri * tor3;
+  if (!WALL) {
+    dTj[0] = T(0) + -crj * tor1;
+    dTj[1] = T(0) + -crj * tor2;
+    dTj[2] = T(0) + -crj * tor3;
+  }
+
+  // rolling friction, constant directional torque
+  if (p.rolling) {
+    const T rmu = c.mur;
+    T rt0 = T(0), rt1 = T(0), rt2 = T(0);
+    if (WALL) {
+      const T wrmag = sqrt(wr1 * wr1 + wr2 * wr2 + wr3 * wr3);
+      if (wrmag > T(0)) {
+        const T Fnw = deltan * kn;
+        rt0 = rmu * Fnw * wr1 / wrmag * reff;
+        rt1 = rmu * Fnw * wr2 / wrmag * reff;
+        rt2 = rmu * Fnw * wr3 / wrmag * reff;
+      }
+    } else {
+      const T w0 = c.wi[0] - c.wj[0], w1 = c.wi[1] - c.wj[1], w2 = c.wi[2] - c.wj[2];
+      const T mag = sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+      if (mag > T(0)) {
+        const T fac = rmu * kn * deltan * reff / mag;
+        rt0 = w0 * fac;
+        rt1 = w1 * fac;
+        rt2 = w2 * fac;
+      }
+    }
+    if (!p.torsion) {
+      const T d = rt0 * enx + rt1 * eny + rt2 * enz;
+      const T n0 = enx * d, n1 = eny * d, n2 = enz * d;
+      rt0 = rt0 - n0;
+      rt1 = rt1 - n1;
+      rt2 = rt2 - n2;
+    }
+    dTi[0] -= rt0;
+    dTi[1] -= rt1;
+    dTi[2] -= rt2;
+    if (!WALL) {
+      dTj[0] += rt0;
+      dTj[1] += rt1;
+      dTj[2] += rt2;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fused step kernel: pair forces over the full list, plane walls, gravity,
+// nve/sphere final integrate of this step and initial integrate of the next,
+// and the skin/2 displacement check of Neighbor::decide for the next step.
+// One thread per particle; reads the state buffer `in`, writes `out`.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
+  typedef typename OmgT<T>::type omg_t;
+  if (a.step >= a.ctrl->stop_at) return;
+  const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
+  const bool active = i < a.n;
+  const DevParams &p = a.prm;
+  unsigned long long my_contacts = 0, my_wall = 0;
+  bool displaced = false;
+
+  if (active) {
+    const double4 pi = a.in.pos[i];
+    const double4 vi = a.in.vel[i];
+    const omg_t wi = ((const omg_t *)a.in.omg)[i];
+    const int itype = (int)wi.w;
+    const int meta0 = a.aux.meta[i];
+    int meta = meta0;
+    const int cnt = meta & kMetaCountMask;
+    const int ntp1 = p.ntp1;
+    const T dt = (T)p.dt;
+    T fx = T(0), fy = T(0), fz = T(0), tx = T(0), ty = T(0), tz = T(0);
+    T *hist = (T *)a.list.hist;
+    const size_t cap = (size_t)a.cap;
+
+    for (int k = 0; k < cnt; k++) {
+      const size_t slot = (size_t)k * cap + i;
+      const uint32_t e = a.list.nbr[slot];
+      const int j = (int)(e & kIndexMask);
+      const double4 pj = a.in.pos[j];
+      double dx, dy, dz;
+      min_image(p, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, dx, dy, dz);
+      const double rsq = dx * dx + dy * dy + dz * dz;
+      const double radsum = pi.w + pj.w;
+      if (!(rsq < radsum * radsum)) continue;
+      // touching: pair_gran_base.h:356-430
+      const double4 vj = a.in.vel[j];
+      const omg_t wj = ((const omg_t *)a.in.omg)[j];
+      const int jtype = (int)wj.w;
+      const double r = sqrt(rsq);
+      const double rinv = 1.0 / r;
+      CIn<T> c;
+      c.radi = (T)pi.w;
+      c.radj = (T)pj.w;
+      c.rinv = (T)rinv;
+      c.meff = (T)(vi.w * vj.w / (vi.w + vj.w));
+      c.deltan = (T)(radsum - r);
+      c.deltan_in = T(0);
+      c.en[0] = (T)(dx * rinv);
+      c.en[1] = (T)(dy * rinv);
+      c.en[2] = (T)(dz * rinv);
+      c.del[0] = (T)dx;
+      c.del[1] = (T)dy;
+      c.del[2] = (T)dz;
+      c.vr[0] = (T)(vi.x - vj.x);
+      c.vr[1] = (T)(vi.y - vj.y);
+      c.vr[2] = (T)(vi.z - vj.z);
+      c.wi[0] = (T)wi.x;
+      c.wi[1] = (T)wi.y;
+      c.wi[2] = (T)wi.z;
+      c.wj[0] = (T)wj.x;
+      c.wj[1] = (T)wj.y;
+      c.wj[2] = (T)wj.z;
+      const int mt = itype * ntp1 + jtype;
+      if (sizeof(T) == 8) {
+        c.Y = (T)a.mat.Yeff[mt];
+        c.G = (T)a.mat.Geff[mt];
+        c.beta = (T)a.mat.beta[mt];
+        c.mu = (T)a.mat.mu[mt];
+        c.mur = (T)a.mat.mur[mt];
+      } else {
+        c.Y = (T)a.mat.fYeff[mt];
+        c.G = (T)a.mat.fGeff[mt];
+        c.beta = (T)a.mat.fbeta[mt];
+        c.mu = (T)a.mat.fmu[mt];
+        c.mur = (T)a.mat.fmur[mt];
+      }
+      const bool flagged = (e & kFlagBit) != 0;
+      T sh[3];
+      const size_t hb = (size_t)k * 3 * cap + i;
+      if (flagged) {
+        sh[0] = hist[hb];
+        sh[1] = hist[hb + cap];
+        sh[2] = hist[hb + 2 * cap];
+      } else {
+        sh[0] = sh[1] = sh[2] = T(0);
+      }
+      T dF[3], dTi[3], dTj[3];
+      contact_chain<T, false>(p, c, dt, a.shearupdate, sh, dF, dTi, dTj);
+      if (a.shearupdate || !flagged) {
+        hist[hb] = sh[0];
+        hist[hb + cap] = sh[1];
+        hist[hb + 2 * cap] = sh[2];
+      }
+      if (!flagged) a.list.nbr[slot] = e | kFlagBit;
+      fx += dF[0];
+      fy += dF[1];
+      fz += dF[2];
+      tx += dTi[0];
+      ty += dTi[1];
+      tz += dTi[2];
+      my_contacts++;
+    }
+
+    // post_force fixes in script order: gravity (fix_gravity.cpp:331-338) and
+    // plane walls (fix_wall_gran.cpp:1024-1153)
+    for (int pass = 0; pass < 2; pass++) {
+      const bool grav_pass = (pass == 0) == (p.grav_first != 0);
+      if (grav_pass) {
+        if (p.has_grav) {
+          fx += (T)(vi.w * p.gacc[0]);
+          fy += (T)(vi.w * p.gacc[1]);
+          fz += (T)(vi.w * p.gacc[2]);
+        }
+        continue;
+      }
+      for (int w = 0; w < p.nwalls; w++) {
+        const int ax = p.wall_axis[w];
+        const double param = p.wall_pos[w];
+        const double xa = ax == 0 ? pi.x : (ax == 1 ? pi.y : pi.z);
+        double deltan, dlt;
+        if (xa > param) {
+          dlt = param - xa;
+          deltan = xa - param - pi.w;
+        } else {
+          dlt = param - xa;
+          deltan = param - xa - pi.w;
+        }
+        T *wh = (T *)a.aux.whist;
+        const size_t wb = (size_t)w * 3 * cap + i;
+        const int bit = 1 << (kMetaWallShift + w);
+        if (deltan > p.cutneighmax) continue;
+        if (deltan <= 0) {
+          // sidata.deltan = -deltan, sidata.delta = -delta; compute_force
+          // (fix_wall_gran_base.h:179-188): r = radi - sidata.deltan
+          const double sdn = -deltan;
+          const double sd = -dlt;
+          const double r = pi.w - sdn;
+          const double rinv = 1.0 / r;
+          CIn<T> c;
+          c.radi = (T)pi.w;
+          c.radj = T(0);
+          c.rinv = (T)rinv;
+          c.meff = (T)vi.w;
+          c.deltan = (T)(pi.w - r);
+          c.deltan_in = (T)sdn;
+          c.del[0] = c.del[1] = c.del[2] = T(0);
+          c.del[ax] = (T)sd;
+          c.en[0] = (T)(0.0 * rinv);
+          c.en[1] = (T)(0.0 * rinv);
+          c.en[2] = (T)(0.0 * rinv);
+          c.en[ax] = (T)(sd * rinv);
+          c.vr[0] = (T)(vi.x - 0.0);
+          c.vr[1] = (T)(vi.y - 0.0);
+          c.vr[2] = (T)(vi.z - 0.0);
+          c.wi[0] = (T)wi.x;
+          c.wi[1] = (T)wi.y;
+          c.wi[2] = (T)wi.z;
+          c.wj[0] = c.wj[1] = c.wj[2] = T(0);
+          const int mt = itype * ntp1 + p.wall_type[w];
+          if (sizeof(T) == 8) {
+            c.Y = (T)a.mat.Yeff[mt];
+            c.G = (T)a.mat.Geff[mt];
+            c.beta = (T)a.mat.beta[mt];
+            c.mu = (T)a.mat.mu[mt];
+            c.mur = (T)a.mat.mur[mt];
+          } else {
+            c.Y = (T)a.mat.fYeff[mt];
+            c.G = (T)a.mat.fGeff[mt];
+            c.beta = (T)a.mat.fbeta[mt];
+            c.mu = (T)a.mat.fmu[mt];
+            c.mur = (T)a.mat.fmur[mt];
+          }
+          T sh[3];
+          if (meta & bit) {
+            sh[0] = wh[wb];
+            sh[1] = wh[wb + cap];
+            sh[2] = wh[wb + 2 * cap];
+          } else {
+            sh[0] = sh[1] = sh[2] = T(0);
+          }
+          T dF[3], dTi[3], dTj[3];
+          contact_chain<T, true>(p, c, dt, a.shearupdate, sh, dF, dTi, dTj);
+          wh[wb] = sh[0];
+          wh[wb + cap] = sh[1];
+          wh[wb + 2 * cap] = sh[2];
+          meta |= bit;
+          fx += dF[0];
+          fy += dF[1];
+          fz += dF[2];
+          tx += dTi[0];
+          ty += dTi[1];
+          tz += dTi[2];
+          my_wall++;
+        } else {
+          meta &= ~bit;  // history zeroed when the contact ends (fix_wall_gran.cpp:1145-1151)
+        }
+      }
+    }
+
+    if (a.store_f) {
+      a.fout[i] = (double)fx;
+      a.fout[cap + i] = (double)fy;
+      a.fout[2 * cap + i] = (double)fz;
+      a.fout[3 * cap + i] = (double)tx;
+      a.fout[4 * cap + i] = (double)ty;
+      a.fout[5 * cap + i] = (double)tz;
+    }
+
+    // nve/sphere: final integrate of this step (fix_nve_sphere.cpp:208-246) and
+    // initial integrate of the next (fix_nve_sphere.cpp:138-206)
+    if (a.do_final || a.do_initial) {
+      double x = pi.x, y = pi.y, z = pi.z;
+      double vx = vi.x, vy = vi.y, vz = vi.z;
+      const double dtfm = p.dtf / (vi.w * 1.0);
+      const double dtirotate = p.dtfrotate / (pi.w * pi.w * vi.w);
+      const double Fx = (double)fx, Fy = (double)fy, Fz = (double)fz;
+      T wx = (T)wi.x, wy = (T)wi.y, wz = (T)wi.z;
+      const T dtir = (T)dtirotate;
+      if (a.do_final) {
+        vx += dtfm * Fx;
+        vy += dtfm * Fy;
+        vz += dtfm * Fz;
+        wx += dtir * tx;
+        wy += dtir * ty;
+        wz += dtir * tz;
+      }
+      if (a.do_initial) {
+        vx += dtfm * Fx;
+        vy += dtfm * Fy;
+        vz += dtfm * Fz;
+        x += p.dt * vx;
+        y += p.dt * vy;
+        z += p.dt * vz;
+        wx += dtir * tx;
+        wy += dtir * ty;
+        wz += dtir * tz;
+        // Neighbor::check_distance, neighbor.cpp:1436-1443
+        const double hx = x - a.xhold[i];
+        const double hy = y - a.xhold[cap + i];
+        const double hz = z - a.xhold[2 * cap + i];
+        const double rsq = hx * hx + hy * hy + hz * hz;
+        displaced = rsq > p.triggersq;
+      }
+      a.out.pos[i] = make_double4(x, y, z, pi.w);
+      a.out.vel[i] = make_double4(vx, vy, vz, vi.w);
+      omg_t wo;
+      wo.x = wx;
+      wo.y = wy;
+      wo.z = wz;
+      wo.w = wi.w;
+      ((omg_t *)a.out.omg)[i] = wo;
+    }
+    if (meta != meta0) a.aux.meta[i] = meta;
+  }
+
+  // rebuild request for the next step
+  if (__ballot(displaced) != 0ull) {
+    if ((threadIdx.x & 63) == 0)
+      atomicMin((unsigned long long *)&a.ctrl->stop_at, (unsigned long long)(a.step + 1));
+  }
+  if (a.count) {
+    __shared__ unsigned long long sc[kBlock / 64], sw[kBlock / 64];
+    unsigned long long c = my_contacts, w = my_wall;
+    for (int off = 32; off > 0; off >>= 1) {
+      c += __shfl_xor(c, off, 64);
+      w += __shfl_xor(w, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      sc[threadIdx.x >> 6] = c;
+      sw[threadIdx.x >> 6] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long tc = 0, tw = 0;
+      for (int k = 0; k < kBlock / 64; k++) {
+        tc += sc[k];
+        tw += sw[k];
+      }
+      if (tc) atomicAdd(&a.ctrl->ncontact_full, tc);
+      if (tw) atomicAdd(&a.ctrl->nwall, tw);
+    }
+  }
+}
+
+// initial integrate of step 1 after setup (the setup force pass has no
+// final integrate), plus the displacement check for step 1. In place.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_initial(StepArgs a) {
+  typedef typename OmgT<T>::type omg_t;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  bool displaced = false;
+  if (i < a.n) {
+    const DevParams &p = a.prm;
+    const size_t cap = (size_t)a.cap;
+    const double4 pi = a.in.pos[i];
+    const double4 vi = a.in.vel[i];
+    omg_t wi = ((omg_t *)a.in.omg)[i];
+    const double dtfm = p.dtf / (vi.w * 1.0);
+    const double dtirotate = p.dtfrotate / (pi.w * pi.w * vi.w);
+    const T dtir = (T)dtirotate;
+    const double vx = vi.x + dtfm * a.fout[i];
+    const double vy = vi.y + dtfm * a.fout[cap + i];
+    const double vz = vi.z + dtfm * a.fout[2 * cap + i];
+    const double x = pi.x + p.dt * vx, y = pi.y + p.dt * vy, z = pi.z + p.dt * vz;
+    wi.x += dtir * (T)a.fout[3 * cap + i];
+    wi.y += dtir * (T)a.fout[4 * cap + i];
+    wi.z += dtir * (T)a.fout[5 * cap + i];
+    a.in.pos[i] = make_double4(x, y, z, pi.w);
+    a.in.vel[i] = make_double4(vx, vy, vz, vi.w);
+    ((omg_t *)a.in.omg)[i] = wi;
+    const double hx = x - a.xhold[i];
+    const double hy = y - a.xhold[cap + i];
+    const double hz = z - a.xhold[2 * cap + i];
+    displaced = hx * hx + hy * hy + hz * hz > p.triggersq;
+  }
+  if (__ballot(displaced) != 0ull && (threadIdx.x & 63) == 0)
+    atomicMin((unsigned long long *)&a.ctrl->stop_at, 1ull);
+}
+
+__global__ void k_set_stop(Ctrl *ctrl, long long v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctrl->stop_at = v;
+    ctrl->ncontact_full = 0;
+    ctrl->nwall = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// rebuild: Domain::pbc (domain.cpp:530-620) + Morton-cell counting sort +
+// gather + full neighbor list with history carry-in (neigh_gran.cpp:485-639,
+// fix_contact_history.cpp:305-425)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_wrap_keys(RebuildArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const DevParams &p = a.prm;
+  double4 q = a.cur.pos[i];
+  double xs[3] = {q.x, q.y, q.z};
+  bool changed = false;
+  for (int d = 0; d < 3; d++) {
+    if (!p.per[d]) continue;
+    const double lo = p.lo[d], hi = p.lo[d] + p.L[d];
+    if (xs[d] < lo) {
+      xs[d] += p.L[d];
+      changed = true;
+    }
+    if (xs[d] >= hi) {
+      xs[d] -= p.L[d];
+      if (xs[d] < lo) xs[d] = lo;
+      changed = true;
+    }
+  }
+  if (changed) {
+    q.x = xs[0];
+    q.y = xs[1];
+    q.z = xs[2];
+    a.cur.pos[i] = q;
+  }
+  const int cx = cell_coord(p, 0, xs[0]);
+  const int cy = cell_coord(p, 1, xs[1]);
+  const int cz = cell_coord(p, 2, xs[2]);
+  const int lin = cx + p.nc[0] * (cy + p.nc[1] * cz);
+  const int key = a.cell_rank[lin];
+  a.key[i] = key;
+  atomicAdd(&a.cell_count[key], 1);
+}
+
+// exclusive scan, three passes (block scan, scan of block sums, add)
+constexpr int kScanPer = 1024;  // elements per scan block (256 threads x 4)
+
+__global__ __launch_bounds__(256) void k_scan_block(const int *in, int *out, int *bsum, int n) {
+  __shared__ int s[256];
+  const int base = blockIdx.x * kScanPer + threadIdx.x * 4;
+  int v[4];
+  int t = 0;
+  for (int k = 0; k < 4; k++) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    t += v[k];
+  }
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int add = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += add;
+    __syncthreads();
+  }
+  int run = s[threadIdx.x] - t;  // exclusive prefix of this thread
+  for (int k = 0; k < 4; k++) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 255) bsum[blockIdx.x] = s[255];
+}
+
+__global__ __launch_bounds__(256) void k_scan_top(int *bsum, int nb, int *total) {
+  __shared__ int s[256];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 256) {
+    const int idx = base + threadIdx.x;
+    const int v = idx < nb ? bsum[idx] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      const int add = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (idx < nb) bsum[idx] = carry + s[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 255) carry += s[255];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(int *out, const int *bsum, int n) {
+  const int base = blockIdx.x * kScanPer + threadIdx.x * 4;
+  const int add = bsum[blockIdx.x];
+  for (int k = 0; k < 4; k++)
+    if (base + k < n) out[base + k] += add;
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero(int *p, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(RebuildArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const int key = a.key[i];
+  const int slot = a.cell_start[key] + atomicAdd(&a.cell_count[key], 1);
+  a.perm[slot] = i;
+}
+
+// order the particles of each cell by tag, so the layout is deterministic
+__global__ __launch_bounds__(kBlock) void k_cell_sort(RebuildArgs a) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= a.ncell) return;
+  const int s = a.cell_start[c], e = a.cell_start[c + 1];
+  for (int k = s + 1; k < e; k++) {
+    const int v = a.perm[k];
+    const int tv = a.acur.tag[v];
+    int m = k - 1;
+    while (m >= s && a.acur.tag[a.perm[m]] > tv) {
+      a.perm[m + 1] = a.perm[m];
+      m--;
+    }
+    a.perm[m + 1] = v;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gather(RebuildArgs a) {
+  typedef typename OmgT<T>::type omg_t;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const int o = a.perm[i];
+  const size_t cap = (size_t)a.cap;
+  const double4 q = a.cur.pos[o];
+  a.alt.pos[i] = q;
+  a.alt.vel[i] = a.cur.vel[o];
+  ((omg_t *)a.alt.omg)[i] = ((const omg_t *)a.cur.omg)[o];
+  a.aalt.tag[i] = a.acur.tag[o];
+  a.aalt.orig[i] = a.acur.orig[o];
+  const int meta = a.acur.meta[o];
+  a.aalt.meta[i] = meta & ~kMetaCountMask;
+  const T *wi = (const T *)a.acur.whist;
+  T *wo = (T *)a.aalt.whist;
+  for (int w = 0; w < a.prm.nwalls; w++) {
+    if (!(meta & (1 << (kMetaWallShift + w)))) continue;
+    for (int d = 0; d < 3; d++) wo[(size_t)(w * 3 + d) * cap + i] = wi[(size_t)(w * 3 + d) * cap + o];
+  }
+  a.xhold[i] = q.x;
+  a.xhold[cap + i] = q.y;
+  a.xhold[2 * cap + i] = q.z;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_build(RebuildArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const bool active = i < a.n;
+  int cnt = 0;
+  if (active) {
+    const DevParams &p = a.prm;
+    const size_t cap = (size_t)a.cap;
+    const double4 pi = a.alt.pos[i];
+    const int io = a.perm[i];
+    const int oldcnt = a.have_old ? (a.acur.meta[io] & kMetaCountMask) : 0;
+    const int ci[3] = {cell_coord(p, 0, pi.x), cell_coord(p, 1, pi.y), cell_coord(p, 2, pi.z)};
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+      if (p.nc[d] == 1) {
+        lo[d] = hi[d] = 0;
+      } else if (p.per[d]) {
+        lo[d] = -1;
+        hi[d] = 1;
+      } else {
+        lo[d] = ci[d] > 0 ? -1 : 0;
+        hi[d] = ci[d] < p.nc[d] - 1 ? 1 : 0;
+      }
+    }
+    const T *ohist = (const T *)a.oldl.hist;
+    T *nhist = (T *)a.newl.hist;
+    for (int dz = lo[2]; dz <= hi[2]; dz++)
+      for (int dy = lo[1]; dy <= hi[1]; dy++)
+        for (int dx = lo[0]; dx <= hi[0]; dx++) {
+          const int cx = (ci[0] + dx + p.nc[0]) % p.nc[0];
+          const int cy = (ci[1] + dy + p.nc[1]) % p.nc[1];
+          const int cz = (ci[2] + dz + p.nc[2]) % p.nc[2];
+          const int rank = a.cell_rank[cx + p.nc[0] * (cy + p.nc[1] * cz)];
+          const int s = a.cell_start[rank], e = a.cell_start[rank + 1];
+          for (int j = s; j < e; j++) {
+            if (j == i) continue;
+            const double4 pj = a.alt.pos[j];
+            double ddx, ddy, ddz;
+            min_image(p, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, ddx, ddy, ddz);
+            const double rsq = ddx * ddx + ddy * ddy + ddz * ddz;
+            const double radsum = (pi.w + pj.w) * p.cdf;
+            const double cutsq = (radsum + p.skin) * (radsum + p.skin);
+            if (!(rsq <= cutsq)) continue;
+            if (cnt < a.W) {
+              uint32_t entry = (uint32_t)j;
+              if (rsq < radsum * radsum && oldcnt > 0) {
+                // carry-in: neigh_gran.cpp:585-607 (partner found -> flag 1, copy)
+                const uint32_t jo = (uint32_t)a.perm[j];
+                for (int m = 0; m < oldcnt; m++) {
+                  const uint32_t oe = a.oldl.nbr[(size_t)m * cap + io];
+                  if ((oe & kFlagBit) && (oe & kIndexMask) == jo) {
+                    const size_t ob = (size_t)m * 3 * cap + io;
+                    const size_t nb = (size_t)cnt * 3 * cap + i;
+                    nhist[nb] = ohist[ob];
+                    nhist[nb + cap] = ohist[ob + cap];
+                    nhist[nb + 2 * cap] = ohist[ob + 2 * cap];
+                    entry |= kFlagBit;
+                    break;
+                  }
+                }
+              }
+              a.newl.nbr[(size_t)cnt * cap + i] = entry;
+            }
+            cnt++;
+          }
+        }
+    const int stored = cnt < a.W ? cnt : a.W;
+    a.aalt.meta[i] = (a.aalt.meta[i] & ~kMetaCountMask) | stored;
+    if (cnt > a.W) atomicMax(&a.ctrl->overflow, cnt);
+  }
+  // widest list and entry count, one atomic per wavefront
+  int wmax = cnt;
+  unsigned long long wsum = (unsigned long long)cnt;
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(wmax, off, 64);
+    wmax = o > wmax ? o : wmax;
+    wsum += __shfl_xor(wsum, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && wsum) {
+    atomicMax(&a.ctrl->maxcount, wmax);
+    atomicAdd(&a.ctrl->npairs_full, wsum);
+  }
+}
+
+__global__ void k_reset_build_counters(Ctrl *ctrl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctrl->overflow = 0;
+    ctrl->maxcount = 0;
+    ctrl->npairs_full = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+int scan_tmp_size(int ncell) { return nblk(ncell, kScanPer) + 16; }
+
+hipError_t launch_rebuild(const RebuildArgs &a, hipStream_t s) {
+  const int nb = nblk(a.n, kBlock);
+  const int ncb = nblk(a.ncell, kBlock);
+  hipLaunchKernelGGL(k_reset_build_counters, dim3(1), dim3(64), 0, s, a.ctrl);
+  hipLaunchKernelGGL(k_zero, dim3(ncb), dim3(kBlock), 0, s, a.cell_count, a.ncell);
+  if (a.n > 0) hipLaunchKernelGGL(k_wrap_keys, dim3(nb), dim3(kBlock), 0, s, a);
+  const int nsb = nblk(a.ncell, kScanPer);
+  hipLaunchKernelGGL(k_scan_block, dim3(nsb), dim3(256), 0, s, a.cell_count, a.cell_start,
+                     a.scan_tmp, a.ncell);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, a.scan_tmp, nsb,
+                     a.cell_start + a.ncell);
+  hipLaunchKernelGGL(k_scan_add, dim3(nsb), dim3(256), 0, s, a.cell_start, a.scan_tmp, a.ncell);
+  hipLaunchKernelGGL(k_zero, dim3(ncb), dim3(kBlock), 0, s, a.cell_count, a.ncell);
+  if (a.n > 0) {
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_cell_sort, dim3(ncb), dim3(kBlock), 0, s, a);
+    if (a.mixed) {
+      hipLaunchKernelGGL(k_gather<float>, dim3(nb), dim3(kBlock), 0, s, a);
+      hipLaunchKernelGGL(k_build<float>, dim3(nb), dim3(kBlock), 0, s, a);
+    } else {
+      hipLaunchKernelGGL(k_gather<double>, dim3(nb), dim3(kBlock), 0, s, a);
+      hipLaunchKernelGGL(k_build<double>, dim3(nb), dim3(kBlock), 0, s, a);
+    }
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_step(const StepArgs &a, hipStream_t s) {
+  const int nb = nblk(a.n > 0 ? a.n : 1, kBlock);
+  if (a.mixed)
+    hipLaunchKernelGGL(k_step<float>, dim3(nb), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_step<double>, dim3(nb), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_initial(const StepArgs &a, hipStream_t s) {
+  if (a.n <= 0) return hipSuccess;
+  const int nb = nblk(a.n, kBlock);
+  if (a.mixed)
+    hipLaunchKernelGGL(k_initial<float>, dim3(nb), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_initial<double>, dim3(nb), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_stop(Ctrl *ctrl, long long v, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_stop, dim3(1), dim3(64), 0, s, ctrl, v);
+  return hipGetLastError();
+}
+
+}  // namespace dem

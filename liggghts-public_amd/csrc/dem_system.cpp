@@ -1,0 +1,829 @@
+// Host side of the DEM step library: the C ABI of include/liggghts_amd.h.
+//
+// A handle owns device buffers for one granular system and drives the
+// per-timestep sequence of Verlet::run (verlet.cpp:264-392):
+//   setup : carry history out, wrap, Morton sort, neighbor build, force pass
+//   steps : fused [pair + walls + gravity + final integrate + next initial
+//           integrate + displacement check] kernel, one launch per step.
+// Steps are enqueued ahead in batches; a kernel whose step index reaches the
+// control block's stop_at (raised by the previous step's displacement check)
+// returns at once, and the host rebuilds before re-launching that step. The
+// host therefore never waits on the GPU between two steps that need no rebuild.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/liggghts_amd.h"
+#include "dem_device.h"
+
+using dem::AuxBuf;
+using dem::Ctrl;
+using dem::ListBuf;
+using dem::StateBuf;
+
+namespace {
+
+constexpr long long kNever = 0x3fffffffffffffffLL;
+constexpr double kMyPi = 3.14159265358979323846;  // math_const.h
+
+struct Wall {
+  int axis;
+  double pos;
+  int type;
+};
+
+std::string g_create_error;
+
+}  // namespace
+
+struct dem_handle {
+  dem_params p{};
+  std::string err;
+  hipStream_t stream = nullptr;
+  bool mixed = false;
+  size_t fsz = 8;  // bytes of one history / omega component
+
+  // materials
+  int ntypes = 0;
+  bool mat_set = false;
+  std::vector<double> hYeff, hGeff, hbeta, hmu, hmur;  // (ntypes+1)^2
+  dem::MatTables mat{};
+  void *mat_block = nullptr;
+
+  std::vector<Wall> walls;
+
+  // particles
+  int64_t n = 0;
+  int cap = 0;
+  int W = 32;
+  StateBuf st[2]{};
+  AuxBuf aux[2]{};
+  ListBuf lst[2]{};
+  int cur_st = 0, cur_aux = 0, cur_l = 0;
+  bool have_list = false;
+  bool setup_done = false;
+  bool can_continue = false;  // fout holds the last force pass in the current order
+  double *xhold = nullptr, *fout = nullptr;
+  double rmax = 0.0;
+
+  // cell grid
+  int nc[3] = {0, 0, 0};
+  int ncell = 0;
+  double cutneighmax = 0.0;
+  int *cell_rank = nullptr, *cell_count = nullptr, *cell_start = nullptr;
+  int *key = nullptr, *perm = nullptr, *scan_tmp = nullptr;
+
+  Ctrl *ctrl = nullptr;
+  Ctrl *ctrl_host = nullptr;
+
+  // counters
+  int64_t ntimestep = 0, nrebuilds = 0;
+  int64_t npairs = 0, ncontacts = 0, nwall_contacts = 0;
+  int maxcount = 0;
+  bool timing = false;
+  double kernel_ms = 0.0;
+  int64_t kernel_launches = 0;
+  double rebuild_ms = 0.0;
+  std::vector<hipEvent_t> ev;
+
+  std::vector<void *> allocs;
+};
+
+namespace {
+
+int fail(dem_handle *h, int code, const std::string &msg) {
+  if (h)
+    h->err = msg;
+  else
+    g_create_error = msg;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                            \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail((h), DEM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int dalloc(dem_handle *h, void **ptr, size_t bytes) {
+  *ptr = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess)
+    return fail(h, DEM_ERR_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  h->allocs.push_back(*ptr);
+  return DEM_OK;
+}
+
+void free_particles(dem_handle *h) {
+  // everything except the material block and control block
+  std::vector<void *> keep;
+  for (void *p : h->allocs) {
+    if (p == h->mat_block || p == (void *)h->ctrl) {
+      keep.push_back(p);
+      continue;
+    }
+    hipFree(p);
+  }
+  h->allocs = keep;
+  for (int b = 0; b < 2; b++) {
+    h->st[b] = StateBuf{};
+    h->aux[b] = AuxBuf{};
+    h->lst[b] = ListBuf{};
+  }
+  h->xhold = h->fout = nullptr;
+  h->cell_rank = h->cell_count = h->cell_start = nullptr;
+  h->key = h->perm = h->scan_tmp = nullptr;
+  h->ncell = 0;
+  h->n = 0;
+  h->cap = 0;
+  h->have_list = false;
+  h->setup_done = false;
+  h->can_continue = false;
+}
+
+dem::DevParams make_devparams(const dem_handle *h) {
+  dem::DevParams d{};
+  const dem_params &p = h->p;
+  d.dt = p.dt;
+  d.dtf = 0.5 * p.dt * 1.0;  // fix_nve.cpp:86 (ftm2v = 1 in SI)
+  d.dtfrotate = d.dtf / 0.4; // fix_nve_sphere.cpp:160
+  d.triggersq = 0.25 * p.skin * p.skin;
+  d.skin = p.skin;
+  d.cdf = p.cdf;
+  for (int k = 0; k < 3; k++) {
+    d.lo[k] = p.boxlo[k];
+    d.L[k] = p.boxhi[k] - p.boxlo[k];
+    d.halfL[k] = 0.5 * d.L[k];
+    d.per[k] = p.periodic[k] ? 1 : 0;
+    d.nc[k] = h->nc[k];
+    d.cinv[k] = h->nc[k] > 0 ? h->nc[k] / d.L[k] : 0.0;
+  }
+  d.rolling = p.rolling_cdt;
+  d.tdamp = p.tangential_damping;
+  d.limitf = p.limit_force;
+  d.torsion = p.torsion_torque;
+  d.has_grav = p.has_gravity;
+  d.grav_first = p.gravity_first;
+  for (int k = 0; k < 3; k++) d.gacc[k] = p.grav_acc[k];
+  d.nwalls = (int)h->walls.size();
+  for (size_t w = 0; w < h->walls.size(); w++) {
+    d.wall_axis[w] = h->walls[w].axis;
+    d.wall_pos[w] = h->walls[w].pos;
+    d.wall_type[w] = h->walls[w].type;
+  }
+  d.cutneighmax = h->cutneighmax;
+  d.ntp1 = h->ntypes + 1;
+  return d;
+}
+
+uint32_t spread_bits(uint32_t v) {
+  // 10-bit value -> every third bit (Morton interleave)
+  v &= 0x3ff;
+  v = (v | (v << 16)) & 0x030000FF;
+  v = (v | (v << 8)) & 0x0300F00F;
+  v = (v | (v << 4)) & 0x030C30C3;
+  v = (v | (v << 2)) & 0x09249249;
+  return v;
+}
+
+// Cell grid with cells no smaller than cutneighmax, so the 27-cell stencil
+// covers every pair within (r_i + r_j) * cdf + skin. Cells are ranked in
+// Morton order; particles are counting-sorted by that rank at each rebuild.
+int setup_grid(dem_handle *h) {
+  h->cutneighmax = (h->rmax + h->rmax) * h->p.cdf + h->p.skin;  // pair_gran.cpp:623-635
+  long long total = 1;
+  for (int d = 0; d < 3; d++) {
+    const double L = h->p.boxhi[d] - h->p.boxlo[d];
+    int nc = (int)(L / h->cutneighmax);
+    if (nc < 1) nc = 1;
+    if (h->p.periodic[d] && nc < 3) nc = 1;
+    if (nc > 1024) nc = 1024;
+    if (h->p.periodic[d] && nc > 1 && L < 2.0 * h->cutneighmax)
+      return fail(h, DEM_ERR_ARG, "periodic box shorter than twice the neighbor cutoff");
+    h->nc[d] = nc;
+    total *= nc;
+  }
+  if (total > (1LL << 27)) return fail(h, DEM_ERR_ARG, "cell grid too large");
+  const int ncell = (int)total;
+  std::vector<std::pair<uint32_t, int>> code(ncell);
+  for (int cz = 0; cz < h->nc[2]; cz++)
+    for (int cy = 0; cy < h->nc[1]; cy++)
+      for (int cx = 0; cx < h->nc[0]; cx++) {
+        const int lin = cx + h->nc[0] * (cy + h->nc[1] * cz);
+        code[lin] = {spread_bits(cx) | (spread_bits(cy) << 1) | (spread_bits(cz) << 2), lin};
+      }
+  std::sort(code.begin(), code.end());
+  std::vector<int> rank(ncell);
+  for (int r = 0; r < ncell; r++) rank[code[r].second] = r;
+  int rc;
+  if ((rc = dalloc(h, (void **)&h->cell_rank, sizeof(int) * ncell))) return rc;
+  if ((rc = dalloc(h, (void **)&h->cell_count, sizeof(int) * ncell))) return rc;
+  if ((rc = dalloc(h, (void **)&h->cell_start, sizeof(int) * (ncell + 1)))) return rc;
+  if ((rc = dalloc(h, (void **)&h->scan_tmp, sizeof(int) * dem::scan_tmp_size(ncell)))) return rc;
+  HIPCHK(h, hipMemcpy(h->cell_rank, rank.data(), sizeof(int) * ncell, hipMemcpyHostToDevice));
+  h->ncell = ncell;
+  return DEM_OK;
+}
+
+dem::RebuildArgs rebuild_args(dem_handle *h) {
+  dem::RebuildArgs a{};
+  a.prm = make_devparams(h);
+  a.n = (int)h->n;
+  a.cap = h->cap;
+  a.W = h->W;
+  a.mixed = h->mixed ? 1 : 0;
+  a.cur = h->st[h->cur_st];
+  a.alt = h->st[1 - h->cur_st];
+  a.acur = h->aux[h->cur_aux];
+  a.aalt = h->aux[1 - h->cur_aux];
+  a.oldl = h->lst[h->cur_l];
+  a.newl = h->lst[1 - h->cur_l];
+  a.have_old = h->have_list ? 1 : 0;
+  a.ncell = h->ncell;
+  a.cell_rank = h->cell_rank;
+  a.cell_count = h->cell_count;
+  a.cell_start = h->cell_start;
+  a.key = h->key;
+  a.perm = h->perm;
+  a.xhold = h->xhold;
+  a.scan_tmp = h->scan_tmp;
+  a.ctrl = h->ctrl;
+  return a;
+}
+
+dem::StepArgs step_args(dem_handle *h, int in_buf) {
+  dem::StepArgs a{};
+  a.prm = make_devparams(h);
+  a.mat = h->mat;
+  a.n = (int)h->n;
+  a.cap = h->cap;
+  a.W = h->W;
+  a.mixed = h->mixed ? 1 : 0;
+  a.in = h->st[in_buf];
+  a.out = h->st[1 - in_buf];
+  a.aux = h->aux[h->cur_aux];
+  a.list = h->lst[h->cur_l];
+  a.xhold = h->xhold;
+  a.fout = h->fout;
+  a.ctrl = h->ctrl;
+  return a;
+}
+
+int read_ctrl(dem_handle *h) {
+  HIPCHK(h, hipMemcpyAsync(h->ctrl_host, h->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DEM_OK;
+}
+
+// Neighbor rebuild (verlet.cpp:313-336): FixContactHistory::pre_exchange,
+// Domain::pbc, atom sort, Neighbor::build with history carry-in.
+int rebuild(dem_handle *h) {
+  hipEvent_t e0, e1;
+  if (h->timing) {
+    HIPCHK(h, hipEventCreate(&e0));
+    HIPCHK(h, hipEventCreate(&e1));
+    HIPCHK(h, hipEventRecord(e0, h->stream));
+  }
+  dem::RebuildArgs a = rebuild_args(h);
+  HIPCHK(h, dem::launch_rebuild(a, h->stream));
+  if (h->timing) HIPCHK(h, hipEventRecord(e1, h->stream));
+  int rc = read_ctrl(h);
+  if (rc) return rc;
+  if (h->timing) {
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+    h->rebuild_ms += ms;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+  }
+  if (h->ctrl_host->overflow > 0) {
+    char buf[200];
+    snprintf(buf, sizeof buf,
+             "Neighbor list overflow, boost neigh_modify one (a particle has %d neighbors, "
+             "limit %d)",
+             h->ctrl_host->overflow, h->W);
+    return fail(h, DEM_ERR_OVERFLOW, buf);
+  }
+  h->cur_st = 1 - h->cur_st;
+  h->cur_aux = 1 - h->cur_aux;
+  h->cur_l = 1 - h->cur_l;
+  h->have_list = true;
+  h->nrebuilds++;
+  h->maxcount = h->ctrl_host->maxcount;
+  h->npairs = (int64_t)(h->ctrl_host->npairs_full / 2);
+  return DEM_OK;
+}
+
+int force_pass(dem_handle *h, int shearupdate) {
+  HIPCHK(h, dem::launch_set_stop(h->ctrl, kNever, h->stream));
+  dem::StepArgs a = step_args(h, h->cur_st);
+  a.step = 0;
+  a.shearupdate = shearupdate;
+  a.do_final = 0;
+  a.do_initial = 0;
+  a.store_f = 1;
+  a.count = 1;
+  HIPCHK(h, dem::launch_step(a, h->stream));
+  int rc = read_ctrl(h);
+  if (rc) return rc;
+  h->ncontacts = (int64_t)(h->ctrl_host->ncontact_full / 2);
+  h->nwall_contacts = (int64_t)h->ctrl_host->nwall;
+  return DEM_OK;
+}
+
+int check_ready(dem_handle *h) {
+  if (!h) return DEM_ERR_ARG;
+  if (!h->mat_set) return fail(h, DEM_ERR_STATE, "materials not set (fix property/global)");
+  if (h->n <= 0) return fail(h, DEM_ERR_STATE, "no particles");
+  for (const Wall &w : h->walls)
+    if (w.type < 1 || w.type > h->ntypes)
+      return fail(h, DEM_ERR_ARG, "wall type out of range of the material tables");
+  return DEM_OK;
+}
+
+int do_setup(dem_handle *h) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  HIPCHK(h, hipSetDevice(h->p.device));
+  if ((rc = rebuild(h))) return rc;
+  if ((rc = force_pass(h, 0))) return rc;  // pair_gran.cpp:665-667: shearupdate 0 in setup
+  h->setup_done = true;
+  h->can_continue = true;
+  return DEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *dem_last_error(const dem_handle *h) {
+  return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+int dem_create(const dem_params *params, dem_handle **out) {
+  if (!params || !out) return fail(nullptr, DEM_ERR_ARG, "null argument");
+  *out = nullptr;
+  const dem_params &p = *params;
+  if (!(p.dt > 0.0)) return fail(nullptr, DEM_ERR_ARG, "timestep must be > 0");
+  if (!(p.skin >= 0.0)) return fail(nullptr, DEM_ERR_ARG, "skin must be >= 0");
+  if (!(p.cdf >= 1.0)) return fail(nullptr, DEM_ERR_ARG, "contact_distance_factor must be >= 1");
+  for (int d = 0; d < 3; d++)
+    if (!(p.boxhi[d] > p.boxlo[d])) return fail(nullptr, DEM_ERR_ARG, "empty box");
+  if (p.precision != DEM_FP64 && p.precision != DEM_MIXED)
+    return fail(nullptr, DEM_ERR_ARG, "unknown precision mode");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= p.device || p.device < 0)
+    return fail(nullptr, DEM_ERR_NODEV, "no HIP device available");
+  if (hipSetDevice(p.device) != hipSuccess) return fail(nullptr, DEM_ERR_NODEV, "hipSetDevice failed");
+  dem_handle *h = new dem_handle();
+  h->p = p;
+  h->mixed = p.precision == DEM_MIXED;
+  h->fsz = h->mixed ? 4 : 8;
+  h->W = p.max_neighbors > 0 ? p.max_neighbors : 32;
+  if (h->W > 0xffff) {
+    delete h;
+    return fail(nullptr, DEM_ERR_ARG, "max_neighbors too large");
+  }
+  if (hipStreamCreate(&h->stream) != hipSuccess) {
+    delete h;
+    return fail(nullptr, DEM_ERR_HIP, "hipStreamCreate failed");
+  }
+  if (dalloc(h, (void **)&h->ctrl, sizeof(Ctrl)) ||
+      hipHostMalloc((void **)&h->ctrl_host, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess) {
+    g_create_error = h->err.empty() ? "pinned allocation failed" : h->err;
+    dem_destroy(h);
+    return DEM_ERR_HIP;
+  }
+  std::memset(h->ctrl_host, 0, sizeof(Ctrl));
+  h->ctrl_host->stop_at = kNever;
+  hipMemcpy(h->ctrl, h->ctrl_host, sizeof(Ctrl), hipMemcpyHostToDevice);
+  h->ev.resize(2 * 512);
+  for (auto &e : h->ev) hipEventCreate(&e);
+  *out = h;
+  return DEM_OK;
+}
+
+int dem_destroy(dem_handle *h) {
+  if (!h) return DEM_OK;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (void *p : h->allocs) hipFree(p);
+  for (auto &e : h->ev)
+    if (e) hipEventDestroy(e);
+  if (h->ctrl_host) hipHostFree(h->ctrl_host);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return DEM_OK;
+}
+
+int dem_set_materials(dem_handle *h, int ntypes, const double *Y, const double *nu,
+                      const double *e, const double *mu, const double *mur) {
+  if (!h) return DEM_ERR_ARG;
+  if (ntypes < 1 || ntypes > dem::kMaxTypes || !Y || !nu || !e || !mu)
+    return fail(h, DEM_ERR_ARG, "bad material arguments");
+  // range checks of global_properties.cpp:360-372, :393-396, :316-320
+  for (int i = 0; i < ntypes; i++) {
+    if (Y[i] < 5e6)
+      return fail(h, DEM_ERR_ARG, "youngsModulus >= 5e6 required for SI units");
+    if (Y[i] > 1e9)
+      return fail(h, DEM_ERR_ARG, "youngsModulus <= 1e9 required for SI units");
+    if (nu[i] < 0. || nu[i] > 0.5) return fail(h, DEM_ERR_ARG, "0 <= poissonsRatio <= 0.5 required");
+  }
+  for (int k = 0; k < ntypes * ntypes; k++)
+    if (e[k] <= 0.05 || e[k] > 1)
+      return fail(h, DEM_ERR_ARG, "0.05 < coefficientRestitution <= 1 required");
+  const int t1 = ntypes + 1;
+  h->ntypes = ntypes;
+  h->hYeff.assign(t1 * t1, 0.0);
+  h->hGeff.assign(t1 * t1, 0.0);
+  h->hbeta.assign(t1 * t1, 0.0);
+  h->hmu.assign(t1 * t1, 0.0);
+  h->hmur.assign(t1 * t1, 0.0);
+  for (int i = 1; i <= ntypes; i++)
+    for (int j = 1; j <= ntypes; j++) {
+      const double Yi = Y[i - 1], Yj = Y[j - 1], vi = nu[i - 1], vj = nu[j - 1];
+      // global_properties.cpp:426-433, :455-464, :518-541
+      h->hYeff[i * t1 + j] = 1. / ((1. - std::pow(vi, 2.)) / Yi + (1. - std::pow(vj, 2.)) / Yj);
+      h->hGeff[i * t1 + j] = 1. / (2. * (2. - vi) * (1. + vi) / Yi + 2. * (2. - vj) * (1. + vj) / Yj);
+      const double le = std::log(e[(i - 1) * ntypes + (j - 1)]);
+      h->hbeta[i * t1 + j] = le / std::sqrt(std::pow(le, 2.) + std::pow(kMyPi, 2.));
+      h->hmu[i * t1 + j] = mu[(i - 1) * ntypes + (j - 1)];
+      h->hmur[i * t1 + j] = mur ? mur[(i - 1) * ntypes + (j - 1)] : 0.0;
+    }
+  if (h->mat_block) {
+    hipFree(h->mat_block);
+    h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), h->mat_block), h->allocs.end());
+    h->mat_block = nullptr;
+  }
+  const size_t nn = (size_t)t1 * t1;
+  std::vector<char> blob(nn * 5 * (8 + 4));
+  double *dd = (double *)blob.data();
+  float *ff = (float *)(blob.data() + nn * 5 * 8);
+  const std::vector<double> *src[5] = {&h->hYeff, &h->hGeff, &h->hbeta, &h->hmu, &h->hmur};
+  for (int m = 0; m < 5; m++)
+    for (size_t k = 0; k < nn; k++) {
+      dd[m * nn + k] = (*src[m])[k];
+      ff[m * nn + k] = (float)(*src[m])[k];
+    }
+  int rc = dalloc(h, &h->mat_block, blob.size());
+  if (rc) return rc;
+  h->mat_block = h->allocs.back();
+  HIPCHK(h, hipMemcpy(h->mat_block, blob.data(), blob.size(), hipMemcpyHostToDevice));
+  double *db = (double *)h->mat_block;
+  float *fb = (float *)((char *)h->mat_block + nn * 5 * 8);
+  h->mat.Yeff = db;
+  h->mat.Geff = db + nn;
+  h->mat.beta = db + 2 * nn;
+  h->mat.mu = db + 3 * nn;
+  h->mat.mur = db + 4 * nn;
+  h->mat.fYeff = fb;
+  h->mat.fGeff = fb + nn;
+  h->mat.fbeta = fb + 2 * nn;
+  h->mat.fmu = fb + 3 * nn;
+  h->mat.fmur = fb + 4 * nn;
+  h->mat_set = true;
+  h->setup_done = false;
+  h->can_continue = false;
+  return DEM_OK;
+}
+
+int dem_add_plane_wall(dem_handle *h, int axis, double position, int wall_type) {
+  if (!h) return DEM_ERR_ARG;
+  if (axis < 0 || axis > 2) return fail(h, DEM_ERR_ARG, "wall axis must be 0, 1 or 2");
+  if ((int)h->walls.size() >= dem::kMaxWalls) return fail(h, DEM_ERR_ARG, "too many walls");
+  if (h->n > 0) return fail(h, DEM_ERR_STATE, "add walls before particles");
+  h->walls.push_back({axis, position, wall_type});
+  return DEM_OK;
+}
+
+int dem_set_particles(dem_handle *h, int64_t n, const double *x, const double *v,
+                      const double *omega, const double *radius, const double *rmass,
+                      const int *type, const int *tag) {
+  if (!h) return DEM_ERR_ARG;
+  if (n <= 0 || n > 0x7fffffffLL / 2 || !x || !v || !omega || !radius || !rmass || !type || !tag)
+    return fail(h, DEM_ERR_ARG, "bad particle arrays");
+  if (!h->mat_set) return fail(h, DEM_ERR_STATE, "set materials before particles");
+  HIPCHK(h, hipSetDevice(h->p.device));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double rmax = 0.0;
+  for (int64_t i = 0; i < n; i++) {
+    if (!(radius[i] > 0.0)) return fail(h, DEM_ERR_ARG, "Fix nve/sphere requires extended particles");
+    if (!(rmass[i] > 0.0)) return fail(h, DEM_ERR_ARG, "particle mass must be > 0");
+    if (type[i] < 1 || type[i] > h->ntypes) return fail(h, DEM_ERR_ARG, "Invalid atom type");
+    if (tag[i] <= 0) return fail(h, DEM_ERR_ARG, "Invalid atom ID");
+    for (int d = 0; d < 3; d++)
+      if (!std::isfinite(x[3 * i + d]) || !std::isfinite(v[3 * i + d]) || !std::isfinite(omega[3 * i + d]))
+        return fail(h, DEM_ERR_ARG, "non-finite particle state");
+    rmax = std::max(rmax, radius[i]);
+  }
+  free_particles(h);
+  h->n = n;
+  h->cap = (int)n;
+  h->rmax = rmax;
+  int rc;
+  if ((rc = setup_grid(h))) return rc;
+  const size_t cap = (size_t)h->cap;
+  const size_t nw = std::max<size_t>(1, h->walls.size());
+  for (int b = 0; b < 2; b++) {
+    if ((rc = dalloc(h, (void **)&h->st[b].pos, sizeof(double4) * cap))) return rc;
+    if ((rc = dalloc(h, (void **)&h->st[b].vel, sizeof(double4) * cap))) return rc;
+    if ((rc = dalloc(h, &h->st[b].omg, (h->mixed ? sizeof(float4) : sizeof(double4)) * cap))) return rc;
+    if ((rc = dalloc(h, (void **)&h->aux[b].tag, sizeof(int) * cap))) return rc;
+    if ((rc = dalloc(h, (void **)&h->aux[b].orig, sizeof(int) * cap))) return rc;
+    if ((rc = dalloc(h, (void **)&h->aux[b].meta, sizeof(int) * cap))) return rc;
+    if ((rc = dalloc(h, &h->aux[b].whist, h->fsz * 3 * nw * cap))) return rc;
+    if ((rc = dalloc(h, (void **)&h->lst[b].nbr, sizeof(uint32_t) * h->W * cap))) return rc;
+    if ((rc = dalloc(h, &h->lst[b].hist, h->fsz * 3 * h->W * cap))) return rc;
+  }
+  if ((rc = dalloc(h, (void **)&h->xhold, sizeof(double) * 3 * cap))) return rc;
+  if ((rc = dalloc(h, (void **)&h->fout, sizeof(double) * 6 * cap))) return rc;
+  if ((rc = dalloc(h, (void **)&h->key, sizeof(int) * cap))) return rc;
+  if ((rc = dalloc(h, (void **)&h->perm, sizeof(int) * cap))) return rc;
+
+  std::vector<double4> pos(n), vel(n);
+  std::vector<int> orig(n), zero(n, 0);
+  for (int64_t i = 0; i < n; i++) {
+    pos[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], radius[i]);
+    vel[i] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], rmass[i]);
+    orig[i] = (int)i;
+  }
+  HIPCHK(h, hipMemcpy(h->st[0].pos, pos.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->st[0].vel, vel.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+  if (h->mixed) {
+    std::vector<float4> w(n);
+    for (int64_t i = 0; i < n; i++)
+      w[i] = make_float4((float)omega[3 * i], (float)omega[3 * i + 1], (float)omega[3 * i + 2], (float)type[i]);
+    HIPCHK(h, hipMemcpy(h->st[0].omg, w.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+  } else {
+    std::vector<double4> w(n);
+    for (int64_t i = 0; i < n; i++)
+      w[i] = make_double4(omega[3 * i], omega[3 * i + 1], omega[3 * i + 2], (double)type[i]);
+    HIPCHK(h, hipMemcpy(h->st[0].omg, w.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+  }
+  HIPCHK(h, hipMemcpy(h->aux[0].tag, tag, sizeof(int) * n, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->aux[0].orig, orig.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->aux[0].meta, zero.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemset(h->fout, 0, sizeof(double) * 6 * cap));
+  h->cur_st = h->cur_aux = h->cur_l = 0;
+  h->have_list = false;
+  h->setup_done = false;
+  return DEM_OK;
+}
+
+int dem_setup(dem_handle *h) {
+  if (!h) return DEM_ERR_ARG;
+  return do_setup(h);
+}
+
+int dem_set_kernel_timing(dem_handle *h, int on) {
+  if (!h) return DEM_ERR_ARG;
+  h->timing = on != 0;
+  h->kernel_ms = 0.0;
+  h->kernel_launches = 0;
+  h->rebuild_ms = 0.0;
+  return DEM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Integrate nsteps steps continuing from the last force pass (whose f and
+// torque are in fout, in the current particle order).
+int run_steps(dem_handle *h, int64_t nsteps) {
+  int rc;
+  if (nsteps == 0) return DEM_OK;
+  h->setup_done = false;
+  h->can_continue = false;
+  HIPCHK(h, dem::launch_set_stop(h->ctrl, kNever, h->stream));
+  {
+    dem::StepArgs a = step_args(h, h->cur_st);
+    HIPCHK(h, dem::launch_initial(a, h->stream));
+  }
+  int64_t t = 1;
+  int batch = 16;
+  while (t <= nsteps) {
+    const int nb = (int)std::min<int64_t>(batch, nsteps - t + 1);
+    for (int b = 0; b < nb; b++) {
+      const int64_t step = t + b;
+      const bool last = step == nsteps;
+      dem::StepArgs a = step_args(h, (h->cur_st + b) & 1);
+      a.step = step;
+      a.shearupdate = 1;
+      a.do_final = 1;
+      a.do_initial = last ? 0 : 1;
+      a.store_f = last ? 1 : 0;
+      a.count = last ? 1 : 0;
+      if (h->timing) HIPCHK(h, hipEventRecord(h->ev[2 * b], h->stream));
+      HIPCHK(h, dem::launch_step(a, h->stream));
+      if (h->timing) HIPCHK(h, hipEventRecord(h->ev[2 * b + 1], h->stream));
+    }
+    if ((rc = read_ctrl(h))) return rc;
+    const long long stop = h->ctrl_host->stop_at;
+    int executed = nb;
+    if (stop <= nsteps) executed = (int)(stop - t);
+    if (executed < 0 || executed > nb) return fail(h, DEM_ERR_STATE, "internal step accounting error");
+    if (h->timing)
+      for (int b = 0; b < executed; b++) {
+        float ms = 0.f;
+        HIPCHK(h, hipEventElapsedTime(&ms, h->ev[2 * b], h->ev[2 * b + 1]));
+        h->kernel_ms += ms;
+        h->kernel_launches++;
+      }
+    h->cur_st = (h->cur_st + executed) & 1;
+    h->ntimestep += executed;
+    t += executed;
+    if (stop <= nsteps) {
+      if ((rc = rebuild(h))) return rc;
+      HIPCHK(h, dem::launch_set_stop(h->ctrl, kNever, h->stream));
+      batch = std::max(4, batch / 2);
+    } else {
+      batch = std::min(256, batch * 2);
+    }
+  }
+  h->ncontacts = (int64_t)(h->ctrl_host->ncontact_full / 2);
+  h->nwall_contacts = (int64_t)h->ctrl_host->nwall;
+  h->can_continue = true;
+  return DEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dem_run(dem_handle *h, int64_t nsteps) {
+  if (!h) return DEM_ERR_ARG;
+  if (nsteps < 0) return fail(h, DEM_ERR_ARG, "negative step count");
+  int rc = do_setup(h);  // Run::command calls Integrate::setup (run.cpp:209)
+  if (rc) return rc;
+  h->can_continue = true;
+  return run_steps(h, nsteps);
+}
+
+int dem_run_continue(dem_handle *h, int64_t nsteps) {
+  if (!h) return DEM_ERR_ARG;
+  if (nsteps < 0) return fail(h, DEM_ERR_ARG, "negative step count");
+  if (!h->can_continue)
+    return fail(h, DEM_ERR_STATE, "run pre no needs a preceding run or setup on this state");
+  return run_steps(h, nsteps);
+}
+
+int dem_compute_forces(dem_handle *h, int shearupdate) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (!h->have_list) {
+    if ((rc = rebuild(h))) return rc;
+  }
+  return force_pass(h, shearupdate ? 1 : 0);
+}
+
+int dem_get_state(dem_handle *h, double *x, double *v, double *omega, double *f, double *torque) {
+  if (!h) return DEM_ERR_ARG;
+  if (h->n <= 0) return fail(h, DEM_ERR_STATE, "no particles");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const int64_t n = h->n;
+  const size_t cap = (size_t)h->cap;
+  std::vector<double4> pos(n), vel(n);
+  std::vector<int> orig(n);
+  const StateBuf &s = h->st[h->cur_st];
+  HIPCHK(h, hipMemcpy(pos.data(), s.pos, sizeof(double4) * n, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(vel.data(), s.vel, sizeof(double4) * n, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(orig.data(), h->aux[h->cur_aux].orig, sizeof(int) * n, hipMemcpyDeviceToHost));
+  std::vector<double> w(3 * n);
+  if (h->mixed) {
+    std::vector<float4> ww(n);
+    HIPCHK(h, hipMemcpy(ww.data(), s.omg, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) {
+      w[3 * i] = ww[i].x;
+      w[3 * i + 1] = ww[i].y;
+      w[3 * i + 2] = ww[i].z;
+    }
+  } else {
+    std::vector<double4> ww(n);
+    HIPCHK(h, hipMemcpy(ww.data(), s.omg, sizeof(double4) * n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) {
+      w[3 * i] = ww[i].x;
+      w[3 * i + 1] = ww[i].y;
+      w[3 * i + 2] = ww[i].z;
+    }
+  }
+  std::vector<double> ft;
+  if (f || torque) {
+    ft.resize(6 * cap);
+    HIPCHK(h, hipMemcpy(ft.data(), h->fout, sizeof(double) * 6 * cap, hipMemcpyDeviceToHost));
+  }
+  for (int64_t i = 0; i < n; i++) {
+    const int o = orig[i];
+    if (o < 0 || o >= n) return fail(h, DEM_ERR_STATE, "corrupt particle index");
+    if (x) {
+      x[3 * o] = pos[i].x;
+      x[3 * o + 1] = pos[i].y;
+      x[3 * o + 2] = pos[i].z;
+    }
+    if (v) {
+      v[3 * o] = vel[i].x;
+      v[3 * o + 1] = vel[i].y;
+      v[3 * o + 2] = vel[i].z;
+    }
+    if (omega)
+      for (int d = 0; d < 3; d++) omega[3 * o + d] = w[3 * i + d];
+    if (f)
+      for (int d = 0; d < 3; d++) f[3 * o + d] = ft[d * cap + i];
+    if (torque)
+      for (int d = 0; d < 3; d++) torque[3 * o + d] = ft[(3 + d) * cap + i];
+  }
+  return DEM_OK;
+}
+
+int dem_get_pairs(dem_handle *h, int64_t cap_out, int *tags, int *flags, double *hist,
+                  int64_t *npairs) {
+  if (!h || !npairs) return DEM_ERR_ARG;
+  *npairs = 0;
+  if (!h->have_list) return DEM_OK;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const int64_t n = h->n;
+  const size_t cap = (size_t)h->cap;
+  const int W = h->W;
+  std::vector<uint32_t> nbr((size_t)W * cap);
+  std::vector<int> meta(n), tag(n);
+  std::vector<char> hraw(h->fsz * 3 * W * cap);
+  const ListBuf &l = h->lst[h->cur_l];
+  HIPCHK(h, hipMemcpy(nbr.data(), l.nbr, sizeof(uint32_t) * W * cap, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(hraw.data(), l.hist, hraw.size(), hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(meta.data(), h->aux[h->cur_aux].meta, sizeof(int) * n, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(tag.data(), h->aux[h->cur_aux].tag, sizeof(int) * n, hipMemcpyDeviceToHost));
+  int64_t np = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const int cnt = meta[i] & dem::kMetaCountMask;
+    for (int k = 0; k < cnt; k++) {
+      const uint32_t e = nbr[(size_t)k * cap + i];
+      const int j = (int)(e & dem::kIndexMask);
+      if (j < 0 || j >= n) return fail(h, DEM_ERR_STATE, "corrupt neighbor entry");
+      if (!(tag[i] < tag[j])) continue;
+      if (np < cap_out) {
+        tags[2 * np] = tag[i];
+        tags[2 * np + 1] = tag[j];
+        const bool fl = (e & dem::kFlagBit) != 0;
+        flags[np] = fl ? 1 : 0;
+        for (int d = 0; d < 3; d++) {
+          const size_t idx = ((size_t)k * 3 + d) * cap + i;
+          double val = h->mixed ? (double)((float *)hraw.data())[idx] : ((double *)hraw.data())[idx];
+          hist[3 * np + d] = fl ? val : 0.0;
+        }
+      }
+      np++;
+    }
+  }
+  *npairs = np;
+  return DEM_OK;
+}
+
+int dem_get_wall_history(dem_handle *h, double *out) {
+  if (!h || !out) return DEM_ERR_ARG;
+  if (h->n <= 0) return fail(h, DEM_ERR_STATE, "no particles");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const int64_t n = h->n;
+  const size_t cap = (size_t)h->cap;
+  const int nw = (int)h->walls.size();
+  if (nw == 0) return DEM_OK;
+  std::vector<int> meta(n), orig(n);
+  std::vector<char> raw(h->fsz * 3 * nw * cap);
+  const AuxBuf &a = h->aux[h->cur_aux];
+  HIPCHK(h, hipMemcpy(meta.data(), a.meta, sizeof(int) * n, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(orig.data(), a.orig, sizeof(int) * n, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(raw.data(), a.whist, raw.size(), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < n; i++) {
+    const int o = orig[i];
+    for (int w = 0; w < nw; w++) {
+      const bool has = meta[i] & (1 << (dem::kMetaWallShift + w));
+      for (int d = 0; d < 3; d++) {
+        const size_t idx = ((size_t)w * 3 + d) * cap + i;
+        const double val = h->mixed ? (double)((float *)raw.data())[idx] : ((double *)raw.data())[idx];
+        out[((size_t)o * nw + w) * 3 + d] = has ? val : 0.0;
+      }
+    }
+  }
+  return DEM_OK;
+}
+
+int dem_get_stats(dem_handle *h, dem_stats *out) {
+  if (!h || !out) return DEM_ERR_ARG;
+  out->ntimestep = h->ntimestep;
+  out->nrebuilds = h->nrebuilds;
+  out->npairs = h->npairs;
+  out->ncontacts = h->ncontacts;
+  out->nwall_contacts = h->nwall_contacts;
+  out->max_neighbors_seen = h->maxcount;
+  out->step_kernel_ms = h->kernel_launches ? h->kernel_ms / (double)h->kernel_launches : 0.0;
+  out->step_kernel_launches = h->kernel_launches;
+  out->rebuild_ms_total = h->rebuild_ms;
+  return DEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,638 @@
+// Input-script front end: runs a LIGGGHTS input script whose commands are
+// the reference's subset for the per-timestep granular path, on the GPU
+// library (Input::file, input.cpp:183; Input::execute_command, input.cpp:560-650).
+//
+// Supported physics commands keep the reference's syntax and meaning:
+//   units si | atom_style granular|sphere | atom_modify ... | boundary
+//   newton off | communicate single vel yes | processors ...
+//   region <id> block xlo xhi ylo yhi zlo zhi [units box] | create_box <n> <region>
+//   read_data <file>  (granular Atoms: id type diameter density x y z; Velocities)
+//   neighbor <skin> bin | neigh_modify delay 0 [every 1] [check yes]
+//                                       [contact_distance_factor f] [one n]
+//   fix <id> <grp> property/global youngsModulus|poissonsRatio peratomtype ...
+//   fix <id> <grp> property/global coefficientRestitution|coefficientFriction|
+//                  coefficientRollingFriction peratomtypepair <n> ...
+//   pair_style gran model hertz tangential history [rolling_friction cdt]
+//              [tangential_damping on|off] [limitForce on|off] [torsionTorque on|off]
+//   pair_coeff * * | timestep <dt>
+//   fix <id> <grp> gravity <g> vector <x> <y> <z>
+//   fix <id> <grp> wall/gran model hertz tangential history [rolling_friction cdt]
+//                  primitive type <t> xplane|yplane|zplane <pos>
+//   fix <id> <grp> nve/sphere
+//   thermo <N> | thermo_style custom step atoms ke ... | thermo_modify ...
+//   variable <name> equal|index <value> | print | echo | log
+//   run <N> [pre no|yes] [post no|yes]
+//   package amd precision fp64|mixed   (selects the GPU arithmetic mode)
+// Output-only commands (dump, compute, undump, ...) are accepted and ignored
+// with a warning; anything that would change the physics and is not listed
+// is an error, as the reference reports for an unknown style.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/liggghts_amd.h"
+
+namespace {
+
+constexpr double kMyPi = 3.14159265358979323846;
+
+struct ScriptError {
+  std::string msg;
+};
+
+[[noreturn]] void die(const std::string &m) { throw ScriptError{m}; }
+
+double num(const std::string &s) {
+  char *end = nullptr;
+  const double v = std::strtod(s.c_str(), &end);
+  if (end == s.c_str() || *end != '\0') die("Expected floating point parameter instead of '" + s + "'");
+  return v;
+}
+
+int inum(const std::string &s) {
+  char *end = nullptr;
+  const long v = std::strtol(s.c_str(), &end, 10);
+  if (end == s.c_str() || *end != '\0') die("Expected integer parameter instead of '" + s + "'");
+  return (int)v;
+}
+
+struct Interp {
+  FILE *out = stdout;
+  // box / domain
+  bool have_box = false;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  int periodic[3] = {1, 1, 1};
+  int ntypes = 0;
+  std::map<std::string, std::vector<double>> regions;
+  // particles
+  std::vector<double> x, v, w, r, m;
+  std::vector<int> type, tag;
+  // neighbor / run
+  double skin = -1.0, cdf = 1.0, dt = -1.0;
+  int one = 0;
+  // materials
+  std::vector<double> Y, nu, e, mu, mur;
+  // pair
+  bool pair_set = false;
+  int rolling = 0, tdamp = 1, limitf = 0, torsion = 0;
+  // fixes
+  bool nve = false;
+  bool gravity = false, gravity_first = true, wall_seen = false;
+  double gacc[3] = {0, 0, 0};
+  struct PlaneWall {
+    int axis;
+    double pos;
+    int type;
+  };
+  std::vector<PlaneWall> walls;
+  int precision = DEM_FP64;
+  // thermo
+  int thermo_every = 0;
+  std::vector<std::string> thermo_cols = {"step", "atoms", "ke"};
+  std::map<std::string, std::string> vars;
+  // system
+  dem_handle *h = nullptr;
+  bool dirty = true;  // particles/params changed since the handle was built
+  int64_t step = 0;
+
+  ~Interp() {
+    if (h) dem_destroy(h);
+  }
+
+  std::string substitute(const std::string &line) {
+    std::string outl;
+    for (size_t k = 0; k < line.size(); k++) {
+      if (line[k] == '$' && k + 1 < line.size()) {
+        std::string name;
+        size_t adv;
+        if (line[k + 1] == '{') {
+          const size_t close = line.find('}', k + 2);
+          if (close == std::string::npos) die("Invalid variable name in " + line);
+          name = line.substr(k + 2, close - k - 2);
+          adv = close;
+        } else {
+          name = line.substr(k + 1, 1);
+          adv = k + 1;
+        }
+        auto it = vars.find(name);
+        if (it == vars.end()) die("Substitution for illegal variable " + name);
+        outl += it->second;
+        k = adv;
+      } else {
+        outl += line[k];
+      }
+    }
+    return outl;
+  }
+
+  static std::vector<std::string> split(const std::string &s) {
+    std::vector<std::string> t;
+    std::istringstream is(s);
+    std::string w;
+    while (is >> w) t.push_back(w);
+    return t;
+  }
+
+  void need_box(const char *cmd) {
+    if (!have_box) die(std::string(cmd) + " command before simulation box is defined");
+  }
+
+  void read_data(const std::string &path) {
+    std::ifstream f(path);
+    if (!f) die("Cannot open file " + path);
+    std::string line;
+    std::getline(f, line);  // title line
+    int64_t natoms = -1;
+    bool box_given[3] = {false, false, false};
+    double blo[3], bhi[3];
+    std::string section;
+    // header
+    while (std::getline(f, line)) {
+      const size_t hash = line.find('#');
+      if (hash != std::string::npos) line = line.substr(0, hash);
+      auto t = split(line);
+      if (t.empty()) continue;
+      if (t.size() >= 2 && t[1] == "atoms") {
+        natoms = std::stoll(t[0]);
+      } else if (t.size() >= 3 && t[1] == "atom" && t[2] == "types") {
+        const int nt = inum(t[0]);
+        if (!have_box) ntypes = nt;
+        else if (nt > ntypes) die("Data file atom types exceed create_box types");
+      } else if (t.size() >= 4 && (t[2] == "xlo" || t[2] == "ylo" || t[2] == "zlo")) {
+        const int d = t[2][0] - 'x';
+        blo[d] = num(t[0]);
+        bhi[d] = num(t[1]);
+        box_given[d] = true;
+      } else if (t[0] == "Atoms" || t[0] == "Velocities") {
+        section = t[0];
+        break;
+      }
+    }
+    if (natoms < 0) die("read_data: no atom count in header");
+    if (!have_box) {
+      for (int d = 0; d < 3; d++) {
+        if (!box_given[d]) die("read_data: box bounds missing");
+        lo[d] = blo[d];
+        hi[d] = bhi[d];
+      }
+      have_box = true;
+    }
+    const size_t base = tag.size();
+    std::map<int, size_t> index;
+    for (size_t k = 0; k < base; k++) index[tag[k]] = k;
+    while (!section.empty()) {
+      std::string next;
+      int64_t got = 0;
+      while (got < natoms && std::getline(f, line)) {
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line = line.substr(0, hash);
+        auto t = split(line);
+        if (t.empty()) continue;
+        if (section == "Atoms") {
+          // atom_vec_sphere.cpp:1055-1100: id type diameter density x y z
+          if (t.size() < 7) die("Incorrect format of Atoms section");
+          const int id = inum(t[0]);
+          const int ty = inum(t[1]);
+          if (id <= 0) die("Invalid atom ID in Atoms section of data file");
+          if (ty <= 0 || ty > ntypes) die("Invalid atom type in Atoms section of data file");
+          const double rad = 0.5 * num(t[2]);
+          const double dens = num(t[3]);
+          if (rad < 0.0) die("Invalid radius in Atoms section of data file");
+          if (dens <= 0.0) die("Invalid density in Atoms section of data file");
+          const double mass = rad == 0.0 ? dens : 4.0 * kMyPi / 3.0 * rad * rad * rad * dens;
+          index[id] = tag.size();
+          tag.push_back(id);
+          type.push_back(ty);
+          r.push_back(rad);
+          m.push_back(mass);
+          for (int d = 0; d < 3; d++) x.push_back(num(t[4 + d]));
+          for (int d = 0; d < 3; d++) {
+            v.push_back(0.0);
+            w.push_back(0.0);
+          }
+        } else {
+          // atom_vec_sphere.cpp:1129-1140: id vx vy vz wx wy wz
+          if (t.size() < 7) die("Incorrect format of Velocities section");
+          const int id = inum(t[0]);
+          auto it = index.find(id);
+          if (it == index.end()) die("Invalid atom ID in Velocities section of data file");
+          for (int d = 0; d < 3; d++) {
+            v[3 * it->second + d] = num(t[1 + d]);
+            w[3 * it->second + d] = num(t[4 + d]);
+          }
+        }
+        got++;
+      }
+      if (got < natoms) die("Unexpected end of data file");
+      section.clear();
+      while (std::getline(f, line)) {
+        auto t = split(line);
+        if (t.empty()) continue;
+        if (t[0] == "Atoms" || t[0] == "Velocities") {
+          section = t[0];
+          break;
+        }
+      }
+    }
+    fprintf(out, "  %zu atoms\n", tag.size() - base);
+    dirty = true;
+  }
+
+  void fix_property_global(const std::vector<std::string> &t) {
+    // fix id group property/global <name> peratomtype|peratomtypepair [n] values...
+    if (t.size() < 6) die("Illegal fix property/global command");
+    const std::string &name = t[3];
+    const std::string &style = t[4];
+    std::vector<double> vals;
+    size_t k = 5;
+    if (style == "peratomtypepair") k = 6;
+    for (; k < t.size(); k++) vals.push_back(num(t[k]));
+    if (name == "youngsModulus") Y = vals;
+    else if (name == "poissonsRatio") nu = vals;
+    else if (name == "coefficientRestitution") e = vals;
+    else if (name == "coefficientFriction") mu = vals;
+    else if (name == "coefficientRollingFriction") mur = vals;
+    else fprintf(out, "WARNING: property/global %s is not used by this path\n", name.c_str());
+    dirty = true;
+  }
+
+  void parse_model(const std::vector<std::string> &t, size_t k, size_t &end, bool pair) {
+    // model hertz tangential history [rolling_friction cdt] [settings...]
+    bool saw_model = false, saw_tang = false;
+    int roll = 0, td = 1, lf = 0, tor = 0;
+    while (k < t.size()) {
+      if (t[k] == "model" && k + 1 < t.size()) {
+        if (t[k + 1] != "hertz") die("Only 'model hertz' is available on this path, got " + t[k + 1]);
+        saw_model = true;
+        k += 2;
+      } else if (t[k] == "tangential" && k + 1 < t.size()) {
+        if (t[k + 1] != "history") die("Only 'tangential history' is available, got " + t[k + 1]);
+        saw_tang = true;
+        k += 2;
+      } else if (t[k] == "rolling_friction" && k + 1 < t.size()) {
+        if (t[k + 1] == "cdt") roll = 1;
+        else if (t[k + 1] != "off") die("Only 'rolling_friction cdt' is available, got " + t[k + 1]);
+        k += 2;
+      } else if (t[k] == "cohesion" && k + 1 < t.size()) {
+        if (t[k + 1] != "off") die("cohesion models are not available on this path");
+        k += 2;
+      } else if (t[k] == "surface" && k + 1 < t.size()) {
+        if (t[k + 1] != "default") die("Only the default surface model is available");
+        k += 2;
+      } else if ((t[k] == "tangential_damping" || t[k] == "limitForce" || t[k] == "torsionTorque") &&
+                 k + 1 < t.size()) {
+        const int on = t[k + 1] == "on" ? 1 : (t[k + 1] == "off" ? 0 : -1);
+        if (on < 0) die("Expected on/off for " + t[k]);
+        if (t[k] == "tangential_damping") td = on;
+        else if (t[k] == "limitForce") lf = on;
+        else tor = on;
+        k += 2;
+      } else {
+        break;
+      }
+    }
+    if (!saw_model || !saw_tang) die("gran model needs 'model hertz tangential history'");
+    end = k;
+    if (pair) {
+      rolling = roll;
+      tdamp = td;
+      limitf = lf;
+      torsion = tor;
+    } else if (roll != rolling || td != tdamp || lf != limitf || tor != torsion) {
+      die("fix wall/gran model settings must match pair_style gran on this path");
+    }
+  }
+
+  void build_handle() {
+    need_box("run");
+    if (!pair_set) die("run: pair_style gran not defined");
+    if (!nve) die("run: no fix nve/sphere, particles would not move");
+    if (dt <= 0) die("run: timestep not set");
+    if (skin < 0) die("run: neighbor skin not set");
+    if ((int)Y.size() != ntypes || (int)nu.size() != ntypes)
+      die("youngsModulus/poissonsRatio need one value per atom type");
+    if ((int)e.size() != ntypes * ntypes || (int)mu.size() != ntypes * ntypes)
+      die("coefficientRestitution/coefficientFriction need ntypes^2 values");
+    if (rolling && (int)mur.size() != ntypes * ntypes)
+      die("rolling_friction cdt needs coefficientRollingFriction (ntypes^2 values)");
+    if (h) {
+      dem_destroy(h);
+      h = nullptr;
+    }
+    dem_params p{};
+    p.dt = dt;
+    p.skin = skin;
+    p.cdf = cdf;
+    for (int d = 0; d < 3; d++) {
+      p.boxlo[d] = lo[d];
+      p.boxhi[d] = hi[d];
+      p.periodic[d] = periodic[d];
+    }
+    p.rolling_cdt = rolling;
+    p.tangential_damping = tdamp;
+    p.limit_force = limitf;
+    p.torsion_torque = torsion;
+    p.has_gravity = gravity;
+    for (int d = 0; d < 3; d++) p.grav_acc[d] = gacc[d];
+    p.gravity_first = gravity_first;
+    p.precision = precision;
+    p.max_neighbors = one;
+    p.device = 0;
+    int rc = dem_create(&p, &h);
+    if (rc) die(std::string("GPU library: ") + dem_last_error(nullptr));
+    if (dem_set_materials(h, ntypes, Y.data(), nu.data(), e.data(), mu.data(),
+                          rolling ? mur.data() : nullptr))
+      die(dem_last_error(h));
+    for (auto &wl : walls)
+      if (dem_add_plane_wall(h, wl.axis, wl.pos, wl.type)) die(dem_last_error(h));
+    if (tag.empty()) die("run: no atoms");
+    if (dem_set_particles(h, (int64_t)tag.size(), x.data(), v.data(), w.data(), r.data(), m.data(),
+                          type.data(), tag.data()))
+      die(dem_last_error(h));
+    dirty = false;
+  }
+
+  void pull_state() {
+    if (dem_get_state(h, x.data(), v.data(), w.data(), nullptr, nullptr)) die(dem_last_error(h));
+  }
+
+  void thermo_header() {
+    std::string hdr;
+    for (auto &c : thermo_cols) {
+      std::string name = c == "step" ? "Step" : c == "atoms" ? "Atoms" : c == "ke" ? "KinEng" : c;
+      char buf[64];
+      snprintf(buf, sizeof buf, "%14s ", name.c_str());
+      hdr += buf;
+    }
+    fprintf(out, "%s\n", hdr.c_str());
+  }
+
+  void thermo_line() {
+    pull_state();
+    double ke = 0.0;
+    for (size_t i = 0; i < tag.size(); i++)
+      ke += 0.5 * m[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+    std::string row;
+    for (auto &c : thermo_cols) {
+      char buf[64];
+      if (c == "step") snprintf(buf, sizeof buf, "%14lld ", (long long)step);
+      else if (c == "atoms") snprintf(buf, sizeof buf, "%14zu ", tag.size());
+      else if (c == "ke") snprintf(buf, sizeof buf, "%14.8g ", ke);
+      else snprintf(buf, sizeof buf, "%14s ", "n/a");
+      row += buf;
+    }
+    fprintf(out, "%s\n", row.c_str());
+  }
+
+  void run(const std::vector<std::string> &t) {
+    if (t.size() < 2) die("Illegal run command");
+    const long long n = std::stoll(t[1]);
+    if (n < 0) die("Illegal run command");
+    bool pre = true;
+    for (size_t k = 2; k + 1 < t.size(); k += 2) {
+      if (t[k] == "pre") pre = t[k + 1] != "no";
+      else if (t[k] == "post" || t[k] == "upto" || t[k] == "every" || t[k] == "start" || t[k] == "stop") {
+        if (t[k] != "post") die("run option " + t[k] + " is not supported on this path");
+      } else die("Illegal run command");
+    }
+    if (dirty || !h) {
+      build_handle();
+      pre = true;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    long long done = 0;
+    const long long chunk = thermo_every > 0 ? thermo_every : n;
+    bool first = true;
+    if (pre) {
+      if (dem_setup(h)) die(dem_last_error(h));
+    }
+    thermo_header();
+    thermo_line();
+    while (done < n) {
+      long long k = chunk - (step % (chunk > 0 ? chunk : 1));
+      if (k <= 0 || k > n - done) k = n - done;
+      int rc = dem_run_continue(h, k);
+      (void)first;
+      first = false;
+      if (rc) die(dem_last_error(h));
+      done += k;
+      step += k;
+      if ((thermo_every > 0 && step % thermo_every == 0) || done == n) thermo_line();
+    }
+    const double secs =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    fprintf(out, "Loop time of %g on 1 procs (1 GPU) for %lld steps with %zu atoms\n", secs, n,
+            tag.size());
+    dem_stats st{};
+    dem_get_stats(h, &st);
+    fprintf(out, "Neighbor list builds = %lld\n", (long long)st.nrebuilds);
+    if (secs > 0)
+      fprintf(out, "Performance: %.4g particle-timesteps/s\n", (double)n * tag.size() / secs);
+  }
+
+  void execute(const std::string &raw) {
+    std::string line = raw;
+    const size_t hash = line.find('#');
+    if (hash != std::string::npos) line = line.substr(0, hash);
+    line = substitute(line);
+    auto t = split(line);
+    if (t.empty()) return;
+    const std::string &c = t[0];
+    if (c == "units") {
+      if (t.size() < 2 || t[1] != "si") die("Only 'units si' is supported on this path");
+    } else if (c == "atom_style") {
+      if (t.size() < 2 || (t[1] != "granular" && t[1] != "sphere"))
+        die("Only atom_style granular/sphere is supported on this path");
+    } else if (c == "atom_modify" || c == "processors" || c == "echo" || c == "log" ||
+               c == "thermo_modify" || c == "dump" || c == "dump_modify" || c == "undump" ||
+               c == "compute" || c == "uncompute" || c == "group" || c == "restart" ||
+               c == "write_restart" || c == "write_dump" || c == "soft_particles" ||
+               c == "hard_particles") {
+      if (c != "atom_modify" && c != "processors" && c != "echo" && c != "log" && c != "thermo_modify")
+        fprintf(out, "WARNING: '%s' is output/bookkeeping only and is ignored on this path\n", c.c_str());
+    } else if (c == "boundary") {
+      if (t.size() < 4) die("Illegal boundary command");
+      for (int d = 0; d < 3; d++) periodic[d] = t[1 + d] == "p" ? 1 : 0;
+    } else if (c == "newton") {
+      if (t.size() < 2 || t[1] != "off")
+        die("Pair granular with shear history requires newton pair off");  // pair_gran.cpp:260
+    } else if (c == "communicate" || c == "comm_modify") {
+      // ghost velocities are always available on the GPU path
+    } else if (c == "region") {
+      if (t.size() < 9 || t[2] != "block") die("Only 'region <id> block ...' is supported");
+      std::vector<double> b;
+      for (int k = 3; k < 9; k++) b.push_back(num(t[k]));
+      regions[t[1]] = b;
+    } else if (c == "create_box") {
+      if (t.size() < 3) die("Illegal create_box command");
+      auto it = regions.find(t[2]);
+      if (it == regions.end()) die("Create_box region ID does not exist");
+      ntypes = inum(t[1]);
+      for (int d = 0; d < 3; d++) {
+        lo[d] = it->second[2 * d];
+        hi[d] = it->second[2 * d + 1];
+      }
+      have_box = true;
+    } else if (c == "read_data") {
+      if (t.size() < 2) die("Illegal read_data command");
+      read_data(t[1]);
+    } else if (c == "neighbor") {
+      if (t.size() < 3 || t[2] != "bin") die("neighbor style must be 'bin' on this path");
+      skin = num(t[1]);
+    } else if (c == "neigh_modify") {
+      for (size_t k = 1; k + 1 < t.size(); k += 2) {
+        if (t[k] == "delay") {
+          if (inum(t[k + 1]) != 0) die("neigh_modify delay must be 0 on this path");
+        } else if (t[k] == "every") {
+          if (inum(t[k + 1]) != 1) die("neigh_modify every must be 1 on this path");
+        } else if (t[k] == "check") {
+          if (t[k + 1] != "yes") die("neigh_modify check must be yes on this path");
+        } else if (t[k] == "contact_distance_factor") {
+          cdf = num(t[k + 1]);
+        } else if (t[k] == "one") {
+          one = inum(t[k + 1]);
+        } else if (t[k] == "page" || t[k] == "binsize") {
+        } else die("Illegal neigh_modify command");
+      }
+      dirty = true;
+    } else if (c == "pair_style") {
+      if (t.size() < 2 || t[1] != "gran") die("Only pair_style gran is available on this path");
+      size_t end;
+      parse_model(t, 2, end, true);
+      if (end != t.size()) die("Illegal pair_style gran argument " + t[end]);
+      pair_set = true;
+      dirty = true;
+    } else if (c == "pair_coeff") {
+      if (t.size() < 3 || t[1] != "*" || t[2] != "*") die("pair_coeff must be '* *' for granular");
+    } else if (c == "timestep") {
+      if (t.size() < 2) die("Illegal timestep command");
+      dt = num(t[1]);
+      dirty = true;
+    } else if (c == "fix") {
+      if (t.size() < 4) die("Illegal fix command");
+      const std::string &style = t[3];
+      if (style == "property/global") {
+        fix_property_global(t);
+      } else if (style == "nve/sphere") {
+        nve = true;
+      } else if (style == "gravity") {
+        if (t.size() < 9 || t[5] != "vector") die("fix gravity: only '<g> vector x y z' is supported");
+        const double g = num(t[4]);
+        const double ax = num(t[6]), ay = num(t[7]), az = num(t[8]);
+        const double len = std::sqrt(ax * ax + ay * ay + az * az);  // fix_gravity.cpp:379
+        if (len == 0.) die("Gravity direction vector = 0");
+        gacc[0] = g * (ax / len);
+        gacc[1] = g * (ay / len);
+        gacc[2] = g * (az / len);
+        gravity = true;
+        gravity_first = !wall_seen;
+        dirty = true;
+      } else if (style == "wall/gran") {
+        size_t k;
+        parse_model(t, 4, k, false);
+        if (k >= t.size() || t[k] != "primitive")
+          die("fix wall/gran: only 'primitive ... xplane|yplane|zplane' walls are available");
+        if (k + 4 >= t.size() || t[k + 1] != "type") die("Illegal fix wall/gran command");
+        const int wt = inum(t[k + 2]);
+        const std::string &shape = t[k + 3];
+        int ax = -1;
+        if (shape == "xplane") ax = 0;
+        else if (shape == "yplane") ax = 1;
+        else if (shape == "zplane") ax = 2;
+        else die("fix wall/gran: primitive wall style " + shape + " is not available");
+        walls.push_back({ax, num(t[k + 4]), wt});
+        if (!gravity) gravity_first = false;
+        wall_seen = true;
+        dirty = true;
+      } else {
+        die("fix style " + style + " is not available on this path");
+      }
+    } else if (c == "unfix") {
+      die("unfix is not supported on this path");
+    } else if (c == "thermo") {
+      if (t.size() < 2) die("Illegal thermo command");
+      thermo_every = inum(t[1]);
+    } else if (c == "thermo_style") {
+      if (t.size() < 2) die("Illegal thermo_style command");
+      thermo_cols.clear();
+      if (t[1] == "custom") {
+        for (size_t k = 2; k < t.size(); k++) thermo_cols.push_back(t[k]);
+      } else {
+        thermo_cols = {"step", "atoms", "ke"};
+      }
+    } else if (c == "variable") {
+      if (t.size() < 4) die("Illegal variable command");
+      if (t[2] != "equal" && t[2] != "index" && t[2] != "string")
+        die("variable style " + t[2] + " is not supported on this path");
+      if (vars.find(t[1]) == vars.end() || t[2] == "equal") vars[t[1]] = t[3];
+    } else if (c == "print") {
+      std::string rest = line.substr(line.find("print") + 5);
+      fprintf(out, "%s\n", rest.c_str());
+    } else if (c == "package") {
+      if (t.size() >= 4 && t[1] == "amd" && t[2] == "precision") {
+        if (t[3] == "fp64") precision = DEM_FP64;
+        else if (t[3] == "mixed") precision = DEM_MIXED;
+        else die("package amd precision must be fp64 or mixed");
+        dirty = true;
+      } else die("Illegal package command");
+    } else if (c == "run") {
+      run(t);
+    } else {
+      die("Unknown command: " + line);
+    }
+  }
+};
+
+}  // namespace
+
+extern "C" int dem_run_script(const char *script_path, const char *log_path) {
+  if (!script_path) return DEM_ERR_ARG;
+  std::ifstream f(script_path);
+  if (!f) {
+    fprintf(stderr, "ERROR: Cannot open input script %s\n", script_path);
+    return DEM_ERR_ARG;
+  }
+  Interp in;
+  FILE *logf = nullptr;
+  if (log_path) {
+    logf = fopen(log_path, "w");
+    if (!logf) {
+      fprintf(stderr, "ERROR: Cannot open logfile %s\n", log_path);
+      return DEM_ERR_ARG;
+    }
+    in.out = logf;
+  }
+  std::string line, acc;
+  int lineno = 0;
+  int rc = DEM_OK;
+  try {
+    while (std::getline(f, line)) {
+      lineno++;
+      // continuation lines end with '&' (input.cpp:211-220)
+      size_t endpos = line.find_last_not_of(" \t\r");
+      if (endpos != std::string::npos && line[endpos] == '&') {
+        acc += line.substr(0, endpos) + " ";
+        continue;
+      }
+      acc += line;
+      in.execute(acc);
+      acc.clear();
+    }
+  } catch (const ScriptError &err) {
+    fprintf(in.out, "ERROR: %s (line %d)\n", err.msg.c_str(), lineno);
+    if (in.out != stderr) fprintf(stderr, "ERROR: %s (line %d)\n", err.msg.c_str(), lineno);
+    rc = DEM_ERR_ARG;
+  } catch (const std::exception &err) {
+    fprintf(stderr, "ERROR: %s (line %d)\n", err.what(), lineno);
+    rc = DEM_ERR_ARG;
+  }
+  if (logf) fclose(logf);
+  return rc;
+}

@@ -1,0 +1,236 @@
+"""ctypes binding of libdem_amd.so (the C ABI in include/liggghts_amd.h).
+
+Used by the tests and bench.py to drive the GPU library from Python with
+numpy arrays. The library must have been built in-tree (``make -C
+liggghts-public_amd/csrc``); loading fails loudly otherwise, there is no
+fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdem_amd.so")
+
+DEM_FP64 = 0
+DEM_MIXED = 1
+
+# every symbol declared in include/liggghts_amd.h
+EXPORTED = [
+    "dem_create", "dem_destroy", "dem_last_error", "dem_set_materials",
+    "dem_set_particles", "dem_add_plane_wall", "dem_setup", "dem_run",
+    "dem_run_continue", "dem_compute_forces", "dem_get_state", "dem_get_pairs",
+    "dem_get_wall_history", "dem_get_stats", "dem_set_kernel_timing",
+    "dem_run_script",
+]
+
+
+class DemParams(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double),
+        ("skin", ctypes.c_double),
+        ("cdf", ctypes.c_double),
+        ("boxlo", ctypes.c_double * 3),
+        ("boxhi", ctypes.c_double * 3),
+        ("periodic", ctypes.c_int * 3),
+        ("rolling_cdt", ctypes.c_int),
+        ("tangential_damping", ctypes.c_int),
+        ("limit_force", ctypes.c_int),
+        ("torsion_torque", ctypes.c_int),
+        ("has_gravity", ctypes.c_int),
+        ("grav_acc", ctypes.c_double * 3),
+        ("gravity_first", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("max_neighbors", ctypes.c_int),
+        ("device", ctypes.c_int),
+    ]
+
+
+class DemStats(ctypes.Structure):
+    _fields_ = [
+        ("ntimestep", ctypes.c_int64),
+        ("nrebuilds", ctypes.c_int64),
+        ("npairs", ctypes.c_int64),
+        ("ncontacts", ctypes.c_int64),
+        ("nwall_contacts", ctypes.c_int64),
+        ("max_neighbors_seen", ctypes.c_int),
+        ("step_kernel_ms", ctypes.c_double),
+        ("step_kernel_launches", ctypes.c_int64),
+        ("rebuild_ms_total", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libdem_amd.so; raise if it is missing (no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it with `make -C liggghts-public_amd/csrc` "
+            "(or __graft_entry__.build()) before using the GPU path")
+    lib = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    dp = P(ctypes.c_double)
+    ip = P(ctypes.c_int)
+    h = ctypes.c_void_p
+    lib.dem_create.argtypes = [P(DemParams), P(ctypes.c_void_p)]
+    lib.dem_destroy.argtypes = [h]
+    lib.dem_last_error.argtypes = [h]
+    lib.dem_last_error.restype = ctypes.c_char_p
+    lib.dem_set_materials.argtypes = [h, ctypes.c_int, dp, dp, dp, dp, dp]
+    lib.dem_set_particles.argtypes = [h, ctypes.c_int64, dp, dp, dp, dp, dp, ip, ip]
+    lib.dem_add_plane_wall.argtypes = [h, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    lib.dem_setup.argtypes = [h]
+    lib.dem_run.argtypes = [h, ctypes.c_int64]
+    lib.dem_run_continue.argtypes = [h, ctypes.c_int64]
+    lib.dem_compute_forces.argtypes = [h, ctypes.c_int]
+    lib.dem_get_state.argtypes = [h, dp, dp, dp, dp, dp]
+    lib.dem_get_pairs.argtypes = [h, ctypes.c_int64, ip, ip, dp, P(ctypes.c_int64)]
+    lib.dem_get_wall_history.argtypes = [h, dp]
+    lib.dem_get_stats.argtypes = [h, P(DemStats)]
+    lib.dem_set_kernel_timing.argtypes = [h, ctypes.c_int]
+    lib.dem_run_script.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class DemError(RuntimeError):
+    pass
+
+
+class GpuSystem:
+    """One granular system on the GPU (a dem_handle)."""
+
+    def __init__(self, params, materials, particles, walls=(), precision=DEM_FP64,
+                 max_neighbors=0, device=0):
+        self.lib = load_library()
+        p = DemParams()
+        p.dt = params["dt"]
+        p.skin = params["skin"]
+        p.cdf = params.get("cdf", 1.0)
+        for d in range(3):
+            p.boxlo[d] = params["boxlo"][d]
+            p.boxhi[d] = params["boxhi"][d]
+            p.periodic[d] = int(params["periodic"][d])
+        p.rolling_cdt = int(params.get("rolling_cdt", 0))
+        p.tangential_damping = int(params.get("tangential_damping", 1))
+        p.limit_force = int(params.get("limit_force", 0))
+        p.torsion_torque = int(params.get("torsion_torque", 0))
+        g = params.get("grav_acc")
+        p.has_gravity = int(g is not None)
+        if g is not None:
+            for d in range(3):
+                p.grav_acc[d] = g[d]
+        p.gravity_first = int(params.get("gravity_first", 1))
+        p.precision = precision
+        p.max_neighbors = max_neighbors
+        p.device = device
+        self.h = ctypes.c_void_p()
+        rc = self.lib.dem_create(ctypes.byref(p), ctypes.byref(self.h))
+        if rc != 0:
+            raise DemError(f"dem_create failed ({rc}): {self.lib.dem_last_error(None).decode()}")
+        self.precision = precision
+        nt = materials["ntypes"]
+        Y = np.ascontiguousarray(materials["youngs"], dtype=np.float64)
+        nu = np.ascontiguousarray(materials["poisson"], dtype=np.float64)
+        e = np.ascontiguousarray(materials["restitution"], dtype=np.float64).reshape(-1)
+        mu = np.ascontiguousarray(materials["friction"], dtype=np.float64).reshape(-1)
+        mur = materials.get("rolling_friction")
+        mur = None if mur is None else np.ascontiguousarray(mur, dtype=np.float64).reshape(-1)
+        self._check(self.lib.dem_set_materials(self.h, nt, _dp(Y), _dp(nu), _dp(e), _dp(mu),
+                                               _dp(mur) if mur is not None else None))
+        self.nwalls = 0
+        for (axis, pos, wtype) in walls:
+            self._check(self.lib.dem_add_plane_wall(self.h, int(axis), float(pos), int(wtype)))
+            self.nwalls += 1
+        self.set_particles(particles)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise DemError(f"libdem_amd error {rc}: {self.lib.dem_last_error(self.h).decode()}")
+
+    def set_particles(self, P):
+        self.n = int(P["x"].shape[0])
+        arr = {k: np.ascontiguousarray(P[k], dtype=np.float64)
+               for k in ("x", "v", "omega", "radius", "rmass")}
+        ty = np.ascontiguousarray(P["type"], dtype=np.int32)
+        tg = np.ascontiguousarray(P["tag"], dtype=np.int32)
+        self._check(self.lib.dem_set_particles(
+            self.h, self.n, _dp(arr["x"]), _dp(arr["v"]), _dp(arr["omega"]),
+            _dp(arr["radius"]), _dp(arr["rmass"]), _ip(ty), _ip(tg)))
+
+    def setup(self):
+        self._check(self.lib.dem_setup(self.h))
+
+    def run(self, nsteps):
+        self._check(self.lib.dem_run(self.h, int(nsteps)))
+
+    def run_continue(self, nsteps):
+        self._check(self.lib.dem_run_continue(self.h, int(nsteps)))
+
+    def compute_forces(self, shearupdate):
+        self._check(self.lib.dem_compute_forces(self.h, int(shearupdate)))
+
+    def set_kernel_timing(self, on):
+        self._check(self.lib.dem_set_kernel_timing(self.h, int(on)))
+
+    def state(self):
+        n = self.n
+        out = {k: np.zeros((n, 3)) for k in ("x", "v", "omega", "f", "torque")}
+        self._check(self.lib.dem_get_state(self.h, _dp(out["x"]), _dp(out["v"]),
+                                           _dp(out["omega"]), _dp(out["f"]),
+                                           _dp(out["torque"])))
+        return out
+
+    def pairs(self):
+        """Half-list pairs: (tags (P,2) lower tag first, flags (P,), hist (P,3))."""
+        total = ctypes.c_int64(0)
+        self._check(self.lib.dem_get_pairs(self.h, 0, None, None, None, ctypes.byref(total)))
+        cap = max(int(total.value), 1)
+        tags = np.zeros((cap, 2), dtype=np.int32)
+        flags = np.zeros(cap, dtype=np.int32)
+        hist = np.zeros((cap, 3))
+        self._check(self.lib.dem_get_pairs(self.h, cap, _ip(tags), _ip(flags), _dp(hist),
+                                           ctypes.byref(total)))
+        npairs = int(total.value)
+        return tags[:npairs], flags[:npairs], hist[:npairs]
+
+    def wall_history(self):
+        out = np.zeros((self.n, max(self.nwalls, 1), 3))
+        if self.nwalls:
+            self._check(self.lib.dem_get_wall_history(self.h, _dp(out)))
+        return out[:, :self.nwalls, :]
+
+    def stats(self):
+        s = DemStats()
+        self._check(self.lib.dem_get_stats(self.h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in DemStats._fields_}
+
+    def close(self):
+        if self.h:
+            self.lib.dem_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_script(script, log=None):
+    lib = load_library()
+    return lib.dem_run_script(script.encode(), None if log is None else log.encode())

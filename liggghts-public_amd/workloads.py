@@ -1,0 +1,135 @@
+"""Seeded generators of the granular workloads, shared by the GPU library's
+tests, the CPU oracle and bench.py, plus a writer of the reference's
+granular data-file format (read_data Atoms ``id type diameter density x y z``
+and Velocities ``id vx vy vz wx wy wz``, atom_vec_sphere.cpp:1055-1140).
+
+Every generator returns ``(params, materials, particles, walls)``:
+  params     run parameters (dt, skin, box, periodicity, model switches, gravity)
+  materials  per-type Young's modulus / Poisson ratio and per-type-pair
+             restitution / friction / rolling friction
+  particles  x, v, omega (n,3), radius, rmass (n,), type, tag (n,) int32
+  walls      [(axis, position, wall_type)] plane walls
+"""
+import math
+
+import numpy as np
+
+MY_PI = 3.14159265358979323846
+
+
+def sphere_mass(radius, density):
+    """rmass as read_data computes it (atom_vec_sphere.cpp:1081-1082)."""
+    return 4.0 * MY_PI / 3.0 * radius * radius * radius * density
+
+
+def materials_single(youngs=5e6, poisson=0.45, restitution=0.3, friction=0.5,
+                     rolling_friction=None):
+    m = {"ntypes": 1, "youngs": [youngs], "poisson": [poisson],
+         "restitution": [[restitution]], "friction": [[friction]]}
+    if rolling_friction is not None:
+        m["rolling_friction"] = [[rolling_friction]]
+    return m
+
+
+def lattice(n, box_lo, box_hi, spacing, jitter, rng, z0=None):
+    """First n sites of a simple cubic lattice filling the box bottom-up,
+    each moved by U(-jitter, +jitter) per component."""
+    nx = int((box_hi[0] - box_lo[0]) / spacing + 1e-9)
+    ny = int((box_hi[1] - box_lo[1]) / spacing + 1e-9)
+    per_layer = nx * ny
+    nz = (n + per_layer - 1) // per_layer
+    k = np.arange(n)
+    ix, iy, iz = k % nx, (k // nx) % ny, k // per_layer
+    if z0 is None:
+        z0 = box_lo[2] + 0.5 * spacing
+    x = np.stack([box_lo[0] + (ix + 0.5) * spacing,
+                  box_lo[1] + (iy + 0.5) * spacing,
+                  z0 + iz * spacing], axis=1)
+    if z0 + (nz - 1) * spacing > box_hi[2]:
+        raise ValueError("lattice does not fit in the box")
+    x += rng.uniform(-jitter, jitter, size=x.shape)
+    return x
+
+
+def config0(seed=0, n=4000):
+    """Plain FP64 Hertz/history sphere settling, CPU-runnable: 4,000 monodisperse
+    spheres r=2 mm, density 2500, Y=5e6, nu=0.45, e=0.3, mu=0.5, box 0.1x0.1x0.2
+    periodic in x and y, zplane wall at 0, gravity -9.81 z, cubic lattice 4.4 mm
+    + U(+-0.1 mm), v ~ N(0, 0.01 m/s), dt=1e-5, skin 1 mm."""
+    rng = np.random.default_rng(seed)
+    lo, hi = (0.0, 0.0, 0.0), (0.1, 0.1, 0.2)
+    r = np.full(n, 0.002)
+    x = lattice(n, lo, hi, 0.0044, 0.0001, rng)
+    v = rng.normal(0.0, 0.01, size=(n, 3))
+    params = {"dt": 1e-5, "skin": 0.001, "cdf": 1.0, "boxlo": lo, "boxhi": hi,
+              "periodic": (1, 1, 0), "grav_acc": (0.0, 0.0, -9.81), "gravity_first": 1,
+              "rolling_cdt": 0}
+    particles = {"x": x, "v": v, "omega": np.zeros((n, 3)), "radius": r,
+                 "rmass": sphere_mass(r, 2500.0), "type": np.ones(n, dtype=np.int32),
+                 "tag": np.arange(1, n + 1, dtype=np.int32)}
+    return params, materials_single(), particles, [(2, 0.0, 1)]
+
+
+def config1(seed=0, n=1_000_000):
+    """1M polydisperse spheres settling: radii U(1.0, 1.5) mm, materials as
+    config0 plus CDT rolling friction 0.1, box 0.3x0.3x0.6 periodic in x and y,
+    zplane floor, jittered 3.3 mm lattice, dt=5e-6, skin 1 mm."""
+    rng = np.random.default_rng(seed)
+    lo, hi = (0.0, 0.0, 0.0), (0.3, 0.3, 0.6)
+    r = rng.uniform(0.001, 0.0015, size=n)
+    x = lattice(n, lo, hi, 0.0033, 0.0001, rng)
+    v = rng.normal(0.0, 0.01, size=(n, 3))
+    params = {"dt": 5e-6, "skin": 0.001, "cdf": 1.0, "boxlo": lo, "boxhi": hi,
+              "periodic": (1, 1, 0), "grav_acc": (0.0, 0.0, -9.81), "gravity_first": 1,
+              "rolling_cdt": 1}
+    particles = {"x": x, "v": v, "omega": np.zeros((n, 3)), "radius": r,
+                 "rmass": sphere_mass(r, 2500.0), "type": np.ones(n, dtype=np.int32),
+                 "tag": np.arange(1, n + 1, dtype=np.int32)}
+    return params, materials_single(rolling_friction=0.1), particles, [(2, 0.0, 1)]
+
+
+def dense_pack(seed=0, n=2000, rolling=True, overlap=0.02, box=None, periodic=(1, 1, 0),
+               spin=20.0, speed=0.05, radius=(0.001, 0.0015)):
+    """A compressed polydisperse pack for single-step parity checks: lattice
+    spacing shorter than the mean diameter, so most neighbors overlap by a few
+    percent of a radius; random translational and angular velocities so every
+    term of the contact law (damping, sliding, rolling) is exercised."""
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(radius[0], radius[1], size=n)
+    spacing = (radius[0] + radius[1]) * (1.0 - overlap)
+    side = int(math.ceil(n ** (1.0 / 3.0)))
+    if box is None:
+        L = side * spacing
+        box = ((0.0, 0.0, 0.0), (L, L, L * 1.5))
+    lo, hi = box
+    x = lattice(n, lo, hi, spacing, 0.05 * spacing, rng, z0=lo[2] + 0.5 * spacing)
+    v = rng.normal(0.0, speed, size=(n, 3))
+    w = rng.normal(0.0, spin, size=(n, 3))
+    params = {"dt": 5e-6, "skin": 0.0005, "cdf": 1.0, "boxlo": lo, "boxhi": hi,
+              "periodic": periodic, "grav_acc": (0.0, 0.0, -9.81), "gravity_first": 1,
+              "rolling_cdt": int(rolling)}
+    particles = {"x": x, "v": v, "omega": w, "radius": r,
+                 "rmass": sphere_mass(r, 2500.0), "type": np.ones(n, dtype=np.int32),
+                 "tag": np.arange(1, n + 1, dtype=np.int32)}
+    mats = materials_single(rolling_friction=0.1 if rolling else None)
+    return params, mats, particles, [(2, lo[2], 1)]
+
+
+def write_data_file(path, params, particles, ntypes=1, density=2500.0):
+    """Write a LIGGGHTS granular data file for read_data."""
+    x, v, w = particles["x"], particles["v"], particles["omega"]
+    r, t, tag = particles["radius"], particles["type"], particles["tag"]
+    lo, hi = params["boxlo"], params["boxhi"]
+    with open(path, "w") as f:
+        f.write("LIGGGHTS data file (granular), seeded synthetic workload\n\n")
+        f.write(f"{len(r)} atoms\n{ntypes} atom types\n\n")
+        for d, nm in enumerate("xyz"):
+            f.write(f"{lo[d]!r} {hi[d]!r} {nm}lo {nm}hi\n")
+        f.write("\nAtoms\n\n")
+        for i in range(len(r)):
+            f.write(f"{tag[i]} {t[i]} {2.0 * r[i]!r} {density!r} "
+                    f"{x[i, 0]!r} {x[i, 1]!r} {x[i, 2]!r}\n")
+        f.write("\nVelocities\n\n")
+        for i in range(len(r)):
+            f.write(f"{tag[i]} {v[i, 0]!r} {v[i, 1]!r} {v[i, 2]!r} "
+                    f"{w[i, 0]!r} {w[i, 1]!r} {w[i, 2]!r}\n")

@@ -1,0 +1,116 @@
+/*
+ * CPU oracle for the DEM step: a plain-C, FP64, single-threaded restatement of
+ * LIGGGHTS-PUBLIC 3.8.0's per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the shipped library links, loads or
+ * calls this code; only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg use it, as the checker.
+ *
+ * Every function cites the reference file:line it restates. Ghost atoms of the
+ * reference are replaced by the minimum-image convention on x_i - x_j: each
+ * periodic pair is stored once instead of once per side (the reference's two
+ * copies are mirror images up to rounding of the ghost coordinate).
+ */
+#ifndef DEM_ORACLE_H
+#define DEM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAXTYPE 8
+#define ORC_MAXPLANE 6
+
+/* per-type-pair mixed material matrices, indexed [itype][jtype], types 1..ntypes */
+typedef struct {
+  int ntypes;
+  double Yeff[ORC_MAXTYPE + 1][ORC_MAXTYPE + 1];
+  double Geff[ORC_MAXTYPE + 1][ORC_MAXTYPE + 1];
+  double betaeff[ORC_MAXTYPE + 1][ORC_MAXTYPE + 1];
+  double mu[ORC_MAXTYPE + 1][ORC_MAXTYPE + 1];
+  double mur[ORC_MAXTYPE + 1][ORC_MAXTYPE + 1];
+} orc_materials;
+
+/* contact-model switches: pair_style gran model hertz tangential history [...] */
+typedef struct {
+  int rolling_cdt;        /* rolling_friction cdt */
+  int tangential_damping; /* default on  (normal_model_hertz.h:90) */
+  int limit_force;        /* default off (normal_model_hertz.h:91) */
+  int torsion_torque;     /* default off (rolling_model_cdt.h:73) */
+} orc_model;
+
+typedef struct {
+  double dt;
+  double skin;
+  double cdf; /* neigh_modify contact_distance_factor (1.0 default) */
+  double boxlo[3], boxhi[3];
+  int periodic[3];
+  orc_model model;
+  int has_gravity;
+  double grav_acc[3]; /* magnitude * unit direction */
+  int nplanes;
+  int plane_axis[ORC_MAXPLANE]; /* 0=x 1=y 2=z */
+  double plane_pos[ORC_MAXPLANE];
+  int plane_type[ORC_MAXPLANE];
+  int gravity_first; /* order of post_force fixes: 1 = gravity before walls */
+} orc_params;
+
+typedef struct orc_sys orc_sys;
+
+/* --- material mixing (global_properties.cpp:411-558) ------------------- */
+void orc_mix_materials(int ntypes, const double *Y, const double *nu,
+                       const double *e /* nt*nt */, const double *mu /* nt*nt */,
+                       const double *mur /* nt*nt or NULL */, orc_materials *out);
+
+/* --- one sphere-sphere contact, i side and j side ----------------------- */
+/* del = x_i - x_j. shear (3) is read and, if shearupdate, written.
+ * Outputs are the contributions to i (dFi, dTi) and to j (dTj; force on j is -dFi).
+ * Returns 1 if the pair touches (rsq < radsum^2) and a force was produced. */
+int orc_contact_pair(const orc_model *mdl, const orc_materials *mat, int itype, int jtype,
+                     const double *del, double radi, double radj, const double *vi,
+                     const double *vj, const double *wi, const double *wj, double mi,
+                     double mj, double dt, int shearupdate, double *shear, double *dFi,
+                     double *dTi, double *dTj);
+
+/* --- one sphere-wall contact (fix_wall_gran_base.h:148-361) ------------- */
+/* sidata_deltan = overlap (>0 when touching), delta = vector from centre to
+ * wall contact point (sign as passed to compute_force after negation),
+ * vwall = wall velocity at the contact. */
+void orc_contact_wall(const orc_model *mdl, const orc_materials *mat, int itype, int wtype,
+                      double sidata_deltan, const double *sidata_delta, double radi,
+                      const double *vi, const double *wi, const double *vwall, double mi,
+                      double dt, int shearupdate, double *shear, double *dFi, double *dTi);
+
+/* --- brute-force O(N^2) half pair set (neigh_gran.cpp:571-582 predicate) - */
+/* writes unordered pairs (i<j by index) into pairs[2*k]; returns the count or
+ * -(needed) if cap is too small */
+int64_t orc_pairs_bruteforce(int n, const double *x, const double *radius,
+                             const orc_params *p, int64_t cap, int *pairs);
+
+/* --- system life cycle (Verlet::setup / Verlet::run) --------------------- */
+orc_sys *orc_create(const orc_params *p, const orc_materials *m, int n, const double *x,
+                    const double *v, const double *omega, const double *radius,
+                    const double *rmass, const int *type, const int *tag);
+void orc_destroy(orc_sys *s);
+void orc_setup(orc_sys *s);
+void orc_run(orc_sys *s, int nsteps);
+/* one force evaluation on the current state (no integration), with
+ * shearupdate as given; leaves f and torque in the system */
+void orc_compute_forces(orc_sys *s, int shearupdate);
+void orc_get_state(const orc_sys *s, double *x, double *v, double *omega, double *f,
+                   double *torque);
+/* pairs of the current half list, as (tag_a, tag_b) with tag_a < tag_b,
+ * contact flag, and the shear history oriented from tag_a towards tag_b */
+int64_t orc_get_pairs(const orc_sys *s, int64_t cap, int *tags, int *flags, double *hist);
+void orc_get_wall_hist(const orc_sys *s, double *hist /* n*nplanes*3 */);
+int64_t orc_nrebuilds(const orc_sys *s);
+int64_t orc_ntimestep(const orc_sys *s);
+/* count of touching half-list pairs and of touching particle-wall contacts */
+void orc_contact_counts(const orc_sys *s, int64_t *npair_touch, int64_t *nwall_touch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,255 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, never by the shipped library.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+MAXT = 8
+MAXPLANE = 6
+
+_D = ctypes.c_double
+_Mat = (_D * (MAXT + 1)) * (MAXT + 1)
+
+
+class OrcMaterials(ctypes.Structure):
+    _fields_ = [("ntypes", ctypes.c_int), ("Yeff", _Mat), ("Geff", _Mat), ("betaeff", _Mat),
+                ("mu", _Mat), ("mur", _Mat)]
+
+
+class OrcModel(ctypes.Structure):
+    _fields_ = [("rolling_cdt", ctypes.c_int), ("tangential_damping", ctypes.c_int),
+                ("limit_force", ctypes.c_int), ("torsion_torque", ctypes.c_int)]
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("dt", _D), ("skin", _D), ("cdf", _D),
+        ("boxlo", _D * 3), ("boxhi", _D * 3), ("periodic", ctypes.c_int * 3),
+        ("model", OrcModel),
+        ("has_gravity", ctypes.c_int), ("grav_acc", _D * 3),
+        ("nplanes", ctypes.c_int), ("plane_axis", ctypes.c_int * MAXPLANE),
+        ("plane_pos", _D * MAXPLANE), ("plane_type", ctypes.c_int * MAXPLANE),
+        ("gravity_first", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        subprocess.check_call(["make", "-C", HERE, "liboracle.so"])
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    dp, ip = P(_D), P(ctypes.c_int)
+    lib.orc_mix_materials.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, P(OrcMaterials)]
+    lib.orc_contact_pair.argtypes = [P(OrcModel), P(OrcMaterials), ctypes.c_int, ctypes.c_int,
+                                     dp, _D, _D, dp, dp, dp, dp, _D, _D, _D, ctypes.c_int,
+                                     dp, dp, dp, dp]
+    lib.orc_contact_pair.restype = ctypes.c_int
+    lib.orc_contact_wall.argtypes = [P(OrcModel), P(OrcMaterials), ctypes.c_int, ctypes.c_int,
+                                     _D, dp, _D, dp, dp, dp, _D, _D, ctypes.c_int, dp, dp, dp]
+    lib.orc_pairs_bruteforce.argtypes = [ctypes.c_int, dp, dp, P(OrcParams), ctypes.c_int64, ip]
+    lib.orc_pairs_bruteforce.restype = ctypes.c_int64
+    lib.orc_create.argtypes = [P(OrcParams), P(OrcMaterials), ctypes.c_int, dp, dp, dp, dp, dp,
+                               ip, ip]
+    lib.orc_create.restype = ctypes.c_void_p
+    lib.orc_destroy.argtypes = [ctypes.c_void_p]
+    lib.orc_setup.argtypes = [ctypes.c_void_p]
+    lib.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.orc_compute_forces.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.orc_get_state.argtypes = [ctypes.c_void_p, dp, dp, dp, dp, dp]
+    lib.orc_get_pairs.argtypes = [ctypes.c_void_p, ctypes.c_int64, ip, ip, dp]
+    lib.orc_get_pairs.restype = ctypes.c_int64
+    lib.orc_get_wall_hist.argtypes = [ctypes.c_void_p, dp]
+    lib.orc_nrebuilds.argtypes = [ctypes.c_void_p]
+    lib.orc_nrebuilds.restype = ctypes.c_int64
+    lib.orc_ntimestep.argtypes = [ctypes.c_void_p]
+    lib.orc_ntimestep.restype = ctypes.c_int64
+    lib.orc_contact_counts.argtypes = [ctypes.c_void_p, P(ctypes.c_int64), P(ctypes.c_int64)]
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(_D))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def materials(mat):
+    """Mix a material dict (youngs, poisson, restitution, friction[, rolling_friction])."""
+    lib = load()
+    nt = mat["ntypes"]
+    Y = np.ascontiguousarray(mat["youngs"], dtype=np.float64)
+    nu = np.ascontiguousarray(mat["poisson"], dtype=np.float64)
+    e = np.ascontiguousarray(mat["restitution"], dtype=np.float64).reshape(-1)
+    mu = np.ascontiguousarray(mat["friction"], dtype=np.float64).reshape(-1)
+    mur = mat.get("rolling_friction")
+    mur = None if mur is None else np.ascontiguousarray(mur, dtype=np.float64).reshape(-1)
+    out = OrcMaterials()
+    lib.orc_mix_materials(nt, _dp(Y), _dp(nu), _dp(e), _dp(mu),
+                          _dp(mur) if mur is not None else None, ctypes.byref(out))
+    return out
+
+
+def model(params):
+    m = OrcModel()
+    m.rolling_cdt = int(params.get("rolling_cdt", 0))
+    m.tangential_damping = int(params.get("tangential_damping", 1))
+    m.limit_force = int(params.get("limit_force", 0))
+    m.torsion_torque = int(params.get("torsion_torque", 0))
+    return m
+
+
+def make_params(params, walls=()):
+    p = OrcParams()
+    p.dt = params["dt"]
+    p.skin = params["skin"]
+    p.cdf = params.get("cdf", 1.0)
+    for d in range(3):
+        p.boxlo[d] = params["boxlo"][d]
+        p.boxhi[d] = params["boxhi"][d]
+        p.periodic[d] = int(params["periodic"][d])
+    p.model = model(params)
+    g = params.get("grav_acc")
+    p.has_gravity = int(g is not None)
+    if g is not None:
+        for d in range(3):
+            p.grav_acc[d] = g[d]
+    p.nplanes = len(walls)
+    for k, (axis, pos, wtype) in enumerate(walls):
+        p.plane_axis[k] = int(axis)
+        p.plane_pos[k] = float(pos)
+        p.plane_type[k] = int(wtype)
+    p.gravity_first = int(params.get("gravity_first", 1))
+    return p
+
+
+def contact_pair(params, mats, itype, jtype, del_, radi, radj, vi, vj, wi, wj, mi, mj, dt,
+                 shearupdate, shear):
+    """One pair contact; returns (touch, dFi, dTi, dTj, shear_after)."""
+    lib = load()
+    mdl = model(params)
+    arr = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    d, vi, vj, wi, wj = arr(del_), arr(vi), arr(vj), arr(wi), arr(wj)
+    sh = arr(shear).copy()
+    dF, dTi, dTj = np.zeros(3), np.zeros(3), np.zeros(3)
+    touch = lib.orc_contact_pair(ctypes.byref(mdl), ctypes.byref(mats), itype, jtype, _dp(d),
+                                 radi, radj, _dp(vi), _dp(vj), _dp(wi), _dp(wj), mi, mj, dt,
+                                 int(shearupdate), _dp(sh), _dp(dF), _dp(dTi), _dp(dTj))
+    return bool(touch), dF, dTi, dTj, sh
+
+
+def contact_wall(params, mats, itype, wtype, overlap, sdelta, radi, vi, wi, vwall, mi, dt,
+                 shearupdate, shear):
+    lib = load()
+    mdl = model(params)
+    arr = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    sd, vi, wi, vw = arr(sdelta), arr(vi), arr(wi), arr(vwall)
+    sh = arr(shear).copy()
+    dF, dT = np.zeros(3), np.zeros(3)
+    lib.orc_contact_wall(ctypes.byref(mdl), ctypes.byref(mats), itype, wtype, overlap, _dp(sd),
+                         radi, _dp(vi), _dp(wi), _dp(vw), mi, dt, int(shearupdate), _dp(sh),
+                         _dp(dF), _dp(dT))
+    return dF, dT, sh
+
+
+def pairs_bruteforce(params, x, radius):
+    lib = load()
+    p = make_params(params)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    r = np.ascontiguousarray(radius, dtype=np.float64)
+    cap = 64 * len(r) + 16
+    while True:
+        out = np.zeros((cap, 2), dtype=np.int32)
+        n = lib.orc_pairs_bruteforce(len(r), _dp(x), _dp(r), ctypes.byref(p), cap, _ip(out))
+        if n >= 0:
+            return out[:n]
+        cap = int(-n) + 16
+
+
+class OracleSystem:
+    """The CPU restatement of Verlet::setup / Verlet::run."""
+
+    def __init__(self, params, mat, particles, walls=()):
+        self.lib = load()
+        self.p = make_params(params, walls)
+        self.m = materials(mat)
+        P = {k: np.ascontiguousarray(particles[k], dtype=np.float64)
+             for k in ("x", "v", "omega", "radius", "rmass")}
+        ty = np.ascontiguousarray(particles["type"], dtype=np.int32)
+        tg = np.ascontiguousarray(particles["tag"], dtype=np.int32)
+        self.n = len(ty)
+        self.nwalls = len(walls)
+        self.s = self.lib.orc_create(ctypes.byref(self.p), ctypes.byref(self.m), self.n,
+                                     _dp(P["x"]), _dp(P["v"]), _dp(P["omega"]),
+                                     _dp(P["radius"]), _dp(P["rmass"]), _ip(ty), _ip(tg))
+
+    def setup(self):
+        self.lib.orc_setup(self.s)
+
+    def run(self, nsteps):
+        """Verlet::setup followed by nsteps steps (one `run` command)."""
+        self.lib.orc_setup(self.s)
+        self.lib.orc_run(self.s, int(nsteps))
+
+    def run_continue(self, nsteps):
+        self.lib.orc_run(self.s, int(nsteps))
+
+    def compute_forces(self, shearupdate):
+        self.lib.orc_compute_forces(self.s, int(shearupdate))
+
+    def state(self):
+        n = self.n
+        out = {k: np.zeros((n, 3)) for k in ("x", "v", "omega", "f", "torque")}
+        self.lib.orc_get_state(self.s, _dp(out["x"]), _dp(out["v"]), _dp(out["omega"]),
+                               _dp(out["f"]), _dp(out["torque"]))
+        return out
+
+    def pairs(self):
+        cap = 64 * self.n + 16
+        while True:
+            tags = np.zeros((cap, 2), dtype=np.int32)
+            flags = np.zeros(cap, dtype=np.int32)
+            hist = np.zeros((cap, 3))
+            n = self.lib.orc_get_pairs(self.s, cap, _ip(tags), _ip(flags), _dp(hist))
+            if n >= 0:
+                return tags[:n], flags[:n], hist[:n]
+            cap = int(-n) + 16
+
+    def wall_history(self):
+        out = np.zeros((self.n, max(self.nwalls, 1), 3))
+        self.lib.orc_get_wall_hist(self.s, _dp(out))
+        return out[:, :self.nwalls, :]
+
+    def nrebuilds(self):
+        return self.lib.orc_nrebuilds(self.s)
+
+    def contact_counts(self):
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self.lib.orc_contact_counts(self.s, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def close(self):
+        if self.s:
+            self.lib.orc_destroy(self.s)
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
