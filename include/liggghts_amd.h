@@ -69,6 +69,14 @@ typedef struct dem_params {
   int device;         /* HIP device ordinal */
 } dem_params;
 
+/* Switches of one contact model (pair_style gran / fix wall/gran keywords). */
+typedef struct dem_contact_model {
+  int rolling_cdt;        /* rolling_friction cdt */
+  int tangential_damping; /* tangential_damping on (default) */
+  int limit_force;        /* limitForce on */
+  int torsion_torque;     /* torsionTorque on */
+} dem_contact_model;
+
 typedef struct dem_stats {
   int64_t ntimestep;       /* steps run since creation (update->ntimestep) */
   int64_t nrebuilds;       /* neighbor list builds, setup builds included */
@@ -76,7 +84,7 @@ typedef struct dem_stats {
   int64_t ncontacts;       /* touching half-list pairs at the last force pass */
   int64_t nwall_contacts;  /* touching particle-wall contacts at the last force pass */
   int max_neighbors_seen;  /* widest per-particle list at the last build */
-  double step_kernel_ms;   /* average duration of the fused step kernel (HIP events) */
+  double step_kernel_ms;   /* average time per fused step-kernel launch (HIP events) */
   int64_t step_kernel_launches;
   double rebuild_ms_total; /* wall time inside neighbor rebuilds */
 } dem_stats;
@@ -110,9 +118,13 @@ int dem_set_particles(dem_handle *h, int64_t n, const double *x, const double *v
                       const double *omega, const double *radius, const double *rmass,
                       const int *type, const int *tag);
 
-/* Adds a planar primitive wall: fix wall/gran ... primitive type <t> {x,y,z}plane <pos>
- * (fix_wall_gran.cpp:188-227, primitive_wall_definitions.h:130-150). axis 0/1/2. */
-int dem_add_plane_wall(dem_handle *h, int axis, double position, int wall_type);
+/* Adds a planar primitive wall: fix wall/gran model hertz tangential history
+ * [rolling_friction cdt] ... primitive type <t> {x,y,z}plane <pos>
+ * (fix_wall_gran.cpp:168-227, primitive_wall_definitions.h:130-150). axis 0/1/2.
+ * model holds the fix's own contact-model switches (each fix wall/gran selects
+ * its model independently of pair_style); NULL means the pair style's. */
+int dem_add_plane_wall(dem_handle *h, int axis, double position, int wall_type,
+                       const dem_contact_model *model);
 
 /* Verlet::setup (verlet.cpp:134-205): carries contact history out, wraps periodic
  * coordinates, reorders particles by Morton cell key, rebuilds the neighbor list
@@ -158,9 +170,11 @@ int dem_get_wall_history(dem_handle *h, double *hist);
 /* Counters and timing of the handle. */
 int dem_get_stats(dem_handle *h, dem_stats *out);
 
-/* Brackets every step-kernel launch of dem_run with HIP events on the handle's
- * stream and accumulates their durations into dem_stats (on != 0), or stops
- * doing so. Resets the accumulated timing either way. No reference counterpart:
+/* Brackets each batch of step-kernel launches of dem_run / dem_run_continue
+ * with a pair of HIP events on the handle's stream and accumulates the
+ * per-launch average into dem_stats (on != 0), or stops doing so; batches
+ * that end in a rebuild request are not counted. Resets the accumulated
+ * timing either way. No reference counterpart:
  * the reference's Timer buckets (timer.h:51-52) play this role on the CPU. */
 int dem_set_kernel_timing(dem_handle *h, int on);
 

@@ -30,6 +30,7 @@ struct DevParams {
   int wall_axis[kMaxWalls];
   double wall_pos[kMaxWalls];
   int wall_type[kMaxWalls];
+  int wall_model[kMaxWalls];  // bit 0 rolling cdt, 1 tangential damping, 2 limitForce, 3 torsion
   double cutneighmax;
   int ntp1;                // ntypes + 1 (row length of the material tables)
   // cell grid

@@ -60,6 +60,8 @@ __device__ __forceinline__ void min_image(const DevParams &p, double xi, double 
   }
 }
 
+constexpr int kGroup = 4;   // neighbor entries whose loads are issued together
+
 template <typename T> struct OmgT;
 template <> struct OmgT<double> { typedef double4 type; };
 template <> struct OmgT<float> { typedef float4 type; };
@@ -79,8 +81,17 @@ struct CIn {
   T Y, G, beta, mu, mur;
 };
 
+// contact-model switches: rolling cdt, tangential damping, limitForce, torsionTorque
+struct Model {
+  int rolling, tdamp, limitf, torsion;
+};
+
+__device__ __forceinline__ Model model_from_bits(int b) {
+  return Model{b & 1, (b >> 1) & 1, (b >> 2) & 1, (b >> 3) & 1};
+}
+
 template <typename T, bool WALL>
-__device__ __forceinline__ void contact_chain(const DevParams &p, const CIn<T> &c, T dt,
+__device__ __forceinline__ void contact_chain(const Model &p, const CIn<T> &c, T dt,
                                               int shearupdate, T sh[3], T dF[3], T dTi[3],
                                               T dTj[3]) {
   const T enx = c.en[0], eny = c.en[1], enz = c.en[2];
@@ -243,11 +254,37 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
     const int cnt = meta & kMetaCountMask;
     const int ntp1 = p.ntp1;
     const T dt = (T)p.dt;
+    const Model pm{p.rolling, p.tdamp, p.limitf, p.torsion};
     T fx = T(0), fy = T(0), fz = T(0), tx = T(0), ty = T(0), tz = T(0);
     T *hist = (T *)a.list.hist;
     const size_t cap = (size_t)a.cap;
 
-    for (int k = 0; k < cnt; k++) {
+    // Broad phase in groups of kGroup entries: the index loads of a group are
+    // independent, then its position gathers, so a wavefront has kGroup
+    // memory requests in flight instead of one dependent chain per entry.
+    // Touching entries of the group are then resolved one by one, reloading
+    // their (cache-resident) index and position so the group's registers are
+    // free during the contact law.
+    for (int k0 = 0; k0 < cnt; k0 += kGroup) {
+      uint32_t ent[kGroup];
+#pragma unroll
+      for (int u = 0; u < kGroup; u++)
+        ent[u] = (k0 + u < cnt) ? a.list.nbr[(size_t)(k0 + u) * cap + i] : (uint32_t)i;
+      double4 pj[kGroup];
+#pragma unroll
+      for (int u = 0; u < kGroup; u++) pj[u] = a.in.pos[ent[u] & kIndexMask];
+      uint32_t bits = 0u;
+#pragma unroll
+      for (int u = 0; u < kGroup; u++) {
+        double dx, dy, dz;
+        min_image(p, pi.x, pi.y, pi.z, pj[u].x, pj[u].y, pj[u].z, dx, dy, dz);
+        const double rsq = dx * dx + dy * dy + dz * dz;
+        const double radsum = pi.w + pj[u].w;
+        if (k0 + u < cnt && rsq < radsum * radsum) bits |= 1u << u;
+      }
+     while (bits) {
+      const int k = k0 + __builtin_ctz(bits);
+      bits &= bits - 1u;
       const size_t slot = (size_t)k * cap + i;
       const uint32_t e = a.list.nbr[slot];
       const int j = (int)(e & kIndexMask);
@@ -256,7 +293,6 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
       min_image(p, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, dx, dy, dz);
       const double rsq = dx * dx + dy * dy + dz * dz;
       const double radsum = pi.w + pj.w;
-      if (!(rsq < radsum * radsum)) continue;
       // touching: pair_gran_base.h:356-430
       const double4 vj = a.in.vel[j];
       const omg_t wj = ((const omg_t *)a.in.omg)[j];
@@ -310,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
         sh[0] = sh[1] = sh[2] = T(0);
       }
       T dF[3], dTi[3], dTj[3];
-      contact_chain<T, false>(p, c, dt, a.shearupdate, sh, dF, dTi, dTj);
+      contact_chain<T, false>(pm, c, dt, a.shearupdate, sh, dF, dTi, dTj);
       if (a.shearupdate || !flagged) {
         hist[hb] = sh[0];
         hist[hb + cap] = sh[1];
@@ -324,6 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
       ty += dTi[1];
       tz += dTi[2];
       my_contacts++;
+     }
     }
 
     // post_force fixes in script order: gravity (fix_gravity.cpp:331-338) and
@@ -368,12 +405,15 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
           c.meff = (T)vi.w;
           c.deltan = (T)(pi.w - r);
           c.deltan_in = (T)sdn;
-          c.del[0] = c.del[1] = c.del[2] = T(0);
-          c.del[ax] = (T)sd;
-          c.en[0] = (T)(0.0 * rinv);
-          c.en[1] = (T)(0.0 * rinv);
-          c.en[2] = (T)(0.0 * rinv);
-          c.en[ax] = (T)(sd * rinv);
+          // select instead of c.del[ax] = ...: a dynamic index would put the
+          // struct in scratch memory
+          const T sdT = (T)sd, enA = (T)(sd * rinv), enZ = (T)(0.0 * rinv);
+          c.del[0] = ax == 0 ? sdT : T(0);
+          c.del[1] = ax == 1 ? sdT : T(0);
+          c.del[2] = ax == 2 ? sdT : T(0);
+          c.en[0] = ax == 0 ? enA : enZ;
+          c.en[1] = ax == 1 ? enA : enZ;
+          c.en[2] = ax == 2 ? enA : enZ;
           c.vr[0] = (T)(vi.x - 0.0);
           c.vr[1] = (T)(vi.y - 0.0);
           c.vr[2] = (T)(vi.z - 0.0);
@@ -404,7 +444,8 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
             sh[0] = sh[1] = sh[2] = T(0);
           }
           T dF[3], dTi[3], dTj[3];
-          contact_chain<T, true>(p, c, dt, a.shearupdate, sh, dF, dTi, dTj);
+          contact_chain<T, true>(model_from_bits(p.wall_model[w]), c, dt, a.shearupdate, sh, dF,
+                                 dTi, dTj);
           wh[wb] = sh[0];
           wh[wb + cap] = sh[1];
           wh[wb + 2 * cap] = sh[2];

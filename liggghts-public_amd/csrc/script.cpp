@@ -90,6 +90,7 @@ struct Interp {
     int axis;
     double pos;
     int type;
+    dem_contact_model model;
   };
   std::vector<PlaneWall> walls;
   int precision = DEM_FP64;
@@ -246,13 +247,22 @@ struct Interp {
   }
 
   void fix_property_global(const std::vector<std::string> &t) {
-    // fix id group property/global <name> peratomtype|peratomtypepair [n] values...
-    if (t.size() < 6) die("Illegal fix property/global command");
-    const std::string &name = t[3];
-    const std::string &style = t[4];
+    // fix id group property/global <name> peratomtype v1..vn
+    // fix id group property/global <name> peratomtypepair n v11 v12 ... vnn
+    // (fix_property_global.cpp:85-102)
+    if (t.size() < 7) die("Illegal fix property/global command");
+    const std::string &name = t[4];
+    const std::string &style = t[5];
     std::vector<double> vals;
-    size_t k = 5;
-    if (style == "peratomtypepair") k = 6;
+    size_t k = 6;
+    if (style == "peratomtypepair") {
+      const int n = inum(t[6]);
+      if (n < 1) die("Illegal fix property/global command");
+      k = 7;
+      if (t.size() - k != (size_t)n * n) die("fix property/global " + name + ": expected n*n values");
+    } else if (style != "peratomtype" && style != "scalar" && style != "vector") {
+      die("Unknown fix property/global style " + style);
+    }
     for (; k < t.size(); k++) vals.push_back(num(t[k]));
     if (name == "youngsModulus") Y = vals;
     else if (name == "poissonsRatio") nu = vals;
@@ -263,7 +273,8 @@ struct Interp {
     dirty = true;
   }
 
-  void parse_model(const std::vector<std::string> &t, size_t k, size_t &end, bool pair) {
+  void parse_model(const std::vector<std::string> &t, size_t k, size_t &end, bool pair,
+                   dem_contact_model *wall_model = nullptr) {
     // model hertz tangential history [rolling_friction cdt] [settings...]
     bool saw_model = false, saw_tang = false;
     int roll = 0, td = 1, lf = 0, tor = 0;
@@ -305,8 +316,11 @@ struct Interp {
       tdamp = td;
       limitf = lf;
       torsion = tor;
-    } else if (roll != rolling || td != tdamp || lf != limitf || tor != torsion) {
-      die("fix wall/gran model settings must match pair_style gran on this path");
+    } else if (wall_model) {
+      wall_model->rolling_cdt = roll;
+      wall_model->tangential_damping = td;
+      wall_model->limit_force = lf;
+      wall_model->torsion_torque = tor;
     }
   }
 
@@ -320,7 +334,9 @@ struct Interp {
       die("youngsModulus/poissonsRatio need one value per atom type");
     if ((int)e.size() != ntypes * ntypes || (int)mu.size() != ntypes * ntypes)
       die("coefficientRestitution/coefficientFriction need ntypes^2 values");
-    if (rolling && (int)mur.size() != ntypes * ntypes)
+    bool any_rolling = rolling;
+    for (auto &wl : walls) any_rolling = any_rolling || wl.model.rolling_cdt;
+    if (any_rolling && (int)mur.size() != ntypes * ntypes)
       die("rolling_friction cdt needs coefficientRollingFriction (ntypes^2 values)");
     if (h) {
       dem_destroy(h);
@@ -348,10 +364,10 @@ struct Interp {
     int rc = dem_create(&p, &h);
     if (rc) die(std::string("GPU library: ") + dem_last_error(nullptr));
     if (dem_set_materials(h, ntypes, Y.data(), nu.data(), e.data(), mu.data(),
-                          rolling ? mur.data() : nullptr))
+                          any_rolling ? mur.data() : nullptr))
       die(dem_last_error(h));
     for (auto &wl : walls)
-      if (dem_add_plane_wall(h, wl.axis, wl.pos, wl.type)) die(dem_last_error(h));
+      if (dem_add_plane_wall(h, wl.axis, wl.pos, wl.type, &wl.model)) die(dem_last_error(h));
     if (tag.empty()) die("run: no atoms");
     if (dem_set_particles(h, (int64_t)tag.size(), x.data(), v.data(), w.data(), r.data(), m.data(),
                           type.data(), tag.data()))
@@ -536,7 +552,8 @@ struct Interp {
         dirty = true;
       } else if (style == "wall/gran") {
         size_t k;
-        parse_model(t, 4, k, false);
+        dem_contact_model wm{};
+        parse_model(t, 4, k, false, &wm);
         if (k >= t.size() || t[k] != "primitive")
           die("fix wall/gran: only 'primitive ... xplane|yplane|zplane' walls are available");
         if (k + 4 >= t.size() || t[k + 1] != "type") die("Illegal fix wall/gran command");
@@ -547,7 +564,7 @@ struct Interp {
         else if (shape == "yplane") ax = 1;
         else if (shape == "zplane") ax = 2;
         else die("fix wall/gran: primitive wall style " + shape + " is not available");
-        walls.push_back({ax, num(t[k + 4]), wt});
+        walls.push_back({ax, num(t[k + 4]), wt, wm});
         if (!gravity) gravity_first = false;
         wall_seen = true;
         dirty = true;

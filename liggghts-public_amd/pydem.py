@@ -47,6 +47,20 @@ class DemParams(ctypes.Structure):
     ]
 
 
+class DemContactModel(ctypes.Structure):
+    _fields_ = [("rolling_cdt", ctypes.c_int), ("tangential_damping", ctypes.c_int),
+                ("limit_force", ctypes.c_int), ("torsion_torque", ctypes.c_int)]
+
+
+def contact_model(d):
+    m = DemContactModel()
+    m.rolling_cdt = int(d.get("rolling_cdt", 0))
+    m.tangential_damping = int(d.get("tangential_damping", 1))
+    m.limit_force = int(d.get("limit_force", 0))
+    m.torsion_torque = int(d.get("torsion_torque", 0))
+    return m
+
+
 class DemStats(ctypes.Structure):
     _fields_ = [
         ("ntimestep", ctypes.c_int64),
@@ -84,7 +98,8 @@ def load_library(path=LIB_PATH):
     lib.dem_last_error.restype = ctypes.c_char_p
     lib.dem_set_materials.argtypes = [h, ctypes.c_int, dp, dp, dp, dp, dp]
     lib.dem_set_particles.argtypes = [h, ctypes.c_int64, dp, dp, dp, dp, dp, ip, ip]
-    lib.dem_add_plane_wall.argtypes = [h, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    lib.dem_add_plane_wall.argtypes = [h, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                       P(DemContactModel)]
     lib.dem_setup.argtypes = [h]
     lib.dem_run.argtypes = [h, ctypes.c_int64]
     lib.dem_run_continue.argtypes = [h, ctypes.c_int64]
@@ -153,8 +168,11 @@ class GpuSystem:
         self._check(self.lib.dem_set_materials(self.h, nt, _dp(Y), _dp(nu), _dp(e), _dp(mu),
                                                _dp(mur) if mur is not None else None))
         self.nwalls = 0
-        for (axis, pos, wtype) in walls:
-            self._check(self.lib.dem_add_plane_wall(self.h, int(axis), float(pos), int(wtype)))
+        for wall in walls:
+            axis, pos, wtype = wall[:3]
+            m = contact_model(wall[3] if len(wall) > 3 else params)
+            self._check(self.lib.dem_add_plane_wall(self.h, int(axis), float(pos), int(wtype),
+                                                    ctypes.byref(m)))
             self.nwalls += 1
         self.set_particles(particles)
 

@@ -117,9 +117,15 @@ def dense_pack(seed=0, n=2000, rolling=True, overlap=0.02, box=None, periodic=(1
 
 def write_data_file(path, params, particles, ntypes=1, density=2500.0):
     """Write a LIGGGHTS granular data file for read_data."""
-    x, v, w = particles["x"], particles["v"], particles["omega"]
-    r, t, tag = particles["radius"], particles["type"], particles["tag"]
-    lo, hi = params["boxlo"], params["boxhi"]
+    x = np.asarray(particles["x"], dtype=np.float64).tolist()
+    v = np.asarray(particles["v"], dtype=np.float64).tolist()
+    w = np.asarray(particles["omega"], dtype=np.float64).tolist()
+    r = np.asarray(particles["radius"], dtype=np.float64).tolist()
+    t = np.asarray(particles["type"]).astype(int).tolist()
+    tag = np.asarray(particles["tag"]).astype(int).tolist()
+    lo = [float(a) for a in params["boxlo"]]
+    hi = [float(a) for a in params["boxhi"]]
+    density = float(density)
     with open(path, "w") as f:
         f.write("LIGGGHTS data file (granular), seeded synthetic workload\n\n")
         f.write(f"{len(r)} atoms\n{ntypes} atom types\n\n")
@@ -128,8 +134,8 @@ def write_data_file(path, params, particles, ntypes=1, density=2500.0):
         f.write("\nAtoms\n\n")
         for i in range(len(r)):
             f.write(f"{tag[i]} {t[i]} {2.0 * r[i]!r} {density!r} "
-                    f"{x[i, 0]!r} {x[i, 1]!r} {x[i, 2]!r}\n")
+                    f"{x[i][0]!r} {x[i][1]!r} {x[i][2]!r}\n")
         f.write("\nVelocities\n\n")
         for i in range(len(r)):
-            f.write(f"{tag[i]} {v[i, 0]!r} {v[i, 1]!r} {v[i, 2]!r} "
-                    f"{w[i, 0]!r} {w[i, 1]!r} {w[i, 2]!r}\n")
+            f.write(f"{tag[i]} {v[i][0]!r} {v[i][1]!r} {v[i][2]!r} "
+                    f"{w[i][0]!r} {w[i][1]!r} {w[i][2]!r}\n")

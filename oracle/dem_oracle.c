@@ -660,7 +660,7 @@ static void wall_compute(orc_sys *s, int shearupdate) {
         if (!(deltan <= 0)) continue; /* surfacesClose: no force, history kept */
         const double sdelta[3] = {-delta[0], -delta[1], -delta[2]};
         double dF[3], dT[3];
-        orc_contact_wall(&s->p.model, &s->m, s->type[i], s->p.plane_type[k], -deltan, sdelta,
+        orc_contact_wall(&s->p.plane_model[k], &s->m, s->type[i], s->p.plane_type[k], -deltan, sdelta,
                          s->r[i], &s->v[3 * i], &s->w[3 * i], zero, s->rm[i], s->p.dt,
                          shearupdate, h, dF, dT);
         for (int d = 0; d < 3; d++) {

@@ -55,6 +55,7 @@ typedef struct {
   double plane_pos[ORC_MAXPLANE];
   int plane_type[ORC_MAXPLANE];
   int gravity_first; /* order of post_force fixes: 1 = gravity before walls */
+  orc_model plane_model[ORC_MAXPLANE]; /* each fix wall/gran's own model switches */
 } orc_params;
 
 typedef struct orc_sys orc_sys;

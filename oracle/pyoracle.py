@@ -37,6 +37,7 @@ class OrcParams(ctypes.Structure):
         ("nplanes", ctypes.c_int), ("plane_axis", ctypes.c_int * MAXPLANE),
         ("plane_pos", _D * MAXPLANE), ("plane_type", ctypes.c_int * MAXPLANE),
         ("gravity_first", ctypes.c_int),
+        ("plane_model", OrcModel * MAXPLANE),
     ]
 
 
@@ -130,10 +131,12 @@ def make_params(params, walls=()):
         for d in range(3):
             p.grav_acc[d] = g[d]
     p.nplanes = len(walls)
-    for k, (axis, pos, wtype) in enumerate(walls):
+    for k, wall in enumerate(walls):
+        axis, pos, wtype = wall[:3]
         p.plane_axis[k] = int(axis)
         p.plane_pos[k] = float(pos)
         p.plane_type[k] = int(wtype)
+        p.plane_model[k] = model(wall[3] if len(wall) > 3 else params)
     p.gravity_first = int(params.get("gravity_first", 1))
     return p
 

@@ -1,0 +1,93 @@
+"""CPU-side checks of the input-script front end (dem_run_script): the
+reference's commands for this path parse, errors surface as a status and a
+message (the reference's error->all), and anything outside the path is
+refused instead of silently ignored. No `run` here, so no GPU is touched,
+except the last check, which asks for a run without a GPU and expects a clean
+error."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+import pydem
+import workloads as wl
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+HEADER = """atom_style granular
+atom_modify map array
+boundary p p f
+newton off
+communicate single vel yes
+units si
+region reg block 0 0.1 0 0.1 0 0.2 units box
+create_box 1 reg
+neighbor 0.001 bin
+neigh_modify delay 0
+fix m1 all property/global youngsModulus peratomtype 5.e6
+fix m2 all property/global poissonsRatio peratomtype 0.45
+fix m3 all property/global coefficientRestitution peratomtypepair 1 0.3
+fix m4 all property/global coefficientFriction peratomtypepair 1 0.5
+fix m5 all property/global coefficientRollingFriction peratomtypepair 1 0.1
+pair_style gran model hertz tangential history rolling_friction cdt
+pair_coeff * *
+timestep 0.00001
+variable dz equal 0.0
+fix gravi all gravity 9.81 vector 0.0 0.0 -1.0
+fix zwall all wall/gran model hertz tangential history rolling_friction cdt primitive type 1 zplane ${dz}
+fix integr all nve/sphere
+compute 1 all erotate/sphere
+thermo_style custom step atoms ke c_1
+thermo 1000
+"""
+
+
+def run_script(tmp_path, text, name="in.test"):
+    p = tmp_path / name
+    p.write_text(text)
+    log = tmp_path / (name + ".log")
+    rc = pydem.run_script(str(p), str(log))
+    return rc, log.read_text() if log.exists() else ""
+
+
+def test_full_command_subset_parses(tmp_path):
+    params, mats, P, walls = wl.config0(seed=0, n=200)
+    data = tmp_path / "d.data"
+    wl.write_data_file(str(data), params, P)
+    rc, log = run_script(tmp_path, HEADER + f"read_data {data}\n")
+    assert rc == 0, log
+    assert "200 atoms" in log
+
+
+@pytest.mark.parametrize("bad,msg", [
+    ("newton on\n", "newton pair off"),
+    ("pair_style gran model hooke tangential history\n", "model hertz"),
+    ("fix w all wall/gran model hertz tangential history mesh n_meshes 1 meshes cad\n", "primitive"),
+    ("neigh_modify delay 10\n", "delay"),
+    ("fix m1 all property/global coefficientRestitution peratomtypepair 2 0.3\n", "n*n"),
+    ("atom_style atomic\n", "atom_style"),
+    ("frobnicate 3\n", "Unknown command"),
+    ("fix x all insert/pack seed 1\n", "not available"),
+])
+def test_unsupported_or_illegal_commands_are_errors(tmp_path, bad, msg):
+    rc, log = run_script(tmp_path, HEADER + bad)
+    assert rc != 0
+    assert msg in log
+
+
+def test_run_without_gpu_fails_cleanly(tmp_path):
+    params, mats, P, walls = wl.config0(seed=0, n=100)
+    data = tmp_path / "d.data"
+    wl.write_data_file(str(data), params, P)
+    script = tmp_path / "in.run"
+    script.write_text(HEADER + f"read_data {data}\nrun 10\n")
+    code = ("import sys; sys.path.insert(0, %r); import pydem; "
+            "print(pydem.run_script(%r, %r))" % (os.path.join(ROOT, "liggghts-public_amd"),
+                                                  str(script), str(tmp_path / "log")))
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1")
+    res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
+                         timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert int(res.stdout.strip().splitlines()[-1]) != 0
+    assert "ERROR" in (tmp_path / "log").read_text()
