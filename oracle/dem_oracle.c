@@ -320,6 +320,37 @@ int64_t orc_pairs_bruteforce(int n, const double *x, const double *radius,
 }
 
 /* ------------------------------------------------------------------------
+ * triangle mesh data (fix mesh/surface): per-element geometry, topology,
+ * node velocities, and per-particle contact history slots
+ * ---------------------------------------------------------------------- */
+#define MESH_MAXNEIGH 5   /* SurfaceMesh<3,5> (tri_mesh.h) */
+#define MESH_MAXSLOT 16   /* history slots per particle */
+#define MESH_MAXCAND 128  /* candidate triangles per particle */
+#define SMALL_TRIMESH (1.e-10)  /* tri_mesh_I.h:47-50 */
+#define LARGE_TRIMESH 1000000
+#define EPSILON_PRECISION 1e-8  /* multi_node_mesh.h:59 */
+#define EPSILON_CURVATURE 0.00001 /* surface_mesh.h:61 */
+#define SMALL_DMBRDR 1.0e-8       /* domain.h:64 */
+
+typedef struct {
+  int ntri;
+  double *node, *center, *rbound, *edgevec, *edgelen, *snorm, *edgenorm, *vnode, *node_last;
+  int *obtuse;
+  unsigned char *eact, *cact;
+  int *nneigh, *neigh;
+  int wall_type;
+  orc_model model;
+  int moving;
+  double origin[3], axis[3], omega;
+  double curvature, precision;
+  /* per particle */
+  int *ncand, *cand;
+  int *npartner, *ptri;
+  unsigned char *keep;
+  double *phist;
+} orc_mesh;
+
+/* ------------------------------------------------------------------------
  * system
  * ---------------------------------------------------------------------- */
 struct orc_sys {
@@ -346,6 +377,10 @@ struct orc_sys {
   int64_t ntimestep;
   int64_t nrebuilds;
   double cutneighmax;
+  int nmesh;
+  orc_mesh mesh[ORC_MAXMESH];
+  int64_t mesh_next_reneighbor;
+  int64_t mesh_forced;
 };
 
 static void *xcalloc(size_t n, size_t sz) {
@@ -387,11 +422,19 @@ orc_sys *orc_create(const orc_params *p, const orc_materials *m, int n, const do
   for (int i = 0; i < n; i++)
     if (radius[i] > rmax) rmax = radius[i];
   s->cutneighmax = (rmax + rmax) * p->cdf + p->skin;
+  s->mesh_next_reneighbor = -1;
   return s;
 }
 
+static void mesh_free(orc_mesh *m);
+static void mesh_rebuild(orc_sys *s);
+static void mesh_move(orc_sys *s);
+static int mesh_decide_rebuild(const orc_sys *s);
+static void mesh_wall_compute(orc_sys *s, int im, int shearupdate);
+
 void orc_destroy(orc_sys *s) {
   if (!s) return;
+  for (int k = 0; k < s->nmesh; k++) mesh_free(&s->mesh[k]);
   free(s->x);
   free(s->v);
   free(s->w);
@@ -595,6 +638,7 @@ static void rebuild(orc_sys *s) {
   carry_out(s);
   pbc(s);
   build(s);
+  mesh_rebuild(s);
 }
 
 /* Neighbor::check_distance, neighbor.cpp:1385-1461 (no box change, radius fixed) */
@@ -685,17 +729,16 @@ static void gravity(orc_sys *s) {
   }
 }
 
+/* post_force fixes in script order: gravity before or after the wall fixes;
+ * plane walls before mesh walls */
 static void forces(orc_sys *s, int shearupdate) {
   memset(s->f, 0, 3 * sizeof(double) * (size_t)s->n);
   memset(s->t, 0, 3 * sizeof(double) * (size_t)s->n);
   pair_compute(s, shearupdate);
-  if (s->p.gravity_first) {
-    gravity(s);
-    wall_compute(s, shearupdate);
-  } else {
-    wall_compute(s, shearupdate);
-    gravity(s);
-  }
+  if (s->p.gravity_first) gravity(s);
+  wall_compute(s, shearupdate);
+  for (int im = 0; im < s->nmesh; im++) mesh_wall_compute(s, im, shearupdate);
+  if (!s->p.gravity_first) gravity(s);
 }
 
 void orc_compute_forces(orc_sys *s, int shearupdate) { forces(s, shearupdate); }
@@ -729,6 +772,9 @@ static void final_integrate(orc_sys *s) {
  * compute forces with shearupdate = 0 (pair_gran.cpp:665-667) */
 void orc_setup(orc_sys *s) {
   rebuild(s);
+  /* FixMoveMesh::setup zeroes the node velocities (fix_move_mesh.cpp:211-216) */
+  for (int im = 0; im < s->nmesh; im++)
+    memset(s->mesh[im].vnode, 0, 9 * sizeof(double) * (size_t)s->mesh[im].ntri);
   forces(s, 0);
 }
 
@@ -737,8 +783,17 @@ void orc_run(orc_sys *s, int nsteps) {
   for (int k = 0; k < nsteps; k++) {
     s->ntimestep++;
     initial_integrate(s);
-    /* Neighbor::decide with delay 0, every 1: check every step */
-    if (check_distance(s)) rebuild(s);
+    mesh_move(s);
+    /* Neighbor::decide with delay 0, every 1: forced by a fix's next_reneighbor
+     * (moving mesh), else check every step */
+    const int forced = s->mesh_next_reneighbor == s->ntimestep;
+    if (forced || check_distance(s)) {
+      if (forced) s->mesh_forced++;
+      rebuild(s);
+    } else if (mesh_decide_rebuild(s)) {
+      /* FixMesh::pre_force on a regular step (fix_mesh.cpp:572-582) */
+      s->mesh_next_reneighbor = s->ntimestep + 1;
+    }
     forces(s, 1);
     final_integrate(s);
   }
@@ -798,4 +853,859 @@ void orc_contact_counts(const orc_sys *s, int64_t *npair_touch, int64_t *nwall_t
     }
   *npair_touch = c;
   *nwall_touch = w;
+}
+
+/* =========================================================================
+ * triangle-mesh walls
+ *   element geometry      surface_mesh_I.h:361-447, multi_node_mesh_I.h:139-172
+ *   topology              surface_mesh_I.h:454-560 (buildNeighbours),
+ *                         :844-1000 (coplanarity, overlap), :1035-1190 (edges,
+ *                         corners), multi_node_mesh_I.h:246-355 (node sharing)
+ *   sphere contact        tri_mesh_I.h:65-251, math_extra_liggghts.h:559-570
+ *   contact history       fix_contact_history_mesh_I.h:51-200,
+ *                         fix_contact_history_mesh.cpp:272-507
+ *   force loop            fix_wall_gran.cpp:845-1018
+ *   rotation              mesh_mover_rotation.cpp:66-139, multi_node_mesh_I.h:620-660,
+ *                         math_extra_liggghts.h:431-470, math_extra.h:596-602
+ *   rebuild trigger       multi_node_mesh_I.h:792-825, fix_mesh.cpp:564-592
+ * ========================================================================= */
+
+static int comp_double(double a, double b, double prec) {
+  if (a == b) return 1;
+  if (b == 0) return a < prec && a > -prec;
+  const double x = (a - b);
+  return x < prec && x > -prec;
+}
+
+static inline void v3_sub(const double *a, const double *b, double *r) {
+  r[0] = a[0] - b[0];
+  r[1] = a[1] - b[1];
+  r[2] = a[2] - b[2];
+}
+static inline double v3_dot(const double *a, const double *b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+static inline double v3_mag(const double *a) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+static inline void v3_cross(const double *a, const double *b, double *r) {
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline void v3_div(double *v, double sc) {
+  const double sinv = 1. / sc;
+  v[0] = sinv * v[0];
+  v[1] = sinv * v[1];
+  v[2] = sinv * v[2];
+}
+
+static void quatquat(const double *a, const double *b, double *c) {
+  c[0] = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+  c[1] = a[0] * b[1] + b[0] * a[1] + a[2] * b[3] - a[3] * b[2];
+  c[2] = a[0] * b[2] + b[0] * a[2] + a[3] * b[1] - a[1] * b[3];
+  c[3] = a[0] * b[3] + b[0] * a[3] + a[1] * b[2] - a[2] * b[1];
+}
+
+/* v <- q v q* */
+static void vec_quat_rotate(double *v, const double *q) {
+  const double vq[4] = {0., v[0], v[1], v[2]};
+  const double qc[4] = {q[0], -q[1], -q[2], -q[3]};
+  double t[4], r[4];
+  quatquat(q, vq, t);
+  quatquat(t, qc, r);
+  v[0] = r[1];
+  v[1] = r[2];
+  v[2] = r[3];
+}
+
+/* edge vectors/lengths, surface normal, edge normals, obtuse index */
+static void mesh_surf_props(orc_mesh *m, int t) {
+  const double *n = &m->node[9 * t];
+  double *ev = &m->edgevec[9 * t], *el = &m->edgelen[3 * t], *sn = &m->snorm[3 * t];
+  double *en = &m->edgenorm[9 * t];
+  for (int i = 0; i < 3; i++) {
+    v3_sub(&n[3 * ((i + 1) % 3)], &n[3 * i], &ev[3 * i]);
+    el[i] = v3_mag(&ev[3 * i]);
+    v3_div(&ev[3 * i], el[i]);
+  }
+  v3_cross(&ev[0], &ev[3], sn);
+  if (v3_mag(sn) < 1e-15) {
+    v3_cross(&ev[3], &ev[6], sn);
+    if (v3_mag(sn) < 1e-15) {
+      v3_cross(&ev[6], &ev[0], sn);
+      if (v3_mag(sn) < 1e-15) {
+        const double tmpvec[] = {1.1233, 2.123231, -3.3343434};
+        v3_cross(&ev[0], tmpvec, sn);
+        if (v3_mag(sn) < 1e-15) {
+          v3_cross(&ev[3], tmpvec, sn);
+          if (v3_mag(sn) < 1e-15) {
+            const double tmpvec2[] = {1.1233, -2.123231, 3.3343434};
+            v3_cross(&ev[0], tmpvec2, sn);
+            if (v3_mag(sn) < 1e-15) v3_cross(&ev[3], tmpvec2, sn);
+          }
+        }
+      }
+    }
+  }
+  v3_div(sn, v3_mag(sn));
+  for (int i = 0; i < 3; i++) {
+    v3_cross(&ev[3 * i], sn, &en[3 * i]);
+    if (v3_mag(&en[3 * i]) < 1e-15) {
+      const int o = (i + 1) % 3;
+      en[3 * i] = ev[3 * o];
+      en[3 * i + 1] = ev[3 * o + 1];
+      en[3 * i + 2] = ev[3 * o + 2];
+    } else {
+      v3_div(&en[3 * i], v3_mag(&en[3 * i]));
+    }
+  }
+  /* calcObtuseAngleIndex is called for every node and each call overwrites
+   * the element's value, so the last node decides (surface_mesh_I.h:438-447) */
+  for (int i = 0; i < 3; i++) {
+    const double dot = v3_dot(&ev[3 * i], &ev[3 * ((i - 1 + 3) % 3)]);
+    m->obtuse[t] = dot > 0. ? i : -1;
+  }
+}
+
+static int nodes_equal(const orc_mesh *m, const double *a, const double *b) {
+  for (int k = 0; k < 3; k++)
+    if (!comp_double(a[k], b[k], m->precision)) return 0;
+  return 1;
+}
+
+static int n_shared_nodes(const orc_mesh *m, int i, int j) {
+  double d[3];
+  v3_sub(&m->center[3 * i], &m->center[3 * j], d);
+  const double radsum = m->rbound[i] + m->rbound[j];
+  if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > radsum * radsum) return 0;
+  int ns = 0;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++)
+      if (nodes_equal(m, &m->node[9 * i + 3 * a], &m->node[9 * j + 3 * b])) ns++;
+  return ns;
+}
+
+static int share_edge(const orc_mesh *m, int i, int j, int *iEdge, int *jEdge) {
+  double d[3];
+  v3_sub(&m->center[3 * i], &m->center[3 * j], d);
+  const double radsum = m->rbound[i] + m->rbound[j];
+  if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > radsum * radsum) return 0;
+  int n = 0, in1 = -1, jn1 = -1, in2 = -1, jn2 = -1;
+  for (int a = 0; a < 3 && n < 2; a++)
+    for (int b = 0; b < 3; b++)
+      if (nodes_equal(m, &m->node[9 * i + 3 * a], &m->node[9 * j + 3 * b])) {
+        if (n == 0) {
+          in1 = a;
+          jn1 = b;
+          n = 1;
+        } else {
+          in2 = a;
+          jn2 = b;
+          n = 2;
+          break;
+        }
+      }
+  if (n < 2) return 0;
+  *iEdge = (in1 + in2 == 2) ? 2 : (in1 < in2 ? in1 : in2);
+  *jEdge = (jn1 + jn2 == 2) ? 2 : (jn1 < jn2 ? jn1 : jn2);
+  return 1;
+}
+
+static int coplanar_neighs_overlap(const orc_mesh *m, int i, int iEdge, int j, int jEdge) {
+  const double *pRef = &m->node[9 * i + 3 * iEdge];
+  const double *edgeN = &m->edgenorm[9 * i + 3 * iEdge];
+  double vI[3], vJ[3];
+  v3_sub(&m->node[9 * i + 3 * ((iEdge + 2) % 3)], pRef, vI);
+  v3_sub(&m->node[9 * j + 3 * ((jEdge + 2) % 3)], pRef, vJ);
+  const double d1 = v3_dot(vI, edgeN), d2 = v3_dot(vJ, edgeN);
+  return d1 * d2 > 0.;
+}
+
+static void handle_shared_edge(orc_mesh *m, int i, int iEdge, int j, int jEdge, int coplanar) {
+  if (m->nneigh[i] < MESH_MAXNEIGH) m->neigh[MESH_MAXNEIGH * i + m->nneigh[i]] = j;
+  if (m->nneigh[j] < MESH_MAXNEIGH) m->neigh[MESH_MAXNEIGH * j + m->nneigh[j]] = i;
+  m->nneigh[i]++;
+  m->nneigh[j]++;
+  if (!coplanar || coplanar_neighs_overlap(m, i, iEdge, j, jEdge)) {
+    if (i < j) {
+      m->eact[3 * i + iEdge] = 0;
+      m->eact[3 * j + jEdge] = 1;
+    } else {
+      m->eact[3 * i + iEdge] = 1;
+      m->eact[3 * j + jEdge] = 0;
+    }
+  } else {
+    m->eact[3 * i + iEdge] = 0;
+    m->eact[3 * j + jEdge] = 0;
+  }
+}
+
+static int contains_node(const orc_mesh *m, int t, const double *p) {
+  for (int a = 0; a < 3; a++) {
+    const double *q = &m->node[9 * t + 3 * a];
+    if (comp_double(q[0], p[0], m->precision) && comp_double(q[1], p[1], m->precision) &&
+        comp_double(q[2], p[2], m->precision))
+      return a;
+  }
+  return -1;
+}
+
+typedef struct {
+  int *visited, nvisited;
+  int *has, nhas;
+  double *edges, *ends; /* 2 per element with the node */
+  int anyActive;
+} corner_scan;
+
+static void check_node_recursive(const orc_mesh *m, int t, const double *p, corner_scan *c) {
+  for (int k = 0; k < c->nvisited; k++)
+    if (c->visited[k] == t) return;
+  c->visited[c->nvisited++] = t;
+  const int a = contains_node(m, t, p);
+  if (a < 0) return;
+  c->has[c->nhas] = t;
+  const int am = (a - 1 + 3) % 3;
+  memcpy(&c->edges[3 * (2 * c->nhas)], &m->edgevec[9 * t + 3 * a], 3 * sizeof(double));
+  memcpy(&c->edges[3 * (2 * c->nhas + 1)], &m->edgevec[9 * t + 3 * am], 3 * sizeof(double));
+  memcpy(&c->ends[3 * (2 * c->nhas)], &m->node[9 * t + 3 * ((a + 1) % 3)], 3 * sizeof(double));
+  memcpy(&c->ends[3 * (2 * c->nhas + 1)], &m->node[9 * t + 3 * am], 3 * sizeof(double));
+  c->nhas++;
+  if (m->eact[3 * t + a])
+    c->anyActive = 1;
+  else if (m->eact[3 * t + am])
+    c->anyActive = 1;
+  for (int k = 0; k < m->nneigh[t] && k < MESH_MAXNEIGH; k++) {
+    const int nb = m->neigh[MESH_MAXNEIGH * t + k];
+    if (nb < 0) return;
+    check_node_recursive(m, nb, p, c);
+  }
+}
+
+static void handle_corner(orc_mesh *m, const orc_params *prm, int t, int a, corner_scan *c) {
+  const double *p = &m->node[9 * t + 3 * a];
+  c->nvisited = c->nhas = 0;
+  c->anyActive = 0;
+  check_node_recursive(m, t, p, c);
+  /* Domain::is_in_subdomain, domain_I.h:65-86 (serial: subdomain = box, padded) */
+  for (int d = 0; d < 3; d++) {
+    const double lo = prm->boxlo[d] - SMALL_DMBRDR, hi = prm->boxhi[d] + SMALL_DMBRDR;
+    if (!(p[d] >= lo && p[d] < hi)) return;
+  }
+  int maxId = -1;
+  for (int k = 0; k < c->nhas; k++)
+    if (c->has[k] > maxId) maxId = c->has[k];
+  int colinear = 0;
+  const int ne = 2 * c->nhas;
+  for (int i = 0; i < ne; i++)
+    for (int j = i + 1; j < ne; j++)
+      if (fabs(v3_dot(&c->edges[3 * i], &c->edges[3 * j])) > m->curvature &&
+          !nodes_equal(m, &c->ends[3 * i], &c->ends[3 * j]))
+        colinear = 1;
+  if (colinear || !c->anyActive)
+    m->cact[3 * t + a] = 0;
+  else
+    m->cact[3 * t + a] = (t == maxId) ? 1 : 0;
+}
+
+static void mesh_topology(orc_mesh *m, const orc_params *prm) {
+  const int T = m->ntri;
+  for (int t = 0; t < T; t++) {
+    m->nneigh[t] = 0;
+    for (int k = 0; k < MESH_MAXNEIGH; k++) m->neigh[MESH_MAXNEIGH * t + k] = -1;
+    for (int k = 0; k < 3; k++) m->eact[3 * t + k] = m->cact[3 * t + k] = 1;
+  }
+  /* every pair sharing an edge, each handled once with the later element as i */
+  for (int i = 0; i < T; i++)
+    for (int j = 0; j < i; j++) {
+      int ie, je;
+      if (share_edge(m, i, j, &ie, &je)) {
+        const int coplanar = fabs(v3_dot(&m->snorm[3 * i], &m->snorm[3 * j])) >= m->curvature;
+        handle_shared_edge(m, i, ie, j, je, coplanar);
+      }
+    }
+  corner_scan c;
+  c.visited = (int *)xcalloc((size_t)T, sizeof(int));
+  c.has = (int *)xcalloc((size_t)T, sizeof(int));
+  c.edges = (double *)xcalloc(6 * (size_t)T, sizeof(double));
+  c.ends = (double *)xcalloc(6 * (size_t)T, sizeof(double));
+  for (int t = 0; t < T; t++)
+    for (int a = 0; a < 3; a++) handle_corner(m, prm, t, a, &c);
+  free(c.visited);
+  free(c.has);
+  free(c.edges);
+  free(c.ends);
+}
+
+/* SurfaceMesh::areCoplanarNodeNeighs, surface_mesh_I.h:892-929 */
+static int coplanar_node_neighs(const orc_mesh *m, int a, int b) {
+  int areNeighs = 0;
+  for (int k = 0; k < m->nneigh[a] && k < MESH_MAXNEIGH; k++)
+    if (m->neigh[MESH_MAXNEIGH * a + k] == b) areNeighs = 1;
+  int found = 0;
+  if (!areNeighs && n_shared_nodes(m, a, b) != 0) found = 1;
+  if (!areNeighs && !found) return 0;
+  return fabs(v3_dot(&m->snorm[3 * a], &m->snorm[3 * b])) > m->curvature;
+}
+
+static void mesh_free(orc_mesh *m) {
+  free(m->node);
+  free(m->center);
+  free(m->rbound);
+  free(m->edgevec);
+  free(m->edgelen);
+  free(m->snorm);
+  free(m->edgenorm);
+  free(m->vnode);
+  free(m->node_last);
+  free(m->obtuse);
+  free(m->eact);
+  free(m->cact);
+  free(m->nneigh);
+  free(m->neigh);
+  free(m->ncand);
+  free(m->cand);
+  free(m->npartner);
+  free(m->ptri);
+  free(m->keep);
+  free(m->phist);
+}
+
+int orc_add_mesh(orc_sys *s, int ntri, const double *nodes, int wall_type,
+                 const orc_model *model) {
+  if (s->nmesh >= ORC_MAXMESH || ntri <= 0) return -1;
+  orc_mesh *m = &s->mesh[s->nmesh];
+  memset(m, 0, sizeof(*m));
+  m->curvature = 1. - EPSILON_CURVATURE;
+  m->precision = EPSILON_PRECISION;
+  m->wall_type = wall_type;
+  m->model = *model;
+  const size_t T = (size_t)ntri;
+  m->node = (double *)xcalloc(9 * T, sizeof(double));
+  m->center = (double *)xcalloc(3 * T, sizeof(double));
+  m->rbound = (double *)xcalloc(T, sizeof(double));
+  m->edgevec = (double *)xcalloc(9 * T, sizeof(double));
+  m->edgelen = (double *)xcalloc(3 * T, sizeof(double));
+  m->snorm = (double *)xcalloc(3 * T, sizeof(double));
+  m->edgenorm = (double *)xcalloc(9 * T, sizeof(double));
+  m->vnode = (double *)xcalloc(9 * T, sizeof(double));
+  m->node_last = (double *)xcalloc(9 * T, sizeof(double));
+  m->obtuse = (int *)xcalloc(T, sizeof(int));
+  m->eact = (unsigned char *)xcalloc(3 * T, 1);
+  m->cact = (unsigned char *)xcalloc(3 * T, 1);
+  m->nneigh = (int *)xcalloc(T, sizeof(int));
+  m->neigh = (int *)xcalloc(MESH_MAXNEIGH * T, sizeof(int));
+  /* MultiNodeMesh::addElement drops elements with coinciding nodes */
+  int nt = 0;
+  for (int t = 0; t < ntri; t++) {
+    const double *q = &nodes[9 * t];
+    if (nodes_equal(m, &q[0], &q[3]) || nodes_equal(m, &q[3], &q[6]) || nodes_equal(m, &q[0], &q[6]))
+      continue;
+    memcpy(&m->node[9 * nt], q, 9 * sizeof(double));
+    double avg[3] = {0., 0., 0.};
+    for (int i = 0; i < 3; i++)
+      for (int k = 0; k < 3; k++) avg[k] = q[3 * i + k] + avg[k];
+    v3_div(avg, 3.0);
+    memcpy(&m->center[3 * nt], avg, 3 * sizeof(double));
+    double rb = 0.;
+    for (int i = 0; i < 3; i++) {
+      double v[3];
+      v3_sub(avg, &q[3 * i], v);
+      const double mg = v3_mag(v);
+      rb = rb > mg ? rb : mg;
+    }
+    m->rbound[nt] = rb;
+    mesh_surf_props(m, nt);
+    nt++;
+  }
+  m->ntri = nt;
+  mesh_topology(m, &s->p);
+  const size_t n = (size_t)(s->n > 0 ? s->n : 1);
+  m->ncand = (int *)xcalloc(n, sizeof(int));
+  m->cand = (int *)xcalloc(n * MESH_MAXCAND, sizeof(int));
+  m->npartner = (int *)xcalloc(n, sizeof(int));
+  m->ptri = (int *)xcalloc(n * MESH_MAXSLOT, sizeof(int));
+  for (size_t k = 0; k < n * MESH_MAXSLOT; k++) m->ptri[k] = -1;
+  m->keep = (unsigned char *)xcalloc(n * MESH_MAXSLOT, 1);
+  m->phist = (double *)xcalloc(n * MESH_MAXSLOT * 3, sizeof(double));
+  return s->nmesh++;
+}
+
+void orc_mesh_rotate(orc_sys *s, int imesh, const double *origin, const double *axis,
+                     double period) {
+  orc_mesh *m = &s->mesh[imesh];
+  m->moving = 1;
+  memcpy(m->origin, origin, 3 * sizeof(double));
+  /* MeshMoverRotate: vectorNormalize3D(axis_), omega_ = 2 pi / period */
+  const double norm = sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+  const double invnorm = (norm == 0.) ? 0. : 1. / norm;
+  for (int k = 0; k < 3; k++) m->axis[k] = axis[k] * invnorm;
+  m->omega = 2. * MY_PI / period; /* M_PI in the reference: same constant */
+}
+
+void orc_mesh_topology(const orc_sys *s, int imesh, int *edge_active, int *corner_active,
+                       int *obtuse) {
+  const orc_mesh *m = &s->mesh[imesh];
+  for (int t = 0; t < m->ntri; t++) {
+    for (int k = 0; k < 3; k++) {
+      edge_active[3 * t + k] = m->eact[3 * t + k];
+      corner_active[3 * t + k] = m->cact[3 * t + k];
+    }
+    obtuse[t] = m->obtuse[t];
+  }
+}
+
+void orc_mesh_nodes(const orc_sys *s, int imesh, double *nodes, double *vnodes) {
+  const orc_mesh *m = &s->mesh[imesh];
+  if (nodes) memcpy(nodes, m->node, 9 * sizeof(double) * (size_t)m->ntri);
+  if (vnodes) memcpy(vnodes, m->vnode, 9 * sizeof(double) * (size_t)m->ntri);
+}
+
+int orc_mesh_history(const orc_sys *s, int imesh, int maxslot, int *tri, double *hist) {
+  const orc_mesh *m = &s->mesh[imesh];
+  for (int i = 0; i < s->n; i++)
+    for (int k = 0; k < maxslot; k++) {
+      const int inside = k < MESH_MAXSLOT;
+      const int tt = inside ? m->ptri[(size_t)i * MESH_MAXSLOT + k] : -1;
+      tri[(size_t)i * maxslot + k] = tt;
+      for (int d = 0; d < 3; d++)
+        hist[((size_t)i * maxslot + k) * 3 + d] =
+            (inside && tt >= 0) ? m->phist[((size_t)i * MESH_MAXSLOT + k) * 3 + d] : 0.0;
+    }
+  return MESH_MAXSLOT;
+}
+
+int64_t orc_mesh_forced_rebuilds(const orc_sys *s) { return s->mesh_forced; }
+
+/* ---- sphere-triangle contact: TriMesh::resolveTriSphereContactBary ---- */
+static double calc_dist(const double *p, const double *cp, double *delta) {
+  v3_sub(cp, p, delta);
+  return sqrt((p[0] - cp[0]) * (p[0] - cp[0]) + (p[1] - cp[1]) * (p[1] - cp[1]) +
+              (p[2] - cp[2]) * (p[2] - cp[2]));
+}
+
+static double resolve_edge(const orc_mesh *m, int t, int iEdge, const double *p, double *delta,
+                           double *bary) {
+  const int ip = (iEdge + 1) % 3, ipp = (iEdge + 2) % 3;
+  const double *n = &m->node[9 * t];
+  const double *ev = &m->edgevec[9 * t + 3 * iEdge];
+  const double len = m->edgelen[3 * t + iEdge];
+  double nodeToP[3];
+  v3_sub(p, &n[3 * iEdge], nodeToP);
+  const double distFromNode = v3_dot(nodeToP, ev);
+  if (distFromNode < -SMALL_TRIMESH) {
+    if (!m->cact[3 * t + iEdge]) return LARGE_TRIMESH;
+    const double d = calc_dist(p, &n[3 * iEdge], delta);
+    bary[iEdge] = 1.;
+    bary[ip] = 0.;
+    bary[ipp] = 0.;
+    return d;
+  } else if (distFromNode > len + SMALL_TRIMESH) {
+    if (!m->cact[3 * t + ip]) return LARGE_TRIMESH;
+    const double d = calc_dist(p, &n[3 * ip], delta);
+    bary[iEdge] = 0.;
+    bary[ip] = 1.;
+    bary[ipp] = 0.;
+    return d;
+  } else {
+    if (!m->eact[3 * t + iEdge]) return LARGE_TRIMESH;
+    double cp[3];
+    for (int k = 0; k < 3; k++) cp[k] = n[3 * iEdge + k] + distFromNode * ev[k];
+    const double d = calc_dist(p, cp, delta);
+    bary[ipp] = 0.;
+    bary[iEdge] = 1. - distFromNode / len;
+    bary[ip] = 1. - bary[iEdge];
+    return d;
+  }
+}
+
+static double resolve_corner(const orc_mesh *m, int t, int iNode, int obtuse, const double *p,
+                             double *delta, double *bary) {
+  const int ip = (iNode + 1) % 3, ipp = (iNode + 2) % 3;
+  const double *n = &m->node[9 * t + 3 * iNode];
+  if (obtuse) {
+    const double *edge = &m->edgevec[9 * t];
+    const double *len = &m->edgelen[3 * t];
+    double nodeToP[3], cp[3];
+    v3_sub(p, n, nodeToP);
+    double distFromNode = v3_dot(nodeToP, &edge[3 * ipp]);
+    if (distFromNode < SMALL_TRIMESH) {
+      if (distFromNode > -len[ipp]) {
+        if (!m->eact[3 * t + ipp]) return LARGE_TRIMESH;
+        for (int k = 0; k < 3; k++) cp[k] = n[k] + distFromNode * edge[3 * ipp + k];
+        bary[ip] = 0.;
+        bary[iNode] = 1. + distFromNode / len[ipp];
+        bary[ipp] = 1. - bary[iNode];
+        return calc_dist(p, cp, delta);
+      } else {
+        if (!m->cact[3 * t + ipp]) return LARGE_TRIMESH;
+        bary[ipp] = 1.;
+        bary[iNode] = bary[ip] = 0.;
+        return calc_dist(p, &m->node[9 * t + 3 * ipp], delta);
+      }
+    }
+    distFromNode = v3_dot(nodeToP, &edge[3 * iNode]);
+    if (distFromNode > -SMALL_TRIMESH) {
+      if (distFromNode < len[iNode]) {
+        if (!m->eact[3 * t + iNode]) return LARGE_TRIMESH;
+        for (int k = 0; k < 3; k++) cp[k] = n[k] + distFromNode * edge[3 * iNode + k];
+        bary[ipp] = 0.;
+        bary[iNode] = 1. - distFromNode / len[iNode];
+        bary[ip] = 1. - bary[iNode];
+        return calc_dist(p, cp, delta);
+      } else {
+        if (!m->cact[3 * t + ip]) return LARGE_TRIMESH;
+        bary[ip] = 1.;
+        bary[iNode] = bary[ipp] = 0.;
+        return calc_dist(p, &m->node[9 * t + 3 * ip], delta);
+      }
+    }
+  }
+  if (!m->cact[3 * t + iNode]) return LARGE_TRIMESH;
+  bary[iNode] = 1.;
+  bary[ip] = bary[ipp] = 0.;
+  return calc_dist(p, n, delta);
+}
+
+static double resolve_tri_sphere(const orc_mesh *m, int t, double r, const double *p,
+                                 double *delta, double *bary, int *barysign) {
+  const double *n = &m->node[9 * t];
+  const int obt = m->obtuse[t];
+  bary[0] = bary[1] = bary[2] = 0.;
+  double n0p[3];
+  v3_sub(p, n, n0p);
+  /* MathExtraLiggghts::calcBaryTriCoords */
+  const double *ev = &m->edgevec[9 * t];
+  const double *el = &m->edgelen[3 * t];
+  const double a = v3_dot(n0p, &ev[0]);
+  const double b = v3_dot(n0p, &ev[6]);
+  const double c = v3_dot(&ev[0], &ev[6]);
+  const double oneMinCSqr = 1 - c * c;
+  bary[1] = (a - b * c) / (el[0] * oneMinCSqr);
+  bary[2] = (a * c - b) / (el[2] * oneMinCSqr);
+  bary[0] = 1. - bary[1] - bary[2];
+  const double invlen = 1. / (2. * m->rbound[t]);
+  const int bs = (bary[0] > -m->precision * invlen) + 2 * (bary[1] > -m->precision * invlen) +
+                 4 * (bary[2] > -m->precision * invlen);
+  *barysign = bs;
+  double d = 0.;
+  switch (bs) {
+    case 1: d = resolve_corner(m, t, 0, obt == 0, p, delta, bary); break;
+    case 2: d = resolve_corner(m, t, 1, obt == 1, p, delta, bary); break;
+    case 3: d = resolve_edge(m, t, 0, p, delta, bary); break;
+    case 4: d = resolve_corner(m, t, 2, obt == 2, p, delta, bary); break;
+    case 5: d = resolve_edge(m, t, 2, p, delta, bary); break;
+    case 6: d = resolve_edge(m, t, 1, p, delta, bary); break;
+    case 7: {
+      const double *sn = &m->snorm[3 * t];
+      const double dNorm = v3_dot(sn, n0p);
+      double cs[3];
+      for (int k = 0; k < 3; k++) cs[k] = p[k] - sn[k] * dNorm;
+      d = calc_dist(p, cs, delta);
+      break;
+    }
+    default: d = 1.; break; /* all three coordinates negative: impossible */
+  }
+  return d - r;
+}
+
+double orc_tri_contact(const double *nodes9, const int *edge_active3, const int *corner_active3,
+                       const double *p, double radius, double *delta, double *bary,
+                       int *barysign) {
+  orc_mesh m;
+  memset(&m, 0, sizeof(m));
+  double node[9], center[3], rb, ev[9], el[3], sn[3], en[9];
+  int obt;
+  unsigned char ea[3], ca[3];
+  memcpy(node, nodes9, sizeof node);
+  m.node = node;
+  m.center = center;
+  m.rbound = &rb;
+  m.edgevec = ev;
+  m.edgelen = el;
+  m.snorm = sn;
+  m.edgenorm = en;
+  m.obtuse = &obt;
+  m.eact = ea;
+  m.cact = ca;
+  m.precision = EPSILON_PRECISION;
+  double avg[3] = {0., 0., 0.};
+  for (int i = 0; i < 3; i++)
+    for (int k = 0; k < 3; k++) avg[k] = node[3 * i + k] + avg[k];
+  v3_div(avg, 3.0);
+  memcpy(center, avg, sizeof avg);
+  rb = 0.;
+  for (int i = 0; i < 3; i++) {
+    double v[3];
+    v3_sub(avg, &node[3 * i], v);
+    const double mg = v3_mag(v);
+    rb = rb > mg ? rb : mg;
+  }
+  mesh_surf_props(&m, 0);
+  for (int k = 0; k < 3; k++) {
+    ea[k] = (unsigned char)(edge_active3[k] != 0);
+    ca[k] = (unsigned char)(corner_active3[k] != 0);
+  }
+  return resolve_tri_sphere(&m, 0, radius, p, delta, bary, barysign);
+}
+
+/* closest point on a triangle to p (Ericson, Real-Time Collision Detection
+ * 5.1.5); used only to select candidate triangles within r + skin */
+static double tri_point_dist2(const double *a, const double *b, const double *c, const double *p) {
+  double ab[3], ac[3], ap[3], q[3];
+  v3_sub(b, a, ab);
+  v3_sub(c, a, ac);
+  v3_sub(p, a, ap);
+  const double d1 = v3_dot(ab, ap), d2 = v3_dot(ac, ap);
+  if (d1 <= 0. && d2 <= 0.) {
+    memcpy(q, a, sizeof q);
+  } else {
+    double bp[3];
+    v3_sub(p, b, bp);
+    const double d3 = v3_dot(ab, bp), d4 = v3_dot(ac, bp);
+    if (d3 >= 0. && d4 <= d3) {
+      memcpy(q, b, sizeof q);
+    } else {
+      const double vc = d1 * d4 - d3 * d2;
+      if (vc <= 0. && d1 >= 0. && d3 <= 0.) {
+        const double v = d1 / (d1 - d3);
+        for (int k = 0; k < 3; k++) q[k] = a[k] + v * ab[k];
+      } else {
+        double cp[3];
+        v3_sub(p, c, cp);
+        const double d5 = v3_dot(ab, cp), d6 = v3_dot(ac, cp);
+        if (d6 >= 0. && d5 <= d6) {
+          memcpy(q, c, sizeof q);
+        } else {
+          const double vb = d5 * d2 - d1 * d6;
+          if (vb <= 0. && d2 >= 0. && d6 <= 0.) {
+            const double w = d2 / (d2 - d6);
+            for (int k = 0; k < 3; k++) q[k] = a[k] + w * ac[k];
+          } else {
+            const double va = d3 * d6 - d5 * d4;
+            if (va <= 0. && (d4 - d3) >= 0. && (d5 - d6) >= 0.) {
+              const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+              for (int k = 0; k < 3; k++) q[k] = b[k] + w * (c[k] - b[k]);
+            } else {
+              const double denom = 1. / (va + vb + vc);
+              const double v = vb * denom, w = vc * denom;
+              for (int k = 0; k < 3; k++) q[k] = a[k] + ab[k] * v + ac[k] * w;
+            }
+          }
+        }
+      }
+    }
+  }
+  double d[3];
+  v3_sub(p, q, d);
+  return v3_dot(d, d);
+}
+
+/* per-particle slot arrays */
+#define SLOT(m, i, k) ((m)->ptri[(size_t)(i) * MESH_MAXSLOT + (k)])
+#define SKEEP(m, i, k) ((m)->keep[(size_t)(i) * MESH_MAXSLOT + (k)])
+#define SHIST(m, i, k) (&(m)->phist[((size_t)(i) * MESH_MAXSLOT + (k)) * 3])
+
+static void slot_swap(orc_mesh *m, int i, int a, int b, int keepflag) {
+  int t = SLOT(m, i, a);
+  SLOT(m, i, a) = SLOT(m, i, b);
+  SLOT(m, i, b) = t;
+  double h[3];
+  memcpy(h, SHIST(m, i, a), sizeof h);
+  memcpy(SHIST(m, i, a), SHIST(m, i, b), sizeof h);
+  memcpy(SHIST(m, i, b), h, sizeof h);
+  if (keepflag) {
+    const unsigned char kf = SKEEP(m, i, a);
+    SKEEP(m, i, a) = SKEEP(m, i, b);
+    SKEEP(m, i, b) = kf;
+  }
+}
+
+/* mesh part of a neighbor rebuild: FixContactHistoryMesh::sort_contacts
+ * (pre_exchange), element geometry refresh and node positions at rebuild
+ * (FixMesh::pre_force -> pbcExchangeBorders -> refreshOwned), candidate
+ * triangles within r + skin, FixContactHistoryMesh::cleanUpContactJumps */
+static void mesh_rebuild(orc_sys *s) {
+  for (int im = 0; im < s->nmesh; im++) {
+    orc_mesh *m = &s->mesh[im];
+    for (int i = 0; i < s->n; i++) {
+      const int nslot = MESH_MAXSLOT;
+      int first_empty, last_filled;
+      do {
+        first_empty = last_filled = -1;
+        for (int j = 0; j < nslot; j++) {
+          if (-1 == first_empty && -1 == SLOT(m, i, j)) first_empty = j;
+          if (SLOT(m, i, j) >= 0) last_filled = j;
+        }
+        if (first_empty > -1 && last_filled > -1 && first_empty < last_filled)
+          slot_swap(m, i, first_empty, last_filled, 1);
+      } while (first_empty > -1 && last_filled > -1 && first_empty < last_filled);
+    }
+    for (int t = 0; t < m->ntri; t++) mesh_surf_props(m, t);
+    memcpy(m->node_last, m->node, 9 * sizeof(double) * (size_t)m->ntri);
+    for (int i = 0; i < s->n; i++) {
+      const double lim = s->r[i] * s->p.cdf + s->p.skin;
+      int nc = 0;
+      for (int t = 0; t < m->ntri; t++) {
+        const double *q = &m->node[9 * t];
+        if (tri_point_dist2(&q[0], &q[3], &q[6], &s->x[3 * i]) <= lim * lim) {
+          if (nc < MESH_MAXCAND) m->cand[(size_t)i * MESH_MAXCAND + nc] = t;
+          nc++;
+        }
+      }
+      if (nc > MESH_MAXCAND) abort();
+      m->ncand[i] = nc;
+      /* cleanUpContactJumps */
+      int ip = 0;
+      while (ip < m->npartner[i]) {
+        const int tt = SLOT(m, i, ip);
+        int inlist = 0;
+        for (int k = 0; k < nc; k++)
+          if (m->cand[(size_t)i * MESH_MAXCAND + k] == tt) inlist = 1;
+        if (!inlist) {
+          SLOT(m, i, ip) = -1;
+          memset(SHIST(m, i, ip), 0, 3 * sizeof(double));
+          slot_swap(m, i, ip, m->npartner[i] - 1, 0);
+          m->npartner[i]--;
+        } else {
+          ip++;
+        }
+      }
+    }
+  }
+}
+
+/* FixMoveMesh::initial_integrate -> MeshMoverRotate::initial_integrate */
+static void mesh_move(orc_sys *s) {
+  for (int im = 0; im < s->nmesh; im++) {
+    orc_mesh *m = &s->mesh[im];
+    if (!m->moving) continue;
+    const double dAngle = m->omega * s->p.dt;
+    double axisNorm[3] = {m->axis[0], m->axis[1], m->axis[2]};
+    v3_div(axisNorm, v3_mag(axisNorm));
+    double dQ[4];
+    dQ[0] = cos(dAngle * 0.5);
+    for (int k = 0; k < 3; k++) dQ[k + 1] = axisNorm[k] * sin(dAngle * 0.5);
+    const double *o = m->origin;
+    const int trans = (o[0] * o[0] + o[1] * o[1] + o[2] * o[2]) > 0.;
+    for (int t = 0; t < m->ntri; t++) {
+      double *c = &m->center[3 * t];
+      c[0] = c[1] = c[2] = 0.;
+      for (int j = 0; j < 3; j++) {
+        double *nd = &m->node[9 * t + 3 * j];
+        if (trans) v3_sub(nd, o, nd);
+        vec_quat_rotate(nd, dQ);
+        if (trans)
+          for (int k = 0; k < 3; k++) nd[k] = nd[k] + o[k];
+        for (int k = 0; k < 3; k++) c[k] = nd[k] + c[k];
+      }
+      v3_div(c, 3.0);
+      /* edgeVec, edgeNorm, surfaceNorm are rotated with the mesh */
+      for (int j = 0; j < 3; j++) {
+        vec_quat_rotate(&m->edgevec[9 * t + 3 * j], dQ);
+        vec_quat_rotate(&m->edgenorm[9 * t + 3 * j], dQ);
+      }
+      vec_quat_rotate(&m->snorm[3 * t], dQ);
+    }
+    /* node velocity: reset, then v += (omega axis) x (node - origin) */
+    double omegaVec[3];
+    for (int k = 0; k < 3; k++) omegaVec[k] = m->axis[k] * m->omega;
+    for (int t = 0; t < m->ntri; t++)
+      for (int j = 0; j < 3; j++) {
+        double rPA[3], vRot[3];
+        v3_sub(&m->node[9 * t + 3 * j], o, rPA);
+        v3_cross(omegaVec, rPA, vRot);
+        double *v = &m->vnode[9 * t + 3 * j];
+        for (int k = 0; k < 3; k++) v[k] = 0.0 + vRot[k];
+      }
+  }
+}
+
+/* MultiNodeMesh::decideRebuild */
+static int mesh_decide_rebuild(const orc_sys *s) {
+  const double triggersq = 0.25 * s->p.skin * s->p.skin;
+  for (int im = 0; im < s->nmesh; im++) {
+    const orc_mesh *m = &s->mesh[im];
+    if (!m->moving) continue;
+    for (int k = 0; k < 3 * m->ntri; k++) {
+      const double *a = &m->node[3 * k], *b = &m->node_last[3 * k];
+      const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+      if (dx * dx + dy * dy + dz * dz > triggersq) return 1;
+    }
+  }
+  return 0;
+}
+
+/* FixContactHistoryMesh::handleContact */
+static int mesh_handle_contact(orc_mesh *m, int i, int tri, double **hist, int faceflag) {
+  for (int k = 0; k < MESH_MAXSLOT; k++)
+    if (SLOT(m, i, k) == tri) {
+      *hist = SHIST(m, i, k);
+      SKEEP(m, i, k) = 1;
+      return 1;
+    }
+  if (faceflag) {
+    for (int k = 0; k < MESH_MAXSLOT; k++) {
+      const int pt = SLOT(m, i, k);
+      if (pt >= 0 && pt != tri && coplanar_node_neighs(m, pt, tri) && SKEEP(m, i, k)) return 0;
+    }
+  }
+  int ic = -1;
+  for (int k = 0; k < MESH_MAXSLOT; k++)
+    if (SLOT(m, i, k) == -1) {
+      ic = k;
+      break;
+    }
+  if (ic < 0) abort(); /* more simultaneous mesh contacts than slots */
+  SLOT(m, i, ic) = tri;
+  SKEEP(m, i, ic) = 1;
+  m->npartner[i]++;
+  *hist = SHIST(m, i, ic);
+  (*hist)[0] = (*hist)[1] = (*hist)[2] = 0.0;
+  if (faceflag)
+    for (int k = 0; k < MESH_MAXSLOT; k++) {
+      const int pt = SLOT(m, i, k);
+      if (pt >= 0 && pt != tri && coplanar_node_neighs(m, pt, tri))
+        memcpy(*hist, SHIST(m, i, k), 3 * sizeof(double));
+    }
+  return 1;
+}
+
+/* FixWallGran::post_force_mesh for one mesh */
+static void mesh_wall_compute(orc_sys *s, int im, int shearupdate) {
+  orc_mesh *m = &s->mesh[im];
+  for (int i = 0; i < s->n; i++) {
+    for (int k = 0; k < MESH_MAXSLOT; k++) SKEEP(m, i, k) = 0; /* markAllContacts */
+    for (int c = 0; c < m->ncand[i]; c++) {
+      const int t = m->cand[(size_t)i * MESH_MAXCAND + c];
+      double delta[3], bary[3];
+      int barysign;
+      const double deltan = resolve_tri_sphere(m, t, s->r[i], &s->x[3 * i], delta, bary, &barysign);
+      if (deltan > s->cutneighmax) continue;
+      if (deltan <= 0 || deltan < (s->p.cdf - 1.0) * s->r[i]) {
+        double *h = NULL;
+        if (!mesh_handle_contact(m, i, t, &h, 7 == barysign)) continue;
+        double vwall[3] = {0., 0., 0.};
+        if (m->moving)
+          for (int k = 0; k < 3; k++)
+            vwall[k] = (bary[0] * m->vnode[9 * t + k] + bary[1] * m->vnode[9 * t + 3 + k] +
+                        bary[2] * m->vnode[9 * t + 6 + k]);
+        if (!(deltan <= 0)) continue; /* surfacesClose: no force */
+        const double sdelta[3] = {-delta[0], -delta[1], -delta[2]};
+        double dF[3], dT[3];
+        orc_contact_wall(&m->model, &s->m, s->type[i], m->wall_type, -deltan, sdelta, s->r[i],
+                         &s->v[3 * i], &s->w[3 * i], vwall, s->rm[i], s->p.dt, shearupdate, h, dF,
+                         dT);
+        for (int d = 0; d < 3; d++) {
+          s->f[3 * i + d] += dF[d];
+          s->t[3 * i + d] += dT[d];
+        }
+      }
+    }
+    /* cleanUpContacts */
+    for (int k = 0; k < MESH_MAXSLOT; k++)
+      if (!SKEEP(m, i, k)) {
+        if (SLOT(m, i, k) > -1) m->npartner[i]--;
+        SLOT(m, i, k) = -1;
+        memset(SHIST(m, i, k), 0, 3 * sizeof(double));
+      }
+  }
 }

@@ -22,6 +22,7 @@ extern "C" {
 
 #define ORC_MAXTYPE 8
 #define ORC_MAXPLANE 6
+#define ORC_MAXMESH 4
 
 /* per-type-pair mixed material matrices, indexed [itype][jtype], types 1..ntypes */
 typedef struct {
@@ -110,6 +111,34 @@ int64_t orc_nrebuilds(const orc_sys *s);
 int64_t orc_ntimestep(const orc_sys *s);
 /* count of touching half-list pairs and of touching particle-wall contacts */
 void orc_contact_counts(const orc_sys *s, int64_t *npair_touch, int64_t *nwall_touch);
+
+/* --- triangle-mesh walls (fix mesh/surface + fix wall/gran ... mesh) --- */
+/* Adds a mesh of ntri triangles (nodes: ntri*9, node-major). Computes the
+ * element geometry (surface_mesh_I.h:361-447) and the topology: edge and
+ * corner activity, neighbor faces (surface_mesh_I.h:454-560, 1035-1155).
+ * Returns the mesh index or -1. Call after orc_create, before orc_setup. */
+int orc_add_mesh(orc_sys *s, int ntri, const double *nodes, int wall_type,
+                 const orc_model *model);
+/* fix move/mesh mesh <id> rotate origin ... axis ... period T
+ * (mesh_mover_rotation.cpp:66-139) */
+void orc_mesh_rotate(orc_sys *s, int imesh, const double *origin, const double *axis,
+                     double period);
+/* topology outputs: edge_active/corner_active ntri*3 (0/1), obtuse ntri (-1 = none) */
+void orc_mesh_topology(const orc_sys *s, int imesh, int *edge_active, int *corner_active,
+                       int *obtuse);
+/* current nodes (ntri*9) and node velocities (ntri*9) */
+void orc_mesh_nodes(const orc_sys *s, int imesh, double *nodes, double *vnodes);
+/* mesh contact history: per particle up to maxslot (triangle index, shear[3]);
+ * triangle -1 marks an empty slot. Returns the slot capacity used. */
+int orc_mesh_history(const orc_sys *s, int imesh, int maxslot, int *tri, double *hist);
+/* TriMesh::resolveTriSphereContactBary on one triangle given by its 3 nodes,
+ * with the given activity flags; returns distance - radius and fills delta
+ * (closest point - centre), bary and barysign. Geometry as orc_add_mesh. */
+double orc_tri_contact(const double *nodes9, const int *edge_active3, const int *corner_active3,
+                       const double *p, double radius, double *delta, double *bary,
+                       int *barysign);
+/* number of neighbor-list rebuilds forced by mesh motion */
+int64_t orc_mesh_forced_rebuilds(const orc_sys *s);
 
 #ifdef __cplusplus
 }

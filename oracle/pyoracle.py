@@ -78,6 +78,17 @@ def load():
     lib.orc_ntimestep.argtypes = [ctypes.c_void_p]
     lib.orc_ntimestep.restype = ctypes.c_int64
     lib.orc_contact_counts.argtypes = [ctypes.c_void_p, P(ctypes.c_int64), P(ctypes.c_int64)]
+    lib.orc_add_mesh.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, ctypes.c_int, P(OrcModel)]
+    lib.orc_add_mesh.restype = ctypes.c_int
+    lib.orc_mesh_rotate.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, _D]
+    lib.orc_mesh_topology.argtypes = [ctypes.c_void_p, ctypes.c_int, ip, ip, ip]
+    lib.orc_mesh_nodes.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
+    lib.orc_mesh_history.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ip, dp]
+    lib.orc_mesh_history.restype = ctypes.c_int
+    lib.orc_tri_contact.argtypes = [dp, ip, ip, dp, _D, dp, dp, ip]
+    lib.orc_tri_contact.restype = _D
+    lib.orc_mesh_forced_rebuilds.argtypes = [ctypes.c_void_p]
+    lib.orc_mesh_forced_rebuilds.restype = ctypes.c_int64
     _lib = lib
     return lib
 
@@ -170,6 +181,21 @@ def contact_wall(params, mats, itype, wtype, overlap, sdelta, radi, vi, wi, vwal
     return dF, dT, sh
 
 
+def tri_contact(nodes, p, radius, edge_active=(1, 1, 1), corner_active=(1, 1, 1)):
+    """TriMesh::resolveTriSphereContactBary on one triangle:
+    returns (distance - radius, delta, bary, barysign)."""
+    lib = load()
+    n = np.ascontiguousarray(nodes, dtype=np.float64).reshape(9)
+    ea = np.ascontiguousarray(edge_active, dtype=np.int32)
+    ca = np.ascontiguousarray(corner_active, dtype=np.int32)
+    pp = np.ascontiguousarray(p, dtype=np.float64)
+    delta, bary = np.zeros(3), np.zeros(3)
+    bs = ctypes.c_int(0)
+    d = lib.orc_tri_contact(_dp(n), _ip(ea), _ip(ca), _dp(pp), float(radius), _dp(delta),
+                            _dp(bary), ctypes.byref(bs))
+    return d, delta, bary, bs.value
+
+
 def pairs_bruteforce(params, x, radius):
     lib = load()
     p = make_params(params)
@@ -187,7 +213,9 @@ def pairs_bruteforce(params, x, radius):
 class OracleSystem:
     """The CPU restatement of Verlet::setup / Verlet::run."""
 
-    def __init__(self, params, mat, particles, walls=()):
+    def __init__(self, params, mat, particles, walls=(), meshes=()):
+        """meshes: [(tris (T,3,3), wall_type, model dict or None, rotate or None)],
+        rotate = (origin, axis, period)."""
         self.lib = load()
         self.p = make_params(params, walls)
         self.m = materials(mat)
@@ -200,6 +228,41 @@ class OracleSystem:
         self.s = self.lib.orc_create(ctypes.byref(self.p), ctypes.byref(self.m), self.n,
                                      _dp(P["x"]), _dp(P["v"]), _dp(P["omega"]),
                                      _dp(P["radius"]), _dp(P["rmass"]), _ip(ty), _ip(tg))
+        self.ntri = []
+        for (tris, wtype, mdl, rot) in meshes:
+            t = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
+            m = model(mdl if mdl is not None else params)
+            k = self.lib.orc_add_mesh(self.s, len(t), _dp(t), int(wtype), ctypes.byref(m))
+            if k < 0:
+                raise RuntimeError("orc_add_mesh failed")
+            if rot is not None:
+                o = np.ascontiguousarray(rot[0], dtype=np.float64)
+                a = np.ascontiguousarray(rot[1], dtype=np.float64)
+                self.lib.orc_mesh_rotate(self.s, k, _dp(o), _dp(a), float(rot[2]))
+            self.ntri.append(len(t))
+
+    def mesh_topology(self, k=0):
+        T = self.ntri[k]
+        ea = np.zeros((T, 3), dtype=np.int32)
+        ca = np.zeros((T, 3), dtype=np.int32)
+        ob = np.zeros(T, dtype=np.int32)
+        self.lib.orc_mesh_topology(self.s, k, _ip(ea), _ip(ca), _ip(ob))
+        return ea, ca, ob
+
+    def mesh_nodes(self, k=0):
+        T = self.ntri[k]
+        nodes, vn = np.zeros((T, 3, 3)), np.zeros((T, 3, 3))
+        self.lib.orc_mesh_nodes(self.s, k, _dp(nodes), _dp(vn))
+        return nodes, vn
+
+    def mesh_history(self, k=0, maxslot=16):
+        tri = np.zeros((self.n, maxslot), dtype=np.int32)
+        hist = np.zeros((self.n, maxslot, 3))
+        self.lib.orc_mesh_history(self.s, k, maxslot, _ip(tri), _dp(hist))
+        return tri, hist
+
+    def mesh_forced_rebuilds(self):
+        return self.lib.orc_mesh_forced_rebuilds(self.s)
 
     def setup(self):
         self.lib.orc_setup(self.s)
