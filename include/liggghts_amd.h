@@ -126,6 +126,36 @@ int dem_set_particles(dem_handle *h, int64_t n, const double *x, const double *v
 int dem_add_plane_wall(dem_handle *h, int axis, double position, int wall_type,
                        const dem_contact_model *model);
 
+/* Adds a triangle mesh wall: fix mesh/surface file <stl> type <wall_type>
+ * (fix_mesh.cpp:75-271, input_mesh_tri.cpp:318) used by fix wall/gran model hertz
+ * tangential history ... mesh n_meshes k meshes <ids> (fix_wall_gran.cpp:228-310).
+ * nodes holds ntri*9 coordinates (three nodes per triangle, as listed in the STL).
+ * Element geometry and topology (edge/corner activity, coplanar neighbors) are
+ * computed here as SurfaceMesh::buildNeighbours does (surface_mesh_I.h:454-560,
+ * 1035-1190). model: the wall fix's contact-model switches (NULL: the pair
+ * style's). *mesh_id receives the mesh index. Call before dem_set_particles. */
+int dem_add_mesh(dem_handle *h, int64_t ntri, const double *nodes, int wall_type,
+                 const dem_contact_model *model, int *mesh_id);
+
+/* fix move/mesh mesh <id> rotate origin ox oy oz axis ax ay az period T
+ * (mesh_mover_rotation.cpp:66-139): the mesh turns by 2*pi*dt/T about the axis
+ * each step and its nodes carry the rigid-body velocity. */
+int dem_mesh_rotate(dem_handle *h, int mesh_id, const double *origin, const double *axis,
+                    double period);
+
+/* Current mesh nodes and node velocities (ntri*9 each; either may be NULL). */
+int dem_get_mesh_nodes(dem_handle *h, int mesh_id, double *nodes, double *vnodes);
+
+/* Mesh topology: edge_active, corner_active (ntri*3, 0/1) and the obtuse-angle
+ * node per triangle (-1 for none), as SurfaceMesh computes them. */
+int dem_get_mesh_topology(dem_handle *h, int mesh_id, int *edge_active, int *corner_active,
+                          int *obtuse);
+
+/* Mesh contact history (FixContactHistoryMesh, fix_contact_history_mesh.cpp):
+ * per particle in caller order, maxslot slots of (triangle index within the
+ * mesh, or -1 for an empty slot) and the shear history (3 values per slot). */
+int dem_get_mesh_history(dem_handle *h, int mesh_id, int maxslot, int *tri, double *hist);
+
 /* Verlet::setup (verlet.cpp:134-205): carries contact history out, wraps periodic
  * coordinates, reorders particles by Morton cell key, rebuilds the neighbor list
  * with history carry-in, and computes forces with shearupdate = 0. */

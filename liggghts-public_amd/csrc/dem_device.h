@@ -15,6 +15,24 @@ constexpr uint32_t kFlagBit = 0x80000000u;  // contact flag stored in the neighb
 constexpr uint32_t kIndexMask = 0x7fffffffu;
 constexpr int kMetaCountMask = 0xffff;       // meta[i]: low 16 bits = list length
 constexpr int kMetaWallShift = 16;           // bits 16..21: wall history present
+constexpr int kMaxMeshes = 4;
+constexpr int kMeshSlots = 8;       // contact-history slots per particle and mesh
+constexpr int kMaxMeshCand = 32;    // candidate triangles per particle (all meshes)
+
+// One triangle of a fix mesh/surface, FP64 (TriMesh element properties).
+struct TriGeom {
+  double n[9];   // nodes
+  double ev[9];  // unit edge vectors, edge k from node k to node k+1
+  double el[3];  // edge lengths
+  double sn[3];  // unit surface normal
+  double c[3];   // centre
+  double rb;     // bounding radius
+  double vn[9];  // node velocities
+  int obtuse;    // node index with the obtuse angle, or -1
+  int flags;     // bits 0-2 edge active, bits 3-5 corner active
+  int mesh;      // mesh index
+  int pad;
+};
 
 // Scalar run parameters, passed by value to every kernel.
 struct DevParams {
@@ -33,6 +51,11 @@ struct DevParams {
   int wall_model[kMaxWalls];  // bit 0 rolling cdt, 1 tangential damping, 2 limitForce, 3 torsion
   double cutneighmax;
   int ntp1;                // ntypes + 1 (row length of the material tables)
+  // triangle meshes
+  int nmesh;
+  int mesh_tri0[kMaxMeshes], mesh_ntri[kMaxMeshes];
+  int mesh_wtype[kMaxMeshes], mesh_model[kMaxMeshes], mesh_moving[kMaxMeshes];
+  double mesh_precision;   // EPSILON_PRECISION of the mesh (multi_node_mesh.h:59)
   // cell grid
   int nc[3];
   double cinv[3];
@@ -53,7 +76,9 @@ struct Ctrl {
   unsigned long long nwall;          // touching wall contacts at the last counted pass
   unsigned long long npairs_full;    // full-list entries at the last build
   int bad;               // non-finite state detected
-  int pad;
+  int mesh_overflow;     // candidate triangles of a particle beyond kMaxMeshCand
+  int slot_overflow;     // simultaneous mesh contacts of a particle beyond kMeshSlots
+  int pad2;
 };
 
 // Particle state read by neighbors: ping-pong buffered across steps.
@@ -70,6 +95,16 @@ struct AuxBuf {
   int *orig;      // caller index
   int *meta;      // list length | wall-history bits
   void *whist;    // wall shear history, [wall][3][cap], double or float
+  int *mslot;     // mesh history slots: triangle index or -1, [mesh][slot][cap]
+  void *mhist;    // mesh shear history, [mesh][slot][3][cap], double or float
+};
+
+// Mesh data used by the step kernel.
+struct MeshStep {
+  TriGeom *tri;
+  const int *copl_off, *copl_idx;  // coplanar node-neighbor lists per triangle
+  const int *cand;                 // [kMaxMeshCand][cap] candidate triangles, ascending
+  const int *ncand;                // [cap]
 };
 
 struct ListBuf {
@@ -85,6 +120,7 @@ struct StepArgs {
   StateBuf in, out;
   AuxBuf aux;
   ListBuf list;
+  MeshStep mesh;
   double *xhold;     // [3][cap]
   double *fout;      // [6][cap] f and torque when store_f
   Ctrl *ctrl;
@@ -109,6 +145,33 @@ struct RebuildArgs {
   double *xhold;
   int *scan_tmp;
   Ctrl *ctrl;
+  // meshes
+  int ntri;                    // all meshes
+  TriGeom *tri;
+  double *node_last;           // [ntri][9] nodes at the last rebuild
+  int *tcount, *toff;          // per triangle: cells touched, offsets
+  int *tpair_cell, *tpair_tri; // triangle-cell pairs
+  int tpair_cap;
+  int *tcell_count, *tcell_start, *tcell_list;
+  int *cand, *ncand;
+  int zero_vnode;              // setup: node velocities start at zero
+};
+
+// Rigid rotation of the moving meshes for one step.
+struct MeshMoveArgs {
+  TriGeom *tri;
+  const double *node_last;
+  int ntri;
+  int moving_mask;            // bit m: mesh m rotates
+  double dQ[kMaxMeshes][4];   // incremental rotation quaternion
+  double origin[kMaxMeshes][3];
+  double omegaVec[kMaxMeshes][3];
+  int trans[kMaxMeshes];
+  double triggersq;
+  long long step;
+  int guard;                  // return if step >= stop_at
+  int decide;                 // raise stop_at = step+1 when a node moved > skin/2
+  Ctrl *ctrl;
 };
 
 // launchers (dem_kernels.hip)
@@ -116,6 +179,15 @@ hipError_t launch_rebuild(const RebuildArgs &a, hipStream_t s);
 hipError_t launch_step(const StepArgs &a, hipStream_t s);
 hipError_t launch_initial(const StepArgs &a, hipStream_t s);
 hipError_t launch_set_stop(Ctrl *ctrl, long long v, hipStream_t s);
+hipError_t launch_mesh_move(const MeshMoveArgs &a, hipStream_t s);
+// Mesh part of a rebuild, after the particle gather and build. Phase 1:
+// element geometry refreshed from the nodes, node positions stored, and the
+// number of (triangle, cell) pairs counted; the total lands in *total (device).
+hipError_t launch_mesh_bin_count(const RebuildArgs &a, int *total, hipStream_t s);
+// Phase 2 (tpair_cap >= total): triangle lists per cell, candidate triangles
+// per particle, slot compaction and removal of slots whose triangle is no
+// longer a candidate.
+hipError_t launch_mesh_bin_fill(const RebuildArgs &a, hipStream_t s);
 int scan_tmp_size(int ncell);
 
 }  // namespace dem

@@ -14,6 +14,7 @@
 // sum rounds as in the CPU reference, so one pair's contribution is bitwise
 // equal to the reference's; only the order of the per-particle sums differs.
 #include "dem_device.h"
+#include "mesh_device.h"
 
 namespace dem {
 
@@ -229,12 +230,167 @@ __device__ __forceinline__ void contact_chain(const Model &p, const CIn<T> &c, T
 }
 
 // ---------------------------------------------------------------------------
+// mesh walls for one particle: FixWallGran::post_force_mesh (fix_wall_gran.cpp:
+// 845-1018) with FixContactHistoryMesh::markAllContacts / handleContact /
+// cleanUpContacts (fix_contact_history_mesh_I.h:51-200,
+// fix_contact_history_mesh.cpp:405-464). Candidates are visited in ascending
+// triangle index, mesh by mesh, as the reference's triangle loop visits them.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool tri_coplanar(const MeshStep &M, int a, int b) {
+  for (int k = M.copl_off[a]; k < M.copl_off[a + 1]; k++)
+    if (M.copl_idx[k] == b) return true;
+  return false;
+}
+
+template <typename T, typename W>
+__device__ inline void mesh_contacts(const StepArgs &a, int i, const double4 &pi,
+                                     const double4 &vi, const W &wi, int itype, T &fx, T &fy,
+                                     T &fz, T &tx, T &ty, T &tz,
+                                     unsigned long long &nwall) {
+  const DevParams &p = a.prm;
+  const MeshStep &M = a.mesh;
+  const int nc = M.ncand[i];
+  if (nc == 0) return;
+  const size_t cap = (size_t)a.cap;
+  const double px[3] = {pi.x, pi.y, pi.z};
+  int c = 0;
+  for (int m = 0; m < p.nmesh; m++) {
+    int *slot = a.aux.mslot + (size_t)m * kMeshSlots * cap + i;
+    T *hist = (T *)a.aux.mhist + (size_t)m * kMeshSlots * 3 * cap + i;
+    const int tend = p.mesh_tri0[m] + p.mesh_ntri[m];
+    const Model mdl = model_from_bits(p.mesh_model[m]);
+    const int mt = itype * p.ntp1 + p.mesh_wtype[m];
+    unsigned keep = 0u;
+    for (; c < nc; c++) {
+      const int t = M.cand[(size_t)c * cap + i];
+      if (t >= tend) break;
+      const TriGeom &tg = M.tri[t];
+      Vec3 delta;
+      Bary bary;
+      int bs;
+      const double deltan = tri_sphere(tg, p.mesh_precision, pi.w, px, delta, bary, bs);
+      if (deltan > p.cutneighmax) continue;
+      if (!(deltan <= 0 || deltan < (p.cdf - 1.0) * pi.w)) continue;
+      // handleContact
+      int s = -1;
+      for (int k = 0; k < kMeshSlots; k++)
+        if (slot[(size_t)k * cap] == t) {
+          s = k;
+          break;
+        }
+      bool fresh = false;
+      if (s >= 0) {
+        keep |= 1u << s;
+      } else {
+        bool skip = false;
+        if (bs == 7)
+          for (int k = 0; k < kMeshSlots; k++) {
+            const int st = slot[(size_t)k * cap];
+            if (st >= 0 && st != t && ((keep >> k) & 1u) && tri_coplanar(M, st, t)) skip = true;
+          }
+        if (skip) continue;
+        for (int k = 0; k < kMeshSlots; k++)
+          if (slot[(size_t)k * cap] == -1) {
+            s = k;
+            break;
+          }
+        if (s < 0) {
+          atomicMax(&a.ctrl->slot_overflow, 1);
+          continue;
+        }
+        slot[(size_t)s * cap] = t;
+        keep |= 1u << s;
+        fresh = true;
+      }
+      T sh[3];
+      if (fresh) {
+        sh[0] = sh[1] = sh[2] = T(0);
+        if (bs == 7)  // checkCoplanarContactHistory: the last coplanar slot wins
+          for (int k = 0; k < kMeshSlots; k++) {
+            const int st = slot[(size_t)k * cap];
+            if (st >= 0 && st != t && tri_coplanar(M, st, t)) {
+              sh[0] = hist[(size_t)(k * 3) * cap];
+              sh[1] = hist[(size_t)(k * 3 + 1) * cap];
+              sh[2] = hist[(size_t)(k * 3 + 2) * cap];
+            }
+          }
+      } else {
+        sh[0] = hist[(size_t)(s * 3) * cap];
+        sh[1] = hist[(size_t)(s * 3 + 1) * cap];
+        sh[2] = hist[(size_t)(s * 3 + 2) * cap];
+      }
+      double vw[3] = {0., 0., 0.};
+      if (p.mesh_moving[m])
+        for (int k = 0; k < 3; k++)
+          vw[k] = (bary.b0 * tg.vn[k] + bary.b1 * tg.vn[3 + k] + bary.b2 * tg.vn[6 + k]);
+      if (deltan <= 0) {
+        // sidata.deltan = -deltan, sidata.delta = -delta; Walls::Granular::compute_force
+        const double sdn = -deltan;
+        const double r = pi.w - sdn;
+        const double rinv = 1.0 / r;
+        const double sd0 = -delta.x, sd1 = -delta.y, sd2 = -delta.z;
+        CIn<T> cc;
+        cc.radi = (T)pi.w;
+        cc.radj = T(0);
+        cc.rinv = (T)rinv;
+        cc.meff = (T)vi.w;
+        cc.deltan = (T)(pi.w - r);
+        cc.deltan_in = (T)sdn;
+        cc.del[0] = (T)sd0;
+        cc.del[1] = (T)sd1;
+        cc.del[2] = (T)sd2;
+        cc.en[0] = (T)(sd0 * rinv);
+        cc.en[1] = (T)(sd1 * rinv);
+        cc.en[2] = (T)(sd2 * rinv);
+        cc.vr[0] = (T)(vi.x - vw[0]);
+        cc.vr[1] = (T)(vi.y - vw[1]);
+        cc.vr[2] = (T)(vi.z - vw[2]);
+        cc.wi[0] = (T)wi.x;
+        cc.wi[1] = (T)wi.y;
+        cc.wi[2] = (T)wi.z;
+        cc.wj[0] = cc.wj[1] = cc.wj[2] = T(0);
+        if (sizeof(T) == 8) {
+          cc.Y = (T)a.mat.Yeff[mt];
+          cc.G = (T)a.mat.Geff[mt];
+          cc.beta = (T)a.mat.beta[mt];
+          cc.mu = (T)a.mat.mu[mt];
+          cc.mur = (T)a.mat.mur[mt];
+        } else {
+          cc.Y = (T)a.mat.fYeff[mt];
+          cc.G = (T)a.mat.fGeff[mt];
+          cc.beta = (T)a.mat.fbeta[mt];
+          cc.mu = (T)a.mat.fmu[mt];
+          cc.mur = (T)a.mat.fmur[mt];
+        }
+        T dF[3], dTi[3], dTj[3];
+        contact_chain<T, true>(mdl, cc, (T)p.dt, a.shearupdate, sh, dF, dTi, dTj);
+        fx += dF[0];
+        fy += dF[1];
+        fz += dF[2];
+        tx += dTi[0];
+        ty += dTi[1];
+        tz += dTi[2];
+        nwall++;
+      }
+      if (a.shearupdate || fresh) {
+        hist[(size_t)(s * 3) * cap] = sh[0];
+        hist[(size_t)(s * 3 + 1) * cap] = sh[1];
+        hist[(size_t)(s * 3 + 2) * cap] = sh[2];
+      }
+    }
+    // cleanUpContacts: slots not touched this pass are freed
+    for (int k = 0; k < kMeshSlots; k++)
+      if (!((keep >> k) & 1u) && slot[(size_t)k * cap] != -1) slot[(size_t)k * cap] = -1;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // fused step kernel: pair forces over the full list, plane walls, gravity,
 // nve/sphere final integrate of this step and initial integrate of the next,
 // and the skin/2 displacement check of Neighbor::decide for the next step.
 // One thread per particle; reads the state buffer `in`, writes `out`.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool MESH>
 __global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
   typedef typename OmgT<T>::type omg_t;
   if (a.step >= a.ctrl->stop_at) return;
@@ -461,6 +617,8 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
           meta &= ~bit;  // history zeroed when the contact ends (fix_wall_gran.cpp:1145-1151)
         }
       }
+      if (MESH)
+        mesh_contacts<T>(a, i, pi, vi, wi, itype, fx, fy, fz, tx, ty, tz, my_wall);
     }
 
     if (a.store_f) {
@@ -744,6 +902,19 @@ __global__ __launch_bounds__(kBlock) void k_gather(RebuildArgs a) {
   a.xhold[i] = q.x;
   a.xhold[cap + i] = q.y;
   a.xhold[2 * cap + i] = q.z;
+  const T *hi = (const T *)a.acur.mhist;
+  T *ho = (T *)a.aalt.mhist;
+  for (int m = 0; m < a.prm.nmesh; m++)
+    for (int k = 0; k < kMeshSlots; k++) {
+      const size_t sb = (size_t)(m * kMeshSlots + k) * cap;
+      const int st = a.acur.mslot[sb + o];
+      a.aalt.mslot[sb + i] = st;
+      if (st >= 0)
+        for (int d = 0; d < 3; d++) {
+          const size_t hb = (size_t)((m * kMeshSlots + k) * 3 + d) * cap;
+          ho[hb + i] = hi[hb + o];
+        }
+    }
 }
 
 template <typename T>
@@ -835,7 +1006,248 @@ __global__ void k_reset_build_counters(Ctrl *ctrl) {
     ctrl->overflow = 0;
     ctrl->maxcount = 0;
     ctrl->npairs_full = 0;
+    ctrl->mesh_overflow = 0;
   }
+}
+
+// ---------------------------------------------------------------------------
+// triangle meshes: rebuild-time geometry refresh, binning and candidates
+// ---------------------------------------------------------------------------
+
+// FixMesh::pre_force on a rebuild step refreshes the element geometry from the
+// nodes and stores the node positions (pbcExchangeBorders -> refreshOwned,
+// multi_node_mesh_I.h:212-226, surface_mesh_I.h:146-200)
+__global__ __launch_bounds__(kBlock) void k_mesh_geom(RebuildArgs a) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= a.ntri) return;
+  TriGeom &tg = a.tri[t];
+  tri_surf_props(tg);
+  for (int k = 0; k < 9; k++) a.node_last[9 * t + k] = tg.n[k];
+  if (a.zero_vnode)
+    for (int k = 0; k < 9; k++) tg.vn[k] = 0.0;
+}
+
+// cells a triangle can reach: centre within (r_max*cdf + skin) + half a cell
+// diagonal of the triangle
+__device__ __forceinline__ void tri_cell_range(const RebuildArgs &a, const TriGeom &tg, int lo[3],
+                                               int hi[3], double &reach2) {
+  const DevParams &p = a.prm;
+  const double e = p.cutneighmax - p.skin;  // (r_max + r_max) * cdf
+  const double reach = 0.5 * e + p.skin;    // r_max * cdf + skin
+  double hd = 0.0;
+  for (int d = 0; d < 3; d++) {
+    double mn = tg.n[d], mx = tg.n[d];
+    for (int j = 1; j < 3; j++) {
+      mn = fmin(mn, tg.n[3 * j + d]);
+      mx = fmax(mx, tg.n[3 * j + d]);
+    }
+    lo[d] = cell_coord(p, d, mn - reach);
+    hi[d] = cell_coord(p, d, mx + reach);
+    const double cs = p.L[d] / p.nc[d];
+    hd += 0.25 * cs * cs;
+  }
+  const double r = reach + sqrt(hd);
+  reach2 = r * r;
+}
+
+__device__ __forceinline__ void cell_center(const DevParams &p, int cx, int cy, int cz, double *c) {
+  c[0] = p.lo[0] + (cx + 0.5) * (p.L[0] / p.nc[0]);
+  c[1] = p.lo[1] + (cy + 0.5) * (p.L[1] / p.nc[1]);
+  c[2] = p.lo[2] + (cz + 0.5) * (p.L[2] / p.nc[2]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_tri_count(RebuildArgs a) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= a.ntri) return;
+  const TriGeom &tg = a.tri[t];
+  int lo[3], hi[3];
+  double reach2;
+  tri_cell_range(a, tg, lo, hi, reach2);
+  int cnt = 0;
+  for (int cz = lo[2]; cz <= hi[2]; cz++)
+    for (int cy = lo[1]; cy <= hi[1]; cy++)
+      for (int cx = lo[0]; cx <= hi[0]; cx++) {
+        double cc[3];
+        cell_center(a.prm, cx, cy, cz, cc);
+        if (tri_point_dist2(&tg.n[0], &tg.n[3], &tg.n[6], cc) <= reach2) cnt++;
+      }
+  a.tcount[t] = cnt;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tri_fill(RebuildArgs a) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= a.ntri) return;
+  const TriGeom &tg = a.tri[t];
+  int lo[3], hi[3];
+  double reach2;
+  tri_cell_range(a, tg, lo, hi, reach2);
+  int w = a.toff[t];
+  const int wend = w + a.tcount[t];
+  for (int cz = lo[2]; cz <= hi[2]; cz++)
+    for (int cy = lo[1]; cy <= hi[1]; cy++)
+      for (int cx = lo[0]; cx <= hi[0]; cx++) {
+        double cc[3];
+        cell_center(a.prm, cx, cy, cz, cc);
+        if (tri_point_dist2(&tg.n[0], &tg.n[3], &tg.n[6], cc) <= reach2 && w < wend) {
+          a.tpair_cell[w] = a.cell_rank[cx + a.prm.nc[0] * (cy + a.prm.nc[1] * cz)];
+          a.tpair_tri[w] = t;
+          w++;
+        }
+      }
+}
+
+__global__ __launch_bounds__(kBlock) void k_tcell_hist(RebuildArgs a, int npairs) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= npairs) return;
+  atomicAdd(&a.tcell_count[a.tpair_cell[k]], 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_tcell_scatter(RebuildArgs a, int npairs) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= npairs) return;
+  const int c = a.tpair_cell[k];
+  const int pos = a.tcell_start[c] + atomicAdd(&a.tcell_count[c], 1);
+  a.tcell_list[pos] = a.tpair_tri[k];
+}
+
+template <typename T>
+__device__ __forceinline__ void slot_swap(int *slot, T *hist, size_t cap, int x, int y) {
+  const int ts = slot[(size_t)x * cap];
+  slot[(size_t)x * cap] = slot[(size_t)y * cap];
+  slot[(size_t)y * cap] = ts;
+  for (int d = 0; d < 3; d++) {
+    const T h = hist[(size_t)(x * 3 + d) * cap];
+    hist[(size_t)(x * 3 + d) * cap] = hist[(size_t)(y * 3 + d) * cap];
+    hist[(size_t)(y * 3 + d) * cap] = h;
+  }
+}
+
+// per particle: FixContactHistoryMesh::sort_contacts (slot compaction,
+// fix_contact_history_mesh.cpp:374-400), candidate triangles within
+// r*cdf + skin in ascending index, and cleanUpContactJumps (:466-500)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mesh_cand(RebuildArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const DevParams &p = a.prm;
+  const size_t cap = (size_t)a.cap;
+  const double4 pi = a.alt.pos[i];
+  // candidates
+  const int cx = cell_coord(p, 0, pi.x), cy = cell_coord(p, 1, pi.y), cz = cell_coord(p, 2, pi.z);
+  const int rank = a.cell_rank[cx + p.nc[0] * (cy + p.nc[1] * cz)];
+  const double lim = pi.w * p.cdf + p.skin;
+  const double px[3] = {pi.x, pi.y, pi.z};
+  int nc = 0;
+  for (int k = a.tcell_start[rank]; k < a.tcell_start[rank + 1]; k++) {
+    const int t = a.tcell_list[k];
+    const TriGeom &tg = a.tri[t];
+    if (tri_point_dist2(&tg.n[0], &tg.n[3], &tg.n[6], px) <= lim * lim) {
+      if (nc < kMaxMeshCand) {
+        // insertion into the ascending list
+        int m = nc - 1;
+        while (m >= 0 && a.cand[(size_t)m * cap + i] > t) {
+          a.cand[(size_t)(m + 1) * cap + i] = a.cand[(size_t)m * cap + i];
+          m--;
+        }
+        a.cand[(size_t)(m + 1) * cap + i] = t;
+      }
+      nc++;
+    }
+  }
+  if (nc > kMaxMeshCand) {
+    atomicMax(&a.ctrl->mesh_overflow, nc);
+    nc = kMaxMeshCand;
+  }
+  a.ncand[i] = nc;
+  // slots, mesh by mesh
+  for (int m = 0; m < p.nmesh; m++) {
+    int *slot = a.aalt.mslot + (size_t)m * kMeshSlots * cap + i;
+    T *hist = (T *)a.aalt.mhist + (size_t)m * kMeshSlots * 3 * cap + i;
+    int first_empty, last_filled;
+    do {
+      first_empty = last_filled = -1;
+      for (int j = 0; j < kMeshSlots; j++) {
+        const int st = slot[(size_t)j * cap];
+        if (-1 == first_empty && -1 == st) first_empty = j;
+        if (st >= 0) last_filled = j;
+      }
+      if (first_empty > -1 && last_filled > -1 && first_empty < last_filled)
+        slot_swap(slot, hist, cap, first_empty, last_filled);
+    } while (first_empty > -1 && last_filled > -1 && first_empty < last_filled);
+    int np = 0;
+    for (int j = 0; j < kMeshSlots; j++) np += slot[(size_t)j * cap] >= 0;
+    int ip = 0;
+    while (ip < np) {
+      const int st = slot[(size_t)ip * cap];
+      bool inlist = false;
+      for (int k = 0; k < nc; k++)
+        if (a.cand[(size_t)k * cap + i] == st) inlist = true;
+      if (!inlist) {
+        slot[(size_t)ip * cap] = -1;
+        for (int d = 0; d < 3; d++) hist[(size_t)(ip * 3 + d) * cap] = T(0);
+        slot_swap(slot, hist, cap, ip, np - 1);
+        np--;
+      } else {
+        ip++;
+      }
+    }
+  }
+}
+
+// FixMoveMesh::initial_integrate -> MeshMoverRotate::initial_integrate
+// (mesh_mover_rotation.cpp:110-139), MultiNodeMesh::rotate
+// (multi_node_mesh_I.h:620-660), and MultiNodeMesh::decideRebuild
+// (multi_node_mesh_I.h:792-825) for the step
+__global__ __launch_bounds__(kBlock) void k_mesh_move(MeshMoveArgs a) {
+  if (a.guard && a.step >= a.ctrl->stop_at) return;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  bool moved = false;
+  if (t < a.ntri) {
+    TriGeom &tg = a.tri[t];
+    const int m = tg.mesh;
+    if ((a.moving_mask >> m) & 1) {
+      const double *dQ = a.dQ[m];
+      const double *o = a.origin[m];
+      double c[3] = {0., 0., 0.};
+      for (int j = 0; j < 3; j++) {
+        double *nd = &tg.n[3 * j];
+        double v[3] = {nd[0], nd[1], nd[2]};
+        if (a.trans[m]) d3_sub(v, o, v);
+        vec_quat_rotate(v, dQ);
+        if (a.trans[m])
+          for (int k = 0; k < 3; k++) v[k] = v[k] + o[k];
+        for (int k = 0; k < 3; k++) {
+          nd[k] = v[k];
+          c[k] = v[k] + c[k];
+        }
+      }
+      d3_div(c, 3.0);
+      for (int k = 0; k < 3; k++) tg.c[k] = c[k];
+      for (int j = 0; j < 3; j++) {
+        double v[3] = {tg.ev[3 * j], tg.ev[3 * j + 1], tg.ev[3 * j + 2]};
+        vec_quat_rotate(v, dQ);
+        for (int k = 0; k < 3; k++) tg.ev[3 * j + k] = v[k];
+      }
+      double sn[3] = {tg.sn[0], tg.sn[1], tg.sn[2]};
+      vec_quat_rotate(sn, dQ);
+      for (int k = 0; k < 3; k++) tg.sn[k] = sn[k];
+      for (int j = 0; j < 3; j++) {
+        double rPA[3], vRot[3];
+        d3_sub(&tg.n[3 * j], o, rPA);
+        d3_cross(a.omegaVec[m], rPA, vRot);
+        for (int k = 0; k < 3; k++) tg.vn[3 * j + k] = 0.0 + vRot[k];
+      }
+      if (a.decide)
+        for (int j = 0; j < 3; j++) {
+          const double dx = tg.n[3 * j] - a.node_last[9 * t + 3 * j];
+          const double dy = tg.n[3 * j + 1] - a.node_last[9 * t + 3 * j + 1];
+          const double dz = tg.n[3 * j + 2] - a.node_last[9 * t + 3 * j + 2];
+          if (dx * dx + dy * dy + dz * dz > a.triggersq) moved = true;
+        }
+    }
+  }
+  if (__ballot(moved) != 0ull && (threadIdx.x & 63) == 0)
+    atomicMin((unsigned long long *)&a.ctrl->stop_at, (unsigned long long)(a.step + 1));
 }
 
 // ---------------------------------------------------------------------------
@@ -843,7 +1255,7 @@ __global__ void k_reset_build_counters(Ctrl *ctrl) {
 // ---------------------------------------------------------------------------
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
-int scan_tmp_size(int ncell) { return nblk(ncell, kScanPer) + 16; }
+int scan_tmp_size(int n) { return nblk(n, kScanPer) + 16; }
 
 hipError_t launch_rebuild(const RebuildArgs &a, hipStream_t s) {
   const int nb = nblk(a.n, kBlock);
@@ -874,10 +1286,18 @@ hipError_t launch_rebuild(const RebuildArgs &a, hipStream_t s) {
 
 hipError_t launch_step(const StepArgs &a, hipStream_t s) {
   const int nb = nblk(a.n > 0 ? a.n : 1, kBlock);
-  if (a.mixed)
-    hipLaunchKernelGGL(k_step<float>, dim3(nb), dim3(kBlock), 0, s, a);
-  else
-    hipLaunchKernelGGL(k_step<double>, dim3(nb), dim3(kBlock), 0, s, a);
+  const bool mesh = a.prm.nmesh > 0;
+  if (a.mixed) {
+    if (mesh)
+      hipLaunchKernelGGL((k_step<float, true>), dim3(nb), dim3(kBlock), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_step<float, false>), dim3(nb), dim3(kBlock), 0, s, a);
+  } else {
+    if (mesh)
+      hipLaunchKernelGGL((k_step<double, true>), dim3(nb), dim3(kBlock), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_step<double, false>), dim3(nb), dim3(kBlock), 0, s, a);
+  }
   return hipGetLastError();
 }
 
@@ -888,6 +1308,53 @@ hipError_t launch_initial(const StepArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_initial<float>, dim3(nb), dim3(kBlock), 0, s, a);
   else
     hipLaunchKernelGGL(k_initial<double>, dim3(nb), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+static void launch_scan(const int *in, int *out, int *tmp, int n, hipStream_t s) {
+  const int nsb = nblk(n, kScanPer);
+  hipLaunchKernelGGL(k_scan_block, dim3(nsb), dim3(256), 0, s, in, out, tmp, n);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, tmp, nsb, out + n);
+  hipLaunchKernelGGL(k_scan_add, dim3(nsb), dim3(256), 0, s, out, tmp, n);
+}
+
+hipError_t launch_mesh_bin_count(const RebuildArgs &a, int *total, hipStream_t s) {
+  if (a.ntri <= 0) return hipSuccess;
+  const int nt = nblk(a.ntri, kBlock);
+  hipLaunchKernelGGL(k_mesh_geom, dim3(nt), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_tri_count, dim3(nt), dim3(kBlock), 0, s, a);
+  // toff has ntri+1 entries; the scan writes the total at toff[ntri]
+  launch_scan(a.tcount, a.toff, a.scan_tmp, a.ntri, s);
+  const hipError_t e = hipMemcpyAsync(total, a.toff + a.ntri, sizeof(int),
+                                      hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+hipError_t launch_mesh_bin_fill(const RebuildArgs &a, hipStream_t s) {
+  if (a.ntri <= 0) return hipSuccess;
+  const int nt = nblk(a.ntri, kBlock);
+  const int ncb = nblk(a.ncell, kBlock);
+  hipLaunchKernelGGL(k_tri_fill, dim3(nt), dim3(kBlock), 0, s, a);
+  const int np = a.tpair_cap;  // exactly the pair total (sized by the caller)
+  hipLaunchKernelGGL(k_zero, dim3(ncb), dim3(kBlock), 0, s, a.tcell_count, a.ncell);
+  if (np > 0) hipLaunchKernelGGL(k_tcell_hist, dim3(nblk(np, kBlock)), dim3(kBlock), 0, s, a, np);
+  launch_scan(a.tcell_count, a.tcell_start, a.scan_tmp, a.ncell, s);
+  hipLaunchKernelGGL(k_zero, dim3(ncb), dim3(kBlock), 0, s, a.tcell_count, a.ncell);
+  if (np > 0)
+    hipLaunchKernelGGL(k_tcell_scatter, dim3(nblk(np, kBlock)), dim3(kBlock), 0, s, a, np);
+  if (a.n > 0) {
+    if (a.mixed)
+      hipLaunchKernelGGL(k_mesh_cand<float>, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, s, a);
+    else
+      hipLaunchKernelGGL(k_mesh_cand<double>, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_mesh_move(const MeshMoveArgs &a, hipStream_t s) {
+  if (a.ntri <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mesh_move, dim3(nblk(a.ntri, kBlock)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 

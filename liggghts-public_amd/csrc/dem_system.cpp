@@ -20,6 +20,7 @@
 
 #include "../../include/liggghts_amd.h"
 #include "dem_device.h"
+#include "mesh_host.h"
 
 using dem::AuxBuf;
 using dem::Ctrl;
@@ -62,6 +63,21 @@ struct dem_handle {
   void *mat_block = nullptr;
 
   std::vector<Wall> walls;
+
+  // triangle meshes (host copies; device copies live with the particles)
+  std::vector<dem::HostMesh> meshes;
+  int ntri = 0;
+  dem::TriGeom *d_tri = nullptr;
+  double *d_node_last = nullptr;
+  int *d_copl_off = nullptr, *d_copl_idx = nullptr;
+  int *d_tcount = nullptr, *d_toff = nullptr;
+  int *d_tpair_cell = nullptr, *d_tpair_tri = nullptr, *d_tcell_list = nullptr;
+  int tpair_cap = 0;
+  int *d_tcell_count = nullptr, *d_tcell_start = nullptr;
+  int *d_cand = nullptr, *d_ncand = nullptr;
+  int *d_total = nullptr;
+  int *h_total = nullptr;  // pinned
+  bool zero_vnode = false;
 
   // particles
   int64_t n = 0;
@@ -144,6 +160,12 @@ void free_particles(dem_handle *h) {
     h->lst[b] = ListBuf{};
   }
   h->xhold = h->fout = nullptr;
+  h->d_tri = nullptr;
+  h->d_node_last = nullptr;
+  h->d_copl_off = h->d_copl_idx = nullptr;
+  h->d_tcount = h->d_toff = h->d_tpair_cell = h->d_tpair_tri = h->d_tcell_list = nullptr;
+  h->tpair_cap = 0;
+  h->d_tcell_count = h->d_tcell_start = h->d_cand = h->d_ncand = h->d_total = nullptr;
   h->cell_rank = h->cell_count = h->cell_start = nullptr;
   h->key = h->perm = h->scan_tmp = nullptr;
   h->ncell = 0;
@@ -187,6 +209,17 @@ dem::DevParams make_devparams(const dem_handle *h) {
   }
   d.cutneighmax = h->cutneighmax;
   d.ntp1 = h->ntypes + 1;
+  d.nmesh = (int)h->meshes.size();
+  int t0 = 0;
+  for (size_t m = 0; m < h->meshes.size(); m++) {
+    d.mesh_tri0[m] = t0;
+    d.mesh_ntri[m] = h->meshes[m].ntri;
+    d.mesh_wtype[m] = h->meshes[m].wall_type;
+    d.mesh_model[m] = h->meshes[m].model;
+    d.mesh_moving[m] = h->meshes[m].moving ? 1 : 0;
+    t0 += h->meshes[m].ntri;
+  }
+  d.mesh_precision = h->meshes.empty() ? 1e-8 : h->meshes[0].precision;
   return d;
 }
 
@@ -233,7 +266,9 @@ int setup_grid(dem_handle *h) {
   if ((rc = dalloc(h, (void **)&h->cell_rank, sizeof(int) * ncell))) return rc;
   if ((rc = dalloc(h, (void **)&h->cell_count, sizeof(int) * ncell))) return rc;
   if ((rc = dalloc(h, (void **)&h->cell_start, sizeof(int) * (ncell + 1)))) return rc;
-  if ((rc = dalloc(h, (void **)&h->scan_tmp, sizeof(int) * dem::scan_tmp_size(ncell)))) return rc;
+  if ((rc = dalloc(h, (void **)&h->scan_tmp,
+                   sizeof(int) * dem::scan_tmp_size(std::max(ncell, h->ntri + 1)))))
+    return rc;
   HIPCHK(h, hipMemcpy(h->cell_rank, rank.data(), sizeof(int) * ncell, hipMemcpyHostToDevice));
   h->ncell = ncell;
   return DEM_OK;
@@ -262,6 +297,20 @@ dem::RebuildArgs rebuild_args(dem_handle *h) {
   a.xhold = h->xhold;
   a.scan_tmp = h->scan_tmp;
   a.ctrl = h->ctrl;
+  a.ntri = h->ntri;
+  a.tri = h->d_tri;
+  a.node_last = h->d_node_last;
+  a.tcount = h->d_tcount;
+  a.toff = h->d_toff;
+  a.tpair_cell = h->d_tpair_cell;
+  a.tpair_tri = h->d_tpair_tri;
+  a.tpair_cap = h->tpair_cap;
+  a.tcell_count = h->d_tcell_count;
+  a.tcell_start = h->d_tcell_start;
+  a.tcell_list = h->d_tcell_list;
+  a.cand = h->d_cand;
+  a.ncand = h->d_ncand;
+  a.zero_vnode = h->zero_vnode ? 1 : 0;
   return a;
 }
 
@@ -277,9 +326,54 @@ dem::StepArgs step_args(dem_handle *h, int in_buf) {
   a.out = h->st[1 - in_buf];
   a.aux = h->aux[h->cur_aux];
   a.list = h->lst[h->cur_l];
+  a.mesh.tri = h->d_tri;
+  a.mesh.copl_off = h->d_copl_off;
+  a.mesh.copl_idx = h->d_copl_idx;
+  a.mesh.cand = h->d_cand;
+  a.mesh.ncand = h->d_ncand;
   a.xhold = h->xhold;
   a.fout = h->fout;
   a.ctrl = h->ctrl;
+  return a;
+}
+
+bool any_moving_mesh(const dem_handle *h) {
+  for (const auto &m : h->meshes)
+    if (m.moving) return true;
+  return false;
+}
+
+// one step of fix move/mesh rotate for every moving mesh
+dem::MeshMoveArgs mesh_move_args(const dem_handle *h, long long step, int guard, int decide) {
+  dem::MeshMoveArgs a{};
+  a.tri = h->d_tri;
+  a.node_last = h->d_node_last;
+  a.ntri = h->ntri;
+  a.triggersq = 0.25 * h->p.skin * h->p.skin;
+  a.step = step;
+  a.guard = guard;
+  a.decide = decide;
+  a.ctrl = h->ctrl;
+  for (size_t m = 0; m < h->meshes.size(); m++) {
+    const dem::HostMesh &hm = h->meshes[m];
+    if (!hm.moving) continue;
+    a.moving_mask |= 1 << m;
+    // MultiNodeMesh::rotate(dAngle, axis, p), multi_node_mesh_I.h:620-641
+    const double dAngle = hm.omega * h->p.dt;
+    double axisNorm[3] = {hm.axis[0], hm.axis[1], hm.axis[2]};
+    const double mg = std::sqrt(axisNorm[0] * axisNorm[0] + axisNorm[1] * axisNorm[1] +
+                                axisNorm[2] * axisNorm[2]);
+    const double sinv = 1. / mg;
+    for (int k = 0; k < 3; k++) axisNorm[k] = sinv * axisNorm[k];
+    a.dQ[m][0] = std::cos(dAngle * 0.5);
+    for (int k = 0; k < 3; k++) a.dQ[m][k + 1] = axisNorm[k] * std::sin(dAngle * 0.5);
+    for (int k = 0; k < 3; k++) {
+      a.origin[m][k] = hm.origin[k];
+      a.omegaVec[m][k] = hm.axis[k] * hm.omega;
+    }
+    a.trans[m] = (hm.origin[0] * hm.origin[0] + hm.origin[1] * hm.origin[1] +
+                  hm.origin[2] * hm.origin[2]) > 0.;
+  }
   return a;
 }
 
@@ -300,9 +394,37 @@ int rebuild(dem_handle *h) {
   }
   dem::RebuildArgs a = rebuild_args(h);
   HIPCHK(h, dem::launch_rebuild(a, h->stream));
+  if (h->ntri > 0) {
+    HIPCHK(h, dem::launch_mesh_bin_count(a, h->d_total, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->h_total, h->d_total, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const int total = *h->h_total;
+    if (total < 0) return fail(h, DEM_ERR_STATE, "mesh binning overflow");
+    if (total > h->tpair_cap) {
+      // grow the (triangle, cell) pair arrays; the old ones stay allocated until
+      // the particle arrays are freed
+      const int cap = total + total / 4 + 1024;
+      int rc2;
+      if ((rc2 = dalloc(h, (void **)&h->d_tpair_cell, sizeof(int) * cap))) return rc2;
+      if ((rc2 = dalloc(h, (void **)&h->d_tpair_tri, sizeof(int) * cap))) return rc2;
+      if ((rc2 = dalloc(h, (void **)&h->d_tcell_list, sizeof(int) * cap))) return rc2;
+      h->tpair_cap = cap;
+      a.tpair_cell = h->d_tpair_cell;
+      a.tpair_tri = h->d_tpair_tri;
+      a.tcell_list = h->d_tcell_list;
+    }
+    a.tpair_cap = total;  // the fill phase takes the exact pair count here
+    HIPCHK(h, dem::launch_mesh_bin_fill(a, h->stream));
+  }
   if (h->timing) HIPCHK(h, hipEventRecord(e1, h->stream));
   int rc = read_ctrl(h);
   if (rc) return rc;
+  if (h->ctrl_host->mesh_overflow > 0) {
+    char buf[200];
+    snprintf(buf, sizeof buf, "a particle has %d candidate mesh triangles (limit %d)",
+             h->ctrl_host->mesh_overflow, dem::kMaxMeshCand);
+    return fail(h, DEM_ERR_OVERFLOW, buf);
+  }
   if (h->timing) {
     float ms = 0.f;
     HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
@@ -359,7 +481,10 @@ int do_setup(dem_handle *h) {
   int rc = check_ready(h);
   if (rc) return rc;
   HIPCHK(h, hipSetDevice(h->p.device));
-  if ((rc = rebuild(h))) return rc;
+  h->zero_vnode = true;  // FixMoveMesh::setup zeroes node velocities (fix_move_mesh.cpp:211-216)
+  rc = rebuild(h);
+  h->zero_vnode = false;
+  if (rc) return rc;
   if ((rc = force_pass(h, 0))) return rc;  // pair_gran.cpp:665-667: shearupdate 0 in setup
   h->setup_done = true;
   h->can_continue = true;
@@ -408,6 +533,11 @@ int dem_create(const dem_params *params, dem_handle **out) {
     dem_destroy(h);
     return DEM_ERR_HIP;
   }
+  if (hipHostMalloc((void **)&h->h_total, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    g_create_error = "pinned allocation failed";
+    dem_destroy(h);
+    return DEM_ERR_HIP;
+  }
   std::memset(h->ctrl_host, 0, sizeof(Ctrl));
   h->ctrl_host->stop_at = kNever;
   hipMemcpy(h->ctrl, h->ctrl_host, sizeof(Ctrl), hipMemcpyHostToDevice);
@@ -424,6 +554,7 @@ int dem_destroy(dem_handle *h) {
   for (auto &e : h->ev)
     if (e) hipEventDestroy(e);
   if (h->ctrl_host) hipHostFree(h->ctrl_host);
+  if (h->h_total) hipHostFree(h->h_total);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return DEM_OK;
@@ -513,6 +644,123 @@ int dem_add_plane_wall(dem_handle *h, int axis, double position, int wall_type,
   return DEM_OK;
 }
 
+int dem_add_mesh(dem_handle *h, int64_t ntri, const double *nodes, int wall_type,
+                 const dem_contact_model *model, int *mesh_id) {
+  if (!h) return DEM_ERR_ARG;
+  if (ntri <= 0 || ntri > (1 << 24) || !nodes) return fail(h, DEM_ERR_ARG, "bad mesh arguments");
+  if ((int)h->meshes.size() >= dem::kMaxMeshes) return fail(h, DEM_ERR_ARG, "too many meshes");
+  if (h->n > 0) return fail(h, DEM_ERR_STATE, "add meshes before particles");
+  dem::HostMesh hm;
+  std::string err;
+  if (!dem::build_host_mesh(hm, (int)ntri, nodes, h->p.boxlo, h->p.boxhi, err))
+    return fail(h, DEM_ERR_ARG, err);
+  hm.wall_type = wall_type;
+  dem_contact_model m{h->p.rolling_cdt, h->p.tangential_damping, h->p.limit_force,
+                      h->p.torsion_torque};
+  if (model) m = *model;
+  hm.model = model_bits(m);
+  h->meshes.push_back(std::move(hm));
+  h->ntri += h->meshes.back().ntri;
+  if (mesh_id) *mesh_id = (int)h->meshes.size() - 1;
+  return DEM_OK;
+}
+
+int dem_mesh_rotate(dem_handle *h, int mesh_id, const double *origin, const double *axis,
+                    double period) {
+  if (!h) return DEM_ERR_ARG;
+  if (mesh_id < 0 || mesh_id >= (int)h->meshes.size() || !origin || !axis || !(period != 0.0))
+    return fail(h, DEM_ERR_ARG, "bad move/mesh rotate arguments");
+  if (h->n > 0) return fail(h, DEM_ERR_STATE, "set mesh motion before particles");
+  dem::HostMesh &m = h->meshes[mesh_id];
+  // MeshMoverRotate: axis normalized, omega = 2 pi / period (mesh_mover_rotation.cpp:66-93)
+  const double norm = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+  if (norm == 0.) return fail(h, DEM_ERR_ARG, "rotation axis is zero");
+  const double invnorm = 1. / norm;
+  for (int k = 0; k < 3; k++) {
+    m.origin[k] = origin[k];
+    m.axis[k] = axis[k] * invnorm;
+  }
+  m.omega = 2. * kMyPi / period;
+  m.moving = true;
+  return DEM_OK;
+}
+
+int dem_get_mesh_nodes(dem_handle *h, int mesh_id, double *nodes, double *vnodes) {
+  if (!h) return DEM_ERR_ARG;
+  if (mesh_id < 0 || mesh_id >= (int)h->meshes.size()) return fail(h, DEM_ERR_ARG, "bad mesh id");
+  int t0 = 0;
+  for (int m = 0; m < mesh_id; m++) t0 += h->meshes[m].ntri;
+  const int T = h->meshes[mesh_id].ntri;
+  if (!h->d_tri) {  // not uploaded yet: host copy
+    for (int t = 0; t < T; t++)
+      for (int k = 0; k < 9; k++) {
+        if (nodes) nodes[9 * t + k] = h->meshes[mesh_id].node[9 * t + k];
+        if (vnodes) vnodes[9 * t + k] = 0.0;
+      }
+    return DEM_OK;
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  std::vector<dem::TriGeom> g(T);
+  HIPCHK(h, hipMemcpy(g.data(), h->d_tri + t0, sizeof(dem::TriGeom) * T, hipMemcpyDeviceToHost));
+  for (int t = 0; t < T; t++)
+    for (int k = 0; k < 9; k++) {
+      if (nodes) nodes[9 * t + k] = g[t].n[k];
+      if (vnodes) vnodes[9 * t + k] = g[t].vn[k];
+    }
+  return DEM_OK;
+}
+
+int dem_get_mesh_topology(dem_handle *h, int mesh_id, int *edge_active, int *corner_active,
+                          int *obtuse) {
+  if (!h) return DEM_ERR_ARG;
+  if (mesh_id < 0 || mesh_id >= (int)h->meshes.size()) return fail(h, DEM_ERR_ARG, "bad mesh id");
+  const dem::HostMesh &m = h->meshes[mesh_id];
+  for (int t = 0; t < m.ntri; t++) {
+    for (int k = 0; k < 3; k++) {
+      if (edge_active) edge_active[3 * t + k] = m.eact[3 * t + k];
+      if (corner_active) corner_active[3 * t + k] = m.cact[3 * t + k];
+    }
+    if (obtuse) obtuse[t] = m.obtuse[t];
+  }
+  return DEM_OK;
+}
+
+int dem_get_mesh_history(dem_handle *h, int mesh_id, int maxslot, int *tri, double *hist) {
+  if (!h || !tri || !hist || maxslot <= 0) return DEM_ERR_ARG;
+  if (mesh_id < 0 || mesh_id >= (int)h->meshes.size()) return fail(h, DEM_ERR_ARG, "bad mesh id");
+  if (h->n <= 0) return fail(h, DEM_ERR_STATE, "no particles");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const int64_t n = h->n;
+  const size_t cap = (size_t)h->cap;
+  int t0 = 0;
+  for (int m = 0; m < mesh_id; m++) t0 += h->meshes[m].ntri;
+  const dem::AuxBuf &a = h->aux[h->cur_aux];
+  std::vector<int> slot((size_t)dem::kMeshSlots * cap), orig(n);
+  std::vector<char> raw(h->fsz * 3 * dem::kMeshSlots * cap);
+  HIPCHK(h, hipMemcpy(slot.data(), a.mslot + (size_t)mesh_id * dem::kMeshSlots * cap,
+                      sizeof(int) * slot.size(), hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(raw.data(), (char *)a.mhist + h->fsz * 3 * dem::kMeshSlots * cap * mesh_id,
+                      raw.size(), hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(orig.data(), a.orig, sizeof(int) * n, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < n; i++) {
+    const int o = orig[i];
+    for (int k = 0; k < maxslot; k++) {
+      const bool inside = k < dem::kMeshSlots;
+      const int st = inside ? slot[(size_t)k * cap + i] : -1;
+      tri[(size_t)o * maxslot + k] = st >= 0 ? st - t0 : -1;
+      for (int d = 0; d < 3; d++) {
+        double val = 0.0;
+        if (st >= 0) {
+          const size_t idx = (size_t)(k * 3 + d) * cap + i;
+          val = h->mixed ? (double)((float *)raw.data())[idx] : ((double *)raw.data())[idx];
+        }
+        hist[((size_t)o * maxslot + k) * 3 + d] = val;
+      }
+    }
+  }
+  return DEM_OK;
+}
+
 int dem_set_particles(dem_handle *h, int64_t n, const double *x, const double *v,
                       const double *omega, const double *radius, const double *rmass,
                       const int *type, const int *tag) {
@@ -556,6 +804,43 @@ int dem_set_particles(dem_handle *h, int64_t n, const double *x, const double *v
   if ((rc = dalloc(h, (void **)&h->fout, sizeof(double) * 6 * cap))) return rc;
   if ((rc = dalloc(h, (void **)&h->key, sizeof(int) * cap))) return rc;
   if ((rc = dalloc(h, (void **)&h->perm, sizeof(int) * cap))) return rc;
+  const size_t nm = std::max<size_t>(1, h->meshes.size());
+  for (int b = 0; b < 2; b++) {
+    if ((rc = dalloc(h, (void **)&h->aux[b].mslot, sizeof(int) * nm * dem::kMeshSlots * cap)))
+      return rc;
+    if ((rc = dalloc(h, &h->aux[b].mhist, h->fsz * 3 * nm * dem::kMeshSlots * cap))) return rc;
+    HIPCHK(h, hipMemset(h->aux[b].mslot, 0xff, sizeof(int) * nm * dem::kMeshSlots * cap));
+  }
+  if (h->ntri > 0) {
+    std::vector<dem::TriGeom> geo;
+    std::vector<int> off(1, 0), idx;
+    int t0 = 0;
+    for (size_t m = 0; m < h->meshes.size(); m++) {
+      dem::fill_tri_geom(h->meshes[m], (int)m, geo);
+      for (int t = 0; t < h->meshes[m].ntri; t++) {
+        for (int b : h->meshes[m].coplanar[t]) idx.push_back(b + t0);
+        off.push_back((int)idx.size());
+      }
+      t0 += h->meshes[m].ntri;
+    }
+    if (idx.empty()) idx.push_back(-1);
+    const int T = h->ntri;
+    if ((rc = dalloc(h, (void **)&h->d_tri, sizeof(dem::TriGeom) * T))) return rc;
+    if ((rc = dalloc(h, (void **)&h->d_node_last, sizeof(double) * 9 * T))) return rc;
+    if ((rc = dalloc(h, (void **)&h->d_copl_off, sizeof(int) * (T + 1)))) return rc;
+    if ((rc = dalloc(h, (void **)&h->d_copl_idx, sizeof(int) * idx.size()))) return rc;
+    if ((rc = dalloc(h, (void **)&h->d_tcount, sizeof(int) * T))) return rc;
+    if ((rc = dalloc(h, (void **)&h->d_toff, sizeof(int) * (T + 1)))) return rc;
+    if ((rc = dalloc(h, (void **)&h->d_tcell_count, sizeof(int) * h->ncell))) return rc;
+    if ((rc = dalloc(h, (void **)&h->d_tcell_start, sizeof(int) * (h->ncell + 1)))) return rc;
+    if ((rc = dalloc(h, (void **)&h->d_total, sizeof(int)))) return rc;
+    HIPCHK(h, hipMemcpy(h->d_tri, geo.data(), sizeof(dem::TriGeom) * T, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_copl_off, off.data(), sizeof(int) * (T + 1), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_copl_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
+  }
+  if ((rc = dalloc(h, (void **)&h->d_cand, sizeof(int) * dem::kMaxMeshCand * cap))) return rc;
+  if ((rc = dalloc(h, (void **)&h->d_ncand, sizeof(int) * cap))) return rc;
+  HIPCHK(h, hipMemset(h->d_ncand, 0, sizeof(int) * cap));
 
   std::vector<double4> pos(n), vel(n);
   std::vector<int> orig(n), zero(n, 0);
@@ -619,11 +904,15 @@ int run_steps(dem_handle *h, int64_t nsteps) {
   }
   int64_t t = 1;
   int batch = 16;
+  const bool moving = any_moving_mesh(h);
+  int64_t mesh_moved_for = 0;  // step whose mesh rotation the host already applied
   while (t <= nsteps) {
     const int nb = (int)std::min<int64_t>(batch, nsteps - t + 1);
     for (int b = 0; b < nb; b++) {
       const int64_t step = t + b;
       const bool last = step == nsteps;
+      if (moving && step != mesh_moved_for)
+        HIPCHK(h, dem::launch_mesh_move(mesh_move_args(h, step, 1, 1), h->stream));
       dem::StepArgs a = step_args(h, (h->cur_st + b) & 1);
       a.step = step;
       a.shearupdate = 1;
@@ -636,6 +925,8 @@ int run_steps(dem_handle *h, int64_t nsteps) {
     }
     if (h->timing) HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
     if ((rc = read_ctrl(h))) return rc;
+    if (h->ctrl_host->slot_overflow)
+      return fail(h, DEM_ERR_OVERFLOW, "more simultaneous mesh contacts on a particle than history slots");
     const long long stop = h->ctrl_host->stop_at;
     int executed = nb;
     if (stop <= nsteps) executed = (int)(stop - t);
@@ -653,6 +944,12 @@ int run_steps(dem_handle *h, int64_t nsteps) {
     h->ntimestep += executed;
     t += executed;
     if (stop <= nsteps) {
+      // the rebuild step's mesh rotation was skipped by its guard: apply it
+      // now, without the rebuild decision (a rebuild step refreshes the mesh)
+      if (moving) {
+        HIPCHK(h, dem::launch_mesh_move(mesh_move_args(h, stop, 0, 0), h->stream));
+        mesh_moved_for = stop;
+      }
       if ((rc = rebuild(h))) return rc;
       HIPCHK(h, dem::launch_set_stop(h->ctrl, kNever, h->stream));
       batch = std::max(4, batch / 2);

@@ -18,6 +18,11 @@
 //   fix <id> <grp> gravity <g> vector <x> <y> <z>
 //   fix <id> <grp> wall/gran model hertz tangential history [rolling_friction cdt]
 //                  primitive type <t> xplane|yplane|zplane <pos>
+//   fix <id> <grp> mesh/surface file <stl> type <t> [scale s] [move dx dy dz]
+//                  [rotate axis ax ay az angle deg] [precision p] [verbose/heal ...]
+//   fix <id> <grp> wall/gran model hertz tangential history [...] mesh n_meshes <k>
+//                  meshes <id1> ... <idk>
+//   fix <id> <grp> move/mesh mesh <meshid> rotate origin ox oy oz axis ax ay az period T
 //   fix <id> <grp> nve/sphere
 //   thermo <N> | thermo_style custom step atoms ke ... | thermo_modify ...
 //   variable <name> equal|index <value> | print | echo | log
@@ -26,6 +31,7 @@
 // Output-only commands (dump, compute, undump, ...) are accepted and ignored
 // with a warning; anything that would change the physics and is not listed
 // is an error, as the reference reports for an unknown style.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -38,6 +44,7 @@
 #include <vector>
 
 #include "../../include/liggghts_amd.h"
+#include "mesh_host.h"
 
 namespace {
 
@@ -93,6 +100,16 @@ struct Interp {
     dem_contact_model model;
   };
   std::vector<PlaneWall> walls;
+  struct MeshDef {
+    std::vector<double> nodes;
+    int type = 1;
+    bool used = false;
+    dem_contact_model model{};
+    bool rotating = false;
+    double origin[3] = {0, 0, 0}, axis[3] = {0, 0, 1}, period = 1.0;
+  };
+  std::map<std::string, MeshDef> mesh_defs;
+  std::vector<std::string> mesh_order;  // meshes in fix wall/gran order
   int precision = DEM_FP64;
   // thermo
   int thermo_every = 0;
@@ -273,6 +290,55 @@ struct Interp {
     dirty = true;
   }
 
+  void fix_mesh_surface(const std::vector<std::string> &t) {
+    // fix id group mesh/surface file <stl> type <t> [scale s] [move x y z]
+    //     [rotate axis x y z angle a] [precision p] [verbose yes|no] [heal ...]
+    if (t.size() < 6 || t[4] != "file") die("fix mesh/surface: expected 'file <stl>'");
+    MeshDef md;
+    std::string err;
+    if (!dem::read_ascii_stl(t[5], md.nodes, err)) die(err);
+    for (size_t k = 6; k < t.size();) {
+      if (t[k] == "type" && k + 1 < t.size()) {
+        md.type = inum(t[k + 1]);
+        k += 2;
+      } else if (t[k] == "scale" && k + 1 < t.size()) {
+        const double f = num(t[k + 1]);
+        for (double &v : md.nodes) v *= f;
+        k += 2;
+      } else if (t[k] == "move" && k + 3 < t.size()) {
+        const double dv[3] = {num(t[k + 1]), num(t[k + 2]), num(t[k + 3])};
+        for (size_t q = 0; q < md.nodes.size(); q++) md.nodes[q] += dv[q % 3];
+        k += 4;
+      } else if (t[k] == "rotate" && k + 6 < t.size() && t[k + 1] == "axis" && t[k + 5] == "angle") {
+        double ax[3] = {num(t[k + 2]), num(t[k + 3]), num(t[k + 4])};
+        const double ang = num(t[k + 6]) * kMyPi / 180.0;
+        const double nrm = std::sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+        if (nrm == 0.) die("fix mesh/surface: rotation axis is zero");
+        for (double &a : ax) a /= nrm;
+        // Rodrigues rotation about the axis through the origin
+        const double c = std::cos(ang), sn = std::sin(ang);
+        for (size_t q = 0; q + 2 < md.nodes.size(); q += 3) {
+          const double v[3] = {md.nodes[q], md.nodes[q + 1], md.nodes[q + 2]};
+          const double kv = ax[0] * v[0] + ax[1] * v[1] + ax[2] * v[2];
+          const double cr[3] = {ax[1] * v[2] - ax[2] * v[1], ax[2] * v[0] - ax[0] * v[2],
+                                ax[0] * v[1] - ax[1] * v[0]};
+          for (int d = 0; d < 3; d++) md.nodes[q + d] = v[d] * c + cr[d] * sn + ax[d] * kv * (1 - c);
+        }
+        k += 7;
+      } else if ((t[k] == "precision" || t[k] == "verbose" || t[k] == "heal" ||
+                  t[k] == "curvature" || t[k] == "curvature_tolerant") && k + 1 < t.size()) {
+        if (t[k] == "precision" || t[k] == "curvature")
+          fprintf(out, "WARNING: fix mesh/surface %s uses the default value on this path\n",
+                  t[k].c_str());
+        k += 2;
+      } else {
+        die("fix mesh/surface: unsupported keyword " + t[k]);
+      }
+    }
+    mesh_defs[t[1]] = md;
+    dirty = true;
+  }
+
   void parse_model(const std::vector<std::string> &t, size_t k, size_t &end, bool pair,
                    dem_contact_model *wall_model = nullptr) {
     // model hertz tangential history [rolling_friction cdt] [settings...]
@@ -336,6 +402,7 @@ struct Interp {
       die("coefficientRestitution/coefficientFriction need ntypes^2 values");
     bool any_rolling = rolling;
     for (auto &wl : walls) any_rolling = any_rolling || wl.model.rolling_cdt;
+    for (auto &name : mesh_order) any_rolling = any_rolling || mesh_defs[name].model.rolling_cdt;
     if (any_rolling && (int)mur.size() != ntypes * ntypes)
       die("rolling_friction cdt needs coefficientRollingFriction (ntypes^2 values)");
     if (h) {
@@ -368,6 +435,14 @@ struct Interp {
       die(dem_last_error(h));
     for (auto &wl : walls)
       if (dem_add_plane_wall(h, wl.axis, wl.pos, wl.type, &wl.model)) die(dem_last_error(h));
+    for (auto &name : mesh_order) {
+      MeshDef &md = mesh_defs[name];
+      int id = -1;
+      if (dem_add_mesh(h, (int64_t)(md.nodes.size() / 9), md.nodes.data(), md.type, &md.model, &id))
+        die(dem_last_error(h));
+      if (md.rotating && dem_mesh_rotate(h, id, md.origin, md.axis, md.period))
+        die(dem_last_error(h));
+    }
     if (tag.empty()) die("run: no atoms");
     if (dem_set_particles(h, (int64_t)tag.size(), x.data(), v.data(), w.data(), r.data(), m.data(),
                           type.data(), tag.data()))
@@ -550,12 +625,50 @@ struct Interp {
         gravity = true;
         gravity_first = !wall_seen;
         dirty = true;
+      } else if (style == "mesh/surface" || style == "mesh/surface/planar") {
+        fix_mesh_surface(t);
+      } else if (style == "move/mesh") {
+        // fix id group move/mesh mesh <meshid> rotate origin x y z axis x y z period T
+        if (t.size() < 17 || t[4] != "mesh" || t[6] != "rotate" || t[7] != "origin" ||
+            t[11] != "axis" || t[15] != "period")
+          die("fix move/mesh: only 'mesh <id> rotate origin x y z axis x y z period T' is available");
+        auto it = mesh_defs.find(t[5]);
+        if (it == mesh_defs.end()) die("fix move/mesh: unknown mesh " + t[5]);
+        for (int d = 0; d < 3; d++) {
+          it->second.origin[d] = num(t[8 + d]);
+          it->second.axis[d] = num(t[12 + d]);
+        }
+        it->second.period = num(t[16]);
+        it->second.rotating = true;
+        dirty = true;
+      } else if (style == "wall/gran" && t.size() > 4 &&
+                 std::find(t.begin(), t.end(), std::string("mesh")) != t.end() &&
+                 std::find(t.begin(), t.end(), std::string("primitive")) == t.end()) {
+        size_t k;
+        dem_contact_model wm{};
+        parse_model(t, 4, k, false, &wm);
+        // mesh n_meshes <k> meshes <ids...>
+        if (k + 3 >= t.size() || t[k] != "mesh" || t[k + 1] != "n_meshes" || t[k + 3] != "meshes")
+          die("fix wall/gran: expected 'mesh n_meshes <k> meshes <ids>'");
+        const int nm = inum(t[k + 2]);
+        if (nm < 1 || k + 4 + nm > t.size()) die("Illegal fix wall/gran command");
+        for (int q = 0; q < nm; q++) {
+          auto it = mesh_defs.find(t[k + 4 + q]);
+          if (it == mesh_defs.end()) die("fix wall/gran: unknown mesh " + t[k + 4 + q]);
+          if (it->second.used) die("fix wall/gran: mesh " + t[k + 4 + q] + " used twice");
+          it->second.used = true;
+          it->second.model = wm;
+          mesh_order.push_back(t[k + 4 + q]);
+        }
+        if (!gravity) gravity_first = false;
+        wall_seen = true;
+        dirty = true;
       } else if (style == "wall/gran") {
         size_t k;
         dem_contact_model wm{};
         parse_model(t, 4, k, false, &wm);
         if (k >= t.size() || t[k] != "primitive")
-          die("fix wall/gran: only 'primitive ... xplane|yplane|zplane' walls are available");
+          die("fix wall/gran: only 'primitive ... xplane|yplane|zplane' and mesh walls are available");
         if (k + 4 >= t.size() || t[k + 1] != "type") die("Illegal fix wall/gran command");
         const int wt = inum(t[k + 2]);
         const std::string &shape = t[k + 3];

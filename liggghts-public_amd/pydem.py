@@ -22,7 +22,8 @@ EXPORTED = [
     "dem_set_particles", "dem_add_plane_wall", "dem_setup", "dem_run",
     "dem_run_continue", "dem_compute_forces", "dem_get_state", "dem_get_pairs",
     "dem_get_wall_history", "dem_get_stats", "dem_set_kernel_timing",
-    "dem_run_script",
+    "dem_run_script", "dem_add_mesh", "dem_mesh_rotate", "dem_get_mesh_nodes",
+    "dem_get_mesh_topology", "dem_get_mesh_history",
 ]
 
 
@@ -110,6 +111,12 @@ def load_library(path=LIB_PATH):
     lib.dem_get_stats.argtypes = [h, P(DemStats)]
     lib.dem_set_kernel_timing.argtypes = [h, ctypes.c_int]
     lib.dem_run_script.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+    lib.dem_add_mesh.argtypes = [h, ctypes.c_int64, dp, ctypes.c_int, P(DemContactModel),
+                                 P(ctypes.c_int)]
+    lib.dem_mesh_rotate.argtypes = [h, ctypes.c_int, dp, dp, ctypes.c_double]
+    lib.dem_get_mesh_nodes.argtypes = [h, ctypes.c_int, dp, dp]
+    lib.dem_get_mesh_topology.argtypes = [h, ctypes.c_int, ip, ip, ip]
+    lib.dem_get_mesh_history.argtypes = [h, ctypes.c_int, ctypes.c_int, ip, dp]
     _lib = lib
     return lib
 
@@ -130,7 +137,9 @@ class GpuSystem:
     """One granular system on the GPU (a dem_handle)."""
 
     def __init__(self, params, materials, particles, walls=(), precision=DEM_FP64,
-                 max_neighbors=0, device=0):
+                 max_neighbors=0, device=0, meshes=()):
+        """meshes: [(tris (T,3,3), wall_type, model dict or None, rotate or None)],
+        rotate = (origin, axis, period) for fix move/mesh rotate."""
         self.lib = load_library()
         p = DemParams()
         p.dt = params["dt"]
@@ -174,6 +183,19 @@ class GpuSystem:
             self._check(self.lib.dem_add_plane_wall(self.h, int(axis), float(pos), int(wtype),
                                                     ctypes.byref(m)))
             self.nwalls += 1
+        self.ntri = []
+        for (tris, wtype, mdl, rot) in meshes:
+            t = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
+            m = contact_model(mdl if mdl is not None else params)
+            mid = ctypes.c_int(-1)
+            self._check(self.lib.dem_add_mesh(self.h, len(t), _dp(t), int(wtype), ctypes.byref(m),
+                                              ctypes.byref(mid)))
+            if rot is not None:
+                o = np.ascontiguousarray(rot[0], dtype=np.float64)
+                a = np.ascontiguousarray(rot[1], dtype=np.float64)
+                self._check(self.lib.dem_mesh_rotate(self.h, mid.value, _dp(o), _dp(a),
+                                                     float(rot[2])))
+            self.ntri.append(len(t))
         self.set_particles(particles)
 
     def _check(self, rc):
@@ -231,6 +253,26 @@ class GpuSystem:
         if self.nwalls:
             self._check(self.lib.dem_get_wall_history(self.h, _dp(out)))
         return out[:, :self.nwalls, :]
+
+    def mesh_nodes(self, k=0):
+        T = self.ntri[k]
+        nodes, vn = np.zeros((T, 3, 3)), np.zeros((T, 3, 3))
+        self._check(self.lib.dem_get_mesh_nodes(self.h, k, _dp(nodes), _dp(vn)))
+        return nodes, vn
+
+    def mesh_topology(self, k=0):
+        T = self.ntri[k]
+        ea = np.zeros((T, 3), dtype=np.int32)
+        ca = np.zeros((T, 3), dtype=np.int32)
+        ob = np.zeros(T, dtype=np.int32)
+        self._check(self.lib.dem_get_mesh_topology(self.h, k, _ip(ea), _ip(ca), _ip(ob)))
+        return ea, ca, ob
+
+    def mesh_history(self, k=0, maxslot=16):
+        tri = np.zeros((self.n, maxslot), dtype=np.int32)
+        hist = np.zeros((self.n, maxslot, 3))
+        self._check(self.lib.dem_get_mesh_history(self.h, k, maxslot, _ip(tri), _dp(hist)))
+        return tri, hist
 
     def stats(self):
         s = DemStats()

@@ -115,6 +115,63 @@ def dense_pack(seed=0, n=2000, rolling=True, overlap=0.02, box=None, periodic=(1
     return params, mats, particles, [(2, lo[2], 1)]
 
 
+def materials_two(mu_pp=0.5, mu_pw=0.4, restitution=0.3, youngs=5e6, poisson=0.45,
+                  rolling_friction=None):
+    """Type 1 = particles, type 2 = wall (fix mesh/surface ... type 2)."""
+    m = {"ntypes": 2, "youngs": [youngs, youngs], "poisson": [poisson, poisson],
+         "restitution": [[restitution] * 2] * 2,
+         "friction": [[mu_pp, mu_pw], [mu_pw, mu_pw]]}
+    if rolling_friction is not None:
+        m["rolling_friction"] = [[rolling_friction] * 2] * 2
+    return m
+
+
+def drum(seed=0, radius=0.15, length=0.1, n_az=64, n_ax=16, r=0.001, spread=0.1, fill=0.5,
+         spacing=None, period=2.0, dt=5e-6, skin=0.0005, max_particles=None):
+    """Rotating drum with mesh walls: closed cylinder (axis along y) of
+    n_az x n_ax mantle quads split into triangles plus fan end caps, turning at
+    2*pi/period about its axis (fix move/mesh rotate); spheres of radius
+    r*(1 +- spread) on a jittered cubic lattice filling the lower `fill`
+    fraction of the drum volume; mu_pp 0.5, mu_pw 0.4, e 0.3, gravity -z."""
+    import meshes as msh
+    rng = np.random.default_rng(seed)
+    rmax = r * (1.0 + spread)
+    a = spacing or 2.0 * rmax * 1.02
+    margin = rmax + 0.25 * a
+    xs = np.arange(-radius + margin, radius - margin + 1e-12, a)
+    ys = np.arange(-0.5 * length + margin, 0.5 * length - margin + 1e-12, a)
+    X, Y, Z = np.meshgrid(xs, ys, xs, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    inside = pts[:, 0] ** 2 + pts[:, 2] ** 2 < (radius - margin) ** 2
+    pts = pts[inside]
+    # lower fill fraction of the disc area: z below the chord height h with
+    # segment area fraction = fill
+    lo_, hi_ = -radius, radius
+    for _ in range(60):
+        h = 0.5 * (lo_ + hi_)
+        th = 2.0 * math.acos(max(-1.0, min(1.0, -h / radius)))
+        frac = (th - math.sin(th)) / (2.0 * math.pi)
+        lo_, hi_ = (h, hi_) if frac < fill else (lo_, h)
+    pts = pts[pts[:, 2] < h]
+    order = np.lexsort((pts[:, 0], pts[:, 1], pts[:, 2]))
+    pts = pts[order]
+    if max_particles is not None:
+        pts = pts[:max_particles]
+    n = len(pts)
+    x = pts + rng.uniform(-0.05 * (a - 2 * rmax) - 1e-9, 0.05 * (a - 2 * rmax) + 1e-9, pts.shape)
+    rad = r * (1.0 + rng.uniform(-spread, spread, n))
+    box_hi = (radius * 1.05, 0.5 * length * 1.2, radius * 1.05)
+    params = {"dt": dt, "skin": skin, "cdf": 1.0,
+              "boxlo": tuple(-b for b in box_hi), "boxhi": box_hi, "periodic": (0, 0, 0),
+              "grav_acc": (0.0, 0.0, -9.81), "gravity_first": 1, "rolling_cdt": 0}
+    particles = {"x": x, "v": np.zeros((n, 3)), "omega": np.zeros((n, 3)), "radius": rad,
+                 "rmass": sphere_mass(rad, 2500.0), "type": np.ones(n, dtype=np.int32),
+                 "tag": np.arange(1, n + 1, dtype=np.int32)}
+    tris = msh.drum(radius, length, n_az, n_ax)
+    meshes = [(tris, 2, None, ((0.0, 0.0, 0.0), (0.0, 1.0, 0.0), period))]
+    return params, materials_two(), particles, [], meshes
+
+
 def write_data_file(path, params, particles, ntypes=1, density=2500.0):
     """Write a LIGGGHTS granular data file for read_data."""
     x = np.asarray(particles["x"], dtype=np.float64).tolist()

@@ -63,7 +63,10 @@ def test_full_command_subset_parses(tmp_path):
 @pytest.mark.parametrize("bad,msg", [
     ("newton on\n", "newton pair off"),
     ("pair_style gran model hooke tangential history\n", "model hertz"),
-    ("fix w all wall/gran model hertz tangential history mesh n_meshes 1 meshes cad\n", "primitive"),
+    ("fix w all wall/gran model hertz tangential history mesh n_meshes 1 meshes cad\n", "unknown mesh cad"),
+    ("fix w all wall/gran model hertz tangential history primitive type 1 cylinder 0.1 0 0\n", "cylinder is not available"),
+    ("fix cad all mesh/surface file /nonexistent.stl type 1\n", "nonexistent"),
+    ("fix m all move/mesh mesh cad linear 1 0 0\n", "rotate"),
     ("neigh_modify delay 10\n", "delay"),
     ("fix m1 all property/global coefficientRestitution peratomtypepair 2 0.3\n", "n*n"),
     ("atom_style atomic\n", "atom_style"),
@@ -91,3 +94,28 @@ def test_run_without_gpu_fails_cleanly(tmp_path):
     assert res.returncode == 0, res.stderr
     assert int(res.stdout.strip().splitlines()[-1]) != 0
     assert "ERROR" in (tmp_path / "log").read_text()
+
+
+def test_mesh_commands_parse(tmp_path):
+    """fix mesh/surface (with scale/move/rotate), fix wall/gran ... mesh and
+    fix move/mesh ... rotate are accepted; a mesh named twice in wall/gran and
+    an unknown move/mesh target are errors."""
+    import meshes as msh
+    stl = tmp_path / "floor.stl"
+    msh.write_stl(str(stl), msh.grid(0.0, 0.1, 0.0, 0.1, 0.0, 4, 4))
+    head = HEADER.replace("create_box 1 reg", "create_box 2 reg").replace(
+        "peratomtypepair 1 0.3", "peratomtypepair 2 0.3 0.3 0.3 0.3").replace(
+        "peratomtypepair 1 0.5", "peratomtypepair 2 0.5 0.4 0.4 0.4").replace(
+        "peratomtypepair 1 0.1", "peratomtypepair 2 0.1 0.1 0.1 0.1").replace(
+        "peratomtype 5.e6", "peratomtype 5.e6 5.e6").replace("peratomtype 0.45", "peratomtype 0.45 0.45")
+    good = head + (f"fix cad all mesh/surface file {stl} type 2 scale 1.0 move 0 0 0.001 "
+                   "rotate axis 0 0 1 angle 0 precision 1e-8\n"
+                   "fix mw all wall/gran model hertz tangential history mesh n_meshes 1 meshes cad\n"
+                   "fix mv all move/mesh mesh cad rotate origin 0.05 0.05 0 axis 0 0 1 period 1.0\n")
+    rc, log = run_script(tmp_path, good, "in.mesh")
+    assert rc == 0, log
+    rc, log = run_script(tmp_path, good + "fix mw2 all wall/gran model hertz tangential history "
+                         "mesh n_meshes 1 meshes cad\n", "in.mesh2")
+    assert rc != 0 and "used twice" in log
+    rc, log = run_script(tmp_path, good.replace("move/mesh mesh cad", "move/mesh mesh nope"), "in.mesh3")
+    assert rc != 0 and "unknown mesh nope" in log
