@@ -87,6 +87,7 @@ typedef struct dem_stats {
   double step_kernel_ms;   /* average time per fused step-kernel launch (HIP events) */
   int64_t step_kernel_launches;
   double rebuild_ms_total; /* wall time inside neighbor rebuilds */
+  int64_t nmesh_candidates; /* (particle, triangle) candidate entries at the last build */
 } dem_stats;
 
 /* Creates a system. Replaces the construction done by input.cpp:560-650 for

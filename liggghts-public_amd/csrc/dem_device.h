@@ -79,6 +79,7 @@ struct Ctrl {
   int mesh_overflow;     // candidate triangles of a particle beyond kMaxMeshCand
   int slot_overflow;     // simultaneous mesh contacts of a particle beyond kMeshSlots
   int pad2;
+  unsigned long long nmesh_cand;  // (particle, triangle) candidates at the last build
 };
 
 // Particle state read by neighbors: ping-pong buffered across steps.

@@ -1007,6 +1007,7 @@ __global__ void k_reset_build_counters(Ctrl *ctrl) {
     ctrl->maxcount = 0;
     ctrl->npairs_full = 0;
     ctrl->mesh_overflow = 0;
+    ctrl->nmesh_cand = 0;
   }
 }
 
@@ -1126,9 +1127,7 @@ __device__ __forceinline__ void slot_swap(int *slot, T *hist, size_t cap, int x,
 // fix_contact_history_mesh.cpp:374-400), candidate triangles within
 // r*cdf + skin in ascending index, and cleanUpContactJumps (:466-500)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mesh_cand(RebuildArgs a) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.n) return;
+__device__ int mesh_cand_one(const RebuildArgs &a, const int i) {
   const DevParams &p = a.prm;
   const size_t cap = (size_t)a.cap;
   const double4 pi = a.alt.pos[i];
@@ -1192,6 +1191,17 @@ __global__ __launch_bounds__(kBlock) void k_mesh_cand(RebuildArgs a) {
       }
     }
   }
+  return nc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mesh_cand(RebuildArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  unsigned long long nc = 0;
+  if (i < a.n) nc = (unsigned long long)mesh_cand_one<T>(a, i);
+  // candidate total, one atomic per wavefront
+  for (int off = 32; off > 0; off >>= 1) nc += __shfl_xor(nc, off, 64);
+  if ((threadIdx.x & 63) == 0 && nc) atomicAdd(&a.ctrl->nmesh_cand, nc);
 }
 
 // FixMoveMesh::initial_integrate -> MeshMoverRotate::initial_integrate

@@ -105,7 +105,7 @@ struct dem_handle {
 
   // counters
   int64_t ntimestep = 0, nrebuilds = 0;
-  int64_t npairs = 0, ncontacts = 0, nwall_contacts = 0;
+  int64_t npairs = 0, ncontacts = 0, nwall_contacts = 0, nmesh_cand = 0;
   int maxcount = 0;
   bool timing = false;
   double kernel_ms = 0.0;
@@ -447,6 +447,7 @@ int rebuild(dem_handle *h) {
   h->nrebuilds++;
   h->maxcount = h->ctrl_host->maxcount;
   h->npairs = (int64_t)(h->ctrl_host->npairs_full / 2);
+  h->nmesh_cand = (int64_t)h->ctrl_host->nmesh_cand;
   return DEM_OK;
 }
 
@@ -1130,6 +1131,7 @@ int dem_get_stats(dem_handle *h, dem_stats *out) {
   out->ncontacts = h->ncontacts;
   out->nwall_contacts = h->nwall_contacts;
   out->max_neighbors_seen = h->maxcount;
+  out->nmesh_candidates = h->nmesh_cand;
   out->step_kernel_ms = h->kernel_launches ? h->kernel_ms / (double)h->kernel_launches : 0.0;
   out->step_kernel_launches = h->kernel_launches;
   out->rebuild_ms_total = h->rebuild_ms;

@@ -73,6 +73,7 @@ class DemStats(ctypes.Structure):
         ("step_kernel_ms", ctypes.c_double),
         ("step_kernel_launches", ctypes.c_int64),
         ("rebuild_ms_total", ctypes.c_double),
+        ("nmesh_candidates", ctypes.c_int64),
     ]
 
 

@@ -51,7 +51,7 @@ def lattice(n, box_lo, box_hi, spacing, jitter, rng, z0=None):
     return x
 
 
-def config0(seed=0, n=4000):
+def settling_4k(seed=0, n=4000):
     """Plain FP64 Hertz/history sphere settling, CPU-runnable: 4,000 monodisperse
     spheres r=2 mm, density 2500, Y=5e6, nu=0.45, e=0.3, mu=0.5, box 0.1x0.1x0.2
     periodic in x and y, zplane wall at 0, gravity -9.81 z, cubic lattice 4.4 mm
@@ -70,9 +70,9 @@ def config0(seed=0, n=4000):
     return params, materials_single(), particles, [(2, 0.0, 1)]
 
 
-def config1(seed=0, n=1_000_000):
+def settling_1m(seed=0, n=1_000_000):
     """1M polydisperse spheres settling: radii U(1.0, 1.5) mm, materials as
-    config0 plus CDT rolling friction 0.1, box 0.3x0.3x0.6 periodic in x and y,
+    settling_4k plus CDT rolling friction 0.1, box 0.3x0.3x0.6 periodic in x and y,
     zplane floor, jittered 3.3 mm lattice, dt=5e-6, skin 1 mm."""
     rng = np.random.default_rng(seed)
     lo, hi = (0.0, 0.0, 0.0), (0.3, 0.3, 0.6)

@@ -204,7 +204,7 @@ def test_settling_statistics_within_one_percent(precision):
     """Config-0 settling (4,000 spheres) over 3,000 steps: the bed hits the
     floor after ~600 steps, so contacts, walls and rebuilds are all active."""
     say(f"[gpu] settling statistics precision={precision}")
-    params, mats, P, walls = wl.config0(seed=0)
+    params, mats, P, walls = wl.settling_4k(seed=0)
     g, o = build(params, mats, P, walls, precision=precision)
     checkpoints = [1000, 1000, 1000]
     g.run(0)
@@ -230,11 +230,11 @@ def test_script_front_end_matches_api(tmp_path):
     """An unchanged-style LIGGGHTS input script run by dem_run_script gives the
     same kinetic energy as the same system driven through the C ABI."""
     say("[gpu] script front end")
-    params, mats, P, walls = wl.config0(seed=3, n=1000)
+    params, mats, P, walls = wl.settling_4k(seed=3, n=1000)
     data = tmp_path / "c0.data"
     wl.write_data_file(str(data), params, P)
     script = tmp_path / "in.c0"
-    script.write_text(f"""# config-0 style settling
+    script.write_text(f"""# 1,000-sphere settling on a zplane floor
 atom_style granular
 atom_modify map array
 boundary p p f

@@ -52,7 +52,7 @@ def run_script(tmp_path, text, name="in.test"):
 
 
 def test_full_command_subset_parses(tmp_path):
-    params, mats, P, walls = wl.config0(seed=0, n=200)
+    params, mats, P, walls = wl.settling_4k(seed=0, n=200)
     data = tmp_path / "d.data"
     wl.write_data_file(str(data), params, P)
     rc, log = run_script(tmp_path, HEADER + f"read_data {data}\n")
@@ -80,7 +80,7 @@ def test_unsupported_or_illegal_commands_are_errors(tmp_path, bad, msg):
 
 
 def test_run_without_gpu_fails_cleanly(tmp_path):
-    params, mats, P, walls = wl.config0(seed=0, n=100)
+    params, mats, P, walls = wl.settling_4k(seed=0, n=100)
     data = tmp_path / "d.data"
     wl.write_data_file(str(data), params, P)
     script = tmp_path / "in.run"
