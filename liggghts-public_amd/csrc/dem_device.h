@@ -113,6 +113,24 @@ struct ListBuf {
   void *hist;     // [W][3][cap] tangential history, double or float
 };
 
+// Rigid rotation of the moving meshes for one step.
+struct MeshMoveArgs {
+  TriGeom *tri;               // geometry of the step before
+  TriGeom *dst;               // where the moved geometry goes (nullptr: in place)
+  const double *node_last;
+  int ntri;
+  int moving_mask;            // bit m: mesh m rotates
+  double dQ[kMaxMeshes][4];   // incremental rotation quaternion
+  double origin[kMaxMeshes][3];
+  double omegaVec[kMaxMeshes][3];
+  int trans[kMaxMeshes];
+  double triggersq;
+  long long step;             // the step the moved geometry belongs to
+  int guard;                  // return if step >= stop_at
+  int decide;                 // raise stop_at = step+1 when a node moved > skin/2
+  Ctrl *ctrl;
+};
+
 struct StepArgs {
   DevParams prm;
   MatTables mat;
@@ -127,6 +145,10 @@ struct StepArgs {
   Ctrl *ctrl;
   long long step;
   int shearupdate, do_final, do_initial, store_f, count;
+  // with moving meshes, blocks [pblocks, gridDim) rotate the meshes into the
+  // next step's geometry buffer (mv.dst) while the others compute this step
+  int pblocks, move_mesh;
+  MeshMoveArgs mv;
 };
 
 struct RebuildArgs {
@@ -148,7 +170,8 @@ struct RebuildArgs {
   Ctrl *ctrl;
   // meshes
   int ntri;                    // all meshes
-  TriGeom *tri;
+  TriGeom *tri;                // geometry of the rebuild step
+  TriGeom *tri_alt;            // the other geometry buffer (nullptr if none)
   double *node_last;           // [ntri][9] nodes at the last rebuild
   int *tcount, *toff;          // per triangle: cells touched, offsets
   int *tpair_cell, *tpair_tri; // triangle-cell pairs
@@ -156,23 +179,6 @@ struct RebuildArgs {
   int *tcell_count, *tcell_start, *tcell_list;
   int *cand, *ncand;
   int zero_vnode;              // setup: node velocities start at zero
-};
-
-// Rigid rotation of the moving meshes for one step.
-struct MeshMoveArgs {
-  TriGeom *tri;
-  const double *node_last;
-  int ntri;
-  int moving_mask;            // bit m: mesh m rotates
-  double dQ[kMaxMeshes][4];   // incremental rotation quaternion
-  double origin[kMaxMeshes][3];
-  double omegaVec[kMaxMeshes][3];
-  int trans[kMaxMeshes];
-  double triggersq;
-  long long step;
-  int guard;                  // return if step >= stop_at
-  int decide;                 // raise stop_at = step+1 when a node moved > skin/2
-  Ctrl *ctrl;
 };
 
 // launchers (dem_kernels.hip)

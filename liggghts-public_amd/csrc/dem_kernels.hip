@@ -385,16 +385,93 @@ __device__ inline void mesh_contacts(const StepArgs &a, int i, const double4 &pi
 }
 
 // ---------------------------------------------------------------------------
-// fused step kernel: pair forces over the full list, plane walls, gravity,
-// nve/sphere final integrate of this step and initial integrate of the next,
-// and the skin/2 displacement check of Neighbor::decide for the next step.
-// One thread per particle; reads the state buffer `in`, writes `out`.
+// fix move/mesh rotate: MeshMoverRotate::initial_integrate
+// (mesh_mover_rotation.cpp:110-139), MultiNodeMesh::rotate
+// (multi_node_mesh_I.h:620-660) and MultiNodeMesh::decideRebuild
+// (multi_node_mesh_I.h:792-825)
+// ---------------------------------------------------------------------------
+// moves triangle t of a moving mesh one step; returns whether one of its
+// nodes is now more than skin/2 from where it was at the last rebuild
+__device__ __forceinline__ bool mesh_move_tri(const MeshMoveArgs &a, const int t) {
+  const TriGeom &src = a.tri[t];
+  const int m = src.mesh;
+  if (!((a.moving_mask >> m) & 1)) return false;
+  TriGeom &dst = a.dst ? a.dst[t] : a.tri[t];
+  const double *dQ = a.dQ[m];
+  const double *o = a.origin[m];
+  double c[3] = {0., 0., 0.};
+  double nd[9];
+  for (int j = 0; j < 3; j++) {
+    double v[3] = {src.n[3 * j], src.n[3 * j + 1], src.n[3 * j + 2]};
+    if (a.trans[m]) d3_sub(v, o, v);
+    vec_quat_rotate(v, dQ);
+    if (a.trans[m])
+      for (int k = 0; k < 3; k++) v[k] = v[k] + o[k];
+    for (int k = 0; k < 3; k++) {
+      nd[3 * j + k] = v[k];
+      c[k] = v[k] + c[k];
+    }
+  }
+  d3_div(c, 3.0);
+  double ev[9], sn[3] = {src.sn[0], src.sn[1], src.sn[2]};
+  for (int j = 0; j < 3; j++) {
+    double v[3] = {src.ev[3 * j], src.ev[3 * j + 1], src.ev[3 * j + 2]};
+    vec_quat_rotate(v, dQ);
+    for (int k = 0; k < 3; k++) ev[3 * j + k] = v[k];
+  }
+  vec_quat_rotate(sn, dQ);
+  if (a.dst) {  // fields the rotation leaves unchanged
+    for (int k = 0; k < 3; k++) dst.el[k] = src.el[k];
+    dst.rb = src.rb;
+    dst.obtuse = src.obtuse;
+    dst.flags = src.flags;
+    dst.mesh = src.mesh;
+    dst.pad = src.pad;
+  }
+  for (int k = 0; k < 9; k++) dst.n[k] = nd[k];
+  for (int k = 0; k < 3; k++) dst.c[k] = c[k];
+  for (int k = 0; k < 9; k++) dst.ev[k] = ev[k];
+  for (int k = 0; k < 3; k++) dst.sn[k] = sn[k];
+  for (int j = 0; j < 3; j++) {
+    double rPA[3], vRot[3];
+    d3_sub(&nd[3 * j], o, rPA);
+    d3_cross(a.omegaVec[m], rPA, vRot);
+    for (int k = 0; k < 3; k++) dst.vn[3 * j + k] = 0.0 + vRot[k];
+  }
+  bool moved = false;
+  if (a.decide)
+    for (int j = 0; j < 3; j++) {
+      const double dx = nd[3 * j] - a.node_last[9 * t + 3 * j];
+      const double dy = nd[3 * j + 1] - a.node_last[9 * t + 3 * j + 1];
+      const double dz = nd[3 * j + 2] - a.node_last[9 * t + 3 * j + 2];
+      if (dx * dx + dy * dy + dz * dz > a.triggersq) moved = true;
+    }
+  return moved;
+}
+
+__device__ __forceinline__ void mesh_move_block(const MeshMoveArgs &a, const int t) {
+  const bool moved = t < a.ntri && mesh_move_tri(a, t);
+  if (__ballot(moved) != 0ull && (threadIdx.x & 63) == 0)
+    atomicMin((unsigned long long *)&a.ctrl->stop_at, (unsigned long long)(a.step + 1));
+}
+
+// ---------------------------------------------------------------------------
+// fused step kernel: pair forces over the full list, plane walls, meshes,
+// gravity, nve/sphere final integrate of this step and initial integrate of
+// the next, and the skin/2 displacement check of Neighbor::decide for the
+// next step. One thread per particle; reads the state buffer `in`, writes
+// `out`. With moving meshes, trailing blocks rotate the meshes into the next
+// step's geometry buffer.
 // ---------------------------------------------------------------------------
 template <typename T, bool MESH>
 __global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
   typedef typename OmgT<T>::type omg_t;
   if (a.step >= a.ctrl->stop_at) return;
-  const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
+  if (MESH && (int)blockIdx.x >= a.pblocks) {
+    mesh_move_block(a.mv, ((int)blockIdx.x - a.pblocks) * kBlock + threadIdx.x);
+    return;
+  }
+  const int i = xcd_block(a.pblocks) * kBlock + threadIdx.x;
   const bool active = i < a.n;
   const DevParams &p = a.prm;
   unsigned long long my_contacts = 0, my_wall = 0;
@@ -1026,6 +1103,8 @@ __global__ __launch_bounds__(kBlock) void k_mesh_geom(RebuildArgs a) {
   for (int k = 0; k < 9; k++) a.node_last[9 * t + k] = tg.n[k];
   if (a.zero_vnode)
     for (int k = 0; k < 9; k++) tg.vn[k] = 0.0;
+  // the other buffer holds the same geometry until a step moves it
+  if (a.tri_alt) a.tri_alt[t] = tg;
 }
 
 // cells a triangle can reach: centre within (r_max*cdf + skin) + half a cell
@@ -1057,44 +1136,69 @@ __device__ __forceinline__ void cell_center(const DevParams &p, int cx, int cy, 
   c[2] = p.lo[2] + (cz + 0.5) * (p.L[2] / p.nc[2]);
 }
 
+// Triangle -> cell binning, one wavefront per triangle: the lanes stride over
+// the cells of the triangle's grown bounding box and keep those whose centre
+// lies within reach (+ half the cell diagonal) of the triangle.
+constexpr int kWavesPerBlock = kBlock / 64;
+
+__device__ __forceinline__ bool tri_cell_hit(const RebuildArgs &a, const TriGeom &tg,
+                                             const int lo[3], const int nx, const int ny,
+                                             const double reach2, const int k, int &cell) {
+  const int cx = lo[0] + k % nx;
+  const int cy = lo[1] + (k / nx) % ny;
+  const int cz = lo[2] + k / (nx * ny);
+  double cc[3];
+  cell_center(a.prm, cx, cy, cz, cc);
+  cell = cx + a.prm.nc[0] * (cy + a.prm.nc[1] * cz);
+  return tri_point_dist2(&tg.n[0], &tg.n[3], &tg.n[6], cc) <= reach2;
+}
+
 __global__ __launch_bounds__(kBlock) void k_tri_count(RebuildArgs a) {
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= a.ntri) return;
+  const int t = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (t >= a.ntri) return;  // uniform per wavefront
   const TriGeom &tg = a.tri[t];
   int lo[3], hi[3];
   double reach2;
   tri_cell_range(a, tg, lo, hi, reach2);
+  const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+  const int ncells = nx * ny * nz;
   int cnt = 0;
-  for (int cz = lo[2]; cz <= hi[2]; cz++)
-    for (int cy = lo[1]; cy <= hi[1]; cy++)
-      for (int cx = lo[0]; cx <= hi[0]; cx++) {
-        double cc[3];
-        cell_center(a.prm, cx, cy, cz, cc);
-        if (tri_point_dist2(&tg.n[0], &tg.n[3], &tg.n[6], cc) <= reach2) cnt++;
-      }
-  a.tcount[t] = cnt;
+  for (int k = lane; k < ncells; k += 64) {
+    int cell;
+    cnt += tri_cell_hit(a, tg, lo, nx, ny, reach2, k, cell) ? 1 : 0;
+  }
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+  if (lane == 0) a.tcount[t] = cnt;
 }
 
 __global__ __launch_bounds__(kBlock) void k_tri_fill(RebuildArgs a) {
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= a.ntri) return;
+  const int t = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (t >= a.ntri) return;  // uniform per wavefront
   const TriGeom &tg = a.tri[t];
   int lo[3], hi[3];
   double reach2;
   tri_cell_range(a, tg, lo, hi, reach2);
+  const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+  const int ncells = nx * ny * nz;
   int w = a.toff[t];
   const int wend = w + a.tcount[t];
-  for (int cz = lo[2]; cz <= hi[2]; cz++)
-    for (int cy = lo[1]; cy <= hi[1]; cy++)
-      for (int cx = lo[0]; cx <= hi[0]; cx++) {
-        double cc[3];
-        cell_center(a.prm, cx, cy, cz, cc);
-        if (tri_point_dist2(&tg.n[0], &tg.n[3], &tg.n[6], cc) <= reach2 && w < wend) {
-          a.tpair_cell[w] = a.cell_rank[cx + a.prm.nc[0] * (cy + a.prm.nc[1] * cz)];
-          a.tpair_tri[w] = t;
-          w++;
-        }
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int k0 = 0; k0 < ncells; k0 += 64) {  // uniform trip count
+    const int k = k0 + lane;
+    int cell = 0;
+    const bool hit = k < ncells && tri_cell_hit(a, tg, lo, nx, ny, reach2, k, cell);
+    const unsigned long long m = __ballot(hit);
+    if (hit) {
+      const int pos = w + __popcll(m & below);
+      if (pos < wend) {
+        a.tpair_cell[pos] = a.cell_rank[cell];
+        a.tpair_tri[pos] = t;
       }
+    }
+    w += __popcll(m);
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_tcell_hist(RebuildArgs a, int npairs) {
@@ -1204,60 +1308,10 @@ __global__ __launch_bounds__(kBlock) void k_mesh_cand(RebuildArgs a) {
   if ((threadIdx.x & 63) == 0 && nc) atomicAdd(&a.ctrl->nmesh_cand, nc);
 }
 
-// FixMoveMesh::initial_integrate -> MeshMoverRotate::initial_integrate
-// (mesh_mover_rotation.cpp:110-139), MultiNodeMesh::rotate
-// (multi_node_mesh_I.h:620-660), and MultiNodeMesh::decideRebuild
-// (multi_node_mesh_I.h:792-825) for the step
+// the first step of a run: meshes rotated in place
 __global__ __launch_bounds__(kBlock) void k_mesh_move(MeshMoveArgs a) {
   if (a.guard && a.step >= a.ctrl->stop_at) return;
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  bool moved = false;
-  if (t < a.ntri) {
-    TriGeom &tg = a.tri[t];
-    const int m = tg.mesh;
-    if ((a.moving_mask >> m) & 1) {
-      const double *dQ = a.dQ[m];
-      const double *o = a.origin[m];
-      double c[3] = {0., 0., 0.};
-      for (int j = 0; j < 3; j++) {
-        double *nd = &tg.n[3 * j];
-        double v[3] = {nd[0], nd[1], nd[2]};
-        if (a.trans[m]) d3_sub(v, o, v);
-        vec_quat_rotate(v, dQ);
-        if (a.trans[m])
-          for (int k = 0; k < 3; k++) v[k] = v[k] + o[k];
-        for (int k = 0; k < 3; k++) {
-          nd[k] = v[k];
-          c[k] = v[k] + c[k];
-        }
-      }
-      d3_div(c, 3.0);
-      for (int k = 0; k < 3; k++) tg.c[k] = c[k];
-      for (int j = 0; j < 3; j++) {
-        double v[3] = {tg.ev[3 * j], tg.ev[3 * j + 1], tg.ev[3 * j + 2]};
-        vec_quat_rotate(v, dQ);
-        for (int k = 0; k < 3; k++) tg.ev[3 * j + k] = v[k];
-      }
-      double sn[3] = {tg.sn[0], tg.sn[1], tg.sn[2]};
-      vec_quat_rotate(sn, dQ);
-      for (int k = 0; k < 3; k++) tg.sn[k] = sn[k];
-      for (int j = 0; j < 3; j++) {
-        double rPA[3], vRot[3];
-        d3_sub(&tg.n[3 * j], o, rPA);
-        d3_cross(a.omegaVec[m], rPA, vRot);
-        for (int k = 0; k < 3; k++) tg.vn[3 * j + k] = 0.0 + vRot[k];
-      }
-      if (a.decide)
-        for (int j = 0; j < 3; j++) {
-          const double dx = tg.n[3 * j] - a.node_last[9 * t + 3 * j];
-          const double dy = tg.n[3 * j + 1] - a.node_last[9 * t + 3 * j + 1];
-          const double dz = tg.n[3 * j + 2] - a.node_last[9 * t + 3 * j + 2];
-          if (dx * dx + dy * dy + dz * dz > a.triggersq) moved = true;
-        }
-    }
-  }
-  if (__ballot(moved) != 0ull && (threadIdx.x & 63) == 0)
-    atomicMin((unsigned long long *)&a.ctrl->stop_at, (unsigned long long)(a.step + 1));
+  mesh_move_block(a, blockIdx.x * kBlock + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1295,8 +1349,10 @@ hipError_t launch_rebuild(const RebuildArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_step(const StepArgs &a, hipStream_t s) {
-  const int nb = nblk(a.n > 0 ? a.n : 1, kBlock);
   const bool mesh = a.prm.nmesh > 0;
+  // pblocks = nblk(max(n, 1), kBlock), set by the caller; trailing blocks move
+  // the meshes when asked to
+  const int nb = a.pblocks + ((mesh && a.move_mesh) ? nblk(a.mv.ntri, kBlock) : 0);
   if (a.mixed) {
     if (mesh)
       hipLaunchKernelGGL((k_step<float, true>), dim3(nb), dim3(kBlock), 0, s, a);
@@ -1332,7 +1388,7 @@ hipError_t launch_mesh_bin_count(const RebuildArgs &a, int *total, hipStream_t s
   if (a.ntri <= 0) return hipSuccess;
   const int nt = nblk(a.ntri, kBlock);
   hipLaunchKernelGGL(k_mesh_geom, dim3(nt), dim3(kBlock), 0, s, a);
-  hipLaunchKernelGGL(k_tri_count, dim3(nt), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_tri_count, dim3(nblk(a.ntri, kWavesPerBlock)), dim3(kBlock), 0, s, a);
   // toff has ntri+1 entries; the scan writes the total at toff[ntri]
   launch_scan(a.tcount, a.toff, a.scan_tmp, a.ntri, s);
   const hipError_t e = hipMemcpyAsync(total, a.toff + a.ntri, sizeof(int),
@@ -1343,9 +1399,8 @@ hipError_t launch_mesh_bin_count(const RebuildArgs &a, int *total, hipStream_t s
 
 hipError_t launch_mesh_bin_fill(const RebuildArgs &a, hipStream_t s) {
   if (a.ntri <= 0) return hipSuccess;
-  const int nt = nblk(a.ntri, kBlock);
   const int ncb = nblk(a.ncell, kBlock);
-  hipLaunchKernelGGL(k_tri_fill, dim3(nt), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_tri_fill, dim3(nblk(a.ntri, kWavesPerBlock)), dim3(kBlock), 0, s, a);
   const int np = a.tpair_cap;  // exactly the pair total (sized by the caller)
   hipLaunchKernelGGL(k_zero, dim3(ncb), dim3(kBlock), 0, s, a.tcell_count, a.ncell);
   if (np > 0) hipLaunchKernelGGL(k_tcell_hist, dim3(nblk(np, kBlock)), dim3(kBlock), 0, s, a, np);

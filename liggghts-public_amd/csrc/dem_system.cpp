@@ -67,7 +67,13 @@ struct dem_handle {
   // triangle meshes (host copies; device copies live with the particles)
   std::vector<dem::HostMesh> meshes;
   int ntri = 0;
-  dem::TriGeom *d_tri = nullptr;
+  // Triangle geometry, two buffers when a mesh moves: a step reads one and
+  // its trailing blocks write the next step's geometry into the other.
+  // tri_cur holds the geometry of the current state; without moving meshes
+  // both entries point to one buffer.
+  dem::TriGeom *d_tri_buf[2] = {nullptr, nullptr};
+  int tri_cur = 0;
+  bool pending_mesh_rebuild = false;  // a moving mesh asked for a rebuild at the next step
   double *d_node_last = nullptr;
   int *d_copl_off = nullptr, *d_copl_idx = nullptr;
   int *d_tcount = nullptr, *d_toff = nullptr;
@@ -160,7 +166,7 @@ void free_particles(dem_handle *h) {
     h->lst[b] = ListBuf{};
   }
   h->xhold = h->fout = nullptr;
-  h->d_tri = nullptr;
+  h->d_tri_buf[0] = h->d_tri_buf[1] = nullptr;
   h->d_node_last = nullptr;
   h->d_copl_off = h->d_copl_idx = nullptr;
   h->d_tcount = h->d_toff = h->d_tpair_cell = h->d_tpair_tri = h->d_tcell_list = nullptr;
@@ -298,7 +304,8 @@ dem::RebuildArgs rebuild_args(dem_handle *h) {
   a.scan_tmp = h->scan_tmp;
   a.ctrl = h->ctrl;
   a.ntri = h->ntri;
-  a.tri = h->d_tri;
+  a.tri = h->d_tri_buf[h->tri_cur];
+  a.tri_alt = h->d_tri_buf[1 - h->tri_cur] != a.tri ? h->d_tri_buf[1 - h->tri_cur] : nullptr;
   a.node_last = h->d_node_last;
   a.tcount = h->d_tcount;
   a.toff = h->d_toff;
@@ -314,7 +321,11 @@ dem::RebuildArgs rebuild_args(dem_handle *h) {
   return a;
 }
 
-dem::StepArgs step_args(dem_handle *h, int in_buf) {
+dem::MeshMoveArgs mesh_move_args(const dem_handle *h, long long step, int guard, int decide);
+
+// in_buf: particle state buffer read by the step; tri_buf: geometry buffer of
+// the step (the next step's geometry goes to the other one)
+dem::StepArgs step_args(dem_handle *h, int in_buf, int tri_buf) {
   dem::StepArgs a{};
   a.prm = make_devparams(h);
   a.mat = h->mat;
@@ -326,7 +337,7 @@ dem::StepArgs step_args(dem_handle *h, int in_buf) {
   a.out = h->st[1 - in_buf];
   a.aux = h->aux[h->cur_aux];
   a.list = h->lst[h->cur_l];
-  a.mesh.tri = h->d_tri;
+  a.mesh.tri = h->d_tri_buf[tri_buf];
   a.mesh.copl_off = h->d_copl_off;
   a.mesh.copl_idx = h->d_copl_idx;
   a.mesh.cand = h->d_cand;
@@ -334,6 +345,13 @@ dem::StepArgs step_args(dem_handle *h, int in_buf) {
   a.xhold = h->xhold;
   a.fout = h->fout;
   a.ctrl = h->ctrl;
+  a.pblocks = (int)((std::max<int64_t>(h->n, 1) + dem::kBlock - 1) / dem::kBlock);
+  a.move_mesh = 0;
+  if (h->ntri > 0) {
+    a.mv = mesh_move_args(h, 0, 0, 1);
+    a.mv.tri = h->d_tri_buf[tri_buf];
+    a.mv.dst = h->d_tri_buf[1 - tri_buf];
+  }
   return a;
 }
 
@@ -346,7 +364,8 @@ bool any_moving_mesh(const dem_handle *h) {
 // one step of fix move/mesh rotate for every moving mesh
 dem::MeshMoveArgs mesh_move_args(const dem_handle *h, long long step, int guard, int decide) {
   dem::MeshMoveArgs a{};
-  a.tri = h->d_tri;
+  a.tri = h->d_tri_buf[h->tri_cur];
+  a.dst = nullptr;
   a.node_last = h->d_node_last;
   a.ntri = h->ntri;
   a.triggersq = 0.25 * h->p.skin * h->p.skin;
@@ -453,7 +472,7 @@ int rebuild(dem_handle *h) {
 
 int force_pass(dem_handle *h, int shearupdate) {
   HIPCHK(h, dem::launch_set_stop(h->ctrl, kNever, h->stream));
-  dem::StepArgs a = step_args(h, h->cur_st);
+  dem::StepArgs a = step_args(h, h->cur_st, h->tri_cur);
   a.step = 0;
   a.shearupdate = shearupdate;
   a.do_final = 0;
@@ -692,7 +711,7 @@ int dem_get_mesh_nodes(dem_handle *h, int mesh_id, double *nodes, double *vnodes
   int t0 = 0;
   for (int m = 0; m < mesh_id; m++) t0 += h->meshes[m].ntri;
   const int T = h->meshes[mesh_id].ntri;
-  if (!h->d_tri) {  // not uploaded yet: host copy
+  if (!h->d_tri_buf[0]) {  // not uploaded yet: host copy
     for (int t = 0; t < T; t++)
       for (int k = 0; k < 9; k++) {
         if (nodes) nodes[9 * t + k] = h->meshes[mesh_id].node[9 * t + k];
@@ -702,7 +721,8 @@ int dem_get_mesh_nodes(dem_handle *h, int mesh_id, double *nodes, double *vnodes
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   std::vector<dem::TriGeom> g(T);
-  HIPCHK(h, hipMemcpy(g.data(), h->d_tri + t0, sizeof(dem::TriGeom) * T, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(g.data(), h->d_tri_buf[h->tri_cur] + t0, sizeof(dem::TriGeom) * T,
+                      hipMemcpyDeviceToHost));
   for (int t = 0; t < T; t++)
     for (int k = 0; k < 9; k++) {
       if (nodes) nodes[9 * t + k] = g[t].n[k];
@@ -826,7 +846,12 @@ int dem_set_particles(dem_handle *h, int64_t n, const double *x, const double *v
     }
     if (idx.empty()) idx.push_back(-1);
     const int T = h->ntri;
-    if ((rc = dalloc(h, (void **)&h->d_tri, sizeof(dem::TriGeom) * T))) return rc;
+    if ((rc = dalloc(h, (void **)&h->d_tri_buf[0], sizeof(dem::TriGeom) * T))) return rc;
+    h->d_tri_buf[1] = h->d_tri_buf[0];
+    if (any_moving_mesh(h) &&
+        (rc = dalloc(h, (void **)&h->d_tri_buf[1], sizeof(dem::TriGeom) * T)))
+      return rc;
+    h->tri_cur = 0;
     if ((rc = dalloc(h, (void **)&h->d_node_last, sizeof(double) * 9 * T))) return rc;
     if ((rc = dalloc(h, (void **)&h->d_copl_off, sizeof(int) * (T + 1)))) return rc;
     if ((rc = dalloc(h, (void **)&h->d_copl_idx, sizeof(int) * idx.size()))) return rc;
@@ -835,7 +860,9 @@ int dem_set_particles(dem_handle *h, int64_t n, const double *x, const double *v
     if ((rc = dalloc(h, (void **)&h->d_tcell_count, sizeof(int) * h->ncell))) return rc;
     if ((rc = dalloc(h, (void **)&h->d_tcell_start, sizeof(int) * (h->ncell + 1)))) return rc;
     if ((rc = dalloc(h, (void **)&h->d_total, sizeof(int)))) return rc;
-    HIPCHK(h, hipMemcpy(h->d_tri, geo.data(), sizeof(dem::TriGeom) * T, hipMemcpyHostToDevice));
+    for (int b = 0; b < 2; b++)
+      HIPCHK(h, hipMemcpy(h->d_tri_buf[b], geo.data(), sizeof(dem::TriGeom) * T,
+                          hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_copl_off, off.data(), sizeof(int) * (T + 1), hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_copl_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
   }
@@ -898,59 +925,68 @@ int run_steps(dem_handle *h, int64_t nsteps) {
   if (nsteps == 0) return DEM_OK;
   h->setup_done = false;
   h->can_continue = false;
-  HIPCHK(h, dem::launch_set_stop(h->ctrl, kNever, h->stream));
+  // a rebuild a moving mesh asked for at the first step of this run
+  // (FixMesh::pre_force set next_reneighbor on the last step of the previous run)
+  HIPCHK(h, dem::launch_set_stop(h->ctrl, h->pending_mesh_rebuild ? 1 : kNever, h->stream));
+  h->pending_mesh_rebuild = false;
+  const bool moving = any_moving_mesh(h);
   {
-    dem::StepArgs a = step_args(h, h->cur_st);
+    dem::StepArgs a = step_args(h, h->cur_st, h->tri_cur);
     HIPCHK(h, dem::launch_initial(a, h->stream));
   }
+  // the first step's mesh rotation, in place; every later step's rotation is
+  // done by the trailing blocks of the step before
+  if (moving) HIPCHK(h, dem::launch_mesh_move(mesh_move_args(h, 1, 0, 1), h->stream));
+  int tc = h->tri_cur;  // geometry buffer of step t
   int64_t t = 1;
   int batch = 16;
-  const bool moving = any_moving_mesh(h);
-  int64_t mesh_moved_for = 0;  // step whose mesh rotation the host already applied
   while (t <= nsteps) {
     const int nb = (int)std::min<int64_t>(batch, nsteps - t + 1);
     for (int b = 0; b < nb; b++) {
       const int64_t step = t + b;
       const bool last = step == nsteps;
-      if (moving && step != mesh_moved_for)
-        HIPCHK(h, dem::launch_mesh_move(mesh_move_args(h, step, 1, 1), h->stream));
-      dem::StepArgs a = step_args(h, (h->cur_st + b) & 1);
+      dem::StepArgs a = step_args(h, (h->cur_st + b) & 1, (tc + b) & 1);
       a.step = step;
       a.shearupdate = 1;
       a.do_final = 1;
       a.do_initial = last ? 0 : 1;
       a.store_f = last ? 1 : 0;
       a.count = last ? 1 : 0;
-      if (h->timing && b == 0) HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
+      a.move_mesh = (moving && !last) ? 1 : 0;
+      a.mv.step = step + 1;
       HIPCHK(h, dem::launch_step(a, h->stream));
+      // the start event goes after the batch's first launch: the GPU is then
+      // busy while the host enqueues the rest, so the pair times the kernels
+      // and not the latency of the first launch
+      if (h->timing && b == 0) HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
     }
     if (h->timing) HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
     if ((rc = read_ctrl(h))) return rc;
     if (h->ctrl_host->slot_overflow)
       return fail(h, DEM_ERR_OVERFLOW, "more simultaneous mesh contacts on a particle than history slots");
     const long long stop = h->ctrl_host->stop_at;
-    int executed = nb;
-    if (stop <= nsteps) executed = (int)(stop - t);
+    // a stop beyond this batch (a mesh asks for a rebuild two steps after the
+    // step that moved it) stays in the control block for the next batch
+    const bool rebuild_now = stop <= nsteps && stop - t <= nb;
+    const int executed = rebuild_now ? (int)(stop - t) : nb;
     if (executed < 0 || executed > nb) return fail(h, DEM_ERR_STATE, "internal step accounting error");
-    // Timing brackets the whole batch with one event pair (events between
-    // launches would add their own gaps); batches that ended in a rebuild
-    // request contain skipped launches and are left out.
-    if (h->timing && executed == nb) {
+    // Timing brackets launches 1..nb-1 of the batch with one event pair
+    // (events between launches would add their own gaps); batches that ended
+    // in a rebuild request contain skipped launches and are left out.
+    if (h->timing && executed == nb && nb > 1) {
       float ms = 0.f;
       HIPCHK(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
       h->kernel_ms += ms;
-      h->kernel_launches += nb;
+      h->kernel_launches += nb - 1;
     }
+    const bool ran_last = t + executed - 1 == nsteps;  // the last step moves no mesh
     h->cur_st = (h->cur_st + executed) & 1;
     h->ntimestep += executed;
+    tc = (tc + executed - (ran_last && moving ? 1 : 0)) & 1;
+    if (!moving) tc = h->tri_cur;
     t += executed;
-    if (stop <= nsteps) {
-      // the rebuild step's mesh rotation was skipped by its guard: apply it
-      // now, without the rebuild decision (a rebuild step refreshes the mesh)
-      if (moving) {
-        HIPCHK(h, dem::launch_mesh_move(mesh_move_args(h, stop, 0, 0), h->stream));
-        mesh_moved_for = stop;
-      }
+    if (rebuild_now) {
+      h->tri_cur = tc;
       if ((rc = rebuild(h))) return rc;
       HIPCHK(h, dem::launch_set_stop(h->ctrl, kNever, h->stream));
       batch = std::max(4, batch / 2);
@@ -958,6 +994,8 @@ int run_steps(dem_handle *h, int64_t nsteps) {
       batch = std::min(256, batch * 2);
     }
   }
+  h->tri_cur = tc;
+  h->pending_mesh_rebuild = h->ctrl_host->stop_at == nsteps + 1;
   h->ncontacts = (int64_t)(h->ctrl_host->ncontact_full / 2);
   h->nwall_contacts = (int64_t)h->ctrl_host->nwall;
   h->can_continue = true;
