@@ -67,6 +67,9 @@ typedef struct dem_params {
   int precision;      /* enum dem_precision */
   int max_neighbors;  /* 0 selects 32 */
   int device;         /* HIP device ordinal */
+  int neigh_every;    /* neigh_modify every N (neighbor.cpp:1907-1911); 0 selects 1 */
+  int neigh_no_check; /* neigh_modify check no (neighbor.cpp:1922-1927): build every
+                         `neigh_every` steps without the skin/2 displacement test */
 } dem_params;
 
 /* Switches of one contact model (pair_style gran / fix wall/gran keywords). */

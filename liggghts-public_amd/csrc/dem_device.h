@@ -38,6 +38,8 @@ struct TriGeom {
 struct DevParams {
   double dt, dtf, dtfrotate;
   double triggersq;        // 0.25 * skin * skin (neighbor.cpp:290)
+  int neigh_every;         // neigh_modify every (>= 1)
+  int neigh_check;         // neigh_modify check yes (1) / no (0)
   double skin, cdf;
   double lo[3], L[3], halfL[3];
   int per[3];
@@ -144,6 +146,7 @@ struct StepArgs {
   double *fout;      // [6][cap] f and torque when store_f
   Ctrl *ctrl;
   long long step;
+  long long ago_next;  // steps since the last build at the step this launch integrates to
   int shearupdate, do_final, do_initial, store_f, count;
   // with moving meshes, blocks [pblocks, gridDim) rotate the meshes into the
   // next step's geometry buffer (mv.dst) while the others compute this step

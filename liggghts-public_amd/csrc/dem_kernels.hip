@@ -735,12 +735,19 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
         wx += dtir * tx;
         wy += dtir * ty;
         wz += dtir * tz;
-        // Neighbor::check_distance, neighbor.cpp:1436-1443
-        const double hx = x - a.xhold[i];
-        const double hy = y - a.xhold[cap + i];
-        const double hz = z - a.xhold[2 * cap + i];
-        const double rsq = hx * hx + hy * hy + hz * hz;
-        displaced = rsq > p.triggersq;
+        // Neighbor::decide (neighbor.cpp:1366-1371) with delay 0, and
+        // Neighbor::check_distance (neighbor.cpp:1436-1443)
+        if (a.ago_next % p.neigh_every == 0) {
+          if (p.neigh_check) {
+            const double hx = x - a.xhold[i];
+            const double hy = y - a.xhold[cap + i];
+            const double hz = z - a.xhold[2 * cap + i];
+            const double rsq = hx * hx + hy * hy + hz * hz;
+            displaced = rsq > p.triggersq;
+          } else {
+            displaced = true;
+          }
+        }
       }
       a.out.pos[i] = make_double4(x, y, z, pi.w);
       a.out.vel[i] = make_double4(vx, vy, vz, vi.w);
@@ -809,10 +816,16 @@ __global__ __launch_bounds__(kBlock) void k_initial(StepArgs a) {
     a.in.pos[i] = make_double4(x, y, z, pi.w);
     a.in.vel[i] = make_double4(vx, vy, vz, vi.w);
     ((omg_t *)a.in.omg)[i] = wi;
-    const double hx = x - a.xhold[i];
-    const double hy = y - a.xhold[cap + i];
-    const double hz = z - a.xhold[2 * cap + i];
-    displaced = hx * hx + hy * hy + hz * hz > p.triggersq;
+    if (a.ago_next % p.neigh_every == 0) {
+      if (p.neigh_check) {
+        const double hx = x - a.xhold[i];
+        const double hy = y - a.xhold[cap + i];
+        const double hz = z - a.xhold[2 * cap + i];
+        displaced = hx * hx + hy * hy + hz * hz > p.triggersq;
+      } else {
+        displaced = true;
+      }
+    }
   }
   if (__ballot(displaced) != 0ull && (threadIdx.x & 63) == 0)
     atomicMin((unsigned long long *)&a.ctrl->stop_at, 1ull);

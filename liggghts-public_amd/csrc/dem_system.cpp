@@ -111,6 +111,7 @@ struct dem_handle {
 
   // counters
   int64_t ntimestep = 0, nrebuilds = 0;
+  int64_t last_build = 0;  // timestep of the last neighbor build
   int64_t npairs = 0, ncontacts = 0, nwall_contacts = 0, nmesh_cand = 0;
   int maxcount = 0;
   bool timing = false;
@@ -189,6 +190,8 @@ dem::DevParams make_devparams(const dem_handle *h) {
   d.dtf = 0.5 * p.dt * 1.0;  // fix_nve.cpp:86 (ftm2v = 1 in SI)
   d.dtfrotate = d.dtf / 0.4; // fix_nve_sphere.cpp:160
   d.triggersq = 0.25 * p.skin * p.skin;
+  d.neigh_every = p.neigh_every > 0 ? p.neigh_every : 1;
+  d.neigh_check = p.neigh_no_check ? 0 : 1;
   d.skin = p.skin;
   d.cdf = p.cdf;
   for (int k = 0; k < 3; k++) {
@@ -464,6 +467,7 @@ int rebuild(dem_handle *h) {
   h->cur_l = 1 - h->cur_l;
   h->have_list = true;
   h->nrebuilds++;
+  h->last_build = h->ntimestep;
   h->maxcount = h->ctrl_host->maxcount;
   h->npairs = (int64_t)(h->ctrl_host->npairs_full / 2);
   h->nmesh_cand = (int64_t)h->ctrl_host->nmesh_cand;
@@ -530,6 +534,7 @@ int dem_create(const dem_params *params, dem_handle **out) {
     if (!(p.boxhi[d] > p.boxlo[d])) return fail(nullptr, DEM_ERR_ARG, "empty box");
   if (p.precision != DEM_FP64 && p.precision != DEM_MIXED)
     return fail(nullptr, DEM_ERR_ARG, "unknown precision mode");
+  if (p.neigh_every < 0) return fail(nullptr, DEM_ERR_ARG, "neigh_modify every must be > 0");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= p.device || p.device < 0)
     return fail(nullptr, DEM_ERR_NODEV, "no HIP device available");
@@ -930,8 +935,10 @@ int run_steps(dem_handle *h, int64_t nsteps) {
   HIPCHK(h, dem::launch_set_stop(h->ctrl, h->pending_mesh_rebuild ? 1 : kNever, h->stream));
   h->pending_mesh_rebuild = false;
   const bool moving = any_moving_mesh(h);
+  const int64_t ts0 = h->ntimestep;  // step t of this run is timestep ts0 + t
   {
     dem::StepArgs a = step_args(h, h->cur_st, h->tri_cur);
+    a.ago_next = ts0 + 1 - h->last_build;
     HIPCHK(h, dem::launch_initial(a, h->stream));
   }
   // the first step's mesh rotation, in place; every later step's rotation is
@@ -954,6 +961,7 @@ int run_steps(dem_handle *h, int64_t nsteps) {
       a.count = last ? 1 : 0;
       a.move_mesh = (moving && !last) ? 1 : 0;
       a.mv.step = step + 1;
+      a.ago_next = ts0 + step + 1 - h->last_build;
       HIPCHK(h, dem::launch_step(a, h->stream));
       // the start event goes after the batch's first launch: the GPU is then
       // busy while the host enqueues the rest, so the pair times the kernels
@@ -988,6 +996,7 @@ int run_steps(dem_handle *h, int64_t nsteps) {
     if (rebuild_now) {
       h->tri_cur = tc;
       if ((rc = rebuild(h))) return rc;
+      h->last_build = ts0 + stop;  // the build belongs to step `stop` (its initial integrate is done)
       HIPCHK(h, dem::launch_set_stop(h->ctrl, kNever, h->stream));
       batch = std::max(4, batch / 2);
     } else {

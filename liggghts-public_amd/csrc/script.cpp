@@ -7,7 +7,7 @@
 //   newton off | communicate single vel yes | processors ...
 //   region <id> block xlo xhi ylo yhi zlo zhi [units box] | create_box <n> <region>
 //   read_data <file>  (granular Atoms: id type diameter density x y z; Velocities)
-//   neighbor <skin> bin | neigh_modify delay 0 [every 1] [check yes]
+//   neighbor <skin> bin | neigh_modify delay 0 [every N] [check yes|no]
 //                                       [contact_distance_factor f] [one n]
 //   fix <id> <grp> property/global youngsModulus|poissonsRatio peratomtype ...
 //   fix <id> <grp> property/global coefficientRestitution|coefficientFriction|
@@ -83,6 +83,7 @@ struct Interp {
   std::vector<int> type, tag;
   // neighbor / run
   double skin = -1.0, cdf = 1.0, dt = -1.0;
+  int every = 1, no_check = 0;
   int one = 0;
   // materials
   std::vector<double> Y, nu, e, mu, mur;
@@ -413,6 +414,8 @@ struct Interp {
     p.dt = dt;
     p.skin = skin;
     p.cdf = cdf;
+    p.neigh_every = every;
+    p.neigh_no_check = no_check;
     for (int d = 0; d < 3; d++) {
       p.boxlo[d] = lo[d];
       p.boxhi[d] = hi[d];
@@ -582,9 +585,12 @@ struct Interp {
         if (t[k] == "delay") {
           if (inum(t[k + 1]) != 0) die("neigh_modify delay must be 0 on this path");
         } else if (t[k] == "every") {
-          if (inum(t[k + 1]) != 1) die("neigh_modify every must be 1 on this path");
+          every = inum(t[k + 1]);
+          if (every <= 0) die("Illegal neigh_modify command");
         } else if (t[k] == "check") {
-          if (t[k + 1] != "yes") die("neigh_modify check must be yes on this path");
+          if (t[k + 1] == "yes") no_check = 0;
+          else if (t[k + 1] == "no") no_check = 1;
+          else die("Illegal neigh_modify command");
         } else if (t[k] == "contact_distance_factor") {
           cdf = num(t[k + 1]);
         } else if (t[k] == "one") {

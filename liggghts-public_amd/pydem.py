@@ -45,6 +45,8 @@ class DemParams(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("max_neighbors", ctypes.c_int),
         ("device", ctypes.c_int),
+        ("neigh_every", ctypes.c_int),
+        ("neigh_no_check", ctypes.c_int),
     ]
 
 
@@ -163,6 +165,8 @@ class GpuSystem:
         p.precision = precision
         p.max_neighbors = max_neighbors
         p.device = device
+        p.neigh_every = int(params.get("neigh_every", 1))
+        p.neigh_no_check = int(not params.get("neigh_check", True))
         self.h = ctypes.c_void_p()
         rc = self.lib.dem_create(ctypes.byref(p), ctypes.byref(self.h))
         if rc != 0:

@@ -380,6 +380,7 @@ struct orc_sys {
   int nmesh;
   orc_mesh mesh[ORC_MAXMESH];
   int64_t mesh_next_reneighbor;
+  int64_t last_build; /* timestep of the last neighbor build (Neighbor::lastcall) */
   int64_t mesh_forced;
 };
 
@@ -635,6 +636,7 @@ static void build(orc_sys *s) {
 
 /* full rebuild sequence of Verlet::run, verlet.cpp:313-336 */
 static void rebuild(orc_sys *s) {
+  s->last_build = s->ntimestep; /* Neighbor::build resets ago (neighbor.cpp) */
   carry_out(s);
   pbc(s);
   build(s);
@@ -653,6 +655,17 @@ static int check_distance(const orc_sys *s) {
     if (rsq > deltasq) flag = 1;
   }
   return flag;
+}
+
+/* Neighbor::decide, neighbor.cpp:1357-1372, with delay 0: a build every
+ * `every` steps since the last one, if check is off or an atom moved more
+ * than skin/2 */
+static int decide(const orc_sys *s) {
+  const int every = s->p.neigh_every > 0 ? s->p.neigh_every : 1;
+  const int64_t ago = s->ntimestep - s->last_build;
+  if (ago % every != 0) return 0;
+  if (s->p.neigh_no_check) return 1;
+  return check_distance(s);
 }
 
 /* pair loop, pair_gran_base.h:255-430 (newton off: j always local here) */
@@ -787,7 +800,7 @@ void orc_run(orc_sys *s, int nsteps) {
     /* Neighbor::decide with delay 0, every 1: forced by a fix's next_reneighbor
      * (moving mesh), else check every step */
     const int forced = s->mesh_next_reneighbor == s->ntimestep;
-    if (forced || check_distance(s)) {
+    if (forced || decide(s)) {
       if (forced) s->mesh_forced++;
       rebuild(s);
     } else if (mesh_decide_rebuild(s)) {

@@ -57,6 +57,8 @@ typedef struct {
   int plane_type[ORC_MAXPLANE];
   int gravity_first; /* order of post_force fixes: 1 = gravity before walls */
   orc_model plane_model[ORC_MAXPLANE]; /* each fix wall/gran's own model switches */
+  int neigh_every;    /* neigh_modify every N (0 is taken as 1) */
+  int neigh_no_check; /* neigh_modify check no */
 } orc_params;
 
 typedef struct orc_sys orc_sys;

@@ -38,6 +38,7 @@ class OrcParams(ctypes.Structure):
         ("plane_pos", _D * MAXPLANE), ("plane_type", ctypes.c_int * MAXPLANE),
         ("gravity_first", ctypes.c_int),
         ("plane_model", OrcModel * MAXPLANE),
+        ("neigh_every", ctypes.c_int), ("neigh_no_check", ctypes.c_int),
     ]
 
 
@@ -149,6 +150,8 @@ def make_params(params, walls=()):
         p.plane_type[k] = int(wtype)
         p.plane_model[k] = model(wall[3] if len(wall) > 3 else params)
     p.gravity_first = int(params.get("gravity_first", 1))
+    p.neigh_every = int(params.get("neigh_every", 1))
+    p.neigh_no_check = int(not params.get("neigh_check", True))
     return p
 
 

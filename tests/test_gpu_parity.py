@@ -151,6 +151,30 @@ def test_history_carried_across_rebuilds_fp64():
     g.close()
 
 
+@pytest.mark.parametrize("every,check", [(3, True), (2, False), (5, False)])
+def test_neigh_modify_every_check_fp64(every, check):
+    """neigh_modify every N check yes|no: the same build schedule as the
+    oracle (build counts equal, across a `run pre no` boundary) and the same
+    state and history to 1e-8 after 60 steps."""
+    say(f"[gpu] neigh_modify every {every} check {check}")
+    params, mats, P, walls = wl.dense_pack(seed=22, n=1500, rolling=True, speed=0.5)
+    params = dict(params, skin=0.00004, neigh_every=every, neigh_check=check)
+    g, o = build(params, mats, P, walls)
+    g.run(37)
+    o.run(37)
+    g.run_continue(23)
+    o.run_continue(23)
+    say(f"  rebuilds gpu={g.stats()['nrebuilds']} oracle={o.nrebuilds()}")
+    assert g.stats()["nrebuilds"] == o.nrebuilds()
+    if not check:
+        assert o.nrebuilds() == 1 + 60 // every
+    compare_pairs(g, o, hist_tol=1e-8)
+    gs, os_ = g.state(), o.state()
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert_close(k, gs[k], os_[k], 1e-8)
+    g.close()
+
+
 def test_run_continue_is_bitwise_one_run():
     say("[gpu] run + run_continue == one run (bitwise)")
     params, mats, P, walls = wl.dense_pack(seed=31, n=1500, rolling=True, speed=0.3)

@@ -363,3 +363,37 @@ def test_history_survives_rebuild_and_non_touching_pairs_reset():
         else:
             assert f == 0 and not h.any()
     assert n_touch > 100
+
+
+def test_neigh_modify_every_and_check():
+    """Neighbor::decide with delay 0: `check no` builds exactly every `every`
+    steps (so 1 + floor(nsteps/every) builds counting setup's); `check yes`
+    with every > 1 never builds more often than with every 1; and the
+    schedule does not change a contact-free gas at all (the list only
+    decides which pairs are tested)."""
+    params, mats, P, walls = wl.dense_pack(seed=3, n=300, rolling=True)
+    for every in (1, 3, 5):
+        s = orc.OracleSystem(dict(params, neigh_every=every, neigh_check=False), mats, P, walls)
+        s.run(40)
+        assert s.nrebuilds() == 1 + 40 // every
+    counts = {}
+    for every in (1, 4):
+        s = orc.OracleSystem(dict(params, neigh_every=every), mats, P, walls)
+        s.run(300)
+        counts[every] = s.nrebuilds()
+    assert 1 < counts[4] <= counts[1]
+    # dilute gas: same trajectory under any schedule
+    gp, gm, gP, gw = wl.settling_4k(seed=1, n=400)
+    rad = np.full(len(gP["radius"]), 0.001)
+    gP = dict(gP, v=gP["v"] * 50.0, radius=rad, rmass=wl.sphere_mass(rad, 2500.0))
+    ref = None
+    for every, check in ((1, True), (2, False), (3, True)):
+        s = orc.OracleSystem(dict(gp, neigh_every=every, neigh_check=check, grav_acc=None), gm,
+                             gP, [])
+        s.run(100)
+        st = s.state()
+        assert s.contact_counts()[0] == 0 and s.nrebuilds() > 1
+        if ref is None:
+            ref = st
+        else:
+            assert np.array_equal(ref["x"], st["x"]) and np.array_equal(ref["v"], st["v"])
