@@ -212,6 +212,48 @@ int dem_get_stats(dem_handle *h, dem_stats *out);
  * the reference's Timer buckets (timer.h:51-52) play this role on the CPU. */
 int dem_set_kernel_timing(dem_handle *h, int on);
 
+/* ---- Independent simulations batched in one handle ----
+ * No counterpart in the reference, which runs one simulation per process (a
+ * calibration sweep there is many lmp_serial runs, each through the entry
+ * points above). Instance q owns particles [c_0 + ... + c_{q-1},
+ * c_0 + ... + c_q) of the arrays dem_set_particles takes, in the same box and
+ * local coordinates as every other instance, with its own copy of the cell
+ * grid and of every wall and mesh, and its own material block (initially the
+ * dem_set_materials one). Instances never interact; one step of the handle
+ * advances all of them, and a neighbor rebuild requested by any instance
+ * rebuilds all. Call before dem_set_particles; ninst <= 65536. */
+int dem_set_instances(dem_handle *h, int ninst, const int64_t *counts);
+
+/* Materials of one instance, as dem_set_materials (same ntypes). */
+int dem_set_instance_materials(dem_handle *h, int inst, int ntypes, const double *Y,
+                               const double *nu, const double *e, const double *mu,
+                               const double *mur);
+
+/* Per instance (first ninst of them): touching particle pairs and touching
+ * wall (plane or mesh) contacts at the last step of the last run. */
+int dem_get_instance_stats(dem_handle *h, int ninst, int64_t *ncontacts, int64_t *nwall);
+
+/* Tile shape of the fused step kernel: variant 0..4 (threads per block /
+ * neighbor entries loaded together: 256/4, 256/2, 256/8, 128/4, 64/4) and a
+ * persistent grid of `persist` blocks per compute unit (0: one block per
+ * tile). variant = -1 asks for autotuning: the next run spends its first
+ * batches timing every (variant, persist in {0, 4, 8}) candidate, 8 steps
+ * each, and keeps the fastest. Every shape gives bitwise the same results.
+ * No reference counterpart (the reference has one CPU loop). */
+int dem_set_step_shape(dem_handle *h, int variant, int persist);
+
+/* on != 0: dem_run / dem_run_continue replay batches of 32 steps as HIP
+ * graphs (captured once per buffer parity and kernel shape, rebuilt after
+ * any change of the particle set, materials or shape) instead of launching
+ * each step; results are bitwise the same. Event timing (dem_set_kernel_timing)
+ * and autotuning batches launch step by step. No reference counterpart. */
+int dem_set_graph_capture(dem_handle *h, int on);
+
+/* The shape in use and, after autotuning, each candidate's measured
+ * microseconds per step (0 if it was not measured); cap bounds the arrays. */
+int dem_get_step_shape(dem_handle *h, int *variant, int *persist, int cap, int *ncand,
+                       int *cand_variant, int *cand_persist, double *cand_us);
+
 /* Runs an input script with the reference's command subset for this path
  * (Input::file, input.cpp:183, and Input::execute_command, input.cpp:560-650).
  * log_path may be NULL (log to stdout). Returns a dem_status. */

@@ -18,6 +18,16 @@ constexpr int kMetaWallShift = 16;           // bits 16..21: wall history presen
 constexpr int kMaxMeshes = 4;
 constexpr int kMeshSlots = 8;       // contact-history slots per particle and mesh
 constexpr int kMaxMeshCand = 32;    // candidate triangles per particle (all meshes)
+// omega.w of a particle holds type + (instance << kTypeBits): independent
+// simulations batched in one system each own a cell grid and a material block
+constexpr int kTypeBits = 8;
+constexpr int kTypeMask = (1 << kTypeBits) - 1;
+constexpr int kMaxInstances = 1 << 16;
+// Tile shapes the step kernel is compiled for (threads per block, neighbor
+// entries whose loads are issued together); variant 0 is the default.
+constexpr int kNumStepVariants = 5;
+constexpr int kStepVariantBlock[kNumStepVariants] = {256, 256, 256, 128, 64};
+constexpr int kStepVariantGroup[kNumStepVariants] = {4, 2, 8, 4, 4};
 
 // One triangle of a fix mesh/surface, FP64 (TriMesh element properties).
 struct TriGeom {
@@ -58,9 +68,12 @@ struct DevParams {
   int mesh_tri0[kMaxMeshes], mesh_ntri[kMaxMeshes];
   int mesh_wtype[kMaxMeshes], mesh_model[kMaxMeshes], mesh_moving[kMaxMeshes];
   double mesh_precision;   // EPSILON_PRECISION of the mesh (multi_node_mesh.h:59)
-  // cell grid
+  // cell grid of one instance; instance k owns global cells
+  // [k * ncell_local, (k + 1) * ncell_local)
   int nc[3];
   double cinv[3];
+  int ninst, ncell_local;
+  int mat_stride;          // ntp1 * ntp1: material block of one instance
 };
 
 // Mixed material tables, [ntp1*ntp1] each, FP64 and FP32 copies.
@@ -82,6 +95,7 @@ struct Ctrl {
   int slot_overflow;     // simultaneous mesh contacts of a particle beyond kMeshSlots
   int pad2;
   unsigned long long nmesh_cand;  // (particle, triangle) candidates at the last build
+  long long step_base, ago_base;  // step index and neighbor age of a graph replay's first launch
 };
 
 // Particle state read by neighbors: ping-pong buffered across steps.
@@ -145,12 +159,16 @@ struct StepArgs {
   double *xhold;     // [3][cap]
   double *fout;      // [6][cap] f and torque when store_f
   Ctrl *ctrl;
+  unsigned long long *inst_stats;  // [2][ninst] touching pairs (full list) and wall contacts, or null
   long long step;
   long long ago_next;  // steps since the last build at the step this launch integrates to
   int shearupdate, do_final, do_initial, store_f, count;
-  // with moving meshes, blocks [pblocks, gridDim) rotate the meshes into the
-  // next step's geometry buffer (mv.dst) while the others compute this step
-  int pblocks, move_mesh;
+  // tile shape (kStepVariant*) and the number of blocks that walk the
+  // particle tiles (all tiles: one each; fewer: a persistent grid); with
+  // moving meshes, blocks [pblocks, gridDim) rotate the meshes into the next
+  // step's geometry buffer (mv.dst) while the others compute this step
+  int variant, pblocks, move_mesh;
+  int step_from_ctrl;  // step = ctrl->step_base + step, ago_next = ctrl->ago_base + step
   MeshMoveArgs mv;
 };
 
@@ -188,8 +206,12 @@ struct RebuildArgs {
 hipError_t launch_rebuild(const RebuildArgs &a, hipStream_t s);
 hipError_t launch_step(const StepArgs &a, hipStream_t s);
 hipError_t launch_initial(const StepArgs &a, hipStream_t s);
-hipError_t launch_set_stop(Ctrl *ctrl, long long v, hipStream_t s);
+// stop_at = v, run counters cleared (and the per-instance ones, nstat entries)
+hipError_t launch_set_stop(Ctrl *ctrl, long long v, hipStream_t s,
+                           unsigned long long *inst_stats = nullptr, int nstat = 0);
 hipError_t launch_mesh_move(const MeshMoveArgs &a, hipStream_t s);
+// step index and neighbor age read by graph-captured step launches
+hipError_t launch_set_base(Ctrl *ctrl, long long step, long long ago, hipStream_t s);
 // Mesh part of a rebuild, after the particle gather and build. Phase 1:
 // element geometry refreshed from the nodes, node positions stored, and the
 // number of (triangle, cell) pairs counted; the total lands in *total (device).

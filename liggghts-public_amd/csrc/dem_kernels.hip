@@ -61,7 +61,6 @@ __device__ __forceinline__ void min_image(const DevParams &p, double xi, double 
   }
 }
 
-constexpr int kGroup = 4;   // neighbor entries whose loads are issued together
 
 template <typename T> struct OmgT;
 template <> struct OmgT<double> { typedef double4 type; };
@@ -244,7 +243,7 @@ __device__ __forceinline__ bool tri_coplanar(const MeshStep &M, int a, int b) {
 
 template <typename T, typename W>
 __device__ inline void mesh_contacts(const StepArgs &a, int i, const double4 &pi,
-                                     const double4 &vi, const W &wi, int itype, T &fx, T &fy,
+                                     const double4 &vi, const W &wi, int irow, T &fx, T &fy,
                                      T &fz, T &tx, T &ty, T &tz,
                                      unsigned long long &nwall) {
   const DevParams &p = a.prm;
@@ -259,7 +258,7 @@ __device__ inline void mesh_contacts(const StepArgs &a, int i, const double4 &pi
     T *hist = (T *)a.aux.mhist + (size_t)m * kMeshSlots * 3 * cap + i;
     const int tend = p.mesh_tri0[m] + p.mesh_ntri[m];
     const Model mdl = model_from_bits(p.mesh_model[m]);
-    const int mt = itype * p.ntp1 + p.mesh_wtype[m];
+    const int mt = irow + p.mesh_wtype[m];
     unsigned keep = 0u;
     for (; c < nc; c++) {
       const int t = M.cand[(size_t)c * cap + i];
@@ -449,325 +448,360 @@ __device__ __forceinline__ bool mesh_move_tri(const MeshMoveArgs &a, const int t
   return moved;
 }
 
-__device__ __forceinline__ void mesh_move_block(const MeshMoveArgs &a, const int t) {
+// step: the step the moved geometry belongs to
+__device__ __forceinline__ void mesh_move_block(const MeshMoveArgs &a, const int t,
+                                                const long long step) {
   const bool moved = t < a.ntri && mesh_move_tri(a, t);
   if (__ballot(moved) != 0ull && (threadIdx.x & 63) == 0)
-    atomicMin((unsigned long long *)&a.ctrl->stop_at, (unsigned long long)(a.step + 1));
+    atomicMin((unsigned long long *)&a.ctrl->stop_at, (unsigned long long)(step + 1));
+}
+
+// One particle's step: contacts over its list (broad phase in groups of
+// GROUP entries), walls, meshes, gravity, integration and the displacement
+// test. Accumulates its touching pair and wall contacts.
+template <typename T, bool MESH, int GROUP>
+__device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
+                                              unsigned long long &my_contacts,
+                                              unsigned long long &my_wall, int &my_inst,
+                                              bool &displaced, const long long ago_next) {
+  typedef typename OmgT<T>::type omg_t;
+  const DevParams &p = a.prm;
+
+  const double4 pi = a.in.pos[i];
+  const double4 vi = a.in.vel[i];
+  const omg_t wi = ((const omg_t *)a.in.omg)[i];
+  const int wbits = (int)wi.w;
+  // material row of this particle: its instance's block, its type's row
+  const int irow = (wbits >> kTypeBits) * p.mat_stride + (wbits & kTypeMask) * p.ntp1;
+  my_inst = wbits >> kTypeBits;
+  const int meta0 = a.aux.meta[i];
+  int meta = meta0;
+  const int cnt = meta & kMetaCountMask;
+  const T dt = (T)p.dt;
+  const Model pm{p.rolling, p.tdamp, p.limitf, p.torsion};
+  T fx = T(0), fy = T(0), fz = T(0), tx = T(0), ty = T(0), tz = T(0);
+  T *hist = (T *)a.list.hist;
+  const size_t cap = (size_t)a.cap;
+
+  // Broad phase in groups of GROUP entries: the index loads of a group are
+  // independent, then its position gathers, so a wavefront has GROUP
+  // memory requests in flight instead of one dependent chain per entry.
+  // Touching entries of the group are then resolved one by one, reloading
+  // their (cache-resident) index and position so the group's registers are
+  // free during the contact law.
+  for (int k0 = 0; k0 < cnt; k0 += GROUP) {
+    uint32_t ent[GROUP];
+#pragma unroll
+    for (int u = 0; u < GROUP; u++)
+      ent[u] = (k0 + u < cnt) ? a.list.nbr[(size_t)(k0 + u) * cap + i] : (uint32_t)i;
+    double4 pj[GROUP];
+#pragma unroll
+    for (int u = 0; u < GROUP; u++) pj[u] = a.in.pos[ent[u] & kIndexMask];
+    uint32_t bits = 0u;
+#pragma unroll
+    for (int u = 0; u < GROUP; u++) {
+      double dx, dy, dz;
+      min_image(p, pi.x, pi.y, pi.z, pj[u].x, pj[u].y, pj[u].z, dx, dy, dz);
+      const double rsq = dx * dx + dy * dy + dz * dz;
+      const double radsum = pi.w + pj[u].w;
+      if (k0 + u < cnt && rsq < radsum * radsum) bits |= 1u << u;
+    }
+   while (bits) {
+    const int k = k0 + __builtin_ctz(bits);
+    bits &= bits - 1u;
+    const size_t slot = (size_t)k * cap + i;
+    const uint32_t e = a.list.nbr[slot];
+    const int j = (int)(e & kIndexMask);
+    const double4 pj = a.in.pos[j];
+    double dx, dy, dz;
+    min_image(p, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, dx, dy, dz);
+    const double rsq = dx * dx + dy * dy + dz * dz;
+    const double radsum = pi.w + pj.w;
+    // touching: pair_gran_base.h:356-430
+    const double4 vj = a.in.vel[j];
+    const omg_t wj = ((const omg_t *)a.in.omg)[j];
+    const int jtype = (int)wj.w & kTypeMask;
+    const double r = sqrt(rsq);
+    const double rinv = 1.0 / r;
+    CIn<T> c;
+    c.radi = (T)pi.w;
+    c.radj = (T)pj.w;
+    c.rinv = (T)rinv;
+    c.meff = (T)(vi.w * vj.w / (vi.w + vj.w));
+    c.deltan = (T)(radsum - r);
+    c.deltan_in = T(0);
+    c.en[0] = (T)(dx * rinv);
+    c.en[1] = (T)(dy * rinv);
+    c.en[2] = (T)(dz * rinv);
+    c.del[0] = (T)dx;
+    c.del[1] = (T)dy;
+    c.del[2] = (T)dz;
+    c.vr[0] = (T)(vi.x - vj.x);
+    c.vr[1] = (T)(vi.y - vj.y);
+    c.vr[2] = (T)(vi.z - vj.z);
+    c.wi[0] = (T)wi.x;
+    c.wi[1] = (T)wi.y;
+    c.wi[2] = (T)wi.z;
+    c.wj[0] = (T)wj.x;
+    c.wj[1] = (T)wj.y;
+    c.wj[2] = (T)wj.z;
+    const int mt = irow + jtype;
+    if (sizeof(T) == 8) {
+      c.Y = (T)a.mat.Yeff[mt];
+      c.G = (T)a.mat.Geff[mt];
+      c.beta = (T)a.mat.beta[mt];
+      c.mu = (T)a.mat.mu[mt];
+      c.mur = (T)a.mat.mur[mt];
+    } else {
+      c.Y = (T)a.mat.fYeff[mt];
+      c.G = (T)a.mat.fGeff[mt];
+      c.beta = (T)a.mat.fbeta[mt];
+      c.mu = (T)a.mat.fmu[mt];
+      c.mur = (T)a.mat.fmur[mt];
+    }
+    const bool flagged = (e & kFlagBit) != 0;
+    T sh[3];
+    const size_t hb = (size_t)k * 3 * cap + i;
+    if (flagged) {
+      sh[0] = hist[hb];
+      sh[1] = hist[hb + cap];
+      sh[2] = hist[hb + 2 * cap];
+    } else {
+      sh[0] = sh[1] = sh[2] = T(0);
+    }
+    T dF[3], dTi[3], dTj[3];
+    contact_chain<T, false>(pm, c, dt, a.shearupdate, sh, dF, dTi, dTj);
+    if (a.shearupdate || !flagged) {
+      hist[hb] = sh[0];
+      hist[hb + cap] = sh[1];
+      hist[hb + 2 * cap] = sh[2];
+    }
+    if (!flagged) a.list.nbr[slot] = e | kFlagBit;
+    fx += dF[0];
+    fy += dF[1];
+    fz += dF[2];
+    tx += dTi[0];
+    ty += dTi[1];
+    tz += dTi[2];
+    my_contacts++;
+   }
+  }
+
+  // post_force fixes in script order: gravity (fix_gravity.cpp:331-338) and
+  // plane walls (fix_wall_gran.cpp:1024-1153)
+  for (int pass = 0; pass < 2; pass++) {
+    const bool grav_pass = (pass == 0) == (p.grav_first != 0);
+    if (grav_pass) {
+      if (p.has_grav) {
+        fx += (T)(vi.w * p.gacc[0]);
+        fy += (T)(vi.w * p.gacc[1]);
+        fz += (T)(vi.w * p.gacc[2]);
+      }
+      continue;
+    }
+    for (int w = 0; w < p.nwalls; w++) {
+      const int ax = p.wall_axis[w];
+      const double param = p.wall_pos[w];
+      const double xa = ax == 0 ? pi.x : (ax == 1 ? pi.y : pi.z);
+      double deltan, dlt;
+      if (xa > param) {
+        dlt = param - xa;
+        deltan = xa - param - pi.w;
+      } else {
+        dlt = param - xa;
+        deltan = param - xa - pi.w;
+      }
+      T *wh = (T *)a.aux.whist;
+      const size_t wb = (size_t)w * 3 * cap + i;
+      const int bit = 1 << (kMetaWallShift + w);
+      if (deltan > p.cutneighmax) continue;
+      if (deltan <= 0) {
+        // sidata.deltan = -deltan, sidata.delta = -delta; compute_force
+        // (fix_wall_gran_base.h:179-188): r = radi - sidata.deltan
+        const double sdn = -deltan;
+        const double sd = -dlt;
+        const double r = pi.w - sdn;
+        const double rinv = 1.0 / r;
+        CIn<T> c;
+        c.radi = (T)pi.w;
+        c.radj = T(0);
+        c.rinv = (T)rinv;
+        c.meff = (T)vi.w;
+        c.deltan = (T)(pi.w - r);
+        c.deltan_in = (T)sdn;
+        // select instead of c.del[ax] = ...: a dynamic index would put the
+        // struct in scratch memory
+        const T sdT = (T)sd, enA = (T)(sd * rinv), enZ = (T)(0.0 * rinv);
+        c.del[0] = ax == 0 ? sdT : T(0);
+        c.del[1] = ax == 1 ? sdT : T(0);
+        c.del[2] = ax == 2 ? sdT : T(0);
+        c.en[0] = ax == 0 ? enA : enZ;
+        c.en[1] = ax == 1 ? enA : enZ;
+        c.en[2] = ax == 2 ? enA : enZ;
+        c.vr[0] = (T)(vi.x - 0.0);
+        c.vr[1] = (T)(vi.y - 0.0);
+        c.vr[2] = (T)(vi.z - 0.0);
+        c.wi[0] = (T)wi.x;
+        c.wi[1] = (T)wi.y;
+        c.wi[2] = (T)wi.z;
+        c.wj[0] = c.wj[1] = c.wj[2] = T(0);
+        const int mt = irow + p.wall_type[w];
+        if (sizeof(T) == 8) {
+          c.Y = (T)a.mat.Yeff[mt];
+          c.G = (T)a.mat.Geff[mt];
+          c.beta = (T)a.mat.beta[mt];
+          c.mu = (T)a.mat.mu[mt];
+          c.mur = (T)a.mat.mur[mt];
+        } else {
+          c.Y = (T)a.mat.fYeff[mt];
+          c.G = (T)a.mat.fGeff[mt];
+          c.beta = (T)a.mat.fbeta[mt];
+          c.mu = (T)a.mat.fmu[mt];
+          c.mur = (T)a.mat.fmur[mt];
+        }
+        T sh[3];
+        if (meta & bit) {
+          sh[0] = wh[wb];
+          sh[1] = wh[wb + cap];
+          sh[2] = wh[wb + 2 * cap];
+        } else {
+          sh[0] = sh[1] = sh[2] = T(0);
+        }
+        T dF[3], dTi[3], dTj[3];
+        contact_chain<T, true>(model_from_bits(p.wall_model[w]), c, dt, a.shearupdate, sh, dF,
+                               dTi, dTj);
+        wh[wb] = sh[0];
+        wh[wb + cap] = sh[1];
+        wh[wb + 2 * cap] = sh[2];
+        meta |= bit;
+        fx += dF[0];
+        fy += dF[1];
+        fz += dF[2];
+        tx += dTi[0];
+        ty += dTi[1];
+        tz += dTi[2];
+        my_wall++;
+      } else {
+        meta &= ~bit;  // history zeroed when the contact ends (fix_wall_gran.cpp:1145-1151)
+      }
+    }
+    if (MESH)
+      mesh_contacts<T>(a, i, pi, vi, wi, irow, fx, fy, fz, tx, ty, tz, my_wall);
+  }
+
+  if (a.store_f) {
+    a.fout[i] = (double)fx;
+    a.fout[cap + i] = (double)fy;
+    a.fout[2 * cap + i] = (double)fz;
+    a.fout[3 * cap + i] = (double)tx;
+    a.fout[4 * cap + i] = (double)ty;
+    a.fout[5 * cap + i] = (double)tz;
+  }
+
+  // nve/sphere: final integrate of this step (fix_nve_sphere.cpp:208-246) and
+  // initial integrate of the next (fix_nve_sphere.cpp:138-206)
+  if (a.do_final || a.do_initial) {
+    double x = pi.x, y = pi.y, z = pi.z;
+    double vx = vi.x, vy = vi.y, vz = vi.z;
+    const double dtfm = p.dtf / (vi.w * 1.0);
+    const double dtirotate = p.dtfrotate / (pi.w * pi.w * vi.w);
+    const double Fx = (double)fx, Fy = (double)fy, Fz = (double)fz;
+    T wx = (T)wi.x, wy = (T)wi.y, wz = (T)wi.z;
+    const T dtir = (T)dtirotate;
+    if (a.do_final) {
+      vx += dtfm * Fx;
+      vy += dtfm * Fy;
+      vz += dtfm * Fz;
+      wx += dtir * tx;
+      wy += dtir * ty;
+      wz += dtir * tz;
+    }
+    if (a.do_initial) {
+      vx += dtfm * Fx;
+      vy += dtfm * Fy;
+      vz += dtfm * Fz;
+      x += p.dt * vx;
+      y += p.dt * vy;
+      z += p.dt * vz;
+      wx += dtir * tx;
+      wy += dtir * ty;
+      wz += dtir * tz;
+      // Neighbor::decide (neighbor.cpp:1366-1371) with delay 0, and
+      // Neighbor::check_distance (neighbor.cpp:1436-1443)
+      if (ago_next % p.neigh_every == 0) {
+        if (p.neigh_check) {
+          const double hx = x - a.xhold[i];
+          const double hy = y - a.xhold[cap + i];
+          const double hz = z - a.xhold[2 * cap + i];
+          const double rsq = hx * hx + hy * hy + hz * hz;
+          displaced = rsq > p.triggersq;
+        } else {
+          displaced = true;
+        }
+      }
+    }
+    a.out.pos[i] = make_double4(x, y, z, pi.w);
+    a.out.vel[i] = make_double4(vx, vy, vz, vi.w);
+    omg_t wo;
+    wo.x = wx;
+    wo.y = wy;
+    wo.z = wz;
+    wo.w = wi.w;
+    ((omg_t *)a.out.omg)[i] = wo;
+  }
+  if (meta != meta0) a.aux.meta[i] = meta;
 }
 
 // ---------------------------------------------------------------------------
 // fused step kernel: pair forces over the full list, plane walls, meshes,
 // gravity, nve/sphere final integrate of this step and initial integrate of
 // the next, and the skin/2 displacement check of Neighbor::decide for the
-// next step. One thread per particle; reads the state buffer `in`, writes
-// `out`. With moving meshes, trailing blocks rotate the meshes into the next
-// step's geometry buffer.
+// next step. One thread per particle in tiles of BLOCK particles; the
+// first pblocks blocks walk the tiles (one tile each, or several when the
+// grid is made persistent). With moving meshes, trailing blocks rotate the
+// meshes into the next step's geometry buffer. Every tile shape computes
+// the same arithmetic in the same order, so results do not depend on it.
 // ---------------------------------------------------------------------------
-template <typename T, bool MESH>
-__global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
-  typedef typename OmgT<T>::type omg_t;
-  if (a.step >= a.ctrl->stop_at) return;
+template <typename T, bool MESH, int BLOCK, int GROUP>
+__global__ __launch_bounds__(BLOCK) void k_step(StepArgs a) {
+  // in a captured graph the step index is the control block's base plus this
+  // launch's offset (a.step)
+  long long step = a.step, ago_next = a.ago_next;
+  if (a.step_from_ctrl) {
+    step = a.ctrl->step_base + a.step;
+    ago_next = a.ctrl->ago_base + a.step;
+  }
+  if (step >= a.ctrl->stop_at) return;
   if (MESH && (int)blockIdx.x >= a.pblocks) {
-    mesh_move_block(a.mv, ((int)blockIdx.x - a.pblocks) * kBlock + threadIdx.x);
+    mesh_move_block(a.mv, ((int)blockIdx.x - a.pblocks) * BLOCK + threadIdx.x, step + 1);
     return;
   }
-  const int i = xcd_block(a.pblocks) * kBlock + threadIdx.x;
-  const bool active = i < a.n;
   const DevParams &p = a.prm;
   unsigned long long my_contacts = 0, my_wall = 0;
   bool displaced = false;
-
-  if (active) {
-    const double4 pi = a.in.pos[i];
-    const double4 vi = a.in.vel[i];
-    const omg_t wi = ((const omg_t *)a.in.omg)[i];
-    const int itype = (int)wi.w;
-    const int meta0 = a.aux.meta[i];
-    int meta = meta0;
-    const int cnt = meta & kMetaCountMask;
-    const int ntp1 = p.ntp1;
-    const T dt = (T)p.dt;
-    const Model pm{p.rolling, p.tdamp, p.limitf, p.torsion};
-    T fx = T(0), fy = T(0), fz = T(0), tx = T(0), ty = T(0), tz = T(0);
-    T *hist = (T *)a.list.hist;
-    const size_t cap = (size_t)a.cap;
-
-    // Broad phase in groups of kGroup entries: the index loads of a group are
-    // independent, then its position gathers, so a wavefront has kGroup
-    // memory requests in flight instead of one dependent chain per entry.
-    // Touching entries of the group are then resolved one by one, reloading
-    // their (cache-resident) index and position so the group's registers are
-    // free during the contact law.
-    for (int k0 = 0; k0 < cnt; k0 += kGroup) {
-      uint32_t ent[kGroup];
-#pragma unroll
-      for (int u = 0; u < kGroup; u++)
-        ent[u] = (k0 + u < cnt) ? a.list.nbr[(size_t)(k0 + u) * cap + i] : (uint32_t)i;
-      double4 pj[kGroup];
-#pragma unroll
-      for (int u = 0; u < kGroup; u++) pj[u] = a.in.pos[ent[u] & kIndexMask];
-      uint32_t bits = 0u;
-#pragma unroll
-      for (int u = 0; u < kGroup; u++) {
-        double dx, dy, dz;
-        min_image(p, pi.x, pi.y, pi.z, pj[u].x, pj[u].y, pj[u].z, dx, dy, dz);
-        const double rsq = dx * dx + dy * dy + dz * dz;
-        const double radsum = pi.w + pj[u].w;
-        if (k0 + u < cnt && rsq < radsum * radsum) bits |= 1u << u;
+  const int ntiles = (a.n + BLOCK - 1) / BLOCK;
+  for (int tile = xcd_block(a.pblocks); tile < ntiles; tile += a.pblocks) {
+    const int i = tile * BLOCK + threadIdx.x;
+    if (i < a.n) {
+      unsigned long long c = 0, w = 0;
+      int inst = 0;
+      step_particle<T, MESH, GROUP>(a, i, c, w, inst, displaced, ago_next);
+      my_contacts += c;
+      my_wall += w;
+      if (a.count && a.inst_stats) {
+        if (c) atomicAdd(&a.inst_stats[inst], c);
+        if (w) atomicAdd(&a.inst_stats[p.ninst + inst], w);
       }
-     while (bits) {
-      const int k = k0 + __builtin_ctz(bits);
-      bits &= bits - 1u;
-      const size_t slot = (size_t)k * cap + i;
-      const uint32_t e = a.list.nbr[slot];
-      const int j = (int)(e & kIndexMask);
-      const double4 pj = a.in.pos[j];
-      double dx, dy, dz;
-      min_image(p, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, dx, dy, dz);
-      const double rsq = dx * dx + dy * dy + dz * dz;
-      const double radsum = pi.w + pj.w;
-      // touching: pair_gran_base.h:356-430
-      const double4 vj = a.in.vel[j];
-      const omg_t wj = ((const omg_t *)a.in.omg)[j];
-      const int jtype = (int)wj.w;
-      const double r = sqrt(rsq);
-      const double rinv = 1.0 / r;
-      CIn<T> c;
-      c.radi = (T)pi.w;
-      c.radj = (T)pj.w;
-      c.rinv = (T)rinv;
-      c.meff = (T)(vi.w * vj.w / (vi.w + vj.w));
-      c.deltan = (T)(radsum - r);
-      c.deltan_in = T(0);
-      c.en[0] = (T)(dx * rinv);
-      c.en[1] = (T)(dy * rinv);
-      c.en[2] = (T)(dz * rinv);
-      c.del[0] = (T)dx;
-      c.del[1] = (T)dy;
-      c.del[2] = (T)dz;
-      c.vr[0] = (T)(vi.x - vj.x);
-      c.vr[1] = (T)(vi.y - vj.y);
-      c.vr[2] = (T)(vi.z - vj.z);
-      c.wi[0] = (T)wi.x;
-      c.wi[1] = (T)wi.y;
-      c.wi[2] = (T)wi.z;
-      c.wj[0] = (T)wj.x;
-      c.wj[1] = (T)wj.y;
-      c.wj[2] = (T)wj.z;
-      const int mt = itype * ntp1 + jtype;
-      if (sizeof(T) == 8) {
-        c.Y = (T)a.mat.Yeff[mt];
-        c.G = (T)a.mat.Geff[mt];
-        c.beta = (T)a.mat.beta[mt];
-        c.mu = (T)a.mat.mu[mt];
-        c.mur = (T)a.mat.mur[mt];
-      } else {
-        c.Y = (T)a.mat.fYeff[mt];
-        c.G = (T)a.mat.fGeff[mt];
-        c.beta = (T)a.mat.fbeta[mt];
-        c.mu = (T)a.mat.fmu[mt];
-        c.mur = (T)a.mat.fmur[mt];
-      }
-      const bool flagged = (e & kFlagBit) != 0;
-      T sh[3];
-      const size_t hb = (size_t)k * 3 * cap + i;
-      if (flagged) {
-        sh[0] = hist[hb];
-        sh[1] = hist[hb + cap];
-        sh[2] = hist[hb + 2 * cap];
-      } else {
-        sh[0] = sh[1] = sh[2] = T(0);
-      }
-      T dF[3], dTi[3], dTj[3];
-      contact_chain<T, false>(pm, c, dt, a.shearupdate, sh, dF, dTi, dTj);
-      if (a.shearupdate || !flagged) {
-        hist[hb] = sh[0];
-        hist[hb + cap] = sh[1];
-        hist[hb + 2 * cap] = sh[2];
-      }
-      if (!flagged) a.list.nbr[slot] = e | kFlagBit;
-      fx += dF[0];
-      fy += dF[1];
-      fz += dF[2];
-      tx += dTi[0];
-      ty += dTi[1];
-      tz += dTi[2];
-      my_contacts++;
-     }
     }
-
-    // post_force fixes in script order: gravity (fix_gravity.cpp:331-338) and
-    // plane walls (fix_wall_gran.cpp:1024-1153)
-    for (int pass = 0; pass < 2; pass++) {
-      const bool grav_pass = (pass == 0) == (p.grav_first != 0);
-      if (grav_pass) {
-        if (p.has_grav) {
-          fx += (T)(vi.w * p.gacc[0]);
-          fy += (T)(vi.w * p.gacc[1]);
-          fz += (T)(vi.w * p.gacc[2]);
-        }
-        continue;
-      }
-      for (int w = 0; w < p.nwalls; w++) {
-        const int ax = p.wall_axis[w];
-        const double param = p.wall_pos[w];
-        const double xa = ax == 0 ? pi.x : (ax == 1 ? pi.y : pi.z);
-        double deltan, dlt;
-        if (xa > param) {
-          dlt = param - xa;
-          deltan = xa - param - pi.w;
-        } else {
-          dlt = param - xa;
-          deltan = param - xa - pi.w;
-        }
-        T *wh = (T *)a.aux.whist;
-        const size_t wb = (size_t)w * 3 * cap + i;
-        const int bit = 1 << (kMetaWallShift + w);
-        if (deltan > p.cutneighmax) continue;
-        if (deltan <= 0) {
-          // sidata.deltan = -deltan, sidata.delta = -delta; compute_force
-          // (fix_wall_gran_base.h:179-188): r = radi - sidata.deltan
-          const double sdn = -deltan;
-          const double sd = -dlt;
-          const double r = pi.w - sdn;
-          const double rinv = 1.0 / r;
-          CIn<T> c;
-          c.radi = (T)pi.w;
-          c.radj = T(0);
-          c.rinv = (T)rinv;
-          c.meff = (T)vi.w;
-          c.deltan = (T)(pi.w - r);
-          c.deltan_in = (T)sdn;
-          // select instead of c.del[ax] = ...: a dynamic index would put the
-          // struct in scratch memory
-          const T sdT = (T)sd, enA = (T)(sd * rinv), enZ = (T)(0.0 * rinv);
-          c.del[0] = ax == 0 ? sdT : T(0);
-          c.del[1] = ax == 1 ? sdT : T(0);
-          c.del[2] = ax == 2 ? sdT : T(0);
-          c.en[0] = ax == 0 ? enA : enZ;
-          c.en[1] = ax == 1 ? enA : enZ;
-          c.en[2] = ax == 2 ? enA : enZ;
-          c.vr[0] = (T)(vi.x - 0.0);
-          c.vr[1] = (T)(vi.y - 0.0);
-          c.vr[2] = (T)(vi.z - 0.0);
-          c.wi[0] = (T)wi.x;
-          c.wi[1] = (T)wi.y;
-          c.wi[2] = (T)wi.z;
-          c.wj[0] = c.wj[1] = c.wj[2] = T(0);
-          const int mt = itype * ntp1 + p.wall_type[w];
-          if (sizeof(T) == 8) {
-            c.Y = (T)a.mat.Yeff[mt];
-            c.G = (T)a.mat.Geff[mt];
-            c.beta = (T)a.mat.beta[mt];
-            c.mu = (T)a.mat.mu[mt];
-            c.mur = (T)a.mat.mur[mt];
-          } else {
-            c.Y = (T)a.mat.fYeff[mt];
-            c.G = (T)a.mat.fGeff[mt];
-            c.beta = (T)a.mat.fbeta[mt];
-            c.mu = (T)a.mat.fmu[mt];
-            c.mur = (T)a.mat.fmur[mt];
-          }
-          T sh[3];
-          if (meta & bit) {
-            sh[0] = wh[wb];
-            sh[1] = wh[wb + cap];
-            sh[2] = wh[wb + 2 * cap];
-          } else {
-            sh[0] = sh[1] = sh[2] = T(0);
-          }
-          T dF[3], dTi[3], dTj[3];
-          contact_chain<T, true>(model_from_bits(p.wall_model[w]), c, dt, a.shearupdate, sh, dF,
-                                 dTi, dTj);
-          wh[wb] = sh[0];
-          wh[wb + cap] = sh[1];
-          wh[wb + 2 * cap] = sh[2];
-          meta |= bit;
-          fx += dF[0];
-          fy += dF[1];
-          fz += dF[2];
-          tx += dTi[0];
-          ty += dTi[1];
-          tz += dTi[2];
-          my_wall++;
-        } else {
-          meta &= ~bit;  // history zeroed when the contact ends (fix_wall_gran.cpp:1145-1151)
-        }
-      }
-      if (MESH)
-        mesh_contacts<T>(a, i, pi, vi, wi, itype, fx, fy, fz, tx, ty, tz, my_wall);
-    }
-
-    if (a.store_f) {
-      a.fout[i] = (double)fx;
-      a.fout[cap + i] = (double)fy;
-      a.fout[2 * cap + i] = (double)fz;
-      a.fout[3 * cap + i] = (double)tx;
-      a.fout[4 * cap + i] = (double)ty;
-      a.fout[5 * cap + i] = (double)tz;
-    }
-
-    // nve/sphere: final integrate of this step (fix_nve_sphere.cpp:208-246) and
-    // initial integrate of the next (fix_nve_sphere.cpp:138-206)
-    if (a.do_final || a.do_initial) {
-      double x = pi.x, y = pi.y, z = pi.z;
-      double vx = vi.x, vy = vi.y, vz = vi.z;
-      const double dtfm = p.dtf / (vi.w * 1.0);
-      const double dtirotate = p.dtfrotate / (pi.w * pi.w * vi.w);
-      const double Fx = (double)fx, Fy = (double)fy, Fz = (double)fz;
-      T wx = (T)wi.x, wy = (T)wi.y, wz = (T)wi.z;
-      const T dtir = (T)dtirotate;
-      if (a.do_final) {
-        vx += dtfm * Fx;
-        vy += dtfm * Fy;
-        vz += dtfm * Fz;
-        wx += dtir * tx;
-        wy += dtir * ty;
-        wz += dtir * tz;
-      }
-      if (a.do_initial) {
-        vx += dtfm * Fx;
-        vy += dtfm * Fy;
-        vz += dtfm * Fz;
-        x += p.dt * vx;
-        y += p.dt * vy;
-        z += p.dt * vz;
-        wx += dtir * tx;
-        wy += dtir * ty;
-        wz += dtir * tz;
-        // Neighbor::decide (neighbor.cpp:1366-1371) with delay 0, and
-        // Neighbor::check_distance (neighbor.cpp:1436-1443)
-        if (a.ago_next % p.neigh_every == 0) {
-          if (p.neigh_check) {
-            const double hx = x - a.xhold[i];
-            const double hy = y - a.xhold[cap + i];
-            const double hz = z - a.xhold[2 * cap + i];
-            const double rsq = hx * hx + hy * hy + hz * hz;
-            displaced = rsq > p.triggersq;
-          } else {
-            displaced = true;
-          }
-        }
-      }
-      a.out.pos[i] = make_double4(x, y, z, pi.w);
-      a.out.vel[i] = make_double4(vx, vy, vz, vi.w);
-      omg_t wo;
-      wo.x = wx;
-      wo.y = wy;
-      wo.z = wz;
-      wo.w = wi.w;
-      ((omg_t *)a.out.omg)[i] = wo;
-    }
-    if (meta != meta0) a.aux.meta[i] = meta;
   }
 
   // rebuild request for the next step
   if (__ballot(displaced) != 0ull) {
     if ((threadIdx.x & 63) == 0)
-      atomicMin((unsigned long long *)&a.ctrl->stop_at, (unsigned long long)(a.step + 1));
+      atomicMin((unsigned long long *)&a.ctrl->stop_at, (unsigned long long)(step + 1));
   }
   if (a.count) {
-    __shared__ unsigned long long sc[kBlock / 64], sw[kBlock / 64];
+    __shared__ unsigned long long sc[BLOCK / 64], sw[BLOCK / 64];
     unsigned long long c = my_contacts, w = my_wall;
     for (int off = 32; off > 0; off >>= 1) {
       c += __shfl_xor(c, off, 64);
@@ -780,7 +814,7 @@ __global__ __launch_bounds__(kBlock) void k_step(StepArgs a) {
     __syncthreads();
     if (threadIdx.x == 0) {
       unsigned long long tc = 0, tw = 0;
-      for (int k = 0; k < kBlock / 64; k++) {
+      for (int k = 0; k < BLOCK / 64; k++) {
         tc += sc[k];
         tw += sw[k];
       }
@@ -831,12 +865,21 @@ __global__ __launch_bounds__(kBlock) void k_initial(StepArgs a) {
     atomicMin((unsigned long long *)&a.ctrl->stop_at, 1ull);
 }
 
-__global__ void k_set_stop(Ctrl *ctrl, long long v) {
+__global__ void k_set_base(Ctrl *ctrl, long long step, long long ago) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctrl->step_base = step;
+    ctrl->ago_base = ago;
+  }
+}
+
+__global__ void k_set_stop(Ctrl *ctrl, long long v, unsigned long long *inst_stats, int nstat) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0) {
     ctrl->stop_at = v;
     ctrl->ncontact_full = 0;
     ctrl->nwall = 0;
   }
+  if (inst_stats && k < nstat) inst_stats[k] = 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -844,6 +887,15 @@ __global__ void k_set_stop(Ctrl *ctrl, long long v) {
 // gather + full neighbor list with history carry-in (neigh_gran.cpp:485-639,
 // fix_contact_history.cpp:305-425)
 // ---------------------------------------------------------------------------
+// first global cell of the instance particle i belongs to (omega.w holds
+// type + instance << kTypeBits)
+__device__ __forceinline__ int inst_cell_base(const DevParams &p, const StateBuf &s, int mixed,
+                                              int i) {
+  if (p.ninst <= 1) return 0;
+  const int w = mixed ? (int)((const float4 *)s.omg)[i].w : (int)((const double4 *)s.omg)[i].w;
+  return (w >> kTypeBits) * p.ncell_local;
+}
+
 __global__ __launch_bounds__(kBlock) void k_wrap_keys(RebuildArgs a) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.n) return;
@@ -874,7 +926,7 @@ __global__ __launch_bounds__(kBlock) void k_wrap_keys(RebuildArgs a) {
   const int cy = cell_coord(p, 1, xs[1]);
   const int cz = cell_coord(p, 2, xs[2]);
   const int lin = cx + p.nc[0] * (cy + p.nc[1] * cz);
-  const int key = a.cell_rank[lin];
+  const int key = inst_cell_base(p, a.cur, a.mixed, i) + a.cell_rank[lin];
   a.key[i] = key;
   atomicAdd(&a.cell_count[key], 1);
 }
@@ -1019,6 +1071,7 @@ __global__ __launch_bounds__(kBlock) void k_build(RebuildArgs a) {
     const int io = a.perm[i];
     const int oldcnt = a.have_old ? (a.acur.meta[io] & kMetaCountMask) : 0;
     const int ci[3] = {cell_coord(p, 0, pi.x), cell_coord(p, 1, pi.y), cell_coord(p, 2, pi.z)};
+    const int cbase = inst_cell_base(p, a.alt, a.mixed, i);
     int lo[3], hi[3];
     for (int d = 0; d < 3; d++) {
       if (p.nc[d] == 1) {
@@ -1039,7 +1092,7 @@ __global__ __launch_bounds__(kBlock) void k_build(RebuildArgs a) {
           const int cx = (ci[0] + dx + p.nc[0]) % p.nc[0];
           const int cy = (ci[1] + dy + p.nc[1]) % p.nc[1];
           const int cz = (ci[2] + dz + p.nc[2]) % p.nc[2];
-          const int rank = a.cell_rank[cx + p.nc[0] * (cy + p.nc[1] * cz)];
+          const int rank = cbase + a.cell_rank[cx + p.nc[0] * (cy + p.nc[1] * cz)];
           const int s = a.cell_start[rank], e = a.cell_start[rank + 1];
           for (int j = s; j < e; j++) {
             if (j == i) continue;
@@ -1214,18 +1267,23 @@ __global__ __launch_bounds__(kBlock) void k_tri_fill(RebuildArgs a) {
   }
 }
 
+// (triangle, cell) pairs hold instance-local cell ranks; every instance sees
+// the same walls, so pair k of instance q lands in global cell
+// q * ncell_local + rank. k runs over npairs * ninst.
 __global__ __launch_bounds__(kBlock) void k_tcell_hist(RebuildArgs a, int npairs) {
   const int k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= npairs) return;
-  atomicAdd(&a.tcell_count[a.tpair_cell[k]], 1);
+  if (k >= npairs * a.prm.ninst) return;
+  const int q = k / npairs, pr = k - q * npairs;
+  atomicAdd(&a.tcell_count[q * a.prm.ncell_local + a.tpair_cell[pr]], 1);
 }
 
 __global__ __launch_bounds__(kBlock) void k_tcell_scatter(RebuildArgs a, int npairs) {
   const int k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= npairs) return;
-  const int c = a.tpair_cell[k];
+  if (k >= npairs * a.prm.ninst) return;
+  const int q = k / npairs, pr = k - q * npairs;
+  const int c = q * a.prm.ncell_local + a.tpair_cell[pr];
   const int pos = a.tcell_start[c] + atomicAdd(&a.tcell_count[c], 1);
-  a.tcell_list[pos] = a.tpair_tri[k];
+  a.tcell_list[pos] = a.tpair_tri[pr];
 }
 
 template <typename T>
@@ -1250,7 +1308,7 @@ __device__ int mesh_cand_one(const RebuildArgs &a, const int i) {
   const double4 pi = a.alt.pos[i];
   // candidates
   const int cx = cell_coord(p, 0, pi.x), cy = cell_coord(p, 1, pi.y), cz = cell_coord(p, 2, pi.z);
-  const int rank = a.cell_rank[cx + p.nc[0] * (cy + p.nc[1] * cz)];
+  const int rank = inst_cell_base(p, a.alt, a.mixed, i) + a.cell_rank[cx + p.nc[0] * (cy + p.nc[1] * cz)];
   const double lim = pi.w * p.cdf + p.skin;
   const double px[3] = {pi.x, pi.y, pi.z};
   int nc = 0;
@@ -1324,7 +1382,7 @@ __global__ __launch_bounds__(kBlock) void k_mesh_cand(RebuildArgs a) {
 // the first step of a run: meshes rotated in place
 __global__ __launch_bounds__(kBlock) void k_mesh_move(MeshMoveArgs a) {
   if (a.guard && a.step >= a.ctrl->stop_at) return;
-  mesh_move_block(a, blockIdx.x * kBlock + threadIdx.x);
+  mesh_move_block(a, blockIdx.x * kBlock + threadIdx.x, a.step);
 }
 
 // ---------------------------------------------------------------------------
@@ -1361,21 +1419,45 @@ hipError_t launch_rebuild(const RebuildArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename T, bool MESH>
+static void launch_step_variant(const StepArgs &a, int nb, hipStream_t s) {
+  const int B = kStepVariantBlock[a.variant];
+  switch (a.variant) {
+    case 1:
+      hipLaunchKernelGGL((k_step<T, MESH, 256, 2>), dim3(nb), dim3(B), 0, s, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_step<T, MESH, 256, 8>), dim3(nb), dim3(B), 0, s, a);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_step<T, MESH, 128, 4>), dim3(nb), dim3(B), 0, s, a);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_step<T, MESH, 64, 4>), dim3(nb), dim3(B), 0, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL((k_step<T, MESH, 256, 4>), dim3(nb), dim3(B), 0, s, a);
+      break;
+  }
+}
+
 hipError_t launch_step(const StepArgs &a, hipStream_t s) {
+  if (a.variant < 0 || a.variant >= kNumStepVariants) return hipErrorInvalidValue;
   const bool mesh = a.prm.nmesh > 0;
-  // pblocks = nblk(max(n, 1), kBlock), set by the caller; trailing blocks move
-  // the meshes when asked to
-  const int nb = a.pblocks + ((mesh && a.move_mesh) ? nblk(a.mv.ntri, kBlock) : 0);
+  // pblocks (tiles walked by the particle blocks) is set by the caller;
+  // trailing blocks move the meshes when asked to
+  const int B = kStepVariantBlock[a.variant];
+  const int nb = a.pblocks + ((mesh && a.move_mesh) ? nblk(a.mv.ntri, B) : 0);
   if (a.mixed) {
     if (mesh)
-      hipLaunchKernelGGL((k_step<float, true>), dim3(nb), dim3(kBlock), 0, s, a);
+      launch_step_variant<float, true>(a, nb, s);
     else
-      hipLaunchKernelGGL((k_step<float, false>), dim3(nb), dim3(kBlock), 0, s, a);
+      launch_step_variant<float, false>(a, nb, s);
   } else {
     if (mesh)
-      hipLaunchKernelGGL((k_step<double, true>), dim3(nb), dim3(kBlock), 0, s, a);
+      launch_step_variant<double, true>(a, nb, s);
     else
-      hipLaunchKernelGGL((k_step<double, false>), dim3(nb), dim3(kBlock), 0, s, a);
+      launch_step_variant<double, false>(a, nb, s);
   }
   return hipGetLastError();
 }
@@ -1416,11 +1498,12 @@ hipError_t launch_mesh_bin_fill(const RebuildArgs &a, hipStream_t s) {
   hipLaunchKernelGGL(k_tri_fill, dim3(nblk(a.ntri, kWavesPerBlock)), dim3(kBlock), 0, s, a);
   const int np = a.tpair_cap;  // exactly the pair total (sized by the caller)
   hipLaunchKernelGGL(k_zero, dim3(ncb), dim3(kBlock), 0, s, a.tcell_count, a.ncell);
-  if (np > 0) hipLaunchKernelGGL(k_tcell_hist, dim3(nblk(np, kBlock)), dim3(kBlock), 0, s, a, np);
+  const int npi = np * a.prm.ninst;
+  if (np > 0) hipLaunchKernelGGL(k_tcell_hist, dim3(nblk(npi, kBlock)), dim3(kBlock), 0, s, a, np);
   launch_scan(a.tcell_count, a.tcell_start, a.scan_tmp, a.ncell, s);
   hipLaunchKernelGGL(k_zero, dim3(ncb), dim3(kBlock), 0, s, a.tcell_count, a.ncell);
   if (np > 0)
-    hipLaunchKernelGGL(k_tcell_scatter, dim3(nblk(np, kBlock)), dim3(kBlock), 0, s, a, np);
+    hipLaunchKernelGGL(k_tcell_scatter, dim3(nblk(npi, kBlock)), dim3(kBlock), 0, s, a, np);
   if (a.n > 0) {
     if (a.mixed)
       hipLaunchKernelGGL(k_mesh_cand<float>, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, s, a);
@@ -1436,8 +1519,15 @@ hipError_t launch_mesh_move(const MeshMoveArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_set_stop(Ctrl *ctrl, long long v, hipStream_t s) {
-  hipLaunchKernelGGL(k_set_stop, dim3(1), dim3(64), 0, s, ctrl, v);
+hipError_t launch_set_base(Ctrl *ctrl, long long step, long long ago, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_base, dim3(1), dim3(64), 0, s, ctrl, step, ago);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_stop(Ctrl *ctrl, long long v, hipStream_t s, unsigned long long *inst_stats,
+                           int nstat) {
+  const int nb = inst_stats ? nblk(nstat, 256) : 1;
+  hipLaunchKernelGGL(k_set_stop, dim3(nb), dim3(256), 0, s, ctrl, v, inst_stats, nstat);
   return hipGetLastError();
 }
 

@@ -58,7 +58,13 @@ struct dem_handle {
   // materials
   int ntypes = 0;
   bool mat_set = false;
-  std::vector<double> hYeff, hGeff, hbeta, hmu, hmur;  // (ntypes+1)^2
+  std::vector<double> hYeff, hGeff, hbeta, hmu, hmur;  // ninst blocks of (ntypes+1)^2
+  bool mat_dirty = false;                               // host tables newer than the device copy
+  // independent simulations batched in this handle
+  int ninst = 1;
+  std::vector<int64_t> inst_count;  // particles of each instance, in caller order
+  int ncell_local = 0;              // cells of one instance's grid
+  unsigned long long *d_inst_stats = nullptr;  // [2][ninst]
   dem::MatTables mat{};
   void *mat_block = nullptr;
 
@@ -115,6 +121,22 @@ struct dem_handle {
   int64_t npairs = 0, ncontacts = 0, nwall_contacts = 0, nmesh_cand = 0;
   int maxcount = 0;
   bool timing = false;
+  // tile shape of the step kernel: variant (dem::kStepVariant*) and a
+  // persistent grid of `persist` blocks per compute unit (0: one block per tile)
+  int variant = 0, persist = 0, ncu = 256;
+  // step batches captured as HIP graphs, one per buffer parity and shape
+  bool use_graphs = false;
+  struct GraphEntry {
+    int key;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
+  struct Tuner {
+    bool active = false;
+    std::vector<std::pair<int, int>> cand;  // (variant, persist)
+    std::vector<double> us;                 // measured microseconds per step
+    int idx = 0, tries = 0;
+  } tuner;
   double kernel_ms = 0.0;
   int64_t kernel_launches = 0;
   double rebuild_ms = 0.0;
@@ -150,7 +172,13 @@ int dalloc(dem_handle *h, void **ptr, size_t bytes) {
   return DEM_OK;
 }
 
+void clear_graphs(dem_handle *h) {
+  for (auto &g : h->graphs) hipGraphExecDestroy(g.exec);
+  h->graphs.clear();
+}
+
 void free_particles(dem_handle *h) {
+  clear_graphs(h);
   // everything except the material block and control block
   std::vector<void *> keep;
   for (void *p : h->allocs) {
@@ -218,6 +246,9 @@ dem::DevParams make_devparams(const dem_handle *h) {
   }
   d.cutneighmax = h->cutneighmax;
   d.ntp1 = h->ntypes + 1;
+  d.mat_stride = d.ntp1 * d.ntp1;
+  d.ninst = h->ninst;
+  d.ncell_local = h->ncell_local;
   d.nmesh = (int)h->meshes.size();
   int t0 = 0;
   for (size_t m = 0; m < h->meshes.size(); m++) {
@@ -259,7 +290,7 @@ int setup_grid(dem_handle *h) {
     h->nc[d] = nc;
     total *= nc;
   }
-  if (total > (1LL << 27)) return fail(h, DEM_ERR_ARG, "cell grid too large");
+  if (total * h->ninst > (1LL << 27)) return fail(h, DEM_ERR_ARG, "cell grid too large");
   const int ncell = (int)total;
   std::vector<std::pair<uint32_t, int>> code(ncell);
   for (int cz = 0; cz < h->nc[2]; cz++)
@@ -271,15 +302,19 @@ int setup_grid(dem_handle *h) {
   std::sort(code.begin(), code.end());
   std::vector<int> rank(ncell);
   for (int r = 0; r < ncell; r++) rank[code[r].second] = r;
+  // every instance has a copy of this grid: the rank table is per instance,
+  // the counts and starts cover all instances
+  const int nglobal = ncell * h->ninst;
   int rc;
   if ((rc = dalloc(h, (void **)&h->cell_rank, sizeof(int) * ncell))) return rc;
-  if ((rc = dalloc(h, (void **)&h->cell_count, sizeof(int) * ncell))) return rc;
-  if ((rc = dalloc(h, (void **)&h->cell_start, sizeof(int) * (ncell + 1)))) return rc;
+  if ((rc = dalloc(h, (void **)&h->cell_count, sizeof(int) * nglobal))) return rc;
+  if ((rc = dalloc(h, (void **)&h->cell_start, sizeof(int) * (nglobal + 1)))) return rc;
   if ((rc = dalloc(h, (void **)&h->scan_tmp,
-                   sizeof(int) * dem::scan_tmp_size(std::max(ncell, h->ntri + 1)))))
+                   sizeof(int) * dem::scan_tmp_size(std::max(nglobal, h->ntri + 1)))))
     return rc;
   HIPCHK(h, hipMemcpy(h->cell_rank, rank.data(), sizeof(int) * ncell, hipMemcpyHostToDevice));
-  h->ncell = ncell;
+  h->ncell_local = ncell;
+  h->ncell = nglobal;
   return DEM_OK;
 }
 
@@ -348,7 +383,13 @@ dem::StepArgs step_args(dem_handle *h, int in_buf, int tri_buf) {
   a.xhold = h->xhold;
   a.fout = h->fout;
   a.ctrl = h->ctrl;
-  a.pblocks = (int)((std::max<int64_t>(h->n, 1) + dem::kBlock - 1) / dem::kBlock);
+  a.inst_stats = h->d_inst_stats;
+  a.variant = h->variant;
+  {
+    const int B = dem::kStepVariantBlock[h->variant];
+    const int ntiles = (int)((std::max<int64_t>(h->n, 1) + B - 1) / B);
+    a.pblocks = h->persist > 0 ? std::min(ntiles, h->persist * h->ncu) : ntiles;
+  }
   a.move_mesh = 0;
   if (h->ntri > 0) {
     a.mv = mesh_move_args(h, 0, 0, 1);
@@ -421,7 +462,8 @@ int rebuild(dem_handle *h) {
     HIPCHK(h, hipMemcpyAsync(h->h_total, h->d_total, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     const int total = *h->h_total;
-    if (total < 0) return fail(h, DEM_ERR_STATE, "mesh binning overflow");
+    if (total < 0 || (int64_t)total * h->ninst > 0x7fffffffLL)
+      return fail(h, DEM_ERR_STATE, "mesh binning overflow");
     if (total > h->tpair_cap) {
       // grow the (triangle, cell) pair arrays; the old ones stay allocated until
       // the particle arrays are freed
@@ -429,7 +471,9 @@ int rebuild(dem_handle *h) {
       int rc2;
       if ((rc2 = dalloc(h, (void **)&h->d_tpair_cell, sizeof(int) * cap))) return rc2;
       if ((rc2 = dalloc(h, (void **)&h->d_tpair_tri, sizeof(int) * cap))) return rc2;
-      if ((rc2 = dalloc(h, (void **)&h->d_tcell_list, sizeof(int) * cap))) return rc2;
+      // every instance gets its own copy of the pairs in its cells
+      if ((rc2 = dalloc(h, (void **)&h->d_tcell_list, sizeof(int) * (size_t)cap * h->ninst)))
+        return rc2;
       h->tpair_cap = cap;
       a.tpair_cell = h->d_tpair_cell;
       a.tpair_tri = h->d_tpair_tri;
@@ -475,7 +519,7 @@ int rebuild(dem_handle *h) {
 }
 
 int force_pass(dem_handle *h, int shearupdate) {
-  HIPCHK(h, dem::launch_set_stop(h->ctrl, kNever, h->stream));
+  HIPCHK(h, dem::launch_set_stop(h->ctrl, kNever, h->stream, h->d_inst_stats, 2 * h->ninst));
   dem::StepArgs a = step_args(h, h->cur_st, h->tri_cur);
   a.step = 0;
   a.shearupdate = shearupdate;
@@ -491,10 +535,103 @@ int force_pass(dem_handle *h, int shearupdate) {
   return DEM_OK;
 }
 
+int check_material_args(dem_handle *h, int ntypes, const double *Y, const double *nu,
+                        const double *e, const double *mu) {
+  if (ntypes < 1 || ntypes > dem::kMaxTypes || !Y || !nu || !e || !mu)
+    return fail(h, DEM_ERR_ARG, "bad material arguments");
+  // range checks of global_properties.cpp:360-372, :393-396, :316-320
+  for (int i = 0; i < ntypes; i++) {
+    if (Y[i] < 5e6)
+      return fail(h, DEM_ERR_ARG, "youngsModulus >= 5e6 required for SI units");
+    if (Y[i] > 1e9)
+      return fail(h, DEM_ERR_ARG, "youngsModulus <= 1e9 required for SI units");
+    if (nu[i] < 0. || nu[i] > 0.5) return fail(h, DEM_ERR_ARG, "0 <= poissonsRatio <= 0.5 required");
+  }
+  for (int k = 0; k < ntypes * ntypes; k++)
+    if (e[k] <= 0.05 || e[k] > 1)
+      return fail(h, DEM_ERR_ARG, "0.05 < coefficientRestitution <= 1 required");
+  return DEM_OK;
+}
+
+// mixes one material block (global_properties.cpp:426-433, :455-464, :518-541)
+void mix_block(dem_handle *h, int block, const double *Y, const double *nu, const double *e,
+               const double *mu, const double *mur) {
+  const int ntypes = h->ntypes, t1 = ntypes + 1;
+  const size_t b0 = (size_t)block * t1 * t1;
+  for (int i = 1; i <= ntypes; i++)
+    for (int j = 1; j <= ntypes; j++) {
+      const double Yi = Y[i - 1], Yj = Y[j - 1], vi = nu[i - 1], vj = nu[j - 1];
+      const size_t k = b0 + i * t1 + j;
+      h->hYeff[k] = 1. / ((1. - std::pow(vi, 2.)) / Yi + (1. - std::pow(vj, 2.)) / Yj);
+      h->hGeff[k] = 1. / (2. * (2. - vi) * (1. + vi) / Yi + 2. * (2. - vj) * (1. + vj) / Yj);
+      const double le = std::log(e[(i - 1) * ntypes + (j - 1)]);
+      h->hbeta[k] = le / std::sqrt(std::pow(le, 2.) + std::pow(kMyPi, 2.));
+      h->hmu[k] = mu[(i - 1) * ntypes + (j - 1)];
+      h->hmur[k] = mur ? mur[(i - 1) * ntypes + (j - 1)] : 0.0;
+    }
+}
+
+// device copy of the material tables (FP64 and FP32), all instance blocks
+int upload_materials(dem_handle *h) {
+  if (!h->mat_dirty) return DEM_OK;
+  clear_graphs(h);  // captured launches hold the old table pointers
+  if (h->mat_block) {
+    hipFree(h->mat_block);
+    h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), h->mat_block), h->allocs.end());
+    h->mat_block = nullptr;
+  }
+  const size_t nn = h->hYeff.size();
+  std::vector<char> blob(nn * 5 * (8 + 4));
+  double *dd = (double *)blob.data();
+  float *ff = (float *)(blob.data() + nn * 5 * 8);
+  const std::vector<double> *src[5] = {&h->hYeff, &h->hGeff, &h->hbeta, &h->hmu, &h->hmur};
+  for (int m = 0; m < 5; m++)
+    for (size_t k = 0; k < nn; k++) {
+      dd[m * nn + k] = (*src[m])[k];
+      ff[m * nn + k] = (float)(*src[m])[k];
+    }
+  int rc = dalloc(h, &h->mat_block, blob.size());
+  if (rc) return rc;
+  h->mat_block = h->allocs.back();
+  HIPCHK(h, hipMemcpy(h->mat_block, blob.data(), blob.size(), hipMemcpyHostToDevice));
+  double *db = (double *)h->mat_block;
+  float *fb = (float *)((char *)h->mat_block + nn * 5 * 8);
+  h->mat.Yeff = db;
+  h->mat.Geff = db + nn;
+  h->mat.beta = db + 2 * nn;
+  h->mat.mu = db + 3 * nn;
+  h->mat.mur = db + 4 * nn;
+  h->mat.fYeff = fb;
+  h->mat.fGeff = fb + nn;
+  h->mat.fbeta = fb + 2 * nn;
+  h->mat.fmu = fb + 3 * nn;
+  h->mat.fmur = fb + 4 * nn;
+  h->mat_dirty = false;
+  return DEM_OK;
+}
+
+// resizes the host tables to ninst blocks, repeating block 0
+void replicate_material_blocks(dem_handle *h) {
+  const size_t bs = (size_t)(h->ntypes + 1) * (h->ntypes + 1);
+  std::vector<double> *tabs[5] = {&h->hYeff, &h->hGeff, &h->hbeta, &h->hmu, &h->hmur};
+  for (auto *t : tabs) {
+    t->resize(bs * h->ninst);
+    for (int q = 1; q < h->ninst; q++) std::copy(t->begin(), t->begin() + bs, t->begin() + q * bs);
+  }
+  h->mat_dirty = true;
+}
+
+// omega.w of a particle: its type plus its instance above kTypeBits
+int type_bits(const int *type, const std::vector<int> &inst_of, int64_t i) {
+  return type[i] + (inst_of.empty() ? 0 : (inst_of[i] << dem::kTypeBits));
+}
+
 int check_ready(dem_handle *h) {
   if (!h) return DEM_ERR_ARG;
   if (!h->mat_set) return fail(h, DEM_ERR_STATE, "materials not set (fix property/global)");
   if (h->n <= 0) return fail(h, DEM_ERR_STATE, "no particles");
+  int rc = upload_materials(h);
+  if (rc) return rc;
   for (const Wall &w : h->walls)
     if (w.type < 1 || w.type > h->ntypes)
       return fail(h, DEM_ERR_ARG, "wall type out of range of the material tables");
@@ -541,6 +678,12 @@ int dem_create(const dem_params *params, dem_handle **out) {
   if (hipSetDevice(p.device) != hipSuccess) return fail(nullptr, DEM_ERR_NODEV, "hipSetDevice failed");
   dem_handle *h = new dem_handle();
   h->p = p;
+  {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p.device) == hipSuccess &&
+        ncu > 0)
+      h->ncu = ncu;
+  }
   h->mixed = p.precision == DEM_MIXED;
   h->fsz = h->mixed ? 4 : 8;
   h->W = p.max_neighbors > 0 ? p.max_neighbors : 32;
@@ -575,6 +718,7 @@ int dem_create(const dem_params *params, dem_handle **out) {
 int dem_destroy(dem_handle *h) {
   if (!h) return DEM_OK;
   if (h->stream) hipStreamSynchronize(h->stream);
+  clear_graphs(h);
   for (void *p : h->allocs) hipFree(p);
   for (auto &e : h->ev)
     if (e) hipEventDestroy(e);
@@ -588,71 +732,67 @@ int dem_destroy(dem_handle *h) {
 int dem_set_materials(dem_handle *h, int ntypes, const double *Y, const double *nu,
                       const double *e, const double *mu, const double *mur) {
   if (!h) return DEM_ERR_ARG;
-  if (ntypes < 1 || ntypes > dem::kMaxTypes || !Y || !nu || !e || !mu)
-    return fail(h, DEM_ERR_ARG, "bad material arguments");
-  // range checks of global_properties.cpp:360-372, :393-396, :316-320
-  for (int i = 0; i < ntypes; i++) {
-    if (Y[i] < 5e6)
-      return fail(h, DEM_ERR_ARG, "youngsModulus >= 5e6 required for SI units");
-    if (Y[i] > 1e9)
-      return fail(h, DEM_ERR_ARG, "youngsModulus <= 1e9 required for SI units");
-    if (nu[i] < 0. || nu[i] > 0.5) return fail(h, DEM_ERR_ARG, "0 <= poissonsRatio <= 0.5 required");
-  }
-  for (int k = 0; k < ntypes * ntypes; k++)
-    if (e[k] <= 0.05 || e[k] > 1)
-      return fail(h, DEM_ERR_ARG, "0.05 < coefficientRestitution <= 1 required");
-  const int t1 = ntypes + 1;
-  h->ntypes = ntypes;
-  h->hYeff.assign(t1 * t1, 0.0);
-  h->hGeff.assign(t1 * t1, 0.0);
-  h->hbeta.assign(t1 * t1, 0.0);
-  h->hmu.assign(t1 * t1, 0.0);
-  h->hmur.assign(t1 * t1, 0.0);
-  for (int i = 1; i <= ntypes; i++)
-    for (int j = 1; j <= ntypes; j++) {
-      const double Yi = Y[i - 1], Yj = Y[j - 1], vi = nu[i - 1], vj = nu[j - 1];
-      // global_properties.cpp:426-433, :455-464, :518-541
-      h->hYeff[i * t1 + j] = 1. / ((1. - std::pow(vi, 2.)) / Yi + (1. - std::pow(vj, 2.)) / Yj);
-      h->hGeff[i * t1 + j] = 1. / (2. * (2. - vi) * (1. + vi) / Yi + 2. * (2. - vj) * (1. + vj) / Yj);
-      const double le = std::log(e[(i - 1) * ntypes + (j - 1)]);
-      h->hbeta[i * t1 + j] = le / std::sqrt(std::pow(le, 2.) + std::pow(kMyPi, 2.));
-      h->hmu[i * t1 + j] = mu[(i - 1) * ntypes + (j - 1)];
-      h->hmur[i * t1 + j] = mur ? mur[(i - 1) * ntypes + (j - 1)] : 0.0;
-    }
-  if (h->mat_block) {
-    hipFree(h->mat_block);
-    h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), h->mat_block), h->allocs.end());
-    h->mat_block = nullptr;
-  }
-  const size_t nn = (size_t)t1 * t1;
-  std::vector<char> blob(nn * 5 * (8 + 4));
-  double *dd = (double *)blob.data();
-  float *ff = (float *)(blob.data() + nn * 5 * 8);
-  const std::vector<double> *src[5] = {&h->hYeff, &h->hGeff, &h->hbeta, &h->hmu, &h->hmur};
-  for (int m = 0; m < 5; m++)
-    for (size_t k = 0; k < nn; k++) {
-      dd[m * nn + k] = (*src[m])[k];
-      ff[m * nn + k] = (float)(*src[m])[k];
-    }
-  int rc = dalloc(h, &h->mat_block, blob.size());
+  int rc = check_material_args(h, ntypes, Y, nu, e, mu);
   if (rc) return rc;
-  h->mat_block = h->allocs.back();
-  HIPCHK(h, hipMemcpy(h->mat_block, blob.data(), blob.size(), hipMemcpyHostToDevice));
-  double *db = (double *)h->mat_block;
-  float *fb = (float *)((char *)h->mat_block + nn * 5 * 8);
-  h->mat.Yeff = db;
-  h->mat.Geff = db + nn;
-  h->mat.beta = db + 2 * nn;
-  h->mat.mu = db + 3 * nn;
-  h->mat.mur = db + 4 * nn;
-  h->mat.fYeff = fb;
-  h->mat.fGeff = fb + nn;
-  h->mat.fbeta = fb + 2 * nn;
-  h->mat.fmu = fb + 3 * nn;
-  h->mat.fmur = fb + 4 * nn;
+  if (h->n > 0 && ntypes < h->ntypes)
+    return fail(h, DEM_ERR_STATE, "fewer atom types than the particles use");
+  h->ntypes = ntypes;
+  const size_t bs = (size_t)(ntypes + 1) * (ntypes + 1);
+  std::vector<double> *tabs[5] = {&h->hYeff, &h->hGeff, &h->hbeta, &h->hmu, &h->hmur};
+  for (auto *t : tabs) t->assign(bs, 0.0);
+  mix_block(h, 0, Y, nu, e, mu, mur);
+  replicate_material_blocks(h);  // every instance starts with these materials
   h->mat_set = true;
   h->setup_done = false;
   h->can_continue = false;
+  return DEM_OK;
+}
+
+int dem_set_instances(dem_handle *h, int ninst, const int64_t *counts) {
+  if (!h) return DEM_ERR_ARG;
+  if (ninst < 1 || ninst > dem::kMaxInstances || !counts)
+    return fail(h, DEM_ERR_ARG, "bad instance arguments");
+  if (h->n > 0) return fail(h, DEM_ERR_STATE, "set instances before particles");
+  for (int q = 0; q < ninst; q++)
+    if (counts[q] <= 0) return fail(h, DEM_ERR_ARG, "every instance needs particles");
+  h->ninst = ninst;
+  h->inst_count.assign(counts, counts + ninst);
+  if (h->mat_set) replicate_material_blocks(h);
+  return DEM_OK;
+}
+
+int dem_set_instance_materials(dem_handle *h, int inst, int ntypes, const double *Y,
+                               const double *nu, const double *e, const double *mu,
+                               const double *mur) {
+  if (!h) return DEM_ERR_ARG;
+  if (!h->mat_set) return fail(h, DEM_ERR_STATE, "call dem_set_materials first");
+  if (inst < 0 || inst >= h->ninst) return fail(h, DEM_ERR_ARG, "bad instance index");
+  if (ntypes != h->ntypes) return fail(h, DEM_ERR_ARG, "every instance has the same atom types");
+  int rc = check_material_args(h, ntypes, Y, nu, e, mu);
+  if (rc) return rc;
+  mix_block(h, inst, Y, nu, e, mu, mur);
+  h->mat_dirty = true;
+  h->setup_done = false;
+  h->can_continue = false;
+  return DEM_OK;
+}
+
+int dem_get_instance_stats(dem_handle *h, int ninst, int64_t *ncontacts, int64_t *nwall) {
+  if (!h || ninst < 1) return DEM_ERR_ARG;
+  if (ninst > h->ninst) return fail(h, DEM_ERR_ARG, "more instances asked for than the system has");
+  std::vector<unsigned long long> st(2 * (size_t)h->ninst, 0ull);
+  if (h->d_inst_stats) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(st.data(), h->d_inst_stats, sizeof(unsigned long long) * st.size(),
+                        hipMemcpyDeviceToHost));
+  } else if (h->ninst == 1) {
+    st[0] = (unsigned long long)h->ncontacts * 2;
+    st[1] = (unsigned long long)h->nwall_contacts;
+  }
+  for (int q = 0; q < ninst; q++) {
+    if (ncontacts) ncontacts[q] = (int64_t)(st[q] / 2);
+    if (nwall) nwall[q] = (int64_t)st[h->ninst + q];
+  }
   return DEM_OK;
 }
 
@@ -807,6 +947,16 @@ int dem_set_particles(dem_handle *h, int64_t n, const double *x, const double *v
         return fail(h, DEM_ERR_ARG, "non-finite particle state");
     rmax = std::max(rmax, radius[i]);
   }
+  std::vector<int> inst_of;  // instance of each particle, caller order
+  if (h->ninst > 1) {
+    int64_t sum = 0;
+    for (int64_t c : h->inst_count) sum += c;
+    if (sum != n) return fail(h, DEM_ERR_ARG, "instance particle counts do not add up to n");
+    inst_of.resize(n);
+    int64_t i0 = 0;
+    for (int q = 0; q < h->ninst; q++)
+      for (int64_t k = 0; k < h->inst_count[q]; k++) inst_of[i0++] = q;
+  }
   free_particles(h);
   h->n = n;
   h->cap = (int)n;
@@ -874,6 +1024,12 @@ int dem_set_particles(dem_handle *h, int64_t n, const double *x, const double *v
   if ((rc = dalloc(h, (void **)&h->d_cand, sizeof(int) * dem::kMaxMeshCand * cap))) return rc;
   if ((rc = dalloc(h, (void **)&h->d_ncand, sizeof(int) * cap))) return rc;
   HIPCHK(h, hipMemset(h->d_ncand, 0, sizeof(int) * cap));
+  h->d_inst_stats = nullptr;
+  if (h->ninst > 1) {
+    if ((rc = dalloc(h, (void **)&h->d_inst_stats, sizeof(unsigned long long) * 2 * h->ninst)))
+      return rc;
+    HIPCHK(h, hipMemset(h->d_inst_stats, 0, sizeof(unsigned long long) * 2 * h->ninst));
+  }
 
   std::vector<double4> pos(n), vel(n);
   std::vector<int> orig(n), zero(n, 0);
@@ -887,12 +1043,14 @@ int dem_set_particles(dem_handle *h, int64_t n, const double *x, const double *v
   if (h->mixed) {
     std::vector<float4> w(n);
     for (int64_t i = 0; i < n; i++)
-      w[i] = make_float4((float)omega[3 * i], (float)omega[3 * i + 1], (float)omega[3 * i + 2], (float)type[i]);
+      w[i] = make_float4((float)omega[3 * i], (float)omega[3 * i + 1], (float)omega[3 * i + 2],
+                         (float)type_bits(type, inst_of, i));
     HIPCHK(h, hipMemcpy(h->st[0].omg, w.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
   } else {
     std::vector<double4> w(n);
     for (int64_t i = 0; i < n; i++)
-      w[i] = make_double4(omega[3 * i], omega[3 * i + 1], omega[3 * i + 2], (double)type[i]);
+      w[i] = make_double4(omega[3 * i], omega[3 * i + 1], omega[3 * i + 2],
+                          (double)type_bits(type, inst_of, i));
     HIPCHK(h, hipMemcpy(h->st[0].omg, w.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
   }
   HIPCHK(h, hipMemcpy(h->aux[0].tag, tag, sizeof(int) * n, hipMemcpyHostToDevice));
@@ -919,12 +1077,113 @@ int dem_set_kernel_timing(dem_handle *h, int on) {
   return DEM_OK;
 }
 
+int dem_set_step_shape(dem_handle *h, int variant, int persist) {
+  if (!h) return DEM_ERR_ARG;
+  if (variant < -1 || variant >= dem::kNumStepVariants || persist < 0 || persist > 64)
+    return fail(h, DEM_ERR_ARG, "bad step kernel shape");
+  clear_graphs(h);
+  auto &tu = h->tuner;
+  if (variant == -1) {
+    tu = dem_handle::Tuner{};
+    tu.active = true;
+    for (int v = 0; v < dem::kNumStepVariants; v++)
+      for (int f : {0, 4, 8}) tu.cand.push_back({v, f});
+    tu.us.assign(tu.cand.size(), 0.0);
+    return DEM_OK;
+  }
+  tu.active = false;
+  h->variant = variant;
+  h->persist = persist;
+  return DEM_OK;
+}
+
+int dem_set_graph_capture(dem_handle *h, int on) {
+  if (!h) return DEM_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  clear_graphs(h);
+  h->use_graphs = on != 0;
+  return DEM_OK;
+}
+
+int dem_get_step_shape(dem_handle *h, int *variant, int *persist, int cap, int *ncand,
+                       int *cand_variant, int *cand_persist, double *cand_us) {
+  if (!h) return DEM_ERR_ARG;
+  if (variant) *variant = h->variant;
+  if (persist) *persist = h->persist;
+  const auto &tu = h->tuner;
+  if (ncand) *ncand = (int)tu.cand.size();
+  for (int k = 0; k < (int)tu.cand.size() && k < cap; k++) {
+    if (cand_variant) cand_variant[k] = tu.cand[k].first;
+    if (cand_persist) cand_persist[k] = tu.cand[k].second;
+    if (cand_us) cand_us[k] = tu.us[k];
+  }
+  return DEM_OK;
+}
+
 }  // extern "C"
 
 namespace {
 
 // Integrate nsteps steps continuing from the last force pass (whose f and
 // torque are in fout, in the current particle order).
+constexpr int kTuneBatch = 8;   // steps per candidate tile shape
+constexpr int kGraphBatch = 32; // steps per captured graph
+
+// The graph of kGraphBatch consecutive (non-final) steps starting with state
+// buffer `st` and geometry buffer `tc`, captured on first use. Its launches
+// take their step index from the control block (launch_set_base).
+int step_graph(dem_handle *h, int st, int tc, bool moving, hipGraphExec_t *out) {
+  const int key = st | (tc << 1) | (h->cur_aux << 2) | (h->variant << 3) | (h->persist << 6) |
+                  ((moving ? 1 : 0) << 13);
+  for (auto &g : h->graphs)
+    if (g.key == key) {
+      *out = g.exec;
+      return DEM_OK;
+    }
+  HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t err = hipSuccess;
+  for (int b = 0; b < kGraphBatch && err == hipSuccess; b++) {
+    dem::StepArgs a = step_args(h, (st + b) & 1, (tc + b) & 1);
+    a.step = b;
+    a.step_from_ctrl = 1;
+    a.shearupdate = 1;
+    a.do_final = 1;
+    a.do_initial = 1;
+    a.store_f = 0;
+    a.count = 0;
+    a.move_mesh = moving ? 1 : 0;
+    err = dem::launch_step(a, h->stream);
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(h->stream, &graph);
+  if (err != hipSuccess) return fail(h, DEM_ERR_HIP, hipGetErrorString(err));
+  if (e2 != hipSuccess) return fail(h, DEM_ERR_HIP, hipGetErrorString(e2));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e3 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e3 != hipSuccess) return fail(h, DEM_ERR_HIP, hipGetErrorString(e3));
+  h->graphs.push_back({key, exec});
+  *out = exec;
+  return DEM_OK;
+}
+
+// picks the fastest measured tile shape
+void finish_tuning(dem_handle *h) {
+  auto &tu = h->tuner;
+  int best = -1;
+  for (int k = 0; k < (int)tu.cand.size(); k++)
+    if (tu.us[k] > 0.0 && (best < 0 || tu.us[k] < tu.us[best])) best = k;
+  clear_graphs(h);
+  if (best >= 0) {
+    h->variant = tu.cand[best].first;
+    h->persist = tu.cand[best].second;
+  } else {
+    h->variant = 0;
+    h->persist = 0;
+  }
+  tu.active = false;
+}
+
 int run_steps(dem_handle *h, int64_t nsteps) {
   int rc;
   if (nsteps == 0) return DEM_OK;
@@ -932,7 +1191,8 @@ int run_steps(dem_handle *h, int64_t nsteps) {
   h->can_continue = false;
   // a rebuild a moving mesh asked for at the first step of this run
   // (FixMesh::pre_force set next_reneighbor on the last step of the previous run)
-  HIPCHK(h, dem::launch_set_stop(h->ctrl, h->pending_mesh_rebuild ? 1 : kNever, h->stream));
+  HIPCHK(h, dem::launch_set_stop(h->ctrl, h->pending_mesh_rebuild ? 1 : kNever, h->stream,
+                                      h->d_inst_stats, 2 * h->ninst));
   h->pending_mesh_rebuild = false;
   const bool moving = any_moving_mesh(h);
   const int64_t ts0 = h->ntimestep;  // step t of this run is timestep ts0 + t
@@ -947,9 +1207,30 @@ int run_steps(dem_handle *h, int64_t nsteps) {
   int tc = h->tri_cur;  // geometry buffer of step t
   int64_t t = 1;
   int batch = 16;
+  auto &tu = h->tuner;
   while (t <= nsteps) {
-    const int nb = (int)std::min<int64_t>(batch, nsteps - t + 1);
-    for (int b = 0; b < nb; b++) {
+    // while tuning, each batch runs one candidate tile shape; every shape
+    // computes the same result, so these are ordinary steps of the run
+    const bool tuning = tu.active && tu.idx < (int)tu.cand.size();
+    if (tuning) {
+      h->variant = tu.cand[tu.idx].first;
+      h->persist = tu.cand[tu.idx].second;
+    }
+    const bool events = h->timing || tuning;
+    int nb = (int)std::min<int64_t>(tuning ? kTuneBatch : batch, nsteps - t + 1);
+    // whole graphs of kGraphBatch steps, none of them the run's last step
+    const bool graphed = h->use_graphs && !events && nb >= kGraphBatch && t + nb - 1 < nsteps;
+    if (graphed) {
+      nb -= nb % kGraphBatch;
+      for (int g0 = 0; g0 < nb; g0 += kGraphBatch) {
+        const int64_t step0 = t + g0;
+        HIPCHK(h, dem::launch_set_base(h->ctrl, step0, ts0 + step0 + 1 - h->last_build, h->stream));
+        hipGraphExec_t exec = nullptr;
+        if ((rc = step_graph(h, (h->cur_st + g0) & 1, (tc + g0) & 1, moving, &exec))) return rc;
+        HIPCHK(h, hipGraphLaunch(exec, h->stream));
+      }
+    }
+    for (int b = 0; b < nb && !graphed; b++) {
       const int64_t step = t + b;
       const bool last = step == nsteps;
       dem::StepArgs a = step_args(h, (h->cur_st + b) & 1, (tc + b) & 1);
@@ -966,9 +1247,9 @@ int run_steps(dem_handle *h, int64_t nsteps) {
       // the start event goes after the batch's first launch: the GPU is then
       // busy while the host enqueues the rest, so the pair times the kernels
       // and not the latency of the first launch
-      if (h->timing && b == 0) HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
+      if (events && b == 0) HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
     }
-    if (h->timing) HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+    if (events) HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
     if ((rc = read_ctrl(h))) return rc;
     if (h->ctrl_host->slot_overflow)
       return fail(h, DEM_ERR_OVERFLOW, "more simultaneous mesh contacts on a particle than history slots");
@@ -981,12 +1262,23 @@ int run_steps(dem_handle *h, int64_t nsteps) {
     // Timing brackets launches 1..nb-1 of the batch with one event pair
     // (events between launches would add their own gaps); batches that ended
     // in a rebuild request contain skipped launches and are left out.
-    if (h->timing && executed == nb && nb > 1) {
+    if (events && executed == nb && nb > 1) {
       float ms = 0.f;
       HIPCHK(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
-      h->kernel_ms += ms;
-      h->kernel_launches += nb - 1;
+      if (h->timing) {
+        h->kernel_ms += ms;
+        h->kernel_launches += nb - 1;
+      }
+      if (tuning) {
+        tu.us[tu.idx] = 1e3 * ms / (nb - 1);
+        tu.idx++;
+        tu.tries = 0;
+      }
+    } else if (tuning && ++tu.tries >= 3) {
+      tu.idx++;  // interrupted by rebuilds three times: leave this shape out
+      tu.tries = 0;
     }
+    if (tuning && tu.idx == (int)tu.cand.size()) finish_tuning(h);
     const bool ran_last = t + executed - 1 == nsteps;  // the last step moves no mesh
     h->cur_st = (h->cur_st + executed) & 1;
     h->ntimestep += executed;
@@ -997,7 +1289,7 @@ int run_steps(dem_handle *h, int64_t nsteps) {
       h->tri_cur = tc;
       if ((rc = rebuild(h))) return rc;
       h->last_build = ts0 + stop;  // the build belongs to step `stop` (its initial integrate is done)
-      HIPCHK(h, dem::launch_set_stop(h->ctrl, kNever, h->stream));
+      HIPCHK(h, dem::launch_set_stop(h->ctrl, kNever, h->stream, h->d_inst_stats, 2 * h->ninst));
       batch = std::max(4, batch / 2);
     } else {
       batch = std::min(256, batch * 2);

@@ -23,7 +23,9 @@ EXPORTED = [
     "dem_run_continue", "dem_compute_forces", "dem_get_state", "dem_get_pairs",
     "dem_get_wall_history", "dem_get_stats", "dem_set_kernel_timing",
     "dem_run_script", "dem_add_mesh", "dem_mesh_rotate", "dem_get_mesh_nodes",
-    "dem_get_mesh_topology", "dem_get_mesh_history",
+    "dem_get_mesh_topology", "dem_get_mesh_history", "dem_set_instances",
+    "dem_set_instance_materials", "dem_get_instance_stats", "dem_set_step_shape",
+    "dem_get_step_shape", "dem_set_graph_capture",
 ]
 
 
@@ -120,6 +122,12 @@ def load_library(path=LIB_PATH):
     lib.dem_get_mesh_nodes.argtypes = [h, ctypes.c_int, dp, dp]
     lib.dem_get_mesh_topology.argtypes = [h, ctypes.c_int, ip, ip, ip]
     lib.dem_get_mesh_history.argtypes = [h, ctypes.c_int, ctypes.c_int, ip, dp]
+    lib.dem_set_instances.argtypes = [h, ctypes.c_int, P(ctypes.c_int64)]
+    lib.dem_set_instance_materials.argtypes = [h, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp]
+    lib.dem_get_instance_stats.argtypes = [h, ctypes.c_int, P(ctypes.c_int64), P(ctypes.c_int64)]
+    lib.dem_set_step_shape.argtypes = [h, ctypes.c_int, ctypes.c_int]
+    lib.dem_set_graph_capture.argtypes = [h, ctypes.c_int]
+    lib.dem_get_step_shape.argtypes = [h, ip, ip, ctypes.c_int, ip, ip, ip, dp]
     _lib = lib
     return lib
 
@@ -136,13 +144,30 @@ class DemError(RuntimeError):
     pass
 
 
+def _material_args(materials):
+    """(ntypes, Y, nu, e, mu, mur) pointers for dem_set_materials; the arrays
+    are kept alive on the returned tuple's owner by the caller's frame."""
+    nt = materials["ntypes"]
+    Y = np.ascontiguousarray(materials["youngs"], dtype=np.float64)
+    nu = np.ascontiguousarray(materials["poisson"], dtype=np.float64)
+    e = np.ascontiguousarray(materials["restitution"], dtype=np.float64).reshape(-1)
+    mu = np.ascontiguousarray(materials["friction"], dtype=np.float64).reshape(-1)
+    mur = materials.get("rolling_friction")
+    mur = None if mur is None else np.ascontiguousarray(mur, dtype=np.float64).reshape(-1)
+    _material_args.keep = (Y, nu, e, mu, mur)
+    return (nt, _dp(Y), _dp(nu), _dp(e), _dp(mu), _dp(mur) if mur is not None else None)
+
+
 class GpuSystem:
     """One granular system on the GPU (a dem_handle)."""
 
     def __init__(self, params, materials, particles, walls=(), precision=DEM_FP64,
-                 max_neighbors=0, device=0, meshes=()):
+                 max_neighbors=0, device=0, meshes=(), instances=None, instance_materials=None):
         """meshes: [(tris (T,3,3), wall_type, model dict or None, rotate or None)],
-        rotate = (origin, axis, period) for fix move/mesh rotate."""
+        rotate = (origin, axis, period) for fix move/mesh rotate.
+        instances: particle counts of independent simulations batched in this
+        system (particles concatenated in that order); instance_materials: one
+        material dict per instance."""
         self.lib = load_library()
         p = DemParams()
         p.dt = params["dt"]
@@ -172,15 +197,15 @@ class GpuSystem:
         if rc != 0:
             raise DemError(f"dem_create failed ({rc}): {self.lib.dem_last_error(None).decode()}")
         self.precision = precision
-        nt = materials["ntypes"]
-        Y = np.ascontiguousarray(materials["youngs"], dtype=np.float64)
-        nu = np.ascontiguousarray(materials["poisson"], dtype=np.float64)
-        e = np.ascontiguousarray(materials["restitution"], dtype=np.float64).reshape(-1)
-        mu = np.ascontiguousarray(materials["friction"], dtype=np.float64).reshape(-1)
-        mur = materials.get("rolling_friction")
-        mur = None if mur is None else np.ascontiguousarray(mur, dtype=np.float64).reshape(-1)
-        self._check(self.lib.dem_set_materials(self.h, nt, _dp(Y), _dp(nu), _dp(e), _dp(mu),
-                                               _dp(mur) if mur is not None else None))
+        self._check(self.lib.dem_set_materials(self.h, *_material_args(materials)))
+        self.ninst = 1
+        if instances is not None:
+            counts = np.ascontiguousarray(instances, dtype=np.int64)
+            self.ninst = len(counts)
+            self._check(self.lib.dem_set_instances(
+                self.h, self.ninst, counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+            for q, m in enumerate(instance_materials or ()):
+                self._check(self.lib.dem_set_instance_materials(self.h, q, *_material_args(m)))
         self.nwalls = 0
         for wall in walls:
             axis, pos, wtype = wall[:3]
@@ -278,6 +303,35 @@ class GpuSystem:
         hist = np.zeros((self.n, maxslot, 3))
         self._check(self.lib.dem_get_mesh_history(self.h, k, maxslot, _ip(tri), _dp(hist)))
         return tri, hist
+
+    def set_step_shape(self, variant, persist=0):
+        """Tile shape of the step kernel; variant -1 autotunes during the next run."""
+        self._check(self.lib.dem_set_step_shape(self.h, int(variant), int(persist)))
+
+    def set_graph_capture(self, on):
+        """Replay step batches as captured HIP graphs."""
+        self._check(self.lib.dem_set_graph_capture(self.h, int(bool(on))))
+
+    def step_shape(self):
+        """(variant, persist, [(variant, persist, us_per_step) per tuned candidate])."""
+        v, f, nc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        cap = 64
+        cv = np.zeros(cap, dtype=np.int32)
+        cf = np.zeros(cap, dtype=np.int32)
+        cu = np.zeros(cap)
+        self._check(self.lib.dem_get_step_shape(self.h, ctypes.byref(v), ctypes.byref(f), cap,
+                                                ctypes.byref(nc), _ip(cv), _ip(cf), _dp(cu)))
+        n = min(nc.value, cap)
+        return v.value, f.value, [(int(cv[k]), int(cf[k]), float(cu[k])) for k in range(n)]
+
+    def instance_stats(self):
+        """(touching pairs, wall contacts) per instance at the last step."""
+        c = np.zeros(self.ninst, dtype=np.int64)
+        w = np.zeros(self.ninst, dtype=np.int64)
+        P = ctypes.POINTER(ctypes.c_int64)
+        self._check(self.lib.dem_get_instance_stats(self.h, self.ninst, c.ctypes.data_as(P),
+                                                    w.ctypes.data_as(P)))
+        return c, w
 
     def stats(self):
         s = DemStats()

@@ -196,3 +196,68 @@ def write_data_file(path, params, particles, ntypes=1, density=2500.0):
         for i in range(len(r)):
             f.write(f"{tag[i]} {v[i][0]!r} {v[i][1]!r} {v[i][2]!r} "
                     f"{w[i][0]!r} {w[i][1]!r} {w[i][2]!r}\n")
+
+
+def heap_sweep(seed=0, ninst=512, nmin=2000, nmax=50000, r=0.0015, floor=0.1, dt=1e-5,
+               skin=0.001, footprint=0.09, counts=None):
+    """Batched calibration sweep: `ninst` independent heap-formation runs.
+    Instance q has N_q spheres (log-uniform in [nmin, nmax], or `counts`) of
+    radius r on a jittered cubic lattice (spacing 2.2 r) over a centred
+    footprint x footprint patch, starting just above a finite floor x floor
+    square at z = 0 (two triangles, fix mesh/surface, wall type 2) with open
+    sides, falling under gravity. Materials per instance: Y 5e6, nu 0.45,
+    friction U(0.2, 0.8) (particle-particle and particle-wall), restitution
+    U(0.1, 0.9), CDT rolling friction U(0, 0.3). All instances share one box
+    (x, y in +-2 floor, z from -floor/2 up to the tallest column) and local
+    coordinates.
+    Returns (params, base_materials, particles, walls, meshes, counts,
+    instance_materials)."""
+    import meshes as msh
+    rng = np.random.default_rng(seed)
+    if counts is None:
+        counts = np.exp(rng.uniform(math.log(nmin), math.log(nmax), ninst)).round().astype(np.int64)
+    counts = np.asarray(counts, dtype=np.int64)
+    a = 2.2 * r
+    nside = max(1, int(footprint / a))
+    per_layer = nside * nside
+    z0 = r + 0.25 * a
+    tallest = int(math.ceil(counts.max() / per_layer))
+    box_hi = (floor, floor, z0 + tallest * a + a)
+    box_lo = (-floor, -floor, -0.5 * floor)
+    xs, vs, rads, tags, inst_mats = [], [], [], [], []
+    c0 = -0.5 * (nside - 1) * a
+    for q, n in enumerate(counts):
+        k = np.arange(n)
+        layer, rem = k // per_layer, k % per_layer
+        ix, iy = rem % nside, rem // nside
+        x = np.stack([c0 + ix * a, c0 + iy * a, z0 + layer * a], axis=1).astype(np.float64)
+        x += rng.uniform(-0.05 * (a - 2 * r), 0.05 * (a - 2 * r), x.shape)
+        xs.append(x)
+        vs.append(np.zeros((n, 3)))
+        rads.append(np.full(n, r))
+        tags.append(np.arange(1, n + 1, dtype=np.int32))
+        mu = float(rng.uniform(0.2, 0.8))
+        e = float(rng.uniform(0.1, 0.9))
+        mur = float(rng.uniform(0.0, 0.3))
+        inst_mats.append({"ntypes": 2, "youngs": [5e6, 5e6], "poisson": [0.45, 0.45],
+                          "restitution": [[e, e], [e, e]], "friction": [[mu, mu], [mu, mu]],
+                          "rolling_friction": [[mur, mur], [mur, mur]]})
+    rad = np.concatenate(rads)
+    N = len(rad)
+    particles = {"x": np.concatenate(xs), "v": np.concatenate(vs), "omega": np.zeros((N, 3)),
+                 "radius": rad, "rmass": sphere_mass(rad, 2500.0),
+                 "type": np.ones(N, dtype=np.int32), "tag": np.concatenate(tags)}
+    params = {"dt": dt, "skin": skin, "cdf": 1.0, "boxlo": box_lo, "boxhi": box_hi,
+              "periodic": (0, 0, 0), "grav_acc": (0.0, 0.0, -9.81), "gravity_first": 1,
+              "rolling_cdt": 1}
+    tris = msh.grid(-0.5 * floor, 0.5 * floor, -0.5 * floor, 0.5 * floor, 0.0, 1, 1, diag="same")
+    meshes = [(tris, 2, {"rolling_cdt": 1}, None)]
+    base = inst_mats[0]
+    return params, base, particles, [], meshes, counts, inst_mats
+
+
+def instance_slice(particles, counts, q):
+    """Particles of instance q of a batched system, as a stand-alone set."""
+    off = int(np.sum(counts[:q]))
+    n = int(counts[q])
+    return {k: v[off:off + n] for k, v in particles.items()}

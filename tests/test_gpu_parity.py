@@ -175,6 +175,44 @@ def test_neigh_modify_every_check_fp64(every, check):
     g.close()
 
 
+def test_step_shapes_are_bitwise_equal():
+    """Every tile shape of the step kernel (threads per block, neighbor loads
+    in flight, persistent grid) and an autotuned run give bitwise the same
+    state: they only change which thread walks which particles."""
+    say("[gpu] step kernel shapes")
+    params, mats, P, walls = wl.dense_pack(seed=33, n=3000, rolling=True, speed=0.3)
+    ref = None
+    shapes = [(v, f) for v in range(5) for f in (0, 4)] + [(-1, 0)]
+    for v, f in shapes:
+        for prec in (pydem.DEM_FP64,):
+            g = pydem.GpuSystem(params, mats, P, walls, precision=prec)
+            g.set_step_shape(v, f)
+            g.run(160 if v == -1 else 40)
+            if v == -1:
+                chosen = g.step_shape()
+                say(f"  autotuned: variant {chosen[0]} persist {chosen[1]}, "
+                    f"{sum(1 for c in chosen[2] if c[2] > 0)} candidates timed")
+                g.close()
+                continue
+            st = g.state()
+            if ref is None:
+                ref = st
+            for k in ("x", "v", "omega", "f", "torque"):
+                assert np.array_equal(st[k], ref[k]), (v, f, k)
+            g.close()
+    # the autotuned run against the default shape over the same 160 steps
+    a = pydem.GpuSystem(params, mats, P, walls)
+    a.run(160)
+    b = pydem.GpuSystem(params, mats, P, walls)
+    b.set_step_shape(-1, 0)
+    b.run(160)
+    sa, sb = a.state(), b.state()
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert np.array_equal(sa[k], sb[k]), k
+    a.close()
+    b.close()
+
+
 def test_run_continue_is_bitwise_one_run():
     say("[gpu] run + run_continue == one run (bitwise)")
     params, mats, P, walls = wl.dense_pack(seed=31, n=1500, rolling=True, speed=0.3)
@@ -519,3 +557,35 @@ def test_rotating_drum_run_in_pieces():
     assert np.array_equal(n1, n2) and np.array_equal(n2, on) and np.array_equal(v2, ov)
     g1.close()
     g2.close()
+
+
+@pytest.mark.parametrize("case", ["pack", "drum"])
+def test_graph_capture_is_bitwise_eager(case):
+    """Steps replayed from captured HIP graphs (32 per replay, rebuilds and
+    other eager work between replays, several `run pre no` pieces) give
+    bitwise the state of step-by-step launches."""
+    say(f"[gpu] graph capture {case}")
+    if case == "pack":
+        params, mats, P, walls = wl.dense_pack(seed=34, n=3000, rolling=True, speed=0.3)
+        meshes = ()
+        pieces = [150, 70, 180]
+    else:
+        params, mats, P, walls, meshes = _small_drum()
+        pieces = [200, 33, 300]
+    a = pydem.GpuSystem(params, mats, P, walls, meshes=meshes)
+    b = pydem.GpuSystem(params, mats, P, walls, meshes=meshes)
+    b.set_graph_capture(True)
+    a.run(pieces[0])
+    b.run(pieces[0])
+    for k in pieces[1:]:
+        a.run_continue(k)
+        b.run_continue(k)
+    sa, sb = a.state(), b.state()
+    say(f"  rebuilds eager={a.stats()['nrebuilds']} graph={b.stats()['nrebuilds']}")
+    assert a.stats()["nrebuilds"] == b.stats()["nrebuilds"]
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert np.array_equal(sa[k], sb[k]), k
+    if meshes:
+        assert np.array_equal(a.mesh_nodes()[0], b.mesh_nodes()[0])
+    a.close()
+    b.close()
