@@ -8,6 +8,15 @@ all particles (pair forces with history, walls, gravity, nve/sphere, neighbor
 decide and any rebuild it triggers). Inputs are resident in HBM before the
 timed region.
 
+`--workload sweep` runs a batched calibration sweep: 512 independent
+heap-formation simulations (N log-uniform in 2k-50k, r = 1.5 mm, finite
+0.1 x 0.1 m mesh floor with open sides, per-instance friction U(0.2, 0.8),
+restitution U(0.1, 0.9), rolling friction U(0, 0.3), mixed precision,
+dt 1e-5 s) advanced together by one step kernel per step, with an autotuned
+tile shape and graph-replayed step batches; it reports the aggregate rate
+and, for a sample of instances, the kinetic-energy and contact-count error
+against the same instance run alone in FP64.
+
 `--workload drum` runs the rotating-drum case instead (closed cylinder
 R = 0.15 m, L = 0.1 m, 64 x 16 mantle quads split into triangles + fan end
 caps = 2,176 triangles turning at 30 rpm via move/mesh rotate, ~304k spheres
@@ -55,9 +64,15 @@ def algorithmic_bytes(n, npairs, ncontacts, nwall, mixed, ntri=0, ncand=0):
     return B
 
 
-def run_mode(params, mats, P, walls, precision, steps, warmup, timing_steps, meshes=()):
+def run_mode(params, mats, P, walls, precision, steps, warmup, timing_steps, meshes=(),
+             instances=None, instance_materials=None, tune=False, graphs=False, keep=False):
     import pydem
-    g = pydem.GpuSystem(params, mats, P, walls, precision=precision, meshes=meshes)
+    g = pydem.GpuSystem(params, mats, P, walls, precision=precision, meshes=meshes,
+                        instances=instances, instance_materials=instance_materials)
+    if tune:
+        g.set_step_shape(-1)
+    if graphs:
+        g.set_graph_capture(True)
     g.run(warmup)
     sync()
     t0 = time.perf_counter()
@@ -69,7 +84,12 @@ def run_mode(params, mats, P, walls, precision, steps, warmup, timing_steps, mes
     g.run_continue(timing_steps)
     st = g.stats()
     g.set_kernel_timing(0)
-    g.close()
+    shape = g.step_shape()
+    extra = {}
+    if keep:
+        extra["system"] = g
+    else:
+        g.close()
     n = P["x"].shape[0]
     mixed = precision == pydem.DEM_MIXED
     ntri = sum(len(m[0]) for m in meshes)
@@ -87,6 +107,8 @@ def run_mode(params, mats, P, walls, precision, steps, warmup, timing_steps, mes
         "nmesh_candidates": st["nmesh_candidates"],
         "nrebuilds_timed": st0["nrebuilds"],
         "rebuild_ms_total_timing_pass": st["rebuild_ms_total"],
+        "step_shape": {"variant": shape[0], "persist": shape[1]},
+        **extra,
     }
 
 
@@ -114,6 +136,34 @@ def cpu_baseline(params, mats, P, walls, nsteps, meshes=()):
                       f"{nsteps} steps after setup ({dt:.1f} s)"}
 
 
+def sweep_vs_alone(params, P, walls, meshes, counts, imats, g, nsteps, sample):
+    """KE and contact-count error of sampled instances of the batched (mixed)
+    run against the same instance run alone in FP64 for the same steps."""
+    import numpy as np
+    import pydem
+    import workloads as wl
+    st = g.state()
+    bc, bw = g.instance_stats()
+    rows = []
+    for q in sample:
+        off = int(np.sum(counts[:q]))
+        n = int(counts[q])
+        Pq = wl.instance_slice(P, counts, q)
+        a = pydem.GpuSystem(params, imats[q], Pq, walls, meshes=meshes)
+        a.run(nsteps)
+        sa = a.state()
+        s2 = a.stats()
+        a.close()
+        ke_b = 0.5 * float(Pq["rmass"] @ (st["v"][off:off + n] ** 2).sum(1))
+        ke_a = 0.5 * float(Pq["rmass"] @ (sa["v"] ** 2).sum(1))
+        ca = s2["ncontacts"] + s2["nwall_contacts"]
+        cb = int(bc[q] + bw[q])
+        rows.append({"instance": int(q), "n": n, "ke_rel_err": abs(ke_b - ke_a) / ke_a,
+                     "contacts_batched": cb, "contacts_alone_fp64": ca,
+                     "contact_rel_err": abs(cb - ca) / max(ca, 1)})
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,7 +174,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default=None)
-    ap.add_argument("--workload", choices=("settling", "drum"), default="settling")
+    ap.add_argument("--workload", choices=("settling", "drum", "sweep"), default="settling")
+    ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--sample", type=int, default=8,
+                    help="sweep: instances rerun alone in FP64 for the error report")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -135,7 +188,19 @@ def main():
     import pydem
     import workloads as wl
     meshes = ()
-    if args.workload == "drum":
+    inst = imats = None
+    tune = graphs = False
+    if args.workload == "sweep":
+        params, mats, P, walls, meshes, inst, imats = wl.heap_sweep(seed=0, ninst=args.instances)
+        n = P["x"].shape[0]
+        workload = (f"batched heap-formation sweep: {len(inst)} instances, {n} spheres in all "
+                    f"(N per instance {int(inst.min())}-{int(inst.max())}, log-uniform 2k-50k), "
+                    "r 1.5 mm, 0.1 m mesh floor with open sides, per-instance mu/e/mu_r, "
+                    "dt 1e-5 s")
+        data = "synthetic (seeded jittered lattices, per-instance seeded materials)"
+        modes = args.modes or "mixed"
+        tune = graphs = True
+    elif args.workload == "drum":
         params, mats, P, walls, meshes = wl.drum(seed=0)
         n = P["x"].shape[0]
         workload = (f"rotating drum, {n} spheres r 1 mm +-10%, 2,176-triangle mesh/surface "
@@ -151,15 +216,30 @@ def main():
         data = "synthetic (seeded jittered lattice, 1M polydisperse spheres)"
         modes = args.modes or "fp64,mixed"
     res = {}
+    sweep_rows = None
     for mode in modes.split(","):
         prec = pydem.DEM_MIXED if mode == "mixed" else pydem.DEM_FP64
         res[mode] = run_mode(params, mats, P, walls, prec, args.steps, args.warmup,
-                             args.timing_steps, meshes)
+                             args.timing_steps, meshes, inst, imats, tune, graphs,
+                             keep=inst is not None)
+        if inst is not None:
+            g = res[mode].pop("system")
+            import numpy as np
+            order = np.argsort(inst)
+            pick = order[np.linspace(0, len(order) - 1, min(args.sample, len(order))).astype(int)]
+            sweep_rows = sweep_vs_alone(params, P, walls, meshes, inst, imats, g,
+                                        args.warmup + args.steps + args.timing_steps, pick)
+            g.close()
         print(f"# {mode}: {res[mode]}", file=sys.stderr, flush=True)
     head = res.get("mixed") or res.get("fp64")
     head_mode = "mixed" if "mixed" in res else "fp64"
+    metric = "particle-timesteps/s at 1M spheres, mixed vs FP64 GPU; HBM GB/s of pair kernel"
+    if args.workload == "sweep":
+        metric = "aggregate particle-timesteps/s, batched heap-formation sweep"
+    elif args.workload == "drum":
+        metric = "particle-timesteps/s, rotating drum with mesh walls"
     out = {
-        "metric": "particle-timesteps/s at 1M spheres, mixed vs FP64 GPU; HBM GB/s of pair kernel",
+        "metric": metric,
         "value": head["throughput"],
         "unit": "particle-timesteps/s",
         "n_gpus": 1,
@@ -180,6 +260,13 @@ def main():
     }
     if "mixed" in res and "fp64" in res:
         out["mixed_over_fp64"] = res["mixed"]["throughput"] / res["fp64"]["throughput"]
+    out["step_shape"] = head["step_shape"]
+    if sweep_rows is not None:
+        out["instances_vs_alone_fp64"] = {
+            "sample": sweep_rows,
+            "max_ke_rel_err": max(r["ke_rel_err"] for r in sweep_rows),
+            "max_contact_rel_err": max(r["contact_rel_err"] for r in sweep_rows),
+        }
     out["roofline"] = {
         "kernel": "dem::k_step (fused pair + walls + gravity + nve/sphere)"
                   + (" with mesh walls" if meshes else ""),
@@ -193,7 +280,15 @@ def main():
         "kernel_ms": head["kernel_ms"],
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(params, mats, P, walls, args.cpu_steps, meshes)
+        if inst is not None:
+            # the oracle on the median-size instance alone (instances are independent,
+            # so the sweep's CPU rate is this per-instance rate)
+            import numpy as np
+            q = int(np.argsort(inst)[len(inst) // 2])
+            out["cpu_baseline"] = cpu_baseline(params, imats[q], wl.instance_slice(P, inst, q),
+                                               walls, args.cpu_steps, meshes)
+        else:
+            out["cpu_baseline"] = cpu_baseline(params, mats, P, walls, args.cpu_steps, meshes)
     print(json.dumps(out), flush=True)
 
 

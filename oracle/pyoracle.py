@@ -322,3 +322,55 @@ class OracleSystem:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# superquadric contact point (sq_oracle.c)
+# ---------------------------------------------------------------------------
+SQ_POSE_DOUBLES = 12  # center[3], quat[4] (w,i,j,k), shape[3], blockiness[2]
+
+
+def _sq_lib():
+    lib = load()
+    if not getattr(lib, "_sq_ready", False):
+        P = ctypes.POINTER
+        dp, ip = P(_D), P(ctypes.c_int)
+        lib.orc_sq_props.argtypes = [dp, dp, dp, dp, dp]
+        lib.orc_sq_contact.argtypes = [dp, dp, dp, dp, dp, dp, ip]
+        lib.orc_sq_contact.restype = ctypes.c_int
+        lib.orc_sq_contact_batch.argtypes = [ctypes.c_int64, dp, dp, dp, dp, dp, dp, ip, ip]
+        lib._sq_ready = True
+    return lib
+
+
+def sq_props(pose, x, hessian=True):
+    """Shape function value, gradient and Hessian of one superquadric pose
+    (12 doubles) at global point x."""
+    lib = _sq_lib()
+    pose = np.ascontiguousarray(pose, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    f = np.zeros(1)
+    g = np.zeros(3)
+    h = np.zeros(9) if hessian else None
+    lib.orc_sq_props(_dp(pose), _dp(x), _dp(f), _dp(g), _dp(h) if h is not None else None)
+    return f[0], g, (h.reshape(3, 3) if h is not None else None)
+
+
+def sq_contact_batch(A, B, init=None):
+    """Contact points of n pose pairs (A, B: (n, 12)); init None = fresh
+    contacts (homotopy), else (n, 3) starting points. Returns
+    (point (n,3), fi, fj, newton_iterations, fail)."""
+    lib = _sq_lib()
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    n = A.shape[0]
+    pt = np.zeros((n, 3))
+    fi = np.zeros(n)
+    fj = np.zeros(n)
+    it = np.zeros(n, dtype=np.int32)
+    fl = np.zeros(n, dtype=np.int32)
+    ini = None if init is None else np.ascontiguousarray(init, dtype=np.float64)
+    lib.orc_sq_contact_batch(n, _dp(A), _dp(B), _dp(ini) if ini is not None else None, _dp(pt),
+                             _dp(fi), _dp(fj), it.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                             fl.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return pt, fi, fj, it, fl
