@@ -168,7 +168,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=200,
+                    help="untimed steps; the first 120 autotune the step kernel's tile shape")
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--timing-steps", type=int, default=500)
     ap.add_argument("--cpu-steps", type=int, default=100)
@@ -189,7 +190,7 @@ def main():
     import workloads as wl
     meshes = ()
     inst = imats = None
-    tune = graphs = False
+    tune = graphs = True
     if args.workload == "sweep":
         params, mats, P, walls, meshes, inst, imats = wl.heap_sweep(seed=0, ninst=args.instances)
         n = P["x"].shape[0]
@@ -199,7 +200,6 @@ def main():
                     "dt 1e-5 s")
         data = "synthetic (seeded jittered lattices, per-instance seeded materials)"
         modes = args.modes or "mixed"
-        tune = graphs = True
     elif args.workload == "drum":
         params, mats, P, walls, meshes = wl.drum(seed=0)
         n = P["x"].shape[0]

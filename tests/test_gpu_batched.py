@@ -32,10 +32,12 @@ def _sweep(every_step):
     params, base, P, walls, meshes, counts, imats = wl.heap_sweep(seed=7, counts=COUNTS)
     if every_step:
         params = dict(params, neigh_every=1, neigh_check=False)
-    # start with a downward speed so the columns reach the floor within the test
+    # random velocities (collisions inside the lattice from the first steps)
+    # and a downward speed so the columns reach the floor within the test
+    rng = np.random.default_rng(11)
     P = dict(P)
-    P["v"] = P["v"].copy()
-    P["v"][:, 2] = -0.4
+    P["v"] = rng.normal(0.0, 0.2, P["v"].shape)
+    P["v"][:, 2] -= 0.4
     return params, base, P, walls, meshes, counts, imats
 
 
@@ -49,7 +51,7 @@ def test_batched_instances_equal_alone_bitwise():
     and the per-instance contact counts are equal."""
     say("[gpu] batched == alone (bitwise, rebuild every step)")
     params, base, P, walls, meshes, counts, imats = _sweep(True)
-    nsteps = 150
+    nsteps = 200
     g = pydem.GpuSystem(params, base, P, walls, meshes=meshes, instances=counts,
                         instance_materials=imats)
     g.run(nsteps)
@@ -69,7 +71,7 @@ def test_batched_instances_equal_alone_bitwise():
         assert gc[q] == st_stats["ncontacts"] and gw[q] == st_stats["nwall_contacts"]
         a.close()
         off += n
-    assert gw.sum() > 100
+    assert gw.sum() > 50 and gc.sum() > 50
     g.close()
 
 
@@ -104,13 +106,14 @@ def test_batched_statistics_vs_alone_fp64(precision):
     run alone in FP64."""
     say(f"[gpu] batched statistics precision={precision}")
     params, base, P, walls, meshes, counts, imats = _sweep(False)
-    nsteps = 800
+    nsteps = 500
     g = pydem.GpuSystem(params, base, P, walls, meshes=meshes, instances=counts,
                         instance_materials=imats, precision=precision)
     g.run(nsteps)
     gs = g.state()
     gc, gw = g.instance_stats()
     off = 0
+    tot_c = 0
     for q, n in enumerate(counts):
         Pq = wl.instance_slice(P, counts, q)
         a = pydem.GpuSystem(params, imats[q], Pq, walls, meshes=meshes)
@@ -124,6 +127,8 @@ def test_batched_statistics_vs_alone_fp64(precision):
         assert ke_b == pytest.approx(ke_a, rel=0.01)
         assert gc[q] == pytest.approx(s2["ncontacts"], rel=0.01, abs=3)
         assert gw[q] == pytest.approx(s2["nwall_contacts"], rel=0.01, abs=3)
+        tot_c += gc[q]
         a.close()
         off += n
+    assert tot_c > 200
     g.close()
