@@ -254,6 +254,64 @@ int dem_set_graph_capture(dem_handle *h, int on);
 int dem_get_step_shape(dem_handle *h, int *variant, int *persist, int cap, int *ncand,
                        int *cand_variant, int *cand_persist, double *cand_us);
 
+/* ---- Superquadric contact points ----
+ * The contact point of two superquadrics (atom_style superquadric: centre,
+ * quaternion, half-axes, blockiness) is the point where their shape functions
+ * are equal with antiparallel gradients. Poses are 12 doubles: centre[3],
+ * quaternion (w, i, j, k)[4], half-axes a, b, c [3], blockiness n1, n2 [2]. */
+typedef struct dem_sq_handle dem_sq_handle;
+
+enum dem_sq_start {
+  DEM_SQ_FRESH = 0,   /* a fresh contact: the homotopy from volume-equivalent spheres,
+                         calc_contact_point_if_no_previous_point_avaialable
+                         (math_extra_liggghts_superquadric.cpp:1390-1430) */
+  DEM_SQ_GIVEN = 1,   /* one calc_contact_point solve from caller-given points
+                         (math_extra_liggghts_superquadric.cpp:778-1022), the fresh
+                         homotopy where it fails or misses the merit tolerance */
+  DEM_SQ_NETWORK = 2  /* as GIVEN, from the warm-start network's predicted points */
+};
+
+int dem_sq_create(int device, dem_sq_handle **out);
+int dem_sq_destroy(dem_sq_handle *h);
+const char *dem_sq_last_error(const dem_sq_handle *h);
+
+/* n pose pairs (A, B), n*12 doubles each; copied to the device. */
+int dem_sq_set_pairs(dem_sq_handle *h, int64_t n, const double *pose_a, const double *pose_b);
+
+/* Solves every pair on the GPU (FP64, one thread per pair). init: n*3 host
+ * points for DEM_SQ_GIVEN, else NULL. Replaces calc_contact_point as called
+ * from SurfaceModel<SURFACE_SUPERQUADRIC>::checkSurfaceIntersect
+ * (surface_model_superquadric.h:108-205). */
+int dem_sq_solve(dem_sq_handle *h, int start, const double *init);
+
+/* Results of the last solve (any pointer may be NULL): contact points (n*3),
+ * shape-function values fi, fj there, Newton iterations per pair, flags
+ * (bit 0: the solve failed, bit 1: the fresh fallback ran), and the time of
+ * the solve and of the network inference before it (ms, HIP events). */
+int dem_sq_get_results(dem_sq_handle *h, double *point, double *fi, double *fj, int *iters, int *flags,
+                       double *solve_ms, double *net_ms);
+
+/* The warm-start network: 18 inputs (B's centre in A's body frame and the
+ * relative quaternion, both shapes relative to their volume-equivalent radii,
+ * blockiness / 8, radius ratio) -> 64 -> 64 (tanh) -> 3 (the contact point in
+ * A's body frame over the sum of the equivalent radii); BF16 matrix-core
+ * products with FP32 accumulation, FP32 master weights, Adam. No counterpart
+ * in the reference. Parameters: W1[18][64], b1[64], W2[64][64], b2[64],
+ * W3[64][3], b3[3] (5571 floats). */
+int dem_sq_get_features(dem_sq_handle *h, float *X);
+int dem_sq_net_init(dem_sq_handle *h, uint64_t seed);
+int dem_sq_net_set_params(dem_sq_handle *h, const float *params);
+int dem_sq_net_get_params(dem_sq_handle *h, float *params);
+int dem_sq_net_forward(dem_sq_handle *h, int64_t n, const float *X, float *Y);
+/* Gradient of the mean squared error over n samples (no update). */
+int dem_sq_net_gradient(dem_sq_handle *h, int64_t n, const float *X, const float *T, float *grad,
+                        float *loss);
+/* `steps` Adam steps on batches drawn from the handle's pairs, the targets
+ * being the points of the last DEM_SQ_FRESH solve; loss_curve (steps floats)
+ * and train_ms may be NULL. */
+int dem_sq_net_train(dem_sq_handle *h, int64_t steps, int64_t batch, float lr, uint64_t seed,
+                     float *loss_curve, double *train_ms);
+
 /* Runs an input script with the reference's command subset for this path
  * (Input::file, input.cpp:183, and Input::execute_command, input.cpp:560-650).
  * log_path may be NULL (log to stdout). Returns a dem_status. */

@@ -25,8 +25,14 @@ EXPORTED = [
     "dem_run_script", "dem_add_mesh", "dem_mesh_rotate", "dem_get_mesh_nodes",
     "dem_get_mesh_topology", "dem_get_mesh_history", "dem_set_instances",
     "dem_set_instance_materials", "dem_get_instance_stats", "dem_set_step_shape",
-    "dem_get_step_shape", "dem_set_graph_capture",
+    "dem_get_step_shape", "dem_set_graph_capture", "dem_sq_create", "dem_sq_destroy",
+    "dem_sq_last_error", "dem_sq_set_pairs", "dem_sq_solve", "dem_sq_get_results",
+    "dem_sq_get_features", "dem_sq_net_init", "dem_sq_net_set_params", "dem_sq_net_get_params",
+    "dem_sq_net_forward", "dem_sq_net_gradient", "dem_sq_net_train",
 ]
+
+DEM_SQ_FRESH, DEM_SQ_GIVEN, DEM_SQ_NETWORK = 0, 1, 2
+SQ_NUM_PARAMS = 5571
 
 
 class DemParams(ctypes.Structure):
@@ -127,6 +133,22 @@ def load_library(path=LIB_PATH):
     lib.dem_get_instance_stats.argtypes = [h, ctypes.c_int, P(ctypes.c_int64), P(ctypes.c_int64)]
     lib.dem_set_step_shape.argtypes = [h, ctypes.c_int, ctypes.c_int]
     lib.dem_set_graph_capture.argtypes = [h, ctypes.c_int]
+    fp = P(ctypes.c_float)
+    lib.dem_sq_create.argtypes = [ctypes.c_int, P(ctypes.c_void_p)]
+    lib.dem_sq_destroy.argtypes = [h]
+    lib.dem_sq_last_error.argtypes = [h]
+    lib.dem_sq_last_error.restype = ctypes.c_char_p
+    lib.dem_sq_set_pairs.argtypes = [h, ctypes.c_int64, dp, dp]
+    lib.dem_sq_solve.argtypes = [h, ctypes.c_int, dp]
+    lib.dem_sq_get_results.argtypes = [h, dp, dp, dp, ip, ip, dp, dp]
+    lib.dem_sq_get_features.argtypes = [h, fp]
+    lib.dem_sq_net_init.argtypes = [h, ctypes.c_uint64]
+    lib.dem_sq_net_set_params.argtypes = [h, fp]
+    lib.dem_sq_net_get_params.argtypes = [h, fp]
+    lib.dem_sq_net_forward.argtypes = [h, ctypes.c_int64, fp, fp]
+    lib.dem_sq_net_gradient.argtypes = [h, ctypes.c_int64, fp, fp, fp, fp]
+    lib.dem_sq_net_train.argtypes = [h, ctypes.c_int64, ctypes.c_int64, ctypes.c_float, ctypes.c_uint64,
+                                     fp, dp]
     lib.dem_get_step_shape.argtypes = [h, ip, ip, ctypes.c_int, ip, ip, ip, dp]
     _lib = lib
     return lib
@@ -353,3 +375,98 @@ class GpuSystem:
 def run_script(script, log=None):
     lib = load_library()
     return lib.dem_run_script(script.encode(), None if log is None else log.encode())
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class SqSolver:
+    """Superquadric contact points on the GPU (dem_sq_* of the C ABI) and the
+    warm-start network."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        self.h = ctypes.c_void_p()
+        rc = self.lib.dem_sq_create(device, ctypes.byref(self.h))
+        if rc != 0:
+            raise DemError(f"dem_sq_create failed ({rc}): {self.lib.dem_sq_last_error(None).decode()}")
+        self.n = 0
+
+    def _check(self, rc):
+        if rc != 0:
+            raise DemError(f"libdem_amd error {rc}: {self.lib.dem_sq_last_error(self.h).decode()}")
+
+    def set_pairs(self, A, B):
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        self.n = A.shape[0]
+        self._check(self.lib.dem_sq_set_pairs(self.h, self.n, _dp(A), _dp(B)))
+
+    def solve(self, start=DEM_SQ_FRESH, init=None):
+        ini = None if init is None else np.ascontiguousarray(init, dtype=np.float64)
+        self._check(self.lib.dem_sq_solve(self.h, int(start), _dp(ini) if ini is not None else None))
+        return self.results()
+
+    def results(self):
+        n = self.n
+        pt = np.zeros((n, 3))
+        fi = np.zeros(n)
+        fj = np.zeros(n)
+        it = np.zeros(n, dtype=np.int32)
+        fl = np.zeros(n, dtype=np.int32)
+        sm = ctypes.c_double()
+        nm = ctypes.c_double()
+        self._check(self.lib.dem_sq_get_results(self.h, _dp(pt), _dp(fi), _dp(fj), _ip(it), _ip(fl),
+                                                ctypes.byref(sm), ctypes.byref(nm)))
+        return {"point": pt, "fi": fi, "fj": fj, "iters": it, "flags": fl, "solve_ms": sm.value,
+                "net_ms": nm.value}
+
+    def features(self):
+        X = np.zeros((self.n, 18), dtype=np.float32)
+        self._check(self.lib.dem_sq_get_features(self.h, _fp(X)))
+        return X
+
+    def net_init(self, seed=0):
+        self._check(self.lib.dem_sq_net_init(self.h, int(seed)))
+
+    def net_params(self):
+        p = np.zeros(SQ_NUM_PARAMS, dtype=np.float32)
+        self._check(self.lib.dem_sq_net_get_params(self.h, _fp(p)))
+        return p
+
+    def set_net_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(self.lib.dem_sq_net_set_params(self.h, _fp(p)))
+
+    def net_forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], 3), dtype=np.float32)
+        self._check(self.lib.dem_sq_net_forward(self.h, X.shape[0], _fp(X), _fp(Y)))
+        return Y
+
+    def net_gradient(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(SQ_NUM_PARAMS, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        self._check(self.lib.dem_sq_net_gradient(self.h, X.shape[0], _fp(X), _fp(T), _fp(g), _fp(loss)))
+        return g, float(loss[0])
+
+    def net_train(self, steps, batch, lr, seed=0):
+        curve = np.zeros(steps, dtype=np.float32)
+        ms = ctypes.c_double()
+        self._check(self.lib.dem_sq_net_train(self.h, int(steps), int(batch), float(lr), int(seed),
+                                              _fp(curve), ctypes.byref(ms)))
+        return curve, ms.value
+
+    def close(self):
+        if self.h:
+            self.lib.dem_sq_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
