@@ -17,6 +17,14 @@ tile shape and graph-replayed step batches; it reports the aggregate rate
 and, for a sample of instances, the kinetic-energy and contact-count error
 against the same instance run alone in FP64.
 
+`--workload superquadric` measures the superquadric contact-point solve:
+the warm-start network (18 -> 64 -> 64 -> 3, BF16 matrix cores) is trained on
+the GPU on ~10M seeded overlapping pose pairs (half-axes U(1, 3) mm,
+blockiness U(2, 8), random orientations) resolved by the plain solve, then
+200k unseen pairs are solved plain (homotopy from spheres) and warm-started;
+it reports contacts/s, Newton iterations per contact and the agreement of
+the converged points.
+
 `--workload drum` runs the rotating-drum case instead (closed cylinder
 R = 0.15 m, L = 0.1 m, 64 x 16 mantle quads split into triangles + fan end
 caps = 2,176 triangles turning at 30 rpm via move/mesh rotate, ~304k spheres
@@ -164,6 +172,92 @@ def sweep_vs_alone(params, P, walls, meshes, counts, imats, g, nsteps, sample):
     return rows
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+NET_FLOPS_PER_SAMPLE = 2 * (18 * 64 + 64 * 64 + 64 * 3) + 2 * (2 * 64 * 3 + 2 * 64 * 64 + 18 * 64)
+
+
+def main_superquadric(args):
+    import numpy as np
+    import pydem
+    import superquadrics as sq
+    t_all = time.perf_counter()
+    ntrain, neval = args.sq_train, args.sq_eval
+    A, B = sq.overlapping_pairs(100, ntrain)
+    s = pydem.SqSolver()
+    s.set_pairs(A, B)
+    del A, B
+    r = s.solve(pydem.DEM_SQ_FRESH)
+    train_solve_ms = r["solve_ms"]
+    s.net_init(seed=1)
+    steps = max(1, int(args.sq_epochs * ntrain / args.sq_batch))
+    curve, train_ms = s.net_train(steps=steps, batch=args.sq_batch, lr=2e-3, seed=7)
+    print(f"# trained {steps} steps x {args.sq_batch} in {train_ms:.0f} ms, loss {curve[0]:.3e} -> "
+          f"{curve[-100:].mean():.3e}", file=sys.stderr, flush=True)
+    A, B = sq.overlapping_pairs(200, neval)
+    s.set_pairs(A, B)
+    res = {}
+    for name, start, pol in (("plain", pydem.DEM_SQ_FRESH, 0), ("warm", pydem.DEM_SQ_NETWORK, 0),
+                             ("plain_polished", pydem.DEM_SQ_FRESH, 4),
+                             ("warm_polished", pydem.DEM_SQ_NETWORK, 4)):
+        s.solve(start, polish=pol)  # untimed: first launch
+        r = s.solve(start, polish=pol)
+        ms = r["solve_ms"] + r["net_ms"]
+        res[name] = {"contacts_per_s": neval / (ms * 1e-3), "ms": ms, "net_ms": r["net_ms"],
+                     "newton_iterations_per_contact": float(r["iters"].mean()),
+                     "fallback_fraction": float(((r["flags"] & 2) != 0).mean()),
+                     "failed": int((r["flags"] & 1).sum()), "point": r["point"]}
+    d = np.linalg.norm(res["warm"]["point"] - res["plain"]["point"], axis=1)
+    d4 = np.linalg.norm(res["warm_polished"]["point"] - res["plain_polished"]["point"], axis=1)
+    for v in res.values():
+        v.pop("point")
+    # network training throughput against the BF16 matrix-core peak
+    train_tflops = NET_FLOPS_PER_SAMPLE * args.sq_batch * steps / (train_ms * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        nc = min(args.sq_cpu, neval)
+        t0 = time.perf_counter()
+        pyoracle.sq_contact_batch(A[:nc], B[:nc])
+        dt = time.perf_counter() - t0
+        cpu = {"value": nc / dt, "unit": "contacts/s", "cores": 1, "kind": "port",
+               "sample": f"oracle/sq_oracle.c plain (homotopy) solve of {nc} of the same pairs, "
+                         f"1 thread ({dt:.1f} s)"}
+    warm = res["warm"]
+    out = {
+        "metric": "superquadric contacts/s (FP64 Newton, network warm start)",
+        "value": warm["contacts_per_s"],
+        "unit": "contacts/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": warm["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (solve) + bf16/f32 (network)",
+        "data": f"synthetic (seeded overlapping pose pairs: {ntrain} training, {neval} evaluation)",
+        "config": {"workload": f"{neval} superquadric pairs, half-axes U(1,3) mm, blockiness U(2,8), "
+                               "random orientations; network 18-64-64-3 trained on "
+                               f"{ntrain} plain-solved pairs ({steps} Adam steps, batch {args.sq_batch})",
+                   "n_pairs": neval},
+        "modes": res,
+        "iteration_ratio_plain_over_warm": res["plain"]["newton_iterations_per_contact"]
+        / warm["newton_iterations_per_contact"],
+        "agreement_m": {"median": float(np.median(d)), "q99": float(np.quantile(d, 0.99)),
+                        "within_1e-9": float((d <= 1e-9).mean()),
+                        "polished_q99": float(np.quantile(d4, 0.99)),
+                        "polished_within_1e-9": float((d4 <= 1e-9).mean())},
+        "training": {"steps": steps, "batch": args.sq_batch, "ms": train_ms,
+                     "loss_first": float(curve[0]), "loss_last": float(curve[-100:].mean()),
+                     "plain_solve_ms_of_training_set": train_solve_ms},
+        "roofline": {"kernel": "sqk::k_mlp_grad + k_mlp_adam (network training step)", "bound": "mfma",
+                     "achieved": train_tflops, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": train_tflops / MFMA_BF16_PEAK_TFLOPS, "traffic": None},
+        "wall_s": time.perf_counter() - t_all,
+    }
+    if cpu:
+        out["cpu_baseline"] = cpu
+    s.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,7 +269,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default=None)
-    ap.add_argument("--workload", choices=("settling", "drum", "sweep"), default="settling")
+    ap.add_argument("--workload", choices=("settling", "drum", "sweep", "superquadric"),
+                    default="settling")
+    ap.add_argument("--sq-train", type=int, default=10_000_000)
+    ap.add_argument("--sq-eval", type=int, default=200_000)
+    ap.add_argument("--sq-batch", type=int, default=8192)
+    ap.add_argument("--sq-epochs", type=float, default=2.0)
+    ap.add_argument("--sq-cpu", type=int, default=20_000)
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--sample", type=int, default=8,
                     help="sweep: instances rerun alone in FP64 for the error report")
@@ -186,6 +286,9 @@ def main():
               "is not built, so --gpus must be 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "superquadric":
+        main_superquadric(args)
+        return
     import pydem
     import workloads as wl
     meshes = ()

@@ -281,8 +281,11 @@ int dem_sq_set_pairs(dem_sq_handle *h, int64_t n, const double *pose_a, const do
 /* Solves every pair on the GPU (FP64, one thread per pair). init: n*3 host
  * points for DEM_SQ_GIVEN, else NULL. Replaces calc_contact_point as called
  * from SurfaceModel<SURFACE_SUPERQUADRIC>::checkSurfaceIntersect
- * (surface_model_superquadric.h:108-205). */
-int dem_sq_solve(dem_sq_handle *h, int start, const double *init);
+ * (surface_model_superquadric.h:108-205). polish > 0 adds up to that many
+ * full Newton steps after the reference's stopping rule (each kept only if it
+ * lowers the residual): no reference counterpart, for callers that need the
+ * point below the reference's merit tolerance; 0 is the reference's solve. */
+int dem_sq_solve(dem_sq_handle *h, int start, const double *init, int polish);
 
 /* Results of the last solve (any pointer may be NULL): contact points (n*3),
  * shape-function values fi, fj there, Newton iterations per pair, flags

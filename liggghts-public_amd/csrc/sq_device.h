@@ -518,6 +518,52 @@ __device__ __noinline__ int contact_point(Sq &A, Sq &B, double ratio, const doub
   return iter + 1;
 }
 
+// Extra full Newton steps on the converged system (F = 0, f_A = f_B) from
+// point cp, each kept only if it lowers the residual; stops after `steps`,
+// at a residual below 1e-30 or when a step does not help. Not part of the
+// reference's solve: an optional refinement past its merit tolerance.
+__device__ __noinline__ int polish(Sq &A, Sq &B, double *cp, double &fi, double &fj, int steps) {
+  double F[4], mu, merit, fA, fB, gA[3], gB[3];
+  double res = calc_F_free(A, B, fA, fB, gA, gB, cp, mu, F, merit);
+  cp3(A.gradient, gA);
+  cp3(B.gradient, gB);
+  int done = 0;
+  for (int it = 0; it < steps; it++) {
+    if (!(res > 1e-30)) break;
+    double fd, gd[3], J4[16], J4inv[16], delta[4] = {0, 0, 0, 0}, d0[4] = {0, 0, 0, 0};
+    props_global<true>(A, cp, fd, gd, A.hessian);
+    props_global<true>(B, cp, fd, gd, B.hessian);
+    const double mu_sq = mu * mu;
+    for (int i = 0; i < 3; i++) {
+      for (int j = 0; j < 3; j++) J4[4 * i + j] = A.hessian[i * 3 + j] + mu_sq * B.hessian[i * 3 + j];
+      J4[3 + 4 * i] = 2.0 * mu * B.gradient[i];
+      J4[i + 4 * 3] = A.gradient[i] - B.gradient[i];
+    }
+    J4[15] = 0.0;
+    if (fabs(det4(J4)) > 1.0) {
+      inv4(J4, J4inv);
+      matvec4(J4inv, F, delta);
+    } else {
+      gmres44(J4, F, d0, delta);
+    }
+    double p_[3], F_[4], m_, fA_, fB_, gA_[3], gB_[3];
+    for (int k = 0; k < 3; k++) p_[k] = cp[k] - delta[k];
+    const double mu_ = mu - delta[3];
+    const double r_ = calc_F_mu(A, B, fA_, fB_, gA_, gB_, p_, mu_, F_, m_);
+    if (!(r_ < res)) break;
+    res = r_;
+    mu = mu_;
+    cp3(cp, p_);
+    cp4(F, F_);
+    cp3(A.gradient, gA_);
+    cp3(B.gradient, gB_);
+    fi = fA_;
+    fj = fB_;
+    done++;
+  }
+  return done;
+}
+
 // the homotopy from volume-equivalent spheres (N+1 solves)
 __device__ __noinline__ int contact_point_fresh(Sq &A, Sq &B, double *cp, double &fi, double &fj, int &fail,
                                                 int max_iter) {

@@ -39,6 +39,7 @@ struct SolveArgs {
   int64_t n;
   int mode;                       // 0 fresh (homotopy), 1 from init with the plain fallback
   int max_iter;
+  int polish;                     // extra Newton refinement steps after the solve (0: none)
   double *point, *fi, *fj;
   int *iters, *fail;              // fail: bit 0 the solve's flag, bit 1 the fallback ran
 };

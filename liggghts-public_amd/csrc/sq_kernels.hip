@@ -44,6 +44,7 @@ __global__ __launch_bounds__(kBlk) void k_sq_solve(SolveArgs a) {
       iters += sqd::contact_point_fresh(A, B, cp, fi, fj, fail, a.max_iter);
     }
   }
+  if (a.polish > 0 && !fail) iters += sqd::polish(A, B, cp, fi, fj, a.polish);
   a.point[3 * k] = cp[0];
   a.point[3 * k + 1] = cp[1];
   a.point[3 * k + 2] = cp[2];

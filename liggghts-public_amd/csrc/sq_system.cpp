@@ -165,12 +165,13 @@ int dem_sq_set_pairs(dem_sq_handle *h, int64_t n, const double *pose_a, const do
   return DEM_OK;
 }
 
-int dem_sq_solve(dem_sq_handle *h, int start, const double *init) {
+int dem_sq_solve(dem_sq_handle *h, int start, const double *init, int polish) {
   if (!h) return DEM_ERR_ARG;
   if (h->n <= 0) return sq_fail(h, DEM_ERR_STATE, "no pose pairs");
   if (start != DEM_SQ_FRESH && start != DEM_SQ_GIVEN && start != DEM_SQ_NETWORK)
     return sq_fail(h, DEM_ERR_ARG, "unknown start");
   if (start == DEM_SQ_GIVEN && !init) return sq_fail(h, DEM_ERR_ARG, "start points missing");
+  if (polish < 0 || polish > 50) return sq_fail(h, DEM_ERR_ARG, "polish steps out of range");
   if (start == DEM_SQ_NETWORK) {
     int rc = ensure_net(h);
     if (rc) return rc;
@@ -191,6 +192,7 @@ int dem_sq_solve(dem_sq_handle *h, int start, const double *init) {
   a.n = h->n;
   a.mode = start == DEM_SQ_FRESH ? 0 : 1;
   a.max_iter = 100000;  // Niter of calc_contact_point (math_extra_liggghts_superquadric.cpp:856)
+  a.polish = polish;
   a.point = h->point;
   a.fi = h->fi;
   a.fj = h->fj;
@@ -204,7 +206,7 @@ int dem_sq_solve(dem_sq_handle *h, int start, const double *init) {
   SQCHK(h, hipEventElapsedTime(&ms1, h->ev[1], h->ev[2]));
   h->net_ms = ms0;
   h->solve_ms = ms1;
-  h->have_fresh = start == DEM_SQ_FRESH;
+  h->have_fresh = start == DEM_SQ_FRESH && polish == 0;
   return DEM_OK;
 }
 

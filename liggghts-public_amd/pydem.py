@@ -139,7 +139,7 @@ def load_library(path=LIB_PATH):
     lib.dem_sq_last_error.argtypes = [h]
     lib.dem_sq_last_error.restype = ctypes.c_char_p
     lib.dem_sq_set_pairs.argtypes = [h, ctypes.c_int64, dp, dp]
-    lib.dem_sq_solve.argtypes = [h, ctypes.c_int, dp]
+    lib.dem_sq_solve.argtypes = [h, ctypes.c_int, dp, ctypes.c_int]
     lib.dem_sq_get_results.argtypes = [h, dp, dp, dp, ip, ip, dp, dp]
     lib.dem_sq_get_features.argtypes = [h, fp]
     lib.dem_sq_net_init.argtypes = [h, ctypes.c_uint64]
@@ -403,9 +403,10 @@ class SqSolver:
         self.n = A.shape[0]
         self._check(self.lib.dem_sq_set_pairs(self.h, self.n, _dp(A), _dp(B)))
 
-    def solve(self, start=DEM_SQ_FRESH, init=None):
+    def solve(self, start=DEM_SQ_FRESH, init=None, polish=0):
         ini = None if init is None else np.ascontiguousarray(init, dtype=np.float64)
-        self._check(self.lib.dem_sq_solve(self.h, int(start), _dp(ini) if ini is not None else None))
+        self._check(self.lib.dem_sq_solve(self.h, int(start), _dp(ini) if ini is not None else None,
+                                          int(polish)))
         return self.results()
 
     def results(self):
