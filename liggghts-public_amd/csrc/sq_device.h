@@ -271,7 +271,7 @@ __device__ __forceinline__ void matvec4(const double *A, const double *b, double
   }
 }
 
-__device__ __noinline__ void gmres44(const double *A, const double *b, const double *x0, double *x) {
+__device__ __forceinline__ void gmres44(const double *A, const double *b, const double *x0, double *x) {
   constexpr int N = 4, MS = 4;
   int m = MS;
   double res[N], Ax0[N];
@@ -348,7 +348,7 @@ __device__ __forceinline__ void cp4(double *d, const double *s) {
 }
 
 // calc_contact_point; returns Newton iterations, sets fail
-__device__ __noinline__ int contact_point(Sq &A, Sq &B, double ratio, const double *init1, double *result,
+__device__ __forceinline__ int contact_point(Sq &A, Sq &B, double ratio, const double *init1, double *result,
                                           double &fi, double &fj, int &fail, int max_iter) {
   const double tol1 = 1e-10, tol2 = 1e-12;
   fail = 0;
@@ -522,7 +522,7 @@ __device__ __noinline__ int contact_point(Sq &A, Sq &B, double ratio, const doub
 // point cp, each kept only if it lowers the residual; stops after `steps`,
 // at a residual below 1e-30 or when a step does not help. Not part of the
 // reference's solve: an optional refinement past its merit tolerance.
-__device__ __noinline__ int polish(Sq &A, Sq &B, double *cp, double &fi, double &fj, int steps) {
+__device__ __forceinline__ int polish(Sq &A, Sq &B, double *cp, double &fi, double &fj, int steps) {
   double F[4], mu, merit, fA, fB, gA[3], gB[3];
   double res = calc_F_free(A, B, fA, fB, gA, gB, cp, mu, F, merit);
   cp3(A.gradient, gA);
@@ -565,7 +565,7 @@ __device__ __noinline__ int polish(Sq &A, Sq &B, double *cp, double &fi, double 
 }
 
 // the homotopy from volume-equivalent spheres (N+1 solves)
-__device__ __noinline__ int contact_point_fresh(Sq &A, Sq &B, double *cp, double &fi, double &fj, int &fail,
+__device__ __forceinline__ int contact_point_fresh(Sq &A, Sq &B, double *cp, double &fi, double &fj, int &fail,
                                                 int max_iter) {
   const double ai = A.shape[0], bi = A.shape[1], ci = A.shape[2];
   const double aj = B.shape[0], bj = B.shape[1], cj = B.shape[2];

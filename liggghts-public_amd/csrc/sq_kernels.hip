@@ -22,7 +22,9 @@ constexpr int kBlk = 256;
 
 // mode 0: fresh contacts (homotopy from spheres); mode 1: one solve from
 // init[k], and the homotopy when that one fails or stops without meeting the
-// merit tolerance (the plain start then judges and replaces it)
+// merit tolerance (the plain start then judges and replaces it). The
+// fallback stays in the same launch: run as a separate launch over the few
+// listed pairs, the hardest homotopies would run alone on an idle GPU.
 __global__ __launch_bounds__(kBlk) void k_sq_solve(SolveArgs a) {
   const int64_t k = (int64_t)blockIdx.x * kBlk + threadIdx.x;
   if (k >= a.n) return;
