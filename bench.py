@@ -25,6 +25,10 @@ blockiness U(2, 8), random orientations) resolved by the plain solve, then
 it reports contacts/s, Newton iterations per contact and the agreement of
 the converged points.
 
+`--workload packed` runs the same 1M spheres from a compressed lattice
+(about six touching neighbors per sphere, tangential history on each), the
+history-heavy regime of a settled bed.
+
 `--workload drum` runs the rotating-drum case instead (closed cylinder
 R = 0.15 m, L = 0.1 m, 64 x 16 mantle quads split into triangles + fan end
 caps = 2,176 triangles turning at 30 rpm via move/mesh rotate, ~304k spheres
@@ -70,6 +74,20 @@ def algorithmic_bytes(n, npairs, ncontacts, nwall, mixed, ntri=0, ncand=0):
     if ntri:
         B += 4 * n + 4 * ncand + TRI_RECORD_BYTES * ntri + 4 * nwall
     return B
+
+
+def measured_traffic(workload, mode):
+    """HBM bytes per launch of the step kernel from the committed FETCH_SIZE /
+    WRITE_SIZE profile of this workload (calibrated against a plain copy), or
+    None when no such profile exists."""
+    path = os.path.join(ROOT, "profiles", f"k_step_{workload}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            vals = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
+            return float(vals[mode]) if mode in vals else None
+    return None
 
 
 def run_mode(params, mats, P, walls, precision, steps, warmup, timing_steps, meshes=(),
@@ -258,6 +276,29 @@ def main_superquadric(args):
     print(json.dumps(out), flush=True)
 
 
+def main_copy(args):
+    """Calibration for hardware counters: a plain device-to-device copy of a
+    known size (torch's copy kernel), repeated; its FETCH_SIZE / WRITE_SIZE per
+    launch against the bytes it must move gives the counters' scale."""
+    import torch
+    nbytes = args.copy_mib << 20
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 20
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"metric": "device copy bandwidth (counter calibration)", "value": 2 * nbytes / dt / 1e9,
+                      "unit": "GB/s", "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+                      "launches": reps + 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -269,8 +310,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default=None)
-    ap.add_argument("--workload", choices=("settling", "drum", "sweep", "superquadric"),
+    ap.add_argument("--workload", choices=("settling", "packed", "drum", "sweep", "superquadric", "copy"),
                     default="settling")
+    ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--sq-train", type=int, default=10_000_000)
     ap.add_argument("--sq-eval", type=int, default=200_000)
     ap.add_argument("--sq-batch", type=int, default=8192)
@@ -289,6 +331,9 @@ def main():
     if args.workload == "superquadric":
         main_superquadric(args)
         return
+    if args.workload == "copy":
+        main_copy(args)
+        return
     import pydem
     import workloads as wl
     meshes = ()
@@ -303,6 +348,14 @@ def main():
                     "dt 1e-5 s")
         data = "synthetic (seeded jittered lattices, per-instance seeded materials)"
         modes = args.modes or "mixed"
+    elif args.workload == "packed":
+        params, mats, P, walls = wl.packed_1m(seed=0, n=args.n)
+        n = P["x"].shape[0]
+        workload = (f"{n}-sphere polydisperse Hertz/history + CDT rolling, compressed lattice "
+                    "(about six touching neighbors per sphere), periodic xy, zplane floor, "
+                    "dt 5e-6 s, skin 1 mm")
+        data = f"synthetic (seeded compressed lattice, {n} polydisperse spheres)"
+        modes = args.modes or "fp64,mixed"
     elif args.workload == "drum":
         params, mats, P, walls, meshes = wl.drum(seed=0)
         n = P["x"].shape[0]
@@ -341,6 +394,8 @@ def main():
         metric = "aggregate particle-timesteps/s, batched heap-formation sweep"
     elif args.workload == "drum":
         metric = "particle-timesteps/s, rotating drum with mesh walls"
+    elif args.workload == "packed":
+        metric = "particle-timesteps/s at 1M spheres in a packed bed, mixed vs FP64 GPU"
     out = {
         "metric": metric,
         "value": head["throughput"],
@@ -378,7 +433,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": (head["achieved_gbs"] / HBM_PEAK_GBS) if head["achieved_gbs"] else None,
-        "traffic": None,
+        "traffic": measured_traffic("1M_settling", head_mode) if args.workload == "settling" else None,
         "bytes_per_launch": head["bytes_per_launch"],
         "kernel_ms": head["kernel_ms"],
     }

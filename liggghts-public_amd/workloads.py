@@ -115,6 +115,18 @@ def dense_pack(seed=0, n=2000, rolling=True, overlap=0.02, box=None, periodic=(1
     return params, mats, particles, [(2, lo[2], 1)]
 
 
+def packed_1m(seed=0, n=1_000_000, overlap=0.01):
+    """The 1M polydisperse settling spheres (radii U(1.0, 1.5) mm, CDT rolling
+    friction 0.1, dt 5e-6 s, skin 1 mm, periodic in x and y, zplane floor)
+    started as a compressed lattice instead of a dilute one: the lattice
+    spacing is 1 % below the mean diameter, so about six neighbors of every
+    sphere touch and the step carries tangential history for each of them."""
+    params, mats, P, walls = dense_pack(seed=seed, n=n, rolling=True, overlap=overlap,
+                                        radius=(0.001, 0.0015), speed=0.01, spin=1.0)
+    params = dict(params, skin=0.001)
+    return params, mats, P, walls
+
+
 def materials_two(mu_pp=0.5, mu_pw=0.4, restitution=0.3, youngs=5e6, poisson=0.45,
                   rolling_friction=None):
     """Type 1 = particles, type 2 = wall (fix mesh/surface ... type 2)."""
