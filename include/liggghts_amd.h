@@ -287,6 +287,27 @@ int dem_sq_set_pairs(dem_sq_handle *h, int64_t n, const double *pose_a, const do
  * point below the reference's merit tolerance; 0 is the reference's solve. */
 int dem_sq_solve(dem_sq_handle *h, int start, const double *init, int polish);
 
+/* Contact detection for every pair: SurfaceModel<SURFACE_SUPERQUADRIC>::
+ * checkSurfaceIntersect (surface_model_superquadric.h:108-205) -- the OBB
+ * pre-check (obb_intersect, math_extra_liggghts_superquadric.cpp:166-203)
+ * unless the pair intersected last step, the contact point (the homotopy for
+ * a pair far apart last step, else calc_contact_point_using_prev_step,
+ * :1434-1466), and for intersecting surfaces the normal en (from j to i) and
+ * the overlap (extended_overlap_algorithm, :1489-1504).
+ *   start   DEM_SQ_FRESH, or DEM_SQ_NETWORK: pairs far apart last step start
+ *           from the warm-start network's point (the homotopy as fallback)
+ *   dt      time step of the previous-step estimate
+ *   vel     n*12 host doubles v_i, v_j, omega_i, omega_j, with mass (n*2 m_i,
+ *           m_j); both NULL for pairs at rest (the estimate is the last point)
+ *   hist    n*7 host doubles, in/out, the pair's contact history: state
+ *           (0 far, 1 close, 2 intersecting), first OBB inequality to try,
+ *           contact point[3], alpha_i, alpha_j; all zero for a new pair
+ * Outputs (each may be NULL): contact (n ints), en (n*3), deltan (n), the
+ * surface points cp_i, cp_j (n*3 each); contact points, Newton iterations and
+ * flags (bit 2: the OBBs are apart, no solve) through dem_sq_get_results. */
+int dem_sq_check(dem_sq_handle *h, int start, double dt, const double *vel, const double *mass, double *hist,
+                 int *contact, double *en, double *deltan, double *cp_i, double *cp_j);
+
 /* Results of the last solve (any pointer may be NULL): contact points (n*3),
  * shape-function values fi, fj there, Newton iterations per pair, flags
  * (bit 0: the solve failed, bit 1: the fresh fallback ran), and the time of

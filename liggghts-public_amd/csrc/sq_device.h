@@ -598,4 +598,200 @@ __device__ __forceinline__ int contact_point_fresh(Sq &A, Sq &B, double *cp, dou
   return iters;
 }
 
+// ---------------------------------------------------------------------------
+// contact detection around the contact point
+// ---------------------------------------------------------------------------
+
+// shape_function_local (superquadric.cpp:109-136): its own arithmetic, not
+// props_local's
+__device__ __forceinline__ double shape_local(const Sq &s, const double *x) {
+  const double n1 = s.blk[0], n2 = s.blk[1];
+  const double xi = x[0] * s.shape_inv[0], yi = x[1] * s.shape_inv[1], zi = x[2] * s.shape_inv[2];
+  double f;
+  if (s.int_blk) {
+    const int n1i = (int)floor(n1 + 1e-2), n2i = (int)floor(n2 + 1e-2);
+    if (n1i == n2i)
+      f = pow_abs_int(xi, n2i) + pow_abs_int(yi, n2i) + pow_abs_int(zi, n1i) - 1.0;
+    else
+      f = pow_abs(pow_abs_int(xi, n2i) + pow_abs_int(yi, n2i), n1 / n2) + pow_abs_int(zi, n1i) - 1.0;
+  } else {
+    f = pow_abs(pow_abs(xi, n2) + pow_abs(yi, n2), n1 / n2) + pow_abs(zi, n1) - 1.0;
+  }
+  return s.koef * f;
+}
+
+// shape_function_gradient_local (superquadric.cpp:139-176)
+__device__ __forceinline__ void gradient_local(const Sq &s, const double *x, double *r) {
+  const double n1 = s.blk[0], n2 = s.blk[1];
+  const double a = s.shape_inv[0], b = s.shape_inv[1], c = s.shape_inv[2];
+  const double xa = x[0] * a, yb = x[1] * b, zc = x[2] * c;
+  double xan21, ybn21, zcn11;
+  if (s.int_blk) {
+    const int n1i = (int)floor(n1 + 1e-2), n2i = (int)floor(n2 + 1e-2);
+    xan21 = pow_abs_int(xa, n2i - 1);
+    ybn21 = pow_abs_int(yb, n2i - 1);
+    zcn11 = pow_abs_int(zc, n1i - 1);
+  } else {
+    xan21 = pow_abs(xa, n2 - 1.0);
+    ybn21 = pow_abs(yb, n2 - 1.0);
+    zcn11 = pow_abs(zc, n1 - 1.0);
+  }
+  const double k = s.koef;
+  double xyp1 = 1.0;
+  const int same = comp_double(n1, n2, 1e-2);
+  if (!same) xyp1 = pow_abs(xan21 * fabs(xa) + ybn21 * fabs(yb), n1 / n2 - 1.0);
+  r[0] = same ? n1 * (k * a) * sgn(xa) * xan21 : n1 * (k * a) * sgn(xa) * xan21 * xyp1;
+  r[1] = same ? n1 * (k * b) * sgn(yb) * ybn21 : n1 * (k * b) * sgn(yb) * ybn21 * xyp1;
+  r[2] = n1 * (k * c) * sgn(zc) * zcn11;
+}
+
+__device__ __forceinline__ double clampd(double v, double lo, double hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// Superquadric::surface_line_intersection (superquadric.cpp:712-816) with the
+// ellipsoid's closed form (ellipsoid_line_intersection_local, :1011-1035),
+// for the overlap algorithm's call (20 iterations, no ellipsoid start for
+// other shapes): the alpha where start + alpha * dir crosses the surface
+__device__ __forceinline__ double line_intersection(const Sq &s, const double *start, const double *dir,
+                                                    double alpha_prev, double *result) {
+  const double *R = s.rot;
+  const double d0[3] = {start[0] - s.center[0], start[1] - s.center[1], start[2] - s.center[2]};
+  const double sl[3] = {R[0] * d0[0] + R[3] * d0[1] + R[6] * d0[2], R[1] * d0[0] + R[4] * d0[1] + R[7] * d0[2],
+                        R[2] * d0[0] + R[5] * d0[1] + R[8] * d0[2]};
+  const double dl[3] = {R[0] * dir[0] + R[3] * dir[1] + R[6] * dir[2],
+                        R[1] * dir[0] + R[4] * dir[1] + R[7] * dir[2],
+                        R[2] * dir[0] + R[5] * dir[1] + R[8] * dir[2]};
+  double alpha = alpha_prev;
+  const double eps0 = 1e-14;
+  const int ellipsoid = comp_double(s.blk[0], 2.0, 1e-2) && comp_double(s.blk[1], 2.0, 1e-2);
+  if (ellipsoid) {
+    const double x0 = sl[0] * s.shape_inv[0], nx = dl[0] * s.shape_inv[0];
+    const double y0 = sl[1] * s.shape_inv[1], ny = dl[1] * s.shape_inv[1];
+    const double z0 = sl[2] * s.shape_inv[2], nz = dl[2] * s.shape_inv[2];
+    const double qa = nx * nx + ny * ny + nz * nz;
+    const double qb = 2.0 * (x0 * nx + y0 * ny + z0 * nz);
+    const double qc = x0 * x0 + y0 * y0 + z0 * z0 - 1.0;
+    const double D = qb * qb - 4.0 * qa * qc;
+    const double two_a_inv = 0.5 / qa;
+    double a1, a2;
+    if (D < 0) {
+      a1 = -qb * two_a_inv;
+      a2 = a1;
+    } else {
+      const double sq = sqrt(D);
+      a1 = (-qb + sq) * two_a_inv;
+      a2 = (-qb - sq) * two_a_inv;
+    }
+    if (a1 > -eps0 && a2 < -eps0)
+      alpha = a1;
+    else if (a1 < -eps0 && a2 > -eps0)
+      alpha = a2;
+    else if (a1 < -eps0 && a2 < -eps0)
+      alpha = a1 > a2 ? a1 : a2;
+    else
+      alpha = a1 < a2 ? a1 : a2;
+  } else {
+    const double a = 0.5 * fmin(fmin(s.shape[0], s.shape[1]), s.shape[2]) /
+                     sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    const double eps = 1e-16;
+    alpha = clampd(alpha, -a, a);
+    double point[3] = {sl[0] + alpha * dl[0], sl[1] + alpha * dl[1], sl[2] + alpha * dl[2]};
+    double f0 = shape_local(s, point);
+    if (!(fabs(f0) < eps)) {
+      for (int i = 0; i < 20; i++) {
+        if (fabs(f0) < eps * s.koef) break;
+        double grad[3];
+        gradient_local(s, point, grad);
+        const double f_der = dot3(grad, dl);
+        double delta;
+        if (fabs(f_der) < 1e-10) {
+          delta = 1e-10;
+          const double p_[3] = {point[0] + delta * dl[0], point[1] + delta * dl[1], point[2] + delta * dl[2]};
+          double g_[3];
+          gradient_local(s, p_, g_);
+          if (dot3(g_, dl) < 0.0) delta = -delta;
+        } else {
+          delta = clampd(-f0 / f_der, -a, a);
+        }
+        if (fabs(delta) <= eps) {
+          alpha += delta;
+          for (int k = 0; k < 3; k++) point[k] = point[k] + delta * dl[k];
+          break;
+        }
+        while (fabs(delta) > eps) {
+          const double p_[3] = {point[0] + delta * dl[0], point[1] + delta * dl[1], point[2] + delta * dl[2]};
+          const double f_ = shape_local(s, p_);
+          if (fabs(f_) < fabs(f0) || fabs(f_) < eps * s.koef) {
+            f0 = f_;
+            alpha += delta;
+            for (int k = 0; k < 3; k++) point[k] = p_[k];
+            break;
+          }
+          delta *= 0.5;
+        }
+      }
+    } else {
+      // already on the surface: the point mapped back (superquadric.cpp:766-771)
+      for (int k = 0; k < 3; k++)
+        result[k] = R[3 * k] * point[0] + R[3 * k + 1] * point[1] + R[3 * k + 2] * point[2] + s.center[k];
+      return alpha;
+    }
+  }
+  for (int k = 0; k < 3; k++) result[k] = start[k] + alpha * dir[k];
+  return alpha;
+}
+
+// check_inequalities (math_extra_liggghts_superquadric.cpp:72-150): axis i
+// separates the two boxes; C = R_A^T R_B, Ad = R_A^T d, Bd = R_B^T d
+__device__ __forceinline__ int obb_separated(int i, const double *a, const double *b, const double *Ad,
+                                             const double *Bd, const double *C) {
+  double R = 0.0, R0 = 0.0, R1 = 0.0;
+  switch (i) {
+    case 0: R0 = a[0]; R1 = b[0] * fabs(C[0]) + b[1] * fabs(C[1]) + b[2] * fabs(C[2]); R = fabs(Ad[0]); break;
+    case 1: R0 = a[1]; R1 = b[0] * fabs(C[3]) + b[1] * fabs(C[4]) + b[2] * fabs(C[5]); R = fabs(Ad[1]); break;
+    case 2: R0 = a[2]; R1 = b[0] * fabs(C[6]) + b[1] * fabs(C[7]) + b[2] * fabs(C[8]); R = fabs(Ad[2]); break;
+    case 3: R0 = a[0] * fabs(C[0]) + a[1] * fabs(C[3]) + a[2] * fabs(C[6]); R1 = b[0]; R = fabs(Bd[0]); break;
+    case 4: R0 = a[0] * fabs(C[1]) + a[1] * fabs(C[4]) + a[2] * fabs(C[7]); R1 = b[1]; R = fabs(Bd[1]); break;
+    case 5: R0 = a[0] * fabs(C[2]) + a[1] * fabs(C[5]) + a[2] * fabs(C[8]); R1 = b[2]; R = fabs(Bd[2]); break;
+    case 6: R0 = a[1] * fabs(C[6]) + a[2] * fabs(C[3]); R1 = b[1] * fabs(C[2]) + b[2] * fabs(C[1]); R = fabs(C[3] * Ad[2] - C[6] * Ad[1]); break;
+    case 7: R0 = a[1] * fabs(C[7]) + a[2] * fabs(C[4]); R1 = b[0] * fabs(C[2]) + b[2] * fabs(C[0]); R = fabs(C[4] * Ad[2] - C[7] * Ad[1]); break;
+    case 8: R0 = a[1] * fabs(C[8]) + a[2] * fabs(C[5]); R1 = b[0] * fabs(C[1]) + b[1] * fabs(C[0]); R = fabs(C[5] * Ad[2] - C[8] * Ad[1]); break;
+    case 9: R0 = a[0] * fabs(C[6]) + a[2] * fabs(C[0]); R1 = b[1] * fabs(C[5]) + b[2] * fabs(C[4]); R = fabs(C[6] * Ad[0] - C[0] * Ad[2]); break;
+    case 10: R0 = a[0] * fabs(C[7]) + a[2] * fabs(C[1]); R1 = b[0] * fabs(C[5]) + b[2] * fabs(C[3]); R = fabs(C[7] * Ad[0] - C[1] * Ad[2]); break;
+    case 11: R0 = a[0] * fabs(C[8]) + a[2] * fabs(C[2]); R1 = b[0] * fabs(C[4]) + b[1] * fabs(C[3]); R = fabs(C[8] * Ad[0] - C[2] * Ad[2]); break;
+    case 12: R0 = a[0] * fabs(C[3]) + a[1] * fabs(C[0]); R1 = b[1] * fabs(C[8]) + b[2] * fabs(C[7]); R = fabs(C[0] * Ad[1] - C[3] * Ad[0]); break;
+    case 13: R0 = a[0] * fabs(C[4]) + a[1] * fabs(C[1]); R1 = b[0] * fabs(C[8]) + b[2] * fabs(C[6]); R = fabs(C[1] * Ad[1] - C[4] * Ad[0]); break;
+    default: R0 = a[0] * fabs(C[5]) + a[1] * fabs(C[2]); R1 = b[0] * fabs(C[7]) + b[1] * fabs(C[6]); R = fabs(C[2] * Ad[1] - C[5] * Ad[0]); break;
+  }
+  if (fabs(R0 + R1) < 1e-14 && fabs(R) < 1e-14) return 0;
+  return R > R0 + R1;
+}
+
+// obb_intersect (math_extra_liggghts_superquadric.cpp:166-203): boxes of the
+// half-axes grown by 1.0001 (scale_koef, math_extra_liggghts_nonspherical.h:60);
+// the last separating axis is tried first
+__device__ __forceinline__ int obb_intersect(const Sq &A, const Sq &B, int &start) {
+  const double sk = 1.0001;
+  const double d[3] = {A.center[0] - B.center[0], A.center[1] - B.center[1], A.center[2] - B.center[2]};
+  const double a[3] = {A.shape[0] * sk, A.shape[1] * sk, A.shape[2] * sk};
+  const double b[3] = {B.shape[0] * sk, B.shape[1] * sk, B.shape[2] * sk};
+  const double *m = A.rot, *m2 = B.rot;
+  double C[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) C[i * 3 + j] = m[i] * m2[j] + m[3 + i] * m2[3 + j] + m[6 + i] * m2[6 + j];
+  const double Ad[3] = {m[0] * d[0] + m[3] * d[1] + m[6] * d[2], m[1] * d[0] + m[4] * d[1] + m[7] * d[2],
+                        m[2] * d[0] + m[5] * d[1] + m[8] * d[2]};
+  const double Bd[3] = {m2[0] * d[0] + m2[3] * d[1] + m2[6] * d[2], m2[1] * d[0] + m2[4] * d[1] + m2[7] * d[2],
+                        m2[2] * d[0] + m2[5] * d[1] + m2[8] * d[2]};
+  for (int i = 0; i < 15; i++) {
+    const int which = i == 0 ? start : (i == start ? 0 : i);
+    if (obb_separated(which, a, b, Ad, Bd, C)) {
+      start = which;
+      return 0;
+    }
+  }
+  return 1;
+}
+
 }  // namespace sqd

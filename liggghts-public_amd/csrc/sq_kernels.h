@@ -44,6 +44,23 @@ struct SolveArgs {
   int *iters, *fail;              // fail: bit 0 the solve's flag, bit 1 the fallback ran
 };
 
+// checkSurfaceIntersect over pose pairs; hist [n][7] = state (0 far, 1 close,
+// 2 intersecting), first OBB inequality, contact point[3], alpha_i, alpha_j
+constexpr int kHist = 7;
+struct CheckArgs {
+  const double *pose_a, *pose_b;  // [n][12]
+  const double *init;             // [n][3] network start points for pairs far apart last step, or null
+  const double *vel, *mass;       // [n][12] v_i, v_j, omega_i, omega_j and [n][2], or null
+  double dt;
+  int64_t n;
+  int max_iter;
+  double *hist;
+  int *contact;
+  double *en, *deltan, *cp_i, *cp_j;  // [n][3], [n], [n][3], [n][3]
+  double *point, *fi, *fj;
+  int *iters, *flags;  // flags: bit 0 the solve's flag, bit 1 the fallback ran, bit 2 no solve (OBB apart)
+};
+
 __device__ __forceinline__ unsigned short pack_bf16(float f) {
   unsigned int u = __float_as_uint(f);
   if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
@@ -70,6 +87,7 @@ __device__ __forceinline__ void write_packed(const NetDev &net, int i, float p) 
 }
 
 hipError_t launch_solve(const SolveArgs &a, hipStream_t s);
+hipError_t launch_check(const CheckArgs &a, hipStream_t s);
 hipError_t launch_features(const double *pa, const double *pb, int64_t n, float *X, hipStream_t s);
 hipError_t launch_to_point(const double *pa, const double *pb, int64_t n, const float *Y, double *P,
                            hipStream_t s);

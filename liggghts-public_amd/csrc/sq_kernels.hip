@@ -56,6 +56,98 @@ __global__ __launch_bounds__(kBlk) void k_sq_solve(SolveArgs a) {
   a.fail[k] = fail | (fallback << 1);
 }
 
+// SurfaceModel<SURFACE_SUPERQUADRIC>::checkSurfaceIntersect
+// (surface_model_superquadric.h:108-205): OBB pre-check unless the pair
+// intersected last step, the contact point (homotopy for a pair far apart
+// last step, else from the previous point moved by the mass-weighted contact
+// velocity, calc_contact_point_using_prev_step, math_extra_liggghts_superquadric.cpp:
+// 1434-1466), and for intersecting surfaces the normal from the two gradients
+// and the overlap along it (extended_overlap_algorithm, :1489-1504). With
+// init, a pair far apart last step starts from the network's point instead of
+// the homotopy, which remains the fallback.
+__global__ __launch_bounds__(kBlk) void k_sq_check(CheckArgs a) {
+  const int64_t k = (int64_t)blockIdx.x * kBlk + threadIdx.x;
+  if (k >= a.n) return;
+  sqd::Sq A, B;
+  sqd::init(A, a.pose_a + 12 * k);
+  sqd::init(B, a.pose_b + 12 * k);
+  double *h = a.hist + kHist * k;
+  const int state = (int)(h[0] + 0.5);
+  int start = (int)(h[1] + 0.5);
+  const int obb = state == 2 ? 1 : sqd::obb_intersect(A, B, start);
+  int contact = 0, iters = 0, fail = 0, fallback = 0;
+  double cp[3] = {h[2], h[3], h[4]}, fi = 0.0, fj = 0.0;
+  double en[3] = {0.0, 0.0, 0.0}, dn = 0.0, cpa[3] = {0.0, 0.0, 0.0}, cpb[3] = {0.0, 0.0, 0.0};
+  if (obb) {
+    const double ri = cbrt(A.shape[0] * A.shape[1] * A.shape[2]);
+    const double rj = cbrt(B.shape[0] * B.shape[1] * B.shape[2]);
+    const double ratio = ri / (ri + rj);
+    int fresh = state == 0;
+    if (!fresh || a.init) {
+      double est[3];
+      if (fresh) {
+        for (int c = 0; c < 3; c++) est[c] = a.init[3 * k + c];
+      } else if (a.vel && a.mass) {
+        const double *v = a.vel + 12 * k;
+        const double mi = a.mass[2 * k], mj = a.mass[2 * k + 1];
+        const double xci[3] = {cp[0] - A.center[0], cp[1] - A.center[1], cp[2] - A.center[2]};
+        const double xcj[3] = {cp[0] - B.center[0], cp[1] - B.center[1], cp[2] - B.center[2]};
+        const double *wi = v + 6, *wj = v + 9;
+        const double ri_[3] = {wi[1] * xci[2] - wi[2] * xci[1], wi[2] * xci[0] - wi[0] * xci[2],
+                               wi[0] * xci[1] - wi[1] * xci[0]};
+        const double rj_[3] = {wj[1] * xcj[2] - wj[2] * xcj[1], wj[2] * xcj[0] - wj[0] * xcj[2],
+                               wj[0] * xcj[1] - wj[1] * xcj[0]};
+        for (int c = 0; c < 3; c++) {
+          const double vi = v[c] + ri_[c], vj = v[3 + c] + rj_[c];
+          const double veff = (vi * mi + vj * mj) / (mi + mj);
+          est[c] = cp[c] + veff * a.dt;
+        }
+      } else {
+        for (int c = 0; c < 3; c++) est[c] = cp[c];
+      }
+      iters = sqd::contact_point(A, B, ratio, est, cp, fi, fj, fail, a.max_iter);
+      if (fresh) {
+        const double merit = sqd::merit_of(A.gradient, B.gradient, fi, fj);
+        if (fail || !(merit < 1e-10)) fallback = 1;
+        fresh = fallback;
+      }
+    }
+    if (fresh) iters += sqd::contact_point_fresh(A, B, cp, fi, fj, fail, a.max_iter);
+    contact = (fi > fj ? fi : fj) < 0.0;
+    if (contact) {
+      for (int c = 0; c < 3; c++) en[c] = B.gradient[c] - A.gradient[c];
+      const double nrm = sqrt(en[0] * en[0] + en[1] * en[1] + en[2] * en[2]);
+      const double inv = nrm == 0. ? 0. : 1. / nrm;
+      for (int c = 0; c < 3; c++) en[c] *= inv;
+      const double men[3] = {-en[0], -en[1], -en[2]};
+      h[5] = sqd::line_intersection(A, cp, men, h[5], cpa);
+      h[6] = sqd::line_intersection(B, cp, en, h[6], cpb);
+      dn = fabs(h[5]) + fabs(h[6]);
+      h[0] = 2.0;
+    } else {
+      h[0] = 1.0;
+    }
+    h[2] = cp[0];
+    h[3] = cp[1];
+    h[4] = cp[2];
+  } else {
+    h[0] = 0.0;
+  }
+  h[1] = (double)start;
+  a.contact[k] = contact;
+  for (int c = 0; c < 3; c++) {
+    a.en[3 * k + c] = en[c];
+    a.cp_i[3 * k + c] = cpa[c];
+    a.cp_j[3 * k + c] = cpb[c];
+    a.point[3 * k + c] = cp[c];
+  }
+  a.deltan[k] = dn;
+  a.fi[k] = fi;
+  a.fj[k] = fj;
+  a.iters[k] = iters;
+  a.flags[k] = fail | (fallback << 1) | ((!obb) << 2);
+}
+
 // ---------------------------------------------------------------------------
 // network inputs and outputs (the encoding of superquadrics.py)
 // ---------------------------------------------------------------------------
@@ -464,6 +556,12 @@ static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / 
 hipError_t launch_solve(const SolveArgs &a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_sq_solve, dim3(nblk(a.n, kBlk)), dim3(kBlk), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_check(const CheckArgs &a, hipStream_t s) {
+  if (a.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sq_check, dim3(nblk(a.n, kBlk)), dim3(kBlk), 0, s, a);
   return hipGetLastError();
 }
 

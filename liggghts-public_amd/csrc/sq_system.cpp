@@ -24,6 +24,11 @@ struct dem_sq_handle {
   int *iters = nullptr, *flags = nullptr;
   float *X = nullptr, *Y = nullptr, *T = nullptr;  // network inputs / outputs / targets of the pairs
   bool have_fresh = false;                         // point holds a fresh (plain) solve
+  // contact detection buffers, allocated with the pair set on first use
+  bool chk_ready = false;
+  double *hist = nullptr, *vel = nullptr, *mass = nullptr, *en = nullptr, *deltan = nullptr, *cp_i = nullptr,
+         *cp_j = nullptr;
+  int *contact = nullptr;
   double solve_ms = 0.0, net_ms = 0.0;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   // network
@@ -148,6 +153,7 @@ int dem_sq_set_pairs(dem_sq_handle *h, int64_t n, const double *pose_a, const do
   free_list(h->pair_allocs);
   h->n = n;
   h->have_fresh = false;
+  h->chk_ready = false;
   const size_t N = (size_t)n;
   int rc;
   auto A = [&](void **p, size_t b) { return alloc(h, h->pair_allocs, p, b); };
@@ -207,6 +213,90 @@ int dem_sq_solve(dem_sq_handle *h, int start, const double *init, int polish) {
   h->net_ms = ms0;
   h->solve_ms = ms1;
   h->have_fresh = start == DEM_SQ_FRESH && polish == 0;
+  return DEM_OK;
+}
+
+int dem_sq_check(dem_sq_handle *h, int start, double dt, const double *vel, const double *mass, double *hist,
+                 int *contact, double *en, double *deltan, double *cp_i, double *cp_j) {
+  if (!h) return DEM_ERR_ARG;
+  if (h->n <= 0) return sq_fail(h, DEM_ERR_STATE, "no pose pairs");
+  if (start != DEM_SQ_FRESH && start != DEM_SQ_NETWORK)
+    return sq_fail(h, DEM_ERR_ARG, "contact detection starts fresh contacts from the homotopy or the network");
+  if (!hist) return sq_fail(h, DEM_ERR_ARG, "contact history missing");
+  if (!vel != !mass) return sq_fail(h, DEM_ERR_ARG, "velocities and masses go together");
+  if (!std::isfinite(dt) || dt < 0.0) return sq_fail(h, DEM_ERR_ARG, "bad time step");
+  const size_t N = (size_t)h->n;
+  for (size_t k = 0; k < N; k++) {
+    const double *q = hist + sqk::kHist * k;
+    for (int c = 0; c < sqk::kHist; c++)
+      if (!std::isfinite(q[c])) return sq_fail(h, DEM_ERR_ARG, "non-finite contact history");
+    if (!(q[0] == 0.0 || q[0] == 1.0 || q[0] == 2.0)) return sq_fail(h, DEM_ERR_ARG, "contact state must be 0, 1 or 2");
+    if (!(q[1] >= 0.0 && q[1] < 15.0)) return sq_fail(h, DEM_ERR_ARG, "OBB inequality index must be in [0, 15)");
+  }
+  if (mass)
+    for (size_t k = 0; k < 2 * N; k++)
+      if (!(mass[k] > 0.0) || !std::isfinite(mass[k])) return sq_fail(h, DEM_ERR_ARG, "masses must be > 0");
+  if (start == DEM_SQ_NETWORK) {
+    int rc = ensure_net(h);
+    if (rc) return rc;
+  }
+  SQCHK(h, hipSetDevice(h->device));
+  if (!h->chk_ready) {
+    int rc;
+    auto A = [&](void **p, size_t b) { return alloc(h, h->pair_allocs, p, b); };
+    if ((rc = A((void **)&h->hist, sizeof(double) * sqk::kHist * N)) || (rc = A((void **)&h->vel, sizeof(double) * 12 * N)) ||
+        (rc = A((void **)&h->mass, sizeof(double) * 2 * N)) || (rc = A((void **)&h->en, sizeof(double) * 3 * N)) ||
+        (rc = A((void **)&h->deltan, sizeof(double) * N)) || (rc = A((void **)&h->cp_i, sizeof(double) * 3 * N)) ||
+        (rc = A((void **)&h->cp_j, sizeof(double) * 3 * N)) || (rc = A((void **)&h->contact, sizeof(int) * N)))
+      return rc;
+    h->chk_ready = true;
+  }
+  SQCHK(h, hipMemcpy(h->hist, hist, sizeof(double) * sqk::kHist * N, hipMemcpyHostToDevice));
+  if (vel) {
+    SQCHK(h, hipMemcpy(h->vel, vel, sizeof(double) * 12 * N, hipMemcpyHostToDevice));
+    SQCHK(h, hipMemcpy(h->mass, mass, sizeof(double) * 2 * N, hipMemcpyHostToDevice));
+  }
+  SQCHK(h, hipEventRecord(h->ev[0], h->stream));
+  if (start == DEM_SQ_NETWORK) {
+    SQCHK(h, sqk::launch_mlp_forward(h->net, h->X, h->n, h->Y, h->stream));
+    SQCHK(h, sqk::launch_to_point(h->pose_a, h->pose_b, h->n, h->Y, h->init, h->stream));
+  }
+  SQCHK(h, hipEventRecord(h->ev[1], h->stream));
+  sqk::CheckArgs a{};
+  a.pose_a = h->pose_a;
+  a.pose_b = h->pose_b;
+  a.init = start == DEM_SQ_NETWORK ? h->init : nullptr;
+  a.vel = vel ? h->vel : nullptr;
+  a.mass = vel ? h->mass : nullptr;
+  a.dt = dt;
+  a.n = h->n;
+  a.max_iter = 100000;  // Niter of calc_contact_point (math_extra_liggghts_superquadric.cpp:856)
+  a.hist = h->hist;
+  a.contact = h->contact;
+  a.en = h->en;
+  a.deltan = h->deltan;
+  a.cp_i = h->cp_i;
+  a.cp_j = h->cp_j;
+  a.point = h->point;
+  a.fi = h->fi;
+  a.fj = h->fj;
+  a.iters = h->iters;
+  a.flags = h->flags;
+  SQCHK(h, sqk::launch_check(a, h->stream));
+  SQCHK(h, hipEventRecord(h->ev[2], h->stream));
+  SQCHK(h, hipEventSynchronize(h->ev[2]));
+  float ms0 = 0.f, ms1 = 0.f;
+  SQCHK(h, hipEventElapsedTime(&ms0, h->ev[0], h->ev[1]));
+  SQCHK(h, hipEventElapsedTime(&ms1, h->ev[1], h->ev[2]));
+  h->net_ms = ms0;
+  h->solve_ms = ms1;
+  h->have_fresh = false;
+  SQCHK(h, hipMemcpy(hist, h->hist, sizeof(double) * sqk::kHist * N, hipMemcpyDeviceToHost));
+  if (contact) SQCHK(h, hipMemcpy(contact, h->contact, sizeof(int) * N, hipMemcpyDeviceToHost));
+  if (en) SQCHK(h, hipMemcpy(en, h->en, sizeof(double) * 3 * N, hipMemcpyDeviceToHost));
+  if (deltan) SQCHK(h, hipMemcpy(deltan, h->deltan, sizeof(double) * N, hipMemcpyDeviceToHost));
+  if (cp_i) SQCHK(h, hipMemcpy(cp_i, h->cp_i, sizeof(double) * 3 * N, hipMemcpyDeviceToHost));
+  if (cp_j) SQCHK(h, hipMemcpy(cp_j, h->cp_j, sizeof(double) * 3 * N, hipMemcpyDeviceToHost));
   return DEM_OK;
 }
 

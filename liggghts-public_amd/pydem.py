@@ -26,12 +26,13 @@ EXPORTED = [
     "dem_get_mesh_topology", "dem_get_mesh_history", "dem_set_instances",
     "dem_set_instance_materials", "dem_get_instance_stats", "dem_set_step_shape",
     "dem_get_step_shape", "dem_set_graph_capture", "dem_sq_create", "dem_sq_destroy",
-    "dem_sq_last_error", "dem_sq_set_pairs", "dem_sq_solve", "dem_sq_get_results",
+    "dem_sq_last_error", "dem_sq_set_pairs", "dem_sq_solve", "dem_sq_check", "dem_sq_get_results",
     "dem_sq_get_features", "dem_sq_net_init", "dem_sq_net_set_params", "dem_sq_net_get_params",
     "dem_sq_net_forward", "dem_sq_net_gradient", "dem_sq_net_train",
 ]
 
 DEM_SQ_FRESH, DEM_SQ_GIVEN, DEM_SQ_NETWORK = 0, 1, 2
+SQ_HIST_DOUBLES = 7  # state, first OBB inequality, contact point[3], alpha_i, alpha_j
 SQ_NUM_PARAMS = 5571
 
 
@@ -141,6 +142,7 @@ def load_library(path=LIB_PATH):
     lib.dem_sq_set_pairs.argtypes = [h, ctypes.c_int64, dp, dp]
     lib.dem_sq_solve.argtypes = [h, ctypes.c_int, dp, ctypes.c_int]
     lib.dem_sq_get_results.argtypes = [h, dp, dp, dp, ip, ip, dp, dp]
+    lib.dem_sq_check.argtypes = [h, ctypes.c_int, ctypes.c_double, dp, dp, dp, ip, dp, dp, dp, dp]
     lib.dem_sq_get_features.argtypes = [h, fp]
     lib.dem_sq_net_init.argtypes = [h, ctypes.c_uint64]
     lib.dem_sq_net_set_params.argtypes = [h, fp]
@@ -408,6 +410,30 @@ class SqSolver:
         self._check(self.lib.dem_sq_solve(self.h, int(start), _dp(ini) if ini is not None else None,
                                           int(polish)))
         return self.results()
+
+    def check(self, hist=None, vel=None, mass=None, dt=0.0, start=DEM_SQ_FRESH):
+        """checkSurfaceIntersect over the pairs. hist (n, 7) is the contact
+        history, updated in place (None: all pairs new); vel (n, 12) and mass
+        (n, 2) feed the previous-step estimate. Returns a dict with hist,
+        contact, en, deltan, cp_i, cp_j and the results() of the solve."""
+        n = self.n
+        if hist is None:
+            hist = np.zeros((n, SQ_HIST_DOUBLES))
+        if hist.shape != (n, SQ_HIST_DOUBLES) or hist.dtype != np.float64 or not hist.flags.c_contiguous:
+            raise ValueError("hist must be a C-contiguous float64 array of shape (n, 7)")
+        v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64)
+        m = None if mass is None else np.ascontiguousarray(mass, dtype=np.float64)
+        contact = np.zeros(n, dtype=np.int32)
+        en = np.zeros((n, 3))
+        dn = np.zeros(n)
+        cpi = np.zeros((n, 3))
+        cpj = np.zeros((n, 3))
+        self._check(self.lib.dem_sq_check(self.h, int(start), float(dt), _dp(v) if v is not None else None,
+                                          _dp(m) if m is not None else None, _dp(hist), _ip(contact), _dp(en),
+                                          _dp(dn), _dp(cpi), _dp(cpj)))
+        out = self.results()
+        out.update(hist=hist, contact=contact, en=en, deltan=dn, cp_i=cpi, cp_j=cpj)
+        return out
 
     def results(self):
         n = self.n

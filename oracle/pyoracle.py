@@ -339,6 +339,11 @@ def _sq_lib():
         lib.orc_sq_contact.argtypes = [dp, dp, dp, dp, dp, dp, ip]
         lib.orc_sq_contact.restype = ctypes.c_int
         lib.orc_sq_contact_batch.argtypes = [ctypes.c_int64, dp, dp, dp, dp, dp, dp, ip, ip]
+        lib.orc_sq_check_batch.argtypes = [ctypes.c_int64, dp, dp, dp, dp, dp, _D, ip, dp, dp, dp, dp, ip]
+        lib.orc_sq_obb.argtypes = [dp, dp, ip]
+        lib.orc_sq_obb.restype = ctypes.c_int
+        lib.orc_sq_line.argtypes = [dp, dp, dp, _D, dp]
+        lib.orc_sq_line.restype = _D
         lib._sq_ready = True
     return lib
 
@@ -374,3 +379,54 @@ def sq_contact_batch(A, B, init=None):
                              _dp(fi), _dp(fj), it.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                              fl.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
     return pt, fi, fj, it, fl
+
+
+SQ_HIST_DOUBLES = 7  # state, first OBB inequality, contact point[3], alpha_i, alpha_j
+SQ_FAR, SQ_CLOSE, SQ_INTERSECT = 0, 1, 2
+
+
+def sq_check_batch(A, B, hist=None, vel=None, mass=None, dt=0.0):
+    """checkSurfaceIntersect over n pose pairs. hist (n, 7) is updated in
+    place (None = all pairs far apart); vel (n, 12) = v_i, v_j, omega_i,
+    omega_j and mass (n, 2) feed the previous-step estimate (None = still).
+    Returns (hist, contact, en (n,3), deltan, cp_i (n,3), cp_j (n,3),
+    newton_iterations)."""
+    lib = _sq_lib()
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    n = A.shape[0]
+    if hist is None:
+        hist = np.zeros((n, SQ_HIST_DOUBLES))
+    assert hist.shape == (n, SQ_HIST_DOUBLES) and hist.dtype == np.float64 and hist.flags.c_contiguous
+    v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64)
+    m = None if mass is None else np.ascontiguousarray(mass, dtype=np.float64)
+    contact = np.zeros(n, dtype=np.int32)
+    en = np.zeros((n, 3))
+    dn = np.zeros(n)
+    cpa = np.zeros((n, 3))
+    cpb = np.zeros((n, 3))
+    it = np.zeros(n, dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.orc_sq_check_batch(n, _dp(A), _dp(B), _dp(hist), _dp(v) if v is not None else None,
+                           _dp(m) if m is not None else None, dt, contact.ctypes.data_as(ip), _dp(en),
+                           _dp(dn), _dp(cpa), _dp(cpb), it.ctypes.data_as(ip))
+    return hist, contact, en, dn, cpa, cpb, it
+
+
+def sq_obb(a, b, start=0):
+    """obb_intersect of two poses: (intersect, first inequality for next time)."""
+    lib = _sq_lib()
+    st = ctypes.c_int(start)
+    r = lib.orc_sq_obb(_dp(np.ascontiguousarray(a, dtype=np.float64)),
+                       _dp(np.ascontiguousarray(b, dtype=np.float64)), ctypes.byref(st))
+    return bool(r), st.value
+
+
+def sq_line(pose, start, direction, alpha_prev=0.0):
+    """surface_line_intersection: (alpha, point) with point = start + alpha * direction."""
+    lib = _sq_lib()
+    res = np.zeros(3)
+    a = lib.orc_sq_line(_dp(np.ascontiguousarray(pose, dtype=np.float64)),
+                        _dp(np.ascontiguousarray(start, dtype=np.float64)),
+                        _dp(np.ascontiguousarray(direction, dtype=np.float64)), alpha_prev, _dp(res))
+    return a, res

@@ -597,6 +597,323 @@ static int contact_point_fresh(sq_t *A, sq_t *B, double *cp, double *fi, double 
   return iters;
 }
 
+/* ---- contact detection around the contact point ---- */
+
+/* shape_function_local (superquadric.cpp:109-136): its own arithmetic, not
+ * props_local's */
+static double shape_local(const sq_t *s, const double *x) {
+  const double n1 = s->blockiness[0], n2 = s->blockiness[1];
+  const double xi = x[0] * s->shape_inv[0], yi = x[1] * s->shape_inv[1], zi = x[2] * s->shape_inv[2];
+  double f;
+  if (s->int_blockiness) {
+    const int n1i = (int)floor(n1 + 1e-2), n2i = (int)floor(n2 + 1e-2);
+    if (n1i == n2i)
+      f = pow_abs_int(xi, n2i) + pow_abs_int(yi, n2i) + pow_abs_int(zi, n1i) - 1.0;
+    else
+      f = (pow_abs(pow_abs_int(xi, n2i) + pow_abs_int(yi, n2i), n1 / n2) + pow_abs_int(zi, n1i) - 1.0);
+  } else {
+    f = (pow_abs(pow_abs(xi, n2) + pow_abs(yi, n2), n1 / n2) + pow_abs(zi, n1) - 1.0);
+  }
+  return s->koef * f;
+}
+
+/* shape_function_gradient_local (superquadric.cpp:139-176) */
+static void gradient_local(const sq_t *s, const double *x, double *r) {
+  const double n1 = s->blockiness[0], n2 = s->blockiness[1];
+  const double a = s->shape_inv[0], b = s->shape_inv[1], c = s->shape_inv[2];
+  const double xa = x[0] * a, yb = x[1] * b, zc = x[2] * c;
+  double xan21, ybn21, zcn11;
+  if (s->int_blockiness) {
+    const int n1i = (int)floor(n1 + 1e-2), n2i = (int)floor(n2 + 1e-2);
+    xan21 = pow_abs_int(xa, n2i - 1);
+    ybn21 = pow_abs_int(yb, n2i - 1);
+    zcn11 = pow_abs_int(zc, n1i - 1);
+  } else {
+    xan21 = pow_abs(xa, n2 - 1.0);
+    ybn21 = pow_abs(yb, n2 - 1.0);
+    zcn11 = pow_abs(zc, n1 - 1.0);
+  }
+  const double k = s->koef;
+  if (comp_double(n1, n2, 1e-2)) {
+    r[0] = n1 * (k * a) * sgn(xa) * xan21;
+    r[1] = n1 * (k * b) * sgn(yb) * ybn21;
+    r[2] = n1 * (k * c) * sgn(zc) * zcn11;
+  } else {
+    const double xy = xan21 * fabs(xa) + ybn21 * fabs(yb);
+    const double xyp1 = pow_abs(xy, n1 / n2 - 1.0);
+    r[0] = n1 * (k * a) * sgn(xa) * xan21 * xyp1;
+    r[1] = n1 * (k * b) * sgn(yb) * ybn21 * xyp1;
+    r[2] = n1 * (k * c) * sgn(zc) * zcn11;
+  }
+}
+
+static double clamp(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* Superquadric::surface_line_intersection (superquadric.cpp:712-816) with
+ * ellipsoid_line_intersection_local (:1011-1035): the point where the line
+ * start + alpha * dir crosses the surface, alpha returned */
+static double surface_line_intersection(const sq_t *s, int max_iters, int use_alpha_ellipsoid,
+                                        const double *start, const double *dir, double alpha_prev,
+                                        double *result) {
+  double sl[3], dl[3], point[3], grad[3];
+  const double d0[3] = {start[0] - s->center[0], start[1] - s->center[1], start[2] - s->center[2]};
+  tmatvec3(s->rot, d0, sl);
+  tmatvec3(s->rot, dir, dl);
+  double alpha = alpha_prev;
+  const double eps0 = 1e-14;
+  double alpha_ell = 0.0;
+  const int ellipsoid = comp_double(s->blockiness[0], 2.0, 1e-2) && comp_double(s->blockiness[1], 2.0, 1e-2);
+  if (ellipsoid || use_alpha_ellipsoid) {
+    const double x0 = sl[0] * s->shape_inv[0], nx = dl[0] * s->shape_inv[0];
+    const double y0 = sl[1] * s->shape_inv[1], ny = dl[1] * s->shape_inv[1];
+    const double z0 = sl[2] * s->shape_inv[2], nz = dl[2] * s->shape_inv[2];
+    const double qa = nx * nx + ny * ny + nz * nz;
+    const double qb = 2.0 * (x0 * nx + y0 * ny + z0 * nz);
+    const double qc = x0 * x0 + y0 * y0 + z0 * z0 - 1.0;
+    const double D = qb * qb - 4.0 * qa * qc;
+    const double two_a_inv = 0.5 / qa;
+    double a1, a2;
+    if (D < 0) {
+      a1 = -qb * two_a_inv;
+      a2 = a1;
+    } else {
+      const double sq = sqrt(D);
+      a1 = (-qb + sq) * two_a_inv;
+      a2 = (-qb - sq) * two_a_inv;
+    }
+    if (ellipsoid) {
+      if (a1 > -eps0 && a2 < -eps0)
+        alpha = a1;
+      else if (a1 < -eps0 && a2 > -eps0)
+        alpha = a2;
+      else if (a1 < -eps0 && a2 < -eps0)
+        alpha = fmax2(a1, a2);
+      else
+        alpha = a1 < a2 ? a1 : a2;
+    } else {
+      const double f = shape_local(s, sl);
+      if (f > eps0)
+        alpha_ell = a1 < a2 ? a1 : a2;
+      else if (f < -eps0)
+        alpha_ell = fmax2(a1, a2);
+      else
+        alpha_ell = alpha = alpha_prev = 0.0;
+    }
+  }
+  if (!ellipsoid) {
+    const double a = 0.5 * min3(s->shape[0], s->shape[1], s->shape[2]) /
+                     sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    const double eps = 1e-16;
+    alpha = clamp(alpha, -a, a);
+    for (int k = 0; k < 3; k++) point[k] = sl[k] + alpha * dl[k];
+    double f0 = shape_local(s, point);
+    if (use_alpha_ellipsoid) {
+      double pe[3];
+      for (int k = 0; k < 3; k++) pe[k] = sl[k] + alpha_ell * dl[k];
+      const double f1 = shape_local(s, pe);
+      if (fabs(f1) < fabs(f0)) {
+        alpha = alpha_ell;
+        f0 = f1;
+        memcpy(point, pe, sizeof pe);
+      }
+    }
+    if (fabs(f0) < eps) {
+      double g[3];
+      matvec3(s->rot, point, g);
+      for (int k = 0; k < 3; k++) result[k] = g[k] + s->center[k];
+      return alpha;
+    }
+    for (int i = 0; i < max_iters; i++) {
+      if (fabs(f0) < eps * s->koef) break;
+      gradient_local(s, point, grad);
+      const double f_der = dot3(grad, dl);
+      double delta;
+      if (fabs(f_der) < 1e-10) {
+        delta = 1e-10;
+        double p_[3], g_[3];
+        for (int k = 0; k < 3; k++) p_[k] = point[k] + delta * dl[k];
+        gradient_local(s, p_, g_);
+        if (dot3(g_, dl) < 0.0) delta = -delta;
+      } else {
+        delta = clamp(-f0 / f_der, -a, a);
+      }
+      if (fabs(delta) <= eps) {
+        alpha += delta;
+        for (int k = 0; k < 3; k++) point[k] = point[k] + delta * dl[k];
+        break;
+      }
+      while (fabs(delta) > eps) {
+        double p_[3];
+        for (int k = 0; k < 3; k++) p_[k] = point[k] + delta * dl[k];
+        const double f_ = shape_local(s, p_);
+        if (fabs(f_) < fabs(f0) || fabs(f_) < eps * s->koef) {
+          f0 = f_;
+          alpha += delta;
+          memcpy(point, p_, sizeof p_);
+          break;
+        }
+        delta *= 0.5;
+      }
+    }
+  }
+  for (int k = 0; k < 3; k++) result[k] = start[k] + alpha * dir[k];
+  return alpha;
+}
+
+/* check_inequalities (math_extra_liggghts_superquadric.cpp:72-150): axis i
+ * separates the two boxes */
+static int obb_separated(int i, const double *a, const double *b, const double *Ad, const double *Bd,
+                         const double *C) {
+  double R = 0.0, R0 = 0.0, R1 = 0.0;
+  switch (i) {
+    case 0: R0 = a[0]; R1 = b[0] * fabs(C[0]) + b[1] * fabs(C[1]) + b[2] * fabs(C[2]); R = fabs(Ad[0]); break;
+    case 1: R0 = a[1]; R1 = b[0] * fabs(C[3]) + b[1] * fabs(C[4]) + b[2] * fabs(C[5]); R = fabs(Ad[1]); break;
+    case 2: R0 = a[2]; R1 = b[0] * fabs(C[6]) + b[1] * fabs(C[7]) + b[2] * fabs(C[8]); R = fabs(Ad[2]); break;
+    case 3: R0 = a[0] * fabs(C[0]) + a[1] * fabs(C[3]) + a[2] * fabs(C[6]); R1 = b[0]; R = fabs(Bd[0]); break;
+    case 4: R0 = a[0] * fabs(C[1]) + a[1] * fabs(C[4]) + a[2] * fabs(C[7]); R1 = b[1]; R = fabs(Bd[1]); break;
+    case 5: R0 = a[0] * fabs(C[2]) + a[1] * fabs(C[5]) + a[2] * fabs(C[8]); R1 = b[2]; R = fabs(Bd[2]); break;
+    case 6: R0 = a[1] * fabs(C[6]) + a[2] * fabs(C[3]); R1 = b[1] * fabs(C[2]) + b[2] * fabs(C[1]); R = fabs(C[3] * Ad[2] - C[6] * Ad[1]); break;
+    case 7: R0 = a[1] * fabs(C[7]) + a[2] * fabs(C[4]); R1 = b[0] * fabs(C[2]) + b[2] * fabs(C[0]); R = fabs(C[4] * Ad[2] - C[7] * Ad[1]); break;
+    case 8: R0 = a[1] * fabs(C[8]) + a[2] * fabs(C[5]); R1 = b[0] * fabs(C[1]) + b[1] * fabs(C[0]); R = fabs(C[5] * Ad[2] - C[8] * Ad[1]); break;
+    case 9: R0 = a[0] * fabs(C[6]) + a[2] * fabs(C[0]); R1 = b[1] * fabs(C[5]) + b[2] * fabs(C[4]); R = fabs(C[6] * Ad[0] - C[0] * Ad[2]); break;
+    case 10: R0 = a[0] * fabs(C[7]) + a[2] * fabs(C[1]); R1 = b[0] * fabs(C[5]) + b[2] * fabs(C[3]); R = fabs(C[7] * Ad[0] - C[1] * Ad[2]); break;
+    case 11: R0 = a[0] * fabs(C[8]) + a[2] * fabs(C[2]); R1 = b[0] * fabs(C[4]) + b[1] * fabs(C[3]); R = fabs(C[8] * Ad[0] - C[2] * Ad[2]); break;
+    case 12: R0 = a[0] * fabs(C[3]) + a[1] * fabs(C[0]); R1 = b[1] * fabs(C[8]) + b[2] * fabs(C[7]); R = fabs(C[0] * Ad[1] - C[3] * Ad[0]); break;
+    case 13: R0 = a[0] * fabs(C[4]) + a[1] * fabs(C[1]); R1 = b[0] * fabs(C[8]) + b[2] * fabs(C[6]); R = fabs(C[1] * Ad[1] - C[4] * Ad[0]); break;
+    case 14: R0 = a[0] * fabs(C[5]) + a[1] * fabs(C[2]); R1 = b[0] * fabs(C[7]) + b[1] * fabs(C[6]); R = fabs(C[2] * Ad[1] - C[5] * Ad[0]); break;
+  }
+  if (fabs(R0 + R1) < 1e-14 && fabs(R) < 1e-14) return 0;
+  return R > R0 + R1;
+}
+
+/* obb_intersect (math_extra_liggghts_superquadric.cpp:166-203): the boxes of
+ * the half-axes grown by scale_koef = 1.0001 (math_extra_liggghts_nonspherical.h:60);
+ * the last separating axis found is tried first next time */
+static int obb_intersect(const sq_t *A, const sq_t *B, int *start) {
+  const double scale_koef = 1.0001;
+  const double delta[3] = {A->center[0] - B->center[0], A->center[1] - B->center[1], A->center[2] - B->center[2]};
+  double a[3], b[3];
+  for (int k = 0; k < 3; k++) {
+    a[k] = A->shape[k] * scale_koef;
+    b[k] = B->shape[k] * scale_koef;
+  }
+  const double *m = A->rot, *m2 = B->rot;
+  double C[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) C[i * 3 + j] = m[0 * 3 + i] * m2[0 * 3 + j] + m[1 * 3 + i] * m2[1 * 3 + j] + m[2 * 3 + i] * m2[2 * 3 + j];
+  double Ad[3], Bd[3];
+  tmatvec3(A->rot, delta, Ad);
+  tmatvec3(B->rot, delta, Bd);
+  for (int i = 0; i < 15; i++) {
+    const int which = i == 0 ? *start : (i == *start ? 0 : i);
+    if (obb_separated(which, a, b, Ad, Bd, C)) {
+      *start = which;
+      return 0;
+    }
+  }
+  return 1;
+}
+
+/* SurfaceModel<SURFACE_SUPERQUADRIC>::checkSurfaceIntersect
+ * (surface_model_superquadric.h:108-205) for one pair, with the contact
+ * history (state, first OBB inequality, contact point, alpha_i, alpha_j) in
+ * hist[7]; calc_contact_point_using_prev_step (math_extra_liggghts_superquadric.cpp:
+ * 1434-1466) takes the pair's velocities (vel: v_i, v_j, omega_i, omega_j),
+ * masses and dt. Returns 1 when the surfaces intersect. */
+int orc_sq_check(const orc_sq_pose *pa, const orc_sq_pose *pb, double *hist, const double *vel,
+                 const double *mass, double dt, double *en, double *deltan, double *cpa, double *cpb,
+                 double *delta, int *iters) {
+  enum { FAR = 0, CLOSE = 1, INTERSECT = 2 };
+  sq_t A, B;
+  sq_init(&A, pa);
+  sq_init(&B, pb);
+  int start = (int)(hist[1] + 0.5);
+  int obb;
+  const int state = (int)(hist[0] + 0.5);
+  if (state == INTERSECT)
+    obb = 1;
+  else
+    obb = obb_intersect(&A, &B, &start);
+  int in_contact = 0;
+  *iters = 0;
+  if (obb) {
+    double fi = 0, fj = 0, cp[3];
+    int fail = 0;
+    const double ri = cbrt(A.shape[0] * A.shape[1] * A.shape[2]);
+    const double rj = cbrt(B.shape[0] * B.shape[1] * B.shape[2]);
+    const double ratio = ri / (ri + rj);
+    if (state == FAR) {
+      *iters = contact_point_fresh(&A, &B, cp, &fi, &fj, &fail);
+    } else {
+      const double *prev = hist + 2;
+      double est[3];
+      if (vel && mass) {
+        const double *vi = vel, *vj = vel + 3, *wi = vel + 6, *wj = vel + 9;
+        double xci[3], xcj[3], ri_[3], rj_[3];
+        for (int k = 0; k < 3; k++) {
+          xci[k] = prev[k] - A.center[k];
+          xcj[k] = prev[k] - B.center[k];
+        }
+        cross3(wi, xci, ri_);
+        cross3(wj, xcj, rj_);
+        for (int k = 0; k < 3; k++) {
+          const double vik = vi[k] + ri_[k], vjk = vj[k] + rj_[k];
+          const double veff = (vik * mass[0] + vjk * mass[1]) / (mass[0] + mass[1]);
+          est[k] = prev[k] + veff * dt;
+        }
+      } else {
+        memcpy(est, prev, sizeof est);
+      }
+      *iters = contact_point(&A, &B, ratio, est, cp, &fi, &fj, &fail);
+    }
+    memcpy(hist + 2, cp, sizeof cp);
+    in_contact = (fi > fj ? fi : fj) < 0.0;
+    if (in_contact) {
+      for (int k = 0; k < 3; k++) en[k] = B.gradient[k] - A.gradient[k];
+      const double nrm = sqrt(en[0] * en[0] + en[1] * en[1] + en[2] * en[2]);
+      const double inv = nrm == 0. ? 0. : 1. / nrm;
+      for (int k = 0; k < 3; k++) en[k] *= inv;
+      /* extended_overlap_algorithm (math_extra_liggghts_superquadric.cpp:1489-1504) */
+      const double men[3] = {-en[0], -en[1], -en[2]};
+      hist[5] = surface_line_intersection(&A, 20, 0, cp, men, hist[5], cpa);
+      hist[6] = surface_line_intersection(&B, 20, 0, cp, en, hist[6], cpb);
+      for (int k = 0; k < 3; k++) delta[k] = cpb[k] - cpa[k];
+      *deltan = fabs(hist[5]) + fabs(hist[6]);
+      hist[0] = INTERSECT;
+    } else {
+      hist[0] = CLOSE;
+    }
+  } else {
+    hist[0] = FAR;
+  }
+  hist[1] = (double)start;
+  return in_contact;
+}
+
+void orc_sq_check_batch(int64_t n, const orc_sq_pose *a, const orc_sq_pose *b, double *hist, const double *vel,
+                        const double *mass, double dt, int *contact, double *en, double *deltan, double *cpa,
+                        double *cpb, int *iters) {
+  for (int64_t k = 0; k < n; k++) {
+    double d[3];
+    contact[k] = orc_sq_check(&a[k], &b[k], hist + 7 * k, vel ? vel + 12 * k : NULL, mass ? mass + 2 * k : NULL,
+                              dt, en + 3 * k, deltan + k, cpa + 3 * k, cpb + 3 * k, d, iters + k);
+  }
+}
+
+int orc_sq_obb(const orc_sq_pose *pa, const orc_sq_pose *pb, int *start) {
+  sq_t A, B;
+  sq_init(&A, pa);
+  sq_init(&B, pb);
+  return obb_intersect(&A, &B, start);
+}
+
+double orc_sq_line(const orc_sq_pose *p, const double *start, const double *dir, double alpha_prev,
+                   double *result) {
+  sq_t s;
+  sq_init(&s, p);
+  return surface_line_intersection(&s, 20, 0, start, dir, alpha_prev, result);
+}
+
 void orc_sq_props(const orc_sq_pose *p, const double *x, double *f, double *grad, double *hess) {
   sq_t s;
   sq_init(&s, p);

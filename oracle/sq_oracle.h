@@ -31,6 +31,24 @@ int orc_sq_contact(const orc_sq_pose *a, const orc_sq_pose *b, const double *ini
 void orc_sq_contact_batch(int64_t n, const orc_sq_pose *a, const orc_sq_pose *b, const double *init,
                           double *point, double *fi, double *fj, int *iters, int *fail);
 
+/* SurfaceModel<SURFACE_SUPERQUADRIC>::checkSurfaceIntersect for one pair:
+ * hist[7] = state (0 far, 1 close, 2 intersecting), first OBB inequality,
+ * contact point[3], alpha_i, alpha_j (in/out); vel = v_i, v_j, omega_i,
+ * omega_j (12) and mass = m_i, m_j may be NULL (a still pair). Returns 1 when
+ * the surfaces intersect, with the normal en (from i to j), the overlap
+ * deltan, the surface points on i and j, delta = cpb - cpa. */
+int orc_sq_check(const orc_sq_pose *a, const orc_sq_pose *b, double *hist, const double *vel,
+                 const double *mass, double dt, double *en, double *deltan, double *cpa, double *cpb,
+                 double *delta, int *iters);
+void orc_sq_check_batch(int64_t n, const orc_sq_pose *a, const orc_sq_pose *b, double *hist, const double *vel,
+                        const double *mass, double dt, int *contact, double *en, double *deltan, double *cpa,
+                        double *cpb, int *iters);
+/* obb_intersect with the first inequality to try (in/out). */
+int orc_sq_obb(const orc_sq_pose *a, const orc_sq_pose *b, int *start);
+/* Superquadric::surface_line_intersection (20 iterations, no ellipsoid start). */
+double orc_sq_line(const orc_sq_pose *p, const double *start, const double *dir, double alpha_prev,
+                   double *result);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,11 @@ network, through the C ABI (dem_sq_*):
   network inputs                 the device encoding equals superquadrics.py
   network forward / gradient     equal to the numpy checker with the same BF16
                                  rounding points (1e-3 norm-wise)
+  contact detection              checkSurfaceIntersect over an approach
+                                 sequence (far -> close -> intersecting):
+                                 states, OBB axes and contact flags equal the
+                                 oracle's; points, normals, overlaps and
+                                 surface points to 1e-12 m (1e-9 normals)
   training + warm start          the trained network cuts Newton iterations
                                  per fresh contact at least 2x; warm-started
                                  points agree with the plain solve (median
@@ -119,6 +124,68 @@ def test_network_forward_and_gradient_match_checker():
     s.close()
 
 
+def _approach(seed, n, steps):
+    """Pose pairs closing in along their line of centres, B turning: the
+    poses of each step and the velocities that move them."""
+    A, B = sq.overlapping_pairs(seed, n)
+    rng = np.random.default_rng(seed)
+    u = B[:, 0:3] - A[:, 0:3]
+    dt = 1e-4
+    spin = rng.normal(0, 20.0, (n, 3))  # rad/s
+    seq = []
+    for t in range(steps):
+        Bt = B.copy()
+        Bt[:, 0:3] = A[:, 0:3] + u * (1.3 - 0.35 * t / (steps - 1))
+        ang = spin * dt * t
+        th = np.linalg.norm(ang, axis=1, keepdims=True)
+        axis = ang / np.maximum(th, 1e-300)
+        dq = np.concatenate([np.cos(th / 2), axis * np.sin(th / 2)], axis=1)
+        Bt[:, 3:7] = sq.quat_mul(dq, B[:, 3:7])
+        vel = np.zeros((n, 12))
+        vel[:, 3:6] = -0.35 / (steps - 1) * u / dt
+        vel[:, 9:12] = spin
+        seq.append((A, Bt, vel))
+    mass = np.stack([np.prod(A[:, 7:10], axis=1), np.prod(B[:, 7:10], axis=1)], axis=1) * 2500.0
+    return seq, mass, dt
+
+
+def test_contact_detection_matches_oracle():
+    say("[gpu] superquadric contact detection over an approach sequence vs oracle")
+    n, steps = 500, 8
+    seq, mass, dt = _approach(41, n, steps)
+    s = pydem.SqSolver()
+    hg = np.zeros((n, 7))
+    ho = np.zeros((n, 7))
+    seen = set()
+    for t, (A, B, vel) in enumerate(seq):
+        s.set_pairs(A, B)
+        r = s.check(hg, vel, mass, dt)
+        ho, oc, oen, odn, ocpa, ocpb, oit = orc.sq_check_batch(A, B, ho, vel, mass, dt)
+        seen.update(np.unique(hg[:, 0]).tolist())
+        assert np.array_equal(hg[:, 0], ho[:, 0]), t
+        assert np.array_equal(hg[:, 1], ho[:, 1]), t
+        assert np.array_equal(r["contact"], oc), t
+        c = oc == 1
+        dp = np.abs(hg[:, 2:5] - ho[:, 2:5]).max(axis=1)
+        same = dp <= 1e-12
+        say(f"  step {t}: far {int((ho[:, 0] == 0).sum())} close {int((ho[:, 0] == 1).sum())} "
+            f"intersecting {int(c.sum())}  |dp| max {dp.max():.2e}  iterations equal "
+            f"{(r['iters'] == oit).mean():.3f}  solve {r['solve_ms']:.2f} ms")
+        assert same.mean() >= 0.98
+        cs = c & same
+        assert np.abs(r["deltan"][cs] - odn[cs]).max(initial=0) <= 1e-12
+        assert np.abs(r["en"][cs] - oen[cs]).max(initial=0) <= 1e-9
+        assert np.abs(r["cp_i"][cs] - ocpa[cs]).max(initial=0) <= 1e-12
+        assert np.abs(r["cp_j"][cs] - ocpb[cs]).max(initial=0) <= 1e-12
+        assert np.abs(hg[cs, 5:7] - ho[cs, 5:7]).max(initial=0) <= 1e-12
+        assert (r["iters"] == oit).mean() >= 0.95
+        assert np.all(((r["flags"] & 4) != 0) == (ho[:, 0] == 0))
+        # continue both from the oracle's history so one near-tie cannot fork the sequences
+        hg = ho.copy()
+    assert seen == {0.0, 1.0, 2.0}
+    s.close()
+
+
 def test_training_and_warm_start_cut_newton_iterations():
     say("[gpu] network training and warm start")
     A, B = sq.overlapping_pairs(24, 60000)
@@ -152,4 +219,15 @@ def test_training_and_warm_start_cut_newton_iterations():
         f"|dp| q99 {np.quantile(d1, 0.99):.2e} max {d1.max():.2e}, within 1e-9 m {(d1 <= 1e-9).mean():.4f}")
     assert (d1 <= 1e-9).mean() >= 0.99
     assert p1["iters"].mean() / w1["iters"].mean() >= 2.0
+    # contact detection of new pairs from the network: the same contacts and
+    # overlaps as from the homotopy, with fewer Newton iterations
+    cf = s.check(None, start=pydem.DEM_SQ_FRESH)
+    cw = s.check(None, start=pydem.DEM_SQ_NETWORK)
+    both = (cf["contact"] == 1) & (cw["contact"] == 1)
+    ddn = np.abs(cw["deltan"] - cf["deltan"])[both]
+    say(f"  contact detection: contacts {int(cf['contact'].sum())} / {int(cw['contact'].sum())}, "
+        f"iterations {cf['iters'].mean():.2f} -> {cw['iters'].mean():.2f}, |d deltan| median {np.median(ddn):.2e}")
+    assert (cf["contact"] == cw["contact"]).mean() >= 0.995
+    assert np.median(ddn) < 1e-9
+    assert cf["iters"].mean() / cw["iters"].mean() >= 2.0
     s.close()
