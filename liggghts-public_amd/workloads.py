@@ -273,3 +273,47 @@ def instance_slice(particles, counts, q):
     off = int(np.sum(counts[:q]))
     n = int(counts[q])
     return {k: v[off:off + n] for k, v in particles.items()}
+
+
+def superquadric_settling(seed=0, n=200_000, base=(0.2, 0.2), axes=(0.001, 0.003), blockiness=(2.0, 8.0),
+                          density=2500.0, dt=1e-5, skin=0.001, gap=0.0005, youngs=5e6, poisson=0.45,
+                          restitution=0.3, friction=0.5):
+    """Superquadrics settling onto a floor: half-axes uniform in ``axes``,
+    blockiness n1, n2 uniform in ``blockiness``, uniformly random unit
+    quaternions, atom_style superquadric data (shape, blockiness, density).
+    The base is periodic in x and y with a primitive zplane floor at z = 0.
+    Particles start at rest on a jittered layer lattice whose spacing in each
+    layer is the largest circumscribing box diagonal there plus ``gap`` (the
+    largest shapes in the lowest layers), so no two start in contact.
+    Returns (params, materials, particles, walls) with particles a dict of x,
+    v, omega (n,3), quat (n,4), shape (n,3), blockiness (n,2), density (n,),
+    type (n,) int32."""
+    rng = np.random.default_rng(seed)
+    shape = rng.uniform(axes[0], axes[1], (n, 3))
+    blk = rng.uniform(blockiness[0], blockiness[1], (n, 2))
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    bound = np.sqrt((shape * shape).sum(axis=1)) * 1.0001  # a superquadric with n >= 2 fits in its box
+    order = np.argsort(-bound, kind="stable")
+    x = np.zeros((n, 3))
+    z = 0.0
+    k = 0
+    while k < n:
+        s = 2.0 * bound[order[k]] + gap
+        nx, ny = max(1, int(base[0] / s)), max(1, int(base[1] / s))
+        sx, sy = base[0] / nx, base[1] / ny
+        m = min(nx * ny, n - k)
+        ids = order[k:k + m]
+        cell = np.arange(m)
+        jit = 0.25 * gap
+        x[ids, 0] = (cell % nx + 0.5) * sx + rng.uniform(-jit, jit, m)
+        x[ids, 1] = (cell // nx + 0.5) * sy + rng.uniform(-jit, jit, m)
+        x[ids, 2] = z + 0.5 * s
+        z += s
+        k += m
+    params = {"dt": dt, "skin": skin, "boxlo": [0.0, 0.0, 0.0], "boxhi": [base[0], base[1], max(z, 0.4) + 0.01],
+              "periodic": [1, 1, 0], "grav_acc": [0.0, 0.0, -9.81], "tangential_damping": 1}
+    mats = materials_single(youngs, poisson, restitution, friction)
+    P = {"x": x, "v": np.zeros((n, 3)), "omega": np.zeros((n, 3)), "quat": q, "shape": shape,
+         "blockiness": blk, "density": np.full(n, density), "type": np.ones(n, dtype=np.int32)}
+    return params, mats, P, [(2, 0.0, 1)]

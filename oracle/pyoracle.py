@@ -430,3 +430,187 @@ def sq_line(pose, start, direction, alpha_prev=0.0):
                         _dp(np.ascontiguousarray(start, dtype=np.float64)),
                         _dp(np.ascontiguousarray(direction, dtype=np.float64)), alpha_prev, _dp(res))
     return a, res
+
+
+# ---------------------------------------------------------------------------
+# superquadric DEM step (sqdem_oracle.c)
+# ---------------------------------------------------------------------------
+SQS_HIST = 10  # state, first OBB inequality, contact point[3], alpha_i, alpha_j, shear[3]
+
+
+class OrcSqsParams(ctypes.Structure):
+    _fields_ = [
+        ("dt", _D), ("skin", _D),
+        ("boxlo", _D * 3), ("boxhi", _D * 3), ("periodic", ctypes.c_int * 3),
+        ("tangential_damping", ctypes.c_int), ("limit_force", ctypes.c_int),
+        ("has_gravity", ctypes.c_int), ("grav_acc", _D * 3),
+        ("nplanes", ctypes.c_int), ("plane_axis", ctypes.c_int * MAXPLANE),
+        ("plane_pos", _D * MAXPLANE), ("plane_type", ctypes.c_int * MAXPLANE),
+        ("gravity_first", ctypes.c_int),
+    ]
+
+
+def _sqs_lib():
+    lib = _sq_lib()
+    if not getattr(lib, "_sqs_ready", False):
+        P = ctypes.POINTER
+        dp, ip, vp = P(_D), P(ctypes.c_int), ctypes.c_void_p
+        i64p = P(ctypes.c_int64)
+        lib.orc_sqs_mass_props.argtypes = [dp, dp, _D, dp, dp]
+        lib.orc_sq_bounding_radius.argtypes = [dp, dp]
+        lib.orc_sq_bounding_radius.restype = _D
+        lib.orc_sq_plane.argtypes = [dp, dp, dp, dp, dp]
+        lib.orc_sq_plane.restype = ctypes.c_int
+        lib.orc_sqs_contact_pair.argtypes = [P(OrcMaterials), ctypes.c_int, ctypes.c_int, _D, ctypes.c_int,
+                                             ctypes.c_int, dp, dp, dp, dp, dp, dp, _D, _D, _D, _D, ctypes.c_int,
+                                             dp, dp, dp, dp]
+        lib.orc_sqs_contact_pair.restype = ctypes.c_int
+        lib.orc_sqs_create.argtypes = [P(OrcSqsParams), P(OrcMaterials), ctypes.c_int, dp, dp, dp, dp, dp, dp,
+                                       dp, ip]
+        lib.orc_sqs_create.restype = vp
+        lib.orc_sqs_destroy.argtypes = [vp]
+        lib.orc_sqs_setup.argtypes = [vp]
+        lib.orc_sqs_run.argtypes = [vp, ctypes.c_int]
+        lib.orc_sqs_get_state.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp]
+        lib.orc_sqs_get_props.argtypes = [vp, dp, dp, dp, dp]
+        lib.orc_sqs_get_pairs.argtypes = [vp, ctypes.c_int64, ip, ip, dp]
+        lib.orc_sqs_get_pairs.restype = ctypes.c_int64
+        lib.orc_sqs_get_wall_hist.argtypes = [vp, dp]
+        lib.orc_sqs_stats.argtypes = [vp, i64p, i64p, i64p, i64p, i64p]
+        lib._sqs_ready = True
+    return lib
+
+
+def sq_mass_props(shape, blk, density):
+    """(volume, principal inertia (3,)) of one superquadric."""
+    lib = _sqs_lib()
+    vol = np.zeros(1)
+    I = np.zeros(3)
+    lib.orc_sqs_mass_props(_dp(np.ascontiguousarray(shape, dtype=np.float64)),
+                           _dp(np.ascontiguousarray(blk, dtype=np.float64)), float(density), _dp(vol), _dp(I))
+    return vol[0], I
+
+
+def sq_bounding_radius(shape, blk):
+    return _sqs_lib().orc_sq_bounding_radius(_dp(np.ascontiguousarray(shape, dtype=np.float64)),
+                                             _dp(np.ascontiguousarray(blk, dtype=np.float64)))
+
+
+def sq_plane(pose, normal, x0):
+    """Superquadric::plane_intersection: (intersect, point of maximal penetration, point of lowest potential)."""
+    lib = _sqs_lib()
+    pm, pl = np.zeros(3), np.zeros(3)
+    hit = lib.orc_sq_plane(_dp(np.ascontiguousarray(pose, dtype=np.float64)),
+                           _dp(np.ascontiguousarray(normal, dtype=np.float64)),
+                           _dp(np.ascontiguousarray(x0, dtype=np.float64)), _dp(pm), _dp(pl))
+    return bool(hit), pm, pl
+
+
+def sqs_contact_pair(mats, a, b, vi, vj, wi, wj, mi, mj, rvi, rvj, dt, shearupdate, hist=None, itype=1, jtype=1,
+                     tangential_damping=1, limit_force=0):
+    """One superquadric pair contact: (touch, dFi, dTi, dTj, hist_after)."""
+    lib = _sqs_lib()
+    arr = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    h = np.zeros(SQS_HIST) if hist is None else arr(hist).copy()
+    dF, dTi, dTj = np.zeros(3), np.zeros(3), np.zeros(3)
+    touch = lib.orc_sqs_contact_pair(ctypes.byref(mats), int(tangential_damping), int(limit_force), float(dt),
+                                     int(itype), int(jtype), _dp(arr(a)), _dp(arr(b)), _dp(arr(vi)), _dp(arr(vj)),
+                                     _dp(arr(wi)), _dp(arr(wj)), float(mi), float(mj), float(rvi), float(rvj),
+                                     int(shearupdate), _dp(h), _dp(dF), _dp(dTi), _dp(dTj))
+    return bool(touch), dF, dTi, dTj, h
+
+
+def make_sqs_params(params, walls=()):
+    p = OrcSqsParams()
+    p.dt = params["dt"]
+    p.skin = params["skin"]
+    for d in range(3):
+        p.boxlo[d] = params["boxlo"][d]
+        p.boxhi[d] = params["boxhi"][d]
+        p.periodic[d] = int(params["periodic"][d])
+    p.tangential_damping = int(params.get("tangential_damping", 1))
+    p.limit_force = int(params.get("limit_force", 0))
+    g = params.get("grav_acc")
+    p.has_gravity = int(g is not None)
+    if g is not None:
+        for d in range(3):
+            p.grav_acc[d] = g[d]
+    p.nplanes = len(walls)
+    for k, wall in enumerate(walls):
+        axis, pos, wtype = wall[:3]
+        p.plane_axis[k] = int(axis)
+        p.plane_pos[k] = float(pos)
+        p.plane_type[k] = int(wtype)
+    p.gravity_first = int(params.get("gravity_first", 1))
+    return p
+
+
+class SqsSystem:
+    """The oracle's superquadric DEM system. P: dict of arrays x, v, omega (n,3),
+    quat (n,4), shape (n,3), blockiness (n,2), density (n,), type (n,)."""
+
+    def __init__(self, params, mats, P, walls=()):
+        lib = _sqs_lib()
+        self.lib = lib
+        self.n = len(P["x"])
+        self.nplanes = len(walls)
+        self.p = make_sqs_params(params, walls)
+        self.mats = mats
+        arr = lambda k, shape: np.ascontiguousarray(P[k], dtype=np.float64).reshape(shape)
+        self._keep = [arr("x", (self.n, 3)), arr("v", (self.n, 3)), arr("omega", (self.n, 3)),
+                      arr("quat", (self.n, 4)), arr("shape", (self.n, 3)), arr("blockiness", (self.n, 2)),
+                      arr("density", (self.n,)), np.ascontiguousarray(P["type"], dtype=np.int32)]
+        k = self._keep
+        self.h = lib.orc_sqs_create(ctypes.byref(self.p), ctypes.byref(mats), self.n, _dp(k[0]), _dp(k[1]),
+                                    _dp(k[2]), _dp(k[3]), _dp(k[4]), _dp(k[5]), _dp(k[6]), _ip(k[7]))
+
+    def setup(self):
+        self.lib.orc_sqs_setup(self.h)
+
+    def run(self, nsteps):
+        self.lib.orc_sqs_run(self.h, int(nsteps))
+
+    def state(self):
+        n = self.n
+        out = {k: np.zeros((n, 3)) for k in ("x", "v", "omega", "angmom", "f", "torque")}
+        out["quat"] = np.zeros((n, 4))
+        self.lib.orc_sqs_get_state(self.h, _dp(out["x"]), _dp(out["v"]), _dp(out["omega"]), _dp(out["angmom"]),
+                                   _dp(out["quat"]), _dp(out["f"]), _dp(out["torque"]))
+        return out
+
+    def props(self):
+        n = self.n
+        m, rv, rb, I = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros((n, 3))
+        self.lib.orc_sqs_get_props(self.h, _dp(m), _dp(rv), _dp(rb), _dp(I))
+        return {"mass": m, "rvol": rv, "rbound": rb, "inertia": I}
+
+    def pairs(self):
+        need = -self.lib.orc_sqs_get_pairs(self.h, 0, None, None, None)
+        cap = max(int(need), 1)
+        ij = np.zeros((cap, 2), dtype=np.int32)
+        fl = np.zeros(cap, dtype=np.int32)
+        h = np.zeros((cap, SQS_HIST))
+        m = self.lib.orc_sqs_get_pairs(self.h, cap, _ip(ij), _ip(fl), _dp(h))
+        return ij[:m], fl[:m], h[:m]
+
+    def wall_hist(self):
+        h = np.zeros((self.n, max(self.nplanes, 1), 3))
+        if self.nplanes:
+            self.lib.orc_sqs_get_wall_hist(self.h, _dp(h))
+        return h[:, :self.nplanes]
+
+    def stats(self):
+        v = [ctypes.c_int64() for _ in range(5)]
+        self.lib.orc_sqs_stats(self.h, *[ctypes.byref(x) for x in v])
+        return dict(zip(("pair_touch", "wall_touch", "newton", "rebuilds", "steps"), [x.value for x in v]))
+
+    def close(self):
+        if self.h:
+            self.lib.orc_sqs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -914,6 +914,93 @@ double orc_sq_line(const orc_sq_pose *p, const double *start, const double *dir,
   return surface_line_intersection(&s, 20, 0, start, dir, alpha_prev, result);
 }
 
+/* Superquadric::plane_intersection (superquadric.cpp:440-507): normal and
+ * x0 as passed by the wall (un-normalised normal; the 1e-10 tests are on its
+ * raw components). Returns 1 when particle and plane intersect. */
+int orc_sq_plane(const orc_sq_pose *p, const double *normal_global, const double *x0_global, double *pmax,
+                 double *plow) {
+  sq_t s;
+  sq_init(&s, p);
+  double nl[3], x0l[3], pm[3], pl[3];
+  tmatvec3(s.rot, normal_global, nl);
+  const double d0[3] = {x0_global[0] - s.center[0], x0_global[1] - s.center[1], x0_global[2] - s.center[2]};
+  tmatvec3(s.rot, d0, x0l);
+  if (dot3(nl, x0l) < 0.0) {
+    nl[0] = -nl[0];
+    nl[1] = -nl[1];
+    nl[2] = -nl[2];
+  }
+  const double n1 = s.blockiness[0], n2 = s.blockiness[1];
+  const double nx = nl[0], ny = nl[1], nz = nl[2];
+  const double a = s.shape[0], b = s.shape[1], c = s.shape[2];
+  const double d = dot3(nl, x0l);
+  if (comp_double(nx, 0.0, 1e-10) && comp_double(ny, 0.0, 1e-10)) {
+    pm[0] = pm[1] = 0.0;
+    pm[2] = 1.0;
+    pl[0] = pl[1] = 0.0;
+    pl[2] = d / nz;
+  } else {
+    if (fabs(nx) > fabs(ny)) {
+      const double alpha = pow_abs((ny * b) / (nx * a), 1.0 / (n2 - 1.0));
+      const double gamma1 = 1.0 + pow_abs(alpha, n2);
+      const double gamma = pow_abs(gamma1, n1 / n2 - 1.0);
+      const double beta = pow_abs((nz * c) / (nx * a) * gamma, 1.0 / (n1 - 1.0));
+      pm[0] = 1.0 / pow_abs(pow_abs(gamma1, n1 / n2) + pow_abs(beta, n1), 1.0 / n1);
+      pm[1] = alpha * pm[0];
+      pm[2] = beta * pm[0];
+      pl[0] = fabs(d) / (fabs(nx) * a + alpha * fabs(ny) * b + beta * fabs(nz) * c);
+      pl[1] = alpha * pl[0];
+      pl[2] = beta * pl[0];
+    } else {
+      const double alpha = pow_abs((nx * a) / (ny * b), 1.0 / (n2 - 1.0));
+      const double gamma1 = 1.0 + pow_abs(alpha, n2);
+      const double gamma = pow_abs(gamma1, n1 / n2 - 1.0);
+      const double beta = pow_abs((nz * c) / (ny * b) * gamma, 1.0 / (n1 - 1.0));
+      pm[1] = 1.0 / pow_abs(pow_abs(gamma1, n1 / n2) + pow_abs(beta, n1), 1.0 / n1);
+      pm[0] = alpha * pm[1];
+      pm[2] = beta * pm[1];
+      pl[1] = fabs(d) / (fabs(ny) * b + alpha * fabs(nx) * a + beta * fabs(nz) * c);
+      pl[0] = alpha * pl[1];
+      pl[2] = beta * pl[1];
+    }
+    for (int i = 0; i < 3; i++) pl[i] *= s.shape[i] * sgn(nl[i]);
+  }
+  for (int i = 0; i < 3; i++) pm[i] *= s.shape[i] * sgn(nl[i]);
+  const double ff = shape_local(&s, pl);
+  double g[3];
+  matvec3(s.rot, pm, g);
+  for (int k = 0; k < 3; k++) pmax[k] = g[k] + s.center[k];
+  matvec3(s.rot, pl, g);
+  for (int k = 0; k < 3; k++) plow[k] = g[k] + s.center[k];
+  return ff < 0.0;
+}
+
+/* bounding_sphere_radius_superquadric (math_extra_liggghts_superquadric.cpp:
+ * 206-236), grown by scale_koef = 1.0001 */
+double orc_sq_bounding_radius(const double *shape, const double *blk) {
+  double r;
+  if (comp_double(blk[0], 2.0, 1e-13) && comp_double(blk[1], 2.0, 1e-13)) {
+    r = fmax2(fmax2(shape[0], shape[1]), shape[2]);
+  } else {
+    const double a = fmax2(shape[0], shape[1]);
+    const double b = shape[0] < shape[1] ? shape[0] : shape[1];
+    const double c = shape[2];
+    if (comp_double(blk[1], 2.0, 1e-13)) {
+      r = sqrt(a * a + c * c);
+    } else {
+      const double n1 = fmax2(2.1, blk[0]);
+      const double n2 = fmax2(2.1, blk[1]);
+      const double alpha = pow_abs(b / a, 2.0 / (n2 - 2.0));
+      const double gamma = pow_abs(1.0 + pow_abs(alpha, n2), n1 / n2 - 1.0);
+      const double beta = pow_abs(c * c / (a * a) * gamma, 1.0 / (n1 - 2.0));
+      const double x0 = 1.0 / pow_abs(pow_abs(1.0 + pow_abs(alpha, n2), n1 / n2) + pow_abs(beta, n1), 1.0 / n1);
+      const double xt = x0 * a, yt = alpha * x0 * b, zt = beta * x0 * c;
+      r = sqrt(xt * xt + yt * yt + zt * zt);
+    }
+  }
+  return r * 1.0001;
+}
+
 void orc_sq_props(const orc_sq_pose *p, const double *x, double *f, double *grad, double *hess) {
   sq_t s;
   sq_init(&s, p);

@@ -43,6 +43,13 @@ int orc_sq_check(const orc_sq_pose *a, const orc_sq_pose *b, double *hist, const
 void orc_sq_check_batch(int64_t n, const orc_sq_pose *a, const orc_sq_pose *b, double *hist, const double *vel,
                         const double *mass, double dt, int *contact, double *en, double *deltan, double *cpa,
                         double *cpb, int *iters);
+/* Superquadric::plane_intersection: the particle's point of maximal
+ * penetration and the wall's point of lowest potential; returns 1 when they
+ * intersect. */
+int orc_sq_plane(const orc_sq_pose *p, const double *normal_global, const double *x0_global, double *pmax,
+                 double *plow);
+/* bounding_sphere_radius_superquadric, times scale_koef. */
+double orc_sq_bounding_radius(const double *shape, const double *blk);
 /* obb_intersect with the first inequality to try (in/out). */
 int orc_sq_obb(const orc_sq_pose *a, const orc_sq_pose *b, int *start);
 /* Superquadric::surface_line_intersection (20 iterations, no ellipsoid start). */
