@@ -336,6 +336,95 @@ int dem_sq_net_gradient(dem_sq_handle *h, int64_t n, const float *X, const float
 int dem_sq_net_train(dem_sq_handle *h, int64_t steps, int64_t batch, float lr, uint64_t seed,
                      float *loss_curve, double *train_ms);
 
+/* ------------------------------------------------------------------------
+ * Superquadric DEM step (atom_style superquadric): FP64 on the GPU, one
+ * handle per simulation. Replaces, for superquadric particles, the per-step
+ * path of Verlet::setup / Verlet::run (verlet.cpp:134-205, 264-392) with
+ *   pair_style gran model hertz tangential history surface superquadric
+ *       (PairGran::compute, pair_gran_base.h:187-500, with
+ *       SurfaceModel<SURFACE_SUPERQUADRIC>, surface_model_superquadric.h:108-220)
+ *   fix wall/gran ... primitive type t {x,y,z}plane pos
+ *       (FixWallGran::post_force_primitive, fix_wall_gran.cpp:1024-1153)
+ *   fix gravity (fix_gravity.cpp:301-350)
+ *   fix nve/superquadric (fix_nve_superquadric.cpp:45-75 over
+ *       FixNVEAsphereBase, fix_nve_asphere_base.cpp:355-610, scheme 1)
+ * and the granular neighbor list on the bounding spheres with contact
+ * history (neigh_gran.cpp:485-639, fix_contact_history.cpp:305-425).
+ * ------------------------------------------------------------------------ */
+typedef struct dem_sqdem_handle dem_sqdem_handle;
+
+typedef struct {
+  double dt;                 /* timestep */
+  double skin;               /* neighbor <skin> bin */
+  double boxlo[3], boxhi[3]; /* region of the simulation box */
+  int periodic[3];           /* boundary p (1) or f (0) per dimension */
+  int tangential_damping;    /* pair_style gran ... tangential_damping on|off (on) */
+  int limit_force;           /* limitForce on|off (off) */
+  int has_gravity;           /* fix gravity present */
+  double gravity[3];         /* magnitude times the unit direction */
+  int gravity_first;         /* fix gravity defined before the walls */
+  int nplanes;               /* fix wall/gran ... primitive planes (<= 6) */
+  int plane_axis[6];         /* 0 xplane, 1 yplane, 2 zplane */
+  double plane_pos[6];
+  int plane_type[6];         /* wall atom type */
+} dem_sqdem_params;
+
+typedef struct {
+  int64_t steps, rebuilds;
+  int64_t pair_touch, wall_touch; /* touching pairs / particle-plane contacts, last step */
+  int64_t newton_last;            /* Newton iterations, last step */
+  int64_t newton, checks;         /* totals: Newton iterations, pair contact detections */
+  int64_t fresh, fresh_iters;     /* totals: solves of pairs far apart the step before (OBBs
+                                     overlapping), their Newton iterations */
+  int64_t fallbacks;              /* totals: network starts replaced by the homotopy */
+  double run_ms;                  /* GPU time of the last dem_sqdem_run (HIP events) */
+} dem_sqdem_stats;
+
+int dem_sqdem_create(int device, const dem_sqdem_params *p, dem_sqdem_handle **out);
+int dem_sqdem_destroy(dem_sqdem_handle *h);
+const char *dem_sqdem_last_error(const dem_sqdem_handle *h);
+
+/* fix property/global youngsModulus / poissonsRatio (per type),
+ * coefficientRestitution / coefficientFriction (ntypes*ntypes), mixed as
+ * global_properties.cpp:411-558 */
+int dem_sqdem_set_materials(dem_sqdem_handle *h, int ntypes, const double *Y, const double *nu, const double *e,
+                            const double *mu);
+
+/* The Atoms of read_data for atom_style superquadric
+ * (AtomVecSuperquadric::data_atom, atom_vec_superquadric.cpp:1150-1212): x, v,
+ * omega (n*3), quat (n*4, w first, unit), shape (half-axes, n*3), blockiness
+ * (n*2, >= 2), density (n), type (n, 1-based). The library derives the
+ * bounding radius, volume, mass and principal inertia as data_atom does. */
+int dem_sqdem_set_particles(dem_sqdem_handle *h, int64_t n, const double *x, const double *v, const double *omega,
+                            const double *quat, const double *shape, const double *blockiness,
+                            const double *density, const int *type);
+
+/* Warm start for pairs far apart the step before: start their Newton solve
+ * from the contact point predicted by the trained network of `net` (same
+ * device; it must outlive its use here and not train meanwhile), with the
+ * homotopy of calc_contact_point_if_no_previous_point_avaialable
+ * (math_extra_liggghts_superquadric.cpp:1390-1430) as fallback. NULL: the
+ * reference's homotopy only. No reference counterpart. */
+int dem_sqdem_set_warm_start(dem_sqdem_handle *h, dem_sq_handle *net);
+
+/* Runs nsteps timesteps; the first call also runs the setup (forces without
+ * history update), as `run` does. */
+int dem_sqdem_run(dem_sqdem_handle *h, int nsteps);
+
+/* Current state (each pointer may be NULL): x, v, omega, angmom, f, torque
+ * (n*3), quat (n*4). */
+int dem_sqdem_get_state(dem_sqdem_handle *h, double *x, double *v, double *omega, double *angmom, double *quat,
+                        double *f, double *torque);
+/* mass, volume-equivalent radius, bounding radius (n each), inertia (n*3) */
+int dem_sqdem_get_props(dem_sqdem_handle *h, double *mass, double *rvol, double *rbound, double *inertia);
+/* The pair list (i < j by index): ij (2 per pair), touch flags, histories
+ * (10 doubles: state, first OBB inequality, contact point[3], alpha_i,
+ * alpha_j, shear[3]). Returns the count, or -count when cap is too small. */
+int64_t dem_sqdem_get_pairs(dem_sqdem_handle *h, int64_t cap, int *ij, int *flags, double *hist);
+/* wall shear history, n*nplanes*3 */
+int dem_sqdem_get_wall_history(dem_sqdem_handle *h, double *hist);
+int dem_sqdem_get_stats(dem_sqdem_handle *h, dem_sqdem_stats *out);
+
 /* Runs an input script with the reference's command subset for this path
  * (Input::file, input.cpp:183, and Input::execute_command, input.cpp:560-650).
  * log_path may be NULL (log to stdout). Returns a dem_status. */

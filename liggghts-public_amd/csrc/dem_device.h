@@ -221,5 +221,8 @@ hipError_t launch_mesh_bin_count(const RebuildArgs &a, int *total, hipStream_t s
 // longer a candidate.
 hipError_t launch_mesh_bin_fill(const RebuildArgs &a, hipStream_t s);
 int scan_tmp_size(int ncell);
+// out[0..n) = exclusive prefix sums of in[0..n), out[n] = total; tmp holds
+// scan_tmp_size(n) ints
+hipError_t launch_exclusive_scan(const int *in, int *out, int *tmp, int n, hipStream_t s);
 
 }  // namespace dem

@@ -1479,6 +1479,12 @@ static void launch_scan(const int *in, int *out, int *tmp, int n, hipStream_t s)
   hipLaunchKernelGGL(k_scan_add, dim3(nsb), dim3(256), 0, s, out, tmp, n);
 }
 
+hipError_t launch_exclusive_scan(const int *in, int *out, int *tmp, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  launch_scan(in, out, tmp, n, s);
+  return hipGetLastError();
+}
+
 hipError_t launch_mesh_bin_count(const RebuildArgs &a, int *total, hipStream_t s) {
   if (a.ntri <= 0) return hipSuccess;
   const int nt = nblk(a.ntri, kBlock);

@@ -98,6 +98,16 @@ int ensure_batch(dem_sq_handle *h, int64_t batch) {
 
 }  // namespace
 
+namespace sqk {
+// the trained network of a solver handle, for the superquadric DEM step's
+// warm start (null when the handle has no network)
+const NetDev *handle_net(const dem_sq_handle *h, int *device) {
+  if (!h || !h->net_ready) return nullptr;
+  if (device) *device = h->device;
+  return &h->net;
+}
+}  // namespace sqk
+
 extern "C" {
 
 const char *dem_sq_last_error(const dem_sq_handle *h) { return h ? h->err.c_str() : g_sq_create_error.c_str(); }

@@ -29,6 +29,9 @@ EXPORTED = [
     "dem_sq_last_error", "dem_sq_set_pairs", "dem_sq_solve", "dem_sq_check", "dem_sq_get_results",
     "dem_sq_get_features", "dem_sq_net_init", "dem_sq_net_set_params", "dem_sq_net_get_params",
     "dem_sq_net_forward", "dem_sq_net_gradient", "dem_sq_net_train",
+    "dem_sqdem_create", "dem_sqdem_destroy", "dem_sqdem_last_error", "dem_sqdem_set_materials",
+    "dem_sqdem_set_particles", "dem_sqdem_set_warm_start", "dem_sqdem_run", "dem_sqdem_get_state",
+    "dem_sqdem_get_props", "dem_sqdem_get_pairs", "dem_sqdem_get_wall_history", "dem_sqdem_get_stats",
 ]
 
 DEM_SQ_FRESH, DEM_SQ_GIVEN, DEM_SQ_NETWORK = 0, 1, 2
@@ -89,6 +92,23 @@ class DemStats(ctypes.Structure):
 
 
 _lib = None
+
+
+class SqDemParams(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double), ("skin", ctypes.c_double),
+        ("boxlo", ctypes.c_double * 3), ("boxhi", ctypes.c_double * 3), ("periodic", ctypes.c_int * 3),
+        ("tangential_damping", ctypes.c_int), ("limit_force", ctypes.c_int),
+        ("has_gravity", ctypes.c_int), ("gravity", ctypes.c_double * 3), ("gravity_first", ctypes.c_int),
+        ("nplanes", ctypes.c_int), ("plane_axis", ctypes.c_int * 6), ("plane_pos", ctypes.c_double * 6),
+        ("plane_type", ctypes.c_int * 6),
+    ]
+
+
+class SqDemStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in ("steps", "rebuilds", "pair_touch", "wall_touch", "newton_last",
+                                              "newton", "checks", "fresh", "fresh_iters", "fallbacks")] + [
+        ("run_ms", ctypes.c_double)]
 
 
 def load_library(path=LIB_PATH):
@@ -152,6 +172,20 @@ def load_library(path=LIB_PATH):
     lib.dem_sq_net_train.argtypes = [h, ctypes.c_int64, ctypes.c_int64, ctypes.c_float, ctypes.c_uint64,
                                      fp, dp]
     lib.dem_get_step_shape.argtypes = [h, ip, ip, ctypes.c_int, ip, ip, ip, dp]
+    lib.dem_sqdem_create.argtypes = [ctypes.c_int, P(SqDemParams), P(ctypes.c_void_p)]
+    lib.dem_sqdem_destroy.argtypes = [h]
+    lib.dem_sqdem_last_error.argtypes = [h]
+    lib.dem_sqdem_last_error.restype = ctypes.c_char_p
+    lib.dem_sqdem_set_materials.argtypes = [h, ctypes.c_int, dp, dp, dp, dp]
+    lib.dem_sqdem_set_particles.argtypes = [h, ctypes.c_int64, dp, dp, dp, dp, dp, dp, dp, ip]
+    lib.dem_sqdem_set_warm_start.argtypes = [h, h]
+    lib.dem_sqdem_run.argtypes = [h, ctypes.c_int]
+    lib.dem_sqdem_get_state.argtypes = [h, dp, dp, dp, dp, dp, dp, dp]
+    lib.dem_sqdem_get_props.argtypes = [h, dp, dp, dp, dp]
+    lib.dem_sqdem_get_pairs.argtypes = [h, ctypes.c_int64, ip, ip, dp]
+    lib.dem_sqdem_get_pairs.restype = ctypes.c_int64
+    lib.dem_sqdem_get_wall_history.argtypes = [h, dp]
+    lib.dem_sqdem_get_stats.argtypes = [h, P(SqDemStats)]
     _lib = lib
     return lib
 
@@ -491,6 +525,109 @@ class SqSolver:
         if self.h:
             self.lib.dem_sq_destroy(self.h)
             self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SqDemSystem:
+    """Superquadric DEM step on the GPU (dem_sqdem_* of the C ABI).
+    params / materials / particles / walls as workloads.superquadric_settling
+    returns them."""
+
+    def __init__(self, params, materials, P, walls=(), device=0):
+        self.lib = load_library()
+        p = SqDemParams()
+        p.dt = params["dt"]
+        p.skin = params["skin"]
+        for d in range(3):
+            p.boxlo[d] = params["boxlo"][d]
+            p.boxhi[d] = params["boxhi"][d]
+            p.periodic[d] = int(params["periodic"][d])
+        p.tangential_damping = int(params.get("tangential_damping", 1))
+        p.limit_force = int(params.get("limit_force", 0))
+        g = params.get("grav_acc")
+        p.has_gravity = int(g is not None)
+        if g is not None:
+            for d in range(3):
+                p.gravity[d] = g[d]
+        p.gravity_first = int(params.get("gravity_first", 1))
+        p.nplanes = len(walls)
+        for k, (axis, pos, wtype) in enumerate(walls):
+            p.plane_axis[k] = int(axis)
+            p.plane_pos[k] = float(pos)
+            p.plane_type[k] = int(wtype)
+        self.h = ctypes.c_void_p()
+        rc = self.lib.dem_sqdem_create(device, ctypes.byref(p), ctypes.byref(self.h))
+        if rc != 0:
+            raise DemError(f"dem_sqdem_create failed ({rc}): {self.lib.dem_sqdem_last_error(None).decode()}")
+        self.nplanes = len(walls)
+        nt, Yp, nup, ep, mup, _ = _material_args(materials)
+        self._check(self.lib.dem_sqdem_set_materials(self.h, int(nt), Yp, nup, ep, mup))
+        arr = lambda k, shape: np.ascontiguousarray(P[k], dtype=np.float64).reshape(shape)
+        self.n = len(P["x"])
+        n = self.n
+        ty = np.ascontiguousarray(P["type"], dtype=np.int32)
+        self._check(self.lib.dem_sqdem_set_particles(
+            self.h, n, _dp(arr("x", (n, 3))), _dp(arr("v", (n, 3))), _dp(arr("omega", (n, 3))),
+            _dp(arr("quat", (n, 4))), _dp(arr("shape", (n, 3))), _dp(arr("blockiness", (n, 2))),
+            _dp(arr("density", (n,))), _ip(ty)))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise DemError(f"libdem_amd error {rc}: {self.lib.dem_sqdem_last_error(self.h).decode()}")
+
+    def set_warm_start(self, solver):
+        self._check(self.lib.dem_sqdem_set_warm_start(self.h, solver.h if solver is not None else None))
+
+    def run(self, nsteps):
+        self._check(self.lib.dem_sqdem_run(self.h, int(nsteps)))
+
+    def state(self):
+        n = self.n
+        out = {k: np.zeros((n, 3)) for k in ("x", "v", "omega", "angmom", "f", "torque")}
+        out["quat"] = np.zeros((n, 4))
+        self._check(self.lib.dem_sqdem_get_state(self.h, _dp(out["x"]), _dp(out["v"]), _dp(out["omega"]),
+                                                 _dp(out["angmom"]), _dp(out["quat"]), _dp(out["f"]),
+                                                 _dp(out["torque"])))
+        return out
+
+    def props(self):
+        n = self.n
+        m, rv, rb, I = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros((n, 3))
+        self._check(self.lib.dem_sqdem_get_props(self.h, _dp(m), _dp(rv), _dp(rb), _dp(I)))
+        return {"mass": m, "rvol": rv, "rbound": rb, "inertia": I}
+
+    def pairs(self):
+        need = -self.lib.dem_sqdem_get_pairs(self.h, 0, None, None, None)
+        if need < 0:
+            raise DemError(self.lib.dem_sqdem_last_error(self.h).decode())
+        cap = max(int(need), 1)
+        ij = np.zeros((cap, 2), dtype=np.int32)
+        fl = np.zeros(cap, dtype=np.int32)
+        hist = np.zeros((cap, 10))
+        m = self.lib.dem_sqdem_get_pairs(self.h, cap, _ip(ij), _ip(fl), _dp(hist))
+        if m < 0:
+            raise DemError(self.lib.dem_sqdem_last_error(self.h).decode())
+        return ij[:m], fl[:m], hist[:m]
+
+    def wall_hist(self):
+        h = np.zeros((self.n, max(self.nplanes, 1), 3))
+        self._check(self.lib.dem_sqdem_get_wall_history(self.h, _dp(h)))
+        return h[:, :self.nplanes]
+
+    def stats(self):
+        st = SqDemStats()
+        self._check(self.lib.dem_sqdem_get_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in SqDemStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dem_sqdem_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:
