@@ -276,6 +276,104 @@ def main_superquadric(args):
     print(json.dumps(out), flush=True)
 
 
+def _sq_ke(g):
+    st, pr = g.state(), g.props()
+    return float(0.5 * (pr["mass"][:, None] * st["v"] ** 2).sum() + 0.5 * (st["omega"] * st["angmom"]).sum())
+
+
+def main_sqsettle(args):
+    """Superquadric settling: the superquadric DEM step (FP64) on n particles
+    (half-axes U(1,3) mm, blockiness U(2,8), random orientations) started
+    agitated (velocities N(0, v0), spins N(0, 30 rad/s)) above a floor in a
+    0.2 x 0.2 m periodic column; timed once with the reference's homotopy for
+    new contacts and once with the network warm start."""
+    import numpy as np
+    import pydem
+    import superquadrics as sq
+    import workloads as wl
+    t_all = time.perf_counter()
+    n = args.sq_n
+
+    def make(nn, base):
+        params, mats, P, walls = wl.superquadric_settling(11, nn, base=base, gap=0.0003)
+        rng = np.random.default_rng(12)
+        P["v"] = rng.normal(0, args.sq_v0, (nn, 3))
+        P["omega"] = rng.normal(0, 30.0, (nn, 3))
+        return params, mats, P, walls
+
+    params, mats, P, walls = make(n, (0.2, 0.2))
+    # warm-start network, trained on plain-solved random overlapping pairs of the same shape ranges
+    solver = pydem.SqSolver()
+    A, B = sq.overlapping_pairs(100, args.sq_ntrain)
+    solver.set_pairs(A, B)
+    del A, B
+    solver.solve(pydem.DEM_SQ_FRESH)
+    solver.net_init(seed=1)
+    tsteps = max(1, int(2.0 * args.sq_ntrain / args.sq_batch))
+    curve, train_ms = solver.net_train(steps=tsteps, batch=args.sq_batch, lr=2e-3, seed=7)
+    print(f"# network: {tsteps} Adam steps in {train_ms:.0f} ms", file=sys.stderr, flush=True)
+    res = {}
+    for mode in ("plain", "warm"):
+        g = pydem.SqDemSystem(params, mats, P, walls)
+        if mode == "warm":
+            g.set_warm_start(solver)
+        g.run(args.warmup)
+        s0 = g.stats()
+        g.run(args.steps)
+        s1 = g.stats()
+        ms = s1["run_ms"]
+        fresh = s1["fresh"] - s0["fresh"]
+        res[mode] = {"ms_per_step": ms / args.steps,
+                     "particle_steps_per_s": n * args.steps / (ms * 1e-3),
+                     "contact_detections_per_s": (s1["checks"] - s0["checks"]) / (ms * 1e-3),
+                     "touching_pairs_last_step": s1["pair_touch"], "wall_contacts_last_step": s1["wall_touch"],
+                     "fresh_solves": fresh,
+                     "newton_iterations_per_fresh_solve": (s1["fresh_iters"] - s0["fresh_iters"]) / max(fresh, 1),
+                     "newton_iterations_per_detection": (s1["newton"] - s0["newton"])
+                     / max(s1["checks"] - s0["checks"], 1),
+                     "fallbacks": s1["fallbacks"] - s0["fallbacks"], "rebuilds": s1["rebuilds"] - s0["rebuilds"],
+                     "kinetic_energy": _sq_ke(g)}
+        print(f"# {mode}: {res[mode]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+        g.close()
+    solver.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        nc = args.sq_cpu_n
+        cp, cm, cP, cw = make(nc, (0.04, 0.04))
+        o = pyoracle.SqsSystem(cp, pyoracle.materials(cm), cP, cw)
+        o.setup()
+        o.run(args.warmup)
+        t0 = time.perf_counter()
+        o.run(args.sq_cpu_steps)
+        dt = time.perf_counter() - t0
+        o.close()
+        cpu = {"value": nc * args.sq_cpu_steps / dt, "unit": "particle-timesteps/s", "cores": 1, "kind": "port",
+               "sample": f"oracle/sqdem_oracle.c, {nc} particles of the same generator in a 0.04 x 0.04 m column, "
+                         f"{args.sq_cpu_steps} steps after {args.warmup}, 1 thread ({dt:.1f} s)"}
+    warm = res["warm"]
+    out = {
+        "metric": "superquadric particle-timesteps/s (FP64, network warm start)",
+        "value": warm["particle_steps_per_s"], "unit": "particle-timesteps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": warm["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (+ bf16/f32 network for new contacts)",
+        "data": "synthetic (seeded superquadrics on a layer lattice, agitated start)",
+        "config": {"workload": f"superquadric settling, {n} particles, half-axes U(1,3) mm, blockiness U(2,8), "
+                               f"random orientations, 0.2 x 0.2 m periodic column, floor, v0 {args.sq_v0} m/s, "
+                               "Hertz/history, dt 1e-5", "n": n},
+        "modes": res,
+        "iteration_ratio_plain_over_warm": res["plain"]["newton_iterations_per_fresh_solve"]
+        / max(warm["newton_iterations_per_fresh_solve"], 1e-30),
+        "network": {"train_pairs": args.sq_ntrain, "steps": tsteps, "train_ms": train_ms},
+        "wall_s": time.perf_counter() - t_all,
+    }
+    if cpu:
+        out["cpu_baseline"] = cpu
+    print(json.dumps(out), flush=True)
+
+
 def main_copy(args):
     """Calibration for hardware counters: a plain device-to-device copy of a
     known size (torch's copy kernel), repeated; its FETCH_SIZE / WRITE_SIZE per
@@ -310,7 +408,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default=None)
-    ap.add_argument("--workload", choices=("settling", "packed", "drum", "sweep", "superquadric", "copy"),
+    ap.add_argument("--workload", choices=("settling", "packed", "drum", "sweep", "superquadric", "sqsettle",
+                                           "copy"),
                     default="settling")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--sq-train", type=int, default=10_000_000)
@@ -318,6 +417,11 @@ def main():
     ap.add_argument("--sq-batch", type=int, default=8192)
     ap.add_argument("--sq-epochs", type=float, default=2.0)
     ap.add_argument("--sq-cpu", type=int, default=20_000)
+    ap.add_argument("--sq-n", type=int, default=200_000)
+    ap.add_argument("--sq-v0", type=float, default=0.3)
+    ap.add_argument("--sq-ntrain", type=int, default=2_000_000)
+    ap.add_argument("--sq-cpu-n", type=int, default=2000)
+    ap.add_argument("--sq-cpu-steps", type=int, default=3000)
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--sample", type=int, default=8,
                     help="sweep: instances rerun alone in FP64 for the error report")
@@ -331,6 +435,8 @@ def main():
     if args.workload == "superquadric":
         main_superquadric(args)
         return
+    if args.workload == "sqsettle":
+        return main_sqsettle(args)
     if args.workload == "copy":
         main_copy(args)
         return
