@@ -407,8 +407,12 @@ int dem_sqdem_set_particles(dem_sqdem_handle *h, int64_t n, const double *x, con
  * reference's homotopy only. No reference counterpart. */
 int dem_sqdem_set_warm_start(dem_sqdem_handle *h, dem_sq_handle *net);
 
+/* Verlet::setup (verlet.cpp:134-205): neighbor build with history carried
+ * over, forces without history update; what `run N` (pre yes) does first. */
+int dem_sqdem_setup(dem_sqdem_handle *h);
+
 /* Runs nsteps timesteps; the first call also runs the setup (forces without
- * history update), as `run` does. */
+ * history update), as `run` does; later calls continue (`run N pre no`). */
 int dem_sqdem_run(dem_sqdem_handle *h, int nsteps);
 
 /* Current state (each pointer may be NULL): x, v, omega, angmom, f, torque

@@ -24,6 +24,13 @@
 //                  meshes <id1> ... <idk>
 //   fix <id> <grp> move/mesh mesh <meshid> rotate origin ox oy oz axis ax ay az period T
 //   fix <id> <grp> nve/sphere
+//   atom_style superquadric  (read_data Atoms: id type density shapex shapey shapez
+//                             blockiness1 blockiness2 x y z), with
+//   pair_style gran model hertz tangential history surface superquadric [...],
+//   fix <id> <grp> wall/gran model hertz tangential history surface superquadric
+//                  primitive type <t> xplane|yplane|zplane <pos>,
+//   fix <id> <grp> nve/superquadric [integration_scheme 1],
+//   set atom|type|group <id> quat <a> <b> <c> <theta> | quat/random <seed>
 //   thermo <N> | thermo_style custom step atoms ke ... | thermo_modify ...
 //   variable <name> equal|index <value> | print | echo | log
 //   run <N> [pre no|yes] [post no|yes]
@@ -70,6 +77,38 @@ int inum(const std::string &s) {
   return (int)v;
 }
 
+// RanPark (random_park.cpp:54-80, 125-157): Park-Miller minimal standard
+// generator, seeded per atom by the Jenkins one-at-a-time hash of the caller's
+// seed and the atom's coordinates, as set ... quat/random does it
+struct RanPark {
+  int seed = 1;
+  double uniform() {
+    const int k = seed / 127773;
+    seed = 16807 * (seed - k * 127773) - 2836 * k;
+    if (seed < 0) seed += 2147483647;
+    return (1.0 / 2147483647) * seed;
+  }
+  void reset(int ibase, const double *coord) {
+    unsigned int hash = 0;
+    auto mix = [&hash](const void *p, int n) {
+      const signed char *c = (const signed char *)p;  // char is signed on x86-64
+      for (int i = 0; i < n; i++) {
+        hash += (unsigned int)(int)c[i];
+        hash += (hash << 10);
+        hash ^= (hash >> 6);
+      }
+    };
+    mix(&ibase, (int)sizeof(int));
+    mix(coord, 3 * (int)sizeof(double));
+    hash += (hash << 3);
+    hash ^= (hash >> 11);
+    hash += (hash << 15);
+    seed = (int)(hash & 0x7ffffff);
+    if (!seed) seed = 1;
+    for (int i = 0; i < 5; i++) uniform();
+  }
+};
+
 struct Interp {
   FILE *out = stdout;
   // box / domain
@@ -87,6 +126,11 @@ struct Interp {
   int one = 0;
   // materials
   std::vector<double> Y, nu, e, mu, mur;
+  // superquadrics (atom_style superquadric)
+  bool sq = false;
+  std::vector<double> sq_shape, sq_blk, sq_dens, sq_quat, sq_mass;
+  bool pair_sq_surface = false, nve_sq = false, omega_warned = false;
+  dem_sqdem_handle *hs = nullptr;
   // pair
   bool pair_set = false;
   int rolling = 0, tdamp = 1, limitf = 0, torsion = 0;
@@ -123,6 +167,7 @@ struct Interp {
 
   ~Interp() {
     if (h) dem_destroy(h);
+    if (hs) dem_sqdem_destroy(hs);
   }
 
   std::string substitute(const std::string &line) {
@@ -214,7 +259,36 @@ struct Interp {
         if (hash != std::string::npos) line = line.substr(0, hash);
         auto t = split(line);
         if (t.empty()) continue;
-        if (section == "Atoms") {
+        if (section == "Atoms" && sq) {
+          // AtomVecSuperquadric::data_atom (atom_vec_superquadric.cpp:1150-1212):
+          // id type density shapex shapey shapez blockiness1 blockiness2 x y z
+          if (t.size() < 11) die("Incorrect format of Atoms section");
+          const int id = inum(t[0]);
+          const int ty = inum(t[1]);
+          if (id <= 0) die("Invalid atom ID in Atoms section of data file");
+          if (ty <= 0 || ty > ntypes) die("Invalid atom type in Atoms section of data file");
+          const double dens = num(t[2]);
+          if (dens <= 0.0) die("Invalid density in Atoms section of data file");
+          for (int d = 0; d < 3; d++)
+            if (num(t[3 + d]) <= 0.0) die("Invalid shape in Atoms section of data file");
+          for (int d = 0; d < 2; d++)
+            if (num(t[6 + d]) < 2.0) die("Invalid blockiness in Atoms section of data file");
+          index[id] = tag.size();
+          tag.push_back(id);
+          type.push_back(ty);
+          r.push_back(0.0);
+          m.push_back(0.0);
+          sq_dens.push_back(dens);
+          for (int d = 0; d < 3; d++) sq_shape.push_back(num(t[3 + d]));
+          for (int d = 0; d < 2; d++) sq_blk.push_back(num(t[6 + d]));
+          const double qid[4] = {1.0, 0.0, 0.0, 0.0};  // quatIdentity4D
+          for (int d = 0; d < 4; d++) sq_quat.push_back(qid[d]);
+          for (int d = 0; d < 3; d++) x.push_back(num(t[8 + d]));
+          for (int d = 0; d < 3; d++) {
+            v.push_back(0.0);
+            w.push_back(0.0);
+          }
+        } else if (section == "Atoms") {
           // atom_vec_sphere.cpp:1055-1100: id type diameter density x y z
           if (t.size() < 7) die("Incorrect format of Atoms section");
           const int id = inum(t[0]);
@@ -245,6 +319,18 @@ struct Interp {
           for (int d = 0; d < 3; d++) {
             v[3 * it->second + d] = num(t[1 + d]);
             w[3 * it->second + d] = num(t[4 + d]);
+          }
+          if (sq) {
+            // AtomVecSuperquadric::data_vel sets omega but not the angular
+            // momentum fix nve/superquadric integrates (atom_vec_superquadric.cpp:
+            // 1231-1239): the particles start from zero angular momentum
+            if (!omega_warned && (w[3 * it->second] != 0.0 || w[3 * it->second + 1] != 0.0 ||
+                                  w[3 * it->second + 2] != 0.0)) {
+              fprintf(out, "WARNING: Velocities omega of superquadrics does not set their angular "
+                           "momentum; they start without rotation\n");
+              omega_warned = true;
+            }
+            for (int d = 0; d < 3; d++) w[3 * it->second + d] = 0.0;
           }
         }
         got++;
@@ -362,7 +448,14 @@ struct Interp {
         if (t[k + 1] != "off") die("cohesion models are not available on this path");
         k += 2;
       } else if (t[k] == "surface" && k + 1 < t.size()) {
-        if (t[k + 1] != "default") die("Only the default surface model is available");
+        if (t[k + 1] == "superquadric") {
+          if (!sq) die("surface superquadric requires atom_style superquadric");
+          if (pair) pair_sq_surface = true;
+        } else if (t[k + 1] != "default") {
+          die("Only the default and superquadric surface models are available");
+        } else if (sq) {
+          die("atom_style superquadric needs 'surface superquadric'");
+        }
         k += 2;
       } else if ((t[k] == "tangential_damping" || t[k] == "limitForce" || t[k] == "torsionTorque") &&
                  k + 1 < t.size()) {
@@ -391,7 +484,120 @@ struct Interp {
     }
   }
 
+  // Set::command for quat and quat/random (set.cpp:345-365, 1041-1065,
+  // 1195-1225); atom, type and group all selectors
+  void set_command(const std::vector<std::string> &t) {
+    if (t.size() < 4) die("Illegal set command");
+    if (!sq) die("set: only quaternions of superquadrics are supported on this path");
+    std::vector<size_t> sel;
+    const std::string &style = t[1], &id = t[2];
+    for (size_t i = 0; i < tag.size(); i++) {
+      bool on = false;
+      if (style == "atom") on = id == "*" || tag[i] == inum(id);
+      else if (style == "type") on = id == "*" || type[i] == inum(id);
+      else if (style == "group") {
+        if (id != "all") die("set group: only group all is available on this path");
+        on = true;
+      } else die("set: style " + style + " is not supported on this path");
+      if (on) sel.push_back(i);
+    }
+    const std::string &kw = t[3];
+    if (kw == "quat") {
+      if (t.size() < 8) die("Illegal set command");
+      const double a = num(t[4]), b = num(t[5]), cc = num(t[6]), wv = num(t[7]);
+      const double theta2 = 1.57079632679489661923 * wv / 180.0;  // MY_PI2
+      const double st = std::sin(theta2);
+      for (size_t i : sel) {
+        double *q = &sq_quat[4 * i];
+        q[0] = std::cos(theta2);
+        q[1] = a * st;
+        q[2] = b * st;
+        q[3] = cc * st;
+        const double nrm = 1.0 / std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        for (int d = 0; d < 4; d++) q[d] *= nrm;
+      }
+    } else if (kw == "quat/random") {
+      if (t.size() < 5) die("Illegal set command");
+      const int seed = inum(t[4]);
+      if (seed <= 0) die("Invalid random number seed in set command");
+      RanPark rng;
+      for (size_t i : sel) {
+        rng.reset(seed, &x[3 * i]);
+        const double sv = rng.uniform();
+        const double t1 = std::sqrt(1.0 - sv), t2 = std::sqrt(sv);
+        const double theta1 = 2.0 * kMyPi * rng.uniform();
+        const double th2 = 2.0 * kMyPi * rng.uniform();
+        double *q = &sq_quat[4 * i];
+        q[0] = std::cos(th2) * t2;
+        q[1] = std::sin(theta1) * t1;
+        q[2] = std::cos(theta1) * t1;
+        q[3] = std::sin(th2) * t2;
+      }
+    } else {
+      die("set keyword " + kw + " is not supported on this path");
+    }
+    fprintf(out, "Setting atom values ...\n  %zu settings made for %s\n", sel.size(), kw.c_str());
+    dirty = true;
+  }
+
+  void build_handle_sq() {
+    need_box("run");
+    if (!pair_set || !pair_sq_surface) die("run: pair_style gran ... surface superquadric not defined");
+    if (!nve_sq) die("run: no fix nve/superquadric, particles would not move");
+    if (dt <= 0) die("run: timestep not set");
+    if (skin < 0) die("run: neighbor skin not set");
+    if (every != 1 || no_check || cdf != 1.0)
+      die("superquadrics: neigh_modify every/check/contact_distance_factor must keep their defaults");
+    if (rolling || torsion) die("superquadrics: rolling friction is not available on this path");
+    for (auto &wl : walls)
+      if (wl.model.rolling_cdt || wl.model.torsion_torque)
+        die("superquadrics: rolling friction is not available on this path");
+    if (!mesh_order.empty()) die("superquadrics: mesh walls are not available on this path");
+    if ((int)Y.size() != ntypes || (int)nu.size() != ntypes)
+      die("youngsModulus/poissonsRatio need one value per atom type");
+    if ((int)e.size() != ntypes * ntypes || (int)mu.size() != ntypes * ntypes)
+      die("coefficientRestitution/coefficientFriction need ntypes^2 values");
+    if (walls.size() > 6) die("at most 6 primitive walls");
+    if (hs) {
+      dem_sqdem_destroy(hs);
+      hs = nullptr;
+    }
+    dem_sqdem_params p{};
+    p.dt = dt;
+    p.skin = skin;
+    for (int d = 0; d < 3; d++) {
+      p.boxlo[d] = lo[d];
+      p.boxhi[d] = hi[d];
+      p.periodic[d] = periodic[d];
+    }
+    p.tangential_damping = tdamp;
+    p.limit_force = limitf;
+    p.has_gravity = gravity;
+    for (int d = 0; d < 3; d++) p.gravity[d] = gacc[d];
+    p.gravity_first = gravity_first;
+    p.nplanes = (int)walls.size();
+    for (size_t k = 0; k < walls.size(); k++) {
+      p.plane_axis[k] = walls[k].axis;
+      p.plane_pos[k] = walls[k].pos;
+      p.plane_type[k] = walls[k].type;
+    }
+    if (dem_sqdem_create(0, &p, &hs)) die(std::string("GPU library: ") + dem_sqdem_last_error(nullptr));
+    if (dem_sqdem_set_materials(hs, ntypes, Y.data(), nu.data(), e.data(), mu.data())) die(dem_sqdem_last_error(hs));
+    if (tag.empty()) die("run: no atoms");
+    if (dem_sqdem_set_particles(hs, (int64_t)tag.size(), x.data(), v.data(), w.data(), sq_quat.data(),
+                                sq_shape.data(), sq_blk.data(), sq_dens.data(), type.data()))
+      die(dem_sqdem_last_error(hs));
+    sq_mass.assign(tag.size(), 0.0);
+    if (dem_sqdem_get_props(hs, sq_mass.data(), nullptr, nullptr, nullptr)) die(dem_sqdem_last_error(hs));
+    m = sq_mass;
+    dirty = false;
+  }
+
   void build_handle() {
+    if (sq) {
+      build_handle_sq();
+      return;
+    }
     need_box("run");
     if (!pair_set) die("run: pair_style gran not defined");
     if (!nve) die("run: no fix nve/sphere, particles would not move");
@@ -454,6 +660,11 @@ struct Interp {
   }
 
   void pull_state() {
+    if (sq) {
+      if (dem_sqdem_get_state(hs, x.data(), v.data(), w.data(), nullptr, sq_quat.data(), nullptr, nullptr))
+        die(dem_sqdem_last_error(hs));
+      return;
+    }
     if (dem_get_state(h, x.data(), v.data(), w.data(), nullptr, nullptr)) die(dem_last_error(h));
   }
 
@@ -496,7 +707,7 @@ struct Interp {
         if (t[k] != "post") die("run option " + t[k] + " is not supported on this path");
       } else die("Illegal run command");
     }
-    if (dirty || !h) {
+    if (dirty || (sq ? !hs : !h)) {
       build_handle();
       pre = true;
     }
@@ -505,17 +716,17 @@ struct Interp {
     const long long chunk = thermo_every > 0 ? thermo_every : n;
     bool first = true;
     if (pre) {
-      if (dem_setup(h)) die(dem_last_error(h));
+      if (sq ? dem_sqdem_setup(hs) : dem_setup(h)) die(sq ? dem_sqdem_last_error(hs) : dem_last_error(h));
     }
     thermo_header();
     thermo_line();
     while (done < n) {
       long long k = chunk - (step % (chunk > 0 ? chunk : 1));
       if (k <= 0 || k > n - done) k = n - done;
-      int rc = dem_run_continue(h, k);
+      int rc = sq ? dem_sqdem_run(hs, (int)k) : dem_run_continue(h, k);
       (void)first;
       first = false;
-      if (rc) die(dem_last_error(h));
+      if (rc) die(sq ? dem_sqdem_last_error(hs) : dem_last_error(h));
       done += k;
       step += k;
       if ((thermo_every > 0 && step % thermo_every == 0) || done == n) thermo_line();
@@ -524,9 +735,17 @@ struct Interp {
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     fprintf(out, "Loop time of %g on 1 procs (1 GPU) for %lld steps with %zu atoms\n", secs, n,
             tag.size());
-    dem_stats st{};
-    dem_get_stats(h, &st);
-    fprintf(out, "Neighbor list builds = %lld\n", (long long)st.nrebuilds);
+    if (sq) {
+      dem_sqdem_stats st{};
+      dem_sqdem_get_stats(hs, &st);
+      fprintf(out, "Neighbor list builds = %lld\n", (long long)st.rebuilds);
+      fprintf(out, "Newton iterations = %lld (%lld contact detections)\n", (long long)st.newton,
+              (long long)st.checks);
+    } else {
+      dem_stats st{};
+      dem_get_stats(h, &st);
+      fprintf(out, "Neighbor list builds = %lld\n", (long long)st.nrebuilds);
+    }
     if (secs > 0)
       fprintf(out, "Performance: %.4g particle-timesteps/s\n", (double)n * tag.size() / secs);
   }
@@ -542,8 +761,12 @@ struct Interp {
     if (c == "units") {
       if (t.size() < 2 || t[1] != "si") die("Only 'units si' is supported on this path");
     } else if (c == "atom_style") {
-      if (t.size() < 2 || (t[1] != "granular" && t[1] != "sphere"))
-        die("Only atom_style granular/sphere is supported on this path");
+      if (t.size() >= 2 && t[1] == "superquadric") {
+        if (!tag.empty()) die("atom_style command after simulation box is defined");
+        sq = true;
+      } else if (t.size() < 2 || (t[1] != "granular" && t[1] != "sphere")) {
+        die("Only atom_style granular/sphere/superquadric is supported on this path");
+      }
     } else if (c == "atom_modify" || c == "processors" || c == "echo" || c == "log" ||
                c == "thermo_modify" || c == "dump" || c == "dump_modify" || c == "undump" ||
                c == "compute" || c == "uncompute" || c == "group" || c == "restart" ||
@@ -618,7 +841,18 @@ struct Interp {
       if (style == "property/global") {
         fix_property_global(t);
       } else if (style == "nve/sphere") {
+        if (sq) die("Fix nve/sphere requires atom style sphere");
         nve = true;
+      } else if (style == "nve/superquadric") {
+        // fix_nve_asphere_base.cpp:60-75: integration_scheme N (default 1)
+        if (!sq) die("Fix nve/superquadric requires atom style superquadric");
+        for (size_t k = 4; k < t.size(); k += 2) {
+          if (t[k] != "integration_scheme" || k + 1 >= t.size()) die("fix nve/superquadric: unknown keyword");
+          if (inum(t[k + 1]) != 1)
+            die("fix nve/superquadric: only integration_scheme 1 (the default) is available on this path");
+        }
+        nve_sq = true;
+        dirty = true;
       } else if (style == "gravity") {
         if (t.size() < 9 || t[5] != "vector") die("fix gravity: only '<g> vector x y z' is supported");
         const double g = num(t[4]);
@@ -650,6 +884,7 @@ struct Interp {
       } else if (style == "wall/gran" && t.size() > 4 &&
                  std::find(t.begin(), t.end(), std::string("mesh")) != t.end() &&
                  std::find(t.begin(), t.end(), std::string("primitive")) == t.end()) {
+        if (sq) die("fix wall/gran: mesh walls for superquadrics are not available on this path");
         size_t k;
         dem_contact_model wm{};
         parse_model(t, 4, k, false, &wm);
@@ -690,6 +925,8 @@ struct Interp {
       } else {
         die("fix style " + style + " is not available on this path");
       }
+    } else if (c == "set") {
+      set_command(t);
     } else if (c == "unfix") {
       die("unfix is not supported on this path");
     } else if (c == "thermo") {

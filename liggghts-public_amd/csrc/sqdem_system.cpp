@@ -660,6 +660,13 @@ int dem_sqdem_set_warm_start(dem_sqdem_handle *h, dem_sq_handle *net) {
   return DEM_OK;
 }
 
+int dem_sqdem_setup(dem_sqdem_handle *h) {
+  if (!h) return DEM_ERR_ARG;
+  if (h->n <= 0) return fail(h, DEM_ERR_STATE, "no particles");
+  SCHK(h, hipSetDevice(h->device));
+  return setup(h);
+}
+
 int dem_sqdem_run(dem_sqdem_handle *h, int nsteps) {
   if (!h) return DEM_ERR_ARG;
   if (h->n <= 0) return fail(h, DEM_ERR_STATE, "no particles");

@@ -30,7 +30,8 @@ EXPORTED = [
     "dem_sq_get_features", "dem_sq_net_init", "dem_sq_net_set_params", "dem_sq_net_get_params",
     "dem_sq_net_forward", "dem_sq_net_gradient", "dem_sq_net_train",
     "dem_sqdem_create", "dem_sqdem_destroy", "dem_sqdem_last_error", "dem_sqdem_set_materials",
-    "dem_sqdem_set_particles", "dem_sqdem_set_warm_start", "dem_sqdem_run", "dem_sqdem_get_state",
+    "dem_sqdem_set_particles", "dem_sqdem_set_warm_start", "dem_sqdem_setup", "dem_sqdem_run",
+    "dem_sqdem_get_state",
     "dem_sqdem_get_props", "dem_sqdem_get_pairs", "dem_sqdem_get_wall_history", "dem_sqdem_get_stats",
 ]
 
@@ -180,6 +181,7 @@ def load_library(path=LIB_PATH):
     lib.dem_sqdem_set_particles.argtypes = [h, ctypes.c_int64, dp, dp, dp, dp, dp, dp, dp, ip]
     lib.dem_sqdem_set_warm_start.argtypes = [h, h]
     lib.dem_sqdem_run.argtypes = [h, ctypes.c_int]
+    lib.dem_sqdem_setup.argtypes = [h]
     lib.dem_sqdem_get_state.argtypes = [h, dp, dp, dp, dp, dp, dp, dp]
     lib.dem_sqdem_get_props.argtypes = [h, dp, dp, dp, dp]
     lib.dem_sqdem_get_pairs.argtypes = [h, ctypes.c_int64, ip, ip, dp]
@@ -582,6 +584,9 @@ class SqDemSystem:
 
     def set_warm_start(self, solver):
         self._check(self.lib.dem_sqdem_set_warm_start(self.h, solver.h if solver is not None else None))
+
+    def setup(self):
+        self._check(self.lib.dem_sqdem_setup(self.h))
 
     def run(self, nsteps):
         self._check(self.lib.dem_sqdem_run(self.h, int(nsteps)))

@@ -317,3 +317,27 @@ def superquadric_settling(seed=0, n=200_000, base=(0.2, 0.2), axes=(0.001, 0.003
     P = {"x": x, "v": np.zeros((n, 3)), "omega": np.zeros((n, 3)), "quat": q, "shape": shape,
          "blockiness": blk, "density": np.full(n, density), "type": np.ones(n, dtype=np.int32)}
     return params, mats, P, [(2, 0.0, 1)]
+
+
+def write_superquadric_data(path, params, particles, ntypes=1):
+    """Writes particles in the reference's data-file format for
+    atom_style superquadric (AtomVecSuperquadric::data_atom / data_vel,
+    atom_vec_superquadric.cpp:1150-1239): Atoms ``id type density shapex
+    shapey shapez blockiness1 blockiness2 x y z``, Velocities ``id vx vy vz
+    wx wy wz``. Orientations are not part of the format (set ... quat)."""
+    P = particles
+    n = len(P["x"])
+    lo, hi = params["boxlo"], params["boxhi"]
+    with open(path, "w") as f:
+        f.write("LIGGGHTS data file (superquadric)\n\n")
+        f.write(f"{n} atoms\n{ntypes} atom types\n\n")
+        for d, ax in enumerate("xyz"):
+            f.write(f"{float(lo[d])!r} {float(hi[d])!r} {ax}lo {ax}hi\n")
+        f.write("\nAtoms\n\n")
+        r = lambda a: " ".join(repr(float(t)) for t in a)
+        for i in range(n):
+            f.write(f"{i + 1} {int(P['type'][i])} {float(P['density'][i])!r} {r(P['shape'][i])} "
+                    f"{r(P['blockiness'][i])} {r(P['x'][i])}\n")
+        f.write("\nVelocities\n\n")
+        for i in range(n):
+            f.write(f"{i + 1} {r(P['v'][i])} {r(P['omega'][i])}\n")

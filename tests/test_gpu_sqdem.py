@@ -186,3 +186,36 @@ def test_warm_start_forms_the_same_contacts_with_fewer_iterations():
     assert iw < ip
     assert abs(sp["pair_touch"] - sw["pair_touch"]) <= max(2, 0.05 * sp["pair_touch"])
     solver.close()
+
+
+def test_script_front_end_matches_api(tmp_path):
+    """A superquadric input script (atom_style superquadric, read_data,
+    set group all quat/random, surface superquadric, fix nve/superquadric)
+    run by dem_run_script gives the kinetic energy of the same system driven
+    through dem_sqdem_*, with orientations from the restated quat/random."""
+    say("[gpu] superquadric script front end")
+    from test_script_superquadric import HEADER, _gas as _sgas, quat_random
+    params, mats, P, walls = _sgas(300)
+    data = tmp_path / "sq.data"
+    wl.write_superquadric_data(str(data), params, P)
+    script = tmp_path / "in.sq"
+    script.write_text(HEADER + f"read_data {data}\nset group all quat/random 4711\n"
+                      "fix integr all nve/superquadric\nthermo_style custom step atoms ke\nthermo 100\nrun 300\n")
+    log = tmp_path / "log.sq"
+    rc = pydem.run_script(str(script), str(log))
+    text = log.read_text()
+    say(text[-500:])
+    assert rc == 0
+    rows = [l.split() for l in text.splitlines() if l.strip() and l.split()[0].isdigit()]
+    assert int(rows[-1][0]) == 300 and int(rows[-1][1]) == 300
+    P2 = dict(P)
+    P2["quat"] = quat_random(4711, P["x"])
+    g = pydem.SqDemSystem(params, mats, P2, walls)
+    kes = []
+    for _ in range(3):
+        g.run(100)
+        st = g.state()
+        kes.append(0.5 * (g.props()["mass"] @ (st["v"] ** 2).sum(1)))
+    for row, ke in zip(rows[1:], kes):
+        assert float(row[2]) == pytest.approx(ke, rel=1e-6)
+    g.close()
