@@ -206,7 +206,8 @@ def test_script_front_end_matches_api(tmp_path):
     text = log.read_text()
     say(text[-500:])
     assert rc == 0
-    rows = [l.split() for l in text.splitlines() if l.strip() and l.split()[0].isdigit()]
+    rows = [l.split() for l in text.splitlines()
+            if len(l.split()) == 3 and l.split()[0].isdigit() and l.split()[1].isdigit()]
     assert int(rows[-1][0]) == 300 and int(rows[-1][1]) == 300
     P2 = dict(P)
     P2["quat"] = quat_random(4711, P["x"])
