@@ -35,14 +35,36 @@ void orc_mix_materials(int nt, const double *Y, const double *nu, const double *
       out->betaeff[i][j] = le / sqrt(pow(le, 2.) + pow(MY_PI, 2.));
       out->mu[i][j] = mu[(i - 1) * nt + (j - 1)];
       out->mur[i][j] = mur ? mur[(i - 1) * nt + (j - 1)] : 0.0;
+      out->loge[i][j] = le; /* createCoeffRestLog, global_properties.cpp:502-520 */
     }
 }
 
+/* MODEL_PARAMS::createKn / createKt / createGamman / createGammat (and the
+ * _abs variants) / createCohesionEnergyDensity / createCharacteristicVelocity
+ * (global_properties.cpp:227-240, 291-294, 735-812) with coarse-graining 1:
+ * the per-type-pair values as given */
+void orc_set_model_coeffs(orc_materials *m, int nt, const double *kn, const double *kt,
+                          const double *gamman, const double *gammat, const double *ced,
+                          double charvel) {
+  const double *src[5] = {kn, kt, gamman, gammat, ced};
+  double(*dst[5])[ORC_MAXTYPE + 1] = {m->kn, m->kt, m->gamman, m->gammat, m->ced};
+  for (int q = 0; q < 5; q++)
+    for (int i = 1; i <= nt; i++)
+      for (int j = 1; j <= nt; j++) dst[q][i][j] = src[q] ? src[q][(i - 1) * nt + (j - 1)] : 0.0;
+  m->charvel = charvel;
+}
+
 /* ------------------------------------------------------------------------
- * shared model chain for pair and wall contacts:
+ * shared model chain for pair and wall contacts, in the order of
+ * ContactModel::surfacesIntersect (contact_models.h:228-238):
  *   SurfaceModel<DEFAULT>::surfacesIntersect   surface_model_default.h:140-205
  *   NormalModel<HERTZ>::surfacesIntersect      normal_model_hertz.h:202-405
+ *     or <HOOKE>                               normal_model_hooke.h:227-436
+ *     or <HERTZ_STIFFNESS>                     normal_model_hertz_stiffness.h:106-202
+ *     or <HOOKE_STIFFNESS>                     normal_model_hooke_stiffness.h:194-365
+ *   CohesionModel<SJKR>::surfacesIntersect     cohesion_model_sjkr.h:86-124
  *   TangentialModel<HISTORY>::surfacesIntersect tangential_model_history.h:130-356
+ *     or <NO_HISTORY>                          tangential_model_no_history.h:112-242
  *   RollingModel<CDT>::surfacesIntersect       rolling_model_cdt.h:89-167
  * ---------------------------------------------------------------------- */
 typedef struct {
@@ -86,20 +108,53 @@ static void model_chain(const orc_model *mdl, const orc_materials *mat, const co
   const double vtr2 = vt2 - (dx * wr3 - dz * wr1);
   const double vtr3 = vt3 - (dy * wr1 - dx * wr2);
 
-  /* normal model: Hertz */
+  /* normal model */
   const int it = c->itype, jt = c->jtype;
   const double reff = c->is_wall ? c->radi : (c->radi * c->radj / (c->radi + c->radj));
   const double meff = c->meff;
-  const double sqrtval = sqrt(reff * deltan);
-  const double Sn = 2. * mat->Yeff[it][jt] * sqrtval;
-  const double St = 8. * mat->Geff[it][jt] * sqrtval;
-  double kn = 4. / 3. * mat->Yeff[it][jt] * sqrtval;
-  double kt = St;
-  const double sqrtFiveOverSix = 0.91287092917527685576161630466800355658790782499663875;
-  const double gamman = -2. * sqrtFiveOverSix * mat->betaeff[it][jt] * sqrt(Sn * meff);
-  const double gammat =
-      mdl->tangential_damping ? -2. * sqrtFiveOverSix * mat->betaeff[it][jt] * sqrt(St * meff)
-                              : 0.0;
+  double kn, kt, gamman, gammat;
+  if (mdl->normal_model == ORC_NORMAL_HOOKE) {
+    /* normal_model_hooke.h:252-286 (viscous off) */
+    const double Y = mat->Yeff[it][jt], cv = mat->charvel;
+    const double sqrtval = sqrt(reff);
+    kn = 16. / 15. * sqrtval * Y * pow(15. * meff * cv * cv / (16. * sqrtval * Y), 0.2);
+    kt = kn;
+    if (mdl->kt_to_kn) kt *= 0.285714286;
+    const double lsq = mat->loge[it][jt] * mat->loge[it][jt];
+    gamman = sqrt(4. * meff * kn * lsq / (lsq + MY_PI * MY_PI));
+    gammat = mdl->tangential_damping ? gamman : 0.0;
+  } else if (mdl->normal_model == ORC_NORMAL_HERTZ_STIFFNESS) {
+    /* normal_model_hertz_stiffness.h:112-123 */
+    const double polyhertz = sqrt(reff * deltan);
+    kn = polyhertz * mat->kn[it][jt];
+    kt = polyhertz * mat->kt[it][jt];
+    gamman = polyhertz * meff * mat->gamman[it][jt];
+    gammat = mdl->tangential_damping ? polyhertz * meff * mat->gammat[it][jt] : 0.0;
+  } else if (mdl->normal_model == ORC_NORMAL_HOOKE_STIFFNESS) {
+    /* normal_model_hooke_stiffness.h:203-226 */
+    kn = mat->kn[it][jt];
+    kt = mat->kt[it][jt];
+    if (mdl->absolute_damping) {
+      gamman = mat->gamman[it][jt];
+      gammat = mat->gammat[it][jt];
+    } else {
+      gamman = meff * mat->gamman[it][jt];
+      gammat = meff * mat->gammat[it][jt];
+    }
+    if (!mdl->tangential_damping) gammat = 0.0;
+  } else {
+    /* Hertz, normal_model_hertz.h:202-260 */
+    const double sqrtval = sqrt(reff * deltan);
+    const double Sn = 2. * mat->Yeff[it][jt] * sqrtval;
+    const double St = 8. * mat->Geff[it][jt] * sqrtval;
+    kn = 4. / 3. * mat->Yeff[it][jt] * sqrtval;
+    kt = St;
+    const double sqrtFiveOverSix = 0.91287092917527685576161630466800355658790782499663875;
+    gamman = -2. * sqrtFiveOverSix * mat->betaeff[it][jt] * sqrt(Sn * meff);
+    gammat = mdl->tangential_damping
+                 ? -2. * sqrtFiveOverSix * mat->betaeff[it][jt] * sqrt(St * meff)
+                 : 0.0;
+  }
   kn /= 1.0; /* force->nktv2p = 1 in SI (update.cpp:223-229) */
   kt /= 1.0;
   const double Fn_damping = -gamman * vn;
@@ -112,6 +167,51 @@ static void model_chain(const orc_model *mdl, const orc_materials *mat, const co
   dTi[0] = dTi[1] = dTi[2] = 0.0;
   dTj[0] = dTj[1] = dTj[2] = 0.0;
 
+  /* cohesion model: SJKR, the contact-circle area times the energy density */
+  if (mdl->cohesion_model == 1) {
+    const double r = c->r, ri = c->radi, rj = c->radj;
+    double Acont;
+    if (c->is_wall)
+      Acont = (ri * ri - r * r) * MY_PI;
+    else
+      Acont = -MY_PI / 4 * ((r - ri - rj) * (r + ri - rj) * (r - ri + rj) * (r + ri + rj)) / (r * r);
+    const double Fn_coh = -mat->ced[it][jt] * Acont;
+    if (mdl->tangential_reduce) Fn += Fn_coh; /* sidata.Fn seen by the tangential model */
+    dFi[0] += Fn_coh * enx;
+    dFi[1] += Fn_coh * eny;
+    dFi[2] += Fn_coh * enz;
+  }
+
+  const double xmu = mat->mu[it][jt];
+  if (mdl->tangential_model == 1) {
+    /* tangential model: no history, viscous friction capped by Coulomb */
+    const double vrel = sqrt(vtr1 * vtr1 + vtr2 * vtr2 + vtr3 * vtr3);
+    const double Ft_friction = xmu * fabs(Fn);
+    double gamma = 0.0;
+    if (Ft_friction < gammat * vrel)
+      gamma = Ft_friction / vrel;
+    else
+      gamma = gammat;
+    const double Ft1 = -gamma * vtr1;
+    const double Ft2 = -gamma * vtr2;
+    const double Ft3 = -gamma * vtr3;
+    const double tor1 = (eny * Ft3 - enz * Ft2);
+    const double tor2 = (enz * Ft1 - enx * Ft3);
+    const double tor3 = (enx * Ft2 - eny * Ft1);
+    dFi[0] += Ft1;
+    dFi[1] += Ft2;
+    dFi[2] += Ft3;
+    dTi[0] += -cri * tor1;
+    dTi[1] += -cri * tor2;
+    dTi[2] += -cri * tor3;
+    if (!c->is_wall) {
+      dTj[0] += -crj * tor1;
+      dTj[1] += -crj * tor2;
+      dTj[2] += -crj * tor3;
+    }
+    goto rolling;
+  }
+
   /* tangential model: history */
   if (shearupdate) {
     shear[0] += vtr1 * dt;
@@ -123,7 +223,6 @@ static void model_chain(const orc_model *mdl, const orc_materials *mat, const co
     shear[2] -= rsht * enz;
   }
   const double shrmag = sqrt(shear[0] * shear[0] + shear[1] * shear[1] + shear[2] * shear[2]);
-  const double xmu = mat->mu[it][jt];
   double Ft1 = -(kt * shear[0]);
   double Ft2 = -(kt * shear[1]);
   double Ft3 = -(kt * shear[2]);
@@ -162,6 +261,7 @@ static void model_chain(const orc_model *mdl, const orc_materials *mat, const co
     dTj[2] += -crj * tor3;
   }
 
+rolling:
   /* rolling model: constant directional torque */
   if (mdl->rolling_cdt) {
     const double rmu = mat->mur[it][jt];

@@ -32,14 +32,38 @@ typedef struct {
   double betaeff[ORC_MAXTYPE + 1][ORC_MAXTYPE + 1];
   double mu[ORC_MAXTYPE + 1][ORC_MAXTYPE + 1];
   double mur[ORC_MAXTYPE + 1][ORC_MAXTYPE + 1];
+  /* coefficients of the sibling models: fix property/global kn, kt, gamman
+   * (or gamman_abs), gammat (or gammat_abs), cohesionEnergyDensity
+   * (peratomtypepair), characteristicVelocity (scalar) and log(e)
+   * (global_properties.cpp:227-240, 291-294, 502-520, 735-812) */
+  double kn[ORC_MAXTYPE + 1][ORC_MAXTYPE + 1];
+  double kt[ORC_MAXTYPE + 1][ORC_MAXTYPE + 1];
+  double gamman[ORC_MAXTYPE + 1][ORC_MAXTYPE + 1];
+  double gammat[ORC_MAXTYPE + 1][ORC_MAXTYPE + 1];
+  double ced[ORC_MAXTYPE + 1][ORC_MAXTYPE + 1];
+  double loge[ORC_MAXTYPE + 1][ORC_MAXTYPE + 1];
+  double charvel;
 } orc_materials;
 
-/* contact-model switches: pair_style gran model hertz tangential history [...] */
+/* normal models (model <name>) */
+#define ORC_NORMAL_HERTZ 0
+#define ORC_NORMAL_HOOKE 1
+#define ORC_NORMAL_HERTZ_STIFFNESS 2
+#define ORC_NORMAL_HOOKE_STIFFNESS 3
+
+/* contact-model switches: pair_style gran model <normal> tangential <t>
+ * [cohesion sjkr] [rolling_friction cdt] [...] */
 typedef struct {
   int rolling_cdt;        /* rolling_friction cdt */
   int tangential_damping; /* default on  (normal_model_hertz.h:90) */
   int limit_force;        /* default off (normal_model_hertz.h:91) */
   int torsion_torque;     /* default off (rolling_model_cdt.h:73) */
+  int normal_model;       /* ORC_NORMAL_* */
+  int tangential_model;   /* 0 history, 1 no_history */
+  int cohesion_model;     /* 0 off, 1 sjkr */
+  int kt_to_kn;           /* hooke: ktToKnUser on (normal_model_hooke.h:101) */
+  int absolute_damping;   /* hooke/stiffness: absolute_damping on (normal_model_hooke_stiffness.h:81) */
+  int tangential_reduce;  /* sjkr: tangential_reduce on (cohesion_model_sjkr.h:72) */
 } orc_model;
 
 typedef struct {
@@ -67,6 +91,12 @@ typedef struct orc_sys orc_sys;
 void orc_mix_materials(int ntypes, const double *Y, const double *nu,
                        const double *e /* nt*nt */, const double *mu /* nt*nt */,
                        const double *mur /* nt*nt or NULL */, orc_materials *out);
+
+/* Fills the sibling models' per-type-pair tables (nt*nt row-major each, any
+ * may be NULL) and the characteristic velocity of model hooke. */
+void orc_set_model_coeffs(orc_materials *m, int nt, const double *kn, const double *kt,
+                          const double *gamman, const double *gammat, const double *ced,
+                          double charvel);
 
 /* --- one sphere-sphere contact, i side and j side ----------------------- */
 /* del = x_i - x_j. shear (3) is read and, if shearupdate, written.

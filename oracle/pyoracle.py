@@ -20,12 +20,22 @@ _Mat = (_D * (MAXT + 1)) * (MAXT + 1)
 
 class OrcMaterials(ctypes.Structure):
     _fields_ = [("ntypes", ctypes.c_int), ("Yeff", _Mat), ("Geff", _Mat), ("betaeff", _Mat),
-                ("mu", _Mat), ("mur", _Mat)]
+                ("mu", _Mat), ("mur", _Mat), ("kn", _Mat), ("kt", _Mat), ("gamman", _Mat),
+                ("gammat", _Mat), ("ced", _Mat), ("loge", _Mat), ("charvel", _D)]
 
 
 class OrcModel(ctypes.Structure):
     _fields_ = [("rolling_cdt", ctypes.c_int), ("tangential_damping", ctypes.c_int),
-                ("limit_force", ctypes.c_int), ("torsion_torque", ctypes.c_int)]
+                ("limit_force", ctypes.c_int), ("torsion_torque", ctypes.c_int),
+                ("normal_model", ctypes.c_int), ("tangential_model", ctypes.c_int),
+                ("cohesion_model", ctypes.c_int), ("kt_to_kn", ctypes.c_int),
+                ("absolute_damping", ctypes.c_int), ("tangential_reduce", ctypes.c_int)]
+
+
+# model <normal> of pair_style gran / fix wall/gran
+NORMAL_MODELS = {"hertz": 0, "hooke": 1, "hertz/stiffness": 2, "hooke/stiffness": 3}
+TANGENTIAL_MODELS = {"history": 0, "no_history": 1}
+COHESION_MODELS = {None: 0, "off": 0, "sjkr": 1}
 
 
 class OrcParams(ctypes.Structure):
@@ -55,6 +65,7 @@ def load():
     P = ctypes.POINTER
     dp, ip = P(_D), P(ctypes.c_int)
     lib.orc_mix_materials.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp, P(OrcMaterials)]
+    lib.orc_set_model_coeffs.argtypes = [P(OrcMaterials), ctypes.c_int, dp, dp, dp, dp, dp, _D]
     lib.orc_contact_pair.argtypes = [P(OrcModel), P(OrcMaterials), ctypes.c_int, ctypes.c_int,
                                      dp, _D, _D, dp, dp, dp, dp, _D, _D, _D, ctypes.c_int,
                                      dp, dp, dp, dp]
@@ -102,8 +113,13 @@ def _ip(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
 
 
+MODEL_COEFFS = ("kn", "kt", "gamman", "gammat", "cohesion_energy_density")
+
+
 def materials(mat):
-    """Mix a material dict (youngs, poisson, restitution, friction[, rolling_friction])."""
+    """Mix a material dict (youngs, poisson, restitution, friction[, rolling_friction]
+    [, kn, kt, gamman, gammat, cohesion_energy_density (ntypes x ntypes),
+    characteristic_velocity])."""
     lib = load()
     nt = mat["ntypes"]
     Y = np.ascontiguousarray(mat["youngs"], dtype=np.float64)
@@ -115,6 +131,11 @@ def materials(mat):
     out = OrcMaterials()
     lib.orc_mix_materials(nt, _dp(Y), _dp(nu), _dp(e), _dp(mu),
                           _dp(mur) if mur is not None else None, ctypes.byref(out))
+    if any(k in mat for k in MODEL_COEFFS + ("characteristic_velocity",)):
+        arrs = [None if mat.get(k) is None else np.ascontiguousarray(mat[k], dtype=np.float64).reshape(-1)
+                for k in MODEL_COEFFS]
+        lib.orc_set_model_coeffs(ctypes.byref(out), nt, *[_dp(a) if a is not None else None for a in arrs],
+                                 float(mat.get("characteristic_velocity", 0.0)))
     return out
 
 
@@ -124,6 +145,12 @@ def model(params):
     m.tangential_damping = int(params.get("tangential_damping", 1))
     m.limit_force = int(params.get("limit_force", 0))
     m.torsion_torque = int(params.get("torsion_torque", 0))
+    m.normal_model = NORMAL_MODELS[params.get("normal", "hertz")]
+    m.tangential_model = TANGENTIAL_MODELS[params.get("tangential", "history")]
+    m.cohesion_model = COHESION_MODELS[params.get("cohesion")]
+    m.kt_to_kn = int(params.get("kt_to_kn", 0))
+    m.absolute_damping = int(params.get("absolute_damping", 0))
+    m.tangential_reduce = int(params.get("tangential_reduce", 0))
     return m
 
 
