@@ -72,12 +72,30 @@ typedef struct dem_params {
                          `neigh_every` steps without the skin/2 displacement test */
 } dem_params;
 
-/* Switches of one contact model (pair_style gran / fix wall/gran keywords). */
+enum dem_normal_model {
+  DEM_NORMAL_HERTZ = 0,           /* model hertz (normal_model_hertz.h) */
+  DEM_NORMAL_HOOKE = 1,           /* model hooke (normal_model_hooke.h), viscous off */
+  DEM_NORMAL_HERTZ_STIFFNESS = 2, /* model hertz/stiffness (normal_model_hertz_stiffness.h) */
+  DEM_NORMAL_HOOKE_STIFFNESS = 3  /* model hooke/stiffness (normal_model_hooke_stiffness.h) */
+};
+
+/* Switches of one contact model (pair_style gran / fix wall/gran keywords):
+ * model <normal> tangential history|no_history [cohesion sjkr]
+ * [rolling_friction cdt] and the on/off settings the models register
+ * (Settings::registerOnOff in each model's registerSettings). Zero-filled
+ * fields beyond the first four select hertz, history, no cohesion. */
 typedef struct dem_contact_model {
   int rolling_cdt;        /* rolling_friction cdt */
   int tangential_damping; /* tangential_damping on (default) */
   int limit_force;        /* limitForce on */
   int torsion_torque;     /* torsionTorque on */
+  int normal_model;       /* enum dem_normal_model */
+  int tangential_model;   /* 0 history (tangential_model_history.h), 1 no_history
+                             (tangential_model_no_history.h) */
+  int cohesion_model;     /* 0 off, 1 sjkr (cohesion_model_sjkr.h) */
+  int kt_to_kn;           /* hooke: ktToKnUser on (normal_model_hooke.h:100) */
+  int absolute_damping;   /* hooke/stiffness: absolute_damping on (normal_model_hooke_stiffness.h:81) */
+  int tangential_reduce;  /* sjkr: tangential_reduce on (cohesion_model_sjkr.h:72) */
 } dem_contact_model;
 
 typedef struct dem_stats {
@@ -106,7 +124,8 @@ const char *dem_last_error(const dem_handle *h);
 /* Material properties of fix property/global youngsModulus / poissonsRatio
  * (peratomtype, ntypes values) and coefficientRestitution / coefficientFriction /
  * coefficientRollingFriction (peratomtypepair, ntypes*ntypes row-major, mur may
- * be NULL). Mixed on the host into Y*, G*, beta per type pair exactly as
+ * be NULL; Y, nu and e may be NULL together when no contact model reads them,
+ * as with the stiffness models). Mixed on the host into Y*, G*, beta per type pair exactly as
  * MODEL_PARAMS::createYeff/createGeff/createBetaEff (global_properties.cpp:411-558),
  * with the reference's range checks (5e6 <= Y <= 1e9, 0 <= nu <= 0.5,
  * 0.05 < e <= 1). Replaces PropertyRegistry::connect of Yeff/Geff/betaeff/
@@ -114,6 +133,28 @@ const char *dem_last_error(const dem_handle *h);
 int dem_set_materials(dem_handle *h, int ntypes, const double *youngs, const double *poisson,
                       const double *restitution, const double *friction,
                       const double *rolling_friction);
+
+/* The pair style's contact model (pair_style gran model ...: the model
+ * selection of PairGranProxy::settings, pair_gran_proxy.cpp:63-79, and the
+ * ContactModel<> chain of contact_models.h:137-260), replacing the four
+ * switches of dem_params.
+ * Walls and meshes added afterwards without their own model take this one.
+ * Call before dem_set_particles. With a model other than hertz + history
+ * the step kernel runs its general contact law in tile shape 0. */
+int dem_set_pair_model(dem_handle *h, const dem_contact_model *model);
+
+/* Coefficients of the sibling models, after dem_set_materials (which clears
+ * them): fix property/global kn / kt / gamman (or gamman_abs) / gammat (or
+ * gammat_abs) peratomtypepair (hertz/stiffness, hooke/stiffness:
+ * MODEL_PARAMS::createKn/Kt/Gamman/Gammat/GammanAbs/GammatAbs,
+ * global_properties.cpp:735-812), cohesionEnergyDensity peratomtypepair (sjkr,
+ * :291-294) and characteristicVelocity scalar (hooke, :227-240; 0 if unused).
+ * Arrays are ntypes*ntypes row-major; kn, kt, gamman, gammat come together or
+ * not at all; cohesion_energy_density may be NULL. With dem_set_instances,
+ * call before dem_set_instance_materials (every instance gets these values). */
+int dem_set_model_coefficients(dem_handle *h, int ntypes, const double *kn, const double *kt,
+                               const double *gamman, const double *gammat,
+                               const double *cohesion_energy_density, double characteristic_velocity);
 
 /* Uploads n particles (read_data Atoms + Velocities, atom_vec_sphere.cpp:1055-1140).
  * x, v, omega are n*3 row-major; rmass is the mass of each sphere; type is 1-based;

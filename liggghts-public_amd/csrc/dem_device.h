@@ -53,14 +53,16 @@ struct DevParams {
   double skin, cdf;
   double lo[3], L[3], halfL[3];
   int per[3];
-  int rolling, tdamp, limitf, torsion;
+  int pair_model;          // contact-model bits of pair_style gran (kModel*)
+  int generic;             // some pair, wall or mesh model is not Hertz / history
+  double char_vel;         // characteristicVelocity of model hooke
   int has_grav, grav_first;
   double gacc[3];
   int nwalls;
   int wall_axis[kMaxWalls];
   double wall_pos[kMaxWalls];
   int wall_type[kMaxWalls];
-  int wall_model[kMaxWalls];  // bit 0 rolling cdt, 1 tangential damping, 2 limitForce, 3 torsion
+  int wall_model[kMaxWalls];  // contact-model bits of each fix wall/gran
   double cutneighmax;
   int ntp1;                // ntypes + 1 (row length of the material tables)
   // triangle meshes
@@ -76,10 +78,25 @@ struct DevParams {
   int mat_stride;          // ntp1 * ntp1: material block of one instance
 };
 
-// Mixed material tables, [ntp1*ntp1] each, FP64 and FP32 copies.
+// Contact-model bits (one int per pair style / wall fix): bit 0 rolling cdt,
+// 1 tangential damping, 2 limitForce, 3 torsionTorque, 4-5 the normal model,
+// 6 tangential no_history, 7 cohesion sjkr, 8 ktToKnUser (hooke),
+// 9 absolute_damping (hooke/stiffness), 10 tangential_reduce (sjkr).
+constexpr int kModelNormalShift = 4;
+constexpr int kNormalHertz = 0, kNormalHooke = 1, kNormalHertzStiffness = 2,
+              kNormalHookeStiffness = 3;
+constexpr int kModelNoHistory = 1 << 6, kModelSjkr = 1 << 7, kModelKtToKn = 1 << 8,
+              kModelAbsDamping = 1 << 9, kModelTangReduce = 1 << 10;
+// bits that need the generic contact law (anything beyond Hertz / history)
+constexpr int kModelGenericMask = ~0xf;
+
+// Mixed material tables, [ntp1*ntp1] each, FP64 and FP32 copies. kn, kt,
+// gamman, gammat, cohesion energy density and log(e) feed the hooke,
+// stiffness and sjkr models.
+constexpr int kNumMatTables = 11;
 struct MatTables {
-  double *Yeff, *Geff, *beta, *mu, *mur;
-  float *fYeff, *fGeff, *fbeta, *fmu, *fmur;
+  double *Yeff, *Geff, *beta, *mu, *mur, *kn, *kt, *gn, *gt, *ced, *loge;
+  float *fYeff, *fGeff, *fbeta, *fmu, *fmur, *fkn, *fkt, *fgn, *fgt, *fced, *floge;
 };
 
 // Control block shared between the step kernels and the host driver.

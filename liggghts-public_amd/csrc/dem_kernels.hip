@@ -18,6 +18,8 @@
 
 namespace dem {
 
+constexpr double kPi = 3.14159265358979323846;  // M_PI
+
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
@@ -67,10 +69,17 @@ template <> struct OmgT<double> { typedef double4 type; };
 template <> struct OmgT<float> { typedef float4 type; };
 
 // ---------------------------------------------------------------------------
-// contact law: SurfaceModel<DEFAULT> (surface_model_default.h:140-205) ->
-// NormalModel<HERTZ> (normal_model_hertz.h:202-405) -> TangentialModel<HISTORY>
-// (tangential_model_history.h:130-356) -> RollingModel<CDT>
-// (rolling_model_cdt.h:89-167). deltan is formed by the caller in FP64.
+// contact law, in the order of ContactModel::surfacesIntersect
+// (contact_models.h:228-238): SurfaceModel<DEFAULT> (surface_model_default.h:
+// 140-205) -> NormalModel<HERTZ> (normal_model_hertz.h:202-405) ->
+// TangentialModel<HISTORY> (tangential_model_history.h:130-356) ->
+// RollingModel<CDT> (rolling_model_cdt.h:89-167). deltan is formed by the
+// caller in FP64. With GEN the normal model may instead be HOOKE
+// (normal_model_hooke.h:227-436), HERTZ_STIFFNESS
+// (normal_model_hertz_stiffness.h:106-202) or HOOKE_STIFFNESS
+// (normal_model_hooke_stiffness.h:194-365), CohesionModel<SJKR>
+// (cohesion_model_sjkr.h:86-124) may follow it, and the tangential model may
+// be NO_HISTORY (tangential_model_no_history.h:112-242).
 // ---------------------------------------------------------------------------
 template <typename T>
 struct CIn {
@@ -79,18 +88,50 @@ struct CIn {
   T vr[3];
   T wi[3], wj[3];
   T Y, G, beta, mu, mur;
+  // GEN only: sibling-model coefficients, characteristic velocity, and the
+  // sjkr contact-circle area (formed in FP64 by the caller)
+  T kn, kt, gn, gt, ced, loge, cv, acont;
 };
 
-// contact-model switches: rolling cdt, tangential damping, limitForce, torsionTorque
+// contact-model switches (the kModel* bits)
 struct Model {
   int rolling, tdamp, limitf, torsion;
+  int normal, nohist, sjkr, kt2kn, absdamp, treduce;
 };
 
 __device__ __forceinline__ Model model_from_bits(int b) {
-  return Model{b & 1, (b >> 1) & 1, (b >> 2) & 1, (b >> 3) & 1};
+  return Model{b & 1, (b >> 1) & 1, (b >> 2) & 1, (b >> 3) & 1,
+               (b >> kModelNormalShift) & 3, (b & kModelNoHistory) != 0, (b & kModelSjkr) != 0,
+               (b & kModelKtToKn) != 0, (b & kModelAbsDamping) != 0, (b & kModelTangReduce) != 0};
 }
 
-template <typename T, bool WALL>
+__device__ __forceinline__ double tpow(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ float tpow(float x, float y) { return powf(x, y); }
+
+// sibling-model coefficients of type pair mt (GEN contacts only)
+template <typename T>
+__device__ __forceinline__ void load_model_coeffs(const MatTables &mt_, const DevParams &p,
+                                                  const Model &m, const int mt, CIn<T> &c) {
+  c.cv = (T)p.char_vel;
+  c.kn = c.kt = c.gn = c.gt = c.ced = c.loge = T(0);
+  if (m.normal == kNormalHooke) c.loge = sizeof(T) == 8 ? (T)mt_.loge[mt] : (T)mt_.floge[mt];
+  if (m.normal >= kNormalHertzStiffness) {
+    if (sizeof(T) == 8) {
+      c.kn = (T)mt_.kn[mt];
+      c.kt = (T)mt_.kt[mt];
+      c.gn = (T)mt_.gn[mt];
+      c.gt = (T)mt_.gt[mt];
+    } else {
+      c.kn = (T)mt_.fkn[mt];
+      c.kt = (T)mt_.fkt[mt];
+      c.gn = (T)mt_.fgn[mt];
+      c.gt = (T)mt_.fgt[mt];
+    }
+  }
+  if (m.sjkr) c.ced = sizeof(T) == 8 ? (T)mt_.ced[mt] : (T)mt_.fced[mt];
+}
+
+template <typename T, bool WALL, bool GEN = false>
 __device__ __forceinline__ void contact_chain(const Model &p, const CIn<T> &c, T dt,
                                               int shearupdate, T sh[3], T dF[3], T dTi[3],
                                               T dTj[3]) {
@@ -120,16 +161,47 @@ __device__ __forceinline__ void contact_chain(const Model &p, const CIn<T> &c, T
   const T vtr2 = vt2 - (dx * wr3 - dz * wr1);
   const T vtr3 = vt3 - (dy * wr1 - dx * wr2);
 
-  // Hertz
   const T reff = WALL ? c.radi : (c.radi * c.radj / (c.radi + c.radj));
-  const T sqrtval = sqrt(reff * deltan);
-  const T Sn = T(2.) * c.Y * sqrtval;
-  const T St = T(8.) * c.G * sqrtval;
-  const T kn = T(4.) / T(3.) * c.Y * sqrtval;
-  const T kt = St;
-  const T s56 = T(0.91287092917527685576161630466800355658790782499663875);
-  const T gamman = T(-2.) * s56 * c.beta * sqrt(Sn * c.meff);
-  const T gammat = p.tdamp ? T(-2.) * s56 * c.beta * sqrt(St * c.meff) : T(0);
+  T kn, kt, gamman, gammat;
+  if (GEN && p.normal == kNormalHooke) {
+    // viscous off
+    const T sqrtval = sqrt(reff);
+    kn = T(16.) / T(15.) * sqrtval * c.Y *
+         tpow(T(15.) * c.meff * c.cv * c.cv / (T(16.) * sqrtval * c.Y), T(0.2));
+    kt = kn;
+    if (p.kt2kn) kt *= T(0.285714286);
+    const T lsq = c.loge * c.loge;
+    const T pi = T(3.14159265358979323846);
+    gamman = sqrt(T(4.) * c.meff * kn * lsq / (lsq + pi * pi));
+    gammat = p.tdamp ? gamman : T(0);
+  } else if (GEN && p.normal == kNormalHertzStiffness) {
+    const T polyhertz = sqrt(reff * deltan);
+    kn = polyhertz * c.kn;
+    kt = polyhertz * c.kt;
+    gamman = polyhertz * c.meff * c.gn;
+    gammat = p.tdamp ? polyhertz * c.meff * c.gt : T(0);
+  } else if (GEN && p.normal == kNormalHookeStiffness) {
+    kn = c.kn;
+    kt = c.kt;
+    if (p.absdamp) {
+      gamman = c.gn;
+      gammat = c.gt;
+    } else {
+      gamman = c.meff * c.gn;
+      gammat = c.meff * c.gt;
+    }
+    if (!p.tdamp) gammat = T(0);
+  } else {
+    // Hertz
+    const T sqrtval = sqrt(reff * deltan);
+    const T Sn = T(2.) * c.Y * sqrtval;
+    const T St = T(8.) * c.G * sqrtval;
+    kn = T(4.) / T(3.) * c.Y * sqrtval;
+    kt = St;
+    const T s56 = T(0.91287092917527685576161630466800355658790782499663875);
+    gamman = T(-2.) * s56 * c.beta * sqrt(Sn * c.meff);
+    gammat = p.tdamp ? T(-2.) * s56 * c.beta * sqrt(St * c.meff) : T(0);
+  }
   const T Fn_damping = -gamman * vn;
   const T Fn_contact = kn * deltan;
   T Fn = Fn_damping + Fn_contact;
@@ -138,6 +210,29 @@ __device__ __forceinline__ void contact_chain(const Model &p, const CIn<T> &c, T
   dF[1] = Fn * eny;
   dF[2] = Fn * enz;
 
+  // cohesion: energy density times the contact-circle area, attractive
+  if (GEN && p.sjkr) {
+    const T Fn_coh = -c.ced * c.acont;
+    if (p.treduce) Fn += Fn_coh;  // the tangential model sees the reduced force
+    dF[0] += Fn_coh * enx;
+    dF[1] += Fn_coh * eny;
+    dF[2] += Fn_coh * enz;
+  }
+
+  T Ft1, Ft2, Ft3;
+  if (GEN && p.nohist) {
+    // viscous friction gammat |vtr| capped by Coulomb; nothing is stored
+    const T vrel = sqrt(vtr1 * vtr1 + vtr2 * vtr2 + vtr3 * vtr3);
+    const T Ft_friction = c.mu * fabs(Fn);
+    T gamma = T(0);
+    if (Ft_friction < gammat * vrel)
+      gamma = Ft_friction / vrel;
+    else
+      gamma = gammat;
+    Ft1 = -gamma * vtr1;
+    Ft2 = -gamma * vtr2;
+    Ft3 = -gamma * vtr3;
+  } else {
   // tangential history
   if (shearupdate) {
     sh[0] += vtr1 * dt;
@@ -149,9 +244,9 @@ __device__ __forceinline__ void contact_chain(const Model &p, const CIn<T> &c, T
     sh[2] -= rsht * enz;
   }
   const T shrmag = sqrt(sh[0] * sh[0] + sh[1] * sh[1] + sh[2] * sh[2]);
-  T Ft1 = -(kt * sh[0]);
-  T Ft2 = -(kt * sh[1]);
-  T Ft3 = -(kt * sh[2]);
+  Ft1 = -(kt * sh[0]);
+  Ft2 = -(kt * sh[1]);
+  Ft3 = -(kt * sh[2]);
   const T Ft_shear = kt * shrmag;
   const T Ft_friction = c.mu * fabs(Fn);
   if (Ft_shear > Ft_friction) {
@@ -172,6 +267,7 @@ __device__ __forceinline__ void contact_chain(const Model &p, const CIn<T> &c, T
     Ft1 -= (gammat * vtr1);
     Ft2 -= (gammat * vtr2);
     Ft3 -= (gammat * vtr3);
+  }
   }
   const T tor1 = eny * Ft3 - enz * Ft2;
   const T tor2 = enz * Ft1 - enx * Ft3;
@@ -241,7 +337,7 @@ __device__ __forceinline__ bool tri_coplanar(const MeshStep &M, int a, int b) {
   return false;
 }
 
-template <typename T, typename W>
+template <typename T, bool GEN, typename W>
 __device__ inline void mesh_contacts(const StepArgs &a, int i, const double4 &pi,
                                      const double4 &vi, const W &wi, int irow, T &fx, T &fy,
                                      T &fz, T &tx, T &ty, T &tz,
@@ -361,8 +457,13 @@ __device__ inline void mesh_contacts(const StepArgs &a, int i, const double4 &pi
           cc.mu = (T)a.mat.fmu[mt];
           cc.mur = (T)a.mat.fmur[mt];
         }
+        if (GEN) {
+          load_model_coeffs(a.mat, p, mdl, mt, cc);
+          // sphere-wall contact area (cohesion_model_sjkr.h:95-96)
+          cc.acont = (T)((pi.w * pi.w - r * r) * kPi);
+        }
         T dF[3], dTi[3], dTj[3];
-        contact_chain<T, true>(mdl, cc, (T)p.dt, a.shearupdate, sh, dF, dTi, dTj);
+        contact_chain<T, true, GEN>(mdl, cc, (T)p.dt, a.shearupdate, sh, dF, dTi, dTj);
         fx += dF[0];
         fy += dF[1];
         fz += dF[2];
@@ -459,7 +560,7 @@ __device__ __forceinline__ void mesh_move_block(const MeshMoveArgs &a, const int
 // One particle's step: contacts over its list (broad phase in groups of
 // GROUP entries), walls, meshes, gravity, integration and the displacement
 // test. Accumulates its touching pair and wall contacts.
-template <typename T, bool MESH, int GROUP>
+template <typename T, bool MESH, int GROUP, bool GEN>
 __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
                                               unsigned long long &my_contacts,
                                               unsigned long long &my_wall, int &my_inst,
@@ -478,7 +579,7 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
   int meta = meta0;
   const int cnt = meta & kMetaCountMask;
   const T dt = (T)p.dt;
-  const Model pm{p.rolling, p.tdamp, p.limitf, p.torsion};
+  const Model pm = model_from_bits(p.pair_model);
   T fx = T(0), fy = T(0), fz = T(0), tx = T(0), ty = T(0), tz = T(0);
   T *hist = (T *)a.list.hist;
   const size_t cap = (size_t)a.cap;
@@ -559,6 +660,12 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
       c.mu = (T)a.mat.fmu[mt];
       c.mur = (T)a.mat.fmur[mt];
     }
+    if (GEN) {
+      load_model_coeffs(a.mat, p, pm, mt, c);
+      // sphere-sphere contact-circle area (cohesion_model_sjkr.h:97-98)
+      const double ri = pi.w, rj = pj.w;
+      c.acont = (T)(-kPi / 4 * ((r - ri - rj) * (r + ri - rj) * (r - ri + rj) * (r + ri + rj)) / (r * r));
+    }
     const bool flagged = (e & kFlagBit) != 0;
     T sh[3];
     const size_t hb = (size_t)k * 3 * cap + i;
@@ -570,7 +677,7 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
       sh[0] = sh[1] = sh[2] = T(0);
     }
     T dF[3], dTi[3], dTj[3];
-    contact_chain<T, false>(pm, c, dt, a.shearupdate, sh, dF, dTi, dTj);
+    contact_chain<T, false, GEN>(pm, c, dt, a.shearupdate, sh, dF, dTi, dTj);
     if (a.shearupdate || !flagged) {
       hist[hb] = sh[0];
       hist[hb + cap] = sh[1];
@@ -659,6 +766,11 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
           c.mu = (T)a.mat.fmu[mt];
           c.mur = (T)a.mat.fmur[mt];
         }
+        const Model wm = model_from_bits(p.wall_model[w]);
+        if (GEN) {
+          load_model_coeffs(a.mat, p, wm, mt, c);
+          c.acont = (T)((pi.w * pi.w - r * r) * kPi);  // cohesion_model_sjkr.h:95-96
+        }
         T sh[3];
         if (meta & bit) {
           sh[0] = wh[wb];
@@ -668,8 +780,7 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
           sh[0] = sh[1] = sh[2] = T(0);
         }
         T dF[3], dTi[3], dTj[3];
-        contact_chain<T, true>(model_from_bits(p.wall_model[w]), c, dt, a.shearupdate, sh, dF,
-                               dTi, dTj);
+        contact_chain<T, true, GEN>(wm, c, dt, a.shearupdate, sh, dF, dTi, dTj);
         wh[wb] = sh[0];
         wh[wb + cap] = sh[1];
         wh[wb + 2 * cap] = sh[2];
@@ -686,7 +797,7 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
       }
     }
     if (MESH)
-      mesh_contacts<T>(a, i, pi, vi, wi, irow, fx, fy, fz, tx, ty, tz, my_wall);
+      mesh_contacts<T, GEN>(a, i, pi, vi, wi, irow, fx, fy, fz, tx, ty, tz, my_wall);
   }
 
   if (a.store_f) {
@@ -762,7 +873,7 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
 // meshes into the next step's geometry buffer. Every tile shape computes
 // the same arithmetic in the same order, so results do not depend on it.
 // ---------------------------------------------------------------------------
-template <typename T, bool MESH, int BLOCK, int GROUP>
+template <typename T, bool MESH, int BLOCK, int GROUP, bool GEN = false>
 __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a) {
   // in a captured graph the step index is the control block's base plus this
   // launch's offset (a.step)
@@ -785,7 +896,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a) {
     if (i < a.n) {
       unsigned long long c = 0, w = 0;
       int inst = 0;
-      step_particle<T, MESH, GROUP>(a, i, c, w, inst, displaced, ago_next);
+      step_particle<T, MESH, GROUP, GEN>(a, i, c, w, inst, displaced, ago_next);
       my_contacts += c;
       my_wall += w;
       if (a.count && a.inst_stats) {
@@ -1422,6 +1533,11 @@ hipError_t launch_rebuild(const RebuildArgs &a, hipStream_t s) {
 template <typename T, bool MESH>
 static void launch_step_variant(const StepArgs &a, int nb, hipStream_t s) {
   const int B = kStepVariantBlock[a.variant];
+  if (a.prm.generic) {
+    // the sibling contact models run in the default tile shape only
+    hipLaunchKernelGGL((k_step<T, MESH, 256, 4, true>), dim3(nb), dim3(B), 0, s, a);
+    return;
+  }
   switch (a.variant) {
     case 1:
       hipLaunchKernelGGL((k_step<T, MESH, 256, 2>), dim3(nb), dim3(B), 0, s, a);
@@ -1443,6 +1559,7 @@ static void launch_step_variant(const StepArgs &a, int nb, hipStream_t s) {
 
 hipError_t launch_step(const StepArgs &a, hipStream_t s) {
   if (a.variant < 0 || a.variant >= kNumStepVariants) return hipErrorInvalidValue;
+  if (a.prm.generic && a.variant != 0) return hipErrorInvalidValue;
   const bool mesh = a.prm.nmesh > 0;
   // pblocks (tiles walked by the particle blocks) is set by the caller;
   // trailing blocks move the meshes when asked to

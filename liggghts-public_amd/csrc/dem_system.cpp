@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -36,12 +37,20 @@ struct Wall {
   int axis;
   double pos;
   int type;
-  int model;  // bit 0 rolling cdt, 1 tangential damping, 2 limitForce, 3 torsion
+  int model;  // contact-model bits (dem::kModel*)
 };
 
 int model_bits(const dem_contact_model &m) {
   return (m.rolling_cdt ? 1 : 0) | (m.tangential_damping ? 2 : 0) | (m.limit_force ? 4 : 0) |
-         (m.torsion_torque ? 8 : 0);
+         (m.torsion_torque ? 8 : 0) | ((m.normal_model & 3) << dem::kModelNormalShift) |
+         (m.tangential_model ? dem::kModelNoHistory : 0) | (m.cohesion_model ? dem::kModelSjkr : 0) |
+         (m.kt_to_kn ? dem::kModelKtToKn : 0) | (m.absolute_damping ? dem::kModelAbsDamping : 0) |
+         (m.tangential_reduce ? dem::kModelTangReduce : 0);
+}
+
+const char *normal_model_name(int nm) {
+  static const char *names[4] = {"hertz", "hooke", "hertz/stiffness", "hooke/stiffness"};
+  return names[nm & 3];
 }
 
 std::string g_create_error;
@@ -58,8 +67,14 @@ struct dem_handle {
   // materials
   int ntypes = 0;
   bool mat_set = false;
-  std::vector<double> hYeff, hGeff, hbeta, hmu, hmur;  // ninst blocks of (ntypes+1)^2
-  bool mat_dirty = false;                               // host tables newer than the device copy
+  // ninst blocks of (ntypes+1)^2 each, in dem::MatTables order
+  std::vector<double> hYeff, hGeff, hbeta, hmu, hmur, hkn, hkt, hgn, hgt, hced, hloge;
+  bool mat_dirty = false;     // host tables newer than the device copy
+  bool have_elastic = false;  // youngsModulus / poissonsRatio / coefficientRestitution given
+  bool have_stiff = false;    // kn / kt / gamman / gammat given
+  bool have_ced = false;      // cohesionEnergyDensity given
+  double char_vel = 0.0;      // characteristicVelocity (0: not given)
+  dem_contact_model pair_model{};  // pair_style gran model switches
   // independent simulations batched in this handle
   int ninst = 1;
   std::vector<int64_t> inst_count;  // particles of each instance, in caller order
@@ -211,6 +226,20 @@ void free_particles(dem_handle *h) {
   h->can_continue = false;
 }
 
+// any pair, wall or mesh contact model beyond Hertz / tangential history
+bool uses_generic_models(const dem_handle *h) {
+  int bits = model_bits(h->pair_model);
+  for (const Wall &w : h->walls) bits |= w.model;
+  for (const auto &m : h->meshes) bits |= m.model;
+  return (bits & dem::kModelGenericMask) != 0;
+}
+
+// the host material tables in dem::MatTables order
+std::array<std::vector<double> *, dem::kNumMatTables> host_tables(dem_handle *h) {
+  return {&h->hYeff, &h->hGeff, &h->hbeta, &h->hmu, &h->hmur, &h->hkn,
+          &h->hkt,   &h->hgn,   &h->hgt,   &h->hced, &h->hloge};
+}
+
 dem::DevParams make_devparams(const dem_handle *h) {
   dem::DevParams d{};
   const dem_params &p = h->p;
@@ -230,10 +259,9 @@ dem::DevParams make_devparams(const dem_handle *h) {
     d.nc[k] = h->nc[k];
     d.cinv[k] = h->nc[k] > 0 ? h->nc[k] / d.L[k] : 0.0;
   }
-  d.rolling = p.rolling_cdt;
-  d.tdamp = p.tangential_damping;
-  d.limitf = p.limit_force;
-  d.torsion = p.torsion_torque;
+  d.pair_model = model_bits(h->pair_model);
+  d.char_vel = h->char_vel;
+  d.generic = uses_generic_models(h) ? 1 : 0;
   d.has_grav = p.has_gravity;
   d.grav_first = p.gravity_first;
   for (int k = 0; k < 3; k++) d.gacc[k] = p.grav_acc[k];
@@ -384,9 +412,9 @@ dem::StepArgs step_args(dem_handle *h, int in_buf, int tri_buf) {
   a.fout = h->fout;
   a.ctrl = h->ctrl;
   a.inst_stats = h->d_inst_stats;
-  a.variant = h->variant;
+  a.variant = a.prm.generic ? 0 : h->variant;  // the sibling models run in shape 0 only
   {
-    const int B = dem::kStepVariantBlock[h->variant];
+    const int B = dem::kStepVariantBlock[a.variant];
     const int ntiles = (int)((std::max<int64_t>(h->n, 1) + B - 1) / B);
     a.pblocks = h->persist > 0 ? std::min(ntiles, h->persist * h->ncu) : ntiles;
   }
@@ -535,10 +563,15 @@ int force_pass(dem_handle *h, int shearupdate) {
   return DEM_OK;
 }
 
+// Y, nu and e may be absent together (models that do not read them: the
+// stiffness models take kn, kt, gamman, gammat instead)
 int check_material_args(dem_handle *h, int ntypes, const double *Y, const double *nu,
                         const double *e, const double *mu) {
-  if (ntypes < 1 || ntypes > dem::kMaxTypes || !Y || !nu || !e || !mu)
+  if (ntypes < 1 || ntypes > dem::kMaxTypes || !mu)
     return fail(h, DEM_ERR_ARG, "bad material arguments");
+  if (!Y && !nu && !e) return DEM_OK;
+  if (!Y || !nu || !e)
+    return fail(h, DEM_ERR_ARG, "youngsModulus, poissonsRatio and coefficientRestitution go together");
   // range checks of global_properties.cpp:360-372, :393-396, :316-320
   for (int i = 0; i < ntypes; i++) {
     if (Y[i] < 5e6)
@@ -560,12 +593,15 @@ void mix_block(dem_handle *h, int block, const double *Y, const double *nu, cons
   const size_t b0 = (size_t)block * t1 * t1;
   for (int i = 1; i <= ntypes; i++)
     for (int j = 1; j <= ntypes; j++) {
-      const double Yi = Y[i - 1], Yj = Y[j - 1], vi = nu[i - 1], vj = nu[j - 1];
       const size_t k = b0 + i * t1 + j;
-      h->hYeff[k] = 1. / ((1. - std::pow(vi, 2.)) / Yi + (1. - std::pow(vj, 2.)) / Yj);
-      h->hGeff[k] = 1. / (2. * (2. - vi) * (1. + vi) / Yi + 2. * (2. - vj) * (1. + vj) / Yj);
-      const double le = std::log(e[(i - 1) * ntypes + (j - 1)]);
-      h->hbeta[k] = le / std::sqrt(std::pow(le, 2.) + std::pow(kMyPi, 2.));
+      if (Y) {
+        const double Yi = Y[i - 1], Yj = Y[j - 1], vi = nu[i - 1], vj = nu[j - 1];
+        h->hYeff[k] = 1. / ((1. - std::pow(vi, 2.)) / Yi + (1. - std::pow(vj, 2.)) / Yj);
+        h->hGeff[k] = 1. / (2. * (2. - vi) * (1. + vi) / Yi + 2. * (2. - vj) * (1. + vj) / Yj);
+        const double le = std::log(e[(i - 1) * ntypes + (j - 1)]);
+        h->hbeta[k] = le / std::sqrt(std::pow(le, 2.) + std::pow(kMyPi, 2.));
+        h->hloge[k] = le;  // createCoeffRestLog (global_properties.cpp:502-520)
+      }
       h->hmu[k] = mu[(i - 1) * ntypes + (j - 1)];
       h->hmur[k] = mur ? mur[(i - 1) * ntypes + (j - 1)] : 0.0;
     }
@@ -580,12 +616,13 @@ int upload_materials(dem_handle *h) {
     h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), h->mat_block), h->allocs.end());
     h->mat_block = nullptr;
   }
+  constexpr int NT = dem::kNumMatTables;
   const size_t nn = h->hYeff.size();
-  std::vector<char> blob(nn * 5 * (8 + 4));
+  std::vector<char> blob(nn * NT * (8 + 4));
   double *dd = (double *)blob.data();
-  float *ff = (float *)(blob.data() + nn * 5 * 8);
-  const std::vector<double> *src[5] = {&h->hYeff, &h->hGeff, &h->hbeta, &h->hmu, &h->hmur};
-  for (int m = 0; m < 5; m++)
+  float *ff = (float *)(blob.data() + nn * NT * 8);
+  const auto src = host_tables(h);
+  for (int m = 0; m < NT; m++)
     for (size_t k = 0; k < nn; k++) {
       dd[m * nn + k] = (*src[m])[k];
       ff[m * nn + k] = (float)(*src[m])[k];
@@ -595,17 +632,15 @@ int upload_materials(dem_handle *h) {
   h->mat_block = h->allocs.back();
   HIPCHK(h, hipMemcpy(h->mat_block, blob.data(), blob.size(), hipMemcpyHostToDevice));
   double *db = (double *)h->mat_block;
-  float *fb = (float *)((char *)h->mat_block + nn * 5 * 8);
-  h->mat.Yeff = db;
-  h->mat.Geff = db + nn;
-  h->mat.beta = db + 2 * nn;
-  h->mat.mu = db + 3 * nn;
-  h->mat.mur = db + 4 * nn;
-  h->mat.fYeff = fb;
-  h->mat.fGeff = fb + nn;
-  h->mat.fbeta = fb + 2 * nn;
-  h->mat.fmu = fb + 3 * nn;
-  h->mat.fmur = fb + 4 * nn;
+  float *fb = (float *)((char *)h->mat_block + nn * NT * 8);
+  double **dptr[NT] = {&h->mat.Yeff, &h->mat.Geff, &h->mat.beta, &h->mat.mu,  &h->mat.mur, &h->mat.kn,
+                       &h->mat.kt,   &h->mat.gn,   &h->mat.gt,   &h->mat.ced, &h->mat.loge};
+  float **fptr[NT] = {&h->mat.fYeff, &h->mat.fGeff, &h->mat.fbeta, &h->mat.fmu,  &h->mat.fmur, &h->mat.fkn,
+                      &h->mat.fkt,   &h->mat.fgn,   &h->mat.fgt,   &h->mat.fced, &h->mat.floge};
+  for (int m = 0; m < NT; m++) {
+    *dptr[m] = db + m * nn;
+    *fptr[m] = fb + m * nn;
+  }
   h->mat_dirty = false;
   return DEM_OK;
 }
@@ -613,8 +648,7 @@ int upload_materials(dem_handle *h) {
 // resizes the host tables to ninst blocks, repeating block 0
 void replicate_material_blocks(dem_handle *h) {
   const size_t bs = (size_t)(h->ntypes + 1) * (h->ntypes + 1);
-  std::vector<double> *tabs[5] = {&h->hYeff, &h->hGeff, &h->hbeta, &h->hmu, &h->hmur};
-  for (auto *t : tabs) {
+  for (auto *t : host_tables(h)) {
     t->resize(bs * h->ninst);
     for (int q = 1; q < h->ninst; q++) std::copy(t->begin(), t->begin() + bs, t->begin() + q * bs);
   }
@@ -635,6 +669,26 @@ int check_ready(dem_handle *h) {
   for (const Wall &w : h->walls)
     if (w.type < 1 || w.type > h->ntypes)
       return fail(h, DEM_ERR_ARG, "wall type out of range of the material tables");
+  // every model in use finds its properties (PropertyRegistry::connect
+  // fails the same way when a fix property/global is missing)
+  std::vector<int> models{model_bits(h->pair_model)};
+  for (const Wall &w : h->walls) models.push_back(w.model);
+  for (const auto &m : h->meshes) models.push_back(m.model);
+  for (int b : models) {
+    const int nm = (b >> dem::kModelNormalShift) & 3;
+    const bool elastic = nm == dem::kNormalHertz || nm == dem::kNormalHooke;
+    if (elastic && !h->have_elastic)
+      return fail(h, DEM_ERR_ARG,
+                  std::string("model ") + normal_model_name(nm) +
+                      " needs youngsModulus, poissonsRatio and coefficientRestitution");
+    if (!elastic && !h->have_stiff)
+      return fail(h, DEM_ERR_ARG,
+                  std::string("model ") + normal_model_name(nm) + " needs kn, kt, gamman and gammat");
+    if (nm == dem::kNormalHooke && h->char_vel < 1e-2)  // global_properties.cpp:233-237
+      return fail(h, DEM_ERR_ARG, "characteristicVelocity >= 1e-2 required for SI units");
+    if ((b & dem::kModelSjkr) && !h->have_ced)
+      return fail(h, DEM_ERR_ARG, "cohesion sjkr needs cohesionEnergyDensity");
+  }
   return DEM_OK;
 }
 
@@ -678,6 +732,10 @@ int dem_create(const dem_params *params, dem_handle **out) {
   if (hipSetDevice(p.device) != hipSuccess) return fail(nullptr, DEM_ERR_NODEV, "hipSetDevice failed");
   dem_handle *h = new dem_handle();
   h->p = p;
+  h->pair_model.rolling_cdt = p.rolling_cdt;
+  h->pair_model.tangential_damping = p.tangential_damping;
+  h->pair_model.limit_force = p.limit_force;
+  h->pair_model.torsion_torque = p.torsion_torque;
   {
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p.device) == hipSuccess &&
@@ -738,11 +796,49 @@ int dem_set_materials(dem_handle *h, int ntypes, const double *Y, const double *
     return fail(h, DEM_ERR_STATE, "fewer atom types than the particles use");
   h->ntypes = ntypes;
   const size_t bs = (size_t)(ntypes + 1) * (ntypes + 1);
-  std::vector<double> *tabs[5] = {&h->hYeff, &h->hGeff, &h->hbeta, &h->hmu, &h->hmur};
-  for (auto *t : tabs) t->assign(bs, 0.0);
+  for (auto *t : host_tables(h)) t->assign(bs, 0.0);
   mix_block(h, 0, Y, nu, e, mu, mur);
   replicate_material_blocks(h);  // every instance starts with these materials
+  h->have_elastic = Y != nullptr;
+  h->have_stiff = h->have_ced = false;  // dem_set_model_coefficients comes after
   h->mat_set = true;
+  h->setup_done = false;
+  h->can_continue = false;
+  return DEM_OK;
+}
+
+int dem_set_pair_model(dem_handle *h, const dem_contact_model *model) {
+  if (!h || !model) return DEM_ERR_ARG;
+  if (model->normal_model < 0 || model->normal_model > 3 || model->tangential_model < 0 ||
+      model->tangential_model > 1 || model->cohesion_model < 0 || model->cohesion_model > 1)
+    return fail(h, DEM_ERR_ARG, "unknown contact model");
+  if (h->n > 0) return fail(h, DEM_ERR_STATE, "set the pair model before particles");
+  h->pair_model = *model;
+  h->setup_done = false;
+  h->can_continue = false;
+  return DEM_OK;
+}
+
+int dem_set_model_coefficients(dem_handle *h, int ntypes, const double *kn, const double *kt,
+                               const double *gamman, const double *gammat,
+                               const double *cohesion_energy_density, double characteristic_velocity) {
+  if (!h) return DEM_ERR_ARG;
+  if (!h->mat_set) return fail(h, DEM_ERR_STATE, "call dem_set_materials first");
+  if (ntypes != h->ntypes) return fail(h, DEM_ERR_ARG, "ntypes differs from the materials'");
+  if ((kn || kt || gamman || gammat) && !(kn && kt && gamman && gammat))
+    return fail(h, DEM_ERR_ARG, "kn, kt, gamman and gammat go together");
+  const int t1 = ntypes + 1;
+  const double *src[5] = {kn, kt, gamman, gammat, cohesion_energy_density};
+  std::vector<double> *dst[5] = {&h->hkn, &h->hkt, &h->hgn, &h->hgt, &h->hced};
+  for (int q = 0; q < 5; q++) {
+    if (!src[q]) continue;
+    for (int i = 1; i <= ntypes; i++)
+      for (int j = 1; j <= ntypes; j++) (*dst[q])[i * t1 + j] = src[q][(i - 1) * ntypes + (j - 1)];
+  }
+  if (kn) h->have_stiff = true;
+  if (cohesion_energy_density) h->have_ced = true;
+  h->char_vel = characteristic_velocity;
+  replicate_material_blocks(h);
   h->setup_done = false;
   h->can_continue = false;
   return DEM_OK;
@@ -802,8 +898,7 @@ int dem_add_plane_wall(dem_handle *h, int axis, double position, int wall_type,
   if (axis < 0 || axis > 2) return fail(h, DEM_ERR_ARG, "wall axis must be 0, 1 or 2");
   if ((int)h->walls.size() >= dem::kMaxWalls) return fail(h, DEM_ERR_ARG, "too many walls");
   if (h->n > 0) return fail(h, DEM_ERR_STATE, "add walls before particles");
-  dem_contact_model m{h->p.rolling_cdt, h->p.tangential_damping, h->p.limit_force,
-                      h->p.torsion_torque};
+  dem_contact_model m = h->pair_model;
   if (model) m = *model;
   h->walls.push_back({axis, position, wall_type, model_bits(m)});
   return DEM_OK;
@@ -820,8 +915,7 @@ int dem_add_mesh(dem_handle *h, int64_t ntri, const double *nodes, int wall_type
   if (!dem::build_host_mesh(hm, (int)ntri, nodes, h->p.boxlo, h->p.boxhi, err))
     return fail(h, DEM_ERR_ARG, err);
   hm.wall_type = wall_type;
-  dem_contact_model m{h->p.rolling_cdt, h->p.tangential_damping, h->p.limit_force,
-                      h->p.torsion_torque};
+  dem_contact_model m = h->pair_model;
   if (model) m = *model;
   hm.model = model_bits(m);
   h->meshes.push_back(std::move(hm));
@@ -1086,7 +1180,8 @@ int dem_set_step_shape(dem_handle *h, int variant, int persist) {
   if (variant == -1) {
     tu = dem_handle::Tuner{};
     tu.active = true;
-    for (int v = 0; v < dem::kNumStepVariants; v++)
+    const int nv = uses_generic_models(h) ? 1 : dem::kNumStepVariants;
+    for (int v = 0; v < nv; v++)
       for (int f : {0, 4, 8}) tu.cand.push_back({v, f});
     tu.us.assign(tu.cand.size(), 0.0);
     return DEM_OK;

@@ -25,7 +25,8 @@ EXPORTED = [
     "dem_run_script", "dem_add_mesh", "dem_mesh_rotate", "dem_get_mesh_nodes",
     "dem_get_mesh_topology", "dem_get_mesh_history", "dem_set_instances",
     "dem_set_instance_materials", "dem_get_instance_stats", "dem_set_step_shape",
-    "dem_get_step_shape", "dem_set_graph_capture", "dem_sq_create", "dem_sq_destroy",
+    "dem_get_step_shape", "dem_set_graph_capture", "dem_set_pair_model", "dem_set_model_coefficients",
+    "dem_sq_create", "dem_sq_destroy",
     "dem_sq_last_error", "dem_sq_set_pairs", "dem_sq_solve", "dem_sq_check", "dem_sq_get_results",
     "dem_sq_get_features", "dem_sq_net_init", "dem_sq_net_set_params", "dem_sq_net_get_params",
     "dem_sq_net_forward", "dem_sq_net_gradient", "dem_sq_net_train",
@@ -65,16 +66,38 @@ class DemParams(ctypes.Structure):
 
 class DemContactModel(ctypes.Structure):
     _fields_ = [("rolling_cdt", ctypes.c_int), ("tangential_damping", ctypes.c_int),
-                ("limit_force", ctypes.c_int), ("torsion_torque", ctypes.c_int)]
+                ("limit_force", ctypes.c_int), ("torsion_torque", ctypes.c_int),
+                ("normal_model", ctypes.c_int), ("tangential_model", ctypes.c_int),
+                ("cohesion_model", ctypes.c_int), ("kt_to_kn", ctypes.c_int),
+                ("absolute_damping", ctypes.c_int), ("tangential_reduce", ctypes.c_int)]
+
+
+# enum dem_normal_model and the tangential / cohesion model codes
+NORMAL_MODELS = {"hertz": 0, "hooke": 1, "hertz/stiffness": 2, "hooke/stiffness": 3}
+TANGENTIAL_MODELS = {"history": 0, "no_history": 1}
+COHESION_MODELS = {None: 0, "off": 0, "sjkr": 1}
 
 
 def contact_model(d):
+    """dem_contact_model from a dict: rolling_cdt, tangential_damping,
+    limit_force, torsion_torque, normal ("hertz" | "hooke" | "hertz/stiffness" |
+    "hooke/stiffness"), tangential ("history" | "no_history"), cohesion
+    (None | "sjkr"), kt_to_kn, absolute_damping, tangential_reduce."""
     m = DemContactModel()
     m.rolling_cdt = int(d.get("rolling_cdt", 0))
     m.tangential_damping = int(d.get("tangential_damping", 1))
     m.limit_force = int(d.get("limit_force", 0))
     m.torsion_torque = int(d.get("torsion_torque", 0))
+    m.normal_model = NORMAL_MODELS[d.get("normal", "hertz")]
+    m.tangential_model = TANGENTIAL_MODELS[d.get("tangential", "history")]
+    m.cohesion_model = COHESION_MODELS[d.get("cohesion")]
+    m.kt_to_kn = int(d.get("kt_to_kn", 0))
+    m.absolute_damping = int(d.get("absolute_damping", 0))
+    m.tangential_reduce = int(d.get("tangential_reduce", 0))
     return m
+
+
+MODEL_COEFFS = ("kn", "kt", "gamman", "gammat", "cohesion_energy_density")
 
 
 class DemStats(ctypes.Structure):
@@ -131,6 +154,8 @@ def load_library(path=LIB_PATH):
     lib.dem_last_error.argtypes = [h]
     lib.dem_last_error.restype = ctypes.c_char_p
     lib.dem_set_materials.argtypes = [h, ctypes.c_int, dp, dp, dp, dp, dp]
+    lib.dem_set_pair_model.argtypes = [h, P(DemContactModel)]
+    lib.dem_set_model_coefficients.argtypes = [h, ctypes.c_int, dp, dp, dp, dp, dp, ctypes.c_double]
     lib.dem_set_particles.argtypes = [h, ctypes.c_int64, dp, dp, dp, dp, dp, ip, ip]
     lib.dem_add_plane_wall.argtypes = [h, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                        P(DemContactModel)]
@@ -204,18 +229,36 @@ class DemError(RuntimeError):
     pass
 
 
+def _opt(a, flat=True):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a.reshape(-1) if flat else a
+
+
 def _material_args(materials):
-    """(ntypes, Y, nu, e, mu, mur) pointers for dem_set_materials; the arrays
-    are kept alive on the returned tuple's owner by the caller's frame."""
+    """(ntypes, Y, nu, e, mu, mur) pointers for dem_set_materials (youngs,
+    poisson and restitution may be absent together); the arrays are kept
+    alive on this function until the next call."""
     nt = materials["ntypes"]
-    Y = np.ascontiguousarray(materials["youngs"], dtype=np.float64)
-    nu = np.ascontiguousarray(materials["poisson"], dtype=np.float64)
-    e = np.ascontiguousarray(materials["restitution"], dtype=np.float64).reshape(-1)
-    mu = np.ascontiguousarray(materials["friction"], dtype=np.float64).reshape(-1)
-    mur = materials.get("rolling_friction")
-    mur = None if mur is None else np.ascontiguousarray(mur, dtype=np.float64).reshape(-1)
+    Y, nu, e = (_opt(materials.get(k)) for k in ("youngs", "poisson", "restitution"))
+    mu = _opt(materials["friction"])
+    mur = _opt(materials.get("rolling_friction"))
     _material_args.keep = (Y, nu, e, mu, mur)
-    return (nt, _dp(Y), _dp(nu), _dp(e), _dp(mu), _dp(mur) if mur is not None else None)
+    ptr = lambda a: _dp(a) if a is not None else None
+    return (nt, ptr(Y), ptr(nu), ptr(e), ptr(mu), ptr(mur))
+
+
+def _coefficient_args(materials):
+    """dem_set_model_coefficients arguments, or None when the material dict
+    holds none of kn / kt / gamman / gammat / cohesion_energy_density /
+    characteristic_velocity."""
+    if not any(k in materials for k in MODEL_COEFFS + ("characteristic_velocity",)):
+        return None
+    arrs = [_opt(materials.get(k)) for k in MODEL_COEFFS]
+    _coefficient_args.keep = arrs
+    return (materials["ntypes"], *[_dp(a) if a is not None else None for a in arrs],
+            float(materials.get("characteristic_velocity", 0.0)))
 
 
 class GpuSystem:
@@ -258,6 +301,11 @@ class GpuSystem:
             raise DemError(f"dem_create failed ({rc}): {self.lib.dem_last_error(None).decode()}")
         self.precision = precision
         self._check(self.lib.dem_set_materials(self.h, *_material_args(materials)))
+        coeffs = _coefficient_args(materials)
+        if coeffs is not None:
+            self._check(self.lib.dem_set_model_coefficients(self.h, *coeffs))
+        pm = contact_model(params)
+        self._check(self.lib.dem_set_pair_model(self.h, ctypes.byref(pm)))
         self.ninst = 1
         if instances is not None:
             counts = np.ascontiguousarray(instances, dtype=np.int64)

@@ -1,0 +1,209 @@
+"""GPU parity of the sibling contact models (pair_style gran model hooke,
+hertz/stiffness, hooke/stiffness; tangential no_history; cohesion sjkr with
+tangential_reduce; ktToKnUser; absolute_damping) and of walls and meshes whose
+fix wall/gran selects its own model, against the CPU oracle:
+
+  pair sets, flags            exact
+  forces / torques / history  1e-10 relative in FP64 (setup and one step)
+  mixed mode                  forces and torques 1e-5 relative to the FP64 oracle
+  300 steps (chaotic)         kinetic energy and contact count within 1 %
+  missing properties          the run refuses to start, naming what is missing
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import meshes as msh
+import pydem
+import pyoracle as orc
+import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+MIXED_TOL = 1e-5
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _stiff(mats, kn, kt, gn, gt):
+    """the stiffness models' coefficients, and no elastic properties"""
+    m = {k: v for k, v in mats.items() if k not in ("youngs", "poisson", "restitution")}
+    m.update(kn=[[kn]], kt=[[kt]], gamman=[[gn]], gammat=[[gt]])
+    return m
+
+
+def _oracle_mats(mats):
+    """the oracle's mixing needs elastic values even where no model reads them"""
+    if "youngs" in mats:
+        return mats
+    return dict(mats, youngs=[5e6], poisson=[0.3], restitution=[[0.5]])
+
+
+def _case(model, mats_fn, wall_model=None, seed=31, n=2000):
+    params, mats, P, walls = wl.dense_pack(seed=seed, n=n, rolling=bool(model.get("rolling_cdt", 1)),
+                                           speed=0.2)
+    params = dict(params, **model)
+    mats = mats_fn(mats)
+    if wall_model is not None:
+        walls = [w[:3] + (dict(wall_model),) for w in walls]
+    return params, mats, P, walls
+
+
+CASES = {
+    "hooke_ktToKn_cdt": lambda: _case({"normal": "hooke", "kt_to_kn": 1, "rolling_cdt": 1},
+                                      lambda m: dict(m, characteristic_velocity=0.5)),
+    "hooke_nohistory_sjkr_reduce": lambda: _case(
+        {"normal": "hooke", "tangential": "no_history", "cohesion": "sjkr", "tangential_reduce": 1,
+         "rolling_cdt": 0},
+        lambda m: dict(m, characteristic_velocity=0.5, cohesion_energy_density=[[3e5]])),
+    "hertz_stiffness_cdt": lambda: _case({"normal": "hertz/stiffness", "rolling_cdt": 1},
+                                         lambda m: _stiff(m, 4e6, 5e6, 2e5, 1e5)),
+    "hooke_stiffness_absolute_notdamp": lambda: _case(
+        {"normal": "hooke/stiffness", "absolute_damping": 1, "tangential_damping": 0, "rolling_cdt": 0},
+        lambda m: _stiff(m, 400.0, 300.0, 0.01, 0.005)),
+    "hertz_sjkr_wall_hooke_nohistory": lambda: _case(
+        {"cohesion": "sjkr", "rolling_cdt": 1},
+        lambda m: dict(m, characteristic_velocity=0.5, cohesion_energy_density=[[2e5]]),
+        wall_model={"normal": "hooke", "tangential": "no_history", "rolling_cdt": 0}),
+}
+
+
+def build(params, mats, P, walls, precision=pydem.DEM_FP64, meshes=(), max_neighbors=0):
+    g = pydem.GpuSystem(params, mats, P, walls, precision=precision, meshes=meshes,
+                        max_neighbors=max_neighbors)
+    o = orc.OracleSystem(params, _oracle_mats(mats), P, walls, meshes=meshes)
+    return g, o
+
+
+def rel(a, b):
+    den = max(np.linalg.norm(b), 1e-300)
+    per = np.linalg.norm(a - b, axis=1).max() / max(np.linalg.norm(b, axis=1).max(), 1e-300)
+    return np.linalg.norm(a - b) / den, per
+
+
+def assert_close(name, a, b, tol):
+    nw, per = rel(a, b)
+    say(f"  {name}: normwise {nw:.3e} per-particle {per:.3e}")
+    assert nw <= tol and per <= tol, f"{name}: {nw:.3e} / {per:.3e} > {tol}"
+
+
+def compare_pairs(g, o, tol=FP64_TOL):
+    gt, gf, gh = g.pairs()
+    ot, of, oh = o.pairs()
+    gm = {(int(a), int(b)): (int(f), h) for (a, b), f, h in zip(gt, gf, gh)}
+    om = {(int(a), int(b)): (int(f), h) for (a, b), f, h in zip(ot, of, oh)}
+    assert set(gm) == set(om), f"pair sets differ: {len(set(gm) ^ set(om))} pairs"
+    scale = max(np.abs(oh).max(), 1e-300)
+    for k, (f, h) in om.items():
+        assert gm[k][0] == f, f"flag differs for pair {k}"
+        assert np.abs(gm[k][1] - h).max() <= tol * scale, f"history differs for pair {k}"
+    return len(om), sum(f for f, _ in om.values())
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_setup_and_one_step_fp64(case):
+    say(f"[gpu] contact models, setup + one step fp64, case={case}")
+    params, mats, P, walls = CASES[case]()
+    g, o = build(params, mats, P, walls)
+    g.setup()
+    o.setup()
+    npairs, nflag = compare_pairs(g, o)
+    say(f"  pairs={npairs} touching={nflag}")
+    assert nflag > 500
+    gs, os_ = g.state(), o.state()
+    assert_close("setup f", gs["f"], os_["f"], FP64_TOL)
+    assert_close("setup torque", gs["torque"], os_["torque"], FP64_TOL)
+    g.run(1)
+    o.run(1)
+    gs, os_ = g.state(), o.state()
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert_close(k, gs[k], os_[k], FP64_TOL)
+    compare_pairs(g, o)
+    gw, ow = g.wall_history(), o.wall_history()
+    assert np.abs(gw - ow).max() <= FP64_TOL * max(np.abs(ow).max(), 1e-300)
+    st = g.stats()
+    assert st["ncontacts"] == o.contact_counts()[0]
+    assert st["nwall_contacts"] == o.contact_counts()[1] > 0
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_mixed_forces_against_fp64_oracle(case):
+    say(f"[gpu] contact models, mixed precision, case={case}")
+    params, mats, P, walls = CASES[case]()
+    g, o = build(params, mats, P, walls, precision=pydem.DEM_MIXED)
+    g.setup()
+    o.setup()
+    gs, os_ = g.state(), o.state()
+    assert_close("f", gs["f"], os_["f"], MIXED_TOL)
+    assert_close("torque", gs["torque"], os_["torque"], MIXED_TOL)
+    g.close()
+    o.close()
+
+
+def _ke(st, m):
+    return 0.5 * (m[:, None] * st["v"] ** 2).sum()
+
+
+@pytest.mark.parametrize("case", ["hooke_nohistory_sjkr_reduce", "hertz_stiffness_cdt"])
+def test_three_hundred_steps_statistics(case):
+    say(f"[gpu] contact models, 300 steps, case={case}")
+    params, mats, P, walls = CASES[case]()
+    g, o = build(params, mats, P, walls, max_neighbors=64)  # cohesion pulls the pack together
+    g.run(300)
+    o.run(300)
+    gs, os_ = g.state(), o.state()
+    kg, ko = _ke(gs, P["rmass"]), _ke(os_, P["rmass"])
+    cg, co = g.stats()["ncontacts"], o.contact_counts()[0]
+    say(f"  KE {kg:.6e} / {ko:.6e}  contacts {cg} / {co}")
+    assert abs(kg - ko) <= 0.01 * ko
+    assert abs(cg - co) <= 0.01 * co
+    g.close()
+    o.close()
+
+
+def test_mesh_wall_with_its_own_model_fp64():
+    """A triangle-mesh floor whose fix wall/gran uses hooke + sjkr while the
+    pair style is hertz: one step equal to the oracle, mesh history included."""
+    say("[gpu] contact models on a mesh floor")
+    params, mats, P, _ = wl.dense_pack(seed=41, n=2000, rolling=True, periodic=(0, 0, 0), speed=0.2)
+    lo, hi = params["boxlo"], params["boxhi"]
+    floor = msh.grid(lo[0] - 0.002, hi[0] + 0.002, lo[1] - 0.002, hi[1] + 0.002, lo[2], 7, 6)
+    mats = dict(wl.materials_two(rolling_friction=0.1), characteristic_velocity=0.5,
+                cohesion_energy_density=[[2e5, 2e5], [2e5, 2e5]])
+    meshes = [(floor, 2, {"normal": "hooke", "cohesion": "sjkr", "rolling_cdt": 1}, None)]
+    g, o = build(params, mats, P, [], meshes=meshes)
+    g.run(1)
+    o.run(1)
+    gs, os_ = g.state(), o.state()
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert_close(k, gs[k], os_[k], FP64_TOL)
+    gt, gh = g.mesh_history()
+    ot, oh = o.mesh_history()
+    assert np.array_equal(gt[:, :8], ot[:, :8])
+    assert (ot >= 0).sum() > 50
+    assert np.abs(gh[:, :8] - oh[:, :8]).max() <= FP64_TOL * max(np.abs(oh).max(), 1e-300)
+    assert g.stats()["nwall_contacts"] > 50  # the oracle's counter covers plane walls only
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("model,mats_fn,msg", [
+    ({"normal": "hertz/stiffness"}, lambda m: m, "needs kn"),
+    ({"normal": "hooke"}, lambda m: m, "characteristicVelocity"),
+    ({"cohesion": "sjkr"}, lambda m: m, "cohesionEnergyDensity"),
+    ({"normal": "hertz"}, lambda m: _stiff(m, 1.0, 1.0, 1.0, 1.0), "youngsModulus"),
+])
+def test_missing_properties_are_refused(model, mats_fn, msg):
+    say(f"[gpu] missing property for {model}")
+    params, mats, P, walls = _case(model, mats_fn, n=200)
+    g = pydem.GpuSystem(params, mats, P, walls)
+    with pytest.raises(pydem.DemError, match=msg):
+        g.run(1)
+    g.close()
