@@ -11,16 +11,20 @@
 //                                       [contact_distance_factor f] [one n]
 //   fix <id> <grp> property/global youngsModulus|poissonsRatio peratomtype ...
 //   fix <id> <grp> property/global coefficientRestitution|coefficientFriction|
-//                  coefficientRollingFriction peratomtypepair <n> ...
-//   pair_style gran model hertz tangential history [rolling_friction cdt]
+//                  coefficientRollingFriction|kn|kt|gamman|gammat|gamman_abs|
+//                  gammat_abs|cohesionEnergyDensity peratomtypepair <n> ...
+//   fix <id> <grp> property/global characteristicVelocity scalar <v>
+//   pair_style gran model hertz|hooke|hertz/stiffness|hooke/stiffness
+//              tangential history|no_history [cohesion sjkr] [rolling_friction cdt]
 //              [tangential_damping on|off] [limitForce on|off] [torsionTorque on|off]
+//              [ktToKnUser on|off] [absolute_damping on|off] [tangential_reduce on|off]
 //   pair_coeff * * | timestep <dt>
 //   fix <id> <grp> gravity <g> vector <x> <y> <z>
-//   fix <id> <grp> wall/gran model hertz tangential history [rolling_friction cdt]
+//   fix <id> <grp> wall/gran model <normal> tangential <tangential> [...]
 //                  primitive type <t> xplane|yplane|zplane <pos>
 //   fix <id> <grp> mesh/surface file <stl> type <t> [scale s] [move dx dy dz]
 //                  [rotate axis ax ay az angle deg] [precision p] [verbose/heal ...]
-//   fix <id> <grp> wall/gran model hertz tangential history [...] mesh n_meshes <k>
+//   fix <id> <grp> wall/gran model <normal> tangential <tangential> [...] mesh n_meshes <k>
 //                  meshes <id1> ... <idk>
 //   fix <id> <grp> move/mesh mesh <meshid> rotate origin ox oy oz axis ax ay az period T
 //   fix <id> <grp> nve/sphere
@@ -126,6 +130,9 @@ struct Interp {
   int one = 0;
   // materials
   std::vector<double> Y, nu, e, mu, mur;
+  // sibling-model properties: kn, kt, gamman, gammat, gamman_abs, gammat_abs,
+  // cohesionEnergyDensity (peratomtypepair) and characteristicVelocity
+  std::map<std::string, std::vector<double>> model_props;
   // superquadrics (atom_style superquadric)
   bool sq = false;
   std::vector<double> sq_shape, sq_blk, sq_dens, sq_quat, sq_mass;
@@ -133,7 +140,8 @@ struct Interp {
   dem_sqdem_handle *hs = nullptr;
   // pair
   bool pair_set = false;
-  int rolling = 0, tdamp = 1, limitf = 0, torsion = 0;
+  // pair_style gran model switches (tangential damping on by default)
+  dem_contact_model pair_m{0, 1, 0, 0, DEM_NORMAL_HERTZ, 0, 0, 0, 0, 0};
   // fixes
   bool nve = false;
   bool gravity = false, gravity_first = true, wall_seen = false;
@@ -373,6 +381,10 @@ struct Interp {
     else if (name == "coefficientRestitution") e = vals;
     else if (name == "coefficientFriction") mu = vals;
     else if (name == "coefficientRollingFriction") mur = vals;
+    else if (name == "kn" || name == "kt" || name == "gamman" || name == "gammat" ||
+             name == "gamman_abs" || name == "gammat_abs" || name == "cohesionEnergyDensity" ||
+             name == "characteristicVelocity")
+      model_props[name] = vals;
     else fprintf(out, "WARNING: property/global %s is not used by this path\n", name.c_str());
     dirty = true;
   }
@@ -428,24 +440,45 @@ struct Interp {
 
   void parse_model(const std::vector<std::string> &t, size_t k, size_t &end, bool pair,
                    dem_contact_model *wall_model = nullptr) {
-    // model hertz tangential history [rolling_friction cdt] [settings...]
+    // model <normal> tangential <tangential> [cohesion sjkr] [rolling_friction cdt]
+    // [settings on|off ...] (the keyword loop of PairGranProxy::settings and
+    // Settings::parseArguments)
     bool saw_model = false, saw_tang = false;
-    int roll = 0, td = 1, lf = 0, tor = 0;
+    dem_contact_model mm{0, 1, 0, 0, DEM_NORMAL_HERTZ, 0, 0, 0, 0, 0};
     while (k < t.size()) {
       if (t[k] == "model" && k + 1 < t.size()) {
-        if (t[k + 1] != "hertz") die("Only 'model hertz' is available on this path, got " + t[k + 1]);
+        const std::string &nm = t[k + 1];
+        if (nm == "hertz") mm.normal_model = DEM_NORMAL_HERTZ;
+        else if (nm == "hooke") mm.normal_model = DEM_NORMAL_HOOKE;
+        else if (nm == "hertz/stiffness") mm.normal_model = DEM_NORMAL_HERTZ_STIFFNESS;
+        else if (nm == "hooke/stiffness") mm.normal_model = DEM_NORMAL_HOOKE_STIFFNESS;
+        else die("model " + nm + " is not available on this path (hertz, hooke, hertz/stiffness, hooke/stiffness)");
         saw_model = true;
         k += 2;
       } else if (t[k] == "tangential" && k + 1 < t.size()) {
-        if (t[k + 1] != "history") die("Only 'tangential history' is available, got " + t[k + 1]);
+        if (t[k + 1] == "history") mm.tangential_model = 0;
+        else if (t[k + 1] == "no_history") mm.tangential_model = 1;
+        else die("tangential " + t[k + 1] + " is not available on this path (history, no_history)");
         saw_tang = true;
         k += 2;
       } else if (t[k] == "rolling_friction" && k + 1 < t.size()) {
-        if (t[k + 1] == "cdt") roll = 1;
+        if (t[k + 1] == "cdt") mm.rolling_cdt = 1;
         else if (t[k + 1] != "off") die("Only 'rolling_friction cdt' is available, got " + t[k + 1]);
         k += 2;
       } else if (t[k] == "cohesion" && k + 1 < t.size()) {
-        if (t[k + 1] != "off") die("cohesion models are not available on this path");
+        if (t[k + 1] == "sjkr") mm.cohesion_model = 1;
+        else if (t[k + 1] != "off") die("cohesion " + t[k + 1] + " is not available on this path (sjkr)");
+        k += 2;
+      } else if (t[k] == "viscous" && k + 1 < t.size()) {
+        if (t[k + 1] != "off") die("model hooke viscous on is not available on this path");
+        k += 2;
+      } else if ((t[k] == "ktToKnUser" || t[k] == "absolute_damping" || t[k] == "tangential_reduce") &&
+                 k + 1 < t.size()) {
+        const int on = t[k + 1] == "on" ? 1 : (t[k + 1] == "off" ? 0 : -1);
+        if (on < 0) die("Expected on/off for " + t[k]);
+        if (t[k] == "ktToKnUser") mm.kt_to_kn = on;
+        else if (t[k] == "absolute_damping") mm.absolute_damping = on;
+        else mm.tangential_reduce = on;
         k += 2;
       } else if (t[k] == "surface" && k + 1 < t.size()) {
         if (t[k + 1] == "superquadric") {
@@ -461,27 +494,18 @@ struct Interp {
                  k + 1 < t.size()) {
         const int on = t[k + 1] == "on" ? 1 : (t[k + 1] == "off" ? 0 : -1);
         if (on < 0) die("Expected on/off for " + t[k]);
-        if (t[k] == "tangential_damping") td = on;
-        else if (t[k] == "limitForce") lf = on;
-        else tor = on;
+        if (t[k] == "tangential_damping") mm.tangential_damping = on;
+        else if (t[k] == "limitForce") mm.limit_force = on;
+        else mm.torsion_torque = on;
         k += 2;
       } else {
         break;
       }
     }
-    if (!saw_model || !saw_tang) die("gran model needs 'model hertz tangential history'");
+    if (!saw_model || !saw_tang) die("gran model needs 'model <normal> tangential <tangential>'");
     end = k;
-    if (pair) {
-      rolling = roll;
-      tdamp = td;
-      limitf = lf;
-      torsion = tor;
-    } else if (wall_model) {
-      wall_model->rolling_cdt = roll;
-      wall_model->tangential_damping = td;
-      wall_model->limit_force = lf;
-      wall_model->torsion_torque = tor;
-    }
+    if (pair) pair_m = mm;
+    else if (wall_model) *wall_model = mm;
   }
 
   // Set::command for quat and quat/random (set.cpp:345-365, 1041-1065,
@@ -548,10 +572,16 @@ struct Interp {
     if (skin < 0) die("run: neighbor skin not set");
     if (every != 1 || no_check || cdf != 1.0)
       die("superquadrics: neigh_modify every/check/contact_distance_factor must keep their defaults");
-    if (rolling || torsion) die("superquadrics: rolling friction is not available on this path");
+    if (pair_m.rolling_cdt || pair_m.torsion_torque)
+      die("superquadrics: rolling friction is not available on this path");
     for (auto &wl : walls)
       if (wl.model.rolling_cdt || wl.model.torsion_torque)
         die("superquadrics: rolling friction is not available on this path");
+    std::vector<const dem_contact_model *> sq_models{&pair_m};
+    for (auto &wl : walls) sq_models.push_back(&wl.model);
+    for (const dem_contact_model *cm : sq_models)
+      if (cm->normal_model != DEM_NORMAL_HERTZ || cm->tangential_model || cm->cohesion_model)
+        die("superquadrics: only model hertz tangential history is available on this path");
     if (!mesh_order.empty()) die("superquadrics: mesh walls are not available on this path");
     if ((int)Y.size() != ntypes || (int)nu.size() != ntypes)
       die("youngsModulus/poissonsRatio need one value per atom type");
@@ -570,8 +600,8 @@ struct Interp {
       p.boxhi[d] = hi[d];
       p.periodic[d] = periodic[d];
     }
-    p.tangential_damping = tdamp;
-    p.limit_force = limitf;
+    p.tangential_damping = pair_m.tangential_damping;
+    p.limit_force = pair_m.limit_force;
     p.has_gravity = gravity;
     for (int d = 0; d < 3; d++) p.gravity[d] = gacc[d];
     p.gravity_first = gravity_first;
@@ -603,15 +633,57 @@ struct Interp {
     if (!nve) die("run: no fix nve/sphere, particles would not move");
     if (dt <= 0) die("run: timestep not set");
     if (skin < 0) die("run: neighbor skin not set");
-    if ((int)Y.size() != ntypes || (int)nu.size() != ntypes)
-      die("youngsModulus/poissonsRatio need one value per atom type");
-    if ((int)e.size() != ntypes * ntypes || (int)mu.size() != ntypes * ntypes)
-      die("coefficientRestitution/coefficientFriction need ntypes^2 values");
-    bool any_rolling = rolling;
-    for (auto &wl : walls) any_rolling = any_rolling || wl.model.rolling_cdt;
-    for (auto &name : mesh_order) any_rolling = any_rolling || mesh_defs[name].model.rolling_cdt;
+    std::vector<const dem_contact_model *> models{&pair_m};
+    for (auto &wl : walls) models.push_back(&wl.model);
+    for (auto &name : mesh_order) models.push_back(&mesh_defs[name].model);
+    bool any_rolling = false, elastic = false, stiff = false, abs_damp = false, rel_damp = false;
+    bool hooke = false, sjkr = false;
+    for (const dem_contact_model *cm : models) {
+      any_rolling = any_rolling || cm->rolling_cdt;
+      const bool st = cm->normal_model == DEM_NORMAL_HERTZ_STIFFNESS ||
+                      cm->normal_model == DEM_NORMAL_HOOKE_STIFFNESS;
+      elastic = elastic || !st;
+      stiff = stiff || st;
+      const bool ab = cm->normal_model == DEM_NORMAL_HOOKE_STIFFNESS && cm->absolute_damping;
+      abs_damp = abs_damp || ab;
+      rel_damp = rel_damp || (st && !ab);
+      hooke = hooke || cm->normal_model == DEM_NORMAL_HOOKE;
+      sjkr = sjkr || cm->cohesion_model;
+    }
+    // Yeff / Geff / restitution are connected by hertz and hooke only
+    // (normal_model_*.h connectToProperties)
+    const bool have_el = !Y.empty() || !nu.empty() || !e.empty();
+    if (elastic || have_el) {
+      if ((int)Y.size() != ntypes || (int)nu.size() != ntypes)
+        die("youngsModulus/poissonsRatio need one value per atom type");
+      if ((int)e.size() != ntypes * ntypes) die("coefficientRestitution needs ntypes^2 values");
+    }
+    if ((int)mu.size() != ntypes * ntypes) die("coefficientFriction needs ntypes^2 values");
     if (any_rolling && (int)mur.size() != ntypes * ntypes)
       die("rolling_friction cdt needs coefficientRollingFriction (ntypes^2 values)");
+    if (abs_damp && rel_damp)
+      die("hooke/stiffness with and without absolute_damping in one run is not available on this path");
+    auto pair_prop = [&](const char *name, bool need) -> const double * {
+      auto it = model_props.find(name);
+      if (it == model_props.end()) {
+        if (need) die(std::string("property/global ") + name + " (peratomtypepair) is needed by the contact model");
+        return nullptr;
+      }
+      if ((int)it->second.size() != ntypes * ntypes)
+        die(std::string("property/global ") + name + " needs ntypes^2 values");
+      return it->second.data();
+    };
+    const double *kn = pair_prop("kn", stiff), *kt = pair_prop("kt", stiff);
+    const double *gn = pair_prop(abs_damp ? "gamman_abs" : "gamman", stiff);
+    const double *gt = pair_prop(abs_damp ? "gammat_abs" : "gammat", stiff);
+    const double *ced = pair_prop("cohesionEnergyDensity", sjkr);
+    double cv = 0.0;
+    if (model_props.count("characteristicVelocity")) {
+      if (model_props["characteristicVelocity"].size() != 1) die("characteristicVelocity is one scalar");
+      cv = model_props["characteristicVelocity"][0];
+    } else if (hooke) {
+      die("model hooke needs fix property/global characteristicVelocity scalar");
+    }
     if (h) {
       dem_destroy(h);
       h = nullptr;
@@ -627,10 +699,10 @@ struct Interp {
       p.boxhi[d] = hi[d];
       p.periodic[d] = periodic[d];
     }
-    p.rolling_cdt = rolling;
-    p.tangential_damping = tdamp;
-    p.limit_force = limitf;
-    p.torsion_torque = torsion;
+    p.rolling_cdt = pair_m.rolling_cdt;
+    p.tangential_damping = pair_m.tangential_damping;
+    p.limit_force = pair_m.limit_force;
+    p.torsion_torque = pair_m.torsion_torque;
     p.has_gravity = gravity;
     for (int d = 0; d < 3; d++) p.grav_acc[d] = gacc[d];
     p.gravity_first = gravity_first;
@@ -639,9 +711,14 @@ struct Interp {
     p.device = 0;
     int rc = dem_create(&p, &h);
     if (rc) die(std::string("GPU library: ") + dem_last_error(nullptr));
-    if (dem_set_materials(h, ntypes, Y.data(), nu.data(), e.data(), mu.data(),
-                          any_rolling ? mur.data() : nullptr))
+    if (dem_set_materials(h, ntypes, have_el ? Y.data() : nullptr, have_el ? nu.data() : nullptr,
+                          have_el ? e.data() : nullptr, mu.data(), any_rolling ? mur.data() : nullptr))
       die(dem_last_error(h));
+    if ((stiff || sjkr || hooke) && dem_set_model_coefficients(h, ntypes, stiff ? kn : nullptr,
+                                                               stiff ? kt : nullptr, stiff ? gn : nullptr,
+                                                               stiff ? gt : nullptr, ced, cv))
+      die(dem_last_error(h));
+    if (dem_set_pair_model(h, &pair_m)) die(dem_last_error(h));
     for (auto &wl : walls)
       if (dem_add_plane_wall(h, wl.axis, wl.pos, wl.type, &wl.model)) die(dem_last_error(h));
     for (auto &name : mesh_order) {

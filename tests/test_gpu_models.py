@@ -207,3 +207,64 @@ def test_missing_properties_are_refused(model, mats_fn, msg):
     with pytest.raises(pydem.DemError, match=msg):
         g.run(1)
     g.close()
+
+
+def test_script_with_sibling_models_matches_api(tmp_path):
+    """An input script with pair_style gran model hooke tangential no_history
+    cohesion sjkr and a hertz/stiffness floor (fix wall/gran with its own
+    model) gives the kinetic energy of the same system driven through the C ABI."""
+    say("[gpu] script front end with sibling contact models")
+    params, mats, P, walls = wl.settling_4k(seed=5, n=800)
+    data = tmp_path / "s.data"
+    wl.write_data_file(str(data), params, P)
+    script = tmp_path / "in.models"
+    script.write_text(f"""atom_style granular
+atom_modify map array
+boundary p p f
+newton off
+communicate single vel yes
+units si
+region reg block 0 0.1 0 0.1 0 0.2 units box
+create_box 1 reg
+neighbor 0.001 bin
+neigh_modify delay 0
+fix m1 all property/global youngsModulus peratomtype 5.e6
+fix m2 all property/global poissonsRatio peratomtype 0.45
+fix m3 all property/global coefficientRestitution peratomtypepair 1 0.3
+fix m4 all property/global coefficientFriction peratomtypepair 1 0.5
+fix m5 all property/global characteristicVelocity scalar 1.0
+fix m6 all property/global cohesionEnergyDensity peratomtypepair 1 50000
+fix m7 all property/global kn peratomtypepair 1 5e6
+fix m8 all property/global kt peratomtypepair 1 4e6
+fix m9 all property/global gamman peratomtypepair 1 3e4
+fix m10 all property/global gammat peratomtypepair 1 2e4
+pair_style gran model hooke tangential no_history cohesion sjkr
+pair_coeff * *
+timestep 0.00001
+fix gravi all gravity 9.81 vector 0.0 0.0 -1.0
+fix zwall all wall/gran model hertz/stiffness tangential history primitive type 1 zplane 0.0
+read_data {data}
+fix integr all nve/sphere
+thermo_style custom step atoms ke
+thermo 500
+run 1500
+""")
+    log = tmp_path / "log.models"
+    rc = pydem.run_script(str(script), str(log))
+    text = log.read_text()
+    say(text[-600:])
+    assert rc == 0
+    rows = [l.split() for l in text.splitlines() if l.strip() and l.split()[0].isdigit()]
+    last = rows[-1]
+    assert int(last[0]) == 1500 and int(last[1]) == 800
+    params = dict(params, normal="hooke", tangential="no_history", cohesion="sjkr", rolling_cdt=0)
+    mats = dict(mats, characteristic_velocity=1.0, cohesion_energy_density=[[5e4]], kn=[[5e6]],
+                kt=[[4e6]], gamman=[[3e4]], gammat=[[2e4]])
+    walls = [w[:3] + ({"normal": "hertz/stiffness"},) for w in walls]
+    g = pydem.GpuSystem(params, mats, P, walls)
+    g.run(1500)
+    st = g.state()
+    ke = 0.5 * (P["rmass"] @ (st["v"] ** 2).sum(1))
+    assert float(last[2]) == pytest.approx(ke, rel=1e-6)
+    assert g.stats()["nwall_contacts"] > 0
+    g.close()

@@ -62,7 +62,10 @@ def test_full_command_subset_parses(tmp_path):
 
 @pytest.mark.parametrize("bad,msg", [
     ("newton on\n", "newton pair off"),
-    ("pair_style gran model hooke tangential history\n", "model hertz"),
+    ("pair_style gran model hooke/hysteresis tangential history\n", "is not available"),
+    ("pair_style gran model hertz tangential history cohesion washino\n", "cohesion washino"),
+    ("pair_style gran model hooke tangential history viscous on\n", "viscous"),
+    ("pair_style gran model hertz tangential history_simple\n", "tangential history_simple"),
     ("fix w all wall/gran model hertz tangential history mesh n_meshes 1 meshes cad\n", "unknown mesh cad"),
     ("fix w all wall/gran model hertz tangential history primitive type 1 cylinder 0.1 0 0\n", "cylinder is not available"),
     ("fix cad all mesh/surface file /nonexistent.stl type 1\n", "nonexistent"),
@@ -119,3 +122,28 @@ def test_mesh_commands_parse(tmp_path):
     assert rc != 0 and "used twice" in log
     rc, log = run_script(tmp_path, good.replace("move/mesh mesh cad", "move/mesh mesh nope"), "in.mesh3")
     assert rc != 0 and "unknown mesh nope" in log
+
+
+def test_sibling_model_commands_parse(tmp_path):
+    """model hooke / hertz/stiffness / hooke/stiffness, tangential no_history,
+    cohesion sjkr and their on/off settings, with the properties they read,
+    parse; the wall fixes may choose their own model."""
+    params, mats, P, walls = wl.settling_4k(seed=0, n=50)
+    data = tmp_path / "d.data"
+    wl.write_data_file(str(data), params, P)
+    props = ("fix m6 all property/global characteristicVelocity scalar 2.0\n"
+             "fix m7 all property/global cohesionEnergyDensity peratomtypepair 1 300000\n"
+             "fix m8 all property/global kn peratomtypepair 1 2e6\n"
+             "fix m9 all property/global kt peratomtypepair 1 2e6\n"
+             "fix m10 all property/global gamman peratomtypepair 1 1e4\n"
+             "fix m11 all property/global gammat peratomtypepair 1 1e4\n")
+    for pair, wall in (("hooke tangential no_history cohesion sjkr tangential_reduce on ktToKnUser on",
+                        "hertz/stiffness tangential history"),
+                       ("hooke/stiffness tangential history absolute_damping off", "hooke tangential no_history")):
+        text = (HEADER.replace("pair_style gran model hertz tangential history rolling_friction cdt",
+                               f"pair_style gran model {pair} rolling_friction cdt")
+                .replace("wall/gran model hertz tangential history rolling_friction cdt",
+                         f"wall/gran model {wall}"))
+        rc, log = run_script(tmp_path, text.replace("pair_coeff", props + "pair_coeff", 1) + f"read_data {data}\n")
+        assert rc == 0, log
+        assert "WARNING: property/global" not in log
