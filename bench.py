@@ -56,7 +56,7 @@ HBM_PEAK_GBS = 8000.0
 TRI_RECORD_BYTES = 312  # one per-triangle geometry record (TriGeom)
 
 
-def algorithmic_bytes(n, npairs, ncontacts, nwall, mixed, ntri=0, ncand=0):
+def algorithmic_bytes(n, npairs, ncontacts, nwall, mixed, ntri=0, ncand=0, history=True):
     """Unique HBM bytes one launch of the fused step kernel must move:
     per particle  read x,y,z,r + v,m (64 B FP64) + omega,type (32 B FP64 /
                   16 B mixed) + xhold (24 B) + list length (4 B);
@@ -67,8 +67,10 @@ def algorithmic_bytes(n, npairs, ncontacts, nwall, mixed, ntri=0, ncand=0):
     with meshes   per particle its candidate count (4 B), per candidate one
                   4-byte triangle index, per triangle its geometry record once,
                   per touching mesh contact its 4-byte slot id (counted in nwall
-                  for the history)."""
-    sh = 12 if mixed else 24
+                  for the history).
+    history=False: a contact model without tangential history (no_history)
+    moves no history bytes."""
+    sh = (12 if mixed else 24) if history else 0
     per_particle = 64 + (16 if mixed else 32) + 24 + 4 + 48 + (12 if mixed else 24)
     B = n * per_particle + 4 * npairs + 2 * sh * (ncontacts + nwall)
     if ntri:
@@ -120,7 +122,7 @@ def run_mode(params, mats, P, walls, precision, steps, warmup, timing_steps, mes
     mixed = precision == pydem.DEM_MIXED
     ntri = sum(len(m[0]) for m in meshes)
     B = algorithmic_bytes(n, st["npairs"], st["ncontacts"], st["nwall_contacts"], mixed,
-                          ntri, st["nmesh_candidates"])
+                          ntri, st["nmesh_candidates"], history=params.get("tangential") != "no_history")
     kern_s = st["step_kernel_ms"] * 1e-3
     return {
         "throughput": n * steps / dt,
@@ -397,6 +399,24 @@ def main_copy(args):
                       "launches": reps + 3}), flush=True)
 
 
+# pair_style gran variants for the settling and packed workloads:
+# (model switches, extra material properties, description)
+CONTACT_MODELS = {
+    "hertz": ({}, {}, "Hertz/history"),
+    "hooke-nohistory-sjkr": ({"normal": "hooke", "tangential": "no_history", "cohesion": "sjkr"},
+                             {"characteristic_velocity": 1.0, "cohesion_energy_density": [[3.0e4]]},
+                             "hooke + tangential no_history + sjkr cohesion (30 kJ/m^3)"),
+    "hertz-stiffness": ({"normal": "hertz/stiffness"},
+                        {"kn": [[4.0e6]], "kt": [[5.0e6]], "gamman": [[2.0e5]], "gammat": [[1.0e5]]},
+                        "hertz/stiffness/history (kn 4e6, kt 5e6, gamman 2e5, gammat 1e5)"),
+}
+
+
+def with_contact_model(name, params, mats):
+    model, props, desc = CONTACT_MODELS[name]
+    return dict(params, **model), dict(mats, **props), desc
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -423,6 +443,8 @@ def main():
     ap.add_argument("--sq-cpu-n", type=int, default=2000)
     ap.add_argument("--sq-cpu-steps", type=int, default=3000)
     ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--contact-model", choices=sorted(CONTACT_MODELS), default="hertz",
+                    help="settling / packed: the pair and wall contact model")
     ap.add_argument("--sample", type=int, default=8,
                     help="sweep: instances rerun alone in FP64 for the error report")
     args = ap.parse_args()
@@ -456,8 +478,9 @@ def main():
         modes = args.modes or "mixed"
     elif args.workload == "packed":
         params, mats, P, walls = wl.packed_1m(seed=0, n=args.n)
+        params, mats, law = with_contact_model(args.contact_model, params, mats)
         n = P["x"].shape[0]
-        workload = (f"{n}-sphere polydisperse Hertz/history + CDT rolling, compressed lattice "
+        workload = (f"{n}-sphere polydisperse {law} + CDT rolling, compressed lattice "
                     "(about six touching neighbors per sphere), periodic xy, zplane floor, "
                     "dt 5e-6 s, skin 1 mm")
         data = f"synthetic (seeded compressed lattice, {n} polydisperse spheres)"
@@ -472,8 +495,9 @@ def main():
         modes = args.modes or "mixed,fp64"
     else:
         params, mats, P, walls = wl.settling_1m(seed=0, n=args.n)
+        params, mats, law = with_contact_model(args.contact_model, params, mats)
         n = args.n
-        workload = (f"{n}-sphere polydisperse Hertz/history + CDT rolling settling, "
+        workload = (f"{n}-sphere polydisperse {law} + CDT rolling settling, "
                     "box 0.3x0.3x0.6 m periodic xy, zplane floor, dt 5e-6 s, skin 1 mm")
         data = "synthetic (seeded jittered lattice, 1M polydisperse spheres)"
         modes = args.modes or "fp64,mixed"
@@ -539,7 +563,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": (head["achieved_gbs"] / HBM_PEAK_GBS) if head["achieved_gbs"] else None,
-        "traffic": measured_traffic("1M_settling", head_mode) if args.workload == "settling" else None,
+        "traffic": (measured_traffic("1M_settling", head_mode)
+                    if args.workload == "settling" and args.contact_model == "hertz" else None),
         "bytes_per_launch": head["bytes_per_launch"],
         "kernel_ms": head["kernel_ms"],
     }

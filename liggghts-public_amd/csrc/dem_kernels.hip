@@ -667,9 +667,12 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
       c.acont = (T)(-kPi / 4 * ((r - ri - rj) * (r + ri - rj) * (r - ri + rj) * (r + ri + rj)) / (r * r));
     }
     const bool flagged = (e & kFlagBit) != 0;
+    // tangential no_history keeps no history: its slots hold the zeros
+    // written at the first touch and are neither read nor rewritten
+    const bool keeps = !(GEN && pm.nohist);
     T sh[3];
     const size_t hb = (size_t)k * 3 * cap + i;
-    if (flagged) {
+    if (flagged && keeps) {
       sh[0] = hist[hb];
       sh[1] = hist[hb + cap];
       sh[2] = hist[hb + 2 * cap];
@@ -678,7 +681,7 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
     }
     T dF[3], dTi[3], dTj[3];
     contact_chain<T, false, GEN>(pm, c, dt, a.shearupdate, sh, dF, dTi, dTj);
-    if (a.shearupdate || !flagged) {
+    if ((a.shearupdate && keeps) || !flagged) {
       hist[hb] = sh[0];
       hist[hb + cap] = sh[1];
       hist[hb + 2 * cap] = sh[2];
@@ -771,8 +774,9 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
           load_model_coeffs(a.mat, p, wm, mt, c);
           c.acont = (T)((pi.w * pi.w - r * r) * kPi);  // cohesion_model_sjkr.h:95-96
         }
+        const bool keeps = !(GEN && wm.nohist);  // as for pairs: no_history slots stay zero
         T sh[3];
-        if (meta & bit) {
+        if ((meta & bit) && keeps) {
           sh[0] = wh[wb];
           sh[1] = wh[wb + cap];
           sh[2] = wh[wb + 2 * cap];
@@ -781,9 +785,11 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
         }
         T dF[3], dTi[3], dTj[3];
         contact_chain<T, true, GEN>(wm, c, dt, a.shearupdate, sh, dF, dTi, dTj);
-        wh[wb] = sh[0];
-        wh[wb + cap] = sh[1];
-        wh[wb + 2 * cap] = sh[2];
+        if (keeps || !(meta & bit)) {
+          wh[wb] = sh[0];
+          wh[wb + cap] = sh[1];
+          wh[wb + 2 * cap] = sh[2];
+        }
         meta |= bit;
         fx += dF[0];
         fy += dF[1];
