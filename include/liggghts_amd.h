@@ -188,6 +188,19 @@ int dem_add_mesh(dem_handle *h, int64_t ntri, const double *nodes, int wall_type
 int dem_mesh_rotate(dem_handle *h, int mesh_id, const double *origin, const double *axis,
                     double period);
 
+/* fix move/mesh mesh <id> linear vx vy vz (MeshMoverLinear,
+ * mesh_mover_linear.cpp:62-101): each step the mesh is shifted by v dt
+ * (MultiNodeMesh::move, multi_node_mesh_I.h:502-527) and its nodes carry the
+ * velocity v. One fix move/mesh per mesh. Call before dem_set_particles. */
+int dem_mesh_linear(dem_handle *h, int mesh_id, const double *velocity);
+
+/* fix move/mesh mesh <id> wiggle amplitude ax ay az period T (MeshMoverWiggle,
+ * mesh_mover_linear.cpp:160-218): with omega = 2 pi / T and the fix's time t
+ * (t += dt at every step over the fix's lifetime, fix_move_mesh.cpp:226) the
+ * nodes move with velocity A omega cos(omega t) and are shifted by that
+ * velocity times dt. One fix move/mesh per mesh. Call before dem_set_particles. */
+int dem_mesh_wiggle(dem_handle *h, int mesh_id, const double *amplitude, double period);
+
 /* Current mesh nodes and node velocities (ntri*9 each; either may be NULL). */
 int dem_get_mesh_nodes(dem_handle *h, int mesh_id, double *nodes, double *vnodes);
 

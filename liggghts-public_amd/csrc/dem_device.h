@@ -147,16 +147,24 @@ struct ListBuf {
 };
 
 // Rigid rotation of the moving meshes for one step.
+// fix move/mesh movement types
+constexpr int kMoverRotate = 1, kMoverLinear = 2, kMoverWiggle = 3;
+
 struct MeshMoveArgs {
   TriGeom *tri;               // geometry of the step before
   TriGeom *dst;               // where the moved geometry goes (nullptr: in place)
   const double *node_last;
   int ntri;
-  int moving_mask;            // bit m: mesh m rotates
-  double dQ[kMaxMeshes][4];   // incremental rotation quaternion
+  int moving_mask;            // bit m: mesh m moves
+  int kind[kMaxMeshes];       // kMover* of mesh m
+  double dQ[kMaxMeshes][4];   // rotate: incremental rotation quaternion
   double origin[kMaxMeshes][3];
   double omegaVec[kMaxMeshes][3];
   int trans[kMaxMeshes];
+  double tvec[kMaxMeshes][3];  // linear: velocity; wiggle: amplitude
+  double womega[kMaxMeshes];   // wiggle: 2 pi / period
+  const double *time_tab;      // wiggle: the mover's time at run step k is time_tab[k]
+  double dt;
   double triggersq;
   long long step;             // the step the moved geometry belongs to
   int guard;                  // return if step >= stop_at

@@ -490,13 +490,61 @@ __device__ inline void mesh_contacts(const StepArgs &a, int i, const double4 &pi
 // (multi_node_mesh_I.h:620-660) and MultiNodeMesh::decideRebuild
 // (multi_node_mesh_I.h:792-825)
 // ---------------------------------------------------------------------------
-// moves triangle t of a moving mesh one step; returns whether one of its
-// nodes is now more than skin/2 from where it was at the last rebuild
-__device__ __forceinline__ bool mesh_move_tri(const MeshMoveArgs &a, const int t) {
+// translating movers (MeshMoverLinear / MeshMoverWiggle::initial_integrate,
+// mesh_mover_linear.cpp:88-101, 203-218): nodes and centre shifted by
+// v dt (TrackingMesh::move -> MultiNodeMesh::move, multi_node_mesh_I.h:
+// 502-527; edge vectors and normals are translation invariant), node
+// velocities reset to v (FixMoveMesh::initial_integrate zeroes them first)
+__device__ __forceinline__ void mesh_translate_tri(const MeshMoveArgs &a, const TriGeom &src,
+                                                   TriGeom &dst, const int m, const long long step) {
+  double vN[3];
+  if (a.kind[m] == kMoverWiggle) {
+    const double T = a.time_tab[step];
+    const double sc = a.womega[m] * cos(a.womega[m] * T);
+    for (int k = 0; k < 3; k++) vN[k] = a.tvec[m][k] * sc;
+  } else {
+    for (int k = 0; k < 3; k++) vN[k] = a.tvec[m][k];
+  }
+  double dx[3];
+  for (int k = 0; k < 3; k++) dx[k] = vN[k] * a.dt;
+  if (a.dst) {  // fields a translation leaves unchanged
+    for (int k = 0; k < 9; k++) dst.ev[k] = src.ev[k];
+    for (int k = 0; k < 3; k++) dst.sn[k] = src.sn[k];
+    for (int k = 0; k < 3; k++) dst.el[k] = src.el[k];
+    dst.rb = src.rb;
+    dst.obtuse = src.obtuse;
+    dst.flags = src.flags;
+    dst.mesh = src.mesh;
+    dst.pad = src.pad;
+  }
+  for (int j = 0; j < 3; j++)
+    for (int k = 0; k < 3; k++) dst.n[3 * j + k] = src.n[3 * j + k] + dx[k];
+  for (int k = 0; k < 3; k++) dst.c[k] = src.c[k] + dx[k];
+  for (int j = 0; j < 3; j++)
+    for (int k = 0; k < 3; k++) dst.vn[3 * j + k] = 0.0 + vN[k];
+}
+
+// moves triangle t of a moving mesh one step (to its geometry of run step
+// `step`); returns whether one of its nodes is now more than skin/2 from where
+// it was at the last rebuild
+__device__ __forceinline__ bool mesh_move_tri(const MeshMoveArgs &a, const int t,
+                                              const long long step) {
   const TriGeom &src = a.tri[t];
   const int m = src.mesh;
   if (!((a.moving_mask >> m) & 1)) return false;
   TriGeom &dst = a.dst ? a.dst[t] : a.tri[t];
+  if (a.kind[m] != kMoverRotate) {
+    mesh_translate_tri(a, src, dst, m, step);
+    bool moved = false;
+    if (a.decide)
+      for (int j = 0; j < 3; j++) {
+        const double dx = dst.n[3 * j] - a.node_last[9 * t + 3 * j];
+        const double dy = dst.n[3 * j + 1] - a.node_last[9 * t + 3 * j + 1];
+        const double dz = dst.n[3 * j + 2] - a.node_last[9 * t + 3 * j + 2];
+        if (dx * dx + dy * dy + dz * dz > a.triggersq) moved = true;
+      }
+    return moved;
+  }
   const double *dQ = a.dQ[m];
   const double *o = a.origin[m];
   double c[3] = {0., 0., 0.};
@@ -552,7 +600,7 @@ __device__ __forceinline__ bool mesh_move_tri(const MeshMoveArgs &a, const int t
 // step: the step the moved geometry belongs to
 __device__ __forceinline__ void mesh_move_block(const MeshMoveArgs &a, const int t,
                                                 const long long step) {
-  const bool moved = t < a.ntri && mesh_move_tri(a, t);
+  const bool moved = t < a.ntri && mesh_move_tri(a, t, step);
   if (__ballot(moved) != 0ull && (threadIdx.x & 63) == 0)
     atomicMin((unsigned long long *)&a.ctrl->stop_at, (unsigned long long)(step + 1));
 }

@@ -96,6 +96,11 @@ struct dem_handle {
   int tri_cur = 0;
   bool pending_mesh_rebuild = false;  // a moving mesh asked for a rebuild at the next step
   double *d_node_last = nullptr;
+  // wiggle movers: the FixMoveMesh time (time_ += dt each step, over the
+  // fix's lifetime) at every step of the current run, and its value so far
+  double *d_mesh_time = nullptr;
+  int64_t mesh_time_cap = 0;
+  double mesh_time = 0.0;
   int *d_copl_off = nullptr, *d_copl_idx = nullptr;
   int *d_tcount = nullptr, *d_toff = nullptr;
   int *d_tpair_cell = nullptr, *d_tpair_tri = nullptr, *d_tcell_list = nullptr;
@@ -212,6 +217,8 @@ void free_particles(dem_handle *h) {
   h->xhold = h->fout = nullptr;
   h->d_tri_buf[0] = h->d_tri_buf[1] = nullptr;
   h->d_node_last = nullptr;
+  h->d_mesh_time = nullptr;
+  h->mesh_time_cap = 0;
   h->d_copl_off = h->d_copl_idx = nullptr;
   h->d_tcount = h->d_toff = h->d_tpair_cell = h->d_tpair_tri = h->d_tcell_list = nullptr;
   h->tpair_cap = 0;
@@ -433,6 +440,29 @@ bool any_moving_mesh(const dem_handle *h) {
   return false;
 }
 
+// the wiggle movers' time at run steps 0..nsteps, accumulated as
+// FixMoveMesh::initial_integrate does (time_ += dt, fix_move_mesh.cpp:226)
+int upload_mesh_time(dem_handle *h, int64_t nsteps, std::vector<double> &tab) {
+  tab.assign((size_t)nsteps + 1, h->mesh_time);
+  for (int64_t k = 1; k <= nsteps; k++) tab[k] = tab[k - 1] + h->p.dt;
+  if (nsteps + 1 > h->mesh_time_cap) {
+    clear_graphs(h);  // captured launches hold the old table pointer
+    if (h->d_mesh_time) {
+      hipFree(h->d_mesh_time);
+      h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), (void *)h->d_mesh_time),
+                      h->allocs.end());
+      h->d_mesh_time = nullptr;
+    }
+    const int64_t cap = std::max<int64_t>(nsteps + 1, 2 * h->mesh_time_cap);
+    int rc = dalloc(h, (void **)&h->d_mesh_time, sizeof(double) * (size_t)cap);
+    if (rc) return rc;
+    h->mesh_time_cap = cap;
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));  // the previous run's launches are done with it
+  HIPCHK(h, hipMemcpy(h->d_mesh_time, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+  return DEM_OK;
+}
+
 // one step of fix move/mesh rotate for every moving mesh
 dem::MeshMoveArgs mesh_move_args(const dem_handle *h, long long step, int guard, int decide) {
   dem::MeshMoveArgs a{};
@@ -445,10 +475,18 @@ dem::MeshMoveArgs mesh_move_args(const dem_handle *h, long long step, int guard,
   a.guard = guard;
   a.decide = decide;
   a.ctrl = h->ctrl;
+  a.time_tab = h->d_mesh_time;
+  a.dt = h->p.dt;
   for (size_t m = 0; m < h->meshes.size(); m++) {
     const dem::HostMesh &hm = h->meshes[m];
     if (!hm.moving) continue;
     a.moving_mask |= 1 << m;
+    a.kind[m] = hm.mover;
+    if (hm.mover != dem::kMoverRotate) {
+      for (int k = 0; k < 3; k++) a.tvec[m][k] = hm.mover == dem::kMoverLinear ? hm.vel[k] : hm.ampl[k];
+      a.womega[m] = hm.womega;
+      continue;
+    }
     // MultiNodeMesh::rotate(dAngle, axis, p), multi_node_mesh_I.h:620-641
     const double dAngle = hm.omega * h->p.dt;
     double axisNorm[3] = {hm.axis[0], hm.axis[1], hm.axis[2]};
@@ -931,6 +969,8 @@ int dem_mesh_rotate(dem_handle *h, int mesh_id, const double *origin, const doub
     return fail(h, DEM_ERR_ARG, "bad move/mesh rotate arguments");
   if (h->n > 0) return fail(h, DEM_ERR_STATE, "set mesh motion before particles");
   dem::HostMesh &m = h->meshes[mesh_id];
+  if (m.moving && m.mover != dem::kMoverRotate)
+    return fail(h, DEM_ERR_ARG, "one fix move/mesh per mesh on this path");
   // MeshMoverRotate: axis normalized, omega = 2 pi / period (mesh_mover_rotation.cpp:66-93)
   const double norm = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
   if (norm == 0.) return fail(h, DEM_ERR_ARG, "rotation axis is zero");
@@ -941,6 +981,35 @@ int dem_mesh_rotate(dem_handle *h, int mesh_id, const double *origin, const doub
   }
   m.omega = 2. * kMyPi / period;
   m.moving = true;
+  m.mover = dem::kMoverRotate;
+  return DEM_OK;
+}
+
+int dem_mesh_linear(dem_handle *h, int mesh_id, const double *velocity) {
+  if (!h) return DEM_ERR_ARG;
+  if (mesh_id < 0 || mesh_id >= (int)h->meshes.size() || !velocity)
+    return fail(h, DEM_ERR_ARG, "bad move/mesh linear arguments");
+  if (h->n > 0) return fail(h, DEM_ERR_STATE, "set mesh motion before particles");
+  dem::HostMesh &m = h->meshes[mesh_id];
+  if (m.moving) return fail(h, DEM_ERR_ARG, "one fix move/mesh per mesh on this path");
+  for (int k = 0; k < 3; k++) m.vel[k] = velocity[k];  // MeshMoverLinear (mesh_mover_linear.cpp:62-75)
+  m.moving = true;
+  m.mover = dem::kMoverLinear;
+  return DEM_OK;
+}
+
+int dem_mesh_wiggle(dem_handle *h, int mesh_id, const double *amplitude, double period) {
+  if (!h) return DEM_ERR_ARG;
+  if (mesh_id < 0 || mesh_id >= (int)h->meshes.size() || !amplitude || !(period != 0.0))
+    return fail(h, DEM_ERR_ARG, "bad move/mesh wiggle arguments");
+  if (h->n > 0) return fail(h, DEM_ERR_STATE, "set mesh motion before particles");
+  dem::HostMesh &m = h->meshes[mesh_id];
+  if (m.moving) return fail(h, DEM_ERR_ARG, "one fix move/mesh per mesh on this path");
+  // MeshMoverWiggle (mesh_mover_linear.cpp:160-183): omega = 2 pi / period
+  for (int k = 0; k < 3; k++) m.ampl[k] = amplitude[k];
+  m.womega = 2. * kMyPi / period;
+  m.moving = true;
+  m.mover = dem::kMoverWiggle;
   return DEM_OK;
 }
 
@@ -1291,6 +1360,10 @@ int run_steps(dem_handle *h, int64_t nsteps) {
   h->pending_mesh_rebuild = false;
   const bool moving = any_moving_mesh(h);
   const int64_t ts0 = h->ntimestep;  // step t of this run is timestep ts0 + t
+  bool wiggle = false;
+  for (const auto &m : h->meshes) wiggle = wiggle || m.mover == dem::kMoverWiggle;
+  std::vector<double> time_tab;
+  if (wiggle && (rc = upload_mesh_time(h, nsteps, time_tab))) return rc;
   {
     dem::StepArgs a = step_args(h, h->cur_st, h->tri_cur);
     a.ago_next = ts0 + 1 - h->last_build;
@@ -1391,6 +1464,7 @@ int run_steps(dem_handle *h, int64_t nsteps) {
     }
   }
   h->tri_cur = tc;
+  if (wiggle) h->mesh_time = time_tab[(size_t)nsteps];
   h->pending_mesh_rebuild = h->ctrl_host->stop_at == nsteps + 1;
   h->ncontacts = (int64_t)(h->ctrl_host->ncontact_full / 2);
   h->nwall_contacts = (int64_t)h->ctrl_host->nwall;

@@ -28,7 +28,10 @@ struct HostMesh {
   int wall_type = 1;
   int model = 0;  // contact-model bits
   bool moving = false;
-  double origin[3] = {0, 0, 0}, axis[3] = {0, 0, 1}, omega = 0.0;
+  int mover = 0;  // dem::kMover* of its fix move/mesh (0: none)
+  double origin[3] = {0, 0, 0}, axis[3] = {0, 0, 1}, omega = 0.0;  // rotate
+  double vel[3] = {0, 0, 0};                                       // linear: velocity
+  double ampl[3] = {0, 0, 0}, womega = 0.0;                        // wiggle
 };
 
 // Builds a mesh from ntri*9 node coordinates: drops elements with coinciding

@@ -27,6 +27,7 @@
 //   fix <id> <grp> wall/gran model <normal> tangential <tangential> [...] mesh n_meshes <k>
 //                  meshes <id1> ... <idk>
 //   fix <id> <grp> move/mesh mesh <meshid> rotate origin ox oy oz axis ax ay az period T
+//                  | linear vx vy vz | wiggle amplitude ax ay az period T
 //   fix <id> <grp> nve/sphere
 //   atom_style superquadric  (read_data Atoms: id type density shapex shapey shapez
 //                             blockiness1 blockiness2 x y z), with
@@ -158,8 +159,9 @@ struct Interp {
     int type = 1;
     bool used = false;
     dem_contact_model model{};
-    bool rotating = false;
     double origin[3] = {0, 0, 0}, axis[3] = {0, 0, 1}, period = 1.0;
+    int mover = 0;  // 0 none, 1 rotate, 2 linear, 3 wiggle
+    double vec[3] = {0, 0, 0};  // linear: velocity; wiggle: amplitude
   };
   std::map<std::string, MeshDef> mesh_defs;
   std::vector<std::string> mesh_order;  // meshes in fix wall/gran order
@@ -726,8 +728,9 @@ struct Interp {
       int id = -1;
       if (dem_add_mesh(h, (int64_t)(md.nodes.size() / 9), md.nodes.data(), md.type, &md.model, &id))
         die(dem_last_error(h));
-      if (md.rotating && dem_mesh_rotate(h, id, md.origin, md.axis, md.period))
-        die(dem_last_error(h));
+      if (md.mover == 1 && dem_mesh_rotate(h, id, md.origin, md.axis, md.period)) die(dem_last_error(h));
+      if (md.mover == 2 && dem_mesh_linear(h, id, md.vec)) die(dem_last_error(h));
+      if (md.mover == 3 && dem_mesh_wiggle(h, id, md.vec, md.period)) die(dem_last_error(h));
     }
     if (tag.empty()) die("run: no atoms");
     if (dem_set_particles(h, (int64_t)tag.size(), x.data(), v.data(), w.data(), r.data(), m.data(),
@@ -946,17 +949,37 @@ struct Interp {
         fix_mesh_surface(t);
       } else if (style == "move/mesh") {
         // fix id group move/mesh mesh <meshid> rotate origin x y z axis x y z period T
-        if (t.size() < 17 || t[4] != "mesh" || t[6] != "rotate" || t[7] != "origin" ||
-            t[11] != "axis" || t[15] != "period")
-          die("fix move/mesh: only 'mesh <id> rotate origin x y z axis x y z period T' is available");
+        //                                      | linear vx vy vz
+        //                                      | wiggle amplitude ax ay az period T
+        // (FixMoveMesh, fix_move_mesh.cpp:70-120; the movers' argument lists)
+        if (t.size() < 7 || t[4] != "mesh") die("fix move/mesh: expected 'mesh <id> <movement>'");
         auto it = mesh_defs.find(t[5]);
         if (it == mesh_defs.end()) die("fix move/mesh: unknown mesh " + t[5]);
-        for (int d = 0; d < 3; d++) {
-          it->second.origin[d] = num(t[8 + d]);
-          it->second.axis[d] = num(t[12 + d]);
+        MeshDef &md = it->second;
+        if (md.mover) die("fix move/mesh: one fix move/mesh per mesh on this path");
+        if (t[6] == "rotate") {
+          if (t.size() < 17 || t[7] != "origin" || t[11] != "axis" || t[15] != "period")
+            die("fix move/mesh rotate: expected 'origin x y z axis x y z period T'");
+          for (int d = 0; d < 3; d++) {
+            md.origin[d] = num(t[8 + d]);
+            md.axis[d] = num(t[12 + d]);
+          }
+          md.period = num(t[16]);
+          md.mover = 1;
+        } else if (t[6] == "linear") {
+          if (t.size() < 10) die("Not enough arguments for movement type linear");
+          for (int d = 0; d < 3; d++) md.vec[d] = num(t[7 + d]);
+          md.mover = 2;
+        } else if (t[6] == "wiggle") {
+          if (t.size() < 13) die("Not enough arguments for movement type wiggle");
+          if (t[7] != "amplitude") die("Expected keyword 'amplitude'");
+          if (t[11] != "period") die("Expected keyword 'period'");
+          for (int d = 0; d < 3; d++) md.vec[d] = num(t[8 + d]);
+          md.period = num(t[12]);
+          md.mover = 3;
+        } else {
+          die("fix move/mesh: movement " + t[6] + " is not available on this path (rotate, linear, wiggle)");
         }
-        it->second.period = num(t[16]);
-        it->second.rotating = true;
         dirty = true;
       } else if (style == "wall/gran" && t.size() > 4 &&
                  std::find(t.begin(), t.end(), std::string("mesh")) != t.end() &&

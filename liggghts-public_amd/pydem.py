@@ -22,7 +22,7 @@ EXPORTED = [
     "dem_set_particles", "dem_add_plane_wall", "dem_setup", "dem_run",
     "dem_run_continue", "dem_compute_forces", "dem_get_state", "dem_get_pairs",
     "dem_get_wall_history", "dem_get_stats", "dem_set_kernel_timing",
-    "dem_run_script", "dem_add_mesh", "dem_mesh_rotate", "dem_get_mesh_nodes",
+    "dem_run_script", "dem_add_mesh", "dem_mesh_rotate", "dem_get_mesh_nodes", "dem_mesh_linear", "dem_mesh_wiggle",
     "dem_get_mesh_topology", "dem_get_mesh_history", "dem_set_instances",
     "dem_set_instance_materials", "dem_get_instance_stats", "dem_set_step_shape",
     "dem_get_step_shape", "dem_set_graph_capture", "dem_set_pair_model", "dem_set_model_coefficients",
@@ -172,6 +172,8 @@ def load_library(path=LIB_PATH):
     lib.dem_add_mesh.argtypes = [h, ctypes.c_int64, dp, ctypes.c_int, P(DemContactModel),
                                  P(ctypes.c_int)]
     lib.dem_mesh_rotate.argtypes = [h, ctypes.c_int, dp, dp, ctypes.c_double]
+    lib.dem_mesh_linear.argtypes = [h, ctypes.c_int, dp]
+    lib.dem_mesh_wiggle.argtypes = [h, ctypes.c_int, dp, ctypes.c_double]
     lib.dem_get_mesh_nodes.argtypes = [h, ctypes.c_int, dp, dp]
     lib.dem_get_mesh_topology.argtypes = [h, ctypes.c_int, ip, ip, ip]
     lib.dem_get_mesh_history.argtypes = [h, ctypes.c_int, ctypes.c_int, ip, dp]
@@ -267,7 +269,8 @@ class GpuSystem:
     def __init__(self, params, materials, particles, walls=(), precision=DEM_FP64,
                  max_neighbors=0, device=0, meshes=(), instances=None, instance_materials=None):
         """meshes: [(tris (T,3,3), wall_type, model dict or None, rotate or None)],
-        rotate = (origin, axis, period) for fix move/mesh rotate.
+        rotate = (origin, axis, period) for fix move/mesh rotate, or
+        {"linear": (vx, vy, vz)} or {"wiggle": ((ax, ay, az), period)}.
         instances: particle counts of independent simulations batched in this
         system (particles concatenated in that order); instance_materials: one
         material dict per instance."""
@@ -328,7 +331,13 @@ class GpuSystem:
             mid = ctypes.c_int(-1)
             self._check(self.lib.dem_add_mesh(self.h, len(t), _dp(t), int(wtype), ctypes.byref(m),
                                               ctypes.byref(mid)))
-            if rot is not None:
+            if isinstance(rot, dict) and "linear" in rot:
+                v = np.ascontiguousarray(rot["linear"], dtype=np.float64)
+                self._check(self.lib.dem_mesh_linear(self.h, mid.value, _dp(v)))
+            elif isinstance(rot, dict) and "wiggle" in rot:
+                a = np.ascontiguousarray(rot["wiggle"][0], dtype=np.float64)
+                self._check(self.lib.dem_mesh_wiggle(self.h, mid.value, _dp(a), float(rot["wiggle"][1])))
+            elif rot is not None:
                 o = np.ascontiguousarray(rot[0], dtype=np.float64)
                 a = np.ascontiguousarray(rot[1], dtype=np.float64)
                 self._check(self.lib.dem_mesh_rotate(self.h, mid.value, _dp(o), _dp(a),

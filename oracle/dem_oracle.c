@@ -440,8 +440,11 @@ typedef struct {
   int *nneigh, *neigh;
   int wall_type;
   orc_model model;
-  int moving;
-  double origin[3], axis[3], omega;
+  int moving; /* 1 rotate, 2 linear, 3 wiggle (its fix move/mesh) */
+  double origin[3], axis[3], omega; /* rotate */
+  double vel[3];                     /* linear */
+  double ampl[3], womega;            /* wiggle */
+  double time;                       /* FixMoveMesh::time_ */
   double curvature, precision;
   /* per particle */
   int *ncand, *cand;
@@ -1354,6 +1357,21 @@ void orc_mesh_rotate(orc_sys *s, int imesh, const double *origin, const double *
   m->omega = 2. * MY_PI / period; /* M_PI in the reference: same constant */
 }
 
+/* MeshMoverLinear, mesh_mover_linear.cpp:62-75 */
+void orc_mesh_linear(orc_sys *s, int imesh, const double *vel) {
+  orc_mesh *m = &s->mesh[imesh];
+  m->moving = 2;
+  memcpy(m->vel, vel, 3 * sizeof(double));
+}
+
+/* MeshMoverWiggle, mesh_mover_linear.cpp:160-183 */
+void orc_mesh_wiggle(orc_sys *s, int imesh, const double *amplitude, double period) {
+  orc_mesh *m = &s->mesh[imesh];
+  m->moving = 3;
+  memcpy(m->ampl, amplitude, 3 * sizeof(double));
+  m->womega = 2. * MY_PI / period;
+}
+
 void orc_mesh_topology(const orc_sys *s, int imesh, int *edge_active, int *corner_active,
                        int *obtuse) {
   const orc_mesh *m = &s->mesh[imesh];
@@ -1686,11 +1704,35 @@ static void mesh_rebuild(orc_sys *s) {
   }
 }
 
-/* FixMoveMesh::initial_integrate -> MeshMoverRotate::initial_integrate */
+/* FixMoveMesh::initial_integrate (fix_move_mesh.cpp:221-238) -> the mover's
+ * initial_integrate: MeshMoverRotate (mesh_mover_rotation.cpp:110-139),
+ * MeshMoverLinear / MeshMoverWiggle (mesh_mover_linear.cpp:88-101, 203-218) */
 static void mesh_move(orc_sys *s) {
   for (int im = 0; im < s->nmesh; im++) {
     orc_mesh *m = &s->mesh[im];
     if (!m->moving) continue;
+    m->time += s->p.dt; /* time_ += update->dt */
+    if (m->moving != 1) {
+      /* translation: v_node reset to zero, vNode = vel_ or
+       * amplitude_ * (omega_ cos(omega_ time_)), dx = vNode dt; nodes and
+       * centre += dx (MultiNodeMesh::move, multi_node_mesh_I.h:502-527) */
+      double vN[3], dx[3];
+      if (m->moving == 3) {
+        const double sc = m->womega * cos(m->womega * m->time);
+        for (int k = 0; k < 3; k++) vN[k] = m->ampl[k] * sc;
+      } else {
+        for (int k = 0; k < 3; k++) vN[k] = m->vel[k];
+      }
+      for (int k = 0; k < 3; k++) dx[k] = vN[k] * s->p.dt;
+      for (int t = 0; t < m->ntri; t++) {
+        for (int j = 0; j < 3; j++)
+          for (int k = 0; k < 3; k++) m->node[9 * t + 3 * j + k] += dx[k];
+        for (int k = 0; k < 3; k++) m->center[3 * t + k] += dx[k];
+        for (int j = 0; j < 3; j++)
+          for (int k = 0; k < 3; k++) m->vnode[9 * t + 3 * j + k] = 0.0 + vN[k];
+      }
+      continue;
+    }
     const double dAngle = m->omega * s->p.dt;
     double axisNorm[3] = {m->axis[0], m->axis[1], m->axis[2]};
     v3_div(axisNorm, v3_mag(axisNorm));

@@ -93,6 +93,8 @@ def load():
     lib.orc_add_mesh.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, ctypes.c_int, P(OrcModel)]
     lib.orc_add_mesh.restype = ctypes.c_int
     lib.orc_mesh_rotate.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, _D]
+    lib.orc_mesh_linear.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
+    lib.orc_mesh_wiggle.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, _D]
     lib.orc_mesh_topology.argtypes = [ctypes.c_void_p, ctypes.c_int, ip, ip, ip]
     lib.orc_mesh_nodes.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
     lib.orc_mesh_history.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ip, dp]
@@ -244,8 +246,9 @@ class OracleSystem:
     """The CPU restatement of Verlet::setup / Verlet::run."""
 
     def __init__(self, params, mat, particles, walls=(), meshes=()):
-        """meshes: [(tris (T,3,3), wall_type, model dict or None, rotate or None)],
-        rotate = (origin, axis, period)."""
+        """meshes: [(tris (T,3,3), wall_type, model dict or None, motion or None)],
+        motion = (origin, axis, period) for fix move/mesh rotate, or
+        {"linear": (vx, vy, vz)} or {"wiggle": ((ax, ay, az), period)}."""
         self.lib = load()
         self.p = make_params(params, walls)
         self.m = materials(mat)
@@ -265,7 +268,13 @@ class OracleSystem:
             k = self.lib.orc_add_mesh(self.s, len(t), _dp(t), int(wtype), ctypes.byref(m))
             if k < 0:
                 raise RuntimeError("orc_add_mesh failed")
-            if rot is not None:
+            if isinstance(rot, dict) and "linear" in rot:
+                v = np.ascontiguousarray(rot["linear"], dtype=np.float64)
+                self.lib.orc_mesh_linear(self.s, k, _dp(v))
+            elif isinstance(rot, dict) and "wiggle" in rot:
+                a = np.ascontiguousarray(rot["wiggle"][0], dtype=np.float64)
+                self.lib.orc_mesh_wiggle(self.s, k, _dp(a), float(rot["wiggle"][1]))
+            elif rot is not None:
                 o = np.ascontiguousarray(rot[0], dtype=np.float64)
                 a = np.ascontiguousarray(rot[1], dtype=np.float64)
                 self.lib.orc_mesh_rotate(self.s, k, _dp(o), _dp(a), float(rot[2]))

@@ -364,10 +364,19 @@ def test_mesh_topology_matches_oracle(name):
 
 
 def _mesh_floor_pack(seed=41, n=2000, moving=False):
+    """moving: False, True (fix move/mesh rotate about the vertical axis),
+    "linear" (a floor moving up and sideways) or "wiggle" (a vertical shaker)"""
     params, mats, P, walls = wl.dense_pack(seed=seed, n=n, rolling=True, periodic=(0, 0, 0))
     lo, hi = params["boxlo"], params["boxhi"]
     floor = msh.grid(lo[0] - 0.002, hi[0] + 0.002, lo[1] - 0.002, hi[1] + 0.002, lo[2], 7, 6)
-    rot = ((0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.0), (0.0, 0.0, 1.0), 0.5) if moving else None
+    if moving == "linear":
+        rot = {"linear": (0.2, -0.1, 0.05)}
+    elif moving == "wiggle":
+        rot = {"wiggle": ((0.0001, 0.0, 0.0003), 0.02)}
+    elif moving:
+        rot = ((0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.0), (0.0, 0.0, 1.0), 0.5)
+    else:
+        rot = None
     mats = wl.materials_two(rolling_friction=0.1)
     return params, mats, P, [], [(floor, 2, {"rolling_cdt": 1}, rot)]
 
@@ -391,7 +400,7 @@ def _compare_mesh_history(g, o, tol):
     return int((ot >= 0).sum())
 
 
-@pytest.mark.parametrize("moving", [False, True])
+@pytest.mark.parametrize("moving", [False, True, "linear", "wiggle"])
 def test_mesh_floor_one_step_fp64(moving):
     say(f"[gpu] mesh floor one step moving={moving}")
     g, o = _build_mesh_case(_mesh_floor_pack(moving=moving))
@@ -421,6 +430,31 @@ def test_mesh_floor_many_steps_fp64():
     g.run(200)
     o.run(200)
     say(f"  rebuilds gpu={g.stats()['nrebuilds']} oracle={o.nrebuilds()}")
+    assert g.stats()["nrebuilds"] == o.nrebuilds() >= 3
+    gs, os_ = g.state(), o.state()
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert_close(k, gs[k], os_[k], 1e-8)
+    _compare_mesh_history(g, o, 1e-8)
+    g.close()
+
+
+@pytest.mark.parametrize("motion", ["linear", "wiggle"])
+def test_translating_floor_many_steps_fp64(motion):
+    """fix move/mesh linear / wiggle under a packed bed, 500 steps (two runs,
+    the second continuing the mover's time): node positions and velocities
+    bitwise equal to the oracle, the same rebuilds, state and mesh history
+    within 1e-8."""
+    say(f"[gpu] translating floor ({motion}) 500 steps")
+    g, o = _build_mesh_case(_mesh_floor_pack(seed=43, n=1500, moving=motion))
+    g.run(300)
+    o.run(300)
+    assert _compare_mesh_history(g, o, 1e-8) >= 5
+    g.run(200)
+    o.run(200)
+    gn, gv = g.mesh_nodes()
+    on, ov = o.mesh_nodes()
+    assert np.array_equal(gn, on) and np.array_equal(gv, ov)
+    say(f"  rebuilds gpu={g.stats()['nrebuilds']} oracle={o.nrebuilds()} forced={o.mesh_forced_rebuilds()}")
     assert g.stats()["nrebuilds"] == o.nrebuilds() >= 3
     gs, os_ = g.state(), o.state()
     for k in ("x", "v", "omega", "f", "torque"):
@@ -528,6 +562,68 @@ run 400
     g.close()
 
 
+def test_script_wiggling_floor_matches_api(tmp_path):
+    """A shaker written as an input script (fix mesh/surface floor from an STL
+    file, fix move/mesh ... wiggle amplitude ... period ..., two runs so the
+    mover's time runs on) gives the kinetic energy of the same system built
+    through the C ABI."""
+    say("[gpu] script wiggling floor")
+    params, mats, P, walls, meshes = _mesh_floor_pack(seed=45, n=1200, moving="wiggle")
+    data = tmp_path / "shake.data"
+    wl.write_data_file(str(data), params, P, ntypes=2)
+    stl = tmp_path / "floor.stl"
+    msh.write_stl(str(stl), meshes[0][0])
+    lo, hi = params["boxlo"], params["boxhi"]
+    amp, period = meshes[0][3]["wiggle"]
+    script = tmp_path / "in.shake"
+    script.write_text(f"""atom_style granular
+atom_modify map array
+boundary f f f
+newton off
+communicate single vel yes
+units si
+region reg block {lo[0] - 0.003!r} {hi[0] + 0.003!r} {lo[1] - 0.003!r} {hi[1] + 0.003!r} {lo[2] - 0.001!r} {hi[2]!r} units box
+create_box 2 reg
+neighbor {params['skin']!r} bin
+neigh_modify delay 0
+fix m1 all property/global youngsModulus peratomtype 5.e6 5.e6
+fix m2 all property/global poissonsRatio peratomtype 0.45 0.45
+fix m3 all property/global coefficientRestitution peratomtypepair 2 0.3 0.3 0.3 0.3
+fix m4 all property/global coefficientFriction peratomtypepair 2 0.5 0.4 0.4 0.4
+fix m5 all property/global coefficientRollingFriction peratomtypepair 2 0.1 0.1 0.1 0.1
+pair_style gran model hertz tangential history rolling_friction cdt
+pair_coeff * *
+timestep {params['dt']!r}
+fix gravi all gravity 9.81 vector 0.0 0.0 -1.0
+fix cad all mesh/surface file {stl} type 2
+fix walls all wall/gran model hertz tangential history rolling_friction cdt mesh n_meshes 1 meshes cad
+fix shake all move/mesh mesh cad wiggle amplitude {amp[0]!r} {amp[1]!r} {amp[2]!r} period {period!r}
+read_data {data}
+fix integr all nve/sphere
+thermo_style custom step atoms ke
+thermo 150
+run 150
+run 150
+""")
+    log = tmp_path / "log.shake"
+    rc = pydem.run_script(str(script), str(log))
+    text = log.read_text()
+    say(text[-600:])
+    assert rc == 0
+    rows = [l.split() for l in text.splitlines() if l.strip() and l.split()[0].isdigit()]
+    last = rows[-1]
+    assert int(last[0]) == 300 and int(last[1]) == len(P["radius"])
+    params = dict(params, boxlo=(lo[0] - 0.003, lo[1] - 0.003, lo[2] - 0.001),
+                  boxhi=(hi[0] + 0.003, hi[1] + 0.003, hi[2]))
+    g = pydem.GpuSystem(params, mats, P, walls, meshes=meshes)
+    g.run(150)
+    g.run(150)
+    st = g.state()
+    ke = 0.5 * (P["rmass"] @ (st["v"] ** 2).sum(1))
+    assert float(last[2]) == pytest.approx(ke, rel=1e-6)
+    g.close()
+
+
 def test_rotating_drum_run_in_pieces():
     """`run N pre no` in uneven pieces on a rotating drum: mesh-forced rebuilds
     that fall on a piece boundary carry over to the next run, so the pieces
@@ -559,7 +655,7 @@ def test_rotating_drum_run_in_pieces():
     g2.close()
 
 
-@pytest.mark.parametrize("case", ["pack", "drum"])
+@pytest.mark.parametrize("case", ["pack", "drum", "wiggle"])
 def test_graph_capture_is_bitwise_eager(case):
     """Steps replayed from captured HIP graphs (32 per replay, rebuilds and
     other eager work between replays, several `run pre no` pieces) give
@@ -569,9 +665,12 @@ def test_graph_capture_is_bitwise_eager(case):
         params, mats, P, walls = wl.dense_pack(seed=34, n=3000, rolling=True, speed=0.3)
         meshes = ()
         pieces = [150, 70, 180]
-    else:
+    elif case == "drum":
         params, mats, P, walls, meshes = _small_drum()
         pieces = [200, 33, 300]
+    else:  # the wiggle mover reads its time table inside the captured steps
+        params, mats, P, walls, meshes = _mesh_floor_pack(seed=44, n=1500, moving="wiggle")
+        pieces = [150, 70, 180]
     a = pydem.GpuSystem(params, mats, P, walls, meshes=meshes)
     b = pydem.GpuSystem(params, mats, P, walls, meshes=meshes)
     b.set_graph_capture(True)

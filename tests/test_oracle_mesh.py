@@ -1,10 +1,13 @@
 """Pins the oracle's triangle-mesh walls (fix mesh/surface + fix wall/gran
-mesh + fix move/mesh rotate) against geometry and physics fixed independently:
+mesh + fix move/mesh rotate, linear, wiggle) against geometry and physics
+fixed independently:
 closest points on triangles by an independent projection/clamping routine,
 mesh topology invariants on meshes whose answer is known by construction,
 a sphere on a tessellated floor behaving exactly like the same sphere on a
 primitive plane (history carried across coplanar triangles), rigid rotation
-against the rotation matrix, and the skin/2 rebuild trigger of moving meshes.
+against the rotation matrix, translation against x0 + n v dt, the wiggle
+against the closed-form discrete sine sum, and the skin/2 rebuild trigger of
+moving meshes.
 """
 import math
 
@@ -243,3 +246,79 @@ def test_rotating_floor_drags_a_resting_sphere():
     vdir = np.cross(w, np.array([xc[0], xc[1], 0.0]))[:2]
     cosang = np.dot(ft, vdir) / np.linalg.norm(ft) / np.linalg.norm(vdir)
     assert cosang > 1 - 1e-6
+
+
+def _mover_case(motion, dt=5e-6, skin=0.0005):
+    tris = meshes.grid(-0.05, 0.05, -0.05, 0.05, 0.0, 4, 4)
+    params = {"dt": dt, "skin": skin, "boxlo": (-0.2, -0.2, -0.2), "boxhi": (0.2, 0.2, 0.2),
+              "periodic": (0, 0, 0)}
+    P = {"x": np.array([[0.0, 0.0, 0.1]]), "v": np.zeros((1, 3)), "omega": np.zeros((1, 3)),
+         "radius": np.array([0.001]), "rmass": np.array([1e-5]), "type": np.array([1]),
+         "tag": np.array([1])}
+    mats = {"ntypes": 2, "youngs": [5e6, 5e6], "poisson": [0.45, 0.45],
+            "restitution": [[0.3, 0.3], [0.3, 0.3]], "friction": [[0.5, 0.4], [0.4, 0.4]]}
+    return tris, orc.OracleSystem(params, mats, P, meshes=[(tris, 2, None, motion)]), params
+
+
+def test_linear_mover_translates_and_triggers_rebuilds():
+    """fix move/mesh linear: after n steps every node sits at x0 + n v dt, node
+    velocities are v, and the first mesh-forced rebuild comes on the step after
+    the displacement first exceeds skin/2."""
+    v = np.array([0.3, -0.2, 0.1])
+    tris, o, params = _mover_case({"linear": v})
+    n = 700
+    o.run(n)
+    nodes, vn = o.mesh_nodes()
+    assert np.allclose(nodes, tris + n * v * params["dt"], rtol=0, atol=1e-14)
+    assert np.array_equal(vn, np.broadcast_to(v, vn.shape))
+    k = 1
+    while (k * np.linalg.norm(v) * params["dt"]) ** 2 <= 0.25 * params["skin"] ** 2:
+        k += 1
+    _, o2, _ = _mover_case({"linear": v})
+    o2.run(k)
+    assert o2.mesh_forced_rebuilds() == 0
+    o2.run_continue(1)
+    assert o2.mesh_forced_rebuilds() == 1
+
+
+def test_wiggle_mover_is_the_discrete_sine_sum():
+    """fix move/mesh wiggle: the node velocity at step k is A w cos(w k dt)
+    (the fix's time is advanced before the move) and the displacement after
+    n steps is A w dt sum_k cos(w k dt) = A w dt sin(n th/2) cos((n+1) th/2) /
+    sin(th/2), th = w dt; over a whole number of periods it returns to the
+    start to O(dt). The time runs on across runs (fix lifetime)."""
+    A = np.array([0.002, 0.0, 0.001])
+    period = 0.01
+    tris, o, params = _mover_case({"wiggle": (A, period)})
+    dt = params["dt"]
+    w = 2 * math.pi / period
+    th = w * dt
+    n1, n2 = 900, 1100
+    o.run(n1)
+    o.run(n2)  # a second run continues the fix's time
+    n = n1 + n2
+    nodes, vn = o.mesh_nodes()
+    disp = A * w * dt * math.sin(n * th / 2) * math.cos((n + 1) * th / 2) / math.sin(th / 2)
+    assert np.allclose(nodes, tris + disp, rtol=0, atol=1e-13)
+    assert np.allclose(vn, np.broadcast_to(A * w * math.cos(w * n * dt), vn.shape), rtol=0, atol=1e-12)
+    assert n * dt / period == pytest.approx(1.0)
+    assert np.abs(nodes - tris).max() < 2 * np.abs(A).max() * th
+
+
+def test_linear_floor_drags_a_resting_sphere():
+    """A sphere at rest on a floor translating along +x: after one step the
+    tangential force on it points along +x (the wall velocity is the
+    translation velocity)."""
+    r = 0.002
+    m = wl.sphere_mass(np.array([r]), 2500.0)
+    params = {"dt": 1e-6, "skin": 0.001, "boxlo": (-0.1, -0.1, -0.01), "boxhi": (0.1, 0.1, 0.05),
+              "periodic": (0, 0, 0)}
+    P = {"x": np.array([[0.01, 0.02, r - 1e-6]]), "v": np.zeros((1, 3)), "omega": np.zeros((1, 3)),
+         "radius": np.array([r]), "rmass": m, "type": np.array([1]), "tag": np.array([1])}
+    mats = {"ntypes": 2, "youngs": [5e6, 5e6], "poisson": [0.45, 0.45],
+            "restitution": [[0.3, 0.3], [0.3, 0.3]], "friction": [[0.5, 0.4], [0.4, 0.4]]}
+    floor = meshes.grid(-0.08, 0.08, -0.08, 0.08, 0.0, 4, 4)
+    o = orc.OracleSystem(params, mats, P, meshes=[(floor, 2, None, {"linear": (0.2, 0.0, 0.0)})])
+    o.run(1)
+    f = o.state()["f"][0]
+    assert f[0] > 0 and abs(f[1]) < 1e-12 * abs(f[0])

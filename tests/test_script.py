@@ -69,7 +69,7 @@ def test_full_command_subset_parses(tmp_path):
     ("fix w all wall/gran model hertz tangential history mesh n_meshes 1 meshes cad\n", "unknown mesh cad"),
     ("fix w all wall/gran model hertz tangential history primitive type 1 cylinder 0.1 0 0\n", "cylinder is not available"),
     ("fix cad all mesh/surface file /nonexistent.stl type 1\n", "nonexistent"),
-    ("fix m all move/mesh mesh cad linear 1 0 0\n", "rotate"),
+    ("fix m all move/mesh mesh cad linear 1 0 0\n", "unknown mesh cad"),
     ("neigh_modify delay 10\n", "delay"),
     ("fix m1 all property/global coefficientRestitution peratomtypepair 2 0.3\n", "n*n"),
     ("atom_style atomic\n", "atom_style"),
@@ -122,6 +122,14 @@ def test_mesh_commands_parse(tmp_path):
     assert rc != 0 and "used twice" in log
     rc, log = run_script(tmp_path, good.replace("move/mesh mesh cad", "move/mesh mesh nope"), "in.mesh3")
     assert rc != 0 and "unknown mesh nope" in log
+    for movement, ok in (("linear 0.1 0 0", True), ("wiggle amplitude 0.001 0 0 period 0.1", True),
+                         ("viblin axis 0 0 1 order 1 amplitude 0.1 phase 0 period 0.1", False),
+                         ("wiggle amp 0.001 0 0 period 0.1", False)):
+        text = good.replace("rotate origin 0.05 0.05 0 axis 0 0 1 period 1.0", movement)
+        rc, log = run_script(tmp_path, text, "in.mesh4")
+        assert (rc == 0) == ok, (movement, log[-300:])
+    rc, log = run_script(tmp_path, good + "fix mv2 all move/mesh mesh cad linear 0 0 1\n", "in.mesh5")
+    assert rc != 0 and "one fix move/mesh per mesh" in log
 
 
 def test_sibling_model_commands_parse(tmp_path):
