@@ -188,6 +188,20 @@ int dem_add_mesh(dem_handle *h, int64_t ntri, const double *nodes, int wall_type
 int dem_mesh_rotate(dem_handle *h, int mesh_id, const double *origin, const double *axis,
                     double period);
 
+/* fix mesh/surface/stress ... [reference_point x y z] (MeshModuleStress,
+ * mesh_module_stress.cpp:51-120): the mesh accumulates, at every counted
+ * force pass (the last step of each dem_run / dem_run_continue, dem_setup,
+ * dem_compute_forces), the total force the particles exert on it and its
+ * torque about reference_point (NULL: the origin), as
+ * add_particle_contribution does (:276-300): minus each contact's force on
+ * the particle, lever arm from the reference point to the contact point. */
+int dem_mesh_stress(dem_handle *h, int mesh_id, const double *reference_point);
+
+/* The total force and torque on a mesh/surface/stress mesh at the last
+ * counted pass: what FixMeshSurface::compute_vector (fix_mesh_surface.cpp:560)
+ * returns as f_ID[1..6] for thermo output. Either pointer may be NULL. */
+int dem_get_mesh_force(dem_handle *h, int mesh_id, double *force, double *torque);
+
 /* fix move/mesh mesh <id> linear vx vy vz (MeshMoverLinear,
  * mesh_mover_linear.cpp:62-101): each step the mesh is shifted by v dt
  * (MultiNodeMesh::move, multi_node_mesh_I.h:502-527) and its nodes carry the

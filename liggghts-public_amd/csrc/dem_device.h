@@ -69,6 +69,8 @@ struct DevParams {
   int nmesh;
   int mesh_tri0[kMaxMeshes], mesh_ntri[kMaxMeshes];
   int mesh_wtype[kMaxMeshes], mesh_model[kMaxMeshes], mesh_moving[kMaxMeshes];
+  int mesh_stress;                 // bit m: mesh m is a mesh/surface/stress
+  double mesh_pref[kMaxMeshes][3];  // its reference_point
   double mesh_precision;   // EPSILON_PRECISION of the mesh (multi_node_mesh.h:59)
   // cell grid of one instance; instance k owns global cells
   // [k * ncell_local, (k + 1) * ncell_local)
@@ -113,6 +115,9 @@ struct Ctrl {
   int pad2;
   unsigned long long nmesh_cand;  // (particle, triangle) candidates at the last build
   long long step_base, ago_base;  // step index and neighbor age of a graph replay's first launch
+  // mesh/surface/stress: total force and torque (about the mesh's reference
+  // point) the particles exert on each mesh, at the last counted pass
+  double mesh_force[kMaxMeshes][6];
 };
 
 // Particle state read by neighbors: ping-pong buffered across steps.

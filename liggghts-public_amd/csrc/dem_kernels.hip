@@ -464,6 +464,21 @@ __device__ inline void mesh_contacts(const StepArgs &a, int i, const double4 &pi
         }
         T dF[3], dTi[3], dTj[3];
         contact_chain<T, true, GEN>(mdl, cc, (T)p.dt, a.shearupdate, sh, dF, dTi, dTj);
+        if (a.count && ((p.mesh_stress >> m) & 1)) {
+          // MeshModuleStress::add_particle_contribution (mesh_module_stress.cpp:
+          // 276-300): the force on the mesh is minus the particle's, its torque
+          // is taken about the reference point at the contact point x + delta
+          const double fm[3] = {-(double)dF[0], -(double)dF[1], -(double)dF[2]};
+          const double rp[3] = {px[0] + delta.x - p.mesh_pref[m][0], px[1] + delta.y - p.mesh_pref[m][1],
+                                px[2] + delta.z - p.mesh_pref[m][2]};
+          double *mf = a.ctrl->mesh_force[m];
+          atomicAdd(&mf[0], fm[0]);
+          atomicAdd(&mf[1], fm[1]);
+          atomicAdd(&mf[2], fm[2]);
+          atomicAdd(&mf[3], rp[1] * fm[2] - rp[2] * fm[1]);
+          atomicAdd(&mf[4], rp[2] * fm[0] - rp[0] * fm[2]);
+          atomicAdd(&mf[5], rp[0] * fm[1] - rp[1] * fm[0]);
+        }
         fx += dF[0];
         fy += dF[1];
         fz += dF[2];
@@ -1044,6 +1059,7 @@ __global__ void k_set_stop(Ctrl *ctrl, long long v, unsigned long long *inst_sta
     ctrl->ncontact_full = 0;
     ctrl->nwall = 0;
   }
+  if (k < kMaxMeshes * 6) ctrl->mesh_force[k / 6][k % 6] = 0.0;
   if (inst_stats && k < nstat) inst_stats[k] = 0ull;
 }
 

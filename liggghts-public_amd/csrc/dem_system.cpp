@@ -292,6 +292,8 @@ dem::DevParams make_devparams(const dem_handle *h) {
     d.mesh_wtype[m] = h->meshes[m].wall_type;
     d.mesh_model[m] = h->meshes[m].model;
     d.mesh_moving[m] = h->meshes[m].moving ? 1 : 0;
+    if (h->meshes[m].stress) d.mesh_stress |= 1 << m;
+    for (int k = 0; k < 3; k++) d.mesh_pref[m][k] = h->meshes[m].p_ref[k];
     t0 += h->meshes[m].ntri;
   }
   d.mesh_precision = h->meshes.empty() ? 1e-8 : h->meshes[0].precision;
@@ -982,6 +984,29 @@ int dem_mesh_rotate(dem_handle *h, int mesh_id, const double *origin, const doub
   m.omega = 2. * kMyPi / period;
   m.moving = true;
   m.mover = dem::kMoverRotate;
+  return DEM_OK;
+}
+
+int dem_mesh_stress(dem_handle *h, int mesh_id, const double *reference_point) {
+  if (!h) return DEM_ERR_ARG;
+  if (mesh_id < 0 || mesh_id >= (int)h->meshes.size()) return fail(h, DEM_ERR_ARG, "bad mesh id");
+  dem::HostMesh &m = h->meshes[mesh_id];
+  m.stress = true;
+  for (int k = 0; k < 3; k++) m.p_ref[k] = reference_point ? reference_point[k] : 0.0;
+  clear_graphs(h);  // the flag travels in the captured kernel arguments
+  return DEM_OK;
+}
+
+int dem_get_mesh_force(dem_handle *h, int mesh_id, double *force, double *torque) {
+  if (!h) return DEM_ERR_ARG;
+  if (mesh_id < 0 || mesh_id >= (int)h->meshes.size()) return fail(h, DEM_ERR_ARG, "bad mesh id");
+  if (!h->meshes[mesh_id].stress) return fail(h, DEM_ERR_STATE, "the mesh does not track its stress");
+  int rc = read_ctrl(h);
+  if (rc) return rc;
+  for (int k = 0; k < 3; k++) {
+    if (force) force[k] = h->ctrl_host->mesh_force[mesh_id][k];
+    if (torque) torque[k] = h->ctrl_host->mesh_force[mesh_id][3 + k];
+  }
   return DEM_OK;
 }
 

@@ -32,6 +32,8 @@ struct HostMesh {
   double origin[3] = {0, 0, 0}, axis[3] = {0, 0, 1}, omega = 0.0;  // rotate
   double vel[3] = {0, 0, 0};                                       // linear: velocity
   double ampl[3] = {0, 0, 0}, womega = 0.0;                        // wiggle
+  bool stress = false;          // fix mesh/surface/stress: total force and torque tracked
+  double p_ref[3] = {0, 0, 0};  // its reference_point
 };
 
 // Builds a mesh from ntri*9 node coordinates: drops elements with coinciding

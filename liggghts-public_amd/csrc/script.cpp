@@ -24,6 +24,9 @@
 //                  primitive type <t> xplane|yplane|zplane <pos>
 //   fix <id> <grp> mesh/surface file <stl> type <t> [scale s] [move dx dy dz]
 //                  [rotate axis ax ay az angle deg] [precision p] [verbose/heal ...]
+//   fix <id> <grp> mesh/surface/stress ... [reference_point x y z] [stress on|off]
+//                  (thermo f_<id>[1..6]: total force and torque on the mesh)
+//   compute <id> <grp> erotate/sphere | ke  (thermo c_<id>)
 //   fix <id> <grp> wall/gran model <normal> tangential <tangential> [...] mesh n_meshes <k>
 //                  meshes <id1> ... <idk>
 //   fix <id> <grp> move/mesh mesh <meshid> rotate origin ox oy oz axis ax ay az period T
@@ -162,7 +165,11 @@ struct Interp {
     double origin[3] = {0, 0, 0}, axis[3] = {0, 0, 1}, period = 1.0;
     int mover = 0;  // 0 none, 1 rotate, 2 linear, 3 wiggle
     double vec[3] = {0, 0, 0};  // linear: velocity; wiggle: amplitude
+    bool stress = false;        // fix mesh/surface/stress (stress on)
+    double pref[3] = {0, 0, 0};  // its reference_point
   };
+  std::map<std::string, int> mesh_ids;  // library mesh id of each fix mesh/surface
+  std::map<std::string, std::string> computes;  // compute id -> style (erotate/sphere, ke)
   std::map<std::string, MeshDef> mesh_defs;
   std::vector<std::string> mesh_order;  // meshes in fix wall/gran order
   int precision = DEM_FP64;
@@ -394,8 +401,11 @@ struct Interp {
   void fix_mesh_surface(const std::vector<std::string> &t) {
     // fix id group mesh/surface file <stl> type <t> [scale s] [move x y z]
     //     [rotate axis x y z angle a] [precision p] [verbose yes|no] [heal ...]
+    // mesh/surface/stress also takes [reference_point x y z] [stress on|off]
+    // (MeshModuleStress, mesh_module_stress.cpp:75-120)
     if (t.size() < 6 || t[4] != "file") die("fix mesh/surface: expected 'file <stl>'");
     MeshDef md;
+    md.stress = t[3] == "mesh/surface/stress";
     std::string err;
     if (!dem::read_ascii_stl(t[5], md.nodes, err)) die(err);
     for (size_t k = 6; k < t.size();) {
@@ -426,6 +436,16 @@ struct Interp {
           for (int d = 0; d < 3; d++) md.nodes[q + d] = v[d] * c + cr[d] * sn + ax[d] * kv * (1 - c);
         }
         k += 7;
+      } else if (t[3] == "mesh/surface/stress" && t[k] == "reference_point" && k + 3 < t.size()) {
+        for (int d = 0; d < 3; d++) md.pref[d] = num(t[k + 1 + d]);
+        k += 4;
+      } else if (t[3] == "mesh/surface/stress" && t[k] == "stress" && k + 1 < t.size()) {
+        if (t[k + 1] != "on" && t[k + 1] != "off") die("expecting 'on' or 'off' as stress argument");
+        md.stress = t[k + 1] == "on";
+        k += 2;
+      } else if (t[3] == "mesh/surface/stress" && t[k] == "wear" && k + 1 < t.size()) {
+        if (t[k + 1] != "off") die("fix mesh/surface/stress: wear is not available on this path");
+        k += 2;
       } else if ((t[k] == "precision" || t[k] == "verbose" || t[k] == "heal" ||
                   t[k] == "curvature" || t[k] == "curvature_tolerant") && k + 1 < t.size()) {
         if (t[k] == "precision" || t[k] == "curvature")
@@ -731,6 +751,8 @@ struct Interp {
       if (md.mover == 1 && dem_mesh_rotate(h, id, md.origin, md.axis, md.period)) die(dem_last_error(h));
       if (md.mover == 2 && dem_mesh_linear(h, id, md.vec)) die(dem_last_error(h));
       if (md.mover == 3 && dem_mesh_wiggle(h, id, md.vec, md.period)) die(dem_last_error(h));
+      if (md.stress && dem_mesh_stress(h, id, md.pref)) die(dem_last_error(h));
+      mesh_ids[name] = id;
     }
     if (tag.empty()) die("run: no atoms");
     if (dem_set_particles(h, (int64_t)tag.size(), x.data(), v.data(), w.data(), r.data(), m.data(),
@@ -759,6 +781,42 @@ struct Interp {
     fprintf(out, "%s\n", hdr.c_str());
   }
 
+  // f_<mesh>[1..6]: force and torque on a mesh/surface/stress
+  // (FixMeshSurface::compute_vector, fix_mesh_surface.cpp:560-570);
+  // c_<id> of compute erotate/sphere (0.5 I w^2 with I = 0.4 m r^2,
+  // compute_erotate_sphere.cpp) or compute ke
+  bool thermo_value(const std::string &c, double ke, double &val) {
+    if (c.size() > 5 && c.compare(0, 2, "f_") == 0 && c.back() == ']') {
+      const size_t lb = c.find('[');
+      if (lb == std::string::npos) return false;
+      const std::string id = c.substr(2, lb - 2);
+      const int k = std::atoi(c.substr(lb + 1, c.size() - lb - 2).c_str());
+      auto it = mesh_ids.find(id);
+      if (sq || !h || it == mesh_ids.end() || !mesh_defs[id].stress || k < 1 || k > 6) return false;
+      double f[3], tq[3];
+      if (dem_get_mesh_force(h, it->second, f, tq)) die(dem_last_error(h));
+      val = k <= 3 ? f[k - 1] : tq[k - 4];
+      return true;
+    }
+    if (c.size() > 2 && c.compare(0, 2, "c_") == 0) {
+      auto it = computes.find(c.substr(2));
+      if (it == computes.end()) return false;
+      if (it->second == "ke") {
+        val = ke;
+        return true;
+      }
+      if (it->second == "erotate/sphere" && !sq) {
+        double er = 0.0;
+        for (size_t i = 0; i < tag.size(); i++)
+          er += 0.4 * m[i] * r[i] * r[i] *
+                (w[3 * i] * w[3 * i] + w[3 * i + 1] * w[3 * i + 1] + w[3 * i + 2] * w[3 * i + 2]);
+        val = 0.5 * er;
+        return true;
+      }
+    }
+    return false;
+  }
+
   void thermo_line() {
     pull_state();
     double ke = 0.0;
@@ -767,9 +825,11 @@ struct Interp {
     std::string row;
     for (auto &c : thermo_cols) {
       char buf[64];
+      double val = 0.0;
       if (c == "step") snprintf(buf, sizeof buf, "%14lld ", (long long)step);
       else if (c == "atoms") snprintf(buf, sizeof buf, "%14zu ", tag.size());
       else if (c == "ke") snprintf(buf, sizeof buf, "%14.8g ", ke);
+      else if (thermo_value(c, ke, val)) snprintf(buf, sizeof buf, "%14.8g ", val);
       else snprintf(buf, sizeof buf, "%14s ", "n/a");
       row += buf;
     }
@@ -849,11 +909,15 @@ struct Interp {
       }
     } else if (c == "atom_modify" || c == "processors" || c == "echo" || c == "log" ||
                c == "thermo_modify" || c == "dump" || c == "dump_modify" || c == "undump" ||
-               c == "compute" || c == "uncompute" || c == "group" || c == "restart" ||
+               (c == "compute" && !(t.size() >= 4 && (t[3] == "erotate/sphere" || t[3] == "ke"))) ||
+               c == "uncompute" || c == "group" || c == "restart" ||
                c == "write_restart" || c == "write_dump" || c == "soft_particles" ||
                c == "hard_particles") {
       if (c != "atom_modify" && c != "processors" && c != "echo" && c != "log" && c != "thermo_modify")
         fprintf(out, "WARNING: '%s' is output/bookkeeping only and is ignored on this path\n", c.c_str());
+    } else if (c == "compute") {
+      // compute id group erotate/sphere | ke: read by thermo as c_id
+      computes[t[1]] = t[3];
     } else if (c == "boundary") {
       if (t.size() < 4) die("Illegal boundary command");
       for (int d = 0; d < 3; d++) periodic[d] = t[1 + d] == "p" ? 1 : 0;
@@ -945,7 +1009,8 @@ struct Interp {
         gravity = true;
         gravity_first = !wall_seen;
         dirty = true;
-      } else if (style == "mesh/surface" || style == "mesh/surface/planar") {
+      } else if (style == "mesh/surface" || style == "mesh/surface/planar" ||
+                 style == "mesh/surface/stress") {
         fix_mesh_surface(t);
       } else if (style == "move/mesh") {
         // fix id group move/mesh mesh <meshid> rotate origin x y z axis x y z period T

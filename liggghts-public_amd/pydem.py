@@ -22,7 +22,7 @@ EXPORTED = [
     "dem_set_particles", "dem_add_plane_wall", "dem_setup", "dem_run",
     "dem_run_continue", "dem_compute_forces", "dem_get_state", "dem_get_pairs",
     "dem_get_wall_history", "dem_get_stats", "dem_set_kernel_timing",
-    "dem_run_script", "dem_add_mesh", "dem_mesh_rotate", "dem_get_mesh_nodes", "dem_mesh_linear", "dem_mesh_wiggle",
+    "dem_run_script", "dem_add_mesh", "dem_mesh_rotate", "dem_get_mesh_nodes", "dem_mesh_linear", "dem_mesh_wiggle", "dem_mesh_stress", "dem_get_mesh_force",
     "dem_get_mesh_topology", "dem_get_mesh_history", "dem_set_instances",
     "dem_set_instance_materials", "dem_get_instance_stats", "dem_set_step_shape",
     "dem_get_step_shape", "dem_set_graph_capture", "dem_set_pair_model", "dem_set_model_coefficients",
@@ -173,6 +173,8 @@ def load_library(path=LIB_PATH):
                                  P(ctypes.c_int)]
     lib.dem_mesh_rotate.argtypes = [h, ctypes.c_int, dp, dp, ctypes.c_double]
     lib.dem_mesh_linear.argtypes = [h, ctypes.c_int, dp]
+    lib.dem_mesh_stress.argtypes = [h, ctypes.c_int, dp]
+    lib.dem_get_mesh_force.argtypes = [h, ctypes.c_int, dp, dp]
     lib.dem_mesh_wiggle.argtypes = [h, ctypes.c_int, dp, ctypes.c_double]
     lib.dem_get_mesh_nodes.argtypes = [h, ctypes.c_int, dp, dp]
     lib.dem_get_mesh_topology.argtypes = [h, ctypes.c_int, ip, ip, ip]
@@ -270,7 +272,8 @@ class GpuSystem:
                  max_neighbors=0, device=0, meshes=(), instances=None, instance_materials=None):
         """meshes: [(tris (T,3,3), wall_type, model dict or None, rotate or None)],
         rotate = (origin, axis, period) for fix move/mesh rotate, or
-        {"linear": (vx, vy, vz)} or {"wiggle": ((ax, ay, az), period)}.
+        {"linear": (vx, vy, vz)} or {"wiggle": ((ax, ay, az), period)}; an optional
+        fifth entry (a reference point) makes the mesh a mesh/surface/stress.
         instances: particle counts of independent simulations batched in this
         system (particles concatenated in that order); instance_materials: one
         material dict per instance."""
@@ -325,7 +328,8 @@ class GpuSystem:
                                                     ctypes.byref(m)))
             self.nwalls += 1
         self.ntri = []
-        for (tris, wtype, mdl, rot) in meshes:
+        for mesh in meshes:
+            tris, wtype, mdl, rot = mesh[:4]
             t = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
             m = contact_model(mdl if mdl is not None else params)
             mid = ctypes.c_int(-1)
@@ -342,6 +346,9 @@ class GpuSystem:
                 a = np.ascontiguousarray(rot[1], dtype=np.float64)
                 self._check(self.lib.dem_mesh_rotate(self.h, mid.value, _dp(o), _dp(a),
                                                      float(rot[2])))
+            if len(mesh) > 4 and mesh[4] is not None:
+                pr = np.ascontiguousarray(mesh[4], dtype=np.float64)
+                self._check(self.lib.dem_mesh_stress(self.h, mid.value, _dp(pr)))
             self.ntri.append(len(t))
         self.set_particles(particles)
 
@@ -440,6 +447,13 @@ class GpuSystem:
                                                 ctypes.byref(nc), _ip(cv), _ip(cf), _dp(cu)))
         n = min(nc.value, cap)
         return v.value, f.value, [(int(cv[k]), int(cf[k]), float(cu[k])) for k in range(n)]
+
+    def mesh_force(self, k=0):
+        """(total force, torque about the reference point) on mesh/surface/stress k
+        at the last counted force pass"""
+        f, t = np.zeros(3), np.zeros(3)
+        self._check(self.lib.dem_get_mesh_force(self.h, k, _dp(f), _dp(t)))
+        return f, t
 
     def instance_stats(self):
         """(touching pairs, wall contacts) per instance at the last step."""

@@ -445,6 +445,8 @@ typedef struct {
   double vel[3];                     /* linear */
   double ampl[3], womega;            /* wiggle */
   double time;                       /* FixMoveMesh::time_ */
+  int stress;                        /* mesh/surface/stress */
+  double p_ref[3], ftot[3], ttot[3]; /* reference point, total force and torque on the mesh */
   double curvature, precision;
   /* per particle */
   int *ncand, *cand;
@@ -1357,6 +1359,19 @@ void orc_mesh_rotate(orc_sys *s, int imesh, const double *origin, const double *
   m->omega = 2. * MY_PI / period; /* M_PI in the reference: same constant */
 }
 
+/* fix mesh/surface/stress reference_point (mesh_module_stress.cpp:80-96) */
+void orc_mesh_stress(orc_sys *s, int imesh, const double *p_ref) {
+  orc_mesh *m = &s->mesh[imesh];
+  m->stress = 1;
+  for (int k = 0; k < 3; k++) m->p_ref[k] = p_ref ? p_ref[k] : 0.0;
+}
+
+void orc_mesh_force(const orc_sys *s, int imesh, double *force, double *torque) {
+  const orc_mesh *m = &s->mesh[imesh];
+  memcpy(force, m->ftot, sizeof(m->ftot));
+  memcpy(torque, m->ttot, sizeof(m->ttot));
+}
+
 /* MeshMoverLinear, mesh_mover_linear.cpp:62-75 */
 void orc_mesh_linear(orc_sys *s, int imesh, const double *vel) {
   orc_mesh *m = &s->mesh[imesh];
@@ -1827,6 +1842,9 @@ static int mesh_handle_contact(orc_mesh *m, int i, int tri, double **hist, int f
 /* FixWallGran::post_force_mesh for one mesh */
 static void mesh_wall_compute(orc_sys *s, int im, int shearupdate) {
   orc_mesh *m = &s->mesh[im];
+  /* MeshModuleStress::pre_force (mesh_module_stress.cpp:225-236): totals restart */
+  memset(m->ftot, 0, sizeof(m->ftot));
+  memset(m->ttot, 0, sizeof(m->ttot));
   for (int i = 0; i < s->n; i++) {
     for (int k = 0; k < MESH_MAXSLOT; k++) SKEEP(m, i, k) = 0; /* markAllContacts */
     for (int c = 0; c < m->ncand[i]; c++) {
@@ -1849,9 +1867,27 @@ static void mesh_wall_compute(orc_sys *s, int im, int shearupdate) {
         orc_contact_wall(&m->model, &s->m, s->type[i], m->wall_type, -deltan, sdelta, s->r[i],
                          &s->v[3 * i], &s->w[3 * i], vwall, s->rm[i], s->p.dt, shearupdate, h, dF,
                          dT);
+        double fold[3];
         for (int d = 0; d < 3; d++) {
+          fold[d] = s->f[3 * i + d];
           s->f[3 * i + d] += dF[d];
           s->t[3 * i + d] += dT[d];
+        }
+        if (m->stress) {
+          /* fix_wall_gran_base.h:344-359: f_pw = f - force_old, handed with
+           * delta = -sidata.delta to MeshModuleStress::add_particle_contribution
+           * (mesh_module_stress.cpp:276-300), which negates it and adds it and
+           * its torque about p_ref at the contact point x + delta */
+          double frc[3], cp[3], rr[3];
+          for (int d = 0; d < 3; d++) frc[d] = -(s->f[3 * i + d] - fold[d]);
+          for (int d = 0; d < 3; d++) cp[d] = s->x[3 * i + d] + (-sdelta[d]);
+          for (int d = 0; d < 3; d++) {
+            m->ftot[d] += frc[d];
+            rr[d] = cp[d] - m->p_ref[d];
+          }
+          m->ttot[0] += rr[1] * frc[2] - rr[2] * frc[1];
+          m->ttot[1] += rr[2] * frc[0] - rr[0] * frc[2];
+          m->ttot[2] += rr[0] * frc[1] - rr[1] * frc[0];
         }
       }
     }

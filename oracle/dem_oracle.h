@@ -153,6 +153,8 @@ int orc_add_mesh(orc_sys *s, int ntri, const double *nodes, int wall_type,
                  const orc_model *model);
 /* fix move/mesh mesh <id> rotate origin ... axis ... period T
  * (mesh_mover_rotation.cpp:66-139) */
+void orc_mesh_stress(orc_sys *s, int imesh, const double *p_ref);
+void orc_mesh_force(const orc_sys *s, int imesh, double *force, double *torque);
 void orc_mesh_linear(orc_sys *s, int imesh, const double *vel);
 void orc_mesh_wiggle(orc_sys *s, int imesh, const double *amplitude, double period);
 void orc_mesh_rotate(orc_sys *s, int imesh, const double *origin, const double *axis,

@@ -94,6 +94,8 @@ def load():
     lib.orc_add_mesh.restype = ctypes.c_int
     lib.orc_mesh_rotate.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, _D]
     lib.orc_mesh_linear.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
+    lib.orc_mesh_stress.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
+    lib.orc_mesh_force.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
     lib.orc_mesh_wiggle.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, _D]
     lib.orc_mesh_topology.argtypes = [ctypes.c_void_p, ctypes.c_int, ip, ip, ip]
     lib.orc_mesh_nodes.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
@@ -246,9 +248,10 @@ class OracleSystem:
     """The CPU restatement of Verlet::setup / Verlet::run."""
 
     def __init__(self, params, mat, particles, walls=(), meshes=()):
-        """meshes: [(tris (T,3,3), wall_type, model dict or None, motion or None)],
-        motion = (origin, axis, period) for fix move/mesh rotate, or
-        {"linear": (vx, vy, vz)} or {"wiggle": ((ax, ay, az), period)}."""
+        """meshes: [(tris (T,3,3), wall_type, model dict or None, motion or None
+        [, stress reference point])], motion = (origin, axis, period) for fix
+        move/mesh rotate, or {"linear": (vx, vy, vz)} or {"wiggle": ((ax, ay, az),
+        period)}; a fifth entry makes the mesh a mesh/surface/stress."""
         self.lib = load()
         self.p = make_params(params, walls)
         self.m = materials(mat)
@@ -262,7 +265,8 @@ class OracleSystem:
                                      _dp(P["x"]), _dp(P["v"]), _dp(P["omega"]),
                                      _dp(P["radius"]), _dp(P["rmass"]), _ip(ty), _ip(tg))
         self.ntri = []
-        for (tris, wtype, mdl, rot) in meshes:
+        for mesh in meshes:
+            tris, wtype, mdl, rot = mesh[:4]
             t = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
             m = model(mdl if mdl is not None else params)
             k = self.lib.orc_add_mesh(self.s, len(t), _dp(t), int(wtype), ctypes.byref(m))
@@ -278,6 +282,9 @@ class OracleSystem:
                 o = np.ascontiguousarray(rot[0], dtype=np.float64)
                 a = np.ascontiguousarray(rot[1], dtype=np.float64)
                 self.lib.orc_mesh_rotate(self.s, k, _dp(o), _dp(a), float(rot[2]))
+            if len(mesh) > 4 and mesh[4] is not None:
+                pr = np.ascontiguousarray(mesh[4], dtype=np.float64)
+                self.lib.orc_mesh_stress(self.s, k, _dp(pr))
             self.ntri.append(len(t))
 
     def mesh_topology(self, k=0):
@@ -299,6 +306,13 @@ class OracleSystem:
         hist = np.zeros((self.n, maxslot, 3))
         self.lib.orc_mesh_history(self.s, k, maxslot, _ip(tri), _dp(hist))
         return tri, hist
+
+    def mesh_force(self, k=0):
+        """(total force, torque about the reference point) on mesh/surface/stress k
+        at the last force pass"""
+        f, t = np.zeros(3), np.zeros(3)
+        self.lib.orc_mesh_force(self.s, k, _dp(f), _dp(t))
+        return f, t
 
     def mesh_forced_rebuilds(self):
         return self.lib.orc_mesh_forced_rebuilds(self.s)

@@ -563,10 +563,11 @@ run 400
 
 
 def test_script_wiggling_floor_matches_api(tmp_path):
-    """A shaker written as an input script (fix mesh/surface floor from an STL
-    file, fix move/mesh ... wiggle amplitude ... period ..., two runs so the
-    mover's time runs on) gives the kinetic energy of the same system built
-    through the C ABI."""
+    """A shaker written as an input script (fix mesh/surface/stress floor from
+    an STL file, fix move/mesh ... wiggle amplitude ... period ..., two runs so
+    the mover's time runs on; thermo with compute erotate/sphere and the
+    floor's f_cad[3] / f_cad[5]) gives the kinetic and rotational energy and
+    the mesh force and torque of the same system built through the C ABI."""
     say("[gpu] script wiggling floor")
     params, mats, P, walls, meshes = _mesh_floor_pack(seed=45, n=1200, moving="wiggle")
     data = tmp_path / "shake.data"
@@ -595,12 +596,13 @@ pair_style gran model hertz tangential history rolling_friction cdt
 pair_coeff * *
 timestep {params['dt']!r}
 fix gravi all gravity 9.81 vector 0.0 0.0 -1.0
-fix cad all mesh/surface file {stl} type 2
+fix cad all mesh/surface/stress file {stl} type 2 reference_point 0.01 0.0 0.0
 fix walls all wall/gran model hertz tangential history rolling_friction cdt mesh n_meshes 1 meshes cad
 fix shake all move/mesh mesh cad wiggle amplitude {amp[0]!r} {amp[1]!r} {amp[2]!r} period {period!r}
 read_data {data}
 fix integr all nve/sphere
-thermo_style custom step atoms ke
+compute er all erotate/sphere
+thermo_style custom step atoms ke c_er f_cad[3] f_cad[5]
 thermo 150
 run 150
 run 150
@@ -615,12 +617,19 @@ run 150
     assert int(last[0]) == 300 and int(last[1]) == len(P["radius"])
     params = dict(params, boxlo=(lo[0] - 0.003, lo[1] - 0.003, lo[2] - 0.001),
                   boxhi=(hi[0] + 0.003, hi[1] + 0.003, hi[2]))
+    meshes = [tuple(meshes[0][:4]) + ((0.01, 0.0, 0.0),)]
     g = pydem.GpuSystem(params, mats, P, walls, meshes=meshes)
     g.run(150)
     g.run(150)
     st = g.state()
     ke = 0.5 * (P["rmass"] @ (st["v"] ** 2).sum(1))
+    er = 0.5 * (0.4 * P["rmass"] * P["radius"] ** 2 @ (st["omega"] ** 2).sum(1))
+    F, T = g.mesh_force()
     assert float(last[2]) == pytest.approx(ke, rel=1e-6)
+    assert float(last[3]) == pytest.approx(er, rel=1e-6)
+    # the mesh totals are atomically summed, so equal to rounding only
+    assert float(last[4]) == pytest.approx(F[2], rel=1e-6) and F[2] < 0
+    assert float(last[5]) == pytest.approx(T[1], rel=1e-6)
     g.close()
 
 
@@ -688,3 +697,27 @@ def test_graph_capture_is_bitwise_eager(case):
         assert np.array_equal(a.mesh_nodes()[0], b.mesh_nodes()[0])
     a.close()
     b.close()
+
+
+def test_mesh_stress_totals_match_oracle():
+    """fix mesh/surface/stress on the small rotating drum: the total force and
+    the torque about the drum axis that the particles exert on the mesh equal
+    the oracle's at setup (1e-10) and after 400 steps (1e-8, where the state
+    agrees to 1e-8)."""
+    say("[gpu] mesh stress totals")
+    params, mats, P, walls, meshes = _small_drum()
+    meshes = [tuple(meshes[0][:4]) + ((0.0, 0.0, 0.0),)]
+    g, o = _build_mesh_case((params, mats, P, walls, meshes))
+    g.setup()
+    o.setup()
+    for tol in (1e-10, 1e-8):
+        gF, gT = g.mesh_force()
+        oF, oT = o.mesh_force()
+        say(f"  F gpu {gF} oracle {oF}  T gpu {gT} oracle {oT}")
+        assert np.linalg.norm(gF - oF) <= tol * max(np.linalg.norm(oF), 1e-300)
+        assert np.linalg.norm(gT - oT) <= tol * max(np.linalg.norm(oT), 1e-300)
+        if tol == 1e-10:
+            g.run(400)
+            o.run(400)
+    assert np.linalg.norm(oF) > 0 and np.linalg.norm(oT) > 0
+    g.close()

@@ -322,3 +322,32 @@ def test_linear_floor_drags_a_resting_sphere():
     o.run(1)
     f = o.state()["f"][0]
     assert f[0] > 0 and abs(f[1]) < 1e-12 * abs(f[0])
+
+
+def test_mesh_stress_of_a_resting_sphere():
+    """fix mesh/surface/stress: a sphere resting on a floor pushes it with its
+    weight (force on the mesh (0, 0, -m g)); the torque about a reference
+    point p is (c - p) x F with the contact point c under the sphere's centre,
+    zero about that point itself."""
+    r, g = 0.002, 9.81
+    m = wl.sphere_mass(np.array([r]), 2500.0)
+    Ys = 5e6 / (2.0 * (1.0 - 0.45 ** 2))
+    d_eq = (m[0] * g / (4.0 / 3.0 * Ys * math.sqrt(r))) ** (2.0 / 3.0)
+    x0, y0 = 0.013, -0.021
+    params = {"dt": 2e-6, "skin": 0.001, "boxlo": (-0.1, -0.1, -0.01), "boxhi": (0.1, 0.1, 0.05),
+              "periodic": (0, 0, 0), "grav_acc": (0.0, 0.0, -g)}
+    P = {"x": np.array([[x0, y0, r - d_eq]]), "v": np.zeros((1, 3)), "omega": np.zeros((1, 3)),
+         "radius": np.array([r]), "rmass": m, "type": np.array([1]), "tag": np.array([1])}
+    mats = {"ntypes": 2, "youngs": [5e6, 5e6], "poisson": [0.45, 0.45],
+            "restitution": [[0.3, 0.3], [0.3, 0.3]], "friction": [[0.5, 0.4], [0.4, 0.4]]}
+    floor = meshes.grid(-0.08, 0.08, -0.08, 0.08, 0.0, 4, 4)
+    W = m[0] * g
+    for pref in ((0.0, 0.0, 0.0), (x0, y0, 0.0), (0.05, 0.01, 0.02)):
+        o = orc.OracleSystem(params, mats, P, meshes=[(floor, 2, None, None, pref)])
+        o.run(3000)
+        F, T = o.mesh_force()
+        assert F[2] == pytest.approx(-W, rel=1e-4)
+        assert abs(F[0]) < 1e-9 * W and abs(F[1]) < 1e-9 * W
+        c = o.state()["x"][0] - np.array([0.0, 0.0, o.state()["x"][0, 2]])  # contact point on the floor
+        assert np.allclose(T, np.cross(c - np.array(pref), F), rtol=1e-6, atol=1e-9 * W * 0.01)
+        o.close()

@@ -130,6 +130,12 @@ def test_mesh_commands_parse(tmp_path):
         assert (rc == 0) == ok, (movement, log[-300:])
     rc, log = run_script(tmp_path, good + "fix mv2 all move/mesh mesh cad linear 0 0 1\n", "in.mesh5")
     assert rc != 0 and "one fix move/mesh per mesh" in log
+    stress = good.replace("mesh/surface file", "mesh/surface/stress file").replace(
+        "precision 1e-8", "precision 1e-8 reference_point 0.05 0.05 0 stress on")
+    rc, log = run_script(tmp_path, stress + "thermo_style custom step atoms ke c_1 f_cad[1] f_cad[6]\n", "in.mesh6")
+    assert rc == 0, log
+    rc, log = run_script(tmp_path, stress.replace("stress on", "wear finnie"), "in.mesh7")
+    assert rc != 0 and "wear" in log
 
 
 def test_sibling_model_commands_parse(tmp_path):
