@@ -43,7 +43,11 @@
 //   variable <name> equal|index <value> | print | echo | log
 //   run <N> [pre no|yes] [post no|yes]
 //   package amd precision fp64|mixed   (selects the GPU arithmetic mode)
-// Output-only commands (dump, compute, undump, ...) are accepted and ignored
+//   dump <id> all custom <N> <file> <columns> | dump_modify <id> ... | undump <id>
+//                  (columns id type x y z vx vy vz fx fy fz omegax omegay omegaz
+//                   tqx tqy tqz radius diameter mass; '*' in the file name: one
+//                   file per snapshot; atoms in ID order)
+// Other output-only commands (compute, restart, ...) are accepted and ignored
 // with a warning; anything that would change the physics and is not listed
 // is an error, as the reference reports for an unknown style.
 #include <algorithm>
@@ -182,9 +186,121 @@ struct Interp {
   bool dirty = true;  // particles/params changed since the handle was built
   int64_t step = 0;
 
+  // dump <id> all custom <N> <file> <columns...> (DumpCustom, dump_custom.cpp)
+  struct Dump {
+    std::string file;  // '*' is replaced by the timestep: one file per snapshot
+    long long every = 0;
+    std::vector<std::string> cols;
+    FILE *fp = nullptr;
+    long long last = -1;  // timestep of the last snapshot written
+  };
+  std::map<std::string, Dump> dumps;
+
   ~Interp() {
+    for (auto &d : dumps)
+      if (d.second.fp) fclose(d.second.fp);
     if (h) dem_destroy(h);
     if (hs) dem_sqdem_destroy(hs);
+  }
+
+  static bool dump_column_known(const std::string &c) {
+    static const char *known[] = {"id", "type", "x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz",
+                                  "omegax", "omegay", "omegaz", "tqx", "tqy", "tqz", "radius",
+                                  "diameter", "mass"};
+    for (const char *k : known)
+      if (c == k) return true;
+    return false;
+  }
+
+  void dump_command(const std::vector<std::string> &t) {
+    // dump id group custom N file col1 col2 ...
+    if (t.size() < 6) die("Illegal dump command");
+    if (t[3] != "custom") die("dump style " + t[3] + " is not available on this path (custom)");
+    if (t[2] != "all") die("dump: only group all is available on this path");
+    if (sq) die("dump custom for superquadrics is not available on this path");
+    if (dumps.count(t[1])) die("Reuse of dump ID " + t[1]);
+    Dump d;
+    d.every = std::stoll(t[4]);
+    if (d.every <= 0) die("Illegal dump command");
+    d.file = t[5];
+    for (size_t k = 6; k < t.size(); k++) {
+      if (!dump_column_known(t[k])) die("Invalid attribute " + t[k] + " in dump custom command");
+      d.cols.push_back(t[k]);
+    }
+    if (d.cols.empty()) die("Illegal dump command");
+    if (d.file.find('*') == std::string::npos) {
+      d.fp = fopen(d.file.c_str(), "w");
+      if (!d.fp) die("Cannot open dump file " + d.file);
+    }
+    dumps[t[1]] = d;
+  }
+
+  // one snapshot in the reference's text layout (DumpCustom::header_item,
+  // dump_custom.cpp:378-393, write_lines: "%d " / "%g " per column), atoms in
+  // ID order (dump_modify sort id)
+  void write_dumps(bool force_all) {
+    bool any = false;
+    for (auto &kv : dumps)
+      if (kv.second.last != step && (force_all || step % kv.second.every == 0)) any = true;
+    if (!any) return;
+    const size_t n = tag.size();
+    std::vector<double> f(3 * n), tq(3 * n);
+    if (dem_get_state(h, x.data(), v.data(), w.data(), f.data(), tq.data())) die(dem_last_error(h));
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return tag[a] < tag[b]; });
+    char bstr[9];
+    for (int d = 0; d < 3; d++) bstr[3 * d] = bstr[3 * d + 1] = periodic[d] ? 'p' : 'f';
+    bstr[2] = bstr[5] = ' ';
+    bstr[8] = '\0';
+    for (auto &kv : dumps) {
+      Dump &d = kv.second;
+      if (d.last == step || !(force_all || step % d.every == 0)) continue;
+      FILE *fp = d.fp;
+      if (!fp) {
+        std::string name = d.file;
+        name.replace(name.find('*'), 1, std::to_string((long long)step));
+        fp = fopen(name.c_str(), "w");
+        if (!fp) die("Cannot open dump file " + name);
+      }
+      std::string cols;
+      for (size_t c = 0; c < d.cols.size(); c++) cols += (c ? " " : "") + d.cols[c];
+      fprintf(fp, "ITEM: TIMESTEP\n%lld\nITEM: NUMBER OF ATOMS\n%zu\nITEM: BOX BOUNDS %s\n", (long long)step, n,
+              bstr);
+      for (int k = 0; k < 3; k++) fprintf(fp, "%g %g\n", lo[k], hi[k]);
+      fprintf(fp, "ITEM: ATOMS %s\n", cols.c_str());
+      for (size_t oi = 0; oi < n; oi++) {
+        const size_t i = order[oi];
+        for (const std::string &c : d.cols) {
+          if (c == "id") fprintf(fp, "%d ", tag[i]);
+          else if (c == "type") fprintf(fp, "%d ", type[i]);
+          else {
+            double val = 0.0;
+            if (c == "x" || c == "y" || c == "z") val = x[3 * i + (c[0] - 'x')];
+            else if (c == "vx" || c == "vy" || c == "vz") val = v[3 * i + (c[1] - 'x')];
+            else if (c == "fx" || c == "fy" || c == "fz") val = f[3 * i + (c[1] - 'x')];
+            else if (c == "omegax" || c == "omegay" || c == "omegaz") val = w[3 * i + (c[5] - 'x')];
+            else if (c == "tqx" || c == "tqy" || c == "tqz") val = tq[3 * i + (c[2] - 'x')];
+            else if (c == "radius") val = r[i];
+            else if (c == "diameter") val = 2.0 * r[i];
+            else if (c == "mass") val = m[i];
+            fprintf(fp, "%g ", val);
+          }
+        }
+        fprintf(fp, "\n");
+      }
+      if (d.fp) fflush(d.fp);
+      else fclose(fp);
+      d.last = step;
+    }
+  }
+
+  // the next step at which a thermo line or a dump is due, at most `limit`
+  long long next_output(long long limit) const {
+    long long k = limit;
+    if (thermo_every > 0) k = std::min<long long>(k, thermo_every - step % thermo_every);
+    for (const auto &kv : dumps) k = std::min<long long>(k, kv.second.every - step % kv.second.every);
+    return k;
   }
 
   std::string substitute(const std::string &line) {
@@ -860,8 +976,9 @@ struct Interp {
     }
     thermo_header();
     thermo_line();
+    if (!sq) write_dumps(false);  // Output::setup writes the dumps due at this step
     while (done < n) {
-      long long k = chunk - (step % (chunk > 0 ? chunk : 1));
+      long long k = sq ? chunk - (step % (chunk > 0 ? chunk : 1)) : next_output(n - done);
       if (k <= 0 || k > n - done) k = n - done;
       int rc = sq ? dem_sqdem_run(hs, (int)k) : dem_run_continue(h, k);
       (void)first;
@@ -870,6 +987,7 @@ struct Interp {
       done += k;
       step += k;
       if ((thermo_every > 0 && step % thermo_every == 0) || done == n) thermo_line();
+      if (!sq) write_dumps(false);
     }
     const double secs =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -908,13 +1026,26 @@ struct Interp {
         die("Only atom_style granular/sphere/superquadric is supported on this path");
       }
     } else if (c == "atom_modify" || c == "processors" || c == "echo" || c == "log" ||
-               c == "thermo_modify" || c == "dump" || c == "dump_modify" || c == "undump" ||
+               c == "thermo_modify" ||
                (c == "compute" && !(t.size() >= 4 && (t[3] == "erotate/sphere" || t[3] == "ke"))) ||
                c == "uncompute" || c == "group" || c == "restart" ||
                c == "write_restart" || c == "write_dump" || c == "soft_particles" ||
                c == "hard_particles") {
       if (c != "atom_modify" && c != "processors" && c != "echo" && c != "log" && c != "thermo_modify")
         fprintf(out, "WARNING: '%s' is output/bookkeeping only and is ignored on this path\n", c.c_str());
+    } else if (c == "dump") {
+      dump_command(t);
+    } else if (c == "dump_modify") {
+      if (t.size() < 2 || !dumps.count(t[1])) die("Could not find dump_modify ID");
+      for (size_t k = 2; k < t.size(); k++)
+        if (t[k] == "sort" && k + 1 < t.size() && t[k + 1] != "id" && t[k + 1] != "off")
+          die("dump_modify sort: atoms are always written in ID order on this path");
+      fprintf(out, "WARNING: dump_modify: atoms are written in ID order with the default formats\n");
+    } else if (c == "undump") {
+      auto it = dumps.find(t.size() > 1 ? t[1] : "");
+      if (it == dumps.end()) die("Could not find undump ID");
+      if (it->second.fp) fclose(it->second.fp);
+      dumps.erase(it);
     } else if (c == "compute") {
       // compute id group erotate/sphere | ke: read by thermo as c_id
       computes[t[1]] = t[3];

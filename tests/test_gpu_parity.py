@@ -721,3 +721,83 @@ def test_mesh_stress_totals_match_oracle():
             o.run(400)
     assert np.linalg.norm(oF) > 0 and np.linalg.norm(oT) > 0
     g.close()
+
+
+def _read_dump(path):
+    """snapshots of a dump custom text file: [(step, columns, rows)]"""
+    snaps, lines, k = [], open(path).read().splitlines(), 0
+    while k < len(lines):
+        assert lines[k] == "ITEM: TIMESTEP"
+        step = int(lines[k + 1])
+        assert lines[k + 2] == "ITEM: NUMBER OF ATOMS"
+        n = int(lines[k + 3])
+        assert lines[k + 4].startswith("ITEM: BOX BOUNDS ")
+        cols = lines[k + 8].split()[2:]
+        rows = np.array([[float(v) for v in l.split()] for l in lines[k + 9:k + 9 + n]])
+        snaps.append((step, cols, rows, lines[k + 4][17:]))
+        k += 9 + n
+    return snaps
+
+
+def test_script_dump_custom_matches_api(tmp_path):
+    """dump custom every 100 steps (one file, and one file per snapshot with
+    '*'), across two runs: the snapshots sit at the right steps, in the
+    reference's layout (box bounds string, %g values, atoms by ID), and equal
+    the state of the same system driven through the C ABI."""
+    say("[gpu] script dump custom")
+    params, mats, P, walls = wl.settling_4k(seed=7, n=500)
+    data = tmp_path / "d.data"
+    wl.write_data_file(str(data), params, P)
+    d1 = tmp_path / "all.txt"
+    script = tmp_path / "in.dump"
+    script.write_text(f"""atom_style granular
+atom_modify map array
+boundary p p f
+newton off
+communicate single vel yes
+units si
+region reg block 0 0.1 0 0.1 0 0.2 units box
+create_box 1 reg
+neighbor 0.001 bin
+neigh_modify delay 0
+fix m1 all property/global youngsModulus peratomtype 5.e6
+fix m2 all property/global poissonsRatio peratomtype 0.45
+fix m3 all property/global coefficientRestitution peratomtypepair 1 0.3
+fix m4 all property/global coefficientFriction peratomtypepair 1 0.5
+pair_style gran model hertz tangential history
+pair_coeff * *
+timestep 0.00001
+fix gravi all gravity 9.81 vector 0.0 0.0 -1.0
+fix zwall all wall/gran model hertz tangential history primitive type 1 zplane 0.0
+read_data {data}
+fix integr all nve/sphere
+dump d1 all custom 100 {d1} id type x y z vx vy vz omegax fz radius mass
+dump d2 all custom 250 {tmp_path}/snap*.txt id x y z
+thermo_style custom step atoms ke
+thermo 1000
+run 300
+run 200
+""")
+    rc = pydem.run_script(str(script), str(tmp_path / "log"))
+    assert rc == 0, (tmp_path / "log").read_text()[-800:]
+    snaps = _read_dump(d1)
+    assert [s[0] for s in snaps] == [0, 100, 200, 300, 400, 500]
+    assert sorted(p.name for p in tmp_path.glob("snap*.txt")) == ["snap0.txt", "snap250.txt", "snap500.txt"]
+    step, cols, rows, bounds = snaps[-1]
+    assert bounds == "pp pp ff"
+    assert cols == ["id", "type", "x", "y", "z", "vx", "vy", "vz", "omegax", "fz", "radius", "mass"]
+    assert np.array_equal(rows[:, 0], np.arange(1, 501))
+    g = pydem.GpuSystem(params, mats, P, walls)
+    g.run(300)
+    g.run(200)
+    st = g.state()
+    order = np.argsort(P["tag"])
+    for j, (key, comp) in enumerate((("x", 0), ("x", 1), ("x", 2), ("v", 0), ("v", 1), ("v", 2),
+                                     ("omega", 0), ("f", 2))):
+        ref = st[key][order, comp]
+        assert np.allclose(rows[:, 2 + j], ref, rtol=1e-5, atol=1e-5 * np.abs(ref).max() + 1e-300), key
+    assert np.allclose(rows[:, 10], P["radius"][order], rtol=1e-5)
+    assert np.allclose(rows[:, 11], P["rmass"][order], rtol=1e-5)
+    s250 = _read_dump(tmp_path / "snap250.txt")
+    assert len(s250) == 1 and s250[0][0] == 250
+    g.close()

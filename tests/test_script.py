@@ -74,6 +74,10 @@ def test_full_command_subset_parses(tmp_path):
     ("fix m1 all property/global coefficientRestitution peratomtypepair 2 0.3\n", "n*n"),
     ("atom_style atomic\n", "atom_style"),
     ("frobnicate 3\n", "Unknown command"),
+    ("dump d1 all custom 100 post/d*.txt id type x y z bogus\n", "Invalid attribute bogus"),
+    ("dump d1 all atom 100 post/d.txt\n", "dump style atom"),
+    ("dump_modify d9 sort id\n", "Could not find dump_modify ID"),
+    ("undump d9\n", "Could not find undump ID"),
     ("fix x all insert/pack seed 1\n", "not available"),
 ])
 def test_unsupported_or_illegal_commands_are_errors(tmp_path, bad, msg):
@@ -161,3 +165,14 @@ def test_sibling_model_commands_parse(tmp_path):
         rc, log = run_script(tmp_path, text.replace("pair_coeff", props + "pair_coeff", 1) + f"read_data {data}\n")
         assert rc == 0, log
         assert "WARNING: property/global" not in log
+
+
+def test_dump_commands_parse(tmp_path):
+    out = tmp_path / "d.txt"
+    text = HEADER + (f"dump d1 all custom 100 {out} id type x y z vx vy vz fx fy fz omegax omegay omegaz "
+                     "tqx tqy tqz radius diameter mass\n"
+                     "dump_modify d1 sort id\n"
+                     f"dump d2 all custom 50 {tmp_path}/snap*.txt id x\nundump d2\n")
+    rc, log = run_script(tmp_path, text)
+    assert rc == 0, log
+    assert out.exists()  # opened at the dump command, as the reference does
