@@ -215,6 +215,20 @@ int dem_mesh_linear(dem_handle *h, int mesh_id, const double *velocity);
  * velocity times dt. One fix move/mesh per mesh. Call before dem_set_particles. */
 int dem_mesh_wiggle(dem_handle *h, int mesh_id, const double *amplitude, double period);
 
+/* fix move/mesh mesh <id> viblin axis ax ay az order n amplitude a1..an
+ * phase p1..pn period T1..Tn (MeshMoverVibLin, mesh_mover_linear.cpp:230-305):
+ * velocity axis * sum_j -a_j w_j sin(w_j t + p_j), w_j = 2 pi / T_j, applied
+ * as for wiggle. Arrays hold `order` (1..30) values. */
+int dem_mesh_viblin(dem_handle *h, int mesh_id, const double *axis, int order, const double *amplitude,
+                    const double *phase, const double *period);
+
+/* fix move/mesh mesh <id> vibrot origin ox oy oz axis ax ay az order n
+ * amplitude a1..an phase p1..pn period T1..Tn (MeshMoverVibRot,
+ * mesh_mover_rotation.cpp:333-430): rotation about the axis through origin
+ * with angular velocity sum_j -a_j w_j sin(w_j t + p_j). */
+int dem_mesh_vibrot(dem_handle *h, int mesh_id, const double *origin, const double *axis, int order,
+                    const double *amplitude, const double *phase, const double *period);
+
 /* Current mesh nodes and node velocities (ntri*9 each; either may be NULL). */
 int dem_get_mesh_nodes(dem_handle *h, int mesh_id, double *nodes, double *vnodes);
 

@@ -153,7 +153,12 @@ struct ListBuf {
 
 // Rigid rotation of the moving meshes for one step.
 // fix move/mesh movement types
-constexpr int kMoverRotate = 1, kMoverLinear = 2, kMoverWiggle = 3;
+constexpr int kMoverRotate = 1, kMoverLinear = 2, kMoverWiggle = 3, kMoverVibLin = 4,
+              kMoverVibRot = 5;
+// motion table of the time-dependent movers: per run step and mesh, the
+// incremental rotation quaternion (4), the angular velocity vector (3) and the
+// velocity factor of a translation (1), evaluated on the host
+constexpr int kMotionStride = 8;
 
 struct MeshMoveArgs {
   TriGeom *tri;               // geometry of the step before
@@ -166,9 +171,8 @@ struct MeshMoveArgs {
   double origin[kMaxMeshes][3];
   double omegaVec[kMaxMeshes][3];
   int trans[kMaxMeshes];
-  double tvec[kMaxMeshes][3];  // linear: velocity; wiggle: amplitude
-  double womega[kMaxMeshes];   // wiggle: 2 pi / period
-  const double *time_tab;      // wiggle: the mover's time at run step k is time_tab[k]
+  double tvec[kMaxMeshes][3];  // linear: velocity; wiggle: amplitude; viblin: unit axis
+  const double *motion;        // [run step][kMaxMeshes][kMotionStride] (wiggle, viblin, vibrot)
   double dt;
   double triggersq;
   long long step;             // the step the moved geometry belongs to

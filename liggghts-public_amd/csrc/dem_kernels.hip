@@ -505,20 +505,21 @@ __device__ inline void mesh_contacts(const StepArgs &a, int i, const double4 &pi
 // (multi_node_mesh_I.h:620-660) and MultiNodeMesh::decideRebuild
 // (multi_node_mesh_I.h:792-825)
 // ---------------------------------------------------------------------------
-// translating movers (MeshMoverLinear / MeshMoverWiggle::initial_integrate,
-// mesh_mover_linear.cpp:88-101, 203-218): nodes and centre shifted by
-// v dt (TrackingMesh::move -> MultiNodeMesh::move, multi_node_mesh_I.h:
-// 502-527; edge vectors and normals are translation invariant), node
-// velocities reset to v (FixMoveMesh::initial_integrate zeroes them first)
+// translating movers (MeshMoverLinear / MeshMoverWiggle / MeshMoverVibLin::
+// initial_integrate, mesh_mover_linear.cpp:88-101, 203-218, 275-305): nodes
+// and centre shifted by v dt (TrackingMesh::move -> MultiNodeMesh::move,
+// multi_node_mesh_I.h:502-527; edge vectors and normals are translation
+// invariant), node velocities reset to v (FixMoveMesh::initial_integrate
+// zeroes them first). Wiggle: v = A (w cos(w t)); viblin: v = axis vA, the
+// factors of run step `step` from the motion table.
 __device__ __forceinline__ void mesh_translate_tri(const MeshMoveArgs &a, const TriGeom &src,
                                                    TriGeom &dst, const int m, const long long step) {
   double vN[3];
-  if (a.kind[m] == kMoverWiggle) {
-    const double T = a.time_tab[step];
-    const double sc = a.womega[m] * cos(a.womega[m] * T);
-    for (int k = 0; k < 3; k++) vN[k] = a.tvec[m][k] * sc;
-  } else {
+  if (a.kind[m] == kMoverLinear) {
     for (int k = 0; k < 3; k++) vN[k] = a.tvec[m][k];
+  } else {
+    const double sc = a.motion[(step * kMaxMeshes + m) * kMotionStride + 7];
+    for (int k = 0; k < 3; k++) vN[k] = a.tvec[m][k] * sc;
   }
   double dx[3];
   for (int k = 0; k < 3; k++) dx[k] = vN[k] * a.dt;
@@ -548,7 +549,7 @@ __device__ __forceinline__ bool mesh_move_tri(const MeshMoveArgs &a, const int t
   const int m = src.mesh;
   if (!((a.moving_mask >> m) & 1)) return false;
   TriGeom &dst = a.dst ? a.dst[t] : a.tri[t];
-  if (a.kind[m] != kMoverRotate) {
+  if (a.kind[m] != kMoverRotate && a.kind[m] != kMoverVibRot) {
     mesh_translate_tri(a, src, dst, m, step);
     bool moved = false;
     if (a.decide)
@@ -560,7 +561,13 @@ __device__ __forceinline__ bool mesh_move_tri(const MeshMoveArgs &a, const int t
       }
     return moved;
   }
-  const double *dQ = a.dQ[m];
+  // rotate: constant increment; vibrot (MeshMoverVibRot::initial_integrate,
+  // mesh_mover_rotation.cpp:397-430): increment vR dt and angular velocity
+  // axis vR of this step from the motion table
+  const bool vib = a.kind[m] == kMoverVibRot;
+  const double *mot = vib ? a.motion + (step * kMaxMeshes + m) * kMotionStride : nullptr;
+  const double *dQ = vib ? mot : a.dQ[m];
+  const double *omegaVec = vib ? mot + 4 : a.omegaVec[m];
   const double *o = a.origin[m];
   double c[3] = {0., 0., 0.};
   double nd[9];
@@ -598,7 +605,7 @@ __device__ __forceinline__ bool mesh_move_tri(const MeshMoveArgs &a, const int t
   for (int j = 0; j < 3; j++) {
     double rPA[3], vRot[3];
     d3_sub(&nd[3 * j], o, rPA);
-    d3_cross(a.omegaVec[m], rPA, vRot);
+    d3_cross(omegaVec, rPA, vRot);
     for (int k = 0; k < 3; k++) dst.vn[3 * j + k] = 0.0 + vRot[k];
   }
   bool moved = false;

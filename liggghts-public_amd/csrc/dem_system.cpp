@@ -31,6 +31,7 @@ using dem::StateBuf;
 namespace {
 
 constexpr long long kNever = 0x3fffffffffffffffLL;
+constexpr int64_t kMotionSteps = 65536;  // steps per run piece with time-dependent movers
 constexpr double kMyPi = 3.14159265358979323846;  // math_const.h
 
 struct Wall {
@@ -96,10 +97,11 @@ struct dem_handle {
   int tri_cur = 0;
   bool pending_mesh_rebuild = false;  // a moving mesh asked for a rebuild at the next step
   double *d_node_last = nullptr;
-  // wiggle movers: the FixMoveMesh time (time_ += dt each step, over the
-  // fix's lifetime) at every step of the current run, and its value so far
-  double *d_mesh_time = nullptr;
-  int64_t mesh_time_cap = 0;
+  // time-dependent movers (wiggle, viblin, vibrot): their motion at every
+  // step of the current run (dem::kMotionStride values per mesh), and the
+  // FixMoveMesh time so far (time_ += dt each step over the fix's lifetime)
+  double *d_motion = nullptr;
+  int64_t motion_cap = 0;  // run steps the device table holds
   double mesh_time = 0.0;
   int *d_copl_off = nullptr, *d_copl_idx = nullptr;
   int *d_tcount = nullptr, *d_toff = nullptr;
@@ -217,8 +219,8 @@ void free_particles(dem_handle *h) {
   h->xhold = h->fout = nullptr;
   h->d_tri_buf[0] = h->d_tri_buf[1] = nullptr;
   h->d_node_last = nullptr;
-  h->d_mesh_time = nullptr;
-  h->mesh_time_cap = 0;
+  h->d_motion = nullptr;
+  h->motion_cap = 0;
   h->d_copl_off = h->d_copl_idx = nullptr;
   h->d_tcount = h->d_toff = h->d_tpair_cell = h->d_tpair_tri = h->d_tcell_list = nullptr;
   h->tpair_cap = 0;
@@ -442,26 +444,66 @@ bool any_moving_mesh(const dem_handle *h) {
   return false;
 }
 
-// the wiggle movers' time at run steps 0..nsteps, accumulated as
-// FixMoveMesh::initial_integrate does (time_ += dt, fix_move_mesh.cpp:226)
-int upload_mesh_time(dem_handle *h, int64_t nsteps, std::vector<double> &tab) {
-  tab.assign((size_t)nsteps + 1, h->mesh_time);
-  for (int64_t k = 1; k <= nsteps; k++) tab[k] = tab[k - 1] + h->p.dt;
-  if (nsteps + 1 > h->mesh_time_cap) {
-    clear_graphs(h);  // captured launches hold the old table pointer
-    if (h->d_mesh_time) {
-      hipFree(h->d_mesh_time);
-      h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), (void *)h->d_mesh_time),
-                      h->allocs.end());
-      h->d_mesh_time = nullptr;
+bool has_timed_movers(const dem_handle *h) {
+  for (const auto &m : h->meshes)
+    if (m.mover == dem::kMoverWiggle || m.mover == dem::kMoverVibLin || m.mover == dem::kMoverVibRot)
+      return true;
+  return false;
+}
+
+// The time-dependent movers' motion at run steps 1..nsteps, evaluated here
+// (host libm, as the reference evaluates it): the mover's time t is
+// accumulated as FixMoveMesh::initial_integrate does (time_ += dt,
+// fix_move_mesh.cpp:226) and
+//   wiggle  w cos(w t)                                  (mesh_mover_linear.cpp:203-218)
+//   viblin  vA = sum_j -a_j w_j sin(w_j t + phi_j)       (:275-305)
+//   vibrot  vR likewise; rotation by vR dt about the axis through the origin,
+//           angular velocity axis vR                     (mesh_mover_rotation.cpp:397-430)
+// Returns the time after the last step in *t_end.
+int upload_mesh_motion(dem_handle *h, int64_t nsteps, double *t_end) {
+  constexpr int S = dem::kMotionStride, M = dem::kMaxMeshes;
+  std::vector<double> tab((size_t)(nsteps + 1) * M * S, 0.0);
+  double T = h->mesh_time;
+  for (int64_t k = 1; k <= nsteps; k++) {
+    T = T + h->p.dt;
+    for (size_t m = 0; m < h->meshes.size(); m++) {
+      const dem::HostMesh &hm = h->meshes[m];
+      double *e = &tab[((size_t)k * M + m) * S];
+      if (hm.mover == dem::kMoverWiggle) {
+        e[7] = hm.womega * std::cos(hm.womega * T);
+      } else if (hm.mover == dem::kMoverVibLin || hm.mover == dem::kMoverVibRot) {
+        double vA = 0.0;
+        for (int j = 0; j < hm.vord; j++)
+          vA = vA - hm.vampl[j] * hm.vomega[j] * std::sin(hm.vomega[j] * T + hm.vphi[j]);
+        e[7] = vA;
+        if (hm.mover == dem::kMoverVibRot) {
+          // MultiNodeMesh::rotate(dAngle, axis, p), multi_node_mesh_I.h:620-641
+          const double dAngle = vA * h->p.dt;
+          double an[3] = {hm.axis[0], hm.axis[1], hm.axis[2]};
+          const double sinv = 1. / std::sqrt(an[0] * an[0] + an[1] * an[1] + an[2] * an[2]);
+          for (int q = 0; q < 3; q++) an[q] = sinv * an[q];
+          e[0] = std::cos(dAngle * 0.5);
+          for (int q = 0; q < 3; q++) e[1 + q] = an[q] * std::sin(dAngle * 0.5);
+          for (int q = 0; q < 3; q++) e[4 + q] = hm.axis[q] * vA;
+        }
+      }
     }
-    const int64_t cap = std::max<int64_t>(nsteps + 1, 2 * h->mesh_time_cap);
-    int rc = dalloc(h, (void **)&h->d_mesh_time, sizeof(double) * (size_t)cap);
+  }
+  *t_end = T;
+  if (nsteps + 1 > h->motion_cap) {
+    clear_graphs(h);  // captured launches hold the old table pointer
+    if (h->d_motion) {
+      hipFree(h->d_motion);
+      h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), (void *)h->d_motion), h->allocs.end());
+      h->d_motion = nullptr;
+    }
+    const int64_t cap = std::max<int64_t>(nsteps + 1, std::min<int64_t>(2 * h->motion_cap, kMotionSteps + 1));
+    int rc = dalloc(h, (void **)&h->d_motion, sizeof(double) * (size_t)cap * M * S);
     if (rc) return rc;
-    h->mesh_time_cap = cap;
+    h->motion_cap = cap;
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));  // the previous run's launches are done with it
-  HIPCHK(h, hipMemcpy(h->d_mesh_time, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->d_motion, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
   return DEM_OK;
 }
 
@@ -477,16 +519,23 @@ dem::MeshMoveArgs mesh_move_args(const dem_handle *h, long long step, int guard,
   a.guard = guard;
   a.decide = decide;
   a.ctrl = h->ctrl;
-  a.time_tab = h->d_mesh_time;
+  a.motion = h->d_motion;
   a.dt = h->p.dt;
   for (size_t m = 0; m < h->meshes.size(); m++) {
     const dem::HostMesh &hm = h->meshes[m];
     if (!hm.moving) continue;
     a.moving_mask |= 1 << m;
     a.kind[m] = hm.mover;
-    if (hm.mover != dem::kMoverRotate) {
-      for (int k = 0; k < 3; k++) a.tvec[m][k] = hm.mover == dem::kMoverLinear ? hm.vel[k] : hm.ampl[k];
-      a.womega[m] = hm.womega;
+    if (hm.mover == dem::kMoverLinear || hm.mover == dem::kMoverWiggle || hm.mover == dem::kMoverVibLin) {
+      for (int k = 0; k < 3; k++)
+        a.tvec[m][k] = hm.mover == dem::kMoverLinear ? hm.vel[k]
+                       : hm.mover == dem::kMoverWiggle ? hm.ampl[k] : hm.axis[k];
+      continue;
+    }
+    if (hm.mover == dem::kMoverVibRot) {  // increments come from the motion table
+      for (int k = 0; k < 3; k++) a.origin[m][k] = hm.origin[k];
+      a.trans[m] = (hm.origin[0] * hm.origin[0] + hm.origin[1] * hm.origin[1] +
+                    hm.origin[2] * hm.origin[2]) > 0.;
       continue;
     }
     // MultiNodeMesh::rotate(dAngle, axis, p), multi_node_mesh_I.h:620-641
@@ -1023,6 +1072,58 @@ int dem_mesh_linear(dem_handle *h, int mesh_id, const double *velocity) {
   return DEM_OK;
 }
 
+int dem_mesh_viblin(dem_handle *h, int mesh_id, const double *axis, int order, const double *amplitude,
+                    const double *phase, const double *period) {
+  if (!h) return DEM_ERR_ARG;
+  if (mesh_id < 0 || mesh_id >= (int)h->meshes.size() || !axis || !amplitude || !phase || !period)
+    return fail(h, DEM_ERR_ARG, "bad move/mesh viblin arguments");
+  if (order < 1 || order > 30) return fail(h, DEM_ERR_ARG, "order can be at most 30 and must be greater 0");
+  if (h->n > 0) return fail(h, DEM_ERR_STATE, "set mesh motion before particles");
+  dem::HostMesh &m = h->meshes[mesh_id];
+  if (m.moving) return fail(h, DEM_ERR_ARG, "one fix move/mesh per mesh on this path");
+  // MeshMoverVibLin (mesh_mover_linear.cpp:230-263): axis normalized, omega_j = 2 pi / period_j
+  const double norm = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+  const double invnorm = (norm == 0.) ? 0. : 1. / norm;
+  for (int k = 0; k < 3; k++) m.axis[k] = axis[k] * invnorm;
+  m.vord = order;
+  for (int j = 0; j < order; j++) {
+    m.vampl[j] = amplitude[j];
+    m.vphi[j] = phase[j];
+    m.vomega[j] = 2. * kMyPi / period[j];
+  }
+  m.moving = true;
+  m.mover = dem::kMoverVibLin;
+  return DEM_OK;
+}
+
+int dem_mesh_vibrot(dem_handle *h, int mesh_id, const double *origin, const double *axis, int order,
+                    const double *amplitude, const double *phase, const double *period) {
+  if (!h) return DEM_ERR_ARG;
+  if (mesh_id < 0 || mesh_id >= (int)h->meshes.size() || !origin || !axis || !amplitude || !phase ||
+      !period)
+    return fail(h, DEM_ERR_ARG, "bad move/mesh vibrot arguments");
+  if (order < 1 || order > 30) return fail(h, DEM_ERR_ARG, "order can be at most 30 and must be greater 0");
+  if (h->n > 0) return fail(h, DEM_ERR_STATE, "set mesh motion before particles");
+  dem::HostMesh &m = h->meshes[mesh_id];
+  if (m.moving) return fail(h, DEM_ERR_ARG, "one fix move/mesh per mesh on this path");
+  // MeshMoverVibRot (mesh_mover_rotation.cpp:333-375)
+  const double norm = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+  const double invnorm = (norm == 0.) ? 0. : 1. / norm;
+  for (int k = 0; k < 3; k++) {
+    m.origin[k] = origin[k];
+    m.axis[k] = axis[k] * invnorm;
+  }
+  m.vord = order;
+  for (int j = 0; j < order; j++) {
+    m.vampl[j] = amplitude[j];
+    m.vphi[j] = phase[j];
+    m.vomega[j] = 2. * kMyPi / period[j];
+  }
+  m.moving = true;
+  m.mover = dem::kMoverVibRot;
+  return DEM_OK;
+}
+
 int dem_mesh_wiggle(dem_handle *h, int mesh_id, const double *amplitude, double period) {
   if (!h) return DEM_ERR_ARG;
   if (mesh_id < 0 || mesh_id >= (int)h->meshes.size() || !amplitude || !(period != 0.0))
@@ -1373,7 +1474,23 @@ void finish_tuning(dem_handle *h) {
   tu.active = false;
 }
 
+int run_steps_piece(dem_handle *h, int64_t nsteps);
+
+// A run with time-dependent movers goes in pieces of at most kMotionSteps
+// steps (bitwise the same as one run: the pieces continue as `run pre no`),
+// so their motion table stays a bounded size.
 int run_steps(dem_handle *h, int64_t nsteps) {
+  if (!has_timed_movers(h)) return run_steps_piece(h, nsteps);
+  for (int64_t done = 0; done < nsteps;) {
+    const int64_t k = std::min<int64_t>(kMotionSteps, nsteps - done);
+    int rc = run_steps_piece(h, k);
+    if (rc) return rc;
+    done += k;
+  }
+  return DEM_OK;
+}
+
+int run_steps_piece(dem_handle *h, int64_t nsteps) {
   int rc;
   if (nsteps == 0) return DEM_OK;
   h->setup_done = false;
@@ -1385,10 +1502,9 @@ int run_steps(dem_handle *h, int64_t nsteps) {
   h->pending_mesh_rebuild = false;
   const bool moving = any_moving_mesh(h);
   const int64_t ts0 = h->ntimestep;  // step t of this run is timestep ts0 + t
-  bool wiggle = false;
-  for (const auto &m : h->meshes) wiggle = wiggle || m.mover == dem::kMoverWiggle;
-  std::vector<double> time_tab;
-  if (wiggle && (rc = upload_mesh_time(h, nsteps, time_tab))) return rc;
+  const bool timed = has_timed_movers(h);
+  double t_end = h->mesh_time;
+  if (timed && (rc = upload_mesh_motion(h, nsteps, &t_end))) return rc;
   {
     dem::StepArgs a = step_args(h, h->cur_st, h->tri_cur);
     a.ago_next = ts0 + 1 - h->last_build;
@@ -1489,7 +1605,7 @@ int run_steps(dem_handle *h, int64_t nsteps) {
     }
   }
   h->tri_cur = tc;
-  if (wiggle) h->mesh_time = time_tab[(size_t)nsteps];
+  if (timed) h->mesh_time = t_end;
   h->pending_mesh_rebuild = h->ctrl_host->stop_at == nsteps + 1;
   h->ncontacts = (int64_t)(h->ctrl_host->ncontact_full / 2);
   h->nwall_contacts = (int64_t)h->ctrl_host->nwall;

@@ -32,6 +32,10 @@ struct HostMesh {
   double origin[3] = {0, 0, 0}, axis[3] = {0, 0, 1}, omega = 0.0;  // rotate
   double vel[3] = {0, 0, 0};                                       // linear: velocity
   double ampl[3] = {0, 0, 0}, womega = 0.0;                        // wiggle
+  // viblin (unit axis in `axis`) and vibrot (origin, unit axis): the order
+  // and per-term amplitude, phase, angular frequency 2 pi / period
+  int vord = 0;
+  double vampl[30] = {}, vphi[30] = {}, vomega[30] = {};
   bool stress = false;          // fix mesh/surface/stress: total force and torque tracked
   double p_ref[3] = {0, 0, 0};  // its reference_point
 };

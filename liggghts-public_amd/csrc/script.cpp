@@ -31,6 +31,8 @@
 //                  meshes <id1> ... <idk>
 //   fix <id> <grp> move/mesh mesh <meshid> rotate origin ox oy oz axis ax ay az period T
 //                  | linear vx vy vz | wiggle amplitude ax ay az period T
+//                  | viblin axis x y z order n amplitude .. phase .. period ..
+//                  | vibrot origin x y z axis x y z order n amplitude .. phase .. period ..
 //   fix <id> <grp> nve/sphere
 //   atom_style superquadric  (read_data Atoms: id type density shapex shapey shapez
 //                             blockiness1 blockiness2 x y z), with
@@ -167,8 +169,9 @@ struct Interp {
     bool used = false;
     dem_contact_model model{};
     double origin[3] = {0, 0, 0}, axis[3] = {0, 0, 1}, period = 1.0;
-    int mover = 0;  // 0 none, 1 rotate, 2 linear, 3 wiggle
+    int mover = 0;  // 0 none, 1 rotate, 2 linear, 3 wiggle, 4 viblin, 5 vibrot
     double vec[3] = {0, 0, 0};  // linear: velocity; wiggle: amplitude
+    std::vector<double> vampl, vphase, vperiod;  // viblin / vibrot modes
     bool stress = false;        // fix mesh/surface/stress (stress on)
     double pref[3] = {0, 0, 0};  // its reference_point
   };
@@ -867,6 +870,12 @@ struct Interp {
       if (md.mover == 1 && dem_mesh_rotate(h, id, md.origin, md.axis, md.period)) die(dem_last_error(h));
       if (md.mover == 2 && dem_mesh_linear(h, id, md.vec)) die(dem_last_error(h));
       if (md.mover == 3 && dem_mesh_wiggle(h, id, md.vec, md.period)) die(dem_last_error(h));
+      if (md.mover == 4 && dem_mesh_viblin(h, id, md.axis, (int)md.vampl.size(), md.vampl.data(),
+                                           md.vphase.data(), md.vperiod.data()))
+        die(dem_last_error(h));
+      if (md.mover == 5 && dem_mesh_vibrot(h, id, md.origin, md.axis, (int)md.vampl.size(), md.vampl.data(),
+                                           md.vphase.data(), md.vperiod.data()))
+        die(dem_last_error(h));
       if (md.stress && dem_mesh_stress(h, id, md.pref)) die(dem_last_error(h));
       mesh_ids[name] = id;
     }
@@ -1173,8 +1182,39 @@ struct Interp {
           for (int d = 0; d < 3; d++) md.vec[d] = num(t[8 + d]);
           md.period = num(t[12]);
           md.mover = 3;
+        } else if (t[6] == "viblin" || t[6] == "vibrot") {
+          // viblin axis x y z order n amplitude a.. phase p.. period T..
+          // vibrot origin x y z axis x y z order n amplitude ... (same tail)
+          const bool rot = t[6] == "vibrot";
+          size_t k = 7;
+          if (rot) {
+            if (t.size() < k + 4 || t[k] != "origin") die("Expected keyword 'origin'");
+            for (int d = 0; d < 3; d++) md.origin[d] = num(t[k + 1 + d]);
+            k += 4;
+          }
+          if (t.size() < k + 6 || t[k] != "axis") die("Expected keyword 'axis'");
+          for (int d = 0; d < 3; d++) md.axis[d] = num(t[k + 1 + d]);
+          k += 4;
+          if (t[k] != "order") die("Expected keyword 'order'");
+          const int ord = inum(t[k + 1]);
+          if (ord > 30 || ord < 1) die("order can be at most 30 and must be greater 0");
+          k += 2;
+          if (t.size() < k + 3 + 3 * (size_t)ord) die("Not enough arguments for movement type " + t[6]);
+          if (t[k] != "amplitude") die("Expected keyword 'amplitude'");
+          if (t[k + 1 + ord] != "phase") die("Expected keyword 'phase'");
+          if (t[k + 2 + 2 * ord] != "period") die("Expected keyword 'period'");
+          md.vampl.clear();
+          md.vphase.clear();
+          md.vperiod.clear();
+          for (int j = 0; j < ord; j++) {
+            md.vampl.push_back(num(t[k + 1 + j]));
+            md.vphase.push_back(num(t[k + 2 + ord + j]));
+            md.vperiod.push_back(num(t[k + 3 + 2 * ord + j]));
+          }
+          md.mover = rot ? 5 : 4;
         } else {
-          die("fix move/mesh: movement " + t[6] + " is not available on this path (rotate, linear, wiggle)");
+          die("fix move/mesh: movement " + t[6] +
+              " is not available on this path (rotate, linear, wiggle, viblin, vibrot)");
         }
         dirty = true;
       } else if (style == "wall/gran" && t.size() > 4 &&

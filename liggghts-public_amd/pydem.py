@@ -22,7 +22,7 @@ EXPORTED = [
     "dem_set_particles", "dem_add_plane_wall", "dem_setup", "dem_run",
     "dem_run_continue", "dem_compute_forces", "dem_get_state", "dem_get_pairs",
     "dem_get_wall_history", "dem_get_stats", "dem_set_kernel_timing",
-    "dem_run_script", "dem_add_mesh", "dem_mesh_rotate", "dem_get_mesh_nodes", "dem_mesh_linear", "dem_mesh_wiggle", "dem_mesh_stress", "dem_get_mesh_force",
+    "dem_run_script", "dem_add_mesh", "dem_mesh_rotate", "dem_get_mesh_nodes", "dem_mesh_linear", "dem_mesh_wiggle", "dem_mesh_stress", "dem_get_mesh_force", "dem_mesh_viblin", "dem_mesh_vibrot",
     "dem_get_mesh_topology", "dem_get_mesh_history", "dem_set_instances",
     "dem_set_instance_materials", "dem_get_instance_stats", "dem_set_step_shape",
     "dem_get_step_shape", "dem_set_graph_capture", "dem_set_pair_model", "dem_set_model_coefficients",
@@ -174,6 +174,8 @@ def load_library(path=LIB_PATH):
     lib.dem_mesh_rotate.argtypes = [h, ctypes.c_int, dp, dp, ctypes.c_double]
     lib.dem_mesh_linear.argtypes = [h, ctypes.c_int, dp]
     lib.dem_mesh_stress.argtypes = [h, ctypes.c_int, dp]
+    lib.dem_mesh_viblin.argtypes = [h, ctypes.c_int, dp, ctypes.c_int, dp, dp, dp]
+    lib.dem_mesh_vibrot.argtypes = [h, ctypes.c_int, dp, dp, ctypes.c_int, dp, dp, dp]
     lib.dem_get_mesh_force.argtypes = [h, ctypes.c_int, dp, dp]
     lib.dem_mesh_wiggle.argtypes = [h, ctypes.c_int, dp, ctypes.c_double]
     lib.dem_get_mesh_nodes.argtypes = [h, ctypes.c_int, dp, dp]
@@ -272,7 +274,9 @@ class GpuSystem:
                  max_neighbors=0, device=0, meshes=(), instances=None, instance_materials=None):
         """meshes: [(tris (T,3,3), wall_type, model dict or None, rotate or None)],
         rotate = (origin, axis, period) for fix move/mesh rotate, or
-        {"linear": (vx, vy, vz)} or {"wiggle": ((ax, ay, az), period)}; an optional
+        {"linear": (vx, vy, vz)}, {"wiggle": ((ax, ay, az), period)},
+        {"viblin": (axis, amplitudes, phases, periods)} or {"vibrot": (origin,
+        axis, amplitudes, phases, periods)}; an optional
         fifth entry (a reference point) makes the mesh a mesh/surface/stress.
         instances: particle counts of independent simulations batched in this
         system (particles concatenated in that order); instance_materials: one
@@ -341,6 +345,16 @@ class GpuSystem:
             elif isinstance(rot, dict) and "wiggle" in rot:
                 a = np.ascontiguousarray(rot["wiggle"][0], dtype=np.float64)
                 self._check(self.lib.dem_mesh_wiggle(self.h, mid.value, _dp(a), float(rot["wiggle"][1])))
+            elif isinstance(rot, dict) and ("viblin" in rot or "vibrot" in rot):
+                args = [np.ascontiguousarray(v, dtype=np.float64) for v in rot.get("viblin", rot.get("vibrot"))]
+                if "viblin" in rot:
+                    ax, am, ph, pe = args
+                    self._check(self.lib.dem_mesh_viblin(self.h, mid.value, _dp(ax), len(am), _dp(am), _dp(ph),
+                                                         _dp(pe)))
+                else:
+                    o, ax, am, ph, pe = args
+                    self._check(self.lib.dem_mesh_vibrot(self.h, mid.value, _dp(o), _dp(ax), len(am), _dp(am),
+                                                         _dp(ph), _dp(pe)))
             elif rot is not None:
                 o = np.ascontiguousarray(rot[0], dtype=np.float64)
                 a = np.ascontiguousarray(rot[1], dtype=np.float64)

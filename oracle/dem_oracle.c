@@ -444,6 +444,8 @@ typedef struct {
   double origin[3], axis[3], omega; /* rotate */
   double vel[3];                     /* linear */
   double ampl[3], womega;            /* wiggle */
+  int vord;                          /* viblin (axis) / vibrot (origin, axis) */
+  double vampl[30], vphi[30], vomega[30];
   double time;                       /* FixMoveMesh::time_ */
   int stress;                        /* mesh/surface/stress */
   double p_ref[3], ftot[3], ttot[3]; /* reference point, total force and torque on the mesh */
@@ -1379,6 +1381,36 @@ void orc_mesh_linear(orc_sys *s, int imesh, const double *vel) {
   memcpy(m->vel, vel, 3 * sizeof(double));
 }
 
+/* MeshMoverVibLin (mesh_mover_linear.cpp:230-263) and MeshMoverVibRot
+ * (mesh_mover_rotation.cpp:333-375): unit axis, omega_j = 2 pi / period_j */
+static void set_vib(orc_mesh *m, const double *axis, int order, const double *amplitude,
+                    const double *phase, const double *period) {
+  const double norm = sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+  const double invnorm = (norm == 0.) ? 0. : 1. / norm;
+  for (int k = 0; k < 3; k++) m->axis[k] = axis[k] * invnorm;
+  m->vord = order;
+  for (int j = 0; j < order; j++) {
+    m->vampl[j] = amplitude[j];
+    m->vphi[j] = phase[j];
+    m->vomega[j] = 2. * MY_PI / period[j];
+  }
+}
+
+void orc_mesh_viblin(orc_sys *s, int imesh, const double *axis, int order, const double *amplitude,
+                     const double *phase, const double *period) {
+  orc_mesh *m = &s->mesh[imesh];
+  m->moving = 4;
+  set_vib(m, axis, order, amplitude, phase, period);
+}
+
+void orc_mesh_vibrot(orc_sys *s, int imesh, const double *origin, const double *axis, int order,
+                     const double *amplitude, const double *phase, const double *period) {
+  orc_mesh *m = &s->mesh[imesh];
+  m->moving = 5;
+  memcpy(m->origin, origin, 3 * sizeof(double));
+  set_vib(m, axis, order, amplitude, phase, period);
+}
+
 /* MeshMoverWiggle, mesh_mover_linear.cpp:160-183 */
 void orc_mesh_wiggle(orc_sys *s, int imesh, const double *amplitude, double period) {
   orc_mesh *m = &s->mesh[imesh];
@@ -1727,7 +1759,12 @@ static void mesh_move(orc_sys *s) {
     orc_mesh *m = &s->mesh[im];
     if (!m->moving) continue;
     m->time += s->p.dt; /* time_ += update->dt */
-    if (m->moving != 1) {
+    /* viblin / vibrot: vA = sum_j -a_j w_j sin(w_j t + phi_j) */
+    double vA = 0.0;
+    if (m->moving >= 4)
+      for (int j = 0; j < m->vord; j++)
+        vA = vA - m->vampl[j] * m->vomega[j] * sin(m->vomega[j] * m->time + m->vphi[j]);
+    if (m->moving != 1 && m->moving != 5) {
       /* translation: v_node reset to zero, vNode = vel_ or
        * amplitude_ * (omega_ cos(omega_ time_)), dx = vNode dt; nodes and
        * centre += dx (MultiNodeMesh::move, multi_node_mesh_I.h:502-527) */
@@ -1735,6 +1772,8 @@ static void mesh_move(orc_sys *s) {
       if (m->moving == 3) {
         const double sc = m->womega * cos(m->womega * m->time);
         for (int k = 0; k < 3; k++) vN[k] = m->ampl[k] * sc;
+      } else if (m->moving == 4) {
+        for (int k = 0; k < 3; k++) vN[k] = m->axis[k] * vA;
       } else {
         for (int k = 0; k < 3; k++) vN[k] = m->vel[k];
       }
@@ -1748,7 +1787,8 @@ static void mesh_move(orc_sys *s) {
       }
       continue;
     }
-    const double dAngle = m->omega * s->p.dt;
+    /* rotate: omega dt; vibrot: vR dt (mesh_mover_rotation.cpp:397-430) */
+    const double dAngle = (m->moving == 5 ? vA : m->omega) * s->p.dt;
     double axisNorm[3] = {m->axis[0], m->axis[1], m->axis[2]};
     v3_div(axisNorm, v3_mag(axisNorm));
     double dQ[4];
@@ -1777,7 +1817,7 @@ static void mesh_move(orc_sys *s) {
     }
     /* node velocity: reset, then v += (omega axis) x (node - origin) */
     double omegaVec[3];
-    for (int k = 0; k < 3; k++) omegaVec[k] = m->axis[k] * m->omega;
+    for (int k = 0; k < 3; k++) omegaVec[k] = m->axis[k] * (m->moving == 5 ? vA : m->omega);
     for (int t = 0; t < m->ntri; t++)
       for (int j = 0; j < 3; j++) {
         double rPA[3], vRot[3];

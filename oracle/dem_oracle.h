@@ -157,6 +157,10 @@ void orc_mesh_stress(orc_sys *s, int imesh, const double *p_ref);
 void orc_mesh_force(const orc_sys *s, int imesh, double *force, double *torque);
 void orc_mesh_linear(orc_sys *s, int imesh, const double *vel);
 void orc_mesh_wiggle(orc_sys *s, int imesh, const double *amplitude, double period);
+void orc_mesh_viblin(orc_sys *s, int imesh, const double *axis, int order, const double *amplitude,
+                     const double *phase, const double *period);
+void orc_mesh_vibrot(orc_sys *s, int imesh, const double *origin, const double *axis, int order,
+                     const double *amplitude, const double *phase, const double *period);
 void orc_mesh_rotate(orc_sys *s, int imesh, const double *origin, const double *axis,
                      double period);
 /* topology outputs: edge_active/corner_active ntri*3 (0/1), obtuse ntri (-1 = none) */

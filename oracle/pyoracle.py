@@ -97,6 +97,8 @@ def load():
     lib.orc_mesh_stress.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
     lib.orc_mesh_force.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
     lib.orc_mesh_wiggle.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, _D]
+    lib.orc_mesh_viblin.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, ctypes.c_int, dp, dp, dp]
+    lib.orc_mesh_vibrot.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, ctypes.c_int, dp, dp, dp]
     lib.orc_mesh_topology.argtypes = [ctypes.c_void_p, ctypes.c_int, ip, ip, ip]
     lib.orc_mesh_nodes.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
     lib.orc_mesh_history.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ip, dp]
@@ -250,8 +252,10 @@ class OracleSystem:
     def __init__(self, params, mat, particles, walls=(), meshes=()):
         """meshes: [(tris (T,3,3), wall_type, model dict or None, motion or None
         [, stress reference point])], motion = (origin, axis, period) for fix
-        move/mesh rotate, or {"linear": (vx, vy, vz)} or {"wiggle": ((ax, ay, az),
-        period)}; a fifth entry makes the mesh a mesh/surface/stress."""
+        move/mesh rotate, or {"linear": (vx, vy, vz)}, {"wiggle": ((ax, ay, az),
+        period)}, {"viblin": (axis, amplitudes, phases, periods)} or
+        {"vibrot": (origin, axis, amplitudes, phases, periods)}; a fifth entry
+        makes the mesh a mesh/surface/stress."""
         self.lib = load()
         self.p = make_params(params, walls)
         self.m = materials(mat)
@@ -278,6 +282,14 @@ class OracleSystem:
             elif isinstance(rot, dict) and "wiggle" in rot:
                 a = np.ascontiguousarray(rot["wiggle"][0], dtype=np.float64)
                 self.lib.orc_mesh_wiggle(self.s, k, _dp(a), float(rot["wiggle"][1]))
+            elif isinstance(rot, dict) and ("viblin" in rot or "vibrot" in rot):
+                args = [np.ascontiguousarray(v, dtype=np.float64) for v in rot.get("viblin", rot.get("vibrot"))]
+                if "viblin" in rot:
+                    ax, am, ph, pe = args
+                    self.lib.orc_mesh_viblin(self.s, k, _dp(ax), len(am), _dp(am), _dp(ph), _dp(pe))
+                else:
+                    o, ax, am, ph, pe = args
+                    self.lib.orc_mesh_vibrot(self.s, k, _dp(o), _dp(ax), len(am), _dp(am), _dp(ph), _dp(pe))
             elif rot is not None:
                 o = np.ascontiguousarray(rot[0], dtype=np.float64)
                 a = np.ascontiguousarray(rot[1], dtype=np.float64)

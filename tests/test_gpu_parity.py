@@ -373,6 +373,11 @@ def _mesh_floor_pack(seed=41, n=2000, moving=False):
         rot = {"linear": (0.2, -0.1, 0.05)}
     elif moving == "wiggle":
         rot = {"wiggle": ((0.0001, 0.0, 0.0003), 0.02)}
+    elif moving == "viblin":
+        rot = {"viblin": ((0.2, 0.0, 1.0), (0.0002, 0.0001), (0.0, 1.0), (0.02, 0.007))}
+    elif moving == "vibrot":
+        rot = {"vibrot": ((0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.0), (0.0, 0.3, 1.0), (0.4,), (0.2,),
+                          (0.01,))}
     elif moving:
         rot = ((0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.0), (0.0, 0.0, 1.0), 0.5)
     else:
@@ -400,7 +405,7 @@ def _compare_mesh_history(g, o, tol):
     return int((ot >= 0).sum())
 
 
-@pytest.mark.parametrize("moving", [False, True, "linear", "wiggle"])
+@pytest.mark.parametrize("moving", [False, True, "linear", "wiggle", "viblin", "vibrot"])
 def test_mesh_floor_one_step_fp64(moving):
     say(f"[gpu] mesh floor one step moving={moving}")
     g, o = _build_mesh_case(_mesh_floor_pack(moving=moving))
@@ -438,9 +443,9 @@ def test_mesh_floor_many_steps_fp64():
     g.close()
 
 
-@pytest.mark.parametrize("motion", ["linear", "wiggle"])
+@pytest.mark.parametrize("motion", ["linear", "wiggle", "viblin", "vibrot"])
 def test_translating_floor_many_steps_fp64(motion):
-    """fix move/mesh linear / wiggle under a packed bed, 500 steps (two runs,
+    """fix move/mesh linear / wiggle / viblin / vibrot under a packed bed, 500 steps (two runs,
     the second continuing the mover's time): node positions and velocities
     bitwise equal to the oracle, the same rebuilds, state and mesh history
     within 1e-8."""
@@ -664,7 +669,7 @@ def test_rotating_drum_run_in_pieces():
     g2.close()
 
 
-@pytest.mark.parametrize("case", ["pack", "drum", "wiggle"])
+@pytest.mark.parametrize("case", ["pack", "drum", "wiggle", "vibrot"])
 def test_graph_capture_is_bitwise_eager(case):
     """Steps replayed from captured HIP graphs (32 per replay, rebuilds and
     other eager work between replays, several `run pre no` pieces) give
@@ -677,8 +682,8 @@ def test_graph_capture_is_bitwise_eager(case):
     elif case == "drum":
         params, mats, P, walls, meshes = _small_drum()
         pieces = [200, 33, 300]
-    else:  # the wiggle mover reads its time table inside the captured steps
-        params, mats, P, walls, meshes = _mesh_floor_pack(seed=44, n=1500, moving="wiggle")
+    else:  # the time-dependent movers read their motion table inside the captured steps
+        params, mats, P, walls, meshes = _mesh_floor_pack(seed=44, n=1500, moving=case)
         pieces = [150, 70, 180]
     a = pydem.GpuSystem(params, mats, P, walls, meshes=meshes)
     b = pydem.GpuSystem(params, mats, P, walls, meshes=meshes)

@@ -351,3 +351,46 @@ def test_mesh_stress_of_a_resting_sphere():
         c = o.state()["x"][0] - np.array([0.0, 0.0, o.state()["x"][0, 2]])  # contact point on the floor
         assert np.allclose(T, np.cross(c - np.array(pref), F), rtol=1e-6, atol=1e-9 * W * 0.01)
         o.close()
+
+
+def _sin_sum(w, phi, dt, n):
+    """sum_{k=1..n} sin(w k dt + phi) in closed form"""
+    th = w * dt
+    return math.sin(phi + (n + 1) * th / 2) * math.sin(n * th / 2) / math.sin(th / 2)
+
+
+def test_viblin_mover_is_the_discrete_sum_of_its_modes():
+    """fix move/mesh viblin (two modes): displacement after n steps =
+    -axis dt sum_j a_j w_j sum_k sin(w_j k dt + phi_j), velocity at the last
+    step axis * (-sum_j a_j w_j sin(w_j n dt + phi_j)); the axis is normalized."""
+    axis = np.array([1.0, 2.0, 2.0])
+    am, ph, pe = np.array([0.002, 0.0005]), np.array([0.3, -1.1]), np.array([0.01, 0.003])
+    tris, o, params = _mover_case({"viblin": (axis, am, ph, pe)})
+    dt, n = params["dt"], 1500
+    o.run(n)
+    u = axis / 3.0
+    w = 2 * math.pi / pe
+    disp = -u * dt * sum(am[j] * w[j] * _sin_sum(w[j], ph[j], dt, n) for j in range(2))
+    nodes, vn = o.mesh_nodes()
+    assert np.allclose(nodes, tris + disp, rtol=0, atol=1e-13)
+    vA = -sum(am[j] * w[j] * math.sin(w[j] * n * dt + ph[j]) for j in range(2))
+    assert np.allclose(vn, np.broadcast_to(u * vA, vn.shape), rtol=0, atol=1e-12)
+
+
+def test_vibrot_mover_turns_by_the_integrated_angle():
+    """fix move/mesh vibrot: after n steps the mesh is turned about the axis
+    through the origin by dt sum_k vR(k dt), vR = -sum_j a_j w_j sin(w_j t + phi_j);
+    node velocities are (axis vR) x (node - origin)."""
+    origin = np.array([0.01, -0.02, 0.0])
+    am, ph, pe = np.array([0.3]), np.array([0.5]), np.array([0.02])
+    tris, o, params = _mover_case({"vibrot": (origin, (0, 0, 2.0), am, ph, pe)})
+    dt, n = params["dt"], 1700
+    o.run(n)
+    w = 2 * math.pi / pe[0]
+    ang = -dt * am[0] * w * _sin_sum(w, ph[0], dt, n)
+    R = np.array([[math.cos(ang), -math.sin(ang), 0], [math.sin(ang), math.cos(ang), 0], [0, 0, 1]])
+    nodes, vn = o.mesh_nodes()
+    assert np.allclose(nodes, (tris - origin) @ R.T + origin, rtol=0, atol=1e-13)
+    vR = -am[0] * w * math.sin(w * n * dt + ph[0])
+    assert np.allclose(vn, np.cross(np.array([0, 0, vR]), nodes - origin), rtol=0, atol=1e-12)
+    assert abs(ang) > 0.05

@@ -127,7 +127,11 @@ def test_mesh_commands_parse(tmp_path):
     rc, log = run_script(tmp_path, good.replace("move/mesh mesh cad", "move/mesh mesh nope"), "in.mesh3")
     assert rc != 0 and "unknown mesh nope" in log
     for movement, ok in (("linear 0.1 0 0", True), ("wiggle amplitude 0.001 0 0 period 0.1", True),
-                         ("viblin axis 0 0 1 order 1 amplitude 0.1 phase 0 period 0.1", False),
+                         ("viblin axis 0 0 1 order 1 amplitude 0.1 phase 0 period 0.1", True),
+                         ("vibrot origin 0 0 0 axis 0 0 1 order 2 amplitude 0.1 0.2 phase 0 1 period 0.1 0.05",
+                          True),
+                         ("viblin axis 0 0 1 order 2 amplitude 0.1 phase 0 period 0.1", False),
+                         ("riggle origin 0 0 0 axis 0 0 1 period 1 amplitude 5", False),
                          ("wiggle amp 0.001 0 0 period 0.1", False)):
         text = good.replace("rotate origin 0.05 0.05 0 axis 0 0 1 period 1.0", movement)
         rc, log = run_script(tmp_path, text, "in.mesh4")
