@@ -51,21 +51,46 @@ __device__ __forceinline__ double pow_abs_int(double a, int b) {
   }
 }
 
+// pow_abs_int for the exponents 0..9 without branches: every candidate
+// product is formed (a handful of multiplications, in the same order as the
+// switch above) and one is selected, so independent calls can overlap
+__device__ __forceinline__ double pow_abs_int_sel(double a, int b) {
+  a = fabs(a);
+  if (b > 9 || b < 0) return pow_abs_int(a, b);
+  const double a2 = a * a, a3 = a2 * a, a4 = a2 * a2, a6 = a3 * a3;
+  const double a5 = a4 * a, a7 = a4 * a3, a8 = a4 * a4, a9 = a6 * a3;
+  double r = 1.0;
+  r = b == 1 ? a : r;
+  r = b == 2 ? a2 : r;
+  r = b == 3 ? a3 : r;
+  r = b == 4 ? a4 : r;
+  r = b == 5 ? a5 : r;
+  r = b == 6 ? a6 : r;
+  r = b == 7 ? a7 : r;
+  r = b == 8 ? a8 : r;
+  r = b == 9 ? a9 : r;
+  return r;
+}
+
+// the same value as the branching form (MathExtraLiggghtsNonspherical::
+// pow_abs): every branch's result is computed and the one the branches would
+// take is selected, so the shape function's independent powers (three per
+// particle, two particles per Newton step) run as overlapping straight-line
+// chains instead of one after another
 __device__ __forceinline__ double pow_abs(double base, double exponent) {
   base = fabs(base);
   const double tol = 1e-12;
-  if (fabs(exponent) < tol || fabs(base - 1.0) < tol) return 1.0;
-  if (fabs(base) < tol) return 0.0;
-  const double n = floor(fabs(exponent));
-  double r = pow_abs_int(base, (int)n);
-  const double nd = fabs(exponent) - n;
-  if (nd > 0.0) {
-    if (fabs(2.0 * nd - 1.0) < 2.0 * tol)
-      r *= sqrt(base);
-    else
-      r *= pow(base, nd);
-  }
-  return exponent > 0.0 ? r : 1.0 / r;
+  const double ae = fabs(exponent);
+  const double n = floor(ae);
+  const double r0 = pow_abs_int_sel(base, (int)n);
+  const double nd = ae - n;
+  const double pw = pow(base, nd);
+  const double sq = sqrt(base);
+  double r = nd > 0.0 ? r0 * (fabs(2.0 * nd - 1.0) < 2.0 * tol ? sq : pw) : r0;
+  r = exponent > 0.0 ? r : 1.0 / r;
+  r = fabs(base) < tol ? 0.0 : r;
+  r = (ae < tol || fabs(base - 1.0) < tol) ? 1.0 : r;
+  return r;
 }
 
 __device__ __forceinline__ double sgn(double v) { return v > 1e-14 ? 1.0 : (v < -1e-14 ? -1.0 : 0.0); }
@@ -120,9 +145,9 @@ __device__ __forceinline__ void props_local(const Sq &s, const double *x, double
   double xan22, ybn22, zcn12;
   if (s.int_blk) {
     const int n1i = (int)floor(n1 + 1e-2), n2i = (int)floor(n2 + 1e-2);
-    xan22 = pow_abs_int(xa, n2i - 2);
-    ybn22 = pow_abs_int(yb, n2i - 2);
-    zcn12 = pow_abs_int(zc, n1i - 2);
+    xan22 = pow_abs_int_sel(xa, n2i - 2);
+    ybn22 = pow_abs_int_sel(yb, n2i - 2);
+    zcn12 = pow_abs_int_sel(zc, n1i - 2);
   } else {
     xan22 = pow_abs(xa, n2 - 2.0);
     ybn22 = pow_abs(yb, n2 - 2.0);
@@ -611,9 +636,9 @@ __device__ __forceinline__ double shape_local(const Sq &s, const double *x) {
   if (s.int_blk) {
     const int n1i = (int)floor(n1 + 1e-2), n2i = (int)floor(n2 + 1e-2);
     if (n1i == n2i)
-      f = pow_abs_int(xi, n2i) + pow_abs_int(yi, n2i) + pow_abs_int(zi, n1i) - 1.0;
+      f = pow_abs_int_sel(xi, n2i) + pow_abs_int_sel(yi, n2i) + pow_abs_int_sel(zi, n1i) - 1.0;
     else
-      f = pow_abs(pow_abs_int(xi, n2i) + pow_abs_int(yi, n2i), n1 / n2) + pow_abs_int(zi, n1i) - 1.0;
+      f = pow_abs(pow_abs_int_sel(xi, n2i) + pow_abs_int_sel(yi, n2i), n1 / n2) + pow_abs_int_sel(zi, n1i) - 1.0;
   } else {
     f = pow_abs(pow_abs(xi, n2) + pow_abs(yi, n2), n1 / n2) + pow_abs(zi, n1) - 1.0;
   }
@@ -628,9 +653,9 @@ __device__ __forceinline__ void gradient_local(const Sq &s, const double *x, dou
   double xan21, ybn21, zcn11;
   if (s.int_blk) {
     const int n1i = (int)floor(n1 + 1e-2), n2i = (int)floor(n2 + 1e-2);
-    xan21 = pow_abs_int(xa, n2i - 1);
-    ybn21 = pow_abs_int(yb, n2i - 1);
-    zcn11 = pow_abs_int(zc, n1i - 1);
+    xan21 = pow_abs_int_sel(xa, n2i - 1);
+    ybn21 = pow_abs_int_sel(yb, n2i - 1);
+    zcn11 = pow_abs_int_sel(zc, n1i - 1);
   } else {
     xan21 = pow_abs(xa, n2 - 1.0);
     ybn21 = pow_abs(yb, n2 - 1.0);
