@@ -197,6 +197,28 @@ int dem_mesh_rotate(dem_handle *h, int mesh_id, const double *origin, const doub
  * the particle, lever arm from the reference point to the contact point. */
 int dem_mesh_stress(dem_handle *h, int mesh_id, const double *reference_point);
 
+/* write_restart <file> (write_restart.cpp with FixContactHistory::
+ * write_restart / pack_restart, fix_contact_history.cpp:588-659): the
+ * particles, their contact histories (pair, wall and mesh), the mesh
+ * geometry and the step counters, in this library's own binary layout. */
+int dem_write_restart(dem_handle *h, const char *path);
+
+/* read_restart <file> (read_restart.cpp): the box, particle count, atom
+ * types and timestep of a restart file, and its particles in the caller order
+ * they were given in (arrays n, n, 3n, 3n, 3n, n, n; any may be NULL), so a
+ * caller can rebuild the system with dem_set_particles before
+ * dem_read_restart. These two need no handle (errors: dem_last_error(NULL)). */
+int dem_restart_info(const char *path, int64_t *n, int *ntypes, double *boxlo, double *boxhi,
+                     int *periodic, int64_t *ntimestep);
+int dem_restart_particles(const char *path, int64_t n, int *tag, int *type, double *x, double *v,
+                          double *omega, double *radius, double *rmass);
+
+/* Restores the device state of a restart file into a handle configured as
+ * the one that wrote it (same particles set in the same order, materials,
+ * walls, meshes, precision, list width, timestep). The next dem_run sets up
+ * and continues bitwise as the writing system's next run would. */
+int dem_read_restart(dem_handle *h, const char *path);
+
 /* The total force and torque on a mesh/surface/stress mesh at the last
  * counted pass: what FixMeshSurface::compute_vector (fix_mesh_surface.cpp:560)
  * returns as f_ID[1..6] for thermo output. Either pointer may be NULL. */

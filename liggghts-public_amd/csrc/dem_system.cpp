@@ -1701,6 +1701,241 @@ int dem_get_state(dem_handle *h, double *x, double *v, double *omega, double *f,
   return DEM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// restart files: the particles in caller order (what read_restart restores
+// into atom arrays) followed by a snapshot of the device state in its own
+// order (state records, neighbor list with contact flags and histories, wall
+// and mesh histories, displacement reference, mesh geometry) and the counters
+// a run continues from. The layout is this library's own.
+// ---------------------------------------------------------------------------
+namespace {
+struct RestartHeader {
+  char magic[8];
+  int32_t version, mixed, ntypes, cap, W, nwalls, nmesh, ntri;
+  int32_t cur_st, cur_l, cur_aux, tri_cur, have_list, pending_mesh_rebuild, periodic[3], pad;
+  int64_t n, ntimestep, nrebuilds, last_build;
+  double mesh_time, dt, boxlo[3], boxhi[3];
+};
+constexpr char kRestartMagic[8] = {'L', 'G', 'A', 'M', 'D', 'R', 'S', 'T'};
+
+bool read_header(FILE *f, RestartHeader &hd) {
+  return fread(&hd, sizeof hd, 1, f) == 1 && std::memcmp(hd.magic, kRestartMagic, 8) == 0 && hd.version == 1;
+}
+
+// device blobs of a restart, in file order
+std::vector<std::pair<void *, size_t>> restart_blobs(dem_handle *h) {
+  const size_t cap = (size_t)h->cap, n = (size_t)h->n;
+  const size_t nw = std::max<size_t>(1, h->walls.size()), nm = std::max<size_t>(1, h->meshes.size());
+  const StateBuf &st = h->st[h->cur_st];
+  const AuxBuf &ax = h->aux[h->cur_aux];
+  const ListBuf &ls = h->lst[h->cur_l];
+  std::vector<std::pair<void *, size_t>> b = {
+      {st.pos, sizeof(double4) * n},
+      {st.vel, sizeof(double4) * n},
+      {st.omg, (h->mixed ? sizeof(float4) : sizeof(double4)) * n},
+      {ax.tag, sizeof(int) * n},
+      {ax.orig, sizeof(int) * n},
+      {ax.meta, sizeof(int) * n},
+      {ax.whist, h->fsz * 3 * nw * cap},
+      {ax.mslot, sizeof(int) * nm * dem::kMeshSlots * cap},
+      {ax.mhist, h->fsz * 3 * nm * dem::kMeshSlots * cap},
+      {ls.nbr, sizeof(uint32_t) * h->W * cap},
+      {ls.hist, h->fsz * 3 * h->W * cap},
+      {h->xhold, sizeof(double) * 3 * cap},
+      {h->fout, sizeof(double) * 6 * cap}};
+  if (h->ntri > 0) {
+    b.push_back({h->d_tri_buf[h->tri_cur], sizeof(dem::TriGeom) * h->ntri});
+    b.push_back({h->d_node_last, sizeof(double) * 9 * h->ntri});
+  }
+  return b;
+}
+}  // namespace
+
+int dem_write_restart(dem_handle *h, const char *path) {
+  if (!h || !path) return DEM_ERR_ARG;
+  if (h->n <= 0 || !h->have_list) return fail(h, DEM_ERR_STATE, "write_restart needs a set-up system");
+  const int64_t n = h->n;
+  std::vector<double> x(3 * n), v(3 * n), w(3 * n);
+  int rc = dem_get_state(h, x.data(), v.data(), w.data(), nullptr, nullptr);
+  if (rc) return rc;
+  // caller-order tag, type, radius, mass
+  std::vector<double4> pos(n), vel(n);
+  std::vector<int> orig(n), tag(n), typ(n), ctag(n), ctype(n);
+  std::vector<double> crad(n), cmass(n);
+  const StateBuf &st = h->st[h->cur_st];
+  const AuxBuf &ax = h->aux[h->cur_aux];
+  HIPCHK(h, hipMemcpy(pos.data(), st.pos, sizeof(double4) * n, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(vel.data(), st.vel, sizeof(double4) * n, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(orig.data(), ax.orig, sizeof(int) * n, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(tag.data(), ax.tag, sizeof(int) * n, hipMemcpyDeviceToHost));
+  if (h->mixed) {
+    std::vector<float4> ww(n);
+    HIPCHK(h, hipMemcpy(ww.data(), st.omg, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) typ[i] = (int)ww[i].w & dem::kTypeMask;
+  } else {
+    std::vector<double4> ww(n);
+    HIPCHK(h, hipMemcpy(ww.data(), st.omg, sizeof(double4) * n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) typ[i] = (int)ww[i].w & dem::kTypeMask;
+  }
+  for (int64_t i = 0; i < n; i++) {
+    const int o = orig[i];
+    ctag[o] = tag[i];
+    ctype[o] = typ[i];
+    crad[o] = pos[i].w;
+    cmass[o] = vel[i].w;
+  }
+  FILE *f = fopen(path, "wb");
+  if (!f) return fail(h, DEM_ERR_ARG, std::string("Cannot open restart file ") + path);
+  RestartHeader hd{};
+  std::memcpy(hd.magic, kRestartMagic, 8);
+  hd.version = 1;
+  hd.mixed = h->mixed ? 1 : 0;
+  hd.ntypes = h->ntypes;
+  hd.cap = h->cap;
+  hd.W = h->W;
+  hd.nwalls = (int32_t)h->walls.size();
+  hd.nmesh = (int32_t)h->meshes.size();
+  hd.ntri = h->ntri;
+  hd.cur_st = h->cur_st;
+  hd.cur_l = h->cur_l;
+  hd.cur_aux = h->cur_aux;
+  hd.tri_cur = h->tri_cur;
+  hd.have_list = h->have_list ? 1 : 0;
+  hd.pending_mesh_rebuild = h->pending_mesh_rebuild ? 1 : 0;
+  hd.n = n;
+  hd.ntimestep = h->ntimestep;
+  hd.nrebuilds = h->nrebuilds;
+  hd.last_build = h->last_build;
+  hd.mesh_time = h->mesh_time;
+  hd.dt = h->p.dt;
+  for (int d = 0; d < 3; d++) {
+    hd.boxlo[d] = h->p.boxlo[d];
+    hd.boxhi[d] = h->p.boxhi[d];
+    hd.periodic[d] = h->p.periodic[d];
+  }
+  bool ok = fwrite(&hd, sizeof hd, 1, f) == 1;
+  ok = ok && fwrite(ctag.data(), sizeof(int), n, f) == (size_t)n;
+  ok = ok && fwrite(ctype.data(), sizeof(int), n, f) == (size_t)n;
+  ok = ok && fwrite(x.data(), sizeof(double), 3 * n, f) == (size_t)(3 * n);
+  ok = ok && fwrite(v.data(), sizeof(double), 3 * n, f) == (size_t)(3 * n);
+  ok = ok && fwrite(w.data(), sizeof(double), 3 * n, f) == (size_t)(3 * n);
+  ok = ok && fwrite(crad.data(), sizeof(double), n, f) == (size_t)n;
+  ok = ok && fwrite(cmass.data(), sizeof(double), n, f) == (size_t)n;
+  for (auto &blob : restart_blobs(h)) {
+    if (!ok) break;
+    std::vector<char> buf(blob.second);
+    if (hipMemcpy(buf.data(), blob.first, blob.second, hipMemcpyDeviceToHost) != hipSuccess) {
+      fclose(f);
+      return fail(h, DEM_ERR_HIP, "restart: device read failed");
+    }
+    ok = fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+  }
+  if (fclose(f) != 0) ok = false;
+  if (!ok) return fail(h, DEM_ERR_ARG, std::string("Error writing restart file ") + path);
+  return DEM_OK;
+}
+
+int dem_restart_info(const char *path, int64_t *n, int *ntypes, double *boxlo, double *boxhi,
+                     int *periodic, int64_t *ntimestep) {
+  if (!path) return DEM_ERR_ARG;
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail(nullptr, DEM_ERR_ARG, std::string("Cannot open restart file ") + path);
+  RestartHeader hd;
+  const bool ok = read_header(f, hd);
+  fclose(f);
+  if (!ok) return fail(nullptr, DEM_ERR_ARG, std::string("Not a restart file of this library: ") + path);
+  if (n) *n = hd.n;
+  if (ntypes) *ntypes = hd.ntypes;
+  for (int d = 0; d < 3; d++) {
+    if (boxlo) boxlo[d] = hd.boxlo[d];
+    if (boxhi) boxhi[d] = hd.boxhi[d];
+    if (periodic) periodic[d] = hd.periodic[d];
+  }
+  if (ntimestep) *ntimestep = hd.ntimestep;
+  return DEM_OK;
+}
+
+int dem_restart_particles(const char *path, int64_t n, int *tag, int *type, double *x, double *v,
+                          double *omega, double *radius, double *rmass) {
+  if (!path) return DEM_ERR_ARG;
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail(nullptr, DEM_ERR_ARG, std::string("Cannot open restart file ") + path);
+  RestartHeader hd;
+  bool ok = read_header(f, hd) && hd.n == n;
+  std::vector<int> ib(n);
+  std::vector<double> db(3 * n);
+  auto rd = [&](void *dst, size_t sz, size_t cnt) {
+    if (!ok) return;
+    void *tgt = dst ? dst : (sz == sizeof(int) ? (void *)ib.data() : (void *)db.data());
+    ok = fread(tgt, sz, cnt, f) == cnt;
+  };
+  rd(tag, sizeof(int), n);
+  rd(type, sizeof(int), n);
+  rd(x, sizeof(double), 3 * n);
+  rd(v, sizeof(double), 3 * n);
+  rd(omega, sizeof(double), 3 * n);
+  rd(radius, sizeof(double), n);
+  rd(rmass, sizeof(double), n);
+  fclose(f);
+  if (!ok) return fail(nullptr, DEM_ERR_ARG, std::string("Bad or mismatched restart file ") + path);
+  return DEM_OK;
+}
+
+int dem_read_restart(dem_handle *h, const char *path) {
+  if (!h || !path) return DEM_ERR_ARG;
+  if (h->n <= 0) return fail(h, DEM_ERR_STATE, "read_restart: set the particles first");
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail(h, DEM_ERR_ARG, std::string("Cannot open restart file ") + path);
+  RestartHeader hd;
+  if (!read_header(f, hd)) {
+    fclose(f);
+    return fail(h, DEM_ERR_ARG, std::string("Not a restart file of this library: ") + path);
+  }
+  if (hd.n != h->n || hd.cap != h->cap || hd.W != h->W || hd.mixed != (h->mixed ? 1 : 0) ||
+      hd.nwalls != (int)h->walls.size() || hd.nmesh != (int)h->meshes.size() || hd.ntri != h->ntri ||
+      hd.dt != h->p.dt) {
+    fclose(f);
+    return fail(h, DEM_ERR_ARG,
+                "read_restart: the system differs from the one written (particles, list width, "
+                "precision, walls, meshes or timestep)");
+  }
+  // skip the caller-order particle table
+  const long skip = (long)(hd.n * (2 * sizeof(int) + 11 * sizeof(double)));
+  if (fseek(f, skip, SEEK_CUR) != 0) {
+    fclose(f);
+    return fail(h, DEM_ERR_ARG, "read_restart: truncated file");
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  clear_graphs(h);
+  h->cur_st = hd.cur_st;
+  h->cur_l = hd.cur_l;
+  h->cur_aux = hd.cur_aux;
+  h->tri_cur = hd.tri_cur;
+  bool ok = true;
+  for (auto &blob : restart_blobs(h)) {
+    std::vector<char> buf(blob.second);
+    if (fread(buf.data(), 1, buf.size(), f) != buf.size()) {
+      ok = false;
+      break;
+    }
+    if (hipMemcpy(blob.first, buf.data(), blob.second, hipMemcpyHostToDevice) != hipSuccess) {
+      fclose(f);
+      return fail(h, DEM_ERR_HIP, "read_restart: device write failed");
+    }
+  }
+  fclose(f);
+  if (!ok) return fail(h, DEM_ERR_ARG, "read_restart: truncated file");
+  h->have_list = hd.have_list != 0;
+  h->pending_mesh_rebuild = hd.pending_mesh_rebuild != 0;
+  h->ntimestep = hd.ntimestep;
+  h->nrebuilds = hd.nrebuilds;
+  h->last_build = hd.last_build;
+  h->mesh_time = hd.mesh_time;
+  h->setup_done = false;
+  h->can_continue = false;  // the next run sets up (Verlet::setup), as after read_restart
+  return DEM_OK;
+}
+
 int dem_get_pairs(dem_handle *h, int64_t cap_out, int *tags, int *flags, double *hist,
                   int64_t *npairs) {
   if (!h || !npairs) return DEM_ERR_ARG;

@@ -49,6 +49,8 @@
 //                  (columns id type x y z vx vy vz fx fy fz omegax omegay omegaz
 //                   tqx tqy tqz radius diameter mass; '*' in the file name: one
 //                   file per snapshot; atoms in ID order)
+//   write_restart <file> | read_restart <file>  (this library's restart layout:
+//                  atoms, contact histories, meshes, timestep)
 // Other output-only commands (compute, restart, ...) are accepted and ignored
 // with a warning; anything that would change the physics and is not listed
 // is an error, as the reference reports for an unknown style.
@@ -198,6 +200,7 @@ struct Interp {
     long long last = -1;  // timestep of the last snapshot written
   };
   std::map<std::string, Dump> dumps;
+  std::string restart_file;  // read_restart: device state applied when the system is built
 
   ~Interp() {
     for (auto &d : dumps)
@@ -883,6 +886,10 @@ struct Interp {
     if (dem_set_particles(h, (int64_t)tag.size(), x.data(), v.data(), w.data(), r.data(), m.data(),
                           type.data(), tag.data()))
       die(dem_last_error(h));
+    if (!restart_file.empty()) {  // contact histories, list and mesh state of read_restart
+      if (dem_read_restart(h, restart_file.c_str())) die(dem_last_error(h));
+      restart_file.clear();
+    }
     dirty = false;
   }
 
@@ -1038,7 +1045,7 @@ struct Interp {
                c == "thermo_modify" ||
                (c == "compute" && !(t.size() >= 4 && (t[3] == "erotate/sphere" || t[3] == "ke"))) ||
                c == "uncompute" || c == "group" || c == "restart" ||
-               c == "write_restart" || c == "write_dump" || c == "soft_particles" ||
+               c == "write_dump" || c == "soft_particles" ||
                c == "hard_particles") {
       if (c != "atom_modify" && c != "processors" && c != "echo" && c != "log" && c != "thermo_modify")
         fprintf(out, "WARNING: '%s' is output/bookkeeping only and is ignored on this path\n", c.c_str());
@@ -1084,6 +1091,41 @@ struct Interp {
     } else if (c == "read_data") {
       if (t.size() < 2) die("Illegal read_data command");
       read_data(t[1]);
+    } else if (c == "write_restart") {
+      // write_restart <file> (write_restart.cpp): after a run, the system and
+      // its contact histories in this library's restart layout
+      if (t.size() < 2) die("Illegal write_restart command");
+      if (sq) die("write_restart for superquadrics is not available on this path");
+      if (!h || dirty) die("write_restart needs a preceding run on this path");
+      if (dem_write_restart(h, t[1].c_str())) die(dem_last_error(h));
+    } else if (c == "read_restart") {
+      // read_restart <file> (read_restart.cpp): box, types, atoms and timestep
+      // now; the device state (histories, list, meshes) when the system is built
+      if (t.size() < 2) die("Illegal read_restart command");
+      if (have_box) die("Cannot read_restart after simulation box is defined");
+      int64_t nr = 0;
+      int nt = 0, per[3];
+      long long ts = 0;
+      int64_t ts64 = 0;
+      if (dem_restart_info(t[1].c_str(), &nr, &nt, lo, hi, per, &ts64)) die(dem_last_error(nullptr));
+      ts = ts64;
+      tag.assign(nr, 0);
+      type.assign(nr, 0);
+      x.assign(3 * nr, 0.0);
+      v.assign(3 * nr, 0.0);
+      w.assign(3 * nr, 0.0);
+      r.assign(nr, 0.0);
+      m.assign(nr, 0.0);
+      if (dem_restart_particles(t[1].c_str(), nr, tag.data(), type.data(), x.data(), v.data(), w.data(),
+                                r.data(), m.data()))
+        die(dem_last_error(nullptr));
+      for (int d = 0; d < 3; d++) periodic[d] = per[d];
+      ntypes = nt;
+      have_box = true;
+      step = ts;
+      restart_file = t[1];
+      fprintf(out, "  %lld atoms\n", (long long)nr);
+      dirty = true;
     } else if (c == "neighbor") {
       if (t.size() < 3 || t[2] != "bin") die("neighbor style must be 'bin' on this path");
       skin = num(t[1]);

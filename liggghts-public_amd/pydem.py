@@ -23,6 +23,7 @@ EXPORTED = [
     "dem_run_continue", "dem_compute_forces", "dem_get_state", "dem_get_pairs",
     "dem_get_wall_history", "dem_get_stats", "dem_set_kernel_timing",
     "dem_run_script", "dem_add_mesh", "dem_mesh_rotate", "dem_get_mesh_nodes", "dem_mesh_linear", "dem_mesh_wiggle", "dem_mesh_stress", "dem_get_mesh_force", "dem_mesh_viblin", "dem_mesh_vibrot",
+    "dem_write_restart", "dem_read_restart", "dem_restart_info", "dem_restart_particles",
     "dem_get_mesh_topology", "dem_get_mesh_history", "dem_set_instances",
     "dem_set_instance_materials", "dem_get_instance_stats", "dem_set_step_shape",
     "dem_get_step_shape", "dem_set_graph_capture", "dem_set_pair_model", "dem_set_model_coefficients",
@@ -174,6 +175,10 @@ def load_library(path=LIB_PATH):
     lib.dem_mesh_rotate.argtypes = [h, ctypes.c_int, dp, dp, ctypes.c_double]
     lib.dem_mesh_linear.argtypes = [h, ctypes.c_int, dp]
     lib.dem_mesh_stress.argtypes = [h, ctypes.c_int, dp]
+    lib.dem_write_restart.argtypes = [h, ctypes.c_char_p]
+    lib.dem_read_restart.argtypes = [h, ctypes.c_char_p]
+    lib.dem_restart_info.argtypes = [ctypes.c_char_p, P(ctypes.c_int64), ip, dp, dp, ip, P(ctypes.c_int64)]
+    lib.dem_restart_particles.argtypes = [ctypes.c_char_p, ctypes.c_int64, ip, ip, dp, dp, dp, dp, dp]
     lib.dem_mesh_viblin.argtypes = [h, ctypes.c_int, dp, ctypes.c_int, dp, dp, dp]
     lib.dem_mesh_vibrot.argtypes = [h, ctypes.c_int, dp, dp, ctypes.c_int, dp, dp, dp]
     lib.dem_get_mesh_force.argtypes = [h, ctypes.c_int, dp, dp]
@@ -462,6 +467,12 @@ class GpuSystem:
         n = min(nc.value, cap)
         return v.value, f.value, [(int(cv[k]), int(cf[k]), float(cu[k])) for k in range(n)]
 
+    def write_restart(self, path):
+        self._check(self.lib.dem_write_restart(self.h, str(path).encode()))
+
+    def read_restart(self, path):
+        self._check(self.lib.dem_read_restart(self.h, str(path).encode()))
+
     def mesh_force(self, k=0):
         """(total force, torque about the reference point) on mesh/surface/stress k
         at the last counted force pass"""
@@ -724,3 +735,21 @@ class SqDemSystem:
             self.close()
         except Exception:
             pass
+
+
+def restart_particles(path):
+    """(header dict, particle dict in the writer's caller order) of a restart file"""
+    lib = load_library()
+    n, nt, ts = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int64()
+    lo, hi = np.zeros(3), np.zeros(3)
+    per = np.zeros(3, dtype=np.int32)
+    b = str(path).encode()
+    if lib.dem_restart_info(b, ctypes.byref(n), ctypes.byref(nt), _dp(lo), _dp(hi), _ip(per), ctypes.byref(ts)):
+        raise DemError(lib.dem_last_error(None).decode())
+    N = n.value
+    P = {"tag": np.zeros(N, dtype=np.int32), "type": np.zeros(N, dtype=np.int32), "x": np.zeros((N, 3)),
+         "v": np.zeros((N, 3)), "omega": np.zeros((N, 3)), "radius": np.zeros(N), "rmass": np.zeros(N)}
+    if lib.dem_restart_particles(b, N, _ip(P["tag"]), _ip(P["type"]), _dp(P["x"]), _dp(P["v"]),
+                                 _dp(P["omega"]), _dp(P["radius"]), _dp(P["rmass"])):
+        raise DemError(lib.dem_last_error(None).decode())
+    return {"n": N, "ntypes": nt.value, "boxlo": lo, "boxhi": hi, "periodic": per, "ntimestep": ts.value}, P

@@ -806,3 +806,96 @@ run 200
     s250 = _read_dump(tmp_path / "snap250.txt")
     assert len(s250) == 1 and s250[0][0] == 250
     g.close()
+
+
+@pytest.mark.parametrize("case", ["pack_walls", "drum"])
+def test_restart_continues_bitwise(tmp_path, case):
+    """write_restart after 200 steps, read_restart into a fresh system built
+    the same way, 100 more steps: bitwise the state, pair set, histories and
+    mesh nodes of the uninterrupted 200 + 100 steps (both second runs set up
+    again, as `run` does after read_restart)."""
+    say(f"[gpu] restart continuation {case}")
+    if case == "drum":
+        params, mats, P, walls, meshes = _small_drum()
+    else:
+        params, mats, P, walls = _walled_pack()
+        meshes = ()
+    a = pydem.GpuSystem(params, mats, P, walls, meshes=meshes)
+    a.run(200)
+    a.run(100)
+    b = pydem.GpuSystem(params, mats, P, walls, meshes=meshes)
+    b.run(200)
+    f = tmp_path / "r.restart"
+    b.write_restart(f)
+    b.close()
+    hd, P2 = pydem.restart_particles(f)
+    assert hd["n"] == len(P["radius"]) and hd["ntimestep"] == 200
+    assert np.array_equal(P2["radius"], P["radius"]) and np.array_equal(P2["tag"], P["tag"])
+    c = pydem.GpuSystem(params, mats, P, walls, meshes=meshes)
+    c.read_restart(f)
+    c.run(100)
+    sa, sc = a.state(), c.state()
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert np.array_equal(sa[k], sc[k]), k
+    pa, pc = a.pairs(), c.pairs()
+    for u, w in zip(pa, pc):
+        assert np.array_equal(u, w)
+    assert a.stats()["ntimestep"] == c.stats()["ntimestep"] == 300
+    if meshes:
+        assert np.array_equal(a.mesh_nodes()[0], c.mesh_nodes()[0])
+        ta, ha = a.mesh_history()
+        tc, hc = c.mesh_history()
+        assert np.array_equal(ta, tc) and np.array_equal(ha, hc)
+    else:
+        assert np.array_equal(a.wall_history(), c.wall_history())
+    a.close()
+    c.close()
+
+
+def test_script_restart_matches_one_run(tmp_path):
+    """A script that runs 300 steps equals one that runs 200, writes a
+    restart, and a second script that reads it, redefines the physics and runs
+    100 more: the same thermo line at step 300."""
+    say("[gpu] script restart")
+    params, mats, P, walls = wl.settling_4k(seed=9, n=600)
+    data = tmp_path / "d.data"
+    wl.write_data_file(str(data), params, P)
+    physics = """neighbor 0.001 bin
+neigh_modify delay 0
+fix m1 all property/global youngsModulus peratomtype 5.e6
+fix m2 all property/global poissonsRatio peratomtype 0.45
+fix m3 all property/global coefficientRestitution peratomtypepair 1 0.3
+fix m4 all property/global coefficientFriction peratomtypepair 1 0.5
+pair_style gran model hertz tangential history
+pair_coeff * *
+timestep 0.00001
+fix gravi all gravity 9.81 vector 0.0 0.0 -1.0
+fix zwall all wall/gran model hertz tangential history primitive type 1 zplane 0.0
+"""
+    head = """atom_style granular
+atom_modify map array
+boundary p p f
+newton off
+communicate single vel yes
+units si
+"""
+    box = "region reg block 0 0.1 0 0.1 0 0.2 units box\ncreate_box 1 reg\n"
+    tail = "fix integr all nve/sphere\nthermo_style custom step atoms ke\nthermo 100\n"
+    rst = tmp_path / "s.restart"
+    scripts = {
+        "one": head + box + physics + f"read_data {data}\n" + tail + "run 200\nrun 100\n",
+        "first": head + box + physics + f"read_data {data}\n" + tail + f"run 200\nwrite_restart {rst}\n",
+        "second": head + f"read_restart {rst}\n" + physics + tail + "run 100\n",
+    }
+    last = {}
+    for name in ("one", "first", "second"):
+        p = tmp_path / f"in.{name}"
+        p.write_text(scripts[name])
+        log = tmp_path / f"log.{name}"
+        rc = pydem.run_script(str(p), str(log))
+        text = log.read_text()
+        assert rc == 0, text[-800:]
+        rows = [l.split() for l in text.splitlines() if l.strip() and l.split()[0].isdigit()]
+        last[name] = rows[-1]
+    say(f"  one {last['one']}  restarted {last['second']}")
+    assert last["second"][0] == "300" and last["second"] == last["one"]

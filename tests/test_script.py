@@ -180,3 +180,14 @@ def test_dump_commands_parse(tmp_path):
     rc, log = run_script(tmp_path, text)
     assert rc == 0, log
     assert out.exists()  # opened at the dump command, as the reference does
+
+
+def test_read_restart_errors(tmp_path):
+    bogus = tmp_path / "bogus.restart"
+    bogus.write_bytes(b"not a restart file")
+    rc, log = run_script(tmp_path, f"atom_style granular\nread_restart {bogus}\n", "in.rr")
+    assert rc != 0 and "Not a restart file" in log
+    rc, log = run_script(tmp_path, HEADER + f"read_restart {bogus}\n", "in.rr2")
+    assert rc != 0 and "after simulation box is defined" in log
+    rc, log = run_script(tmp_path, HEADER + f"write_restart {tmp_path}/x.restart\n", "in.wr")
+    assert rc != 0 and "preceding run" in log
