@@ -81,7 +81,7 @@ enum dem_normal_model {
 
 /* Switches of one contact model (pair_style gran / fix wall/gran keywords):
  * model <normal> tangential history|no_history [cohesion sjkr]
- * [rolling_friction cdt] and the on/off settings the models register
+ * [rolling_friction cdt|epsd2] and the on/off settings the models register
  * (Settings::registerOnOff in each model's registerSettings). Zero-filled
  * fields beyond the first four select hertz, history, no cohesion. */
 typedef struct dem_contact_model {
@@ -96,6 +96,9 @@ typedef struct dem_contact_model {
   int kt_to_kn;           /* hooke: ktToKnUser on (normal_model_hooke.h:100) */
   int absolute_damping;   /* hooke/stiffness: absolute_damping on (normal_model_hooke_stiffness.h:81) */
   int tangential_reduce;  /* sjkr: tangential_reduce on (cohesion_model_sjkr.h:72) */
+  int rolling_epsd2;      /* rolling_friction epsd2 (rolling_model_epsd2.h): an elastic
+                             rolling spring whose torque is kept as contact history;
+                             pair and plane-wall contacts with tangential history */
 } dem_contact_model;
 
 typedef struct dem_stats {
@@ -304,6 +307,14 @@ int dem_get_pairs(dem_handle *h, int64_t cap, int *tags, int *flags, double *his
 /* Primitive-wall shear history, n*nwalls*3 values, wall-major inside each particle
  * (FixWallGran's property/atom history, fix_wall_gran.cpp:498-522). */
 int dem_get_wall_history(dem_handle *h, double *hist);
+
+/* The rolling-torque history of rolling_friction epsd2 (the r_torque{x,y,z}_old
+ * values RollingModel<EPSD2> registers, rolling_model_epsd2.h:75-77), for the
+ * pairs of dem_get_pairs in the same order and orientation (rhist[3k..3k+2]),
+ * and for the primitive walls as dem_get_wall_history. DEM_ERR_STATE when no
+ * contact model uses epsd2. */
+int dem_get_pair_rolling(dem_handle *h, int64_t cap, double *rhist, int64_t *npairs);
+int dem_get_wall_rolling(dem_handle *h, double *rhist);
 
 /* Counters and timing of the handle. */
 int dem_get_stats(dem_handle *h, dem_stats *out);

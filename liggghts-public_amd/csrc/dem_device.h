@@ -55,6 +55,8 @@ struct DevParams {
   int per[3];
   int pair_model;          // contact-model bits of pair_style gran (kModel*)
   int generic;             // some pair, wall or mesh model is not Hertz / history
+  int hist_w;              // history values per pair / wall contact: 3 shear, 6 with
+                           // the epsd2 rolling torque (generic kernels only)
   double char_vel;         // characteristicVelocity of model hooke
   int has_grav, grav_first;
   double gacc[3];
@@ -83,12 +85,13 @@ struct DevParams {
 // Contact-model bits (one int per pair style / wall fix): bit 0 rolling cdt,
 // 1 tangential damping, 2 limitForce, 3 torsionTorque, 4-5 the normal model,
 // 6 tangential no_history, 7 cohesion sjkr, 8 ktToKnUser (hooke),
-// 9 absolute_damping (hooke/stiffness), 10 tangential_reduce (sjkr).
+// 9 absolute_damping (hooke/stiffness), 10 tangential_reduce (sjkr),
+// 11 rolling_friction epsd2 (elastic rolling spring with a torque history).
 constexpr int kModelNormalShift = 4;
 constexpr int kNormalHertz = 0, kNormalHooke = 1, kNormalHertzStiffness = 2,
               kNormalHookeStiffness = 3;
 constexpr int kModelNoHistory = 1 << 6, kModelSjkr = 1 << 7, kModelKtToKn = 1 << 8,
-              kModelAbsDamping = 1 << 9, kModelTangReduce = 1 << 10;
+              kModelAbsDamping = 1 << 9, kModelTangReduce = 1 << 10, kModelEpsd2 = 1 << 11;
 // bits that need the generic contact law (anything beyond Hertz / history)
 constexpr int kModelGenericMask = ~0xf;
 
@@ -133,7 +136,7 @@ struct AuxBuf {
   int *tag;
   int *orig;      // caller index
   int *meta;      // list length | wall-history bits
-  void *whist;    // wall shear history, [wall][3][cap], double or float
+  void *whist;    // wall history, [wall][hist_w][cap], double or float
   int *mslot;     // mesh history slots: triangle index or -1, [mesh][slot][cap]
   void *mhist;    // mesh shear history, [mesh][slot][3][cap], double or float
 };
@@ -148,7 +151,7 @@ struct MeshStep {
 
 struct ListBuf {
   uint32_t *nbr;  // [W][cap] neighbor index | contact flag
-  void *hist;     // [W][3][cap] tangential history, double or float
+  void *hist;     // [W][hist_w][cap] tangential (+ rolling) history, double or float
 };
 
 // Rigid rotation of the moving meshes for one step.

@@ -96,13 +96,14 @@ struct CIn {
 // contact-model switches (the kModel* bits)
 struct Model {
   int rolling, tdamp, limitf, torsion;
-  int normal, nohist, sjkr, kt2kn, absdamp, treduce;
+  int normal, nohist, sjkr, kt2kn, absdamp, treduce, epsd2;
 };
 
 __device__ __forceinline__ Model model_from_bits(int b) {
   return Model{b & 1, (b >> 1) & 1, (b >> 2) & 1, (b >> 3) & 1,
                (b >> kModelNormalShift) & 3, (b & kModelNoHistory) != 0, (b & kModelSjkr) != 0,
-               (b & kModelKtToKn) != 0, (b & kModelAbsDamping) != 0, (b & kModelTangReduce) != 0};
+               (b & kModelKtToKn) != 0, (b & kModelAbsDamping) != 0, (b & kModelTangReduce) != 0,
+               (b & kModelEpsd2) != 0};
 }
 
 __device__ __forceinline__ double tpow(double x, double y) { return pow(x, y); }
@@ -133,8 +134,8 @@ __device__ __forceinline__ void load_model_coeffs(const MatTables &mt_, const De
 
 template <typename T, bool WALL, bool GEN = false>
 __device__ __forceinline__ void contact_chain(const Model &p, const CIn<T> &c, T dt,
-                                              int shearupdate, T sh[3], T dF[3], T dTi[3],
-                                              T dTj[3]) {
+                                              int shearupdate, T sh[3], T rh[3], T dF[3],
+                                              T dTi[3], T dTj[3]) {
   const T enx = c.en[0], eny = c.en[1], enz = c.en[2];
   const T vr1 = c.vr[0], vr2 = c.vr[1], vr3 = c.vr[2];
   const T vn = vr1 * enx + vr2 * eny + vr3 * enz;
@@ -284,8 +285,53 @@ __device__ __forceinline__ void contact_chain(const Model &p, const CIn<T> &c, T
     dTj[2] = T(0) + -crj * tor3;
   }
 
-  // rolling friction, constant directional torque
-  if (p.rolling) {
+  if (GEN && p.epsd2) {
+    // RollingModel<EPSD2> (rolling_model_epsd2.h:98-221): elastic spring
+    // kr = kt R*^2 on the relative spin (its twist about en dropped unless
+    // torsionTorque), capped at mu_r |Fn| R*; rh carries the torque
+    T w0, w1, w2;
+    if (WALL) {
+      w0 = wr1;
+      w1 = wr2;
+      w2 = wr3;
+    } else {
+      w0 = c.wi[0] - c.wj[0];
+      w1 = c.wi[1] - c.wj[1];
+      w2 = c.wi[2] - c.wj[2];
+    }
+    if (!p.torsion) {
+      const T d = w0 * enx + w1 * eny + w2 * enz;
+      const T n0 = enx * d, n1 = eny * d, n2 = enz * d;
+      w0 = w0 - n0;
+      w1 = w1 - n1;
+      w2 = w2 - n2;
+    }
+    const T kr = kt * reff * reff;
+    const T dk = dt * kr;
+    T rt0 = rh[0] + w0 * dk, rt1 = rh[1] + w1 * dk, rt2 = rh[2] + w2 * dk;
+    const T rmag = sqrt(rt0 * rt0 + rt1 * rt1 + rt2 * rt2);
+    const T rmax = fabs(Fn) * reff * c.mur;
+    if (rmag > rmax) {
+      const T factor = rmax / rmag;
+      rt0 *= factor;
+      rt1 *= factor;
+      rt2 *= factor;
+    }
+    if (shearupdate) {
+      rh[0] = rt0;
+      rh[1] = rt1;
+      rh[2] = rt2;
+    }
+    dTi[0] -= rt0;
+    dTi[1] -= rt1;
+    dTi[2] -= rt2;
+    if (!WALL) {
+      dTj[0] += rt0;
+      dTj[1] += rt1;
+      dTj[2] += rt2;
+    }
+  } else if (p.rolling) {
+    // rolling friction, constant directional torque
     const T rmu = c.mur;
     T rt0 = T(0), rt1 = T(0), rt2 = T(0);
     if (WALL) {
@@ -463,7 +509,8 @@ __device__ inline void mesh_contacts(const StepArgs &a, int i, const double4 &pi
           cc.acont = (T)((pi.w * pi.w - r * r) * kPi);
         }
         T dF[3], dTi[3], dTj[3];
-        contact_chain<T, true, GEN>(mdl, cc, (T)p.dt, a.shearupdate, sh, dF, dTi, dTj);
+        T rh[3] = {T(0), T(0), T(0)};  // epsd2 is not taken on meshes
+        contact_chain<T, true, GEN>(mdl, cc, (T)p.dt, a.shearupdate, sh, rh, dF, dTi, dTj);
         if (a.count && ((p.mesh_stress >> m) & 1)) {
           // MeshModuleStress::add_particle_contribution (mesh_module_stress.cpp:
           // 276-300): the force on the mesh is minus the particle's, its torque
@@ -653,6 +700,7 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
   T fx = T(0), fy = T(0), fz = T(0), tx = T(0), ty = T(0), tz = T(0);
   T *hist = (T *)a.list.hist;
   const size_t cap = (size_t)a.cap;
+  const size_t hw = GEN ? (size_t)p.hist_w : 3;  // history values per contact
 
   // Broad phase in groups of GROUP entries: the index loads of a group are
   // independent, then its position gathers, so a wavefront has GROUP
@@ -740,21 +788,31 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
     // tangential no_history keeps no history: its slots hold the zeros
     // written at the first touch and are neither read nor rewritten
     const bool keeps = !(GEN && pm.nohist);
-    T sh[3];
-    const size_t hb = (size_t)k * 3 * cap + i;
+    T sh[3], rh[3] = {T(0), T(0), T(0)};
+    const size_t hb = (size_t)k * hw * cap + i;
     if (flagged && keeps) {
       sh[0] = hist[hb];
       sh[1] = hist[hb + cap];
       sh[2] = hist[hb + 2 * cap];
+      if (GEN && pm.epsd2) {
+        rh[0] = hist[hb + 3 * cap];
+        rh[1] = hist[hb + 4 * cap];
+        rh[2] = hist[hb + 5 * cap];
+      }
     } else {
       sh[0] = sh[1] = sh[2] = T(0);
     }
     T dF[3], dTi[3], dTj[3];
-    contact_chain<T, false, GEN>(pm, c, dt, a.shearupdate, sh, dF, dTi, dTj);
+    contact_chain<T, false, GEN>(pm, c, dt, a.shearupdate, sh, rh, dF, dTi, dTj);
     if ((a.shearupdate && keeps) || !flagged) {
       hist[hb] = sh[0];
       hist[hb + cap] = sh[1];
       hist[hb + 2 * cap] = sh[2];
+      if (GEN && hw == 6) {
+        hist[hb + 3 * cap] = rh[0];
+        hist[hb + 4 * cap] = rh[1];
+        hist[hb + 5 * cap] = rh[2];
+      }
     }
     if (!flagged) a.list.nbr[slot] = e | kFlagBit;
     fx += dF[0];
@@ -792,7 +850,7 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
         deltan = param - xa - pi.w;
       }
       T *wh = (T *)a.aux.whist;
-      const size_t wb = (size_t)w * 3 * cap + i;
+      const size_t wb = (size_t)w * hw * cap + i;
       const int bit = 1 << (kMetaWallShift + w);
       if (deltan > p.cutneighmax) continue;
       if (deltan <= 0) {
@@ -845,20 +903,30 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
           c.acont = (T)((pi.w * pi.w - r * r) * kPi);  // cohesion_model_sjkr.h:95-96
         }
         const bool keeps = !(GEN && wm.nohist);  // as for pairs: no_history slots stay zero
-        T sh[3];
+        T sh[3], rh[3] = {T(0), T(0), T(0)};
         if ((meta & bit) && keeps) {
           sh[0] = wh[wb];
           sh[1] = wh[wb + cap];
           sh[2] = wh[wb + 2 * cap];
+          if (GEN && wm.epsd2) {
+            rh[0] = wh[wb + 3 * cap];
+            rh[1] = wh[wb + 4 * cap];
+            rh[2] = wh[wb + 5 * cap];
+          }
         } else {
           sh[0] = sh[1] = sh[2] = T(0);
         }
         T dF[3], dTi[3], dTj[3];
-        contact_chain<T, true, GEN>(wm, c, dt, a.shearupdate, sh, dF, dTi, dTj);
+        contact_chain<T, true, GEN>(wm, c, dt, a.shearupdate, sh, rh, dF, dTi, dTj);
         if (keeps || !(meta & bit)) {
           wh[wb] = sh[0];
           wh[wb + cap] = sh[1];
           wh[wb + 2 * cap] = sh[2];
+          if (GEN && hw == 6) {
+            wh[wb + 3 * cap] = rh[0];
+            wh[wb + 4 * cap] = rh[1];
+            wh[wb + 5 * cap] = rh[2];
+          }
         }
         meta |= bit;
         fx += dF[0];
@@ -1225,9 +1293,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(RebuildArgs a) {
   a.aalt.meta[i] = meta & ~kMetaCountMask;
   const T *wi = (const T *)a.acur.whist;
   T *wo = (T *)a.aalt.whist;
+  const int hw = a.prm.hist_w;
   for (int w = 0; w < a.prm.nwalls; w++) {
     if (!(meta & (1 << (kMetaWallShift + w)))) continue;
-    for (int d = 0; d < 3; d++) wo[(size_t)(w * 3 + d) * cap + i] = wi[(size_t)(w * 3 + d) * cap + o];
+    for (int d = 0; d < hw; d++) wo[(size_t)(w * hw + d) * cap + i] = wi[(size_t)(w * hw + d) * cap + o];
   }
   a.xhold[i] = q.x;
   a.xhold[cap + i] = q.y;
@@ -1299,11 +1368,10 @@ __global__ __launch_bounds__(kBlock) void k_build(RebuildArgs a) {
                 for (int m = 0; m < oldcnt; m++) {
                   const uint32_t oe = a.oldl.nbr[(size_t)m * cap + io];
                   if ((oe & kFlagBit) && (oe & kIndexMask) == jo) {
-                    const size_t ob = (size_t)m * 3 * cap + io;
-                    const size_t nb = (size_t)cnt * 3 * cap + i;
-                    nhist[nb] = ohist[ob];
-                    nhist[nb + cap] = ohist[ob + cap];
-                    nhist[nb + 2 * cap] = ohist[ob + 2 * cap];
+                    const size_t hw = (size_t)p.hist_w;
+                    const size_t ob = (size_t)m * hw * cap + io;
+                    const size_t nb = (size_t)cnt * hw * cap + i;
+                    for (size_t d = 0; d < hw; d++) nhist[nb + d * cap] = ohist[ob + d * cap];
                     entry |= kFlagBit;
                     break;
                   }

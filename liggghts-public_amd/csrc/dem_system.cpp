@@ -46,7 +46,7 @@ int model_bits(const dem_contact_model &m) {
          (m.torsion_torque ? 8 : 0) | ((m.normal_model & 3) << dem::kModelNormalShift) |
          (m.tangential_model ? dem::kModelNoHistory : 0) | (m.cohesion_model ? dem::kModelSjkr : 0) |
          (m.kt_to_kn ? dem::kModelKtToKn : 0) | (m.absolute_damping ? dem::kModelAbsDamping : 0) |
-         (m.tangential_reduce ? dem::kModelTangReduce : 0);
+         (m.tangential_reduce ? dem::kModelTangReduce : 0) | (m.rolling_epsd2 ? dem::kModelEpsd2 : 0);
 }
 
 const char *normal_model_name(int nm) {
@@ -64,6 +64,7 @@ struct dem_handle {
   hipStream_t stream = nullptr;
   bool mixed = false;
   size_t fsz = 8;  // bytes of one history / omega component
+  int hw = 3;      // history values per pair / plane contact (6 with epsd2)
 
   // materials
   int ntypes = 0;
@@ -199,6 +200,7 @@ void clear_graphs(dem_handle *h) {
   h->graphs.clear();
 }
 
+
 void free_particles(dem_handle *h) {
   clear_graphs(h);
   // everything except the material block and control block
@@ -270,7 +272,8 @@ dem::DevParams make_devparams(const dem_handle *h) {
   }
   d.pair_model = model_bits(h->pair_model);
   d.char_vel = h->char_vel;
-  d.generic = uses_generic_models(h) ? 1 : 0;
+  d.generic = (uses_generic_models(h) || h->hw != 3) ? 1 : 0;
+  d.hist_w = h->hw;
   d.has_grav = p.has_gravity;
   d.grav_first = p.gravity_first;
   for (int k = 0; k < 3; k++) d.gacc[k] = p.grav_acc[k];
@@ -749,6 +752,15 @@ int type_bits(const int *type, const std::vector<int> &inst_of, int64_t i) {
   return type[i] + (inst_of.empty() ? 0 : (inst_of[i] << dem::kTypeBits));
 }
 
+// history values per pair / plane-wall contact: the shear (3), and the
+// rolling torque beside it (3 more) when some model is epsd2. Models, walls
+// and meshes are fixed before the particles, which allocate the histories.
+int needed_hist_width(const dem_handle *h) {
+  bool epsd2 = (model_bits(h->pair_model) & dem::kModelEpsd2) != 0;
+  for (const Wall &w : h->walls) epsd2 = epsd2 || (w.model & dem::kModelEpsd2);
+  return epsd2 ? 6 : 3;
+}
+
 int check_ready(dem_handle *h) {
   if (!h) return DEM_ERR_ARG;
   if (!h->mat_set) return fail(h, DEM_ERR_STATE, "materials not set (fix property/global)");
@@ -777,7 +789,17 @@ int check_ready(dem_handle *h) {
       return fail(h, DEM_ERR_ARG, "characteristicVelocity >= 1e-2 required for SI units");
     if ((b & dem::kModelSjkr) && !h->have_ced)
       return fail(h, DEM_ERR_ARG, "cohesion sjkr needs cohesionEnergyDensity");
+    // the rolling torque is kept beside the tangential history
+    // (rolling_model_epsd2.h:172)
+    if ((b & dem::kModelEpsd2) && (b & dem::kModelNoHistory))
+      return fail(h, DEM_ERR_ARG, "rolling_friction epsd2 needs tangential history");
+    if ((b & dem::kModelEpsd2) && (b & 1))
+      return fail(h, DEM_ERR_ARG, "rolling_friction: cdt and epsd2 are exclusive");
   }
+  for (const auto &m : h->meshes)
+    if (m.model & dem::kModelEpsd2)
+      return fail(h, DEM_ERR_ARG, "rolling_friction epsd2 is not available on mesh walls on this path");
+  if (needed_hist_width(h) != h->hw) return fail(h, DEM_ERR_STATE, "contact history width out of date");
   return DEM_OK;
 }
 
@@ -1247,6 +1269,7 @@ int dem_set_particles(dem_handle *h, int64_t n, const double *x, const double *v
       for (int64_t k = 0; k < h->inst_count[q]; k++) inst_of[i0++] = q;
   }
   free_particles(h);
+  h->hw = needed_hist_width(h);
   h->n = n;
   h->cap = (int)n;
   h->rmax = rmax;
@@ -1261,9 +1284,9 @@ int dem_set_particles(dem_handle *h, int64_t n, const double *x, const double *v
     if ((rc = dalloc(h, (void **)&h->aux[b].tag, sizeof(int) * cap))) return rc;
     if ((rc = dalloc(h, (void **)&h->aux[b].orig, sizeof(int) * cap))) return rc;
     if ((rc = dalloc(h, (void **)&h->aux[b].meta, sizeof(int) * cap))) return rc;
-    if ((rc = dalloc(h, &h->aux[b].whist, h->fsz * 3 * nw * cap))) return rc;
+    if ((rc = dalloc(h, &h->aux[b].whist, h->fsz * h->hw * nw * cap))) return rc;
     if ((rc = dalloc(h, (void **)&h->lst[b].nbr, sizeof(uint32_t) * h->W * cap))) return rc;
-    if ((rc = dalloc(h, &h->lst[b].hist, h->fsz * 3 * h->W * cap))) return rc;
+    if ((rc = dalloc(h, &h->lst[b].hist, h->fsz * h->hw * h->W * cap))) return rc;
   }
   if ((rc = dalloc(h, (void **)&h->xhold, sizeof(double) * 3 * cap))) return rc;
   if ((rc = dalloc(h, (void **)&h->fout, sizeof(double) * 6 * cap))) return rc;
@@ -1712,7 +1735,8 @@ namespace {
 struct RestartHeader {
   char magic[8];
   int32_t version, mixed, ntypes, cap, W, nwalls, nmesh, ntri;
-  int32_t cur_st, cur_l, cur_aux, tri_cur, have_list, pending_mesh_rebuild, periodic[3], pad;
+  int32_t cur_st, cur_l, cur_aux, tri_cur, have_list, pending_mesh_rebuild, periodic[3];
+  int32_t hist_w;  // history values per pair / plane contact (files without it: 0, meaning 3)
   int64_t n, ntimestep, nrebuilds, last_build;
   double mesh_time, dt, boxlo[3], boxhi[3];
 };
@@ -1736,11 +1760,11 @@ std::vector<std::pair<void *, size_t>> restart_blobs(dem_handle *h) {
       {ax.tag, sizeof(int) * n},
       {ax.orig, sizeof(int) * n},
       {ax.meta, sizeof(int) * n},
-      {ax.whist, h->fsz * 3 * nw * cap},
+      {ax.whist, h->fsz * h->hw * nw * cap},
       {ax.mslot, sizeof(int) * nm * dem::kMeshSlots * cap},
       {ax.mhist, h->fsz * 3 * nm * dem::kMeshSlots * cap},
       {ls.nbr, sizeof(uint32_t) * h->W * cap},
-      {ls.hist, h->fsz * 3 * h->W * cap},
+      {ls.hist, h->fsz * h->hw * h->W * cap},
       {h->xhold, sizeof(double) * 3 * cap},
       {h->fout, sizeof(double) * 6 * cap}};
   if (h->ntri > 0) {
@@ -1793,6 +1817,7 @@ int dem_write_restart(dem_handle *h, const char *path) {
   hd.ntypes = h->ntypes;
   hd.cap = h->cap;
   hd.W = h->W;
+  hd.hist_w = h->hw;
   hd.nwalls = (int32_t)h->walls.size();
   hd.nmesh = (int32_t)h->meshes.size();
   hd.ntri = h->ntri;
@@ -1891,6 +1916,11 @@ int dem_read_restart(dem_handle *h, const char *path) {
     fclose(f);
     return fail(h, DEM_ERR_ARG, std::string("Not a restart file of this library: ") + path);
   }
+  if ((hd.hist_w ? hd.hist_w : 3) != h->hw) {
+    fclose(f);
+    return fail(h, DEM_ERR_ARG,
+                "read_restart: the file was written with a different rolling model (epsd2 history)");
+  }
   if (hd.n != h->n || hd.cap != h->cap || hd.W != h->W || hd.mixed != (h->mixed ? 1 : 0) ||
       hd.nwalls != (int)h->walls.size() || hd.nmesh != (int)h->meshes.size() || hd.ntri != h->ntri ||
       hd.dt != h->p.dt) {
@@ -1936,8 +1966,11 @@ int dem_read_restart(dem_handle *h, const char *path) {
   return DEM_OK;
 }
 
-int dem_get_pairs(dem_handle *h, int64_t cap_out, int *tags, int *flags, double *hist,
-                  int64_t *npairs) {
+namespace {
+// pairs of the half list with history values comp0..comp0+2 (0 shear, 3 the
+// epsd2 rolling torque)
+int get_pairs(dem_handle *h, int64_t cap_out, int *tags, int *flags, double *hist,
+              int64_t *npairs, int comp0) {
   if (!h || !npairs) return DEM_ERR_ARG;
   *npairs = 0;
   if (!h->have_list) return DEM_OK;
@@ -1947,7 +1980,7 @@ int dem_get_pairs(dem_handle *h, int64_t cap_out, int *tags, int *flags, double 
   const int W = h->W;
   std::vector<uint32_t> nbr((size_t)W * cap);
   std::vector<int> meta(n), tag(n);
-  std::vector<char> hraw(h->fsz * 3 * W * cap);
+  std::vector<char> hraw(h->fsz * h->hw * W * cap);
   const ListBuf &l = h->lst[h->cur_l];
   HIPCHK(h, hipMemcpy(nbr.data(), l.nbr, sizeof(uint32_t) * W * cap, hipMemcpyDeviceToHost));
   HIPCHK(h, hipMemcpy(hraw.data(), l.hist, hraw.size(), hipMemcpyDeviceToHost));
@@ -1967,7 +2000,7 @@ int dem_get_pairs(dem_handle *h, int64_t cap_out, int *tags, int *flags, double 
         const bool fl = (e & dem::kFlagBit) != 0;
         flags[np] = fl ? 1 : 0;
         for (int d = 0; d < 3; d++) {
-          const size_t idx = ((size_t)k * 3 + d) * cap + i;
+          const size_t idx = ((size_t)k * h->hw + comp0 + d) * cap + i;
           double val = h->mixed ? (double)((float *)hraw.data())[idx] : ((double *)hraw.data())[idx];
           hist[3 * np + d] = fl ? val : 0.0;
         }
@@ -1979,7 +2012,7 @@ int dem_get_pairs(dem_handle *h, int64_t cap_out, int *tags, int *flags, double 
   return DEM_OK;
 }
 
-int dem_get_wall_history(dem_handle *h, double *out) {
+int get_wall_history(dem_handle *h, double *out, int comp0) {
   if (!h || !out) return DEM_ERR_ARG;
   if (h->n <= 0) return fail(h, DEM_ERR_STATE, "no particles");
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1988,7 +2021,7 @@ int dem_get_wall_history(dem_handle *h, double *out) {
   const int nw = (int)h->walls.size();
   if (nw == 0) return DEM_OK;
   std::vector<int> meta(n), orig(n);
-  std::vector<char> raw(h->fsz * 3 * nw * cap);
+  std::vector<char> raw(h->fsz * h->hw * nw * cap);
   const AuxBuf &a = h->aux[h->cur_aux];
   HIPCHK(h, hipMemcpy(meta.data(), a.meta, sizeof(int) * n, hipMemcpyDeviceToHost));
   HIPCHK(h, hipMemcpy(orig.data(), a.orig, sizeof(int) * n, hipMemcpyDeviceToHost));
@@ -1998,13 +2031,34 @@ int dem_get_wall_history(dem_handle *h, double *out) {
     for (int w = 0; w < nw; w++) {
       const bool has = meta[i] & (1 << (dem::kMetaWallShift + w));
       for (int d = 0; d < 3; d++) {
-        const size_t idx = ((size_t)w * 3 + d) * cap + i;
+        const size_t idx = ((size_t)w * h->hw + comp0 + d) * cap + i;
         const double val = h->mixed ? (double)((float *)raw.data())[idx] : ((double *)raw.data())[idx];
         out[((size_t)o * nw + w) * 3 + d] = has ? val : 0.0;
       }
     }
   }
   return DEM_OK;
+}
+}  // namespace
+
+int dem_get_pairs(dem_handle *h, int64_t cap_out, int *tags, int *flags, double *hist,
+                  int64_t *npairs) {
+  return get_pairs(h, cap_out, tags, flags, hist, npairs, 0);
+}
+
+int dem_get_wall_history(dem_handle *h, double *out) { return get_wall_history(h, out, 0); }
+
+int dem_get_pair_rolling(dem_handle *h, int64_t cap_out, double *rhist, int64_t *npairs) {
+  if (!h || !npairs) return DEM_ERR_ARG;
+  if (h->hw != 6) return fail(h, DEM_ERR_STATE, "no rolling history: no contact model uses epsd2");
+  std::vector<int> tags(2 * std::max<int64_t>(cap_out, 1)), flags(std::max<int64_t>(cap_out, 1));
+  return get_pairs(h, cap_out, tags.data(), flags.data(), rhist, npairs, 3);
+}
+
+int dem_get_wall_rolling(dem_handle *h, double *out) {
+  if (!h) return DEM_ERR_ARG;
+  if (h->hw != 6) return fail(h, DEM_ERR_STATE, "no rolling history: no contact model uses epsd2");
+  return get_wall_history(h, out, 3);
 }
 
 int dem_get_stats(dem_handle *h, dem_stats *out) {

@@ -15,7 +15,7 @@
 //                  gammat_abs|cohesionEnergyDensity peratomtypepair <n> ...
 //   fix <id> <grp> property/global characteristicVelocity scalar <v>
 //   pair_style gran model hertz|hooke|hertz/stiffness|hooke/stiffness
-//              tangential history|no_history [cohesion sjkr] [rolling_friction cdt]
+//              tangential history|no_history [cohesion sjkr] [rolling_friction cdt|epsd2]
 //              [tangential_damping on|off] [limitForce on|off] [torsionTorque on|off]
 //              [ktToKnUser on|off] [absolute_damping on|off] [tangential_reduce on|off]
 //   pair_coeff * * | timestep <dt>
@@ -607,7 +607,9 @@ struct Interp {
         k += 2;
       } else if (t[k] == "rolling_friction" && k + 1 < t.size()) {
         if (t[k + 1] == "cdt") mm.rolling_cdt = 1;
-        else if (t[k + 1] != "off") die("Only 'rolling_friction cdt' is available, got " + t[k + 1]);
+        else if (t[k + 1] == "epsd2") mm.rolling_epsd2 = 1;
+        else if (t[k + 1] != "off")
+          die("Only 'rolling_friction cdt' and 'rolling_friction epsd2' are available, got " + t[k + 1]);
         k += 2;
       } else if (t[k] == "cohesion" && k + 1 < t.size()) {
         if (t[k + 1] == "sjkr") mm.cohesion_model = 1;
@@ -716,10 +718,10 @@ struct Interp {
     if (skin < 0) die("run: neighbor skin not set");
     if (every != 1 || no_check || cdf != 1.0)
       die("superquadrics: neigh_modify every/check/contact_distance_factor must keep their defaults");
-    if (pair_m.rolling_cdt || pair_m.torsion_torque)
+    if (pair_m.rolling_cdt || pair_m.rolling_epsd2 || pair_m.torsion_torque)
       die("superquadrics: rolling friction is not available on this path");
     for (auto &wl : walls)
-      if (wl.model.rolling_cdt || wl.model.torsion_torque)
+      if (wl.model.rolling_cdt || wl.model.rolling_epsd2 || wl.model.torsion_torque)
         die("superquadrics: rolling friction is not available on this path");
     std::vector<const dem_contact_model *> sq_models{&pair_m};
     for (auto &wl : walls) sq_models.push_back(&wl.model);
@@ -783,7 +785,7 @@ struct Interp {
     bool any_rolling = false, elastic = false, stiff = false, abs_damp = false, rel_damp = false;
     bool hooke = false, sjkr = false;
     for (const dem_contact_model *cm : models) {
-      any_rolling = any_rolling || cm->rolling_cdt;
+      any_rolling = any_rolling || cm->rolling_cdt || cm->rolling_epsd2;
       const bool st = cm->normal_model == DEM_NORMAL_HERTZ_STIFFNESS ||
                       cm->normal_model == DEM_NORMAL_HOOKE_STIFFNESS;
       elastic = elastic || !st;
@@ -804,7 +806,7 @@ struct Interp {
     }
     if ((int)mu.size() != ntypes * ntypes) die("coefficientFriction needs ntypes^2 values");
     if (any_rolling && (int)mur.size() != ntypes * ntypes)
-      die("rolling_friction cdt needs coefficientRollingFriction (ntypes^2 values)");
+      die("rolling_friction needs coefficientRollingFriction (ntypes^2 values)");
     if (abs_damp && rel_damp)
       die("hooke/stiffness with and without absolute_damping in one run is not available on this path");
     auto pair_prop = [&](const char *name, bool need) -> const double * {

@@ -21,7 +21,8 @@ EXPORTED = [
     "dem_create", "dem_destroy", "dem_last_error", "dem_set_materials",
     "dem_set_particles", "dem_add_plane_wall", "dem_setup", "dem_run",
     "dem_run_continue", "dem_compute_forces", "dem_get_state", "dem_get_pairs",
-    "dem_get_wall_history", "dem_get_stats", "dem_set_kernel_timing",
+    "dem_get_wall_history", "dem_get_pair_rolling", "dem_get_wall_rolling", "dem_get_stats",
+    "dem_set_kernel_timing",
     "dem_run_script", "dem_add_mesh", "dem_mesh_rotate", "dem_get_mesh_nodes", "dem_mesh_linear", "dem_mesh_wiggle", "dem_mesh_stress", "dem_get_mesh_force", "dem_mesh_viblin", "dem_mesh_vibrot",
     "dem_write_restart", "dem_read_restart", "dem_restart_info", "dem_restart_particles",
     "dem_get_mesh_topology", "dem_get_mesh_history", "dem_set_instances",
@@ -70,7 +71,8 @@ class DemContactModel(ctypes.Structure):
                 ("limit_force", ctypes.c_int), ("torsion_torque", ctypes.c_int),
                 ("normal_model", ctypes.c_int), ("tangential_model", ctypes.c_int),
                 ("cohesion_model", ctypes.c_int), ("kt_to_kn", ctypes.c_int),
-                ("absolute_damping", ctypes.c_int), ("tangential_reduce", ctypes.c_int)]
+                ("absolute_damping", ctypes.c_int), ("tangential_reduce", ctypes.c_int),
+                ("rolling_epsd2", ctypes.c_int)]
 
 
 # enum dem_normal_model and the tangential / cohesion model codes
@@ -83,7 +85,7 @@ def contact_model(d):
     """dem_contact_model from a dict: rolling_cdt, tangential_damping,
     limit_force, torsion_torque, normal ("hertz" | "hooke" | "hertz/stiffness" |
     "hooke/stiffness"), tangential ("history" | "no_history"), cohesion
-    (None | "sjkr"), kt_to_kn, absolute_damping, tangential_reduce."""
+    (None | "sjkr"), kt_to_kn, absolute_damping, tangential_reduce, rolling_epsd2."""
     m = DemContactModel()
     m.rolling_cdt = int(d.get("rolling_cdt", 0))
     m.tangential_damping = int(d.get("tangential_damping", 1))
@@ -95,6 +97,7 @@ def contact_model(d):
     m.kt_to_kn = int(d.get("kt_to_kn", 0))
     m.absolute_damping = int(d.get("absolute_damping", 0))
     m.tangential_reduce = int(d.get("tangential_reduce", 0))
+    m.rolling_epsd2 = int(d.get("rolling_epsd2", 0))
     return m
 
 
@@ -167,6 +170,8 @@ def load_library(path=LIB_PATH):
     lib.dem_get_state.argtypes = [h, dp, dp, dp, dp, dp]
     lib.dem_get_pairs.argtypes = [h, ctypes.c_int64, ip, ip, dp, P(ctypes.c_int64)]
     lib.dem_get_wall_history.argtypes = [h, dp]
+    lib.dem_get_pair_rolling.argtypes = [h, ctypes.c_int64, dp, P(ctypes.c_int64)]
+    lib.dem_get_wall_rolling.argtypes = [h, dp]
     lib.dem_get_stats.argtypes = [h, P(DemStats)]
     lib.dem_set_kernel_timing.argtypes = [h, ctypes.c_int]
     lib.dem_run_script.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
@@ -425,6 +430,21 @@ class GpuSystem:
         out = np.zeros((self.n, max(self.nwalls, 1), 3))
         if self.nwalls:
             self._check(self.lib.dem_get_wall_history(self.h, _dp(out)))
+        return out[:, :self.nwalls, :]
+
+    def pair_rolling(self):
+        """epsd2 rolling-torque histories (P,3), in the order and orientation of pairs()."""
+        total = ctypes.c_int64(0)
+        self._check(self.lib.dem_get_pair_rolling(self.h, 0, None, ctypes.byref(total)))
+        cap = max(int(total.value), 1)
+        out = np.zeros((cap, 3))
+        self._check(self.lib.dem_get_pair_rolling(self.h, cap, _dp(out), ctypes.byref(total)))
+        return out[:int(total.value)]
+
+    def wall_rolling(self):
+        out = np.zeros((self.n, max(self.nwalls, 1), 3))
+        if self.nwalls:
+            self._check(self.lib.dem_get_wall_rolling(self.h, _dp(out)))
         return out[:, :self.nwalls, :]
 
     def mesh_nodes(self, k=0):

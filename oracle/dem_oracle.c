@@ -77,7 +77,7 @@ typedef struct {
 } contact_in;
 
 static void model_chain(const orc_model *mdl, const orc_materials *mat, const contact_in *c,
-                        double dt, int shearupdate, double *shear, double dFi[3],
+                        double dt, int shearupdate, double *shear, double *rhist, double dFi[3],
                         double dTi[3], double dTj[3]) {
   const double enx = c->en[0], eny = c->en[1], enz = c->en[2];
   /* surface model */
@@ -262,8 +262,48 @@ static void model_chain(const orc_model *mdl, const orc_materials *mat, const co
   }
 
 rolling:
-  /* rolling model: constant directional torque */
-  if (mdl->rolling_cdt) {
+  if (mdl->rolling_epsd2) {
+    /* RollingModel<EPSD2>::surfacesIntersect / calcRollTorque
+     * (rolling_model_epsd2.h:98-221): an elastic rolling spring kr = kt R*^2
+     * on the relative rotation (less its twist about en unless torsionTorque),
+     * capped at mu_r |Fn| R*; the torque is the contact's history */
+    const double rmu = mat->mur[it][jt];
+    double wr_tot[3];
+    if (c->is_wall) {
+      wr_tot[0] = wr1;
+      wr_tot[1] = wr2;
+      wr_tot[2] = wr3;
+    } else {
+      wr_tot[0] = c->wi[0] - c->wj[0];
+      wr_tot[1] = c->wi[1] - c->wj[1];
+      wr_tot[2] = c->wi[2] - c->wj[2];
+    }
+    if (!mdl->torsion_torque) {
+      const double wr_dot_delta = wr_tot[0] * enx + wr_tot[1] * eny + wr_tot[2] * enz;
+      const double wr_n[3] = {enx * wr_dot_delta, eny * wr_dot_delta, enz * wr_dot_delta};
+      for (int k = 0; k < 3; k++) wr_tot[k] = wr_tot[k] - wr_n[k];
+    }
+    const double kr = kt * reff * reff;
+    double rt[3];
+    for (int k = 0; k < 3; k++) rt[k] = rhist[k] + wr_tot[k] * (dt * kr);
+    const double rt_mag = sqrt(rt[0] * rt[0] + rt[1] * rt[1] + rt[2] * rt[2]);
+    const double rt_max = fabs(Fn) * reff * rmu;
+    if (rt_mag > rt_max) {
+      const double factor = rt_max / rt_mag;
+      rt[0] *= factor;
+      rt[1] *= factor;
+      rt[2] *= factor;
+    }
+    if (shearupdate)
+      for (int k = 0; k < 3; k++) rhist[k] = rt[k];
+    dTi[0] -= rt[0];
+    dTi[1] -= rt[1];
+    dTi[2] -= rt[2];
+    dTj[0] += rt[0];
+    dTj[1] += rt[1];
+    dTj[2] += rt[2];
+  } else if (mdl->rolling_cdt) {
+    /* rolling model: constant directional torque */
     const double rmu = mat->mur[it][jt];
     double rt[3] = {0.0, 0.0, 0.0};
     if (c->is_wall) {
@@ -313,6 +353,16 @@ int orc_contact_pair(const orc_model *mdl, const orc_materials *mat, int itype, 
                      const double *vj, const double *wi, const double *wj, double mi,
                      double mj, double dt, int shearupdate, double *shear, double *dFi,
                      double *dTi, double *dTj) {
+  double rh[3] = {0.0, 0.0, 0.0};
+  return orc_contact_pair_r(mdl, mat, itype, jtype, del, radi, radj, vi, vj, wi, wj, mi, mj, dt,
+                            shearupdate, shear, rh, dFi, dTi, dTj);
+}
+
+int orc_contact_pair_r(const orc_model *mdl, const orc_materials *mat, int itype, int jtype,
+                       const double *del, double radi, double radj, const double *vi,
+                       const double *vj, const double *wi, const double *wj, double mi,
+                       double mj, double dt, int shearupdate, double *shear, double *rhist,
+                       double *dFi, double *dTi, double *dTj) {
   const double rsq = del[0] * del[0] + del[1] * del[1] + del[2] * del[2];
   const double radsum = radi + radj;
   if (!(rsq < radsum * radsum)) return 0;
@@ -337,7 +387,7 @@ int orc_contact_pair(const orc_model *mdl, const orc_materials *mat, int itype, 
   c.deltan_in = 0.0;
   c.itype = itype;
   c.jtype = jtype;
-  model_chain(mdl, mat, &c, dt, shearupdate, shear, dFi, dTi, dTj);
+  model_chain(mdl, mat, &c, dt, shearupdate, shear, rhist, dFi, dTi, dTj);
   return 1;
 }
 
@@ -346,6 +396,16 @@ void orc_contact_wall(const orc_model *mdl, const orc_materials *mat, int itype,
                       double sidata_deltan, const double *sidata_delta, double radi,
                       const double *vi, const double *wi, const double *vwall, double mi,
                       double dt, int shearupdate, double *shear, double *dFi, double *dTi) {
+  double rh[3] = {0.0, 0.0, 0.0};
+  orc_contact_wall_r(mdl, mat, itype, wtype, sidata_deltan, sidata_delta, radi, vi, wi, vwall, mi,
+                     dt, shearupdate, shear, rh, dFi, dTi);
+}
+
+void orc_contact_wall_r(const orc_model *mdl, const orc_materials *mat, int itype, int wtype,
+                        double sidata_deltan, const double *sidata_delta, double radi,
+                        const double *vi, const double *wi, const double *vwall, double mi,
+                        double dt, int shearupdate, double *shear, double *rhist, double *dFi,
+                        double *dTi) {
   contact_in c;
   double dTj[3];
   const double zero[3] = {0.0, 0.0, 0.0};
@@ -369,7 +429,7 @@ void orc_contact_wall(const orc_model *mdl, const orc_materials *mat, int itype,
   c.deltan_in = sidata_deltan;
   c.itype = itype;
   c.jtype = wtype;
-  model_chain(mdl, mat, &c, dt, shearupdate, shear, dFi, dTi, dTj);
+  model_chain(mdl, mat, &c, dt, shearupdate, shear, rhist, dFi, dTi, dTj);
 }
 
 /* ------------------------------------------------------------------------
@@ -471,6 +531,7 @@ struct orc_sys {
   int64_t *first;
   int *nbr, *flag;
   double *hist;
+  double *rhist; /* rolling-torque history of epsd2, beside hist */
   int64_t cap;
   int built;
   /* per-atom partner arrays, fix_contact_history.cpp:305-425 */
@@ -478,9 +539,11 @@ struct orc_sys {
   int64_t *poff;
   int *partner;
   double *phist;
+  double *prhist;
   int64_t pcap;
   /* primitive wall history, one property/atom vector per wall (fix_wall_gran.cpp:498-522) */
   double *whist;
+  double *wrhist;
   int64_t ntimestep;
   int64_t nrebuilds;
   double cutneighmax;
@@ -525,6 +588,7 @@ orc_sys *orc_create(const orc_params *p, const orc_materials *m, int n, const do
   s->npartner = (int *)xcalloc((size_t)n, sizeof(int));
   s->poff = (int64_t *)xcalloc((size_t)n + 1, sizeof(int64_t));
   s->whist = (double *)xcalloc((size_t)n * (p->nplanes > 0 ? p->nplanes : 1) * 3, sizeof(double));
+  s->wrhist = (double *)xcalloc((size_t)n * (p->nplanes > 0 ? p->nplanes : 1) * 3, sizeof(double));
   /* cutneighmax = max_ij (rmax_i + rmax_j) * cdf + skin, pair_gran.cpp:623-635, neighbor.cpp:310-332 */
   double rmax = 0.0;
   for (int i = 0; i < n; i++)
@@ -557,11 +621,14 @@ void orc_destroy(orc_sys *s) {
   free(s->nbr);
   free(s->flag);
   free(s->hist);
+  free(s->rhist);
   free(s->npartner);
   free(s->poff);
   free(s->partner);
   free(s->phist);
+  free(s->prhist);
   free(s->whist);
+  free(s->wrhist);
   free(s);
 }
 
@@ -602,8 +669,10 @@ static void carry_out(orc_sys *s) {
     s->pcap = s->poff[n] + 1024;
     free(s->partner);
     free(s->phist);
+    free(s->prhist);
     s->partner = (int *)xcalloc((size_t)s->pcap, sizeof(int));
     s->phist = (double *)xcalloc(3 * (size_t)s->pcap, sizeof(double));
+    s->prhist = (double *)xcalloc(3 * (size_t)s->pcap, sizeof(double));
   }
   memset(s->npartner, 0, sizeof(int) * (size_t)n);
   for (int i = 0; i < n; i++)
@@ -613,9 +682,13 @@ static void carry_out(orc_sys *s) {
         int64_t m = s->poff[i] + s->npartner[i]++;
         s->partner[m] = s->tag[j];
         for (int d = 0; d < 3; d++) s->phist[3 * m + d] = s->hist[3 * e + d];
+        for (int d = 0; d < 3; d++) s->prhist[3 * m + d] = s->rhist[3 * e + d];
         m = s->poff[j] + s->npartner[j]++;
         s->partner[m] = s->tag[i];
+        /* every history value is registered with newtonflag 1: the partner's
+         * copy is negated (fix_contact_history.cpp:400-410) */
         for (int d = 0; d < 3; d++) s->phist[3 * m + d] = -s->hist[3 * e + d];
+        for (int d = 0; d < 3; d++) s->prhist[3 * m + d] = -s->rhist[3 * e + d];
       }
 }
 
@@ -705,8 +778,10 @@ static void build(orc_sys *s) {
                 s->nbr[np] = j;
                 const double radsum = (s->r[i] + s->r[j]) * s->p.cdf;
                 double *h = &s->hist[3 * np];
+                double *rh = &s->rhist[3 * np];
                 int fl = 0;
                 h[0] = h[1] = h[2] = 0.0;
+                rh[0] = rh[1] = rh[2] = 0.0;
                 if (rsq < radsum * radsum) {
                   for (int64_t m = s->poff[i]; m < s->poff[i] + s->npartner[i]; m++)
                     if (s->partner[m] == s->tag[j]) {
@@ -714,6 +789,9 @@ static void build(orc_sys *s) {
                       h[0] = s->phist[3 * m];
                       h[1] = s->phist[3 * m + 1];
                       h[2] = s->phist[3 * m + 2];
+                      rh[0] = s->prhist[3 * m];
+                      rh[1] = s->prhist[3 * m + 1];
+                      rh[2] = s->prhist[3 * m + 2];
                       break;
                     }
                 }
@@ -728,9 +806,11 @@ static void build(orc_sys *s) {
       free(s->nbr);
       free(s->flag);
       free(s->hist);
+      free(s->rhist);
       s->nbr = (int *)xcalloc((size_t)s->cap, sizeof(int));
       s->flag = (int *)xcalloc((size_t)s->cap, sizeof(int));
       s->hist = (double *)xcalloc(3 * (size_t)s->cap, sizeof(double));
+      s->rhist = (double *)xcalloc(3 * (size_t)s->cap, sizeof(double));
     }
   }
   s->first[n] = np;
@@ -782,9 +862,10 @@ static void pair_compute(orc_sys *s, int shearupdate) {
       const int j = s->nbr[e];
       double del[3], dFi[3], dTi[3], dTj[3];
       min_image_del(&s->p, &s->x[3 * i], &s->x[3 * j], del);
-      if (orc_contact_pair(&s->p.model, &s->m, s->type[i], s->type[j], del, s->r[i], s->r[j],
-                           &s->v[3 * i], &s->v[3 * j], &s->w[3 * i], &s->w[3 * j], s->rm[i],
-                           s->rm[j], s->p.dt, shearupdate, &s->hist[3 * e], dFi, dTi, dTj)) {
+      if (orc_contact_pair_r(&s->p.model, &s->m, s->type[i], s->type[j], del, s->r[i], s->r[j],
+                             &s->v[3 * i], &s->v[3 * j], &s->w[3 * i], &s->w[3 * j], s->rm[i],
+                             s->rm[j], s->p.dt, shearupdate, &s->hist[3 * e], &s->rhist[3 * e],
+                             dFi, dTi, dTj)) {
         /* contact flag set by Hertz / tangential (normal_model_hertz.h:204,
          * tangential_model_history.h:137) */
         s->flag[e] = 1;
@@ -809,6 +890,7 @@ static void wall_compute(orc_sys *s, int shearupdate) {
     const double param = s->p.plane_pos[k];
     for (int i = 0; i < s->n; i++) {
       double *h = &s->whist[3 * ((size_t)i * s->p.nplanes + k)];
+      double *rh = &s->wrhist[3 * ((size_t)i * s->p.nplanes + k)];
       const double *xi = &s->x[3 * i];
       double delta[3] = {0.0, 0.0, 0.0};
       double deltan;
@@ -824,15 +906,16 @@ static void wall_compute(orc_sys *s, int shearupdate) {
         if (!(deltan <= 0)) continue; /* surfacesClose: no force, history kept */
         const double sdelta[3] = {-delta[0], -delta[1], -delta[2]};
         double dF[3], dT[3];
-        orc_contact_wall(&s->p.plane_model[k], &s->m, s->type[i], s->p.plane_type[k], -deltan, sdelta,
-                         s->r[i], &s->v[3 * i], &s->w[3 * i], zero, s->rm[i], s->p.dt,
-                         shearupdate, h, dF, dT);
+        orc_contact_wall_r(&s->p.plane_model[k], &s->m, s->type[i], s->p.plane_type[k], -deltan,
+                           sdelta, s->r[i], &s->v[3 * i], &s->w[3 * i], zero, s->rm[i], s->p.dt,
+                           shearupdate, h, rh, dF, dT);
         for (int d = 0; d < 3; d++) {
           s->f[3 * i + d] += dF[d];
           s->t[3 * i + d] += dT[d];
         }
       } else {
         h[0] = h[1] = h[2] = 0.0;
+        rh[0] = rh[1] = rh[2] = 0.0;
       }
     }
   }
@@ -948,6 +1031,21 @@ int64_t orc_get_pairs(const orc_sys *s, int64_t cap, int *tags, int *flags, doub
 
 void orc_get_wall_hist(const orc_sys *s, double *hist) {
   memcpy(hist, s->whist, sizeof(double) * 3 * (size_t)s->n * (size_t)(s->p.nplanes > 0 ? s->p.nplanes : 1));
+}
+
+int64_t orc_get_pair_rolling(const orc_sys *s, int64_t cap, double *rhist) {
+  const int64_t np = s->first[s->n];
+  if (np > cap) return -np;
+  for (int i = 0; i < s->n; i++)
+    for (int64_t e = s->first[i]; e < s->first[i + 1]; e++) {
+      const double sg = s->tag[i] < s->tag[s->nbr[e]] ? 1.0 : -1.0;
+      for (int d = 0; d < 3; d++) rhist[3 * e + d] = sg * s->rhist[3 * e + d];
+    }
+  return np;
+}
+
+void orc_get_wall_rolling(const orc_sys *s, double *rhist) {
+  memcpy(rhist, s->wrhist, sizeof(double) * 3 * (size_t)s->n * (size_t)(s->p.nplanes > 0 ? s->p.nplanes : 1));
 }
 
 int64_t orc_nrebuilds(const orc_sys *s) { return s->nrebuilds; }
@@ -1292,6 +1390,8 @@ static void mesh_free(orc_mesh *m) {
 int orc_add_mesh(orc_sys *s, int ntri, const double *nodes, int wall_type,
                  const orc_model *model) {
   if (s->nmesh >= ORC_MAXMESH || ntri <= 0) return -1;
+  /* the mesh contact history here holds the shear only: epsd2 is not restated on meshes */
+  if (model->rolling_epsd2) return -1;
   orc_mesh *m = &s->mesh[s->nmesh];
   memset(m, 0, sizeof(*m));
   m->curvature = 1. - EPSILON_CURVATURE;

@@ -64,6 +64,7 @@ typedef struct {
   int kt_to_kn;           /* hooke: ktToKnUser on (normal_model_hooke.h:101) */
   int absolute_damping;   /* hooke/stiffness: absolute_damping on (normal_model_hooke_stiffness.h:81) */
   int tangential_reduce;  /* sjkr: tangential_reduce on (cohesion_model_sjkr.h:72) */
+  int rolling_epsd2;      /* rolling_friction epsd2 (rolling_model_epsd2.h), instead of cdt */
 } orc_model;
 
 typedef struct {
@@ -107,6 +108,13 @@ int orc_contact_pair(const orc_model *mdl, const orc_materials *mat, int itype, 
                      const double *vj, const double *wi, const double *wj, double mi,
                      double mj, double dt, int shearupdate, double *shear, double *dFi,
                      double *dTi, double *dTj);
+/* the same with the rolling-torque history of rolling_friction epsd2 (3
+ * values, read and, if shearupdate, written; may be NULL without epsd2) */
+int orc_contact_pair_r(const orc_model *mdl, const orc_materials *mat, int itype, int jtype,
+                       const double *del, double radi, double radj, const double *vi,
+                       const double *vj, const double *wi, const double *wj, double mi,
+                       double mj, double dt, int shearupdate, double *shear, double *rhist,
+                       double *dFi, double *dTi, double *dTj);
 
 /* --- one sphere-wall contact (fix_wall_gran_base.h:148-361) ------------- */
 /* sidata_deltan = overlap (>0 when touching), delta = vector from centre to
@@ -116,6 +124,11 @@ void orc_contact_wall(const orc_model *mdl, const orc_materials *mat, int itype,
                       double sidata_deltan, const double *sidata_delta, double radi,
                       const double *vi, const double *wi, const double *vwall, double mi,
                       double dt, int shearupdate, double *shear, double *dFi, double *dTi);
+void orc_contact_wall_r(const orc_model *mdl, const orc_materials *mat, int itype, int wtype,
+                        double sidata_deltan, const double *sidata_delta, double radi,
+                        const double *vi, const double *wi, const double *vwall, double mi,
+                        double dt, int shearupdate, double *shear, double *rhist, double *dFi,
+                        double *dTi);
 
 /* --- brute-force O(N^2) half pair set (neigh_gran.cpp:571-582 predicate) - */
 /* writes unordered pairs (i<j by index) into pairs[2*k]; returns the count or
@@ -139,6 +152,9 @@ void orc_get_state(const orc_sys *s, double *x, double *v, double *omega, double
  * contact flag, and the shear history oriented from tag_a towards tag_b */
 int64_t orc_get_pairs(const orc_sys *s, int64_t cap, int *tags, int *flags, double *hist);
 void orc_get_wall_hist(const orc_sys *s, double *hist /* n*nplanes*3 */);
+/* rolling-torque histories (epsd2), in the orders of orc_get_pairs / orc_get_wall_hist */
+int64_t orc_get_pair_rolling(const orc_sys *s, int64_t cap, double *rhist);
+void orc_get_wall_rolling(const orc_sys *s, double *rhist);
 int64_t orc_nrebuilds(const orc_sys *s);
 int64_t orc_ntimestep(const orc_sys *s);
 /* count of touching half-list pairs and of touching particle-wall contacts */

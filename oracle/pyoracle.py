@@ -29,7 +29,8 @@ class OrcModel(ctypes.Structure):
                 ("limit_force", ctypes.c_int), ("torsion_torque", ctypes.c_int),
                 ("normal_model", ctypes.c_int), ("tangential_model", ctypes.c_int),
                 ("cohesion_model", ctypes.c_int), ("kt_to_kn", ctypes.c_int),
-                ("absolute_damping", ctypes.c_int), ("tangential_reduce", ctypes.c_int)]
+                ("absolute_damping", ctypes.c_int), ("tangential_reduce", ctypes.c_int),
+                ("rolling_epsd2", ctypes.c_int)]
 
 
 # model <normal> of pair_style gran / fix wall/gran
@@ -72,6 +73,16 @@ def load():
     lib.orc_contact_pair.restype = ctypes.c_int
     lib.orc_contact_wall.argtypes = [P(OrcModel), P(OrcMaterials), ctypes.c_int, ctypes.c_int,
                                      _D, dp, _D, dp, dp, dp, _D, _D, ctypes.c_int, dp, dp, dp]
+    lib.orc_contact_pair_r.argtypes = [P(OrcModel), P(OrcMaterials), ctypes.c_int, ctypes.c_int,
+                                       dp, _D, _D, dp, dp, dp, dp, _D, _D, _D, ctypes.c_int,
+                                       dp, dp, dp, dp, dp]
+    lib.orc_contact_pair_r.restype = ctypes.c_int
+    lib.orc_contact_wall_r.argtypes = [P(OrcModel), P(OrcMaterials), ctypes.c_int, ctypes.c_int,
+                                       _D, dp, _D, dp, dp, dp, _D, _D, ctypes.c_int, dp, dp, dp,
+                                       dp]
+    lib.orc_get_pair_rolling.argtypes = [ctypes.c_void_p, ctypes.c_int64, dp]
+    lib.orc_get_pair_rolling.restype = ctypes.c_int64
+    lib.orc_get_wall_rolling.argtypes = [ctypes.c_void_p, dp]
     lib.orc_pairs_bruteforce.argtypes = [ctypes.c_int, dp, dp, P(OrcParams), ctypes.c_int64, ip]
     lib.orc_pairs_bruteforce.restype = ctypes.c_int64
     lib.orc_create.argtypes = [P(OrcParams), P(OrcMaterials), ctypes.c_int, dp, dp, dp, dp, dp,
@@ -157,6 +168,7 @@ def model(params):
     m.kt_to_kn = int(params.get("kt_to_kn", 0))
     m.absolute_damping = int(params.get("absolute_damping", 0))
     m.tangential_reduce = int(params.get("tangential_reduce", 0))
+    m.rolling_epsd2 = int(params.get("rolling_epsd2", 0))
     return m
 
 
@@ -215,6 +227,37 @@ def contact_wall(params, mats, itype, wtype, overlap, sdelta, radi, vi, wi, vwal
                          radi, _dp(vi), _dp(wi), _dp(vw), mi, dt, int(shearupdate), _dp(sh),
                          _dp(dF), _dp(dT))
     return dF, dT, sh
+
+
+def contact_pair_rolling(params, mats, itype, jtype, del_, radi, radj, vi, vj, wi, wj, mi, mj,
+                         dt, shearupdate, shear, rhist):
+    """One pair contact with the epsd2 rolling history; returns
+    (touch, dFi, dTi, dTj, shear_after, rhist_after)."""
+    lib = load()
+    mdl = model(params)
+    arr = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    d, vi, vj, wi, wj = arr(del_), arr(vi), arr(vj), arr(wi), arr(wj)
+    sh, rh = arr(shear).copy(), arr(rhist).copy()
+    dF, dTi, dTj = np.zeros(3), np.zeros(3), np.zeros(3)
+    touch = lib.orc_contact_pair_r(ctypes.byref(mdl), ctypes.byref(mats), itype, jtype, _dp(d),
+                                   radi, radj, _dp(vi), _dp(vj), _dp(wi), _dp(wj), mi, mj, dt,
+                                   int(shearupdate), _dp(sh), _dp(rh), _dp(dF), _dp(dTi),
+                                   _dp(dTj))
+    return bool(touch), dF, dTi, dTj, sh, rh
+
+
+def contact_wall_rolling(params, mats, itype, wtype, overlap, sdelta, radi, vi, wi, vwall, mi,
+                         dt, shearupdate, shear, rhist):
+    lib = load()
+    mdl = model(params)
+    arr = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    sd, vi, wi, vw = arr(sdelta), arr(vi), arr(wi), arr(vwall)
+    sh, rh = arr(shear).copy(), arr(rhist).copy()
+    dF, dT = np.zeros(3), np.zeros(3)
+    lib.orc_contact_wall_r(ctypes.byref(mdl), ctypes.byref(mats), itype, wtype, overlap, _dp(sd),
+                           radi, _dp(vi), _dp(wi), _dp(vw), mi, dt, int(shearupdate), _dp(sh),
+                           _dp(rh), _dp(dF), _dp(dT))
+    return dF, dT, sh, rh
 
 
 def tri_contact(nodes, p, radius, edge_active=(1, 1, 1), corner_active=(1, 1, 1)):
@@ -364,6 +407,21 @@ class OracleSystem:
     def wall_history(self):
         out = np.zeros((self.n, max(self.nwalls, 1), 3))
         self.lib.orc_get_wall_hist(self.s, _dp(out))
+        return out[:, :self.nwalls, :]
+
+    def pair_rolling(self):
+        """epsd2 rolling-torque histories, in the order and orientation of pairs()."""
+        cap = 64 * self.n + 16
+        while True:
+            out = np.zeros((cap, 3))
+            n = self.lib.orc_get_pair_rolling(self.s, cap, _dp(out))
+            if n >= 0:
+                return out[:n]
+            cap = int(-n) + 16
+
+    def wall_rolling(self):
+        out = np.zeros((self.n, max(self.nwalls, 1), 3))
+        self.lib.orc_get_wall_rolling(self.s, _dp(out))
         return out[:, :self.nwalls, :]
 
     def nrebuilds(self):

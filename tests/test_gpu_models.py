@@ -1,12 +1,16 @@
 """GPU parity of the sibling contact models (pair_style gran model hooke,
 hertz/stiffness, hooke/stiffness; tangential no_history; cohesion sjkr with
-tangential_reduce; ktToKnUser; absolute_damping) and of walls and meshes whose
-fix wall/gran selects its own model, against the CPU oracle:
+tangential_reduce; ktToKnUser; absolute_damping; rolling_friction epsd2) and of
+walls and meshes whose fix wall/gran selects its own model, against the CPU
+oracle:
 
   pair sets, flags            exact
   forces / torques / history  1e-10 relative in FP64 (setup and one step)
   mixed mode                  forces and torques 1e-5 relative to the FP64 oracle
   300 steps (chaotic)         kinetic energy and contact count within 1 %
+  epsd2 over 10 steps         state and rolling histories 1e-9 relative in FP64
+                              (round-off differences grow over the steps)
+  epsd2 restart               continuation bitwise equal to the uninterrupted run
   missing properties          the run refuses to start, naming what is missing
 """
 import sys
@@ -70,6 +74,12 @@ CASES = {
         {"cohesion": "sjkr", "rolling_cdt": 1},
         lambda m: dict(m, characteristic_velocity=0.5, cohesion_energy_density=[[2e5]]),
         wall_model={"normal": "hooke", "tangential": "no_history", "rolling_cdt": 0}),
+    "hertz_epsd2": lambda: _case({"rolling_epsd2": 1, "rolling_cdt": 0},
+                                 lambda m: dict(m, rolling_friction=[[0.1]])),
+    "hooke_epsd2_torsion_wall_cdt": lambda: _case(
+        {"normal": "hooke", "rolling_epsd2": 1, "rolling_cdt": 0, "torsion_torque": 1},
+        lambda m: dict(m, characteristic_velocity=0.5, rolling_friction=[[0.2]]),
+        wall_model={"rolling_cdt": 1}),
 }
 
 
@@ -268,3 +278,69 @@ run 1500
     assert float(last[2]) == pytest.approx(ke, rel=1e-6)
     assert g.stats()["nwall_contacts"] > 0
     g.close()
+
+
+def compare_rolling(g, o, tol):
+    """pair rolling histories keyed by the pair's tags, and the plane walls'"""
+    gt, _, _ = g.pairs()
+    ot, _, _ = o.pairs()
+    gr, orr = g.pair_rolling(), o.pair_rolling()
+    gm = {(int(a), int(b)): h for (a, b), h in zip(gt, gr)}
+    om = {(int(a), int(b)): h for (a, b), h in zip(ot, orr)}
+    assert set(gm) == set(om)
+    scale = max(np.abs(orr).max(), 1e-300)
+    worst = max(np.abs(gm[k] - h).max() for k, h in om.items()) / scale
+    say(f"  pair rolling history: max |r| {scale:.3e}, worst {worst:.3e}")
+    assert scale > 0 and worst <= tol
+    gw, ow = g.wall_rolling(), o.wall_rolling()
+    wscale = max(np.abs(ow).max(), 1e-300)
+    say(f"  wall rolling history: max |r| {wscale:.3e}")
+    assert np.abs(gw - ow).max() <= tol * wscale
+
+
+@pytest.mark.parametrize("case", ["hertz_epsd2", "hooke_epsd2_torsion_wall_cdt"])
+def test_epsd2_ten_steps_fp64(case):
+    say(f"[gpu] epsd2 rolling, 10 steps fp64, case={case}")
+    params, mats, P, walls = CASES[case]()
+    g, o = build(params, mats, P, walls)
+    g.run(10)
+    o.run(10)
+    gs, os_ = g.state(), o.state()
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert_close(k, gs[k], os_[k], 1e-9)
+    compare_pairs(g, o, tol=1e-9)
+    compare_rolling(g, o, 1e-9)
+    if case == "hooke_epsd2_torsion_wall_cdt":
+        assert not np.any(g.wall_rolling())  # the cdt walls keep no rolling torque
+    else:
+        assert np.abs(o.wall_rolling()).max() > 0
+    g.close()
+    o.close()
+
+
+def test_epsd2_restart_continues_bitwise(tmp_path):
+    say("[gpu] epsd2 restart continuation")
+    params, mats, P, walls = CASES["hertz_epsd2"]()
+    a = pydem.GpuSystem(params, mats, P, walls)
+    a.run(40)
+    a.run(20)
+    b = pydem.GpuSystem(params, mats, P, walls)
+    b.run(40)
+    f = tmp_path / "e.restart"
+    b.write_restart(f)
+    b.close()
+    c = pydem.GpuSystem(params, mats, P, walls)
+    c.read_restart(f)
+    c.run(20)
+    sa, sc = a.state(), c.state()
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert np.array_equal(sa[k], sc[k]), k
+    assert np.array_equal(a.pair_rolling(), c.pair_rolling())
+    assert np.array_equal(a.wall_rolling(), c.wall_rolling())
+    # a system without epsd2 cannot take the file
+    d = pydem.GpuSystem(dict(params, rolling_epsd2=0), mats, P, [w[:3] + ({},) for w in walls])
+    with pytest.raises(pydem.DemError):
+        d.read_restart(f)
+    a.close()
+    c.close()
+    d.close()

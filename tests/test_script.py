@@ -79,6 +79,7 @@ def test_full_command_subset_parses(tmp_path):
     ("dump_modify d9 sort id\n", "Could not find dump_modify ID"),
     ("undump d9\n", "Could not find undump ID"),
     ("fix x all insert/pack seed 1\n", "not available"),
+    ("pair_style gran model hertz tangential history rolling_friction epsd3\n", "'rolling_friction epsd2'"),
 ])
 def test_unsupported_or_illegal_commands_are_errors(tmp_path, bad, msg):
     rc, log = run_script(tmp_path, HEADER + bad)
@@ -161,9 +162,12 @@ def test_sibling_model_commands_parse(tmp_path):
              "fix m11 all property/global gammat peratomtypepair 1 1e4\n")
     for pair, wall in (("hooke tangential no_history cohesion sjkr tangential_reduce on ktToKnUser on",
                         "hertz/stiffness tangential history"),
-                       ("hooke/stiffness tangential history absolute_damping off", "hooke tangential no_history")):
+                       ("hooke/stiffness tangential history absolute_damping off", "hooke tangential no_history"),
+                       ("hertz tangential history rolling_friction epsd2 torsionTorque on",
+                        "hertz tangential history rolling_friction epsd2")):
+        roll = "" if "rolling_friction" in pair else " rolling_friction cdt"
         text = (HEADER.replace("pair_style gran model hertz tangential history rolling_friction cdt",
-                               f"pair_style gran model {pair} rolling_friction cdt")
+                               f"pair_style gran model {pair}{roll}")
                 .replace("wall/gran model hertz tangential history rolling_friction cdt",
                          f"wall/gran model {wall}"))
         rc, log = run_script(tmp_path, text.replace("pair_coeff", props + "pair_coeff", 1) + f"read_data {data}\n")
