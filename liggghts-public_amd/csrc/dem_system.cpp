@@ -1731,6 +1731,8 @@ int dem_get_state(dem_handle *h, double *x, double *v, double *omega, double *f,
 // and mesh histories, displacement reference, mesh geometry) and the counters
 // a run continues from. The layout is this library's own.
 // ---------------------------------------------------------------------------
+}  // extern "C"
+
 namespace {
 struct RestartHeader {
   char magic[8];
@@ -1774,6 +1776,8 @@ std::vector<std::pair<void *, size_t>> restart_blobs(dem_handle *h) {
   return b;
 }
 }  // namespace
+
+extern "C" {
 
 int dem_write_restart(dem_handle *h, const char *path) {
   if (!h || !path) return DEM_ERR_ARG;
@@ -1966,6 +1970,8 @@ int dem_read_restart(dem_handle *h, const char *path) {
   return DEM_OK;
 }
 
+}  // extern "C"
+
 namespace {
 // pairs of the half list with history values comp0..comp0+2 (0 shear, 3 the
 // epsd2 rolling torque)
@@ -2040,6 +2046,8 @@ int get_wall_history(dem_handle *h, double *out, int comp0) {
   return DEM_OK;
 }
 }  // namespace
+
+extern "C" {
 
 int dem_get_pairs(dem_handle *h, int64_t cap_out, int *tags, int *flags, double *hist,
                   int64_t *npairs) {
