@@ -45,6 +45,7 @@
 //   variable <name> equal|index <value> | print | echo | log
 //   run <N> [pre no|yes] [post no|yes]
 //   package amd precision fp64|mixed   (selects the GPU arithmetic mode)
+//   dump <id> all mesh/stl <N> <file*.stl> [binary] [region <id>] [mesh ids]
 //   dump <id> all custom <N> <file> <columns> | dump_modify <id> ... | undump <id>
 //                  (columns id type x y z vx vy vz fx fy fz omegax omegay omegaz
 //                   tqx tqy tqz radius diameter mass; '*' in the file name: one
@@ -198,7 +199,16 @@ struct Interp {
     std::vector<std::string> cols;
     FILE *fp = nullptr;
     long long last = -1;  // timestep of the last snapshot written
+    bool stl = false;     // dump mesh/stl
+    bool binary = false, ghost = false;
+    std::vector<std::string> meshes;  // mesh/stl: the meshes dumped, in order
+    std::string region;               // mesh/stl: only triangles whose centre is inside
   };
+  std::vector<std::string> mesh_def_order;  // fix mesh/surface in definition order
+  // thermo `time` (Update::atime / atimestep, update.cpp:434-446) and `cpu`
+  double atime = 0.0;
+  long long atimestep = 0;
+  std::chrono::steady_clock::time_point run_t0 = std::chrono::steady_clock::now();
   std::map<std::string, Dump> dumps;
   std::string restart_file;  // read_restart: device state applied when the system is built
 
@@ -218,10 +228,115 @@ struct Interp {
     return false;
   }
 
+  // dump id group mesh/stl N file*.stl [binary] [all|local|ghost] [region id] [mesh ...]
+  // (DumpMeshSTL's constructor, dump_mesh_stl.cpp:71-158): no mesh named
+  // means every fix mesh/surface, in the order they were defined
+  void dump_stl_command(const std::vector<std::string> &t) {
+    if (t.size() < 6) die("Illegal dump mesh/stl command");
+    if (dumps.count(t[1])) die("Reuse of dump ID " + t[1]);
+    Dump d;
+    d.stl = true;
+    d.every = std::stoll(t[4]);
+    if (d.every <= 0) die("Illegal dump command");
+    d.file = t[5];
+    if (d.file.find('*') == std::string::npos)
+      die("You should use a filename like 'dump*.stl' for the 'mesh/stl' command to produce one file "
+          "per time-step");
+    for (size_t k = 6; k < t.size(); k++) {
+      if (t[k].compare(0, 6, "binary") == 0) d.binary = true;
+      else if (t[k] == "all" || t[k] == "local") d.ghost = false;
+      else if (t[k] == "ghost") d.ghost = true;
+      else if (t[k] == "region") {
+        if (k + 1 >= t.size()) die("Illegal dump mesh/stl command, not enough arguments");
+        if (!regions.count(t[k + 1])) die("Illegal dump mesh/stl command, region ID does not exist");
+        d.region = t[++k];
+      } else if (mesh_defs.count(t[k])) {
+        d.meshes.push_back(t[k]);
+      } else {
+        die("Illegal dump mesh/stl command, unknown keyword or mesh");
+      }
+    }
+    if (d.meshes.empty()) d.meshes = mesh_def_order;
+    if (d.meshes.empty()) fprintf(out, "WARNING: Dump mesh/stl cannot find any fix of type 'mesh/surface' to dump\n");
+    dumps[t[1]] = d;
+  }
+
+  // one STL snapshot (DumpMeshSTL::pack / write_data_ascii / write_data_binary,
+  // dump_mesh_stl.cpp:306-384): per triangle the unit normal of its current
+  // nodes and the three nodes; one process owns every triangle, so `ghost`
+  // selects none
+  void write_stl(const Dump &d) {
+    std::string name = d.file;
+    name.replace(name.find('*'), 1, std::to_string((long long)step));
+    std::vector<double> rec;  // 12 values per triangle
+    for (const std::string &mn : d.meshes) {
+      auto it = mesh_ids.find(mn);
+      if (it == mesh_ids.end())
+        die("dump mesh/stl: mesh " + mn + " is not used by a fix wall/gran on this path");
+      if (d.ghost) continue;
+      const int T = (int)(mesh_defs[mn].nodes.size() / 9);
+      std::vector<double> nodes(9 * (size_t)T), vn(9 * (size_t)T);
+      if (dem_get_mesh_nodes(h, it->second, nodes.data(), vn.data())) die(dem_last_error(h));
+      for (int tri = 0; tri < T; tri++) {
+        const double *p = &nodes[9 * (size_t)tri];
+        if (!d.region.empty()) {
+          const std::vector<double> &b = regions[d.region];
+          bool in = true;
+          for (int k = 0; k < 3; k++) {
+            const double c = (p[k] + p[3 + k] + p[6 + k]) / 3.0;
+            in = in && c >= b[2 * k] && c <= b[2 * k + 1];
+          }
+          if (!in) continue;
+        }
+        const double e0[3] = {p[3] - p[0], p[4] - p[1], p[5] - p[2]};
+        const double e1[3] = {p[6] - p[3], p[7] - p[4], p[8] - p[5]};
+        double nrm[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2],
+                         e0[0] * e1[1] - e0[1] * e1[0]};
+        const double len = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+        for (int k = 0; k < 3; k++) rec.push_back(nrm[k] / len);
+        for (int k = 0; k < 9; k++) rec.push_back(p[k]);
+      }
+    }
+    const size_t nt = rec.size() / 12;
+    FILE *fp = fopen(name.c_str(), d.binary ? "wb" : "w");
+    if (!fp) die("Cannot open dump file " + name);
+    if (d.binary) {
+      char header[81];
+      std::snprintf(header, sizeof header, "%-80s", "STL File written by LIGGGHTS");
+      fwrite(header, 1, 80, fp);
+      const uint32_t n32 = (uint32_t)nt;
+      fwrite(&n32, 4, 1, fp);
+      for (size_t i = 0; i < nt; i++) {
+        for (int k = 0; k < 12; k++) {
+          const float f = (float)rec[12 * i + k];
+          fwrite(&f, 4, 1, fp);
+        }
+        const uint16_t attr = 0;
+        fwrite(&attr, 2, 1, fp);
+      }
+    } else {
+      fprintf(fp, "solid LIGGGHTS_STL_EXPORT\n");
+      for (size_t i = 0; i < nt; i++) {
+        const double *q = &rec[12 * i];
+        fprintf(fp,
+                "  facet normal %g %g %g\n    outer loop\n      vertex %g %g %g\n"
+                "      vertex %g %g %g\n      vertex %g %g %g\n    endloop\n  endfacet\n",
+                q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9], q[10], q[11]);
+      }
+      fprintf(fp, "endsolid LIGGGHTS_STL_EXPORT\n");
+    }
+    fclose(fp);
+  }
+
   void dump_command(const std::vector<std::string> &t) {
     // dump id group custom N file col1 col2 ...
     if (t.size() < 6) die("Illegal dump command");
-    if (t[3] != "custom") die("dump style " + t[3] + " is not available on this path (custom)");
+    if (t[3] == "mesh/stl") {
+      if (sq) die("dump mesh/stl for superquadrics is not available on this path");
+      dump_stl_command(t);
+      return;
+    }
+    if (t[3] != "custom") die("dump style " + t[3] + " is not available on this path (custom, mesh/stl)");
     if (t[2] != "all") die("dump: only group all is available on this path");
     if (sq) die("dump custom for superquadrics is not available on this path");
     if (dumps.count(t[1])) die("Reuse of dump ID " + t[1]);
@@ -262,6 +377,11 @@ struct Interp {
     for (auto &kv : dumps) {
       Dump &d = kv.second;
       if (d.last == step || !(force_all || step % d.every == 0)) continue;
+      if (d.stl) {
+        write_stl(d);
+        d.last = step;
+        continue;
+      }
       FILE *fp = d.fp;
       if (!fp) {
         std::string name = d.file;
@@ -578,6 +698,7 @@ struct Interp {
         die("fix mesh/surface: unsupported keyword " + t[k]);
       }
     }
+    if (!mesh_defs.count(t[1])) mesh_def_order.push_back(t[1]);
     mesh_defs[t[1]] = md;
     dirty = true;
   }
@@ -920,6 +1041,19 @@ struct Interp {
   // c_<id> of compute erotate/sphere (0.5 I w^2 with I = 0.4 m r^2,
   // compute_erotate_sphere.cpp) or compute ke
   bool thermo_value(const std::string &c, double ke, double &val) {
+    // Thermo::compute_vol / compute_time / compute_cpu (thermo.cpp:931-1003)
+    if (c == "vol") {
+      val = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+      return true;
+    }
+    if (c == "time") {
+      val = atime + (double)(step - atimestep) * dt;
+      return true;
+    }
+    if (c == "cpu") {
+      val = std::chrono::duration<double>(std::chrono::steady_clock::now() - run_t0).count();
+      return true;
+    }
     if (c.size() > 5 && c.compare(0, 2, "f_") == 0 && c.back() == ']') {
       const size_t lb = c.find('[');
       if (lb == std::string::npos) return false;
@@ -986,6 +1120,7 @@ struct Interp {
       pre = true;
     }
     const auto t0 = std::chrono::steady_clock::now();
+    run_t0 = t0;
     long long done = 0;
     const long long chunk = thermo_every > 0 ? thermo_every : n;
     bool first = true;
@@ -1161,6 +1296,8 @@ struct Interp {
       if (t.size() < 3 || t[1] != "*" || t[2] != "*") die("pair_coeff must be '* *' for granular");
     } else if (c == "timestep") {
       if (t.size() < 2) die("Illegal timestep command");
+      if (dt > 0) atime += (double)(step - atimestep) * dt;  // Update::reset_timestep bookkeeping
+      atimestep = step;
       dt = num(t[1]);
       dirty = true;
     } else if (c == "fix") {

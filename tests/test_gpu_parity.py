@@ -572,7 +572,9 @@ def test_script_wiggling_floor_matches_api(tmp_path):
     an STL file, fix move/mesh ... wiggle amplitude ... period ..., two runs so
     the mover's time runs on; thermo with compute erotate/sphere and the
     floor's f_cad[3] / f_cad[5]) gives the kinetic and rotational energy and
-    the mesh force and torque of the same system built through the C ABI."""
+    the mesh force and torque of the same system built through the C ABI;
+    thermo vol / time give the box volume and step * dt, and dump mesh/stl
+    writes the moved floor (ASCII and binary) at steps 0, 150 and 300."""
     say("[gpu] script wiggling floor")
     params, mats, P, walls, meshes = _mesh_floor_pack(seed=45, n=1200, moving="wiggle")
     data = tmp_path / "shake.data"
@@ -607,8 +609,10 @@ fix shake all move/mesh mesh cad wiggle amplitude {amp[0]!r} {amp[1]!r} {amp[2]!
 read_data {data}
 fix integr all nve/sphere
 compute er all erotate/sphere
-thermo_style custom step atoms ke c_er f_cad[3] f_cad[5]
+thermo_style custom step atoms ke c_er f_cad[3] f_cad[5] vol time
 thermo 150
+dump stl all mesh/stl 150 {tmp_path}/floor*.stl
+dump stlb all mesh/stl 300 {tmp_path}/floorb*.stl binary cad
 run 150
 run 150
 """)
@@ -635,6 +639,27 @@ run 150
     # the mesh totals are atomically summed, so equal to rounding only
     assert float(last[4]) == pytest.approx(F[2], rel=1e-6) and F[2] < 0
     assert float(last[5]) == pytest.approx(T[1], rel=1e-6)
+    vol = (hi[0] - lo[0] + 0.006) * (hi[1] - lo[1] + 0.006) * (hi[2] - lo[2] + 0.001)
+    assert float(last[6]) == pytest.approx(vol, rel=1e-7)
+    assert float(last[7]) == pytest.approx(300 * params["dt"], rel=1e-7)
+    nodes, _ = g.mesh_nodes()
+    for stepno in (0, 150, 300):
+        assert (tmp_path / f"floor{stepno}.stl").exists()
+    lines = (tmp_path / "floor300.stl").read_text().splitlines()
+    assert lines[0] == "solid LIGGGHTS_STL_EXPORT" and lines[-1] == "endsolid LIGGGHTS_STL_EXPORT"
+    verts = np.array([[float(w) for w in l.split()[1:]] for l in lines if l.strip().startswith("vertex")])
+    normals = np.array([[float(w) for w in l.split()[2:]] for l in lines if l.strip().startswith("facet")])
+    assert verts.shape == (3 * len(nodes), 3) and normals.shape == (len(nodes), 3)
+    np.testing.assert_allclose(verts, nodes.reshape(-1, 3), rtol=1e-5, atol=1e-9)  # %g: 6 digits
+    cr = np.cross(nodes[:, 1] - nodes[:, 0], nodes[:, 2] - nodes[:, 1])
+    np.testing.assert_allclose(normals, cr / np.linalg.norm(cr, axis=1)[:, None], rtol=1e-5, atol=1e-6)
+    raw = (tmp_path / "floorb300.stl").read_bytes()
+    assert raw[:28] == b"STL File written by LIGGGHTS" and raw[28:80] == b" " * 52
+    ntri = int(np.frombuffer(raw[80:84], dtype=np.uint32)[0])
+    assert ntri == len(nodes) and len(raw) == 84 + 50 * ntri
+    rec = np.frombuffer(raw[84:], dtype=np.dtype([("v", "<f4", 12), ("a", "<u2")]))
+    np.testing.assert_array_equal(rec["v"][:, 3:], nodes.reshape(-1, 9).astype(np.float32))
+    assert not (tmp_path / "floorb150.stl").exists()
     g.close()
 
 

@@ -80,6 +80,9 @@ def test_full_command_subset_parses(tmp_path):
     ("undump d9\n", "Could not find undump ID"),
     ("fix x all insert/pack seed 1\n", "not available"),
     ("pair_style gran model hertz tangential history rolling_friction epsd3\n", "'rolling_friction epsd2'"),
+    ("dump s all mesh/stl 100 post/mesh.stl\n", "one file per time-step"),
+    ("dump s all mesh/stl 100 post/mesh*.stl nosuchmesh\n", "unknown keyword or mesh"),
+    ("dump s all mesh/stl 100 post/mesh*.stl region nowhere\n", "region ID does not exist"),
 ])
 def test_unsupported_or_illegal_commands_are_errors(tmp_path, bad, msg):
     rc, log = run_script(tmp_path, HEADER + bad)
