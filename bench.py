@@ -56,7 +56,8 @@ HBM_PEAK_GBS = 8000.0
 TRI_RECORD_BYTES = 312  # one per-triangle geometry record (TriGeom)
 
 
-def algorithmic_bytes(n, npairs, ncontacts, nwall, mixed, ntri=0, ncand=0, history=True):
+def algorithmic_bytes(n, npairs, ncontacts, nwall, mixed, ntri=0, ncand=0, history=True,
+                      rolling_history=False):
     """Unique HBM bytes one launch of the fused step kernel must move:
     per particle  read x,y,z,r + v,m (64 B FP64) + omega,type (32 B FP64 /
                   16 B mixed) + xhold (24 B) + list length (4 B);
@@ -69,8 +70,11 @@ def algorithmic_bytes(n, npairs, ncontacts, nwall, mixed, ntri=0, ncand=0, histo
                   per touching mesh contact its 4-byte slot id (counted in nwall
                   for the history).
     history=False: a contact model without tangential history (no_history)
-    moves no history bytes."""
+    moves no history bytes; rolling_history=True (rolling_friction epsd2) moves
+    the rolling torque beside the shear, twice the history bytes."""
     sh = (12 if mixed else 24) if history else 0
+    if rolling_history:
+        sh *= 2
     per_particle = 64 + (16 if mixed else 32) + 24 + 4 + 48 + (12 if mixed else 24)
     B = n * per_particle + 4 * npairs + 2 * sh * (ncontacts + nwall)
     if ntri:
@@ -122,7 +126,8 @@ def run_mode(params, mats, P, walls, precision, steps, warmup, timing_steps, mes
     mixed = precision == pydem.DEM_MIXED
     ntri = sum(len(m[0]) for m in meshes)
     B = algorithmic_bytes(n, st["npairs"], st["ncontacts"], st["nwall_contacts"], mixed,
-                          ntri, st["nmesh_candidates"], history=params.get("tangential") != "no_history")
+                          ntri, st["nmesh_candidates"], history=params.get("tangential") != "no_history",
+                          rolling_history=bool(params.get("rolling_epsd2")))
     kern_s = st["step_kernel_ms"] * 1e-3
     return {
         "throughput": n * steps / dt,
@@ -409,6 +414,8 @@ CONTACT_MODELS = {
     "hertz-stiffness": ({"normal": "hertz/stiffness"},
                         {"kn": [[4.0e6]], "kt": [[5.0e6]], "gamman": [[2.0e5]], "gammat": [[1.0e5]]},
                         "hertz/stiffness/history (kn 4e6, kt 5e6, gamman 2e5, gammat 1e5)"),
+    "hertz-epsd2": ({"rolling_cdt": 0, "rolling_epsd2": 1}, {},
+                    "Hertz/history + rolling_friction epsd2 (rolling-torque history)"),
 }
 
 
