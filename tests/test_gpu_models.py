@@ -161,7 +161,7 @@ def _ke(st, m):
     return 0.5 * (m[:, None] * st["v"] ** 2).sum()
 
 
-@pytest.mark.parametrize("case", ["hooke_nohistory_sjkr_reduce", "hertz_stiffness_cdt"])
+@pytest.mark.parametrize("case", ["hooke_nohistory_sjkr_reduce", "hertz_stiffness_cdt", "hertz_epsd2"])
 def test_three_hundred_steps_statistics(case):
     say(f"[gpu] contact models, 300 steps, case={case}")
     params, mats, P, walls = CASES[case]()
