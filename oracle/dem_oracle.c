@@ -855,8 +855,48 @@ static int decide(const orc_sys *s) {
   return check_distance(s);
 }
 
+/* Bits of a list entry's contact flag (contact_model_constants.h:64-68). A
+ * build stores 1 (the normal bit) for a carried pair (neigh_gran.cpp:592). */
+#define FLAG_NORMAL 1
+#define FLAG_COHESION 2
+#define FLAG_TANGENTIAL 4
+#define FLAG_ROLLING 8
+
+/* the normal models whose surfacesIntersect / surfacesClose set the normal
+ * bit (normal_model_hertz.h:204,409; normal_model_hooke.h:229,440;
+ * normal_model_hooke_stiffness.h:196,369); hertz/stiffness sets none */
+static int sets_normal_bit(const orc_model *m) {
+  return m->normal_model != ORC_NORMAL_HERTZ_STIFFNESS;
+}
+
+/* ContactModel::surfacesIntersect's flag bits (tangential_model_history.h:137,
+ * tangential_model_no_history.h:113, cohesion_model_sjkr.h:102,
+ * rolling_model_epsd2.h:103) */
+static int touch_flag_bits(const orc_model *m) {
+  int b = FLAG_TANGENTIAL;
+  if (sets_normal_bit(m)) b |= FLAG_NORMAL;
+  if (m->cohesion_model) b |= FLAG_COHESION;
+  if (m->rolling_epsd2) b |= FLAG_ROLLING;
+  return b;
+}
+
+/* ContactModel::surfacesClose (contact_models.h:246-253) for a pair within
+ * contact_distance_factor * radsum that does not touch: the normal models
+ * above set their bit, sjkr, both tangential models and epsd2 clear theirs
+ * (cohesion_model_sjkr.h:129-133, tangential_model_no_history.h:244-247,
+ * tangential_model_history.h:358-369, rolling_model_epsd2.h:144-151); the
+ * shear history and the epsd2 rolling torque are zeroed whatever shearupdate
+ * says */
+static int close_flag(const orc_model *m, int flag) {
+  if (sets_normal_bit(m)) flag |= FLAG_NORMAL;
+  flag &= ~(FLAG_COHESION | FLAG_TANGENTIAL | FLAG_ROLLING);
+  return flag;
+}
+
 /* pair loop, pair_gran_base.h:255-430 (newton off: j always local here) */
 static void pair_compute(orc_sys *s, int shearupdate) {
+  /* contactDistanceMultiplier, pair_gran_base.h:224 */
+  const double cdm = s->p.cdf * s->p.cdf;
   for (int i = 0; i < s->n; i++)
     for (int64_t e = s->first[i]; e < s->first[i + 1]; e++) {
       const int j = s->nbr[e];
@@ -866,18 +906,26 @@ static void pair_compute(orc_sys *s, int shearupdate) {
                              &s->v[3 * i], &s->v[3 * j], &s->w[3 * i], &s->w[3 * j], s->rm[i],
                              s->rm[j], s->p.dt, shearupdate, &s->hist[3 * e], &s->rhist[3 * e],
                              dFi, dTi, dTj)) {
-        /* contact flag set by Hertz / tangential (normal_model_hertz.h:204,
-         * tangential_model_history.h:137) */
-        s->flag[e] = 1;
+        s->flag[e] |= touch_flag_bits(&s->p.model);
         for (int d = 0; d < 3; d++) {
           s->f[3 * i + d] += dFi[d];
           s->t[3 * i + d] += dTi[d];
           s->f[3 * j + d] += -dFi[d];
           s->t[3 * j + d] += dTj[d];
         }
+        continue;
       }
-      /* with contact_distance_factor 1 the surfacesClose branch
-       * (pair_gran_base.h:414-417) is unreachable: the pair keeps its history */
+      /* surfacesClose branch, pair_gran_base.h:414-417 (reachable only with
+       * contact_distance_factor > 1) */
+      const double rsq = del[0] * del[0] + del[1] * del[1] + del[2] * del[2];
+      const double radsum = s->r[i] + s->r[j];
+      if (rsq < cdm * radsum * radsum) {
+        s->flag[e] = close_flag(&s->p.model, s->flag[e]);
+        if (s->p.model.tangential_model == 0)
+          s->hist[3 * e] = s->hist[3 * e + 1] = s->hist[3 * e + 2] = 0.0;
+        if (s->p.model.rolling_epsd2)
+          s->rhist[3 * e] = s->rhist[3 * e + 1] = s->rhist[3 * e + 2] = 0.0;
+      }
     }
 }
 
@@ -903,7 +951,14 @@ static void wall_compute(orc_sys *s, int shearupdate) {
       }
       if (deltan > s->cutneighmax) continue;
       if (deltan <= 0 || deltan < (s->p.cdf - 1.0) * s->r[i]) {
-        if (!(deltan <= 0)) continue; /* surfacesClose: no force, history kept */
+        if (!(deltan <= 0)) {
+          /* compute_force with intersectflag false: surfacesClose zeroes the
+           * shear (tangential_model_history.h:358-369) and the epsd2 rolling
+           * torque (rolling_model_epsd2.h:144-151); no force */
+          if (s->p.plane_model[k].tangential_model == 0) h[0] = h[1] = h[2] = 0.0;
+          if (s->p.plane_model[k].rolling_epsd2) rh[0] = rh[1] = rh[2] = 0.0;
+          continue;
+        }
         const double sdelta[3] = {-delta[0], -delta[1], -delta[2]};
         double dF[3], dT[3];
         orc_contact_wall_r(&s->p.plane_model[k], &s->m, s->type[i], s->p.plane_type[k], -deltan,
@@ -1023,7 +1078,7 @@ int64_t orc_get_pairs(const orc_sys *s, int64_t cap, int *tags, int *flags, doub
       const double sg = ti < tj ? 1.0 : -1.0;
       tags[2 * e] = ti < tj ? ti : tj;
       tags[2 * e + 1] = ti < tj ? tj : ti;
-      flags[e] = s->flag[e];
+      flags[e] = s->flag[e] != 0; /* the flag's bits collapse to set / unset */
       for (int d = 0; d < 3; d++) hist[3 * e + d] = sg * s->hist[3 * e + d];
     }
   return np;
@@ -2001,7 +2056,13 @@ static void mesh_wall_compute(orc_sys *s, int im, int shearupdate) {
           for (int k = 0; k < 3; k++)
             vwall[k] = (bary[0] * m->vnode[9 * t + k] + bary[1] * m->vnode[9 * t + 3 + k] +
                         bary[2] * m->vnode[9 * t + 6 + k]);
-        if (!(deltan <= 0)) continue; /* surfacesClose: no force */
+        if (!(deltan <= 0)) {
+          /* compute_force with intersectflag false (fix_wall_gran.cpp:987-1006,
+           * fix_wall_gran_base.h:283-289): surfacesClose zeroes the slot's shear
+           * (tangential_model_history.h:358-369); no force */
+          if (m->model.tangential_model == 0) h[0] = h[1] = h[2] = 0.0;
+          continue;
+        }
         const double sdelta[3] = {-delta[0], -delta[1], -delta[2]};
         double dF[3], dT[3];
         orc_contact_wall(&m->model, &s->m, s->type[i], m->wall_type, -deltan, sdelta, s->r[i],

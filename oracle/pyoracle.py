@@ -394,7 +394,8 @@ class OracleSystem:
         return out
 
     def pairs(self):
-        cap = 64 * self.n + 16
+        # ask for the count first (a negative return is minus the pairs held)
+        cap = max(-int(self.lib.orc_get_pairs(self.s, 0, None, None, None)), 1)
         while True:
             tags = np.zeros((cap, 2), dtype=np.int32)
             flags = np.zeros(cap, dtype=np.int32)
