@@ -26,7 +26,7 @@ it reports contacts/s, Newton iterations per contact and the agreement of
 the converged points.
 
 `--workload packed` runs the same 1M spheres from a compressed lattice
-(about six touching neighbors per sphere, tangential history on each), the
+(about three touching neighbors per sphere, tangential history on each), the
 history-heavy regime of a settled bed.
 
 `--workload drum` runs the rotating-drum case instead (closed cylinder
@@ -96,13 +96,33 @@ def measured_traffic(workload, mode):
     return None
 
 
-def run_mode(params, mats, P, walls, precision, steps, warmup, timing_steps, meshes=(),
-             instances=None, instance_materials=None, tune=False, graphs=False, keep=False):
+TUNE_STEPS = 600  # covers every candidate tile shape (8 steps each, up to 3 tries)
+
+
+def tune_shape(params, mats, P, walls, precision, meshes=(), instances=None, instance_materials=None):
+    """Autotunes the step kernel's tile shape on a separate system of the same
+    workload, so that the timed system starts with its shape fixed and no
+    timed step runs a candidate shape. Returns (variant, persist, candidates)."""
     import pydem
     g = pydem.GpuSystem(params, mats, P, walls, precision=precision, meshes=meshes,
                         instances=instances, instance_materials=instance_materials)
+    g.set_step_shape(-1)
+    g.run(TUNE_STEPS)
+    v, f, cand = g.step_shape()
+    g.close()
+    return v, f, cand
+
+
+def run_mode(params, mats, P, walls, precision, steps, warmup, timing_steps, meshes=(),
+             instances=None, instance_materials=None, tune=False, graphs=False, keep=False):
+    import pydem
+    tuned = None
     if tune:
-        g.set_step_shape(-1)
+        tuned = tune_shape(params, mats, P, walls, precision, meshes, instances, instance_materials)
+    g = pydem.GpuSystem(params, mats, P, walls, precision=precision, meshes=meshes,
+                        instances=instances, instance_materials=instance_materials)
+    if tuned is not None:
+        g.set_step_shape(tuned[0], tuned[1])
     if graphs:
         g.set_graph_capture(True)
     g.run(warmup)
@@ -141,6 +161,8 @@ def run_mode(params, mats, P, walls, precision, steps, warmup, timing_steps, mes
         "nrebuilds_timed": st0["nrebuilds"],
         "rebuild_ms_total_timing_pass": st["rebuild_ms_total"],
         "step_shape": {"variant": shape[0], "persist": shape[1]},
+        "tuned_candidates_us": ([[c[0], c[1], round(c[2], 2)] for c in tuned[2]]
+                                if tuned is not None else None),
         **extra,
     }
 
@@ -167,6 +189,33 @@ def cpu_baseline(params, mats, P, walls, nsteps, meshes=()):
     return {"value": n * nsteps / dt, "unit": "particle-timesteps/s", "cores": 1, "kind": "port",
             "sample": f"oracle/dem_oracle.c (FP64, 1 thread) on the same {n}-sphere workload, "
                       f"{nsteps} steps after setup ({dt:.1f} s)"}
+
+
+def mixed_error_vs_oracle(params, mats, P, walls):
+    """Setup + one step of the mixed GPU mode against the FP64 oracle on the
+    same input: per-particle force and torque error (max_i |a_i - b_i| over
+    max_i |b_i|) and norm-wise error."""
+    import numpy as np
+    import pydem
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    o = pyoracle.OracleSystem(params, mats, P, walls)
+    o.run(1)
+    os_ = o.state()
+    ncont = o.contact_counts()
+    o.close()
+    g = pydem.GpuSystem(params, mats, P, walls, precision=pydem.DEM_MIXED)
+    g.run(1)
+    gs = g.state()
+    g.close()
+    out = {"touching_pairs": int(ncont[0]), "wall_contacts": int(ncont[1])}
+    for k in ("f", "torque"):
+        a, b = gs[k], os_[k]
+        scale = float(np.linalg.norm(b, axis=1).max())
+        den = float(np.linalg.norm(b))
+        out[k] = {"per_particle": float(np.linalg.norm(a - b, axis=1).max() / scale) if scale > 0 else None,
+                  "normwise": float(np.linalg.norm(a - b) / den) if den > 0 else None}
+    return out
 
 
 def sweep_vs_alone(params, P, walls, meshes, counts, imats, g, nsteps, sample):
@@ -429,11 +478,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200,
-                    help="untimed steps; the first 120 autotune the step kernel's tile shape")
+                    help="untimed steps before the timed ones (the tile shape is tuned beforehand "
+                         "on a separate system of the same workload)")
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--timing-steps", type=int, default=500)
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="settling: skip the packed-bed figure and the error against the oracle")
+    ap.add_argument("--packed-steps", type=int, default=500)
     ap.add_argument("--modes", default=None)
     ap.add_argument("--workload", choices=("settling", "packed", "drum", "sweep", "superquadric", "sqsettle",
                                            "copy"),
@@ -488,7 +541,7 @@ def main():
         params, mats, law = with_contact_model(args.contact_model, params, mats)
         n = P["x"].shape[0]
         workload = (f"{n}-sphere polydisperse {law} + CDT rolling, compressed lattice "
-                    "(about six touching neighbors per sphere), periodic xy, zplane floor, "
+                    "(about three touching neighbors per sphere), periodic xy, zplane floor, "
                     "dt 5e-6 s, skin 1 mm")
         data = f"synthetic (seeded compressed lattice, {n} polydisperse spheres)"
         modes = args.modes or "fp64,mixed"
@@ -572,9 +625,42 @@ def main():
         "frac": (head["achieved_gbs"] / HBM_PEAK_GBS) if head["achieved_gbs"] else None,
         "traffic": (measured_traffic("1M_settling", head_mode)
                     if args.workload == "settling" and args.contact_model == "hertz" else None),
+        "traffic_source": ("profiles/k_step_1M_settling_pmc.md: FETCH_SIZE + WRITE_SIZE per launch of "
+                           "this kernel, from rocprofv3 --pmc runs of this workload, scaled by a plain "
+                           "copy's counters (not measured in this run)"
+                           if args.workload == "settling" and args.contact_model == "hertz" else None),
         "bytes_per_launch": head["bytes_per_launch"],
         "kernel_ms": head["kernel_ms"],
     }
+    if args.workload == "settling" and args.contact_model == "hertz" and not args.no_extras:
+        # the same 1M spheres from a compressed lattice (about three touching
+        # neighbors each): the contact-bearing regime, timed over fewer steps
+        pp, pm, pP, pw = wl.packed_1m(seed=0, n=n)
+        packed = {}
+        for mode in modes.split(","):
+            prec = pydem.DEM_MIXED if mode == "mixed" else pydem.DEM_FP64
+            r = run_mode(pp, pm, pP, pw, prec, args.packed_steps, 100, 200, (), None, None, True, True)
+            packed[mode] = {"value": r["throughput"], "ms_per_step": r["ms_per_step"],
+                            "kernel_ms": r["kernel_ms"], "achieved_gbs": r["achieved_gbs"],
+                            "frac": r["achieved_gbs"] / HBM_PEAK_GBS if r["achieved_gbs"] else None,
+                            "npairs": r["npairs"], "ncontacts": r["ncontacts"], "nwall": r["nwall"],
+                            "rebuilds_in_timed_steps": r["nrebuilds_timed"], "step_shape": r["step_shape"]}
+            print(f"# packed {mode}: {packed[mode]}", file=sys.stderr, flush=True)
+        if "mixed" in packed and "fp64" in packed:
+            packed["mixed_over_fp64"] = packed["mixed"]["value"] / packed["fp64"]["value"]
+        packed["workload"] = (f"{n}-sphere packed bed (compressed polydisperse lattice), Hertz/history + CDT, "
+                              f"{args.packed_steps} steps after 100")
+        out["packed_bed"] = packed
+        # each result's error against the FP64 oracle, at 1M on both states
+        err = {"settling_1m": mixed_error_vs_oracle(params, mats, P, walls),
+               "packed_1m": mixed_error_vs_oracle(pp, pm, pP, pw),
+               "definition": "setup + one step, mixed GPU mode vs the FP64 CPU oracle on the same input; "
+                             "per_particle = max_i |a_i - b_i| / max_i |b_i|"}
+        out["mixed_error_vs_fp64_oracle"] = err
+        # the contact-bearing state carries the meaningful figure (the dilute
+        # lattice has no contacts: its forces are gravity alone)
+        out["mixed_force_rel_err"] = err["packed_1m"]["f"]["per_particle"]
+        out["mixed_torque_rel_err"] = err["packed_1m"]["torque"]["per_particle"]
     if not args.no_cpu_baseline:
         if inst is not None:
             # the oracle on the median-size instance alone (instances are independent,

@@ -70,6 +70,16 @@ typedef struct dem_params {
   int neigh_every;    /* neigh_modify every N (neighbor.cpp:1907-1911); 0 selects 1 */
   int neigh_no_check; /* neigh_modify check no (neighbor.cpp:1922-1927): build every
                          `neigh_every` steps without the skin/2 displacement test */
+  /* boundary s / m faces of the non-periodic dimensions: bit 2*d the lower,
+   * bit 2*d+1 the upper face of dimension d (shrink_min: those that are m).
+   * A shrink-wrapped face follows the atoms at every neighbor build
+   * (Domain::reset_box, domain.cpp:381-460) and loses none. A non-periodic
+   * face without its bit is f: an atom beyond it at a neighbor build is lost
+   * (Comm::exchange, comm.cpp:945-1003), and the step returns DEM_ERR_STATE
+   * "Lost atoms: original N current M" (Thermo::lost_check, thermo.cpp:
+   * 410-452, with thermo_modify lost error, the default). */
+  int shrink_faces;
+  int shrink_min;
 } dem_params;
 
 enum dem_normal_model {
@@ -185,6 +195,19 @@ int dem_add_plane_wall(dem_handle *h, int axis, double position, int wall_type,
 int dem_add_mesh(dem_handle *h, int64_t ntri, const double *nodes, int wall_type,
                  const dem_contact_model *model, int *mesh_id);
 
+/* The triangles of fix mesh/surface ... generate <type> (InputMeshTri::
+ * meshgenerator, input_mesh_tri.cpp:608-1107), centred on the origin, in the
+ * generator's order: type "cube" (dims[0] = size), "box" (x, y, z sizes),
+ * "cylinder" (radius, height), "pipe" (inner radius, outer radius, height),
+ * "disk" (radius), "plane" (size); nsegments for cylinder / pipe / disk; mask
+ * the faces (cube / box: top 1, bottom 2, front 4, back 8, left 16, right 32;
+ * cylinder: top 1, bottom 2, side 4; pipe: inner top 1, outer top 2, inner
+ * bottom 4, outer bottom 8, inner side 16, outer side 32). Writes up to cap
+ * triangles (9 doubles each) to nodes (may be NULL) and their number to *ntri.
+ * Host only; errors through dem_last_error(NULL). */
+int dem_mesh_generate(const char *type, const double *dims, int nsegments, int mask, int64_t cap, double *nodes,
+                      int64_t *ntri);
+
 /* fix move/mesh mesh <id> rotate origin ox oy oz axis ax ay az period T
  * (mesh_mover_rotation.cpp:66-139): the mesh turns by 2*pi*dt/T about the axis
  * each step and its nodes carry the rigid-body velocity. */
@@ -293,6 +316,12 @@ int dem_compute_forces(dem_handle *h, int shearupdate);
 
 /* Copies the state back, in caller order; any pointer may be NULL. f and torque
  * are those of the last force pass (setup, step or dem_compute_forces). */
+/* The simulation box at the last neighbor build (boxlo, boxhi, 3 each): the
+ * created box in periodic and f dimensions; a shrink-wrapped face at the
+ * extent of the atoms there plus 1e-4 of the created box's length, an m face
+ * never inside its created position (Domain::reset_box, domain.cpp:432-449). */
+int dem_get_box(dem_handle *h, double *boxlo, double *boxhi);
+
 int dem_get_state(dem_handle *h, double *x, double *v, double *omega, double *f,
                   double *torque);
 

@@ -11,8 +11,14 @@ namespace dem {
 constexpr int kMaxWalls = 6;
 constexpr int kMaxTypes = 8;
 constexpr int kBlock = 256;
-constexpr uint32_t kFlagBit = 0x80000000u;  // contact flag stored in the neighbor entry
-constexpr uint32_t kIndexMask = 0x7fffffffu;
+// A neighbor entry is the neighbor's index plus two bits: kFlagBit is the
+// reference's contact flag being nonzero (the pair's history is valid and is
+// carried over at the next build), kCarryBit that the build carried the pair
+// over (the build stores the flag as 1, the normal-model bit, which a
+// hertz/stiffness pair keeps in the surfacesClose band and others set anyway).
+constexpr uint32_t kFlagBit = 0x80000000u;
+constexpr uint32_t kCarryBit = 0x40000000u;
+constexpr uint32_t kIndexMask = 0x3fffffffu;
 constexpr int kMetaCountMask = 0xffff;       // meta[i]: low 16 bits = list length
 constexpr int kMetaWallShift = 16;           // bits 16..21: wall history present
 constexpr int kMaxMeshes = 4;
@@ -24,10 +30,13 @@ constexpr int kTypeBits = 8;
 constexpr int kTypeMask = (1 << kTypeBits) - 1;
 constexpr int kMaxInstances = 1 << 16;
 // Tile shapes the step kernel is compiled for (threads per block, neighbor
-// entries whose loads are issued together); variant 0 is the default.
-constexpr int kNumStepVariants = 5;
-constexpr int kStepVariantBlock[kNumStepVariants] = {256, 256, 256, 128, 64};
-constexpr int kStepVariantGroup[kNumStepVariants] = {4, 2, 8, 4, 4};
+// entries whose loads are issued together, and whether the touching entries
+// are compacted over the wavefront before the contact law); variant 0 is the
+// default. The sibling contact models run in variants 0 and 5.
+constexpr int kNumStepVariants = 7;
+constexpr int kStepVariantBlock[kNumStepVariants] = {256, 256, 256, 128, 64, 256, 256};
+constexpr int kStepVariantGroup[kNumStepVariants] = {4, 2, 8, 4, 4, 4, 8};
+constexpr int kStepVariantWave[kNumStepVariants] = {0, 0, 0, 0, 0, 1, 1};
 
 // One triangle of a fix mesh/surface, FP64 (TriMesh element properties).
 struct TriGeom {
@@ -51,8 +60,13 @@ struct DevParams {
   int neigh_every;         // neigh_modify every (>= 1)
   int neigh_check;         // neigh_modify check yes (1) / no (0)
   double skin, cdf;
+  double cdm;              // contactDistanceMultiplier = cdf * cdf (pair_gran_base.h:224)
+  int close_band;          // cdf > 1: pairs within cdf * radsum that do not touch
+                           // take the surfacesClose branch
   double lo[3], L[3], halfL[3];
   int per[3];
+  int lost_faces;          // f faces of non-periodic dimensions (bit 2d lower, 2d+1 upper):
+                           // an atom beyond one at a build is lost
   int pair_model;          // contact-model bits of pair_style gran (kModel*)
   int generic;             // some pair, wall or mesh model is not Hertz / history
   int hist_w;              // history values per pair / wall contact: 3 shear, 6 with
@@ -121,6 +135,7 @@ struct Ctrl {
   // mesh/surface/stress: total force and torque (about the mesh's reference
   // point) the particles exert on each mesh, at the last counted pass
   double mesh_force[kMaxMeshes][6];
+  unsigned long long nlost;  // atoms beyond an f face at the last build
 };
 
 // Particle state read by neighbors: ping-pong buffered across steps.

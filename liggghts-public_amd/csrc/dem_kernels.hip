@@ -384,7 +384,7 @@ __device__ __forceinline__ bool tri_coplanar(const MeshStep &M, int a, int b) {
 }
 
 template <typename T, bool GEN, typename W>
-__device__ inline void mesh_contacts(const StepArgs &a, int i, const double4 &pi,
+__device__ __forceinline__ void mesh_contacts(const StepArgs &a, int i, const double4 &pi,
                                      const double4 &vi, const W &wi, int irow, T &fx, T &fy,
                                      T &fz, T &tx, T &ty, T &tz,
                                      unsigned long long &nwall) {
@@ -460,6 +460,11 @@ __device__ inline void mesh_contacts(const StepArgs &a, int i, const double4 &pi
         sh[1] = hist[(size_t)(s * 3 + 1) * cap];
         sh[2] = hist[(size_t)(s * 3 + 2) * cap];
       }
+      // within (cdf - 1) r of the triangle but not touching: compute_force with
+      // intersectflag false, whose surfacesClose zeroes the slot's shear
+      // (fix_wall_gran_base.h:283-289, tangential_model_history.h:358-369)
+      const bool band = !(deltan <= 0);
+      if (band && !mdl.nohist) sh[0] = sh[1] = sh[2] = T(0);
       double vw[3] = {0., 0., 0.};
       if (p.mesh_moving[m])
         for (int k = 0; k < 3; k++)
@@ -534,7 +539,7 @@ __device__ inline void mesh_contacts(const StepArgs &a, int i, const double4 &pi
         tz += dTi[2];
         nwall++;
       }
-      if (a.shearupdate || fresh) {
+      if (a.shearupdate || fresh || band) {
         hist[(size_t)(s * 3) * cap] = sh[0];
         hist[(size_t)(s * 3 + 1) * cap] = sh[1];
         hist[(size_t)(s * 3 + 2) * cap] = sh[2];
@@ -674,156 +679,331 @@ __device__ __forceinline__ void mesh_move_block(const MeshMoveArgs &a, const int
     atomicMin((unsigned long long *)&a.ctrl->stop_at, (unsigned long long)(step + 1));
 }
 
-// One particle's step: contacts over its list (broad phase in groups of
-// GROUP entries), walls, meshes, gravity, integration and the displacement
-// test. Accumulates its touching pair and wall contacts.
-template <typename T, bool MESH, int GROUP, bool GEN>
-__device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
+// The surfacesClose branch of the pair loop (pair_gran_base.h:414-417) for the
+// entries of bit mask `close` (group k0, entries ent): within cdf * radsum but
+// not touching. ContactModel::surfacesClose zeroes the shear history
+// (tangential_model_history.h:358-369) and the epsd2 rolling torque
+// (rolling_model_epsd2.h:144-151) and leaves the contact flag nonzero only if
+// its normal-model bit is set: hertz, hooke and hooke/stiffness set it here
+// (normal_model_hertz.h:407-411 and likewise), hertz/stiffness keeps it only
+// from a carry-over at the last build.
+template <typename T, bool GEN, int GROUP>
+__device__ __forceinline__ void close_entries(const StepArgs &a, const Model &pm, const int i,
+                                              const int k0, uint32_t close,
+                                              const uint32_t (&ent)[GROUP], T *hist,
+                                              const size_t cap, const size_t hw) {
+  const bool keeps = !(GEN && pm.nohist);
+  const bool normal_bit = pm.normal != kNormalHertzStiffness;
+  while (close) {
+    const int u = __builtin_ctz(close);
+    close &= close - 1u;
+    const int k = k0 + u;
+    const uint32_t e = ent[u];
+    if (keeps) {
+      const size_t hb = (size_t)k * hw * cap + i;
+      for (size_t d = 0; d < hw; d++) hist[hb + d * cap] = T(0);
+    }
+    const uint32_t ne = (normal_bit || (e & kCarryBit)) ? (e | kFlagBit) : (e & ~kFlagBit);
+    if (ne != e) a.list.nbr[(size_t)k * cap + i] = ne;
+  }
+}
+
+// One touching list entry (k, e) of particle i: pair_gran_base.h:356-430 with
+// the contact chain, the history read / write of the entry and its flag.
+// Returns the contribution to i's force and torque.
+template <typename T, bool GEN, typename W>
+__device__ __forceinline__ void pair_contact(const StepArgs &a, const Model &pm, const int i,
+                                             const int k, const uint32_t e, const double4 &pi,
+                                             const double4 &vi, const W &wi, const int irow,
+                                             T dF[3], T dTi[3]) {
+  typedef typename OmgT<T>::type omg_t;
+  const DevParams &p = a.prm;
+  const size_t cap = (size_t)a.cap;
+  const size_t hw = GEN ? (size_t)p.hist_w : 3;  // history values per contact
+  T *hist = (T *)a.list.hist;
+  const int j = (int)(e & kIndexMask);
+  const double4 pj = a.in.pos[j];
+  const double4 vj = a.in.vel[j];
+  const omg_t wj = ((const omg_t *)a.in.omg)[j];
+  double dx, dy, dz;
+  min_image(p, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, dx, dy, dz);
+  const double rsq = dx * dx + dy * dy + dz * dz;
+  const double radsum = pi.w + pj.w;
+  const int jtype = (int)wj.w & kTypeMask;
+  const double r = sqrt(rsq);
+  const double rinv = 1.0 / r;
+  CIn<T> c;
+  c.radi = (T)pi.w;
+  c.radj = (T)pj.w;
+  c.rinv = (T)rinv;
+  c.meff = (T)(vi.w * vj.w / (vi.w + vj.w));
+  c.deltan = (T)(radsum - r);
+  c.deltan_in = T(0);
+  c.en[0] = (T)(dx * rinv);
+  c.en[1] = (T)(dy * rinv);
+  c.en[2] = (T)(dz * rinv);
+  c.del[0] = (T)dx;
+  c.del[1] = (T)dy;
+  c.del[2] = (T)dz;
+  c.vr[0] = (T)(vi.x - vj.x);
+  c.vr[1] = (T)(vi.y - vj.y);
+  c.vr[2] = (T)(vi.z - vj.z);
+  c.wi[0] = (T)wi.x;
+  c.wi[1] = (T)wi.y;
+  c.wi[2] = (T)wi.z;
+  c.wj[0] = (T)wj.x;
+  c.wj[1] = (T)wj.y;
+  c.wj[2] = (T)wj.z;
+  const int mt = irow + jtype;
+  if (sizeof(T) == 8) {
+    c.Y = (T)a.mat.Yeff[mt];
+    c.G = (T)a.mat.Geff[mt];
+    c.beta = (T)a.mat.beta[mt];
+    c.mu = (T)a.mat.mu[mt];
+    c.mur = (T)a.mat.mur[mt];
+  } else {
+    c.Y = (T)a.mat.fYeff[mt];
+    c.G = (T)a.mat.fGeff[mt];
+    c.beta = (T)a.mat.fbeta[mt];
+    c.mu = (T)a.mat.fmu[mt];
+    c.mur = (T)a.mat.fmur[mt];
+  }
+  if (GEN) {
+    load_model_coeffs(a.mat, p, pm, mt, c);
+    // sphere-sphere contact-circle area (cohesion_model_sjkr.h:97-98)
+    const double ri = pi.w, rj = pj.w;
+    c.acont = (T)(-kPi / 4 * ((r - ri - rj) * (r + ri - rj) * (r - ri + rj) * (r + ri + rj)) / (r * r));
+  }
+  const bool flagged = (e & kFlagBit) != 0;
+  // tangential no_history keeps no history: its slots hold the zeros
+  // written at the first touch and are neither read nor rewritten
+  const bool keeps = !(GEN && pm.nohist);
+  T sh[3], rh[3] = {T(0), T(0), T(0)};
+  const size_t hb = (size_t)k * hw * cap + i;
+  if (flagged && keeps) {
+    sh[0] = hist[hb];
+    sh[1] = hist[hb + cap];
+    sh[2] = hist[hb + 2 * cap];
+    if (GEN && pm.epsd2) {
+      rh[0] = hist[hb + 3 * cap];
+      rh[1] = hist[hb + 4 * cap];
+      rh[2] = hist[hb + 5 * cap];
+    }
+  } else {
+    sh[0] = sh[1] = sh[2] = T(0);
+  }
+  T dTj[3];
+  contact_chain<T, false, GEN>(pm, c, (T)p.dt, a.shearupdate, sh, rh, dF, dTi, dTj);
+  if ((a.shearupdate && keeps) || !flagged) {
+    hist[hb] = sh[0];
+    hist[hb + cap] = sh[1];
+    hist[hb + 2 * cap] = sh[2];
+    if (GEN && hw == 6) {
+      hist[hb + 3 * cap] = rh[0];
+      hist[hb + 4 * cap] = rh[1];
+      hist[hb + 5 * cap] = rh[2];
+    }
+  }
+  if (!flagged) a.list.nbr[(size_t)k * cap + i] = e | kFlagBit;
+}
+
+// Broad phase of list entries k0 .. k0+GROUP-1 of particle i (cnt entries):
+// the index loads of the group are independent, then its position gathers,
+// so a wavefront has GROUP memory requests in flight instead of one
+// dependent chain per entry. Entries past cnt gather particle `pad` (a valid
+// index: i itself, or 0 for a lane past the last particle) and are ignored.
+// Returns the touching entries as bits; entries in the surfacesClose band are
+// handled here.
+template <typename T, bool GEN, int GROUP>
+__device__ __forceinline__ uint32_t broad_phase(const StepArgs &a, const Model &pm, const int i,
+                                                const int pad, const int cnt, const int k0,
+                                                const double4 &pi, uint32_t (&ent)[GROUP]) {
+  const DevParams &p = a.prm;
+  const size_t cap = (size_t)a.cap;
+#pragma unroll
+  for (int u = 0; u < GROUP; u++)
+    ent[u] = (k0 + u < cnt) ? a.list.nbr[(size_t)(k0 + u) * cap + i] : (uint32_t)pad;
+  double4 pj[GROUP];
+#pragma unroll
+  for (int u = 0; u < GROUP; u++) pj[u] = a.in.pos[ent[u] & kIndexMask];
+  uint32_t bits = 0u, close = 0u;
+#pragma unroll
+  for (int u = 0; u < GROUP; u++) {
+    double dx, dy, dz;
+    min_image(p, pi.x, pi.y, pi.z, pj[u].x, pj[u].y, pj[u].z, dx, dy, dz);
+    const double rsq = dx * dx + dy * dy + dz * dz;
+    const double radsum = pi.w + pj[u].w;
+    if (k0 + u < cnt) {
+      if (rsq < radsum * radsum)
+        bits |= 1u << u;
+      else if (p.close_band && rsq < p.cdm * radsum * radsum)
+        close |= 1u << u;
+    }
+  }
+  if (close) {
+    const size_t hw = GEN ? (size_t)p.hist_w : 3;
+    close_entries<T, GEN, GROUP>(a, pm, i, k0, close, ent, (T *)a.list.hist, cap, hw);
+  }
+  return bits;
+}
+
+// Orders LDS accesses of one wavefront: the writes before it are visible to
+// the reads after it, by any lane of the same wavefront.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS of one wavefront for the compacted pair phase: the touching entries of
+// a group of the 64 particles (neighbor entry, owner lane << 16 | list slot)
+// and the contributions of one pass of the contact law.
+template <typename T, int GROUP>
+struct WaveLds {
+  uint2 queue[64 * GROUP];
+  T contrib[6][64];
+};
+
+// Pair forces of the wavefront's 64 particles with the touching entries
+// compacted: each group of GROUP list entries per particle is broad-phased by
+// its own lane, the touching entries of all lanes are ranked by a ballot
+// prefix sum (lane order, list order within a lane) into an LDS queue, and the
+// whole wavefront runs the contact law on 64 queued entries at a time; each
+// particle then adds its entries' contributions in list order, so the sums are
+// bitwise those of the per-lane path. Every lane of the wavefront takes part;
+// lanes past the last particle have cnt = 0.
+template <typename T, bool GEN, int GROUP>
+__device__ __forceinline__ void pair_forces_wave(const StepArgs &a, const Model &pm, const int i,
+                                                 const int pad, const int cnt, const double4 &pi,
+                                                 T &fx, T &fy,
+                                                 T &fz, T &tx, T &ty, T &tz,
+                                                 unsigned long long &my_contacts,
+                                                 WaveLds<T, GROUP> &L) {
+  typedef typename OmgT<T>::type omg_t;
+  const DevParams &p = a.prm;
+  const int lane = threadIdx.x & 63;
+  const int ibase = i - lane;  // particle of lane 0
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int kmax = cnt;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(kmax, off, 64);
+    kmax = o > kmax ? o : kmax;
+  }
+  for (int k0 = 0; k0 < kmax; k0 += GROUP) {
+    uint32_t ent[GROUP];
+    const uint32_t bits = broad_phase<T, GEN, GROUP>(a, pm, i, pad, cnt, k0, pi, ent);
+    // rank of this lane's first touching entry among the wavefront's
+    int first = 0, total = 0;
+#pragma unroll
+    for (int u = 0; u < GROUP; u++) {
+      const unsigned long long m = __ballot((bits >> u) & 1u);
+      first += __popcll(m & below);
+      total += __popcll(m);
+    }
+    const int mine = __popc(bits);
+    {
+      int r = first;
+#pragma unroll
+      for (int u = 0; u < GROUP; u++)
+        if ((bits >> u) & 1u) L.queue[r++] = make_uint2(ent[u], ((uint32_t)lane << 16) | (uint32_t)(k0 + u));
+    }
+    my_contacts += (unsigned long long)mine;
+    wave_sync();
+    for (int c0 = 0; c0 < total; c0 += 64) {
+      const int q = c0 + lane;
+      T dF[3] = {T(0), T(0), T(0)}, dTi[3] = {T(0), T(0), T(0)};
+      if (q < total) {
+        const uint2 qe = L.queue[q];
+        const int io = ibase + (int)(qe.y >> 16);
+        const int k = (int)(qe.y & 0xffffu);
+        const double4 po = a.in.pos[io];
+        const double4 vo = a.in.vel[io];
+        const omg_t wo = ((const omg_t *)a.in.omg)[io];
+        const int wb = (int)wo.w;
+        const int orow = (wb >> kTypeBits) * p.mat_stride + (wb & kTypeMask) * p.ntp1;
+        pair_contact<T, GEN>(a, pm, io, k, qe.x, po, vo, wo, orow, dF, dTi);
+      }
+      L.contrib[0][lane] = dF[0];
+      L.contrib[1][lane] = dF[1];
+      L.contrib[2][lane] = dF[2];
+      L.contrib[3][lane] = dTi[0];
+      L.contrib[4][lane] = dTi[1];
+      L.contrib[5][lane] = dTi[2];
+      wave_sync();
+      const int s = first > c0 ? first : c0;
+      const int e = (first + mine) < (c0 + 64) ? (first + mine) : (c0 + 64);
+      for (int q2 = s; q2 < e; q2++) {
+        const int l = q2 - c0;
+        fx += L.contrib[0][l];
+        fy += L.contrib[1][l];
+        fz += L.contrib[2][l];
+        tx += L.contrib[3][l];
+        ty += L.contrib[4][l];
+        tz += L.contrib[5][l];
+      }
+      wave_sync();
+    }
+  }
+}
+
+// One particle's step: contacts over its list, walls, meshes, gravity,
+// integration and the displacement test. Accumulates its touching pair and
+// wall contacts. WAVE: the pair phase is compacted over the wavefront
+// (pair_forces_wave; every lane calls, active says whether it holds a
+// particle); otherwise each lane resolves its own touching entries.
+template <typename T, bool MESH, int GROUP, bool GEN, bool WAVE>
+__device__ __forceinline__ void step_particle(const StepArgs &a, const int i, const bool active,
                                               unsigned long long &my_contacts,
                                               unsigned long long &my_wall, int &my_inst,
-                                              bool &displaced, const long long ago_next) {
+                                              bool &displaced, const long long ago_next,
+                                              WaveLds<T, GROUP> *lds) {
   typedef typename OmgT<T>::type omg_t;
   const DevParams &p = a.prm;
 
-  const double4 pi = a.in.pos[i];
-  const double4 vi = a.in.vel[i];
-  const omg_t wi = ((const omg_t *)a.in.omg)[i];
+  const int ii = active ? i : 0;
+  const double4 pi = a.in.pos[ii];
+  const double4 vi = a.in.vel[ii];
+  const omg_t wi = ((const omg_t *)a.in.omg)[ii];
   const int wbits = (int)wi.w;
   // material row of this particle: its instance's block, its type's row
   const int irow = (wbits >> kTypeBits) * p.mat_stride + (wbits & kTypeMask) * p.ntp1;
   my_inst = wbits >> kTypeBits;
-  const int meta0 = a.aux.meta[i];
+  const int meta0 = active ? a.aux.meta[i] : 0;
   int meta = meta0;
   const int cnt = meta & kMetaCountMask;
   const T dt = (T)p.dt;
   const Model pm = model_from_bits(p.pair_model);
   T fx = T(0), fy = T(0), fz = T(0), tx = T(0), ty = T(0), tz = T(0);
-  T *hist = (T *)a.list.hist;
+
+  if (WAVE) {
+    pair_forces_wave<T, GEN, GROUP>(a, pm, i, ii, cnt, pi, fx, fy, fz, tx, ty, tz, my_contacts, *lds);
+    if (!active) return;
+  } else {
+    // Touching entries of a group are resolved one by one, reloading their
+    // (cache-resident) index so the group's registers are free during the
+    // contact law.
+    for (int k0 = 0; k0 < cnt; k0 += GROUP) {
+      uint32_t ent[GROUP];
+      uint32_t bits = broad_phase<T, GEN, GROUP>(a, pm, i, i, cnt, k0, pi, ent);
+      while (bits) {
+        const int k = k0 + __builtin_ctz(bits);
+        bits &= bits - 1u;
+        const uint32_t e = a.list.nbr[(size_t)k * a.cap + i];
+        T dF[3], dTi[3];
+        pair_contact<T, GEN>(a, pm, i, k, e, pi, vi, wi, irow, dF, dTi);
+        fx += dF[0];
+        fy += dF[1];
+        fz += dF[2];
+        tx += dTi[0];
+        ty += dTi[1];
+        tz += dTi[2];
+        my_contacts++;
+      }
+    }
+  }
   const size_t cap = (size_t)a.cap;
   const size_t hw = GEN ? (size_t)p.hist_w : 3;  // history values per contact
-
-  // Broad phase in groups of GROUP entries: the index loads of a group are
-  // independent, then its position gathers, so a wavefront has GROUP
-  // memory requests in flight instead of one dependent chain per entry.
-  // Touching entries of the group are then resolved one by one, reloading
-  // their (cache-resident) index and position so the group's registers are
-  // free during the contact law.
-  for (int k0 = 0; k0 < cnt; k0 += GROUP) {
-    uint32_t ent[GROUP];
-#pragma unroll
-    for (int u = 0; u < GROUP; u++)
-      ent[u] = (k0 + u < cnt) ? a.list.nbr[(size_t)(k0 + u) * cap + i] : (uint32_t)i;
-    double4 pj[GROUP];
-#pragma unroll
-    for (int u = 0; u < GROUP; u++) pj[u] = a.in.pos[ent[u] & kIndexMask];
-    uint32_t bits = 0u;
-#pragma unroll
-    for (int u = 0; u < GROUP; u++) {
-      double dx, dy, dz;
-      min_image(p, pi.x, pi.y, pi.z, pj[u].x, pj[u].y, pj[u].z, dx, dy, dz);
-      const double rsq = dx * dx + dy * dy + dz * dz;
-      const double radsum = pi.w + pj[u].w;
-      if (k0 + u < cnt && rsq < radsum * radsum) bits |= 1u << u;
-    }
-   while (bits) {
-    const int k = k0 + __builtin_ctz(bits);
-    bits &= bits - 1u;
-    const size_t slot = (size_t)k * cap + i;
-    const uint32_t e = a.list.nbr[slot];
-    const int j = (int)(e & kIndexMask);
-    const double4 pj = a.in.pos[j];
-    double dx, dy, dz;
-    min_image(p, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, dx, dy, dz);
-    const double rsq = dx * dx + dy * dy + dz * dz;
-    const double radsum = pi.w + pj.w;
-    // touching: pair_gran_base.h:356-430
-    const double4 vj = a.in.vel[j];
-    const omg_t wj = ((const omg_t *)a.in.omg)[j];
-    const int jtype = (int)wj.w & kTypeMask;
-    const double r = sqrt(rsq);
-    const double rinv = 1.0 / r;
-    CIn<T> c;
-    c.radi = (T)pi.w;
-    c.radj = (T)pj.w;
-    c.rinv = (T)rinv;
-    c.meff = (T)(vi.w * vj.w / (vi.w + vj.w));
-    c.deltan = (T)(radsum - r);
-    c.deltan_in = T(0);
-    c.en[0] = (T)(dx * rinv);
-    c.en[1] = (T)(dy * rinv);
-    c.en[2] = (T)(dz * rinv);
-    c.del[0] = (T)dx;
-    c.del[1] = (T)dy;
-    c.del[2] = (T)dz;
-    c.vr[0] = (T)(vi.x - vj.x);
-    c.vr[1] = (T)(vi.y - vj.y);
-    c.vr[2] = (T)(vi.z - vj.z);
-    c.wi[0] = (T)wi.x;
-    c.wi[1] = (T)wi.y;
-    c.wi[2] = (T)wi.z;
-    c.wj[0] = (T)wj.x;
-    c.wj[1] = (T)wj.y;
-    c.wj[2] = (T)wj.z;
-    const int mt = irow + jtype;
-    if (sizeof(T) == 8) {
-      c.Y = (T)a.mat.Yeff[mt];
-      c.G = (T)a.mat.Geff[mt];
-      c.beta = (T)a.mat.beta[mt];
-      c.mu = (T)a.mat.mu[mt];
-      c.mur = (T)a.mat.mur[mt];
-    } else {
-      c.Y = (T)a.mat.fYeff[mt];
-      c.G = (T)a.mat.fGeff[mt];
-      c.beta = (T)a.mat.fbeta[mt];
-      c.mu = (T)a.mat.fmu[mt];
-      c.mur = (T)a.mat.fmur[mt];
-    }
-    if (GEN) {
-      load_model_coeffs(a.mat, p, pm, mt, c);
-      // sphere-sphere contact-circle area (cohesion_model_sjkr.h:97-98)
-      const double ri = pi.w, rj = pj.w;
-      c.acont = (T)(-kPi / 4 * ((r - ri - rj) * (r + ri - rj) * (r - ri + rj) * (r + ri + rj)) / (r * r));
-    }
-    const bool flagged = (e & kFlagBit) != 0;
-    // tangential no_history keeps no history: its slots hold the zeros
-    // written at the first touch and are neither read nor rewritten
-    const bool keeps = !(GEN && pm.nohist);
-    T sh[3], rh[3] = {T(0), T(0), T(0)};
-    const size_t hb = (size_t)k * hw * cap + i;
-    if (flagged && keeps) {
-      sh[0] = hist[hb];
-      sh[1] = hist[hb + cap];
-      sh[2] = hist[hb + 2 * cap];
-      if (GEN && pm.epsd2) {
-        rh[0] = hist[hb + 3 * cap];
-        rh[1] = hist[hb + 4 * cap];
-        rh[2] = hist[hb + 5 * cap];
-      }
-    } else {
-      sh[0] = sh[1] = sh[2] = T(0);
-    }
-    T dF[3], dTi[3], dTj[3];
-    contact_chain<T, false, GEN>(pm, c, dt, a.shearupdate, sh, rh, dF, dTi, dTj);
-    if ((a.shearupdate && keeps) || !flagged) {
-      hist[hb] = sh[0];
-      hist[hb + cap] = sh[1];
-      hist[hb + 2 * cap] = sh[2];
-      if (GEN && hw == 6) {
-        hist[hb + 3 * cap] = rh[0];
-        hist[hb + 4 * cap] = rh[1];
-        hist[hb + 5 * cap] = rh[2];
-      }
-    }
-    if (!flagged) a.list.nbr[slot] = e | kFlagBit;
-    fx += dF[0];
-    fy += dF[1];
-    fz += dF[2];
-    tx += dTi[0];
-    ty += dTi[1];
-    tz += dTi[2];
-    my_contacts++;
-   }
-  }
 
   // post_force fixes in script order: gravity (fix_gravity.cpp:331-338) and
   // plane walls (fix_wall_gran.cpp:1024-1153)
@@ -1017,7 +1197,7 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i,
 // meshes into the next step's geometry buffer. Every tile shape computes
 // the same arithmetic in the same order, so results do not depend on it.
 // ---------------------------------------------------------------------------
-template <typename T, bool MESH, int BLOCK, int GROUP, bool GEN = false>
+template <typename T, bool MESH, int BLOCK, int GROUP, bool GEN = false, bool WAVE = false>
 __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a) {
   // in a captured graph the step index is the control block's base plus this
   // launch's offset (a.step)
@@ -1032,18 +1212,21 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a) {
     return;
   }
   const DevParams &p = a.prm;
+  __shared__ WaveLds<T, GROUP> lds_all[WAVE ? BLOCK / 64 : 1];
+  WaveLds<T, GROUP> *lds = &lds_all[WAVE ? (threadIdx.x >> 6) : 0];
   unsigned long long my_contacts = 0, my_wall = 0;
   bool displaced = false;
   const int ntiles = (a.n + BLOCK - 1) / BLOCK;
   for (int tile = xcd_block(a.pblocks); tile < ntiles; tile += a.pblocks) {
     const int i = tile * BLOCK + threadIdx.x;
-    if (i < a.n) {
+    const bool active = i < a.n;
+    if (WAVE || active) {
       unsigned long long c = 0, w = 0;
       int inst = 0;
-      step_particle<T, MESH, GROUP, GEN>(a, i, c, w, inst, displaced, ago_next);
+      step_particle<T, MESH, GROUP, GEN, WAVE>(a, i, active, c, w, inst, displaced, ago_next, lds);
       my_contacts += c;
       my_wall += w;
-      if (a.count && a.inst_stats) {
+      if (active && a.count && a.inst_stats) {
         if (c) atomicAdd(&a.inst_stats[inst], c);
         if (w) atomicAdd(&a.inst_stats[p.ninst + inst], w);
       }
@@ -1178,6 +1361,17 @@ __global__ __launch_bounds__(kBlock) void k_wrap_keys(RebuildArgs a) {
     q.z = xs[2];
     a.cur.pos[i] = q;
   }
+  // Comm::exchange (comm.cpp:945-1003) with one process: an atom with
+  // x < lo or x >= hi in a dimension is sent to itself and taken back only
+  // inside the box, so beyond an f face it is lost
+  bool lost = false;
+  for (int d = 0; d < 3; d++) {
+    if (p.per[d]) continue;
+    const double lo = p.lo[d], hi = p.lo[d] + p.L[d];
+    if (((p.lost_faces >> (2 * d)) & 1) && xs[d] < lo) lost = true;
+    if (((p.lost_faces >> (2 * d + 1)) & 1) && xs[d] >= hi) lost = true;
+  }
+  if (lost) atomicAdd(&a.ctrl->nlost, 1ull);
   const int cx = cell_coord(p, 0, xs[0]);
   const int cy = cell_coord(p, 1, xs[1]);
   const int cz = cell_coord(p, 2, xs[2]);
@@ -1372,7 +1566,7 @@ __global__ __launch_bounds__(kBlock) void k_build(RebuildArgs a) {
                     const size_t ob = (size_t)m * hw * cap + io;
                     const size_t nb = (size_t)cnt * hw * cap + i;
                     for (size_t d = 0; d < hw; d++) nhist[nb + d * cap] = ohist[ob + d * cap];
-                    entry |= kFlagBit;
+                    entry |= kFlagBit | kCarryBit;
                     break;
                   }
                 }
@@ -1407,6 +1601,7 @@ __global__ void k_reset_build_counters(Ctrl *ctrl) {
     ctrl->npairs_full = 0;
     ctrl->mesh_overflow = 0;
     ctrl->nmesh_cand = 0;
+    ctrl->nlost = 0;
   }
 }
 
@@ -1679,8 +1874,12 @@ template <typename T, bool MESH>
 static void launch_step_variant(const StepArgs &a, int nb, hipStream_t s) {
   const int B = kStepVariantBlock[a.variant];
   if (a.prm.generic) {
-    // the sibling contact models run in the default tile shape only
-    hipLaunchKernelGGL((k_step<T, MESH, 256, 4, true>), dim3(nb), dim3(B), 0, s, a);
+    // the sibling contact models run in the default tile shape or its
+    // wave-compacted form
+    if (a.variant == 5)
+      hipLaunchKernelGGL((k_step<T, MESH, 256, 4, true, true>), dim3(nb), dim3(B), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_step<T, MESH, 256, 4, true>), dim3(nb), dim3(B), 0, s, a);
     return;
   }
   switch (a.variant) {
@@ -1696,6 +1895,12 @@ static void launch_step_variant(const StepArgs &a, int nb, hipStream_t s) {
     case 4:
       hipLaunchKernelGGL((k_step<T, MESH, 64, 4>), dim3(nb), dim3(B), 0, s, a);
       break;
+    case 5:
+      hipLaunchKernelGGL((k_step<T, MESH, 256, 4, false, true>), dim3(nb), dim3(B), 0, s, a);
+      break;
+    case 6:
+      hipLaunchKernelGGL((k_step<T, MESH, 256, 8, false, true>), dim3(nb), dim3(B), 0, s, a);
+      break;
     default:
       hipLaunchKernelGGL((k_step<T, MESH, 256, 4>), dim3(nb), dim3(B), 0, s, a);
       break;
@@ -1704,7 +1909,7 @@ static void launch_step_variant(const StepArgs &a, int nb, hipStream_t s) {
 
 hipError_t launch_step(const StepArgs &a, hipStream_t s) {
   if (a.variant < 0 || a.variant >= kNumStepVariants) return hipErrorInvalidValue;
-  if (a.prm.generic && a.variant != 0) return hipErrorInvalidValue;
+  if (a.prm.generic && a.variant != 0 && a.variant != 5) return hipErrorInvalidValue;
   const bool mesh = a.prm.nmesh > 0;
   // pblocks (tiles walked by the particle blocks) is set by the caller;
   // trailing blocks move the meshes when asked to

@@ -262,11 +262,16 @@ dem::DevParams make_devparams(const dem_handle *h) {
   d.neigh_check = p.neigh_no_check ? 0 : 1;
   d.skin = p.skin;
   d.cdf = p.cdf;
+  d.cdm = p.cdf * p.cdf;
+  d.close_band = p.cdf > 1.0 ? 1 : 0;
   for (int k = 0; k < 3; k++) {
     d.lo[k] = p.boxlo[k];
     d.L[k] = p.boxhi[k] - p.boxlo[k];
     d.halfL[k] = 0.5 * d.L[k];
     d.per[k] = p.periodic[k] ? 1 : 0;
+    if (!p.periodic[k])
+      for (int side = 0; side < 2; side++)
+        if (!((p.shrink_faces >> (2 * k + side)) & 1)) d.lost_faces |= 1 << (2 * k + side);
     d.nc[k] = h->nc[k];
     d.cinv[k] = h->nc[k] > 0 ? h->nc[k] / d.L[k] : 0.0;
   }
@@ -426,7 +431,8 @@ dem::StepArgs step_args(dem_handle *h, int in_buf, int tri_buf) {
   a.fout = h->fout;
   a.ctrl = h->ctrl;
   a.inst_stats = h->d_inst_stats;
-  a.variant = a.prm.generic ? 0 : h->variant;  // the sibling models run in shape 0 only
+  // the sibling models run in shapes 0 and 5 only
+  a.variant = (a.prm.generic && h->variant != 5) ? 0 : h->variant;
   {
     const int B = dem::kStepVariantBlock[a.variant];
     const int ntiles = (int)((std::max<int64_t>(h->n, 1) + B - 1) / B);
@@ -617,6 +623,13 @@ int rebuild(dem_handle *h) {
     h->rebuild_ms += ms;
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+  }
+  if (h->ctrl_host->nlost > 0) {
+    // Thermo::lost_check with thermo_modify lost error (thermo.cpp:434-441)
+    char buf[160];
+    snprintf(buf, sizeof buf, "Lost atoms: original %lld current %lld", (long long)h->n,
+             (long long)(h->n - (int64_t)h->ctrl_host->nlost));
+    return fail(h, DEM_ERR_STATE, buf);
   }
   if (h->ctrl_host->overflow > 0) {
     char buf[200];
@@ -837,6 +850,11 @@ int dem_create(const dem_params *params, dem_handle **out) {
   if (p.precision != DEM_FP64 && p.precision != DEM_MIXED)
     return fail(nullptr, DEM_ERR_ARG, "unknown precision mode");
   if (p.neigh_every < 0) return fail(nullptr, DEM_ERR_ARG, "neigh_modify every must be > 0");
+  for (int d = 0; d < 3; d++)
+    if (p.periodic[d] && ((p.shrink_faces >> (2 * d)) & 3))
+      return fail(nullptr, DEM_ERR_ARG, "Both sides of boundary must be periodic");
+  if ((p.shrink_min & ~p.shrink_faces) || (p.shrink_faces & ~0x3f))
+    return fail(nullptr, DEM_ERR_ARG, "bad shrink-wrapped face bits");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= p.device || p.device < 0)
     return fail(nullptr, DEM_ERR_NODEV, "no HIP device available");
@@ -1032,6 +1050,27 @@ int dem_add_mesh(dem_handle *h, int64_t ntri, const double *nodes, int wall_type
   h->meshes.push_back(std::move(hm));
   h->ntri += h->meshes.back().ntri;
   if (mesh_id) *mesh_id = (int)h->meshes.size() - 1;
+  return DEM_OK;
+}
+
+int dem_mesh_generate(const char *type, const double *dims, int nsegments, int mask, int64_t cap, double *nodes,
+                      int64_t *ntri) {
+  if (!type || !dims || !ntri) return fail(nullptr, DEM_ERR_ARG, "null argument");
+  *ntri = 0;
+  const std::string ty(type);
+  const int nd = ty == "box" ? 3 : ty == "pipe" ? 3 : ty == "cylinder" ? 2 : 1;
+  for (int k = 0; k < nd; k++)
+    if (!(dims[k] > 0)) return fail(nullptr, DEM_ERR_ARG, "generator sizes must be > 0");
+  const bool segs = ty == "cylinder" || ty == "pipe" || ty == "disk";
+  const bool masked = ty == "cube" || ty == "box" || ty == "cylinder" || ty == "pipe";
+  if ((segs && nsegments <= 0) || (masked && mask <= 0))
+    return fail(nullptr, DEM_ERR_ARG, "nsegments and mask must be > 0");
+  double dv[3] = {dims[0], nd > 1 ? dims[1] : 0.0, nd > 2 ? dims[2] : 0.0};
+  std::vector<double> out;
+  std::string err;
+  if (!dem::generate_mesh(ty, dv, nsegments, mask, out, err)) return fail(nullptr, DEM_ERR_ARG, err);
+  *ntri = (int64_t)(out.size() / 9);
+  if (nodes) std::copy(out.begin(), out.begin() + 9 * std::min<int64_t>(cap, *ntri), nodes);
   return DEM_OK;
 }
 
@@ -1398,9 +1437,11 @@ int dem_set_step_shape(dem_handle *h, int variant, int persist) {
   if (variant == -1) {
     tu = dem_handle::Tuner{};
     tu.active = true;
-    const int nv = uses_generic_models(h) ? 1 : dem::kNumStepVariants;
-    for (int v = 0; v < nv; v++)
+    const bool gen = uses_generic_models(h);
+    for (int v = 0; v < dem::kNumStepVariants; v++) {
+      if (gen && v != 0 && v != 5) continue;  // the sibling models' shapes
       for (int f : {0, 4, 8}) tu.cand.push_back({v, f});
+    }
     tu.us.assign(tu.cand.size(), 0.0);
     return DEM_OK;
   }
@@ -1664,6 +1705,41 @@ int dem_compute_forces(dem_handle *h, int shearupdate) {
     if ((rc = rebuild(h))) return rc;
   }
   return force_pass(h, shearupdate ? 1 : 0);
+}
+
+int dem_get_box(dem_handle *h, double *boxlo, double *boxhi) {
+  if (!h || !boxlo || !boxhi) return DEM_ERR_ARG;
+  const dem_params &p = h->p;
+  for (int d = 0; d < 3; d++) {
+    boxlo[d] = p.boxlo[d];
+    boxhi[d] = p.boxhi[d];
+  }
+  if (!p.shrink_faces) return DEM_OK;
+  if (h->n <= 0 || !h->have_list) return DEM_OK;
+  // Domain::reset_box (domain.cpp:381-449): the extent of the atoms where the
+  // last build saw them (xhold) plus small = 1e-4 of the created box length
+  // (domain.cpp:85, 254-259); an m face not inside its created position
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const size_t cap = (size_t)h->cap;
+  std::vector<double> xh(3 * cap);
+  HIPCHK(h, hipMemcpy(xh.data(), h->xhold, sizeof(double) * 3 * cap, hipMemcpyDeviceToHost));
+  for (int d = 0; d < 3; d++) {
+    double mn = 1.0e20, mx = -1.0e20;  // BIG of domain.cpp:84
+    for (int64_t i = 0; i < h->n; i++) {
+      mn = std::min(mn, xh[d * cap + i]);
+      mx = std::max(mx, xh[d * cap + i]);
+    }
+    const double small = 1.0e-4 * (p.boxhi[d] - p.boxlo[d]);
+    if ((p.shrink_faces >> (2 * d)) & 1) {
+      boxlo[d] = -(-mn) - small;
+      if ((p.shrink_min >> (2 * d)) & 1) boxlo[d] = std::min(boxlo[d], p.boxlo[d]);
+    }
+    if ((p.shrink_faces >> (2 * d + 1)) & 1) {
+      boxhi[d] = mx + small;
+      if ((p.shrink_min >> (2 * d + 1)) & 1) boxhi[d] = std::max(boxhi[d], p.boxhi[d]);
+    }
+  }
+  return DEM_OK;
 }
 
 int dem_get_state(dem_handle *h, double *x, double *v, double *omega, double *f, double *torque) {

@@ -345,4 +345,126 @@ bool read_ascii_stl(const std::string &path, std::vector<double> &nodes, std::st
   return true;
 }
 
+namespace {
+
+constexpr double kGenPi = 3.14159265358979323846;  // M_PI
+
+void add_tri(std::vector<double> &out, double ax, double ay, double az, double bx, double by, double bz, double cx,
+             double cy, double cz) {
+  const double v[9] = {ax, ay, az, bx, by, bz, cx, cy, cz};
+  out.insert(out.end(), v, v + 9);
+}
+
+// points (r cos(i da), r sin(i da)), da = 2 pi / nseg (input_mesh_tri.cpp:820-828)
+std::vector<double> ring(double r, int nseg) {
+  std::vector<double> out(2 * (size_t)nseg);
+  const double da = (2 * kGenPi) / (double)(nseg);
+  for (int i = 0; i < nseg; ++i) {
+    out[2 * i] = r * std::cos(i * da);
+    out[2 * i + 1] = r * std::sin(i * da);
+  }
+  return out;
+}
+
+}  // namespace
+
+bool generate_mesh(const std::string &type, const double *dv, int nseg, int mask, std::vector<double> &nodes,
+                   std::string &err) {
+  nodes.clear();
+  if (type == "cube" || type == "box") {
+    // face bits: top 1, bottom 2, front 4, back 8, left 16, right 32
+    // (InputMeshTri::BoxMask, input_mesh_tri.h:66-73)
+    const double x = dv[0] / 2.0, y = (type == "cube" ? dv[0] : dv[1]) / 2.0,
+                 z = (type == "cube" ? dv[0] : dv[2]) / 2.0;
+    if (mask & 16) {
+      add_tri(nodes, -x, y, z, -x, y, -z, -x, -y, z);
+      add_tri(nodes, -x, -y, z, -x, y, -z, -x, -y, -z);
+    }
+    if (mask & 32) {
+      add_tri(nodes, x, y, -z, x, y, z, x, -y, -z);
+      add_tri(nodes, x, -y, -z, x, y, z, x, -y, z);
+    }
+    if (mask & 1) {
+      add_tri(nodes, x, y, z, -x, y, z, x, -y, z);
+      add_tri(nodes, x, -y, z, -x, y, z, -x, -y, z);
+    }
+    if (mask & 2) {
+      add_tri(nodes, -x, y, -z, x, y, -z, -x, -y, -z);
+      add_tri(nodes, -x, -y, -z, x, y, -z, x, -y, -z);
+    }
+    if (mask & 4) {
+      add_tri(nodes, x, -y, -z, x, -y, z, -x, -y, -z);
+      add_tri(nodes, -x, -y, -z, x, -y, z, -x, -y, z);
+    }
+    if (mask & 8) {
+      add_tri(nodes, x, y, z, x, y, -z, -x, y, z);
+      add_tri(nodes, -x, y, z, x, y, -z, -x, y, -z);
+    }
+  } else if (type == "cylinder") {
+    // face bits: top 1, bottom 2, side 4 (InputMeshTri::CylinderMask)
+    const double z = dv[1] / 2.0;
+    const std::vector<double> R = ring(dv[0], nseg);
+    for (int i = 0; i < nseg; ++i) {
+      const int p = i == 0 ? nseg - 1 : i - 1;
+      const double xi = R[2 * i], yi = R[2 * i + 1], xp = R[2 * p], yp = R[2 * p + 1];
+      if (mask & 1) add_tri(nodes, 0, 0, z, xi, yi, z, xp, yp, z);
+      if (mask & 2) add_tri(nodes, 0, 0, -z, xi, yi, -z, xp, yp, -z);
+      if (mask & 4) {
+        add_tri(nodes, xi, yi, z, xp, yp, z, xi, yi, -z);
+        add_tri(nodes, xp, yp, z, xp, yp, -z, xi, yi, -z);
+      }
+    }
+  } else if (type == "pipe") {
+    // face bits: inner top 1, outer top 2, inner bottom 4, outer bottom 8,
+    // inner side 16, outer side 32 (InputMeshTri::PipeMask)
+    if (!(dv[1] > dv[0])) {
+      err = "fix mesh/surface generate pipe: outer_radius must exceed inner_radius";
+      return false;
+    }
+    const double z = dv[2] / 2.0;
+    const std::vector<double> I = ring(dv[0], nseg), O = ring(dv[1], nseg);
+    for (int i = 0; i < nseg; ++i) {
+      const int p = i == 0 ? nseg - 1 : i - 1;
+      const double ix = I[2 * i], iy = I[2 * i + 1], ipx = I[2 * p], ipy = I[2 * p + 1];
+      const double ox = O[2 * i], oy = O[2 * i + 1], opx = O[2 * p], opy = O[2 * p + 1];
+      if (mask & 1) add_tri(nodes, 0, 0, z, ix, iy, z, ipx, ipy, z);
+      if (mask & 2) {
+        add_tri(nodes, ix, iy, z, ipx, ipy, z, ox, oy, z);
+        add_tri(nodes, ipx, ipy, z, opx, opy, z, ox, oy, z);
+      }
+      if (mask & 4) add_tri(nodes, 0, 0, -z, ix, iy, -z, ipx, ipy, -z);
+      if (mask & 8) {
+        add_tri(nodes, ix, iy, -z, ipx, ipy, -z, ox, oy, -z);
+        add_tri(nodes, ipx, ipy, -z, opx, opy, -z, ox, oy, -z);
+      }
+      if (mask & 16) {
+        add_tri(nodes, ix, iy, z, ipx, ipy, z, ix, iy, -z);
+        add_tri(nodes, ipx, ipy, z, ipx, ipy, -z, ix, iy, -z);
+      }
+      if (mask & 32) {
+        add_tri(nodes, ox, oy, z, opx, opy, z, ox, oy, -z);
+        add_tri(nodes, opx, opy, z, opx, opy, -z, ox, oy, -z);
+      }
+    }
+  } else if (type == "disk") {
+    const std::vector<double> R = ring(dv[0], nseg);
+    for (int i = 0; i < nseg; ++i) {
+      const int p = i == 0 ? nseg - 1 : i - 1;
+      add_tri(nodes, 0, 0, 0, R[2 * i], R[2 * i + 1], 0, R[2 * p], R[2 * p + 1], 0);
+    }
+  } else if (type == "plane") {
+    const double a = dv[0] / 2.0;
+    add_tri(nodes, a, a, 0, -a, a, 0, a, -a, 0);
+    add_tri(nodes, a, -a, 0, -a, a, 0, -a, -a, 0);
+  } else {
+    err = "unknown generator type '" + type + "'";
+    return false;
+  }
+  if (nodes.empty()) {
+    err = "fix mesh/surface generate " + type + ": the mask selects no face";
+    return false;
+  }
+  return true;
+}
+
 }  // namespace dem

@@ -54,4 +54,15 @@ void fill_tri_geom(const HostMesh &hm, int mesh_index, std::vector<TriGeom> &out
 // Reads the triangles of an ASCII STL file (input_mesh_tri.cpp:318).
 bool read_ascii_stl(const std::string &path, std::vector<double> &nodes, std::string &err);
 
+// fix mesh/surface ... generate <type> <values>: the triangles of the mesh
+// generators (InputMeshTri::generate_box / cylinder / pipe / disk / plane,
+// input_mesh_tri.cpp:688-1107), centred on the origin, in the generator's
+// triangle and node order (the order fixes the element ids the topology rules
+// use). dv: cube size | box xsize ysize zsize | cylinder radius zsize | pipe
+// inner_radius outer_radius zsize | disk radius | plane size; nseg: number of
+// segments (cylinder, pipe, disk); mask: the face bits (cube, box, cylinder,
+// pipe).
+bool generate_mesh(const std::string &type, const double *dv, int nseg, int mask, std::vector<double> &nodes,
+                   std::string &err);
+
 }  // namespace dem

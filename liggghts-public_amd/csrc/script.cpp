@@ -22,7 +22,8 @@
 //   fix <id> <grp> gravity <g> vector <x> <y> <z>
 //   fix <id> <grp> wall/gran model <normal> tangential <tangential> [...]
 //                  primitive type <t> xplane|yplane|zplane <pos>
-//   fix <id> <grp> mesh/surface file <stl> type <t> [scale s] [move dx dy dz]
+//   fix <id> <grp> mesh/surface file <stl> | generate <cube|box|cylinder|pipe|disk|plane> <values>
+//       type <t> [scale s] [move dx dy dz]
 //                  [rotate axis ax ay az angle deg] [precision p] [verbose/heal ...]
 //   fix <id> <grp> mesh/surface/stress ... [reference_point x y z] [stress on|off]
 //                  (thermo f_<id>[1..6]: total force and torque on the mesh)
@@ -132,6 +133,12 @@ struct Interp {
   bool have_box = false;
   double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   int periodic[3] = {1, 1, 1};
+  // boundary s / m faces (bit 2d lower, 2d+1 upper face of dimension d) and
+  // the face letters for dump box headers (Domain::boundary_string)
+  int shrink_faces = 0, shrink_min = 0;
+  char face[3][2] = {{'p', 'p'}, {'p', 'p'}, {'p', 'p'}};
+  // thermo_modify lost error | warn | ignore (Thermo::lostflag, thermo.cpp:410-452)
+  std::string lost_policy = "error";
   int ntypes = 0;
   std::map<std::string, std::vector<double>> regions;
   // particles
@@ -371,9 +378,14 @@ struct Interp {
     for (size_t i = 0; i < n; i++) order[i] = i;
     std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return tag[a] < tag[b]; });
     char bstr[9];
-    for (int d = 0; d < 3; d++) bstr[3 * d] = bstr[3 * d + 1] = periodic[d] ? 'p' : 'f';
+    for (int d = 0; d < 3; d++) {
+      bstr[3 * d] = face[d][0];
+      bstr[3 * d + 1] = face[d][1];
+    }
     bstr[2] = bstr[5] = ' ';
     bstr[8] = '\0';
+    double blo[3], bhi[3];
+    current_box(blo, bhi);
     for (auto &kv : dumps) {
       Dump &d = kv.second;
       if (d.last == step || !(force_all || step % d.every == 0)) continue;
@@ -393,7 +405,7 @@ struct Interp {
       for (size_t c = 0; c < d.cols.size(); c++) cols += (c ? " " : "") + d.cols[c];
       fprintf(fp, "ITEM: TIMESTEP\n%lld\nITEM: NUMBER OF ATOMS\n%zu\nITEM: BOX BOUNDS %s\n", (long long)step, n,
               bstr);
-      for (int k = 0; k < 3; k++) fprintf(fp, "%g %g\n", lo[k], hi[k]);
+      for (int k = 0; k < 3; k++) fprintf(fp, "%g %g\n", blo[k], bhi[k]);
       fprintf(fp, "ITEM: ATOMS %s\n", cols.c_str());
       for (size_t oi = 0; oi < n; oi++) {
         const size_t i = order[oi];
@@ -645,12 +657,47 @@ struct Interp {
     //     [rotate axis x y z angle a] [precision p] [verbose yes|no] [heal ...]
     // mesh/surface/stress also takes [reference_point x y z] [stress on|off]
     // (MeshModuleStress, mesh_module_stress.cpp:75-120)
-    if (t.size() < 6 || t[4] != "file") die("fix mesh/surface: expected 'file <stl>'");
+    if (t.size() < 6 || (t[4] != "file" && t[4] != "generate"))
+      die("fix mesh/surface: expecting keyword 'file', 'generate' or 'fix'");
     MeshDef md;
     md.stress = t[3] == "mesh/surface/stress";
     std::string err;
-    if (!dem::read_ascii_stl(t[5], md.nodes, err)) die(err);
-    for (size_t k = 6; k < t.size();) {
+    size_t k0 = 6;
+    if (t[4] == "file") {
+      if (!dem::read_ascii_stl(t[5], md.nodes, err)) die(err);
+    } else {
+      // FixMesh::parse_and_consume_generator_args (fix_mesh.cpp:767-874): the
+      // values in order, each > 0, and at least one more word after each
+      const std::string &gt = t[5];
+      std::vector<std::string> names;
+      if (gt == "cube") names = {"size", "mask"};
+      else if (gt == "box") names = {"xsize", "ysize", "zsize", "mask"};
+      else if (gt == "cylinder") names = {"radius", "zsize", "nsegments", "mask"};
+      else if (gt == "pipe") names = {"inner_radius", "outer_radius", "zsize", "nsegments", "mask"};
+      else if (gt == "disk") names = {"radius", "nsegments"};
+      else if (gt == "plane") names = {"size"};
+      else die("fix mesh/surface: unknown generator type '" + gt + "'");
+      double dv[3] = {0, 0, 0};
+      int nd = 0, nseg = 0, mask = 0;
+      size_t a = 6;
+      for (const std::string &nm : names) {
+        if (a + 1 >= t.size()) die("fix mesh/surface: missing generator argument: " + nm);
+        const bool integer = nm == "mask" || nm == "nsegments";
+        char *end = nullptr;
+        const double val = integer ? (double)std::strtol(t[a].c_str(), &end, 10) : std::strtod(t[a].c_str(), &end);
+        if (end == t[a].c_str() || *end != '\0')
+          die("fix mesh/surface: generator argument '" + nm + "' is not a valid value");
+        if (val <= 0) die("fix mesh/surface: generator argument '" + nm + "' must be > 0");
+        if (nm == "mask") mask = (int)val;
+        else if (nm == "nsegments") nseg = (int)val;
+        else dv[nd++] = val;
+        a++;
+      }
+      if (!dem::generate_mesh(gt, dv, nseg, mask, md.nodes, err)) die(err);
+      fprintf(out, "%s mesh generated: %zu triangels\n", gt == "cube" ? "box" : gt.c_str(), md.nodes.size() / 9);
+      k0 = a;
+    }
+    for (size_t k = k0; k < t.size();) {
       if (t[k] == "type" && k + 1 < t.size()) {
         md.type = inum(t[k + 1]);
         k += 2;
@@ -966,6 +1013,8 @@ struct Interp {
       p.boxhi[d] = hi[d];
       p.periodic[d] = periodic[d];
     }
+    p.shrink_faces = shrink_faces;
+    p.shrink_min = shrink_min;
     p.rolling_cdt = pair_m.rolling_cdt;
     p.tangential_damping = pair_m.tangential_damping;
     p.limit_force = pair_m.limit_force;
@@ -1036,6 +1085,16 @@ struct Interp {
     fprintf(out, "%s\n", hdr.c_str());
   }
 
+  // the box at the last neighbor build: shrink-wrapped faces follow the atoms
+  // (Domain::reset_box); the created box before any run
+  void current_box(double *blo, double *bhi) {
+    for (int d = 0; d < 3; d++) {
+      blo[d] = lo[d];
+      bhi[d] = hi[d];
+    }
+    if (!sq && h && shrink_faces && dem_get_box(h, blo, bhi)) die(dem_last_error(h));
+  }
+
   // f_<mesh>[1..6]: force and torque on a mesh/surface/stress
   // (FixMeshSurface::compute_vector, fix_mesh_surface.cpp:560-570);
   // c_<id> of compute erotate/sphere (0.5 I w^2 with I = 0.4 m r^2,
@@ -1043,7 +1102,9 @@ struct Interp {
   bool thermo_value(const std::string &c, double ke, double &val) {
     // Thermo::compute_vol / compute_time / compute_cpu (thermo.cpp:931-1003)
     if (c == "vol") {
-      val = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+      double blo[3], bhi[3];
+      current_box(blo, bhi);
+      val = (bhi[0] - blo[0]) * (bhi[1] - blo[1]) * (bhi[2] - blo[2]);
       return true;
     }
     if (c == "time") {
@@ -1136,7 +1197,14 @@ struct Interp {
       int rc = sq ? dem_sqdem_run(hs, (int)k) : dem_run_continue(h, k);
       (void)first;
       first = false;
-      if (rc) die(sq ? dem_sqdem_last_error(hs) : dem_last_error(h));
+      if (rc) {
+        std::string msg = sq ? dem_sqdem_last_error(hs) : dem_last_error(h);
+        // thermo_modify lost ignore / warn would delete the atom and go on; this
+        // path keeps every atom, so a loss ends the run whatever the setting
+        if (lost_policy != "error" && msg.rfind("Lost atoms", 0) == 0)
+          msg += " (thermo_modify lost " + lost_policy + ": atoms are not deleted on this path)";
+        die(msg);
+      }
       done += k;
       step += k;
       if ((thermo_every > 0 && step % thermo_every == 0) || done == n) thermo_line();
@@ -1179,13 +1247,21 @@ struct Interp {
         die("Only atom_style granular/sphere/superquadric is supported on this path");
       }
     } else if (c == "atom_modify" || c == "processors" || c == "echo" || c == "log" ||
-               c == "thermo_modify" ||
                (c == "compute" && !(t.size() >= 4 && (t[3] == "erotate/sphere" || t[3] == "ke"))) ||
                c == "uncompute" || c == "group" || c == "restart" ||
                c == "write_dump" || c == "soft_particles" ||
                c == "hard_particles") {
-      if (c != "atom_modify" && c != "processors" && c != "echo" && c != "log" && c != "thermo_modify")
+      if (c != "atom_modify" && c != "processors" && c != "echo" && c != "log")
         fprintf(out, "WARNING: '%s' is output/bookkeeping only and is ignored on this path\n", c.c_str());
+    } else if (c == "thermo_modify") {
+      // Thermo::modify_params (thermo.cpp:476-602): lost error|warn|ignore is
+      // kept; the formatting keywords change output only
+      for (size_t k = 1; k < t.size(); k++)
+        if (t[k] == "lost") {
+          if (k + 1 >= t.size() || (t[k + 1] != "error" && t[k + 1] != "warn" && t[k + 1] != "ignore"))
+            die("Illegal thermo_modify command");
+          lost_policy = t[++k];
+        }
     } else if (c == "dump") {
       dump_command(t);
     } else if (c == "dump_modify") {
@@ -1203,8 +1279,25 @@ struct Interp {
       // compute id group erotate/sphere | ke: read by thermo as c_id
       computes[t[1]] = t[3];
     } else if (c == "boundary") {
+      // Domain::set_boundary (domain.cpp:194-238): p, or one or two of f s m
       if (t.size() < 4) die("Illegal boundary command");
-      for (int d = 0; d < 3; d++) periodic[d] = t[1 + d] == "p" ? 1 : 0;
+      shrink_faces = shrink_min = 0;
+      for (int d = 0; d < 3; d++) {
+        const std::string &b = t[1 + d];
+        if (b.empty() || b.size() > 2) die("Illegal boundary command");
+        const char lo_c = b[0], hi_c = b.size() == 2 ? b[1] : b[0];
+        for (char ch : {lo_c, hi_c})
+          if (ch != 'p' && ch != 'f' && ch != 's' && ch != 'm') die("Illegal boundary command");
+        if ((lo_c == 'p') != (hi_c == 'p')) die("Both sides of boundary must be periodic");
+        periodic[d] = lo_c == 'p' ? 1 : 0;
+        face[d][0] = lo_c;
+        face[d][1] = hi_c;
+        for (int side = 0; side < 2; side++) {
+          const char ch = face[d][side];
+          if (ch == 's' || ch == 'm') shrink_faces |= 1 << (2 * d + side);
+          if (ch == 'm') shrink_min |= 1 << (2 * d + side);
+        }
+      }
     } else if (c == "newton") {
       if (t.size() < 2 || t[1] != "off")
         die("Pair granular with shear history requires newton pair off");  // pair_gran.cpp:260
@@ -1256,7 +1349,12 @@ struct Interp {
       if (dem_restart_particles(t[1].c_str(), nr, tag.data(), type.data(), x.data(), v.data(), w.data(),
                                 r.data(), m.data()))
         die(dem_last_error(nullptr));
-      for (int d = 0; d < 3; d++) periodic[d] = per[d];
+      for (int d = 0; d < 3; d++) {
+        periodic[d] = per[d];
+        // the file keeps periodicity only: non-periodic faces come back as f
+        for (int side = 0; side < 2; side++) face[d][side] = per[d] ? 'p' : 'f';
+      }
+      shrink_faces = shrink_min = 0;
       ntypes = nt;
       have_box = true;
       step = ts;

@@ -25,7 +25,8 @@ EXPORTED = [
     "dem_set_kernel_timing",
     "dem_run_script", "dem_add_mesh", "dem_mesh_rotate", "dem_get_mesh_nodes", "dem_mesh_linear", "dem_mesh_wiggle", "dem_mesh_stress", "dem_get_mesh_force", "dem_mesh_viblin", "dem_mesh_vibrot",
     "dem_write_restart", "dem_read_restart", "dem_restart_info", "dem_restart_particles",
-    "dem_get_mesh_topology", "dem_get_mesh_history", "dem_set_instances",
+    "dem_get_mesh_topology", "dem_get_mesh_history", "dem_set_instances", "dem_get_box",
+    "dem_mesh_generate",
     "dem_set_instance_materials", "dem_get_instance_stats", "dem_set_step_shape",
     "dem_get_step_shape", "dem_set_graph_capture", "dem_set_pair_model", "dem_set_model_coefficients",
     "dem_sq_create", "dem_sq_destroy",
@@ -63,6 +64,8 @@ class DemParams(ctypes.Structure):
         ("device", ctypes.c_int),
         ("neigh_every", ctypes.c_int),
         ("neigh_no_check", ctypes.c_int),
+        ("shrink_faces", ctypes.c_int),
+        ("shrink_min", ctypes.c_int),
     ]
 
 
@@ -202,6 +205,9 @@ def load_library(path=LIB_PATH):
     lib.dem_sq_last_error.argtypes = [h]
     lib.dem_sq_last_error.restype = ctypes.c_char_p
     lib.dem_sq_set_pairs.argtypes = [h, ctypes.c_int64, dp, dp]
+    lib.dem_mesh_generate.argtypes = [ctypes.c_char_p, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, dp,
+                                      P(ctypes.c_int64)]
+    lib.dem_get_box.argtypes = [h, dp, dp]
     lib.dem_sq_solve.argtypes = [h, ctypes.c_int, dp, ctypes.c_int]
     lib.dem_sq_get_results.argtypes = [h, dp, dp, dp, ip, ip, dp, dp]
     lib.dem_sq_check.argtypes = [h, ctypes.c_int, ctypes.c_double, dp, dp, dp, ip, dp, dp, dp, dp]
@@ -315,6 +321,9 @@ class GpuSystem:
         p.device = device
         p.neigh_every = int(params.get("neigh_every", 1))
         p.neigh_no_check = int(not params.get("neigh_check", True))
+        # boundary s / m faces (bit 2d lower, 2d+1 upper face of dimension d)
+        p.shrink_faces = int(params.get("shrink_faces", 0))
+        p.shrink_min = int(params.get("shrink_min", 0))
         self.h = ctypes.c_void_p()
         rc = self.lib.dem_create(ctypes.byref(p), ctypes.byref(self.h))
         if rc != 0:
@@ -426,6 +435,13 @@ class GpuSystem:
         npairs = int(total.value)
         return tags[:npairs], flags[:npairs], hist[:npairs]
 
+    def box(self):
+        """(boxlo, boxhi) at the last neighbor build (shrink-wrapped faces
+        follow the atoms)."""
+        lo, hi = np.zeros(3), np.zeros(3)
+        self._check(self.lib.dem_get_box(self.h, _dp(lo), _dp(hi)))
+        return lo, hi
+
     def wall_history(self):
         out = np.zeros((self.n, max(self.nwalls, 1), 3))
         if self.nwalls:
@@ -524,6 +540,20 @@ class GpuSystem:
             self.close()
         except Exception:
             pass
+
+
+def mesh_generate(kind, dims, nsegments=0, mask=0):
+    """Triangles (T, 3, 3) of fix mesh/surface ... generate <kind> (host only)."""
+    lib = load_library()
+    d = np.zeros(3)
+    d[:len(dims)] = dims
+    nt = ctypes.c_int64()
+    rc = lib.dem_mesh_generate(kind.encode(), _dp(d), int(nsegments), int(mask), 0, None, ctypes.byref(nt))
+    if rc:
+        raise DemError(lib.dem_last_error(None).decode())
+    out = np.zeros((max(nt.value, 1), 9))
+    lib.dem_mesh_generate(kind.encode(), _dp(d), int(nsegments), int(mask), nt.value, _dp(out), ctypes.byref(nt))
+    return out[:nt.value].reshape(-1, 3, 3)
 
 
 def run_script(script, log=None):

@@ -119,7 +119,7 @@ def packed_1m(seed=0, n=1_000_000, overlap=0.01):
     """The 1M polydisperse settling spheres (radii U(1.0, 1.5) mm, CDT rolling
     friction 0.1, dt 5e-6 s, skin 1 mm, periodic in x and y, zplane floor)
     started as a compressed lattice instead of a dilute one: the lattice
-    spacing is 1 % below the mean diameter, so about six neighbors of every
+    spacing is 1 % below the mean diameter, so about three neighbors of every
     sphere touch and the step carries tangential history for each of them."""
     params, mats, P, walls = dense_pack(seed=seed, n=n, rolling=True, overlap=overlap,
                                         radius=(0.001, 0.0015), speed=0.01, spin=1.0)

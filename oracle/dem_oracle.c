@@ -552,6 +552,9 @@ struct orc_sys {
   int64_t mesh_next_reneighbor;
   int64_t last_build; /* timestep of the last neighbor build (Neighbor::lastcall) */
   int64_t mesh_forced;
+  int64_t nlost;            /* atoms beyond an f face at the last build */
+  int64_t first_lost_step;  /* first timestep whose build lost atoms, -1 if none */
+  double box_lo[3], box_hi[3]; /* box after the last build's reset_box */
 };
 
 static void *xcalloc(size_t n, size_t sz) {
@@ -567,6 +570,11 @@ orc_sys *orc_create(const orc_params *p, const orc_materials *m, int n, const do
   s->p = *p;
   s->m = *m;
   s->n = n;
+  s->first_lost_step = -1;
+  for (int d = 0; d < 3; d++) {
+    s->box_lo[d] = p->boxlo[d];
+    s->box_hi[d] = p->boxhi[d];
+  }
   s->x = (double *)xcalloc(3 * (size_t)n, sizeof(double));
   s->v = (double *)xcalloc(3 * (size_t)n, sizeof(double));
   s->w = (double *)xcalloc(3 * (size_t)n, sizeof(double));
@@ -645,6 +653,66 @@ static void pbc(orc_sys *s) {
         if (*xi < lo) *xi = lo;
       }
     }
+}
+
+/* Domain::reset_box (domain.cpp:381-449) for the shrink-wrapped faces: the
+ * atom extent (start values BIG = 1e20, domain_definitions.h:60) plus
+ * small = SMALL * created box length (SMALL 1e-4, domain.cpp:85, 254-259);
+ * an m face takes the outer of that and its created position */
+static void reset_box(orc_sys *s) {
+  for (int d = 0; d < 3; d++) {
+    s->box_lo[d] = s->p.boxlo[d];
+    s->box_hi[d] = s->p.boxhi[d];
+    if (s->p.periodic[d]) continue;
+    const int slo = (s->p.shrink_faces >> (2 * d)) & 1, shi = (s->p.shrink_faces >> (2 * d + 1)) & 1;
+    if (!slo && !shi) continue;
+    double mn = 1.0e20, mx = -1.0e20;
+    for (int i = 0; i < s->n; i++) {
+      if (s->x[3 * i + d] < mn) mn = s->x[3 * i + d];
+      if (s->x[3 * i + d] > mx) mx = s->x[3 * i + d];
+    }
+    const double small = 1.0e-4 * (s->p.boxhi[d] - s->p.boxlo[d]);
+    if (slo) {
+      s->box_lo[d] = mn - small;
+      if (((s->p.shrink_min >> (2 * d)) & 1) && s->p.boxlo[d] < s->box_lo[d]) s->box_lo[d] = s->p.boxlo[d];
+    }
+    if (shi) {
+      s->box_hi[d] = mx + small;
+      if (((s->p.shrink_min >> (2 * d + 1)) & 1) && s->p.boxhi[d] > s->box_hi[d]) s->box_hi[d] = s->p.boxhi[d];
+    }
+  }
+}
+
+/* Comm::exchange with one process (comm.cpp:945-1003): an atom with
+ * x < lo or x >= hi in a dimension is sent to itself and kept only inside the
+ * box, so an atom beyond an f face is lost. Counted here (the atoms stay in the
+ * arrays; Thermo::lost_check turns a loss into an error by default). */
+static void count_lost(orc_sys *s) {
+  int64_t lost = 0;
+  for (int i = 0; i < s->n; i++) {
+    int out = 0;
+    for (int d = 0; d < 3; d++) {
+      if (s->p.periodic[d]) continue;
+      const double xd = s->x[3 * i + d];
+      if (!((s->p.shrink_faces >> (2 * d)) & 1) && xd < s->box_lo[d]) out = 1;
+      if (!((s->p.shrink_faces >> (2 * d + 1)) & 1) && xd >= s->box_hi[d]) out = 1;
+    }
+    lost += out;
+  }
+  s->nlost = lost;
+  if (lost && s->first_lost_step < 0) s->first_lost_step = s->ntimestep;
+}
+
+int64_t orc_lost_atoms(const orc_sys *s, int64_t *first_step) {
+  if (first_step) *first_step = s->first_lost_step;
+  return s->nlost;
+}
+
+void orc_get_box(const orc_sys *s, double *boxlo, double *boxhi) {
+  for (int d = 0; d < 3; d++) {
+    boxlo[d] = s->box_lo[d];
+    boxhi[d] = s->box_hi[d];
+  }
 }
 
 /* FixContactHistory::pre_exchange, fix_contact_history.cpp:305-425: copy the
@@ -826,6 +894,8 @@ static void rebuild(orc_sys *s) {
   s->last_build = s->ntimestep; /* Neighbor::build resets ago (neighbor.cpp) */
   carry_out(s);
   pbc(s);
+  reset_box(s);  /* box_change with shrink-wrapped faces (verlet.cpp:322-325) */
+  count_lost(s);
   build(s);
   mesh_rebuild(s);
 }

@@ -84,6 +84,10 @@ typedef struct {
   orc_model plane_model[ORC_MAXPLANE]; /* each fix wall/gran's own model switches */
   int neigh_every;    /* neigh_modify every N (0 is taken as 1) */
   int neigh_no_check; /* neigh_modify check no */
+  /* boundary s / m faces of non-periodic dimensions (bit 2d lower, 2d+1 upper
+   * face of dimension d; shrink_min: the m ones); other non-periodic faces are f */
+  int shrink_faces;
+  int shrink_min;
 } orc_params;
 
 typedef struct orc_sys orc_sys;
@@ -156,6 +160,11 @@ void orc_get_wall_hist(const orc_sys *s, double *hist /* n*nplanes*3 */);
 int64_t orc_get_pair_rolling(const orc_sys *s, int64_t cap, double *rhist);
 void orc_get_wall_rolling(const orc_sys *s, double *rhist);
 int64_t orc_nrebuilds(const orc_sys *s);
+/* atoms beyond an f face at the last neighbor build (lost by Comm::exchange),
+ * and the first timestep whose build lost atoms (-1: none so far) */
+int64_t orc_lost_atoms(const orc_sys *s, int64_t *first_step);
+/* the box after the last build's Domain::reset_box (shrink-wrapped faces) */
+void orc_get_box(const orc_sys *s, double *boxlo, double *boxhi);
 int64_t orc_ntimestep(const orc_sys *s);
 /* count of touching half-list pairs and of touching particle-wall contacts */
 void orc_contact_counts(const orc_sys *s, int64_t *npair_touch, int64_t *nwall_touch);

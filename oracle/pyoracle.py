@@ -50,6 +50,7 @@ class OrcParams(ctypes.Structure):
         ("gravity_first", ctypes.c_int),
         ("plane_model", OrcModel * MAXPLANE),
         ("neigh_every", ctypes.c_int), ("neigh_no_check", ctypes.c_int),
+        ("shrink_faces", ctypes.c_int), ("shrink_min", ctypes.c_int),
     ]
 
 
@@ -96,6 +97,9 @@ def load():
     lib.orc_get_pairs.argtypes = [ctypes.c_void_p, ctypes.c_int64, ip, ip, dp]
     lib.orc_get_pairs.restype = ctypes.c_int64
     lib.orc_get_wall_hist.argtypes = [ctypes.c_void_p, dp]
+    lib.orc_lost_atoms.argtypes = [ctypes.c_void_p, P(ctypes.c_int64)]
+    lib.orc_lost_atoms.restype = ctypes.c_int64
+    lib.orc_get_box.argtypes = [ctypes.c_void_p, dp, dp]
     lib.orc_nrebuilds.argtypes = [ctypes.c_void_p]
     lib.orc_nrebuilds.restype = ctypes.c_int64
     lib.orc_ntimestep.argtypes = [ctypes.c_void_p]
@@ -197,6 +201,8 @@ def make_params(params, walls=()):
     p.gravity_first = int(params.get("gravity_first", 1))
     p.neigh_every = int(params.get("neigh_every", 1))
     p.neigh_no_check = int(not params.get("neigh_check", True))
+    p.shrink_faces = int(params.get("shrink_faces", 0))
+    p.shrink_min = int(params.get("shrink_min", 0))
     return p
 
 
@@ -427,6 +433,18 @@ class OracleSystem:
 
     def nrebuilds(self):
         return self.lib.orc_nrebuilds(self.s)
+
+    def lost_atoms(self):
+        """(atoms beyond an f face at the last build, first timestep whose
+        build lost atoms or -1)"""
+        first = ctypes.c_int64()
+        n = self.lib.orc_lost_atoms(self.s, ctypes.byref(first))
+        return int(n), int(first.value)
+
+    def box(self):
+        lo, hi = np.zeros(3), np.zeros(3)
+        self.lib.orc_get_box(self.s, _dp(lo), _dp(hi))
+        return lo, hi
 
     def contact_counts(self):
         a, b = ctypes.c_int64(), ctypes.c_int64()

@@ -151,6 +151,54 @@ def test_history_carried_across_rebuilds_fp64():
     g.close()
 
 
+STIFF = {"kn": [[4.0e6]], "kt": [[5.0e6]], "gamman": [[2.0e5]], "gammat": [[1.0e5]]}
+
+
+@pytest.mark.parametrize("normal", ["hertz", "hertz/stiffness"])
+def test_contact_distance_factor_band_fp64(normal):
+    """neigh_modify contact_distance_factor 1.1 on an agitated pack with a
+    small skin (builds every few steps): pairs leaving contact pass through
+    the surfacesClose band, where the shear is zeroed and the contact flag
+    follows the normal model's rule (hertz keeps it, hertz/stiffness only for
+    a pair carried over at the last build). Pair sets, flags, histories,
+    build counts and state equal the oracle's after 40 steps."""
+    say(f"[gpu] contact_distance_factor 1.1 band, {normal}")
+    params, mats, P, walls = wl.dense_pack(seed=23, n=2000, rolling=normal == "hertz", speed=0.5)
+    params = dict(params, skin=0.00004, cdf=1.1, normal=normal)
+    if normal == "hertz/stiffness":
+        mats = dict(mats, **STIFF)
+    g, o = build(params, mats, P, walls)
+    g.run(1)
+    o.run(1)
+    compare_pairs(g, o)
+    g.run_continue(39)
+    o.run_continue(39)
+    say(f"  rebuilds gpu={g.stats()['nrebuilds']} oracle={o.nrebuilds()}")
+    assert g.stats()["nrebuilds"] == o.nrebuilds() >= 3
+    n, nflag = compare_pairs(g, o, hist_tol=1e-8)
+    ot, of, oh = o.pairs()
+    # pairs in the band: apart, within 1.1 radsum
+    st = o.state()
+    idx = {int(t): k for k, t in enumerate(P["tag"])}
+    a = np.array([idx[int(t)] for t in ot[:, 0]])
+    b = np.array([idx[int(t)] for t in ot[:, 1]])
+    d = st["x"][a] - st["x"][b]
+    L = np.array(params["boxhi"]) - np.array(params["boxlo"])
+    per = np.array(params["periodic"], dtype=bool)
+    d[:, per] -= np.round(d[:, per] / L[per]) * L[per]
+    rs = P["radius"][a] + P["radius"][b]
+    dist = np.linalg.norm(d, axis=1)
+    band = (dist >= rs) & (dist < 1.1 * rs)
+    say(f"  pairs {n}, flagged {nflag}, in the band {int(band.sum())} "
+        f"(flagged {int(of[band].sum())}, max |shear| there {np.abs(oh[band]).max():.1e})")
+    assert band.sum() > 100
+    assert np.abs(oh[band]).max() == 0.0
+    gs, os_ = g.state(), o.state()
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert_close(k, gs[k], os_[k], 1e-8)
+    g.close()
+
+
 @pytest.mark.parametrize("every,check", [(3, True), (2, False), (5, False)])
 def test_neigh_modify_every_check_fp64(every, check):
     """neigh_modify every N check yes|no: the same build schedule as the
@@ -182,7 +230,7 @@ def test_step_shapes_are_bitwise_equal():
     say("[gpu] step kernel shapes")
     params, mats, P, walls = wl.dense_pack(seed=33, n=3000, rolling=True, speed=0.3)
     ref = None
-    shapes = [(v, f) for v in range(5) for f in (0, 4)] + [(-1, 0)]
+    shapes = [(v, f) for v in range(7) for f in (0, 4)] + [(-1, 0)]
     for v, f in shapes:
         for prec in (pydem.DEM_FP64,):
             g = pydem.GpuSystem(params, mats, P, walls, precision=prec)
@@ -211,6 +259,31 @@ def test_step_shapes_are_bitwise_equal():
         assert np.array_equal(sa[k], sb[k]), k
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("prec", [pydem.DEM_FP64, pydem.DEM_MIXED])
+def test_sibling_models_wave_compaction_is_bitwise(prec):
+    """The sibling contact models' kernel (hooke + no_history + sjkr pairs,
+    a hertz/stiffness + epsd2 floor) in its per-lane and wavefront-compacted
+    forms: bitwise the same state and pair histories after 60 steps."""
+    say("[gpu] sibling models: per-lane vs wave-compacted step kernel")
+    params, mats, P, walls = wl.dense_pack(seed=34, n=2500, rolling=False, speed=0.3)
+    params = dict(params, normal="hooke", tangential="no_history", cohesion="sjkr")
+    mats = dict(mats, characteristic_velocity=1.0, cohesion_energy_density=[[3.0e4]],
+                rolling_friction=[[0.1]], **STIFF)
+    walls = [(2, params["boxlo"][2], 1, {"normal": "hertz/stiffness", "rolling_epsd2": 1})]
+    out = []
+    for v in (0, 5):
+        g = pydem.GpuSystem(params, mats, P, walls, precision=prec)
+        g.set_step_shape(v, 0)
+        g.run(60)
+        out.append((g.state(), g.pairs(), g.wall_history()))
+        g.close()
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    for a, b in zip(out[0][1], out[1][1]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(out[0][2], out[1][2])
 
 
 def test_run_continue_is_bitwise_one_run():
@@ -440,6 +513,31 @@ def test_mesh_floor_many_steps_fp64():
     for k in ("x", "v", "omega", "f", "torque"):
         assert_close(k, gs[k], os_[k], 1e-8)
     _compare_mesh_history(g, o, 1e-8)
+    g.close()
+
+
+def test_mesh_floor_contact_distance_factor_fp64():
+    """A tessellated floor with contact_distance_factor 1.2: particles
+    rising off the floor within 0.2 r keep their slot with zeroed shear
+    (surfacesClose through compute_force), as the oracle's slots do."""
+    say("[gpu] mesh floor, contact_distance_factor 1.2")
+    case = _mesh_floor_pack(seed=43, n=1500)
+    # side faces shrink-wrapped: the compressed pack ejects a few spheres sideways
+    params = dict(case[0], skin=0.00005, cdf=1.2, shrink_faces=0b1111)
+    P = dict(case[2])
+    P["v"] = P["v"] + np.array([0.0, 0.0, 0.05])  # lift off the floor
+    case = (params, case[1], P) + case[3:]
+    g, o = _build_mesh_case(case)
+    g.run(120)
+    o.run(120)
+    assert g.stats()["nrebuilds"] == o.nrebuilds()
+    gs, os_ = g.state(), o.state()
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert_close(k, gs[k], os_[k], 1e-8)
+    nslots = _compare_mesh_history(g, o, 1e-8)
+    ot, oh = o.mesh_history()
+    zero = ((ot[:, :8] >= 0) & (np.abs(oh[:, :8]).max(axis=-1) == 0.0)).sum()
+    say(f"  mesh slots {nslots}, of them with zero shear {int(zero)}")
     g.close()
 
 
