@@ -573,6 +573,101 @@ int64_t dem_sqdem_get_pairs(dem_sqdem_handle *h, int64_t cap, int *ij, int *flag
 int dem_sqdem_get_wall_history(dem_sqdem_handle *h, double *hist);
 int dem_sqdem_get_stats(dem_sqdem_handle *h, dem_sqdem_stats *out);
 
+/* ---- Particle insertion ----
+ * Appends n particles at the next step boundary: they join at the first step
+ * of the next dem_run / dem_run_continue, after that step's nve/sphere initial
+ * integrate of the particles already there and before its neighbor build,
+ * which is forced (FixInsert::pre_exchange, fix_insert.cpp:652-826, on a step
+ * whose next_reneighbor the fix set; ParticleToInsert::insert,
+ * particleToInsert.cpp:99-147, creates the atoms with no contact history).
+ * Arrays as dem_set_particles; tags must be new. Works on a handle that has no
+ * particles yet (dem_set_particles not called): it then starts with these.
+ * Calls before one run accumulate. */
+int dem_insert_particles(dem_handle *h, int64_t n, const double *x, const double *v,
+                         const double *omega, const double *radius, const double *rmass,
+                         const int *type, const int *tag);
+
+/* The largest particle radius the cell grid must cover, beyond the particles
+ * present: PairGran::init_one's "future particles as dynamic" (pair_gran.cpp:
+ * 538-551) takes max_rad of every insertion fix into the neighbor cutoff. */
+int dem_set_max_radius(dem_handle *h, double rmax);
+
+/* fix insert/pack with fix particletemplate/sphere and
+ * fix particledistribution/discrete on a block region: where the particles of
+ * one insertion go (FixInsertPack, fix_insert_pack.cpp:74-500; FixInsert,
+ * fix_insert.cpp:78-972; FixParticledistributionDiscrete::randomize_list,
+ * fix_particledistribution_discrete.cpp:395-469; FixTemplateSphere,
+ * fix_template_sphere.cpp:72-321; Region::generate_random*, volume_mc,
+ * region.cpp:499-701; RanPark, random_park.cpp:54-116), as host code of this
+ * library; the particles then enter a system through dem_insert_particles.
+ * Radius and density are constant per template, as the reference allows
+ * (fix_template_sphere.cpp:180-215). */
+#define DEM_INSERT_MAXTEMPLATES 8
+typedef struct dem_insert_params {
+  /* particletemplate/sphere <seed> atom_type <t> density constant <rho>
+     radius constant <r>, in the order the distribution lists them */
+  int ntemplates;
+  int template_seed[DEM_INSERT_MAXTEMPLATES];
+  int template_type[DEM_INSERT_MAXTEMPLATES];
+  double template_radius[DEM_INSERT_MAXTEMPLATES];
+  double template_density[DEM_INSERT_MAXTEMPLATES];
+  /* particledistribution/discrete[/numberbased] <seed> <n> <id> <weight> ... */
+  int distribution_seed;
+  int distribution_numberbased;
+  double distribution_weight[DEM_INSERT_MAXTEMPLATES];
+  /* insert/pack seed <s> maxattempt <m> overlapcheck yes|no all_in yes|no
+     random_distribute exact|uncorrelated vel constant|uniform|gaussian ...
+     omega constant ... check_dist_from_subdomain_border yes|no ntry_mc <n>,
+     exactly one of volumefraction_region / particles_in_region /
+     mass_in_region, region <block> */
+  int seed;
+  int maxattempt;
+  int overlapcheck;
+  int all_in;
+  int exact_number;
+  int vel_style; /* 0 constant, 1 uniform (v +- dv), 2 gaussian (v, sigma dv) */
+  double v_insert[3], v_fluct[3], omega[3];
+  int check_dist_from_subdomain_border;
+  int ntry_mc;
+  double volumefraction_region;
+  int particles_in_region;
+  double mass_in_region;
+  double region_lo[3], region_hi[3];
+  /* the simulation box when the fix is set up (its first run) */
+  double boxlo[3], boxhi[3];
+  /* boundary s or m in some dimension (Domain::box_change, domain.cpp:192-
+     208): FixInsertPack::insertion_fraction re-measures the region volume
+     with the current box at every insertion (fix_insert_pack.cpp:356-362) */
+  int box_change;
+  /* warn_region yes (default) and some face of the box is f: a region whose
+     bounding box reaches outside the box is an error
+     (fix_insert_pack.cpp:252-261) */
+  int warn_region;
+  int fixed_boundary;
+} dem_insert_params;
+
+typedef struct dem_inserter dem_inserter;
+
+/* Builds the fix (constructors + FixInsertPack::calc_insertion_properties:
+ * region random stream reset to seed + 12, region volume by Monte Carlo).
+ * Errors are the reference's error->all cases of these steps. */
+int dem_inserter_create(const dem_insert_params *params, dem_inserter **out);
+int dem_inserter_destroy(dem_inserter *ins);
+const char *dem_inserter_last_error(const dem_inserter *ins);
+/* One insertion with the current particles (x n*3, radius, rmass, any order)
+ * and the current box: writes up to cap particles in insertion order
+ * (position, velocity, angular velocity n*3 each; radius, density, mass;
+ * atom type). *ninserted receives how many the fix placed, *nrequested how
+ * many it asked for; more than cap placed is DEM_ERR_OVERFLOW. */
+int dem_inserter_step(dem_inserter *ins, int64_t n, const double *x, const double *radius,
+                      const double *rmass, const double *boxlo, const double *boxhi, int64_t cap,
+                      double *x_out, double *v_out, double *omega_out, double *radius_out,
+                      double *density_out, double *rmass_out, int *type_out, int64_t *ninserted,
+                      int64_t *nrequested);
+/* region volume the fix measured and the distribution's expectations */
+int dem_inserter_info(const dem_inserter *ins, double *region_volume, double *volume_expect,
+                      double *mass_expect, double *max_radius);
+
 /* Runs an input script with the reference's command subset for this path
  * (Input::file, input.cpp:183, and Input::execute_command, input.cpp:560-650).
  * log_path may be NULL (log to stdout). Returns a dem_status. */

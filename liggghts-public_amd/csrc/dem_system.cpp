@@ -127,6 +127,12 @@ struct dem_handle {
   bool can_continue = false;  // fout holds the last force pass in the current order
   double *xhold = nullptr, *fout = nullptr;
   double rmax = 0.0;
+  double rmax_future = 0.0;  // dem_set_max_radius: particles still to come
+  // particles waiting to join at the next run's first step (dem_insert_particles)
+  struct Pending {
+    std::vector<double> x, v, w, r, m;
+    std::vector<int> type, tag;
+  } pend;
 
   // cell grid
   int nc[3] = {0, 0, 0};
@@ -198,6 +204,14 @@ int dalloc(dem_handle *h, void **ptr, size_t bytes) {
 void clear_graphs(dem_handle *h) {
   for (auto &g : h->graphs) hipGraphExecDestroy(g.exec);
   h->graphs.clear();
+}
+
+// frees one allocation made by dalloc
+void dfree(dem_handle *h, void *p) {
+  if (!p) return;
+  auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
+  if (it != h->allocs.end()) h->allocs.erase(it);
+  hipFree(p);
 }
 
 
@@ -324,7 +338,9 @@ uint32_t spread_bits(uint32_t v) {
 // covers every pair within (r_i + r_j) * cdf + skin. Cells are ranked in
 // Morton order; particles are counting-sorted by that rank at each rebuild.
 int setup_grid(dem_handle *h) {
-  h->cutneighmax = (h->rmax + h->rmax) * h->p.cdf + h->p.skin;  // pair_gran.cpp:623-635
+  // pair_gran.cpp:623-635, with future particles counted as dynamic (:538-551)
+  const double rm = std::max(h->rmax, h->rmax_future);
+  h->cutneighmax = (rm + rm) * h->p.cdf + h->p.skin;
   long long total = 1;
   for (int d = 0; d < 3; d++) {
     const double L = h->p.boxhi[d] - h->p.boxlo[d];
@@ -817,6 +833,14 @@ int check_ready(dem_handle *h) {
 }
 
 int do_setup(dem_handle *h) {
+  if (h && h->n == 0 && !h->pend.tag.empty()) {
+    // Verlet::setup of a system whose first particles are still to be
+    // inserted: nothing to build or compute yet
+    if (!h->mat_set) return fail(h, DEM_ERR_STATE, "materials not set (fix property/global)");
+    h->setup_done = true;
+    h->can_continue = true;
+    return DEM_OK;
+  }
   int rc = check_ready(h);
   if (rc) return rc;
   HIPCHK(h, hipSetDevice(h->p.device));
@@ -827,6 +851,132 @@ int do_setup(dem_handle *h) {
   if ((rc = force_pass(h, 0))) return rc;  // pair_gran.cpp:665-667: shearupdate 0 in setup
   h->setup_done = true;
   h->can_continue = true;
+  return DEM_OK;
+}
+
+// Re-allocates every per-particle array with room for newcap particles. The
+// arrays are column-major with the capacity as column stride; the first n
+// entries of every column are kept, new memory is zero (mesh slots: empty).
+int grow_particles(dem_handle *h, int newcap) {
+  const int oc = h->cap;
+  const int64_t n = h->n;
+  const int nw = (int)std::max<size_t>(1, h->walls.size());
+  const int nm = (int)std::max<size_t>(1, h->meshes.size());
+  const size_t osz = h->mixed ? sizeof(float4) : sizeof(double4);
+  struct Col {
+    void **p;
+    size_t elem;
+    int cols;
+    int fill;   // byte value of new memory
+    bool keep;  // copy the first n entries of every column
+  };
+  std::vector<Col> cols;
+  for (int b = 0; b < 2; b++) {
+    cols.push_back({(void **)&h->st[b].pos, sizeof(double4), 1, 0, true});
+    cols.push_back({(void **)&h->st[b].vel, sizeof(double4), 1, 0, true});
+    cols.push_back({&h->st[b].omg, osz, 1, 0, true});
+    cols.push_back({(void **)&h->aux[b].tag, sizeof(int), 1, 0, true});
+    cols.push_back({(void **)&h->aux[b].orig, sizeof(int), 1, 0, true});
+    cols.push_back({(void **)&h->aux[b].meta, sizeof(int), 1, 0, true});
+    cols.push_back({&h->aux[b].whist, h->fsz, h->hw * nw, 0, true});
+    cols.push_back({(void **)&h->aux[b].mslot, sizeof(int), nm * dem::kMeshSlots, 0xff, true});
+    cols.push_back({&h->aux[b].mhist, h->fsz, 3 * nm * dem::kMeshSlots, 0, true});
+    cols.push_back({(void **)&h->lst[b].nbr, sizeof(uint32_t), h->W, 0, true});
+    cols.push_back({&h->lst[b].hist, h->fsz, h->hw * h->W, 0, true});
+  }
+  cols.push_back({(void **)&h->xhold, sizeof(double), 3, 0, true});
+  cols.push_back({(void **)&h->fout, sizeof(double), 6, 0, true});
+  cols.push_back({(void **)&h->key, sizeof(int), 1, 0, false});
+  cols.push_back({(void **)&h->perm, sizeof(int), 1, 0, false});
+  cols.push_back({(void **)&h->d_cand, sizeof(int), dem::kMaxMeshCand, 0, true});
+  cols.push_back({(void **)&h->d_ncand, sizeof(int), 1, 0, true});
+  int rc;
+  for (Col &c : cols) {
+    void *np = nullptr;
+    const size_t bytes = c.elem * (size_t)c.cols * (size_t)newcap;
+    if ((rc = dalloc(h, &np, bytes))) return rc;
+    HIPCHK(h, hipMemsetAsync(np, c.fill, bytes, h->stream));
+    if (c.keep && n > 0 && *c.p)
+      HIPCHK(h, hipMemcpy2DAsync(np, c.elem * newcap, *c.p, c.elem * oc, c.elem * n, c.cols,
+                                 hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    dfree(h, *c.p);
+    *c.p = np;
+  }
+  h->cap = newcap;
+  clear_graphs(h);
+  return DEM_OK;
+}
+
+// Particles of dem_insert_particles join the system: appended behind the
+// present ones in the current buffers, with no list, wall or mesh history.
+// The caller forces a neighbor build before their first force pass.
+int append_pending(dem_handle *h) {
+  dem_handle::Pending q;
+  std::swap(q, h->pend);
+  const int64_t m = (int64_t)q.tag.size();
+  if (m == 0) return DEM_OK;
+  if (h->n == 0)  // the handle's first particles
+    return dem_set_particles(h, m, q.x.data(), q.v.data(), q.w.data(), q.r.data(), q.m.data(),
+                             q.type.data(), q.tag.data());
+  const int64_t n0 = h->n, n1 = n0 + m;
+  if (n1 > 0x7fffffffLL / 2) return fail(h, DEM_ERR_ARG, "too many particles");
+  int rc;
+  if (n1 > h->cap && (rc = grow_particles(h, (int)std::max<int64_t>(n1, (int64_t)h->cap + h->cap / 2))))
+    return rc;
+  std::vector<double4> pos(m), vel(m);
+  std::vector<int> orig(m), zero(m, 0);
+  for (int64_t k = 0; k < m; k++) {
+    pos[k] = make_double4(q.x[3 * k], q.x[3 * k + 1], q.x[3 * k + 2], q.r[k]);
+    vel[k] = make_double4(q.v[3 * k], q.v[3 * k + 1], q.v[3 * k + 2], q.m[k]);
+    orig[k] = (int)(n0 + k);
+  }
+  const StateBuf &st = h->st[h->cur_st];
+  HIPCHK(h, hipMemcpyAsync(st.pos + n0, pos.data(), sizeof(double4) * m, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(st.vel + n0, vel.data(), sizeof(double4) * m, hipMemcpyHostToDevice, h->stream));
+  std::vector<float4> wf;
+  std::vector<double4> wd;
+  if (h->mixed) {
+    wf.resize(m);
+    for (int64_t k = 0; k < m; k++)
+      wf[k] = make_float4((float)q.w[3 * k], (float)q.w[3 * k + 1], (float)q.w[3 * k + 2], (float)q.type[k]);
+    HIPCHK(h, hipMemcpyAsync((float4 *)st.omg + n0, wf.data(), sizeof(float4) * m, hipMemcpyHostToDevice,
+                             h->stream));
+  } else {
+    wd.resize(m);
+    for (int64_t k = 0; k < m; k++)
+      wd[k] = make_double4(q.w[3 * k], q.w[3 * k + 1], q.w[3 * k + 2], (double)q.type[k]);
+    HIPCHK(h, hipMemcpyAsync((double4 *)st.omg + n0, wd.data(), sizeof(double4) * m, hipMemcpyHostToDevice,
+                             h->stream));
+  }
+  const AuxBuf &ax = h->aux[h->cur_aux];
+  HIPCHK(h, hipMemcpyAsync(ax.tag + n0, q.tag.data(), sizeof(int) * m, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(ax.orig + n0, orig.data(), sizeof(int) * m, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(ax.meta + n0, zero.data(), sizeof(int) * m, hipMemcpyHostToDevice, h->stream));
+  const int nm = (int)std::max<size_t>(1, h->meshes.size());
+  for (int b = 0; b < 2; b++)
+    HIPCHK(h, hipMemset2DAsync(h->aux[b].mslot + n0, sizeof(int) * (size_t)h->cap, 0xff, sizeof(int) * m,
+                               (size_t)nm * dem::kMeshSlots, h->stream));
+  HIPCHK(h, hipMemsetAsync(h->d_ncand + n0, 0, sizeof(int) * m, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->n = n1;
+  clear_graphs(h);
+  // a larger particle may need larger cells
+  for (int64_t k = 0; k < m; k++) h->rmax = std::max(h->rmax, q.r[k]);
+  const double rm = std::max(h->rmax, h->rmax_future);
+  if ((rm + rm) * h->p.cdf + h->p.skin != h->cutneighmax) {
+    for (int **p : {&h->cell_rank, &h->cell_count, &h->cell_start, &h->scan_tmp}) {
+      dfree(h, *p);
+      *p = nullptr;
+    }
+    if ((rc = setup_grid(h))) return rc;
+    if (h->ntri > 0) {
+      dfree(h, h->d_tcell_count);
+      dfree(h, h->d_tcell_start);
+      if ((rc = dalloc(h, (void **)&h->d_tcell_count, sizeof(int) * h->ncell))) return rc;
+      if ((rc = dalloc(h, (void **)&h->d_tcell_start, sizeof(int) * (h->ncell + 1)))) return rc;
+    }
+  }
   return DEM_OK;
 }
 
@@ -1414,6 +1564,60 @@ int dem_set_particles(dem_handle *h, int64_t n, const double *x, const double *v
   return DEM_OK;
 }
 
+int dem_insert_particles(dem_handle *h, int64_t n, const double *x, const double *v,
+                         const double *omega, const double *radius, const double *rmass,
+                         const int *type, const int *tag) {
+  if (!h) return DEM_ERR_ARG;
+  if (n < 0 || (n > 0 && (!x || !v || !omega || !radius || !rmass || !type || !tag)))
+    return fail(h, DEM_ERR_ARG, "bad particle arrays");
+  if (!h->mat_set) return fail(h, DEM_ERR_STATE, "set materials before particles");
+  if (h->ninst > 1) return fail(h, DEM_ERR_STATE, "insertion into batched instances is not available");
+  for (int64_t i = 0; i < n; i++) {
+    if (!(radius[i] > 0.0)) return fail(h, DEM_ERR_ARG, "Fix nve/sphere requires extended particles");
+    if (!(rmass[i] > 0.0)) return fail(h, DEM_ERR_ARG, "particle mass must be > 0");
+    if (type[i] < 1 || type[i] > h->ntypes) return fail(h, DEM_ERR_ARG, "Invalid atom type");
+    if (tag[i] <= 0) return fail(h, DEM_ERR_ARG, "Invalid atom ID");
+    for (int d = 0; d < 3; d++)
+      if (!std::isfinite(x[3 * i + d]) || !std::isfinite(v[3 * i + d]) || !std::isfinite(omega[3 * i + d]))
+        return fail(h, DEM_ERR_ARG, "non-finite particle state");
+  }
+  auto &q = h->pend;
+  q.x.insert(q.x.end(), x, x + 3 * n);
+  q.v.insert(q.v.end(), v, v + 3 * n);
+  q.w.insert(q.w.end(), omega, omega + 3 * n);
+  q.r.insert(q.r.end(), radius, radius + n);
+  q.m.insert(q.m.end(), rmass, rmass + n);
+  q.type.insert(q.type.end(), type, type + n);
+  q.tag.insert(q.tag.end(), tag, tag + n);
+  return DEM_OK;
+}
+
+int dem_set_max_radius(dem_handle *h, double rmax) {
+  if (!h) return DEM_ERR_ARG;
+  if (!(rmax >= 0.0) || !std::isfinite(rmax)) return fail(h, DEM_ERR_ARG, "bad radius");
+  h->rmax_future = rmax;
+  if (h->n > 0) {
+    // the cells follow at the next build
+    const double rm = std::max(h->rmax, h->rmax_future);
+    if ((rm + rm) * h->p.cdf + h->p.skin != h->cutneighmax) {
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      for (int **p : {&h->cell_rank, &h->cell_count, &h->cell_start, &h->scan_tmp}) {
+        dfree(h, *p);
+        *p = nullptr;
+      }
+      int rc = setup_grid(h);
+      if (rc) return rc;
+      if (h->ntri > 0) {
+        dfree(h, h->d_tcell_count);
+        dfree(h, h->d_tcell_start);
+        if ((rc = dalloc(h, (void **)&h->d_tcell_count, sizeof(int) * h->ncell))) return rc;
+        if ((rc = dalloc(h, (void **)&h->d_tcell_start, sizeof(int) * (h->ncell + 1)))) return rc;
+      }
+    }
+  }
+  return DEM_OK;
+}
+
 int dem_setup(dem_handle *h) {
   if (!h) return DEM_ERR_ARG;
   return do_setup(h);
@@ -1557,11 +1761,22 @@ int run_steps(dem_handle *h, int64_t nsteps) {
 int run_steps_piece(dem_handle *h, int64_t nsteps) {
   int rc;
   if (nsteps == 0) return DEM_OK;
+  // particles of dem_insert_particles join at this run's first step
+  const bool joining = !h->pend.tag.empty();
+  if (h->n == 0) {
+    if (any_moving_mesh(h)) return fail(h, DEM_ERR_STATE, "moving meshes need particles on this path");
+    if (!joining) {  // nothing to integrate: the steps pass
+      h->ntimestep += nsteps;
+      h->can_continue = true;
+      return DEM_OK;
+    }
+  }
   h->setup_done = false;
   h->can_continue = false;
   // a rebuild a moving mesh asked for at the first step of this run
-  // (FixMesh::pre_force set next_reneighbor on the last step of the previous run)
-  HIPCHK(h, dem::launch_set_stop(h->ctrl, h->pending_mesh_rebuild ? 1 : kNever, h->stream,
+  // (FixMesh::pre_force set next_reneighbor on the last step of the previous
+  // run), or the build an insertion forces (fix_insert.cpp:311-319)
+  HIPCHK(h, dem::launch_set_stop(h->ctrl, (h->pending_mesh_rebuild || joining) ? 1 : kNever, h->stream,
                                       h->d_inst_stats, 2 * h->ninst));
   h->pending_mesh_rebuild = false;
   const bool moving = any_moving_mesh(h);
@@ -1569,10 +1784,17 @@ int run_steps_piece(dem_handle *h, int64_t nsteps) {
   const bool timed = has_timed_movers(h);
   double t_end = h->mesh_time;
   if (timed && (rc = upload_mesh_motion(h, nsteps, &t_end))) return rc;
-  {
+  if (h->n > 0) {
     dem::StepArgs a = step_args(h, h->cur_st, h->tri_cur);
     a.ago_next = ts0 + 1 - h->last_build;
     HIPCHK(h, dem::launch_initial(a, h->stream));
+  }
+  // the inserted particles: after the initial integrate of the others, before
+  // the forced build (Verlet::run: initial_integrate, then pre_exchange on a
+  // build step, verlet.cpp:283-320)
+  if (joining) {
+    if ((rc = append_pending(h))) return rc;
+    HIPCHK(h, dem::launch_set_stop(h->ctrl, 1, h->stream, h->d_inst_stats, 2 * h->ninst));
   }
   // the first step's mesh rotation, in place; every later step's rotation is
   // done by the trailing blocks of the step before

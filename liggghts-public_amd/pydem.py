@@ -37,9 +37,66 @@ EXPORTED = [
     "dem_sqdem_set_particles", "dem_sqdem_set_warm_start", "dem_sqdem_setup", "dem_sqdem_run",
     "dem_sqdem_get_state",
     "dem_sqdem_get_props", "dem_sqdem_get_pairs", "dem_sqdem_get_wall_history", "dem_sqdem_get_stats",
+    "dem_insert_particles", "dem_set_max_radius", "dem_inserter_create", "dem_inserter_destroy",
+    "dem_inserter_last_error", "dem_inserter_step", "dem_inserter_info",
 ]
 
 DEM_SQ_FRESH, DEM_SQ_GIVEN, DEM_SQ_NETWORK = 0, 1, 2
+DEM_INSERT_MAXTEMPLATES = 8
+
+
+class DemInsertParams(ctypes.Structure):
+    _fields_ = [
+        ("ntemplates", ctypes.c_int),
+        ("template_seed", ctypes.c_int * DEM_INSERT_MAXTEMPLATES),
+        ("template_type", ctypes.c_int * DEM_INSERT_MAXTEMPLATES),
+        ("template_radius", ctypes.c_double * DEM_INSERT_MAXTEMPLATES),
+        ("template_density", ctypes.c_double * DEM_INSERT_MAXTEMPLATES),
+        ("distribution_seed", ctypes.c_int),
+        ("distribution_numberbased", ctypes.c_int),
+        ("distribution_weight", ctypes.c_double * DEM_INSERT_MAXTEMPLATES),
+        ("seed", ctypes.c_int), ("maxattempt", ctypes.c_int), ("overlapcheck", ctypes.c_int),
+        ("all_in", ctypes.c_int), ("exact_number", ctypes.c_int), ("vel_style", ctypes.c_int),
+        ("v_insert", ctypes.c_double * 3), ("v_fluct", ctypes.c_double * 3), ("omega", ctypes.c_double * 3),
+        ("check_dist_from_subdomain_border", ctypes.c_int), ("ntry_mc", ctypes.c_int),
+        ("volumefraction_region", ctypes.c_double), ("particles_in_region", ctypes.c_int),
+        ("mass_in_region", ctypes.c_double), ("region_lo", ctypes.c_double * 3),
+        ("region_hi", ctypes.c_double * 3), ("boxlo", ctypes.c_double * 3), ("boxhi", ctypes.c_double * 3),
+        ("box_change", ctypes.c_int), ("warn_region", ctypes.c_int), ("fixed_boundary", ctypes.c_int),
+    ]
+
+
+INSERT_DEFAULTS = dict(maxattempt=50, overlapcheck=1, all_in=0, exact_number=1, vel_style=0,
+                       v_insert=(0.0, 0.0, 0.0), v_fluct=(0.0, 0.0, 0.0), omega=(0.0, 0.0, 0.0),
+                       check_dist_from_subdomain_border=1, ntry_mc=100000, volumefraction_region=0.0,
+                       particles_in_region=0, mass_in_region=0.0, box_change=0, dist_numberbased=0,
+                       warn_region=1, fixed_boundary=0)
+
+
+def insert_params(**kw):
+    """DemInsertParams from keyword arguments: templates=[(seed, type,
+    radius, density)], weights, dist_seed, seed, region=(lo, hi), box=(lo, hi)
+    and the insert/pack options (INSERT_DEFAULTS)."""
+    d = dict(INSERT_DEFAULTS, **kw)
+    p = DemInsertParams()
+    p.ntemplates = len(d["templates"])
+    for i, (sd, ty, r, rho) in enumerate(d["templates"]):
+        p.template_seed[i], p.template_type[i] = sd, ty
+        p.template_radius[i], p.template_density[i] = r, rho
+        p.distribution_weight[i] = d["weights"][i]
+    p.distribution_seed = d["dist_seed"]
+    p.distribution_numberbased = int(d["dist_numberbased"])
+    for k in ("seed", "maxattempt", "overlapcheck", "all_in", "exact_number", "vel_style",
+              "check_dist_from_subdomain_border", "ntry_mc", "particles_in_region", "box_change",
+              "warn_region", "fixed_boundary"):
+        setattr(p, k, int(d[k]))
+    p.volumefraction_region = float(d["volumefraction_region"])
+    p.mass_in_region = float(d["mass_in_region"])
+    for dim in range(3):
+        p.v_insert[dim], p.v_fluct[dim], p.omega[dim] = d["v_insert"][dim], d["v_fluct"][dim], d["omega"][dim]
+        p.region_lo[dim], p.region_hi[dim] = d["region"][0][dim], d["region"][1][dim]
+        p.boxlo[dim], p.boxhi[dim] = d["box"][0][dim], d["box"][1][dim]
+    return p
 SQ_HIST_DOUBLES = 7  # state, first OBB inequality, contact point[3], alpha_i, alpha_j
 SQ_NUM_PARAMS = 5571
 
@@ -235,6 +292,16 @@ def load_library(path=LIB_PATH):
     lib.dem_sqdem_get_pairs.restype = ctypes.c_int64
     lib.dem_sqdem_get_wall_history.argtypes = [h, dp]
     lib.dem_sqdem_get_stats.argtypes = [h, P(SqDemStats)]
+    lib.dem_insert_particles.argtypes = [h, ctypes.c_int64, dp, dp, dp, dp, dp, ip, ip]
+    lib.dem_set_max_radius.argtypes = [h, ctypes.c_double]
+    lib.dem_inserter_create.argtypes = [P(DemInsertParams), P(ctypes.c_void_p)]
+    lib.dem_inserter_destroy.argtypes = [h]
+    lib.dem_inserter_last_error.argtypes = [h]
+    lib.dem_inserter_last_error.restype = ctypes.c_char_p
+    i64p = P(ctypes.c_int64)
+    lib.dem_inserter_step.argtypes = [h, ctypes.c_int64, dp, dp, dp, dp, dp, ctypes.c_int64,
+                                      dp, dp, dp, dp, dp, dp, ip, i64p, i64p]
+    lib.dem_inserter_info.argtypes = [h, dp, dp, dp, dp]
     _lib = lib
     return lib
 
@@ -389,7 +456,30 @@ class GpuSystem:
         if rc != 0:
             raise DemError(f"libdem_amd error {rc}: {self.lib.dem_last_error(self.h).decode()}")
 
+    def insert(self, P):
+        """dem_insert_particles: the particles join at the next run's first step."""
+        m = int(P["x"].shape[0])
+        arr = {k: np.ascontiguousarray(P[k], dtype=np.float64)
+               for k in ("x", "v", "omega", "radius", "rmass")}
+        ty = np.ascontiguousarray(P["type"], dtype=np.int32)
+        tg = np.ascontiguousarray(P["tag"], dtype=np.int32)
+        self._check(self.lib.dem_insert_particles(
+            self.h, m, _dp(arr["x"]), _dp(arr["v"]), _dp(arr["omega"]),
+            _dp(arr["radius"]), _dp(arr["rmass"]), _ip(ty), _ip(tg)))
+        self._pending += m
+
+    def set_max_radius(self, r):
+        self._check(self.lib.dem_set_max_radius(self.h, float(r)))
+
+    def _joined(self):
+        self.n += self._pending
+        self._pending = 0
+
     def set_particles(self, P):
+        self._pending = 0
+        if P is None:  # an empty system, filled by insert()
+            self.n = 0
+            return
         self.n = int(P["x"].shape[0])
         arr = {k: np.ascontiguousarray(P[k], dtype=np.float64)
                for k in ("x", "v", "omega", "radius", "rmass")}
@@ -404,9 +494,13 @@ class GpuSystem:
 
     def run(self, nsteps):
         self._check(self.lib.dem_run(self.h, int(nsteps)))
+        if nsteps > 0:
+            self._joined()
 
     def run_continue(self, nsteps):
         self._check(self.lib.dem_run_continue(self.h, int(nsteps)))
+        if nsteps > 0:
+            self._joined()
 
     def compute_forces(self, shearupdate):
         self._check(self.lib.dem_compute_forces(self.h, int(shearupdate)))
@@ -803,3 +897,52 @@ def restart_particles(path):
                                  _dp(P["omega"]), _dp(P["radius"]), _dp(P["rmass"])):
         raise DemError(lib.dem_last_error(None).decode())
     return {"n": N, "ntypes": nt.value, "boxlo": lo, "boxhi": hi, "periodic": per, "ntimestep": ts.value}, P
+
+
+class Inserter:
+    """fix insert/pack + particletemplate/sphere + particledistribution/discrete
+    (dem_inserter_*)."""
+
+    def __init__(self, **kw):
+        self.lib = load_library()
+        self.p = insert_params(**kw)
+        self.h = ctypes.c_void_p()
+        rc = self.lib.dem_inserter_create(ctypes.byref(self.p), ctypes.byref(self.h))
+        if rc != 0:
+            raise DemError(f"dem_inserter_create failed ({rc}): {self.lib.dem_inserter_last_error(None).decode()}")
+
+    def info(self):
+        v = [ctypes.c_double() for _ in range(4)]
+        self.lib.dem_inserter_info(self.h, *[ctypes.byref(a) for a in v])
+        return dict(zip(("region_volume", "volume_expect", "mass_expect", "max_radius"), (a.value for a in v)))
+
+    def step(self, x=None, radius=None, rmass=None, box=None, cap=200000):
+        """One insertion; returns (dict of the inserted particles, requested)."""
+        x = np.zeros((0, 3)) if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        radius = np.zeros(0) if radius is None else np.ascontiguousarray(radius, dtype=np.float64)
+        rmass = np.zeros(0) if rmass is None else np.ascontiguousarray(rmass, dtype=np.float64)
+        blo = np.ascontiguousarray(box[0] if box else self.p.boxlo, dtype=np.float64)
+        bhi = np.ascontiguousarray(box[1] if box else self.p.boxhi, dtype=np.float64)
+        out = {k: np.zeros((cap, 3)) for k in ("x", "v", "omega")}
+        out.update({k: np.zeros(cap) for k in ("radius", "density", "rmass")})
+        out["type"] = np.zeros(cap, dtype=np.int32)
+        nins, nreq = ctypes.c_int64(), ctypes.c_int64()
+        rc = self.lib.dem_inserter_step(self.h, len(x), _dp(x), _dp(radius), _dp(rmass), _dp(blo), _dp(bhi),
+                                        cap, _dp(out["x"]), _dp(out["v"]), _dp(out["omega"]),
+                                        _dp(out["radius"]), _dp(out["density"]), _dp(out["rmass"]),
+                                        _ip(out["type"]), ctypes.byref(nins), ctypes.byref(nreq))
+        if rc != 0:
+            raise DemError(f"dem_inserter_step failed ({rc}): {self.lib.dem_inserter_last_error(self.h).decode()}")
+        k = nins.value
+        return {kk: vv[:k] for kk, vv in out.items()}, nreq.value
+
+    def close(self):
+        if self.h:
+            self.lib.dem_inserter_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

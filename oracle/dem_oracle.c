@@ -555,6 +555,11 @@ struct orc_sys {
   int64_t nlost;            /* atoms beyond an f face at the last build */
   int64_t first_lost_step;  /* first timestep whose build lost atoms, -1 if none */
   double box_lo[3], box_hi[3]; /* box after the last build's reset_box */
+  /* particles waiting to be inserted at the next step (orc_insert) */
+  int npend;
+  double *pend; /* 11 values each: x[3] v[3] omega[3] radius rmass */
+  int *pend_it; /* type, tag */
+  double rmax_future; /* orc_set_max_radius */
 };
 
 static void *xcalloc(size_t n, size_t sz) {
@@ -606,6 +611,95 @@ orc_sys *orc_create(const orc_params *p, const orc_materials *m, int n, const do
   return s;
 }
 
+/* PairGran::init_one takes the insertion fixes' max_rad as dynamic radii
+ * (pair_gran.cpp:538-551): cutneighmax covers particles still to come */
+void orc_set_max_radius(orc_sys *s, double rmax_future) {
+  double rmax = rmax_future;
+  for (int i = 0; i < s->n; i++)
+    if (s->r[i] > rmax) rmax = s->r[i];
+  s->rmax_future = rmax_future;
+  s->cutneighmax = (rmax + rmax) * s->p.cdf + s->p.skin;
+}
+
+void orc_insert(orc_sys *s, int n, const double *x, const double *v, const double *omega,
+                const double *radius, const double *rmass, const int *type, const int *tag) {
+  s->pend = (double *)realloc(s->pend, 11 * sizeof(double) * (size_t)(s->npend + n + 1));
+  s->pend_it = (int *)realloc(s->pend_it, 2 * sizeof(int) * (size_t)(s->npend + n + 1));
+  if (!s->pend || !s->pend_it) abort();
+  for (int k = 0; k < n; k++) {
+    double *q = s->pend + 11 * (size_t)(s->npend + k);
+    for (int d = 0; d < 3; d++) {
+      q[d] = x[3 * k + d];
+      q[3 + d] = v[3 * k + d];
+      q[6 + d] = omega[3 * k + d];
+    }
+    q[9] = radius[k];
+    q[10] = rmass[k];
+    s->pend_it[2 * (s->npend + k)] = type[k];
+    s->pend_it[2 * (s->npend + k) + 1] = tag[k];
+  }
+  s->npend += n;
+}
+
+static void *grow_zero(void *p, size_t old_bytes, size_t new_bytes) {
+  unsigned char *q = (unsigned char *)realloc(p, new_bytes ? new_bytes : 1);
+  if (!q) abort();
+  if (new_bytes > old_bytes) memset(q + old_bytes, 0, new_bytes - old_bytes);
+  return q;
+}
+
+/* ParticleToInsert::insert (particleToInsert.cpp:99-147): the queued atoms are
+ * created behind the present ones with no contact history */
+static void insert_pending(orc_sys *s) {
+  const size_t n0 = (size_t)s->n, n1 = n0 + (size_t)s->npend, D = sizeof(double);
+  const size_t np = (size_t)(s->p.nplanes > 0 ? s->p.nplanes : 1);
+  s->x = (double *)grow_zero(s->x, 3 * D * n0, 3 * D * n1);
+  s->v = (double *)grow_zero(s->v, 3 * D * n0, 3 * D * n1);
+  s->w = (double *)grow_zero(s->w, 3 * D * n0, 3 * D * n1);
+  s->f = (double *)grow_zero(s->f, 3 * D * n0, 3 * D * n1);
+  s->t = (double *)grow_zero(s->t, 3 * D * n0, 3 * D * n1);
+  s->xhold = (double *)grow_zero(s->xhold, 3 * D * n0, 3 * D * n1);
+  s->r = (double *)grow_zero(s->r, D * n0, D * n1);
+  s->rm = (double *)grow_zero(s->rm, D * n0, D * n1);
+  s->type = (int *)grow_zero(s->type, sizeof(int) * n0, sizeof(int) * n1);
+  s->tag = (int *)grow_zero(s->tag, sizeof(int) * n0, sizeof(int) * n1);
+  s->first = (int64_t *)grow_zero(s->first, sizeof(int64_t) * (n0 + 1), sizeof(int64_t) * (n1 + 1));
+  for (size_t i = n0 + 1; i <= n1; i++) s->first[i] = s->first[n0]; /* empty rows */
+  s->npartner = (int *)grow_zero(s->npartner, sizeof(int) * n0, sizeof(int) * n1);
+  s->poff = (int64_t *)grow_zero(s->poff, sizeof(int64_t) * (n0 + 1), sizeof(int64_t) * (n1 + 1));
+  s->whist = (double *)grow_zero(s->whist, 3 * D * np * n0, 3 * D * np * n1);
+  s->wrhist = (double *)grow_zero(s->wrhist, 3 * D * np * n0, 3 * D * np * n1);
+  for (int k = 0; k < s->nmesh; k++) {
+    orc_mesh *m = &s->mesh[k];
+    m->ncand = (int *)grow_zero(m->ncand, sizeof(int) * n0, sizeof(int) * n1);
+    m->cand = (int *)grow_zero(m->cand, sizeof(int) * MESH_MAXCAND * n0, sizeof(int) * MESH_MAXCAND * n1);
+    m->npartner = (int *)grow_zero(m->npartner, sizeof(int) * n0, sizeof(int) * n1);
+    m->ptri = (int *)grow_zero(m->ptri, sizeof(int) * MESH_MAXSLOT * n0, sizeof(int) * MESH_MAXSLOT * n1);
+    for (size_t q = MESH_MAXSLOT * n0; q < MESH_MAXSLOT * n1; q++) m->ptri[q] = -1;
+    m->keep = (unsigned char *)grow_zero(m->keep, MESH_MAXSLOT * n0, MESH_MAXSLOT * n1);
+    m->phist = (double *)grow_zero(m->phist, 3 * D * MESH_MAXSLOT * n0, 3 * D * MESH_MAXSLOT * n1);
+  }
+  double rmax = s->rmax_future;
+  for (size_t k = 0; k < (size_t)s->npend; k++) {
+    const double *q = s->pend + 11 * k;
+    const size_t i = n0 + k;
+    for (int d = 0; d < 3; d++) {
+      s->x[3 * i + d] = q[d];
+      s->v[3 * i + d] = q[3 + d];
+      s->w[3 * i + d] = q[6 + d];
+    }
+    s->r[i] = q[9];
+    s->rm[i] = q[10];
+    s->type[i] = s->pend_it[2 * k];
+    s->tag[i] = s->pend_it[2 * k + 1];
+  }
+  s->n = (int)n1;
+  s->npend = 0;
+  for (size_t i = 0; i < n1; i++)
+    if (s->r[i] > rmax) rmax = s->r[i];
+  s->cutneighmax = (rmax + rmax) * s->p.cdf + s->p.skin;
+}
+
 static void mesh_free(orc_mesh *m);
 static void mesh_rebuild(orc_sys *s);
 static void mesh_move(orc_sys *s);
@@ -637,6 +731,8 @@ void orc_destroy(orc_sys *s) {
   free(s->prhist);
   free(s->whist);
   free(s->wrhist);
+  free(s->pend);
+  free(s->pend_it);
   free(s);
 }
 
@@ -1113,10 +1209,13 @@ void orc_run(orc_sys *s, int nsteps) {
     initial_integrate(s);
     mesh_move(s);
     /* Neighbor::decide with delay 0, every 1: forced by a fix's next_reneighbor
-     * (moving mesh), else check every step */
-    const int forced = s->mesh_next_reneighbor == s->ntimestep;
+     * (moving mesh, or an insertion, fix_insert.cpp:311-319), else check every
+     * step; the insertion is a pre_exchange fix of the build (verlet.cpp:313-320) */
+    const int inserting = s->npend > 0;
+    if (inserting) insert_pending(s);
+    const int forced = s->mesh_next_reneighbor == s->ntimestep || inserting;
     if (forced || decide(s)) {
-      if (forced) s->mesh_forced++;
+      if (s->mesh_next_reneighbor == s->ntimestep) s->mesh_forced++;
       rebuild(s);
     } else if (mesh_decide_rebuild(s)) {
       /* FixMesh::pre_force on a regular step (fix_mesh.cpp:572-582) */
@@ -1175,6 +1274,7 @@ void orc_get_wall_rolling(const orc_sys *s, double *rhist) {
 
 int64_t orc_nrebuilds(const orc_sys *s) { return s->nrebuilds; }
 int64_t orc_ntimestep(const orc_sys *s) { return s->ntimestep; }
+int orc_natoms(const orc_sys *s) { return s->n; }
 
 void orc_contact_counts(const orc_sys *s, int64_t *npair_touch, int64_t *nwall_touch) {
   int64_t c = 0, w = 0;

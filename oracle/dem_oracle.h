@@ -145,6 +145,14 @@ orc_sys *orc_create(const orc_params *p, const orc_materials *m, int n, const do
                     const double *v, const double *omega, const double *radius,
                     const double *rmass, const int *type, const int *tag);
 void orc_destroy(orc_sys *s);
+/* Queues n particles that are inserted at the next step of orc_run, after its
+ * initial integrate and before its (forced) neighbor build (FixInsert::
+ * pre_exchange, fix_insert.cpp:652-826). Arrays as orc_create. */
+void orc_insert(orc_sys *s, int n, const double *x, const double *v, const double *omega,
+                const double *radius, const double *rmass, const int *type, const int *tag);
+/* the largest radius of particles still to come (PairGran::init_one,
+ * pair_gran.cpp:538-551); sets the cell size of the binned build */
+void orc_set_max_radius(orc_sys *s, double rmax_future);
 void orc_setup(orc_sys *s);
 void orc_run(orc_sys *s, int nsteps);
 /* one force evaluation on the current state (no integration), with
@@ -166,6 +174,7 @@ int64_t orc_lost_atoms(const orc_sys *s, int64_t *first_step);
 /* the box after the last build's Domain::reset_box (shrink-wrapped faces) */
 void orc_get_box(const orc_sys *s, double *boxlo, double *boxhi);
 int64_t orc_ntimestep(const orc_sys *s);
+int orc_natoms(const orc_sys *s);
 /* count of touching half-list pairs and of touching particle-wall contacts */
 void orc_contact_counts(const orc_sys *s, int64_t *npair_touch, int64_t *nwall_touch);
 

@@ -54,6 +54,29 @@ class OrcParams(ctypes.Structure):
     ]
 
 
+ORI_MAXTMPL = 8
+
+
+class OriParams(ctypes.Structure):
+    _fields_ = [
+        ("ntemplates", ctypes.c_int), ("tmpl_seed", ctypes.c_int * ORI_MAXTMPL),
+        ("tmpl_type", ctypes.c_int * ORI_MAXTMPL), ("tmpl_radius", _D * ORI_MAXTMPL),
+        ("tmpl_density", _D * ORI_MAXTMPL), ("dist_seed", ctypes.c_int),
+        ("dist_numberbased", ctypes.c_int), ("dist_weight", _D * ORI_MAXTMPL),
+        ("seed", ctypes.c_int), ("maxattempt", ctypes.c_int), ("overlapcheck", ctypes.c_int),
+        ("all_in", ctypes.c_int), ("exact_number", ctypes.c_int), ("vel_style", ctypes.c_int),
+        ("v_insert", _D * 3), ("v_fluct", _D * 3), ("omega", _D * 3),
+        ("check_dist_from_subdomain_border", ctypes.c_int), ("ntry_mc", ctypes.c_int),
+        ("volumefraction_region", _D), ("particles_in_region", ctypes.c_int),
+        ("mass_in_region", _D), ("region_lo", _D * 3), ("region_hi", _D * 3),
+        ("boxlo", _D * 3), ("boxhi", _D * 3), ("box_change", ctypes.c_int),
+    ]
+
+
+class OriPark(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_int), ("save", ctypes.c_int), ("second", _D)]
+
+
 _lib = None
 
 
@@ -122,6 +145,25 @@ def load():
     lib.orc_tri_contact.restype = _D
     lib.orc_mesh_forced_rebuilds.argtypes = [ctypes.c_void_p]
     lib.orc_mesh_forced_rebuilds.restype = ctypes.c_int64
+    lib.orc_insert.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, dp, dp, dp, ip, ip]
+    lib.orc_set_max_radius.argtypes = [ctypes.c_void_p, _D]
+    lib.orc_natoms.argtypes = [ctypes.c_void_p]
+    lib.orc_natoms.restype = ctypes.c_int
+    lib.ori_park_init.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.ori_park_uniform.argtypes = [ctypes.c_void_p]
+    lib.ori_park_uniform.restype = _D
+    lib.ori_park_gaussian.argtypes = [ctypes.c_void_p]
+    lib.ori_park_gaussian.restype = _D
+    lib.ori_create.argtypes = [P(OriParams)]
+    lib.ori_create.restype = ctypes.c_void_p
+    lib.ori_destroy.argtypes = [ctypes.c_void_p]
+    lib.ori_region_volume.argtypes = [ctypes.c_void_p]
+    lib.ori_region_volume.restype = _D
+    lib.ori_distribution.argtypes = [ctypes.c_void_p, dp, dp, dp, ip]
+    lib.ori_insert_step.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, dp, dp, dp, ctypes.c_int,
+                                    dp, dp, dp, dp, dp, dp, ip, ip]
+    lib.ori_insert_step.restype = ctypes.c_int
+    lib.ori_rng_states.argtypes = [ctypes.c_void_p, ip]
     _lib = lib
     return lib
 
@@ -378,6 +420,18 @@ class OracleSystem:
     def mesh_forced_rebuilds(self):
         return self.lib.orc_mesh_forced_rebuilds(self.s)
 
+    def insert(self, P):
+        """Queues particles inserted at the next step of a run (after its
+        initial integrate, before its forced neighbor build)."""
+        arr = {k: np.ascontiguousarray(P[k], dtype=np.float64) for k in ("x", "v", "omega", "radius", "rmass")}
+        ty = np.ascontiguousarray(P["type"], dtype=np.int32)
+        tg = np.ascontiguousarray(P["tag"], dtype=np.int32)
+        self.lib.orc_insert(self.s, len(ty), _dp(arr["x"]), _dp(arr["v"]), _dp(arr["omega"]),
+                            _dp(arr["radius"]), _dp(arr["rmass"]), _ip(ty), _ip(tg))
+
+    def set_max_radius(self, r):
+        self.lib.orc_set_max_radius(self.s, float(r))
+
     def setup(self):
         self.lib.orc_setup(self.s)
 
@@ -393,7 +447,7 @@ class OracleSystem:
         self.lib.orc_compute_forces(self.s, int(shearupdate))
 
     def state(self):
-        n = self.n
+        n = self.n = self.lib.orc_natoms(self.s)
         out = {k: np.zeros((n, 3)) for k in ("x", "v", "omega", "f", "torque")}
         self.lib.orc_get_state(self.s, _dp(out["x"]), _dp(out["v"]), _dp(out["omega"]),
                                _dp(out["f"]), _dp(out["torque"]))
@@ -753,3 +807,106 @@ class SqsSystem:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# particle insertion: fix insert/pack + particletemplate/sphere +
+# particledistribution/discrete (insert_oracle.c)
+# ---------------------------------------------------------------------------
+INSERT_DEFAULTS = dict(maxattempt=50, overlapcheck=1, all_in=0, exact_number=1, vel_style=0,
+                       v_insert=(0.0, 0.0, 0.0), v_fluct=(0.0, 0.0, 0.0), omega=(0.0, 0.0, 0.0),
+                       check_dist_from_subdomain_border=1, ntry_mc=100000, volumefraction_region=0.0,
+                       particles_in_region=0, mass_in_region=0.0, box_change=0, dist_numberbased=0)
+
+
+def insert_params(cls, **kw):
+    """Fills an OriParams-like ctypes structure (the oracle's or pydem's, whose
+    field names are given by `names`) from keyword arguments:
+    templates=[(seed, type, radius, density)], weights=[...], dist_seed, seed,
+    region=(lo, hi), box=(lo, hi) and the insert/pack options."""
+    d = dict(INSERT_DEFAULTS, **kw)
+    p = cls()
+    t = d["templates"]
+    p.ntemplates = len(t)
+    for i, (sd, ty, r, rho) in enumerate(t):
+        p.tmpl_seed[i], p.tmpl_type[i], p.tmpl_radius[i], p.tmpl_density[i] = sd, ty, r, rho
+        p.dist_weight[i] = d["weights"][i]
+    p.dist_seed = d["dist_seed"]
+    p.dist_numberbased = int(d["dist_numberbased"])
+    for k in ("seed", "maxattempt", "overlapcheck", "all_in", "exact_number", "vel_style",
+              "check_dist_from_subdomain_border", "ntry_mc", "particles_in_region", "box_change"):
+        setattr(p, k, int(d[k]))
+    p.volumefraction_region = float(d["volumefraction_region"])
+    p.mass_in_region = float(d["mass_in_region"])
+    for dim in range(3):
+        p.v_insert[dim], p.v_fluct[dim], p.omega[dim] = d["v_insert"][dim], d["v_fluct"][dim], d["omega"][dim]
+        p.region_lo[dim], p.region_hi[dim] = d["region"][0][dim], d["region"][1][dim]
+        p.boxlo[dim], p.boxhi[dim] = d["box"][0][dim], d["box"][1][dim]
+    return p
+
+
+class OracleInserter:
+    """fix insert/pack (oracle restatement)."""
+
+    def __init__(self, **kw):
+        self.lib = load()
+        self.p = insert_params(OriParams, **kw)
+        self.s = self.lib.ori_create(ctypes.byref(self.p))
+        if not self.s:
+            raise ValueError("insert/pack: invalid setting")
+
+    def region_volume(self):
+        return self.lib.ori_region_volume(self.s)
+
+    def distribution(self):
+        nt = self.p.ntemplates
+        ve, me = ctypes.c_double(), ctypes.c_double()
+        w = np.zeros(nt)
+        order = np.zeros(nt, dtype=np.int32)
+        self.lib.ori_distribution(self.s, ctypes.byref(ve), ctypes.byref(me), _dp(w), _ip(order))
+        return ve.value, me.value, w, order
+
+    def rng_states(self):
+        st = np.zeros(4, dtype=np.int32)
+        self.lib.ori_rng_states(self.s, _ip(st))
+        return st
+
+    def step(self, x=None, radius=None, rmass=None, box=None, cap=None):
+        """One insertion; returns (dict of the inserted particles, requested)."""
+        x = np.zeros((0, 3)) if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        n = len(x)
+        radius = np.zeros(0) if radius is None else np.ascontiguousarray(radius, dtype=np.float64)
+        rmass = np.zeros(0) if rmass is None else np.ascontiguousarray(rmass, dtype=np.float64)
+        blo = np.ascontiguousarray(box[0] if box else self.p.boxlo, dtype=np.float64)
+        bhi = np.ascontiguousarray(box[1] if box else self.p.boxhi, dtype=np.float64)
+        cap = cap or 200000
+        out = {k: np.zeros((cap, 3)) for k in ("x", "v", "omega")}
+        out.update({k: np.zeros(cap) for k in ("radius", "density", "rmass")})
+        out["type"] = np.zeros(cap, dtype=np.int32)
+        req = ctypes.c_int()
+        k = self.lib.ori_insert_step(self.s, n, _dp(x), _dp(radius), _dp(rmass), _dp(blo), _dp(bhi), cap,
+                                     _dp(out["x"]), _dp(out["v"]), _dp(out["omega"]), _dp(out["radius"]),
+                                     _dp(out["density"]), _dp(out["rmass"]), _ip(out["type"]),
+                                     ctypes.byref(req))
+        if k < 0:
+            raise ValueError("insert/pack: insertion failed")
+        return {kk: vv[:k] for kk, vv in out.items()}, req.value
+
+    def close(self):
+        if self.s:
+            self.lib.ori_destroy(self.s)
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def park_uniforms(seed, n):
+    """n draws of the oracle's Park-Miller stream from `seed`."""
+    lib = load()
+    r = OriPark()
+    lib.ori_park_init(ctypes.byref(r), int(seed))
+    return np.array([lib.ori_park_uniform(ctypes.byref(r)) for _ in range(n)]), r.seed
