@@ -655,10 +655,13 @@ int dem_inserter_create(const dem_insert_params *params, dem_inserter **out);
 int dem_inserter_destroy(dem_inserter *ins);
 const char *dem_inserter_last_error(const dem_inserter *ins);
 /* One insertion with the current particles (x n*3, radius, rmass, any order)
- * and the current box: writes up to cap particles in insertion order
+ * and the current box: writes the placed particles in insertion order
  * (position, velocity, angular velocity n*3 each; radius, density, mass;
  * atom type). *ninserted receives how many the fix placed, *nrequested how
- * many it asked for; more than cap placed is DEM_ERR_OVERFLOW. */
+ * many it asked for. When cap is below the number asked for, the call
+ * returns DEM_ERR_OVERFLOW having changed nothing (its random streams
+ * included), with that number in *nrequested, so it can be repeated with
+ * room enough. */
 int dem_inserter_step(dem_inserter *ins, int64_t n, const double *x, const double *radius,
                       const double *rmass, const double *boxlo, const double *boxhi, int64_t cap,
                       double *x_out, double *v_out, double *omega_out, double *radius_out,

@@ -833,9 +833,9 @@ int check_ready(dem_handle *h) {
 }
 
 int do_setup(dem_handle *h) {
-  if (h && h->n == 0 && !h->pend.tag.empty()) {
-    // Verlet::setup of a system whose first particles are still to be
-    // inserted: nothing to build or compute yet
+  if (h && h->n == 0) {
+    // Verlet::setup of a system whose particles are still to be inserted:
+    // nothing to build or compute yet
     if (!h->mat_set) return fail(h, DEM_ERR_STATE, "materials not set (fix property/global)");
     h->setup_done = true;
     h->can_continue = true;

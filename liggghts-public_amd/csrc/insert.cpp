@@ -321,6 +321,9 @@ int dem_inserter_step(dem_inserter *s, int64_t n, const double *x, const double 
   const dem_insert_params &p = s->p;
   if (ninserted) *ninserted = 0;
   if (nrequested) *nrequested = 0;
+  // what a call that finds too little room in the output gives back
+  const Park keep_rnd = s->rnd, keep_region = s->region_rnd, keep_dist = s->dist_rnd;
+  const double keep_vol = s->region_volume, keep_vol_local = s->region_volume_local;
 
   // a region reaching outside a box with f faces (fix_insert_pack.cpp:252-261,
   // Region::bbox_extends_outside_box, region.cpp:704-711)
@@ -400,6 +403,14 @@ int dem_inserter_step(dem_inserter *s, int64_t n, const double *x, const double 
   const int64_t nplan = (int64_t)plan.size();
   if (nrequested) *nrequested = nplan;
   if (nplan == 0) return DEM_OK;
+  if (nplan > cap) {  // nothing happened: the caller can retry with room for nplan
+    s->rnd = keep_rnd;
+    s->region_rnd = keep_region;
+    s->dist_rnd = keep_dist;
+    s->region_volume = keep_vol;
+    s->region_volume_local = keep_vol_local;
+    return ins_fail(s, DEM_ERR_OVERFLOW, "the output holds fewer particles than the insertion asks for");
+  }
 
   // particles to check against: those near the region inside the fix's
   // bounding box (FixInsert::load_xnear, fix_insert.cpp:916-948;
@@ -478,7 +489,7 @@ int dem_inserter_step(dem_inserter *s, int64_t n, const double *x, const double 
       candidate(rb, pos);
       velocity(v);
     }
-    if (done < cap) {
+    {
       for (int d = 0; d < 3; d++) {
         x_out[3 * done + d] = pos[d];
         v_out[3 * done + d] = v[d];
@@ -492,7 +503,6 @@ int dem_inserter_step(dem_inserter *s, int64_t n, const double *x, const double 
     done++;
   }
   if (ninserted) *ninserted = done;
-  if (done > cap) return ins_fail(s, DEM_ERR_OVERFLOW, "more particles inserted than the output holds");
   return DEM_OK;
 }
 

@@ -57,6 +57,7 @@
 // with a warning; anything that would change the physics and is not listed
 // is an error, as the reference reports for an unknown style.
 #include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -194,6 +195,7 @@ struct Interp {
   int thermo_every = 0;
   std::vector<std::string> thermo_cols = {"step", "atoms", "ke"};
   std::map<std::string, std::string> vars;
+  std::map<std::string, std::string> var_style;  // equal | index | string
   // system
   dem_handle *h = nullptr;
   bool dirty = true;  // particles/params changed since the handle was built
@@ -219,11 +221,351 @@ struct Interp {
   std::map<std::string, Dump> dumps;
   std::string restart_file;  // read_restart: device state applied when the system is built
 
+  // fix particletemplate/sphere (fix_template_sphere.cpp:72-245): constant
+  // radius and density
+  struct Template {
+    int seed = 0, type = 1;
+    double radius = 0, density = 0;
+  };
+  std::map<std::string, Template> templates;
+  // fix particledistribution/discrete (fix_particledistribution_discrete.cpp:69-244)
+  struct Distribution {
+    int seed = 0;
+    bool numberbased = false;
+    std::vector<std::string> tmpl;
+    std::vector<double> weight;
+  };
+  std::map<std::string, Distribution> distributions;
+  // fix insert/pack (fix_insert.cpp:78-340, fix_insert_pack.cpp:74-160)
+  struct InsertFix {
+    dem_insert_params p{};
+    std::string group, region;
+    long long next = 0;           // next_reneighbor: the step of the next insertion, 0 none
+    long long every = 0;          // insert_every (0: once)
+    dem_inserter *ins = nullptr;  // built at its first run's setup (FixInsert::setup)
+  };
+  std::map<std::string, InsertFix> inserts;
+  std::vector<std::string> insert_order;  // pre_exchange runs the fixes in definition order
+  // groups (group <id> region <rid>): whether every atom belongs to it; the
+  // GPU path integrates and walls every atom, so a fix on a group that does
+  // not hold them all is refused at run time
+  std::map<std::string, bool> group_all;
+  std::vector<std::pair<std::string, std::string>> physics_groups;  // (fix id, group) of the GPU fixes
+
   ~Interp() {
     for (auto &d : dumps)
       if (d.second.fp) fclose(d.second.fp);
+    for (auto &kv : inserts)
+      if (kv.second.ins) dem_inserter_destroy(kv.second.ins);
     if (h) dem_destroy(h);
     if (hs) dem_sqdem_destroy(hs);
+  }
+
+  // Park-Miller seeds must be primes above 10000 (Random::Random,
+  // random.cpp:71-92, with seed_check_error on by default, input.cpp:157)
+  static int prime_seed(const std::string &s) {
+    const int v = inum(s);
+    bool prime = v > 1;
+    for (long long d = 2; prime && d * d <= v; d++) prime = v % d != 0;
+    if (v < 10000 || !prime)
+      die("Random number generation: It is required that the random seed value is > 10000 and a prime "
+          "number. The random seed used was " + s);
+    return v;
+  }
+
+  // fix <id> <group> particletemplate/sphere <seed> atom_type <t>
+  //     density constant <rho> radius constant <r> [volume_limit <v>]
+  void fix_template(const std::vector<std::string> &t) {
+    if (t.size() < 5) die("fix particletemplate/sphere: not enough arguments");
+    Template tp;
+    tp.seed = prime_seed(t[4]);
+    double vol_limit = 1e-12;
+    for (size_t k = 5; k < t.size();) {
+      if (t[k] == "atom_type" && k + 1 < t.size()) {
+        tp.type = inum(t[k + 1]);
+        if (tp.type < 1) die("fix particletemplate/sphere: invalid atom type (must be >=1)");
+        k += 2;
+      } else if ((t[k] == "density" || t[k] == "radius") && k + 2 < t.size()) {
+        if (t[k + 1] != "constant")
+          die("fix particletemplate/sphere: invalid " + t[k] + " random style");  // :180-215
+        const double v = num(t[k + 2]);
+        if (v <= 0.) die("fix particletemplate/sphere: " + t[k] + " must be >= 0");
+        (t[k] == "density" ? tp.density : tp.radius) = v;
+        k += 3;
+      } else if (t[k] == "volume_limit" && k + 1 < t.size()) {
+        vol_limit = num(t[k + 1]);
+        if (vol_limit <= 0) die("fix particletemplate/sphere: volume_limit > 0 required");
+        k += 2;
+      } else {
+        die("fix particletemplate/sphere: unrecognized keyword " + t[k]);
+      }
+    }
+    if (tp.density <= 0) die("fix particletemplate/sphere: have to define 'density'");
+    if (tp.radius <= 0) die("fix particletemplate/sphere: have to define 'radius'");
+    if (tp.radius * tp.radius * tp.radius * 4. * kMyPi / 3. < vol_limit)
+      die("fix particletemplate/sphere: Volume expectancy too small");
+    if (tp.type > ntypes) die("fix particletemplate/sphere: atom type beyond create_box's types");
+    templates[t[1]] = tp;
+  }
+
+  // fix <id> <group> particledistribution/discrete[/numberbased|/massbased]
+  //     <seed> <n> <template1> <weight1> ...
+  void fix_distribution(const std::vector<std::string> &t) {
+    if (t.size() < 7) die("fix particledistribution/discrete: not enough arguments");
+    Distribution d;
+    d.numberbased = t[3].find("numberbased") != std::string::npos;
+    d.seed = prime_seed(t[4]);
+    const int nt = inum(t[5]);
+    if (nt < 1) die("fix particledistribution/discrete: illegal number of templates");
+    if (nt > DEM_INSERT_MAXTEMPLATES) die("fix particledistribution/discrete: at most 8 templates on this path");
+    if ((int)t.size() != 6 + 2 * nt)
+      die("fix particledistribution/discrete: # of templates does not match # of arguments");
+    for (int i = 0; i < nt; i++) {
+      const std::string &id = t[6 + 2 * i];
+      if (!templates.count(id)) die("fix particledistribution/discrete: invalid ID for fix particletemplate provided");
+      for (const auto &prev : d.tmpl)
+        if (prev == id) die("fix particledistribution/discrete: cannot use the same template twice");
+      d.tmpl.push_back(id);
+      const double w = num(t[7 + 2 * i]);
+      if (w < 0) die("fix particledistribution/discrete: invalid weight");
+      d.weight.push_back(w);
+    }
+    double ws = 0;
+    for (double w : d.weight) ws += w;
+    if (std::fabs(ws - 1.) > 0.00001)
+      fprintf(out, "WARNING: particledistribution/discrete: sum of distribution weights != 1, normalizing "
+                   "distribution\n");
+    distributions[t[1]] = d;
+  }
+
+  // fix <id> <group> insert/pack seed <s> distributiontemplate <d> [...] region <r>
+  //     volumefraction_region <f> | particles_in_region <n> | mass_in_region <m>
+  void fix_insert_pack(const std::vector<std::string> &t) {
+    if (t.size() < 6 || t[4] != "seed") die("fix insert/pack: expecting keyword 'seed'");
+    InsertFix f;
+    dem_insert_params &p = f.p;
+    p.seed = prime_seed(t[5]);
+    p.maxattempt = 50;
+    p.overlapcheck = 1;
+    p.exact_number = 1;
+    p.check_dist_from_subdomain_border = 1;
+    p.ntry_mc = 100000;
+    p.warn_region = 1;
+    f.group = t[2];
+    f.next = step + 1;  // first insertion on the next step (fix_insert.cpp:394-405)
+    f.every = -1;
+    std::string dist;
+    for (size_t k = 6; k < t.size();) {
+      const std::string &kw = t[k];
+      auto need = [&](size_t m) {
+        if (k + m >= t.size()) die("fix insert/pack: not enough arguments for " + kw);
+      };
+      if (kw == "distributiontemplate") {
+        need(1);
+        dist = t[k + 1];
+        k += 2;
+      } else if (kw == "maxattempt") {
+        need(1);
+        p.maxattempt = inum(t[k + 1]);
+        k += 2;
+      } else if (kw == "insert_every" || kw == "every") {
+        need(1);
+        f.every = t[k + 1] == "once" ? 0 : inum(t[k + 1]);
+        if (f.every < 0) die("fix insert/pack: insert_every must be >= 0");
+        k += 2;
+      } else if (kw == "insert_every_time") {
+        need(1);
+        if (dt <= 0) die("fix insert/pack: need so set 'timestep' before");
+        f.every = t[k + 1] == "once" ? 0 : (long long)(num(t[k + 1]) / dt);
+        k += 2;
+      } else if (kw == "start") {
+        need(1);
+        f.next = inum(t[k + 1]);
+        if (f.next < step + 1) die("fix insert/pack: 'start' step can not be before current step");
+        k += 2;
+      } else if (kw == "overlapcheck" || kw == "all_in" || kw == "warn_region" ||
+                 kw == "check_dist_from_subdomain_border") {
+        need(1);
+        if (t[k + 1] != "yes" && t[k + 1] != "no") die("fix insert/pack: expecting 'yes' or 'no' after " + kw);
+        const int v = t[k + 1] == "yes";
+        if (kw == "overlapcheck") p.overlapcheck = v;
+        else if (kw == "all_in") p.all_in = v;
+        else if (kw == "warn_region") p.warn_region = v;
+        else p.check_dist_from_subdomain_border = v;
+        k += 2;
+      } else if (kw == "random_distribute") {
+        need(1);
+        if (t[k + 1] == "uncorrelated") p.exact_number = 0;
+        else if (t[k + 1] == "exact") p.exact_number = 1;
+        else die("fix insert/pack: expecting 'uncorrelated' or 'exact' after 'random_distribute'");
+        k += 2;
+      } else if (kw == "vel") {
+        need(4);
+        const std::string &st = t[k + 1];
+        const int nv = st == "constant" ? 3 : (st == "uniform" || st == "gaussian") ? 6 : -1;
+        if (nv < 0) die("fix insert/pack: expecting 'constant', 'uniform' or 'gaussian' after 'vel'");
+        need(nv);
+        p.vel_style = st == "constant" ? 0 : st == "uniform" ? 1 : 2;
+        for (int d = 0; d < 3; d++) p.v_insert[d] = num(t[k + 2 + d]);
+        if (nv == 6)
+          for (int d = 0; d < 3; d++) p.v_fluct[d] = num(t[k + 5 + d]);
+        k += 2 + nv;
+      } else if (kw == "omega") {
+        need(4);
+        if (t[k + 1] != "constant") die("fix insert/pack: expecting 'constant' after 'omega'");
+        for (int d = 0; d < 3; d++) p.omega[d] = num(t[k + 2 + d]);
+        k += 5;
+      } else if (kw == "region") {
+        need(1);
+        if (!regions.count(t[k + 1])) die("fix insert/pack: region ID does not exist");
+        f.region = t[k + 1];
+        k += 2;
+      } else if (kw == "volumefraction_region") {
+        need(1);
+        p.volumefraction_region = num(t[k + 1]);
+        if (p.volumefraction_region < 0. || p.volumefraction_region > 1.)
+          die("fix insert/pack: Invalid volumefraction");
+        k += 2;
+      } else if (kw == "particles_in_region") {
+        need(1);
+        p.particles_in_region = inum(t[k + 1]);
+        if (p.particles_in_region <= 0) die("fix insert/pack: 'ntotal_region' > 0 required");
+        k += 2;
+      } else if (kw == "mass_in_region") {
+        need(1);
+        p.mass_in_region = num(t[k + 1]);
+        if (p.mass_in_region <= 0.) die("fix insert/pack: 'masstotal_region' > 0 required");
+        k += 2;
+      } else if (kw == "ntry_mc") {
+        need(1);
+        p.ntry_mc = inum(t[k + 1]);
+        if (p.ntry_mc < 1000) die("fix insert/pack: ntry_mc must be > 1000");
+        k += 2;
+      } else {
+        die("fix insert/pack: unknown keyword " + kw);
+      }
+    }
+    if (dist.empty() || !distributions.count(dist))
+      die("Fix insert requires you to define a valid ID for a fix of type particledistribution/discrete");
+    if (f.region.empty()) die("fix insert/pack: must define an insertion region");
+    if (f.every < 0) die("fix insert/pack: must define 'insert_every'");
+    const Distribution &d = distributions[dist];
+    p.ntemplates = (int)d.tmpl.size();
+    p.distribution_seed = d.seed;
+    p.distribution_numberbased = d.numberbased ? 1 : 0;
+    std::vector<int> seeds{p.seed, d.seed};
+    for (int i = 0; i < p.ntemplates; i++) {
+      const Template &tp = templates[d.tmpl[i]];
+      p.template_seed[i] = tp.seed;
+      p.template_type[i] = tp.type;
+      p.template_radius[i] = tp.radius;
+      p.template_density[i] = tp.density;
+      p.distribution_weight[i] = d.weight[i];
+      seeds.push_back(tp.seed);
+    }
+    // FixInsert::sanity_check (fix_insert.cpp:434-482): every stream its own seed
+    std::sort(seeds.begin(), seeds.end());
+    if (std::unique(seeds.begin(), seeds.end()) != seeds.end())
+      die("Fix insert/pack: Random number generation: It is required that all the random seeds of this fix "
+          "insert/*, the random seed of particle distribution fix and all random seeds of the templates are "
+          "different");
+    const std::vector<double> &b = regions[f.region];
+    for (int k = 0; k < 3; k++) {
+      p.region_lo[k] = b[2 * k];
+      p.region_hi[k] = b[2 * k + 1];
+    }
+    if (inserts.count(t[1])) die("Replacing a fix insert is not available on this path");
+    inserts[t[1]] = f;
+    insert_order.push_back(t[1]);
+  }
+
+  // the largest radius insertion fixes may bring (FixInsert::max_rad for
+  // PairGran::init_one, pair_gran.cpp:538-551)
+  double insert_max_radius() {
+    double r = 0.0;
+    for (auto &kv : inserts)
+      for (int i = 0; i < kv.second.p.ntemplates; i++) r = std::max(r, kv.second.p.template_radius[i]);
+    return r;
+  }
+
+  // One FixInsert::pre_exchange (fix_insert.cpp:652-826) at the current step:
+  // place the fix's particles among the present ones, append them to the
+  // script's atoms with new tags (Atom::tag_extend) and hand them to the
+  // library, where they join at the next step's force pass.
+  void do_insertion(const std::string &id, InsertFix &f) {
+    double blo[3], bhi[3];
+    current_box(blo, bhi);
+    if (!f.ins) {  // FixInsert::setup: the region volume with the box of the first run
+      dem_insert_params p = f.p;
+      for (int d = 0; d < 3; d++) {
+        p.boxlo[d] = blo[d];
+        p.boxhi[d] = bhi[d];
+      }
+      p.box_change = shrink_faces ? 1 : 0;
+      bool fixed = false;
+      for (int d = 0; d < 3; d++)
+        for (int s = 0; s < 2; s++) fixed = fixed || face[d][s] == 'f';
+      p.fixed_boundary = fixed ? 1 : 0;
+      if (dem_inserter_create(&p, &f.ins)) die("fix " + id + ": " + dem_inserter_last_error(nullptr));
+    }
+    if (h && !tag.empty()) pull_state();
+    const int64_t n = (int64_t)tag.size();
+    int64_t cap = 4096, nins = 0, nreq = 0;
+    std::vector<double> xo, vo, wo, ro, rho, mo;
+    std::vector<int> to;
+    for (int attempt = 0;; attempt++) {
+      xo.resize(3 * cap);
+      vo.resize(3 * cap);
+      wo.resize(3 * cap);
+      ro.resize(cap);
+      rho.resize(cap);
+      mo.resize(cap);
+      to.resize(cap);
+      const int rc = dem_inserter_step(f.ins, n, x.data(), r.data(), m.data(), blo, bhi, cap, xo.data(), vo.data(),
+                                       wo.data(), ro.data(), rho.data(), mo.data(), to.data(), &nins, &nreq);
+      if (rc == DEM_ERR_OVERFLOW && attempt == 0) {  // nothing changed: again with room enough
+        cap = nreq;
+        continue;
+      }
+      if (rc) die("fix " + id + ": " + dem_inserter_last_error(f.ins));
+      break;
+    }
+    if (nins < nreq)
+      fprintf(out, "WARNING: Particle insertion: Less insertions than requested\n");
+    fprintf(out, "INFO: Particle insertion %s: inserted %lld particle templates (mass %g) at step %lld\n",
+            id.c_str(), (long long)nins, [&] {
+              double s = 0;
+              for (int64_t k = 0; k < nins; k++) s += mo[k];
+              return s;
+            }(), (long long)step + 1);
+    // the next insertion (fix_insert.cpp:700-708, 821-825)
+    f.next = f.every > 0 ? f.next + f.every : 0;
+    if (nins == 0) return;
+    int maxtag = 0;
+    for (int tg : tag) maxtag = std::max(maxtag, tg);
+    std::vector<int> newtag(nins);
+    for (int64_t k = 0; k < nins; k++) newtag[k] = maxtag + 1 + (int)k;
+    if (h && dem_insert_particles(h, nins, xo.data(), vo.data(), wo.data(), ro.data(), mo.data(), to.data(),
+                                  newtag.data()))
+      die(dem_last_error(h));
+    x.insert(x.end(), xo.begin(), xo.begin() + 3 * nins);
+    v.insert(v.end(), vo.begin(), vo.begin() + 3 * nins);
+    w.insert(w.end(), wo.begin(), wo.begin() + 3 * nins);
+    r.insert(r.end(), ro.begin(), ro.begin() + nins);
+    m.insert(m.end(), mo.begin(), mo.begin() + nins);
+    type.insert(type.end(), to.begin(), to.begin() + nins);
+    tag.insert(tag.end(), newtag.begin(), newtag.end());
+    // the new atoms join "all" and the fix's group only
+    for (auto &g : group_all)
+      if (g.first != f.group) g.second = false;
+  }
+
+  // the earliest pending insertion step, 0 if none
+  long long next_insertion() const {
+    long long s = 0;
+    for (const auto &kv : inserts)
+      if (kv.second.next > 0 && (s == 0 || kv.second.next < s)) s = kv.second.next;
+    return s;
   }
 
   static bool dump_column_known(const std::string &c) {
@@ -458,7 +800,13 @@ struct Interp {
         }
         auto it = vars.find(name);
         if (it == vars.end()) die("Substitution for illegal variable " + name);
-        outl += it->second;
+        if (var_style[name] == "equal") {  // Variable::retrieve: the formula's value, %.15g
+          char buf[64];
+          snprintf(buf, sizeof buf, "%.15g", eval_variable(name));
+          outl += buf;
+        } else {
+          outl += it->second;
+        }
         k = adv;
       } else {
         outl += line[k];
@@ -467,12 +815,239 @@ struct Interp {
     return outl;
   }
 
+  // words of a command; text in double quotes is one word, without the quotes
+  // (Input::parse, input.cpp:440-520)
   static std::vector<std::string> split(const std::string &s) {
     std::vector<std::string> t;
-    std::istringstream is(s);
-    std::string w;
-    while (is >> w) t.push_back(w);
+    size_t k = 0;
+    while (k < s.size()) {
+      while (k < s.size() && std::isspace((unsigned char)s[k])) k++;
+      if (k >= s.size()) break;
+      if (s[k] == '"') {
+        const size_t e = s.find('"', k + 1);
+        if (e == std::string::npos) die("Unbalanced quotes in input line");
+        t.push_back(s.substr(k + 1, e - k - 1));
+        k = e + 1;
+      } else {
+        const size_t b = k;
+        while (k < s.size() && !std::isspace((unsigned char)s[k])) k++;
+        t.push_back(s.substr(b, k - b));
+      }
+    }
     return t;
+  }
+
+  // ---- equal-style variables (Variable::evaluate, variable.cpp:800-1620) ----
+  // Numbers, + - * / % ^, unary - and !, comparisons, && ||, parentheses,
+  // the math functions of Variable::math_function (variable.cpp:2481-2700),
+  // v_name, thermo keywords and c_ID of the computes thermo knows; operator
+  // precedence as variable.cpp:132-140 (unary above ^ above * / % above + -),
+  // every binary operator left-associative.
+  struct Expr {
+    Interp &in;
+    const std::string &s;
+    size_t k = 0;
+    int depth;
+    Expr(Interp &i, const std::string &str, int d) : in(i), s(str), depth(d) {}
+    void ws() {
+      while (k < s.size() && std::isspace((unsigned char)s[k])) k++;
+    }
+    [[noreturn]] void bad(const std::string &m) { die("Invalid syntax in variable formula '" + s + "': " + m); }
+    static int prec(const std::string &op) {
+      if (op == "||") return 1;
+      if (op == "&&") return 2;
+      if (op == "==" || op == "!=") return 3;
+      if (op == "<" || op == "<=" || op == ">" || op == ">=") return 4;
+      if (op == "+" || op == "-") return 5;
+      if (op == "*" || op == "/" || op == "%") return 6;
+      if (op == "^") return 7;
+      return -1;
+    }
+    std::string peek_op() {
+      ws();
+      if (k >= s.size()) return "";
+      static const char *two[] = {"||", "&&", "==", "!=", "<=", ">="};
+      for (const char *o : two)
+        if (s.compare(k, 2, o) == 0) return o;
+      const char c = s[k];
+      if (std::strchr("+-*/%^<>", c)) return std::string(1, c);
+      return "";
+    }
+    double binary(int minp) {
+      double a = unary();
+      for (;;) {
+        const std::string op = peek_op();
+        const int p = op.empty() ? -1 : prec(op);
+        if (p < minp) return a;
+        k += op.size();
+        const double b = binary(p + 1);
+        if (op == "+") a = a + b;
+        else if (op == "-") a = a - b;
+        else if (op == "*") a = a * b;
+        else if (op == "/") {
+          if (b == 0.0) die("Divide by 0 in variable formula");
+          a = a / b;
+        } else if (op == "%") {
+          if (b == 0.0) die("Modulo 0 in variable formula");
+          a = std::fmod(a, b);
+        } else if (op == "^") {
+          if (b == 0.0 && a == 0.0) die("Power by 0 in variable formula");
+          a = std::pow(a, b);
+        } else if (op == "<") a = a < b;
+        else if (op == "<=") a = a <= b;
+        else if (op == ">") a = a > b;
+        else if (op == ">=") a = a >= b;
+        else if (op == "==") a = a == b;
+        else if (op == "!=") a = a != b;
+        else if (op == "&&") a = (a != 0.0 && b != 0.0);
+        else a = (a != 0.0 || b != 0.0);
+      }
+    }
+    double unary() {
+      ws();
+      if (k < s.size() && s[k] == '-') {
+        k++;
+        return -unary();
+      }
+      if (k < s.size() && s[k] == '!' && (k + 1 >= s.size() || s[k + 1] != '=')) {
+        k++;
+        return unary() == 0.0 ? 1.0 : 0.0;
+      }
+      return primary();
+    }
+    double primary() {
+      ws();
+      if (k >= s.size()) bad("unexpected end");
+      if (s[k] == '(') {
+        k++;
+        const double v = binary(0);
+        ws();
+        if (k >= s.size() || s[k] != ')') bad("missing )");
+        k++;
+        return v;
+      }
+      if (std::isdigit((unsigned char)s[k]) || s[k] == '.') {
+        char *end = nullptr;
+        const double v = std::strtod(s.c_str() + k, &end);
+        k = end - s.c_str();
+        return v;
+      }
+      size_t b = k;
+      while (k < s.size() && (std::isalnum((unsigned char)s[k]) || s[k] == '_' || s[k] == '[' || s[k] == ']'))
+        k++;
+      const std::string word = s.substr(b, k - b);
+      if (word.empty()) bad("unexpected '" + s.substr(k, 1) + "'");
+      ws();
+      if (k < s.size() && s[k] == '(') {  // math function
+        k++;
+        std::vector<double> args;
+        ws();
+        if (k < s.size() && s[k] == ')') k++;
+        else
+          for (;;) {
+            args.push_back(binary(0));
+            ws();
+            if (k < s.size() && s[k] == ',') {
+              k++;
+              continue;
+            }
+            if (k < s.size() && s[k] == ')') {
+              k++;
+              break;
+            }
+            bad("missing ) after arguments of " + word);
+          }
+        return func(word, args);
+      }
+      if (word.compare(0, 2, "v_") == 0) return in.eval_variable(word.substr(2), depth + 1);
+      double val = 0.0;
+      if (in.formula_keyword(word, val)) return val;
+      bad("unknown word '" + word + "'");
+    }
+    double func(const std::string &f, const std::vector<double> &a) {
+      auto one = [&]() {
+        if (a.size() != 1) bad("function " + f + " takes one argument");
+        return a[0];
+      };
+      if (f == "sqrt") {
+        if (one() < 0.0) die("Sqrt of negative value in variable formula");
+        return std::sqrt(a[0]);
+      }
+      if (f == "exp") return std::exp(one());
+      if (f == "ln") {
+        if (one() <= 0.0) die("Log of zero/negative value in variable formula");
+        return std::log(a[0]);
+      }
+      if (f == "log") {
+        if (one() <= 0.0) die("Log of zero/negative value in variable formula");
+        return std::log10(a[0]);
+      }
+      if (f == "abs") return std::fabs(one());
+      if (f == "sin") return std::sin(one());
+      if (f == "cos") return std::cos(one());
+      if (f == "tan") return std::tan(one());
+      if (f == "asin") {
+        if (one() < -1.0 || a[0] > 1.0) die("Arcsin of invalid value in variable formula");
+        return std::asin(a[0]);
+      }
+      if (f == "acos") {
+        if (one() < -1.0 || a[0] > 1.0) die("Arccos of invalid value in variable formula");
+        return std::acos(a[0]);
+      }
+      if (f == "atan") return std::atan(one());
+      if (f == "atan2") {
+        if (a.size() != 2) bad("atan2 takes two arguments");
+        return std::atan2(a[0], a[1]);
+      }
+      if (f == "ceil") return std::ceil(one());
+      if (f == "floor") return std::floor(one());
+      if (f == "round") return std::round(one());
+      die("function " + f + " in a variable formula is not available on this path");
+    }
+  };
+
+  // the value of an equal-style variable (or of an index/string one holding a number)
+  double eval_variable(const std::string &name, int depth = 0) {
+    if (depth > 50) die("Variable formula refers to itself: " + name);
+    auto it = vars.find(name);
+    if (it == vars.end()) die("Invalid variable name in variable formula: " + name);
+    Expr e(*this, it->second, depth);
+    const double v = e.binary(0);
+    e.ws();
+    if (e.k != it->second.size()) e.bad("trailing text");
+    return v;
+  }
+
+  // thermo keywords and computes a formula may read (Variable::evaluate's
+  // thermo-keyword branch, variable.cpp:1380-1410)
+  bool formula_keyword(const std::string &word, double &val) {
+    if (word == "step") {
+      val = (double)step;
+      return true;
+    }
+    if (word == "atoms") {
+      val = (double)tag.size();
+      return true;
+    }
+    if (word == "dt") {
+      val = dt;
+      return true;
+    }
+    if (word == "PI") {
+      val = kMyPi;
+      return true;
+    }
+    double ke = 0.0;
+    if (word == "ke" || word.compare(0, 2, "c_") == 0) {
+      if (h && !tag.empty()) pull_state();
+      for (size_t i = 0; i < tag.size(); i++)
+        ke += 0.5 * m[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+      if (word == "ke") {
+        val = ke;
+        return true;
+      }
+    }
+    return thermo_value(word, ke, val);
   }
 
   void need_box(const char *cmd) {
@@ -1054,10 +1629,13 @@ struct Interp {
       if (md.stress && dem_mesh_stress(h, id, md.pref)) die(dem_last_error(h));
       mesh_ids[name] = id;
     }
-    if (tag.empty()) die("run: no atoms");
-    if (dem_set_particles(h, (int64_t)tag.size(), x.data(), v.data(), w.data(), r.data(), m.data(),
-                          type.data(), tag.data()))
+    if (dem_set_max_radius(h, insert_max_radius())) die(dem_last_error(h));
+    if (tag.empty()) {
+      if (next_insertion() == 0) die("run: no atoms");
+    } else if (dem_set_particles(h, (int64_t)tag.size(), x.data(), v.data(), w.data(), r.data(), m.data(),
+                                 type.data(), tag.data())) {
       die(dem_last_error(h));
+    }
     if (!restart_file.empty()) {  // contact histories, list and mesh state of read_restart
       if (dem_read_restart(h, restart_file.c_str())) die(dem_last_error(h));
       restart_file.clear();
@@ -1066,6 +1644,7 @@ struct Interp {
   }
 
   void pull_state() {
+    if (tag.empty()) return;
     if (sq) {
       if (dem_sqdem_get_state(hs, x.data(), v.data(), w.data(), nullptr, sq_quat.data(), nullptr, nullptr))
         die(dem_sqdem_last_error(hs));
@@ -1165,20 +1744,43 @@ struct Interp {
     fprintf(out, "%s\n", row.c_str());
   }
 
+  // fixes of the GPU path act on every atom: their groups must hold them all
+  void check_groups() {
+    for (const auto &fg : physics_groups)
+      if (fg.second != "all" && !group_all[fg.second])
+        die("fix " + fg.first + ": group " + fg.second +
+            " does not hold every atom; fixes on part of the atoms are not available on this path");
+  }
+
+  // run N [upto] [pre yes|no] [post yes|no] (Run::command, run.cpp:40-230)
   void run(const std::vector<std::string> &t) {
     if (t.size() < 2) die("Illegal run command");
-    const long long n = std::stoll(t[1]);
-    if (n < 0) die("Illegal run command");
-    bool pre = true;
-    for (size_t k = 2; k + 1 < t.size(); k += 2) {
+    long long n = std::stoll(t[1]);
+    bool pre = true, upto = false;
+    for (size_t k = 2; k < t.size();) {
+      if (t[k] == "upto") {
+        upto = true;
+        k++;
+        continue;
+      }
+      if (k + 1 >= t.size()) die("Illegal run command");
       if (t[k] == "pre") pre = t[k + 1] != "no";
-      else if (t[k] == "post" || t[k] == "upto" || t[k] == "every" || t[k] == "start" || t[k] == "stop") {
-        if (t[k] != "post") die("run option " + t[k] + " is not supported on this path");
-      } else die("Illegal run command");
+      else if (t[k] == "every" || t[k] == "start" || t[k] == "stop")
+        die("run option " + t[k] + " is not supported on this path");
+      else if (t[k] != "post") die("Illegal run command");
+      k += 2;
     }
+    if (upto) {  // N is the last timestep of the run (run.cpp:118-121)
+      if (n < step) die("Invalid run command upto value");
+      n -= step;
+    }
+    if (n < 0) die("Illegal run command");
+    check_groups();
     if (dirty || (sq ? !hs : !h)) {
       build_handle();
       pre = true;
+    } else if (!sq && dem_set_max_radius(h, insert_max_radius())) {
+      die(dem_last_error(h));
     }
     const auto t0 = std::chrono::steady_clock::now();
     run_t0 = t0;
@@ -1192,8 +1794,17 @@ struct Interp {
     thermo_line();
     if (!sq) write_dumps(false);  // Output::setup writes the dumps due at this step
     while (done < n) {
+      // insertions due at the next step: FixInsert::pre_exchange on the step
+      // whose next_reneighbor it set (fix_insert.cpp:652-660), fixes in order
+      for (const std::string &id : insert_order) {
+        InsertFix &f = inserts[id];
+        if (f.next == step + 1) do_insertion(id, f);
+      }
+      check_groups();
       long long k = sq ? chunk - (step % (chunk > 0 ? chunk : 1)) : next_output(n - done);
       if (k <= 0 || k > n - done) k = n - done;
+      const long long ni = next_insertion();  // the next piece ends just before it
+      if (ni > step + 1 && ni - 1 - step < k) k = ni - 1 - step;
       int rc = sq ? dem_sqdem_run(hs, (int)k) : dem_run_continue(h, k);
       (void)first;
       first = false;
@@ -1248,11 +1859,31 @@ struct Interp {
       }
     } else if (c == "atom_modify" || c == "processors" || c == "echo" || c == "log" ||
                (c == "compute" && !(t.size() >= 4 && (t[3] == "erotate/sphere" || t[3] == "ke"))) ||
-               c == "uncompute" || c == "group" || c == "restart" ||
+               c == "uncompute" || c == "restart" ||
                c == "write_dump" || c == "soft_particles" ||
                c == "hard_particles") {
       if (c != "atom_modify" && c != "processors" && c != "echo" && c != "log")
         fprintf(out, "WARNING: '%s' is output/bookkeeping only and is ignored on this path\n", c.c_str());
+    } else if (c == "group") {
+      // group <id> region <rid> (Group::assign, group.cpp:120-200) holds the
+      // atoms inside the region now; atoms inserted later join only "all" and
+      // their fix's group. Other group styles are kept as groups that do not
+      // hold every atom.
+      if (t.size() < 3) die("Illegal group command");
+      if (t[1] == "all") die("Cannot redefine group all");
+      bool all = false;
+      if (t[2] == "region" && t.size() >= 4) {
+        auto it = regions.find(t[3]);
+        if (it == regions.end()) die("Group region ID does not exist");
+        const std::vector<double> &b = it->second;
+        all = true;
+        for (size_t i = 0; i < tag.size() && all; i++)
+          for (int d = 0; d < 3; d++)
+            if (x[3 * i + d] < b[2 * d] || x[3 * i + d] > b[2 * d + 1]) all = false;
+      } else {
+        fprintf(out, "WARNING: group style %s is kept as a group that does not hold every atom\n", t[2].c_str());
+      }
+      group_all[t[1]] = all;
     } else if (c == "thermo_modify") {
       // Thermo::modify_params (thermo.cpp:476-602): lost error|warn|ignore is
       // kept; the formatting keywords change output only
@@ -1401,6 +2032,10 @@ struct Interp {
     } else if (c == "fix") {
       if (t.size() < 4) die("Illegal fix command");
       const std::string &style = t[3];
+      if (style == "nve/sphere" || style == "gravity" || style == "wall/gran" || style == "nve/superquadric") {
+        if (t[2] != "all" && !group_all.count(t[2])) die("Could not find fix group ID " + t[2]);
+        physics_groups.push_back({t[1], t[2]});
+      }
       if (style == "property/global") {
         fix_property_global(t);
       } else if (style == "nve/sphere") {
@@ -1537,13 +2172,37 @@ struct Interp {
         if (!gravity) gravity_first = false;
         wall_seen = true;
         dirty = true;
+      } else if (style == "particletemplate/sphere") {
+        need_box("fix particletemplate/sphere");
+        fix_template(t);
+      } else if (style.compare(0, 29, "particledistribution/discrete") == 0) {
+        fix_distribution(t);
+      } else if (style == "insert/pack") {
+        need_box("fix insert/pack");
+        if (sq) die("fix insert/pack for superquadrics is not available on this path");
+        if (t[2] != "all" && !group_all.count(t[2])) die("Could not find fix group ID " + t[2]);
+        fix_insert_pack(t);
       } else {
         die("fix style " + style + " is not available on this path");
       }
     } else if (c == "set") {
       set_command(t);
     } else if (c == "unfix") {
-      die("unfix is not supported on this path");
+      // Modify::delete_fix: insertion fixes and their templates leave; the
+      // other fixes of this path shape the GPU system and stay
+      if (t.size() < 2) die("Illegal unfix command");
+      if (inserts.count(t[1])) {
+        if (inserts[t[1]].ins) dem_inserter_destroy(inserts[t[1]].ins);
+        inserts.erase(t[1]);
+        insert_order.erase(std::find(insert_order.begin(), insert_order.end(), t[1]));
+      } else if (templates.count(t[1])) {
+        templates.erase(t[1]);
+      } else if (distributions.count(t[1])) {
+        distributions.erase(t[1]);
+      } else {
+        die("unfix " + t[1] + ": only insertion fixes (insert/pack, particletemplate/sphere, "
+            "particledistribution/discrete) can be removed on this path");
+      }
     } else if (c == "thermo") {
       if (t.size() < 2) die("Illegal thermo command");
       thermo_every = inum(t[1]);
@@ -1559,10 +2218,16 @@ struct Interp {
       if (t.size() < 4) die("Illegal variable command");
       if (t[2] != "equal" && t[2] != "index" && t[2] != "string")
         die("variable style " + t[2] + " is not supported on this path");
-      if (vars.find(t[1]) == vars.end() || t[2] == "equal") vars[t[1]] = t[3];
+      if ((t[2] == "equal" || t[2] == "string") && t.size() != 4) die("Illegal variable command");
+      // an index or string variable keeps its first definition (variable.cpp:160-200)
+      if (vars.find(t[1]) == vars.end() || t[2] == "equal") {
+        vars[t[1]] = t[3];
+        var_style[t[1]] = t[2];
+      }
     } else if (c == "print") {
-      std::string rest = line.substr(line.find("print") + 5);
-      fprintf(out, "%s\n", rest.c_str());
+      // Input::print (input.cpp:1010-1060): the first argument, quotes removed
+      if (t.size() < 2) die("Illegal print command");
+      fprintf(out, "%s\n", t[1].c_str());
     } else if (c == "package") {
       if (t.size() >= 4 && t[1] == "amd" && t[2] == "precision") {
         if (t[3] == "fp64") precision = DEM_FP64;

@@ -78,7 +78,10 @@ def test_full_command_subset_parses(tmp_path):
     ("dump d1 all atom 100 post/d.txt\n", "dump style atom"),
     ("dump_modify d9 sort id\n", "Could not find dump_modify ID"),
     ("undump d9\n", "Could not find undump ID"),
-    ("fix x all insert/pack seed 1\n", "not available"),
+    ("fix x all insert/stream seed 15485863\n", "not available"),
+    ("fix x all insert/pack seed 1\n", "prime number"),
+    ("fix t all particletemplate/sphere 15485863 atom_type 1 density constant 2500 radius uniform 1 2\n",
+     "invalid radius random style"),
     ("pair_style gran model hertz tangential history rolling_friction epsd3\n", "'rolling_friction epsd2'"),
     ("dump s all mesh/stl 100 post/mesh.stl\n", "one file per time-step"),
     ("dump s all mesh/stl 100 post/mesh*.stl nosuchmesh\n", "unknown keyword or mesh"),
@@ -198,3 +201,95 @@ def test_read_restart_errors(tmp_path):
     assert rc != 0 and "after simulation box is defined" in log
     rc, log = run_script(tmp_path, HEADER + f"write_restart {tmp_path}/x.restart\n", "in.wr")
     assert rc != 0 and "preceding run" in log
+
+
+# the command sequence of the reference's packing tutorial (insertion by
+# volume fraction into an empty box with m boundaries, equal-style
+# variables in its arguments and in a print), with this test's own numbers
+PACKING_SCRIPT = """atom_style granular
+atom_modify map array
+boundary m m m
+newton off
+communicate single vel yes
+units si
+region reg block -0.02 0.02 -0.02 0.02 0. 0.06 units box
+create_box 1 reg
+neighbor 0.0005 bin
+neigh_modify delay 0
+fix m1 all property/global youngsModulus peratomtype 5.e6
+fix m2 all property/global poissonsRatio peratomtype 0.45
+fix m3 all property/global coefficientRestitution peratomtypepair 1 0.3
+fix m4 all property/global coefficientFriction peratomtypepair 1 0.5
+pair_style gran model hertz tangential history
+pair_coeff * *
+timestep 0.00001
+fix xwalls1 all wall/gran model hertz tangential history primitive type 1 xplane -0.02
+fix xwalls2 all wall/gran model hertz tangential history primitive type 1 xplane +0.02
+fix ywalls1 all wall/gran model hertz tangential history primitive type 1 yplane -0.02
+fix ywalls2 all wall/gran model hertz tangential history primitive type 1 yplane +0.02
+fix zwalls1 all wall/gran model hertz tangential history primitive type 1 zplane 0.00
+fix zwalls2 all wall/gran model hertz tangential history primitive type 1 zplane 0.06
+fix pts1 all particletemplate/sphere 15485863 atom_type 1 density constant 2500 radius constant 0.001
+fix pts2 all particletemplate/sphere 15485867 atom_type 1 density constant 2500 radius constant 0.0016
+fix pdd1 all particledistribution/discrete 32452843 2 pts1 0.3 pts2 0.7
+variable alphastart equal 0.25
+variable alphatarget equal 0.67
+variable growts equal 50000
+variable growevery equal 40
+group nve_group region reg
+fix ins nve_group insert/pack seed 32452867 distributiontemplate pdd1 &
+    maxattempt 200 insert_every once overlapcheck yes all_in yes vel constant 0. 0. 0. &
+    region reg volumefraction_region ${alphastart}
+fix integr nve_group nve/sphere
+compute 1 all erotate/sphere
+thermo_style custom step atoms ke c_1 vol
+thermo 1000
+thermo_modify lost ignore norm no
+variable Rgrowrate equal (${alphatarget}/${alphastart})^(${growevery}/(3.*${growts}))
+print "The radius grow rate is ${Rgrowrate}"
+"""
+
+
+def test_insertion_script_parses(tmp_path):
+    rc, log = run_script(tmp_path, PACKING_SCRIPT)
+    assert rc == 0, log
+    rate = (0.67 / 0.25) ** (40 / (3.0 * 50000))
+    assert f"The radius grow rate is {rate:.15g}" in log
+
+
+@pytest.mark.parametrize("expr,value", [
+    ("-2^2", 4.0), ("2^3^2", 64.0), ("1+2*3", 7.0), ("(1+2)*3", 9.0), ("7%4", 3.0),
+    ("sqrt(16)+ln(1)+log(100)", 6.0), ('"2<3 && 3<2"', 0.0), ('"2<3 || 3<2"', 1.0),
+    ("atan2(1,1)*4", 3.14159265358979),
+    ("v_a*2", 5.0),
+])
+def test_equal_style_formulas(tmp_path, expr, value):
+    """Variable::evaluate's precedence (variable.cpp:132-140): unary minus
+    binds tighter than ^, every binary operator is left-associative."""
+    rc, log = run_script(tmp_path, f"variable a equal 2.5\nvariable f equal {expr}\nprint \"f=${{f}}\"\n")
+    assert rc == 0, log
+    got = float(log.strip().split("f=")[-1].split()[0])
+    assert abs(got - value) <= 1e-12 * max(1.0, abs(value))
+
+
+@pytest.mark.parametrize("bad,msg", [
+    ("fix ins all insert/pack seed 32452867 distributiontemplate pdd1 insert_every once region reg "
+     "volumefraction_region 0.3 particles_in_region 10\n", None),
+    ("fix ins all insert/pack seed 32452843 distributiontemplate pdd1 insert_every once region reg "
+     "volumefraction_region 0.3\n", "different"),
+    ("fix ins all insert/pack seed 32452867 distributiontemplate pdd9 insert_every once region reg "
+     "volumefraction_region 0.3\n", "particledistribution/discrete"),
+    ("fix ins all insert/pack seed 32452867 distributiontemplate pdd1 region reg volumefraction_region 0.3\n",
+     "insert_every"),
+    ("unfix zwalls1\n", "only insertion fixes"),
+    ("variable q equal 1/0\nprint \"${q}\"\n", "Divide by 0"),
+    ("variable q equal 1 + 2\n", "Illegal variable command"),
+])
+def test_insertion_script_errors(tmp_path, bad, msg):
+    text = PACKING_SCRIPT.split("fix ins nve_group")[0] + bad
+    rc, log = run_script(tmp_path, text)
+    if msg is None:  # exactly one target is checked at the first run (FixInsertPack::calc_insertion_properties)
+        assert rc == 0, log
+        return
+    assert rc != 0
+    assert msg in log
