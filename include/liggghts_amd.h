@@ -322,6 +322,9 @@ int dem_compute_forces(dem_handle *h, int shearupdate);
  * never inside its created position (Domain::reset_box, domain.cpp:432-449). */
 int dem_get_box(dem_handle *h, double *boxlo, double *boxhi);
 
+/* Per-particle state in the caller's order ([n][3] each; f and torque of the
+ * last force pass). A system without particles yet (dem_insert_particles
+ * pending) has nothing to copy and returns DEM_OK. */
 int dem_get_state(dem_handle *h, double *x, double *v, double *omega, double *f,
                   double *torque);
 

@@ -1169,7 +1169,9 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i, co
           const double hy = y - a.xhold[cap + i];
           const double hz = z - a.xhold[2 * cap + i];
           const double rsq = hx * hx + hy * hy + hz * hz;
-          displaced = rsq > p.triggersq;
+          // the lane may hold several tiles (persistent grid): keep a
+          // request an earlier tile made
+          displaced = displaced || rsq > p.triggersq;
         } else {
           displaced = true;
         }
@@ -1510,9 +1512,82 @@ __global__ __launch_bounds__(kBlock) void k_gather(RebuildArgs a) {
     }
 }
 
+// Flagged partners of a particle's old list row that the build looks up by
+// binary search: up to kCarrySlots per particle, in LDS, keyed by the
+// partner's index in the old particle order (ascending), with the old list
+// slot beside it. A row with more flagged entries falls back to scanning the
+// old row in global memory.
+constexpr int kCarrySlots = 16;
+
+struct CarryLds {
+  uint32_t key[kCarrySlots][kBlock];
+  uint16_t slot[kCarrySlots][kBlock];
+};
+
+// Loads the flagged entries of old row io (oldcnt entries) into this lane's
+// sorted LDS column; returns their number, or -1 when they do not fit.
+__device__ __forceinline__ int carry_table(const RebuildArgs &a, const size_t cap, const int io,
+                                           const int oldcnt, CarryLds &L, const int t) {
+  int nf = 0;
+  for (int m0 = 0; m0 < oldcnt; m0 += 4) {
+    uint32_t oe[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) oe[u] = (m0 + u < oldcnt) ? a.oldl.nbr[(size_t)(m0 + u) * cap + io] : 0u;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (!(oe[u] & kFlagBit)) continue;
+      if (nf == kCarrySlots) return -1;
+      // insertion into the ascending key column
+      const uint32_t key = oe[u] & kIndexMask;
+      int q = nf;
+      while (q > 0 && L.key[q - 1][t] > key) {
+        L.key[q][t] = L.key[q - 1][t];
+        L.slot[q][t] = L.slot[q - 1][t];
+        q--;
+      }
+      L.key[q][t] = key;
+      L.slot[q][t] = (uint16_t)(m0 + u);
+      nf++;
+    }
+  }
+  return nf;
+}
+
+// Old list slot of flagged partner jo, or -1: binary search of the LDS column
+// (nf >= 0) or a scan of the old row (nf < 0).
+__device__ __forceinline__ int carry_find(const RebuildArgs &a, const size_t cap, const int io, const int oldcnt,
+                                          const CarryLds &L, const int t, const int nf, const uint32_t jo) {
+  if (nf >= 0) {
+    int lo = 0, hi = nf;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (L.key[mid][t] < jo)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    return (lo < nf && L.key[lo][t] == jo) ? (int)L.slot[lo][t] : -1;
+  }
+  for (int m = 0; m < oldcnt; m++) {
+    const uint32_t oe = a.oldl.nbr[(size_t)m * cap + io];
+    if ((oe & kFlagBit) && (oe & kIndexMask) == jo) return m;
+  }
+  return -1;
+}
+
+// Neighbor::granular_bin_no_newton (neigh_gran.cpp:485-639) as a full list,
+// one thread per particle: the reference's predicate rsq <= ((ri+rj)*cdf +
+// skin)^2 over the 27-cell stencil in (z, y, x) cell order, j ascending within
+// a cell; history carry-in for pairs within (ri+rj)*cdf (neigh_gran.cpp:
+// 585-607: a flagged partner of the old row passes its history and flag 1).
+// The 27 cell ranges are loaded together up front, the candidates of each
+// stencil row (three cells in x) are tested in batches of four independent
+// position loads, and old partners are found by binary search in LDS.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_build(RebuildArgs a) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  __shared__ CarryLds carry;
+  const int t = threadIdx.x;
+  const int i = blockIdx.x * kBlock + t;
   const bool active = i < a.n;
   int cnt = 0;
   if (active) {
@@ -1535,47 +1610,71 @@ __global__ __launch_bounds__(kBlock) void k_build(RebuildArgs a) {
         hi[d] = ci[d] < p.nc[d] - 1 ? 1 : 0;
       }
     }
+    // cell ranges of the stencil, cell c = (dz+1)*9 + (dy+1)*3 + (dx+1)
+    int rk[27];
+#pragma unroll
+    for (int c = 0; c < 27; c++) {
+      const int dz = c / 9 - 1, dy = (c / 3) % 3 - 1, dx = c % 3 - 1;
+      const bool ok = dz >= lo[2] && dz <= hi[2] && dy >= lo[1] && dy <= hi[1] && dx >= lo[0] && dx <= hi[0];
+      const int cx = (ci[0] + dx + p.nc[0]) % p.nc[0];
+      const int cy = (ci[1] + dy + p.nc[1]) % p.nc[1];
+      const int cz = (ci[2] + dz + p.nc[2]) % p.nc[2];
+      rk[c] = ok ? cbase + a.cell_rank[cx + p.nc[0] * (cy + p.nc[1] * cz)] : -1;
+    }
+    int cs[27], cn[27];
+#pragma unroll
+    for (int c = 0; c < 27; c++) {
+      const int r = rk[c] < 0 ? 0 : rk[c];
+      const int s0 = a.cell_start[r], s1 = a.cell_start[r + 1];
+      cs[c] = s0;
+      cn[c] = rk[c] < 0 ? 0 : s1 - s0;
+    }
+    const int nf = oldcnt > 0 ? carry_table(a, cap, io, oldcnt, carry, t) : 0;
     const T *ohist = (const T *)a.oldl.hist;
     T *nhist = (T *)a.newl.hist;
-    for (int dz = lo[2]; dz <= hi[2]; dz++)
-      for (int dy = lo[1]; dy <= hi[1]; dy++)
-        for (int dx = lo[0]; dx <= hi[0]; dx++) {
-          const int cx = (ci[0] + dx + p.nc[0]) % p.nc[0];
-          const int cy = (ci[1] + dy + p.nc[1]) % p.nc[1];
-          const int cz = (ci[2] + dz + p.nc[2]) % p.nc[2];
-          const int rank = cbase + a.cell_rank[cx + p.nc[0] * (cy + p.nc[1] * cz)];
-          const int s = a.cell_start[rank], e = a.cell_start[rank + 1];
-          for (int j = s; j < e; j++) {
-            if (j == i) continue;
-            const double4 pj = a.alt.pos[j];
-            double ddx, ddy, ddz;
-            min_image(p, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, ddx, ddy, ddz);
-            const double rsq = ddx * ddx + ddy * ddy + ddz * ddz;
-            const double radsum = (pi.w + pj.w) * p.cdf;
-            const double cutsq = (radsum + p.skin) * (radsum + p.skin);
-            if (!(rsq <= cutsq)) continue;
-            if (cnt < a.W) {
-              uint32_t entry = (uint32_t)j;
-              if (rsq < radsum * radsum && oldcnt > 0) {
-                // carry-in: neigh_gran.cpp:585-607 (partner found -> flag 1, copy)
-                const uint32_t jo = (uint32_t)a.perm[j];
-                for (int m = 0; m < oldcnt; m++) {
-                  const uint32_t oe = a.oldl.nbr[(size_t)m * cap + io];
-                  if ((oe & kFlagBit) && (oe & kIndexMask) == jo) {
-                    const size_t hw = (size_t)p.hist_w;
-                    const size_t ob = (size_t)m * hw * cap + io;
-                    const size_t nb = (size_t)cnt * hw * cap + i;
-                    for (size_t d = 0; d < hw; d++) nhist[nb + d * cap] = ohist[ob + d * cap];
-                    entry |= kFlagBit | kCarryBit;
-                    break;
-                  }
-                }
-              }
-              a.newl.nbr[(size_t)cnt * cap + i] = entry;
-            }
-            cnt++;
-          }
+    const size_t hw = (size_t)p.hist_w;
+#pragma unroll
+    for (int row = 0; row < 9; row++) {
+      const int s0 = cs[3 * row], s1 = cs[3 * row + 1], s2 = cs[3 * row + 2];
+      const int n0 = cn[3 * row], n01 = n0 + cn[3 * row + 1], nt = n01 + cn[3 * row + 2];
+      for (int k0 = 0; k0 < nt; k0 += 4) {
+        int jj[4];
+        double4 pj[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int k = k0 + u;
+          const int j = k < n0 ? s0 + k : (k < n01 ? s1 + (k - n0) : s2 + (k - n01));
+          jj[u] = k < nt ? j : i;
         }
+#pragma unroll
+        for (int u = 0; u < 4; u++) pj[u] = a.alt.pos[jj[u]];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int j = jj[u];
+          if (j == i) continue;  // i itself, or past the row's candidates
+          double ddx, ddy, ddz;
+          min_image(p, pi.x, pi.y, pi.z, pj[u].x, pj[u].y, pj[u].z, ddx, ddy, ddz);
+          const double rsq = ddx * ddx + ddy * ddy + ddz * ddz;
+          const double radsum = (pi.w + pj[u].w) * p.cdf;
+          const double cutsq = (radsum + p.skin) * (radsum + p.skin);
+          if (!(rsq <= cutsq)) continue;
+          if (cnt < a.W) {
+            uint32_t entry = (uint32_t)j;
+            if (rsq < radsum * radsum && oldcnt > 0) {
+              const int m = carry_find(a, cap, io, oldcnt, carry, t, nf, (uint32_t)a.perm[j]);
+              if (m >= 0) {
+                const size_t ob = (size_t)m * hw * cap + io;
+                const size_t nb = (size_t)cnt * hw * cap + i;
+                for (size_t d = 0; d < hw; d++) nhist[nb + d * cap] = ohist[ob + d * cap];
+                entry |= kFlagBit | kCarryBit;
+              }
+            }
+            a.newl.nbr[(size_t)cnt * cap + i] = entry;
+          }
+          cnt++;
+        }
+      }
+    }
     const int stored = cnt < a.W ? cnt : a.W;
     a.aalt.meta[i] = (a.aalt.meta[i] & ~kMetaCountMask) | stored;
     if (cnt > a.W) atomicMax(&a.ctrl->overflow, cnt);

@@ -790,10 +790,13 @@ int needed_hist_width(const dem_handle *h) {
   return epsd2 ? 6 : 3;
 }
 
-int check_ready(dem_handle *h) {
+// Uploads the material tables the step kernel reads and checks that every
+// model finds its properties. need_particles = false: the set-up of a system
+// whose particles are still to be inserted.
+int check_ready(dem_handle *h, bool need_particles = true) {
   if (!h) return DEM_ERR_ARG;
   if (!h->mat_set) return fail(h, DEM_ERR_STATE, "materials not set (fix property/global)");
-  if (h->n <= 0) return fail(h, DEM_ERR_STATE, "no particles");
+  if (need_particles && h->n <= 0) return fail(h, DEM_ERR_STATE, "no particles");
   int rc = upload_materials(h);
   if (rc) return rc;
   for (const Wall &w : h->walls)
@@ -828,15 +831,18 @@ int check_ready(dem_handle *h) {
   for (const auto &m : h->meshes)
     if (m.model & dem::kModelEpsd2)
       return fail(h, DEM_ERR_ARG, "rolling_friction epsd2 is not available on mesh walls on this path");
-  if (needed_hist_width(h) != h->hw) return fail(h, DEM_ERR_STATE, "contact history width out of date");
+  if (h->n > 0 && needed_hist_width(h) != h->hw)
+    return fail(h, DEM_ERR_STATE, "contact history width out of date");
   return DEM_OK;
 }
 
 int do_setup(dem_handle *h) {
   if (h && h->n == 0) {
     // Verlet::setup of a system whose particles are still to be inserted:
-    // nothing to build or compute yet
-    if (!h->mat_set) return fail(h, DEM_ERR_STATE, "materials not set (fix property/global)");
+    // nothing to build or compute yet, but the material tables go to the
+    // device now (the particles that join later are stepped with them)
+    int rc = check_ready(h, false);
+    if (rc) return rc;
     h->setup_done = true;
     h->can_continue = true;
     return DEM_OK;
@@ -916,9 +922,14 @@ int append_pending(dem_handle *h) {
   std::swap(q, h->pend);
   const int64_t m = (int64_t)q.tag.size();
   if (m == 0) return DEM_OK;
-  if (h->n == 0)  // the handle's first particles
-    return dem_set_particles(h, m, q.x.data(), q.v.data(), q.w.data(), q.r.data(), q.m.data(),
-                             q.type.data(), q.tag.data());
+  if (h->n == 0) {  // the handle's first particles
+    int rc = dem_set_particles(h, m, q.x.data(), q.v.data(), q.w.data(), q.r.data(), q.m.data(),
+                               q.type.data(), q.tag.data());
+    if (rc) return rc;
+    // the step kernel reads the material tables: make sure they are on the
+    // device whatever happened since the set-up
+    return check_ready(h);
+  }
   const int64_t n0 = h->n, n1 = n0 + m;
   if (n1 > 0x7fffffffLL / 2) return fail(h, DEM_ERR_ARG, "too many particles");
   int rc;
@@ -1966,7 +1977,7 @@ int dem_get_box(dem_handle *h, double *boxlo, double *boxhi) {
 
 int dem_get_state(dem_handle *h, double *x, double *v, double *omega, double *f, double *torque) {
   if (!h) return DEM_ERR_ARG;
-  if (h->n <= 0) return fail(h, DEM_ERR_STATE, "no particles");
+  if (h->n <= 0) return DEM_OK;  // an empty system (particles still to be inserted): nothing to copy
   HIPCHK(h, hipStreamSynchronize(h->stream));
   const int64_t n = h->n;
   const size_t cap = (size_t)h->cap;

@@ -715,7 +715,8 @@ struct Interp {
     if (!any) return;
     const size_t n = tag.size();
     std::vector<double> f(3 * n), tq(3 * n);
-    if (dem_get_state(h, x.data(), v.data(), w.data(), f.data(), tq.data())) die(dem_last_error(h));
+    // an empty system (its particles still to be inserted) dumps no atoms
+    if (n > 0 && dem_get_state(h, x.data(), v.data(), w.data(), f.data(), tq.data())) die(dem_last_error(h));
     std::vector<size_t> order(n);
     for (size_t i = 0; i < n; i++) order[i] = i;
     std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return tag[a] < tag[b]; });

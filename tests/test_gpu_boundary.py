@@ -1,6 +1,6 @@
 """GPU checks of the non-periodic boundary faces against the oracle:
 
-  f face   a sphere thrown out of a dense pack through the upper x face:
+  f face   a sphere thrown out of a loose pack through the upper x face:
            the run stops with "Lost atoms: original N current N-1" at the
            step whose neighbor build sees it beyond the face, the oracle's
            first loss step; the state before that step equals the oracle's
@@ -29,7 +29,10 @@ def say(msg):
 
 
 def _throw(faces=(0, 0)):
-    params, mats, P, walls = wl.dense_pack(seed=51, n=1000, rolling=True, periodic=(0, 1, 0))
+    # lattice spacing above the largest diameter: no pair overlaps at the
+    # start, so only the thrown sphere reaches an x face (a compressed pack
+    # would push its outer layers through them)
+    params, mats, P, walls = wl.dense_pack(seed=51, n=1000, rolling=True, periodic=(0, 1, 0), overlap=-0.25)
     P = {k: v.copy() for k, v in P.items()}
     lo, hi = params["boxlo"], params["boxhi"]
     # the particle nearest the upper x face leaves at 3 m/s
@@ -61,7 +64,7 @@ def test_f_face_loses_the_atom_at_the_oracles_step():
         step += 1
         prev = g.state()
     say(f"  gpu: error at step {step + 1}: {err}")
-    assert err is not None and err.startswith(f"Lost atoms: original {len(P['tag'])} current {len(P['tag']) - 1}")
+    assert err is not None and f"Lost atoms: original {len(P['tag'])} current {len(P['tag']) - 1}" in err
     assert step + 1 == first
     # the state up to the step before agrees with the oracle's
     o2 = orc.OracleSystem(params, mats, P, walls)

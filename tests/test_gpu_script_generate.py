@@ -41,10 +41,12 @@ fix gravi all gravity 9.81 vector 0.0 0.0 -1.0
 """
 
 
-def _bed(n=600):
+def _bed(n=600, z0=0.00215):
+    """Spheres r = 2 mm on a 4.5 mm lattice; by default the lowest layer starts
+    0.05-0.25 mm above z = 0 and lands there within the run."""
     rng = np.random.default_rng(8)
     r = np.full(n, 0.002)
-    x = wl.lattice(n, (0.02, 0.02, 0.0), (0.08, 0.08, 0.1), 0.0045, 0.0002, rng, z0=0.004)
+    x = wl.lattice(n, (0.02, 0.02, 0.0), (0.08, 0.08, 0.1), 0.0045, 0.0001, rng, z0=z0)
     P = {"x": x, "v": np.zeros((n, 3)), "omega": np.zeros((n, 3)), "radius": r,
          "rmass": wl.sphere_mass(r, 2500.0), "type": np.ones(n, dtype=np.int32),
          "tag": np.arange(1, n + 1, dtype=np.int32)}
@@ -88,7 +90,9 @@ run 800
 
 def test_generated_closed_cylinder_holds_the_bed(tmp_path):
     say("[gpu] script: fix mesh/surface generate cylinder (closed)")
-    P = _bed(400)
+    # starting 2 mm above the bottom cap: near the cap's centre every one of
+    # its 48 fan triangles would be a candidate (the limit is 32 per particle)
+    P = _bed(400, z0=0.004)
     params = {"boxlo": (0.0, 0.0, -0.01), "boxhi": (0.1, 0.1, 0.1)}
     data = tmp_path / "bed.data"
     wl.write_data_file(str(data), params, P)

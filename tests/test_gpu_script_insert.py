@@ -83,22 +83,23 @@ run 200
 
 
 def test_repeated_insertion_script_equals_the_api(tmp_path):
-    """insert_every 50 with particles_in_region topping a region up, velocity
-    constant, through `run ... upto`: insertions at steps 1, 51, 101, 151."""
+    """insert_every 500 with particles_in_region topping a region up (the
+    spheres leave it downwards at 1 m/s between insertions), velocity
+    constant, through `run ... upto`: insertions at steps 1, 501, 1001, 1501."""
     say("[gpu] script: repeated insertion, particles_in_region, run upto")
     head = PACKING_SCRIPT.split("variable alphastart")[0].replace(
         "fix pdd1 all particledistribution/discrete 32452843 2 pts1 0.3 pts2 0.7",
         "fix pdd1 all particledistribution/discrete 32452843 1 pts2 1.0")
     script = tmp_path / "in.stream"
-    script.write_text(head + f"""region bc block -0.018 0.018 -0.018 0.018 0.03 0.058 units box
+    script.write_text(head + f"""region bc block -0.018 0.018 -0.018 0.018 0.045 0.058 units box
 fix gravi all gravity 9.81 vector 0.0 0.0 -1.0
-fix ins all insert/pack seed 32452867 distributiontemplate pdd1 vel constant 0. 0. -0.3 &
-    insert_every 50 overlapcheck yes all_in yes particles_in_region 150 region bc
+fix ins all insert/pack seed 32452867 distributiontemplate pdd1 vel constant 0. 0. -1.0 &
+    insert_every 500 overlapcheck yes all_in yes particles_in_region 150 region bc
 fix integr all nve/sphere
 thermo_style custom step atoms ke
-thermo 40
-dump d1 all custom 160 {tmp_path}/st_*.dump id x y z vx vy vz
-run 160 upto
+thermo 500
+dump d1 all custom 1510 {tmp_path}/st_*.dump id x y z vx vy vz
+run 1510 upto
 """)
     log = tmp_path / "log.stream"
     rc = pydem.run_script(str(script), str(log))
@@ -106,10 +107,10 @@ run 160 upto
     say(text[-1200:])
     assert rc == 0, text
     params, mats, walls = api_system()
-    reg = ((-0.018, -0.018, 0.03), (0.018, 0.018, 0.058))
+    reg = ((-0.018, -0.018, 0.045), (0.018, 0.018, 0.058))
     ins = pydem.Inserter(templates=[TEMPLATES[1]], weights=[1.0], dist_seed=32452843, seed=32452867,
                          all_in=1, particles_in_region=150, region=reg, box=(LO, HI), box_change=1,
-                         v_insert=(0.0, 0.0, -0.3))
+                         v_insert=(0.0, 0.0, -1.0))
     g = pydem.GpuSystem(params, mats, None, walls)
     g.set_max_radius(0.0016)
     g.run(0)
@@ -117,7 +118,7 @@ run 160 upto
     m = np.zeros(0)
     tags = np.zeros(0, dtype=np.int32)
     counts = []
-    for s in (1, 51, 101, 151):
+    for s in (1, 501, 1001, 1501):
         st = g.state()
         box = g.box() if len(r) else (LO, HI)
         out, req = ins.step(st["x"], r, m, box=box)
@@ -129,10 +130,10 @@ run 160 upto
         r = np.concatenate([r, P["radius"]])
         m = np.concatenate([m, P["rmass"]])
         g.insert(P)
-        g.run_continue(50 if s < 151 else 10)
+        g.run_continue(500 if s < 1501 else 10)
     say(f"API insertions: {counts}")
     assert counts[0] == 150 and sum(counts) > 150
     for c in counts:
         assert f"inserted {c} particle templates" in text
-    compare_dump(read_dump(tmp_path / "st_160.dump"), g.state(), tags)
+    compare_dump(read_dump(tmp_path / "st_1510.dump"), g.state(), tags)
     g.close()
