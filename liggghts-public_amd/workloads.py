@@ -105,9 +105,12 @@ def dense_pack(seed=0, n=2000, rolling=True, overlap=0.02, box=None, periodic=(1
     x = lattice(n, lo, hi, spacing, 0.05 * spacing, rng, z0=lo[2] + 0.5 * spacing)
     v = rng.normal(0.0, speed, size=(n, 3))
     w = rng.normal(0.0, spin, size=(n, 3))
+    # boundary m on the non-periodic faces (as the packing tutorial): the
+    # compressed pack spreads, and an f face would lose the spheres it pushes out
+    faces = sum(3 << (2 * d) for d in range(3) if not periodic[d])
     params = {"dt": 5e-6, "skin": 0.0005, "cdf": 1.0, "boxlo": lo, "boxhi": hi,
               "periodic": periodic, "grav_acc": (0.0, 0.0, -9.81), "gravity_first": 1,
-              "rolling_cdt": int(rolling)}
+              "rolling_cdt": int(rolling), "shrink_faces": faces, "shrink_min": faces}
     particles = {"x": x, "v": v, "omega": w, "radius": r,
                  "rmass": sphere_mass(r, 2500.0), "type": np.ones(n, dtype=np.int32),
                  "tag": np.arange(1, n + 1, dtype=np.int32)}
