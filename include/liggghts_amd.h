@@ -340,6 +340,24 @@ int dem_get_pairs(dem_handle *h, int64_t cap, int *tags, int *flags, double *his
  * (FixWallGran's property/atom history, fix_wall_gran.cpp:498-522). */
 int dem_get_wall_history(dem_handle *h, double *hist);
 
+/* compute pair/gran/local (compute_pair_gran_local.cpp:81-110, 312-470) as
+ * evaluated through PairGran::compute_pgl (pair_gran.cpp:676-708): one pass
+ * over the current half neighbor list with shearupdate off, on the GPU,
+ * changing nothing. Every half-list pair gets a row (cpl_add_pair is called
+ * for each pair of the list, pair_gran_base.h:446-447): tags[2k] < tags[2k+1],
+ * and data[k * ncol ...] = x_i[3], x_j[3] (remapped into the periodic box),
+ * v_i[3], v_j[3], the force and the torque the contact exerts on particle i
+ * (the lower tag; zero when the pair does not touch), 1 if the pair crosses a
+ * periodic boundary else 0, r_i, r_j, then the pair's stored history oriented
+ * from i to j (3 shear values, and 3 rolling-torque values with epsd2). *ncol
+ * receives 21 + the history width. Rows of one particle are consecutive; *npairs
+ * receives the number of pairs, min(cap, *npairs) rows are written (cap 0:
+ * count only). Deviation: the reference orients a pair by its local atom
+ * indices, this path by tags. */
+#define DEM_PAIR_LOCAL_FIXED_COLS 21
+int dem_compute_pair_local(dem_handle *h, int64_t cap, int *tags, double *data, int64_t *npairs,
+                           int *ncol);
+
 /* The rolling-torque history of rolling_friction epsd2 (the r_torque{x,y,z}_old
  * values RollingModel<EPSD2> registers, rolling_model_epsd2.h:75-77), for the
  * pairs of dem_get_pairs in the same order and orientation (rhist[3k..3k+2]),

@@ -254,10 +254,25 @@ struct RebuildArgs {
   int zero_vnode;              // setup: node velocities start at zero
 };
 
+// compute pair/gran/local: per particle the number of its half-list pairs
+// (those whose partner has a higher tag), their offsets, and per pair the two
+// tags and a record of ncol doubles: x_i[3], x_j[3], v_i[3], v_j[3], force on
+// i[3], torque on i[3], periodic-image flag, r_i, r_j, history[hist_w]
+constexpr int kPairLocalFixedCols = 21;
+struct PairLocalArgs {
+  int *count;      // [n]
+  int *offset;     // [n + 1]
+  int *tags;       // [npairs][2]
+  double *rec;     // [npairs][ncol]
+  int ncol;
+};
+
 // launchers (dem_kernels.hip)
 hipError_t launch_rebuild(const RebuildArgs &a, hipStream_t s);
 hipError_t launch_step(const StepArgs &a, hipStream_t s);
 hipError_t launch_initial(const StepArgs &a, hipStream_t s);
+hipError_t launch_pair_local_count(const StepArgs &a, const PairLocalArgs &o, hipStream_t s);
+hipError_t launch_pair_local_fill(const StepArgs &a, const PairLocalArgs &o, hipStream_t s);
 // stop_at = v, run counters cleared (and the per-instance ones, nstat entries)
 hipError_t launch_set_stop(Ctrl *ctrl, long long v, hipStream_t s,
                            unsigned long long *inst_stats = nullptr, int nstat = 0);

@@ -711,7 +711,9 @@ __device__ __forceinline__ void close_entries(const StepArgs &a, const Model &pm
 // One touching list entry (k, e) of particle i: pair_gran_base.h:356-430 with
 // the contact chain, the history read / write of the entry and its flag.
 // Returns the contribution to i's force and torque.
-template <typename T, bool GEN, typename W>
+// STORE = false: the evaluation of compute pair/gran/local, which writes
+// neither history nor flag.
+template <typename T, bool GEN, typename W, bool STORE = true>
 __device__ __forceinline__ void pair_contact(const StepArgs &a, const Model &pm, const int i,
                                              const int k, const uint32_t e, const double4 &pi,
                                              const double4 &vi, const W &wi, const int irow,
@@ -793,7 +795,8 @@ __device__ __forceinline__ void pair_contact(const StepArgs &a, const Model &pm,
     sh[0] = sh[1] = sh[2] = T(0);
   }
   T dTj[3];
-  contact_chain<T, false, GEN>(pm, c, (T)p.dt, a.shearupdate, sh, rh, dF, dTi, dTj);
+  contact_chain<T, false, GEN>(pm, c, (T)p.dt, STORE ? a.shearupdate : 0, sh, rh, dF, dTi, dTj);
+  if (!STORE) return;
   if ((a.shearupdate && keeps) || !flagged) {
     hist[hb] = sh[0];
     hist[hb + cap] = sh[1];
@@ -866,12 +869,15 @@ struct WaveLds {
 
 // Pair forces of the wavefront's 64 particles with the touching entries
 // compacted: each group of GROUP list entries per particle is broad-phased by
-// its own lane, the touching entries of all lanes are ranked by a ballot
-// prefix sum (lane order, list order within a lane) into an LDS queue, and the
-// whole wavefront runs the contact law on 64 queued entries at a time; each
-// particle then adds its entries' contributions in list order, so the sums are
-// bitwise those of the per-lane path. Every lane of the wavefront takes part;
-// lanes past the last particle have cnt = 0.
+// its own lane, the touching entries of all lanes are ranked by ballot prefix
+// sums into an LDS queue slot-major (entry u of every lane, in lane order,
+// before entry u+1 of any lane), and the whole wavefront runs the contact law
+// on 64 queued entries at a time. Slot-major order puts the entries of
+// neighboring particles at the same list slot side by side, so the history
+// and owner accesses of one pass fall in a few cache lines. Each particle then
+// adds its entries' contributions in list order (its queue positions rise with
+// the slot), so the sums are bitwise those of the per-lane path. Every lane of
+// the wavefront takes part; lanes past the last particle have cnt = 0.
 template <typename T, bool GEN, int GROUP>
 __device__ __forceinline__ void pair_forces_wave(const StepArgs &a, const Model &pm, const int i,
                                                  const int pad, const int cnt, const double4 &pi,
@@ -893,22 +899,20 @@ __device__ __forceinline__ void pair_forces_wave(const StepArgs &a, const Model 
   for (int k0 = 0; k0 < kmax; k0 += GROUP) {
     uint32_t ent[GROUP];
     const uint32_t bits = broad_phase<T, GEN, GROUP>(a, pm, i, pad, cnt, k0, pi, ent);
-    // rank of this lane's first touching entry among the wavefront's
-    int first = 0, total = 0;
+    // queue position of this lane's entry u: the touching entries of slots
+    // below u, then those of lower lanes at slot u
+    int pos[GROUP];
+    int total = 0;
 #pragma unroll
     for (int u = 0; u < GROUP; u++) {
       const unsigned long long m = __ballot((bits >> u) & 1u);
-      first += __popcll(m & below);
+      pos[u] = total + __popcll(m & below);
       total += __popcll(m);
     }
-    const int mine = __popc(bits);
-    {
-      int r = first;
 #pragma unroll
-      for (int u = 0; u < GROUP; u++)
-        if ((bits >> u) & 1u) L.queue[r++] = make_uint2(ent[u], ((uint32_t)lane << 16) | (uint32_t)(k0 + u));
-    }
-    my_contacts += (unsigned long long)mine;
+    for (int u = 0; u < GROUP; u++)
+      if ((bits >> u) & 1u) L.queue[pos[u]] = make_uint2(ent[u], ((uint32_t)lane << 16) | (uint32_t)(k0 + u));
+    my_contacts += (unsigned long long)__popc(bits);
     wave_sync();
     for (int c0 = 0; c0 < total; c0 += 64) {
       const int q = c0 + lane;
@@ -931,16 +935,18 @@ __device__ __forceinline__ void pair_forces_wave(const StepArgs &a, const Model 
       L.contrib[4][lane] = dTi[1];
       L.contrib[5][lane] = dTi[2];
       wave_sync();
-      const int s = first > c0 ? first : c0;
-      const int e = (first + mine) < (c0 + 64) ? (first + mine) : (c0 + 64);
-      for (int q2 = s; q2 < e; q2++) {
-        const int l = q2 - c0;
-        fx += L.contrib[0][l];
-        fy += L.contrib[1][l];
-        fz += L.contrib[2][l];
-        tx += L.contrib[3][l];
-        ty += L.contrib[4][l];
-        tz += L.contrib[5][l];
+      // this pass's entries of this particle, in list order
+#pragma unroll
+      for (int u = 0; u < GROUP; u++) {
+        const int l = pos[u] - c0;
+        if (((bits >> u) & 1u) && l >= 0 && l < 64) {
+          fx += L.contrib[0][l];
+          fy += L.contrib[1][l];
+          fz += L.contrib[2][l];
+          tx += L.contrib[3][l];
+          ty += L.contrib[4][l];
+          tz += L.contrib[5][l];
+        }
       }
       wave_sync();
     }
@@ -1261,6 +1267,89 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a) {
       if (tc) atomicAdd(&a.ctrl->ncontact_full, tc);
       if (tw) atomicAdd(&a.ctrl->nwall, tw);
     }
+  }
+}
+
+// compute pair/gran/local (compute_pair_gran_local.cpp:312-470) through
+// PairGran::compute_pgl (pair_gran.cpp:676-708): one pass over the half list
+// with shearupdate off that records, per pair, the force and torque the
+// contact puts on its first particle and the pair's stored history
+// (PairGran::cpl_add_pair, pair_gran.cpp:798-813, is called for every pair of
+// the list; pairs that do not touch get zero force). Each half-list pair is
+// taken from the row of its lower-tag particle, whose history entry is
+// oriented from that particle to its partner. Count pass, then fill pass at
+// the scanned offsets: a particle's pairs are consecutive, in list order.
+__global__ __launch_bounds__(kBlock) void k_pair_local_count(StepArgs a, PairLocalArgs o) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const int cnt = a.aux.meta[i] & kMetaCountMask;
+  const int ti = a.aux.tag[i];
+  int c = 0;
+  for (int k = 0; k < cnt; k++) {
+    const int j = (int)(a.list.nbr[(size_t)k * a.cap + i] & kIndexMask);
+    if (ti < a.aux.tag[j]) c++;
+  }
+  o.count[i] = c;
+}
+
+template <typename T, bool GEN>
+__global__ __launch_bounds__(kBlock) void k_pair_local_fill(StepArgs a, PairLocalArgs o) {
+  typedef typename OmgT<T>::type omg_t;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const DevParams &p = a.prm;
+  const size_t cap = (size_t)a.cap;
+  const size_t hw = (size_t)p.hist_w;
+  const double4 pi = a.in.pos[i];
+  const double4 vi = a.in.vel[i];
+  const omg_t wi = ((const omg_t *)a.in.omg)[i];
+  const int wbits = (int)wi.w;
+  const int irow = (wbits >> kTypeBits) * p.mat_stride + (wbits & kTypeMask) * p.ntp1;
+  const Model pm = model_from_bits(p.pair_model);
+  const int cnt = a.aux.meta[i] & kMetaCountMask;
+  const int ti = a.aux.tag[i];
+  const T *hist = (const T *)a.list.hist;
+  int64_t r = o.offset[i];
+  for (int k = 0; k < cnt; k++) {
+    const uint32_t e = a.list.nbr[(size_t)k * cap + i];
+    const int j = (int)(e & kIndexMask);
+    const int tj = a.aux.tag[j];
+    if (!(ti < tj)) continue;
+    const double4 pj = a.in.pos[j];
+    const double4 vj = a.in.vel[j];
+    double dx, dy, dz;
+    min_image(p, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, dx, dy, dz);
+    const double rsq = dx * dx + dy * dy + dz * dz;
+    const double radsum = pi.w + pj.w;
+    T dF[3] = {T(0), T(0), T(0)}, dTi[3] = {T(0), T(0), T(0)};
+    if (rsq < radsum * radsum) pair_contact<T, GEN, omg_t, false>(a, pm, i, k, e, pi, vi, wi, irow, dF, dTi);
+    double *rec = o.rec + (size_t)r * (size_t)o.ncol;
+    rec[0] = pi.x;
+    rec[1] = pi.y;
+    rec[2] = pi.z;
+    rec[3] = pj.x;
+    rec[4] = pj.y;
+    rec[5] = pj.z;
+    rec[6] = vi.x;
+    rec[7] = vi.y;
+    rec[8] = vi.z;
+    rec[9] = vj.x;
+    rec[10] = vj.y;
+    rec[11] = vj.z;
+    for (int d = 0; d < 3; d++) {
+      rec[12 + d] = (double)dF[d];
+      rec[15 + d] = (double)dTi[d];
+    }
+    // the pair crosses a periodic boundary (its partner is a periodic image)
+    rec[18] = (dx != pi.x - pj.x || dy != pi.y - pj.y || dz != pi.z - pj.z) ? 1.0 : 0.0;
+    rec[19] = pi.w;
+    rec[20] = pj.w;
+    const bool flagged = (e & kFlagBit) != 0;
+    for (size_t d = 0; d < hw; d++)
+      rec[kPairLocalFixedCols + d] = flagged ? (double)hist[((size_t)k * hw + d) * cap + i] : 0.0;
+    o.tags[2 * r] = ti;
+    o.tags[2 * r + 1] = tj;
+    r++;
   }
 }
 
@@ -2024,6 +2113,29 @@ hipError_t launch_step(const StepArgs &a, hipStream_t s) {
       launch_step_variant<double, true>(a, nb, s);
     else
       launch_step_variant<double, false>(a, nb, s);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_local_count(const StepArgs &a, const PairLocalArgs &o, hipStream_t s) {
+  if (a.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pair_local_count, dim3(nblk(a.n, kBlock)), dim3(kBlock), 0, s, a, o);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_local_fill(const StepArgs &a, const PairLocalArgs &o, hipStream_t s) {
+  if (a.n <= 0) return hipSuccess;
+  const dim3 g(nblk(a.n, kBlock)), b(kBlock);
+  if (a.mixed) {
+    if (a.prm.generic)
+      hipLaunchKernelGGL((k_pair_local_fill<float, true>), g, b, 0, s, a, o);
+    else
+      hipLaunchKernelGGL((k_pair_local_fill<float, false>), g, b, 0, s, a, o);
+  } else {
+    if (a.prm.generic)
+      hipLaunchKernelGGL((k_pair_local_fill<double, true>), g, b, 0, s, a, o);
+    else
+      hipLaunchKernelGGL((k_pair_local_fill<double, false>), g, b, 0, s, a, o);
   }
   return hipGetLastError();
 }

@@ -2365,6 +2365,79 @@ int dem_get_pairs(dem_handle *h, int64_t cap_out, int *tags, int *flags, double 
 
 int dem_get_wall_history(dem_handle *h, double *out) { return get_wall_history(h, out, 0); }
 
+static_assert(dem::kPairLocalFixedCols == DEM_PAIR_LOCAL_FIXED_COLS, "pair-local record layout and the header disagree");
+
+int dem_compute_pair_local(dem_handle *h, int64_t cap_out, int *tags, double *data, int64_t *npairs,
+                           int *ncol) {
+  if (!h || !npairs) return DEM_ERR_ARG;
+  *npairs = 0;
+  const int nc = dem::kPairLocalFixedCols + h->hw;
+  if (ncol) *ncol = nc;
+  if (h->n <= 0 || !h->have_list) return DEM_OK;
+  if (cap_out > 0 && (!tags || !data)) return fail(h, DEM_ERR_ARG, "null output arrays");
+  HIPCHK(h, hipSetDevice(h->p.device));
+  int rc = check_ready(h);
+  if (rc) return rc;
+  const int n = (int)h->n;
+  std::vector<void *> tmp;
+  auto tfree = [&]() {
+    for (void *q : tmp) hipFree(q);
+  };
+  auto talloc = [&](void **q, size_t bytes) -> int {
+    if (hipMalloc(q, bytes ? bytes : 16) != hipSuccess) return fail(h, DEM_ERR_HIP, "hipMalloc failed");
+    tmp.push_back(*q);
+    return DEM_OK;
+  };
+  dem::PairLocalArgs o{};
+  o.ncol = nc;
+  int *scan = nullptr;
+  if ((rc = talloc((void **)&o.count, sizeof(int) * n)) || (rc = talloc((void **)&o.offset, sizeof(int) * (n + 1))) ||
+      (rc = talloc((void **)&scan, sizeof(int) * dem::scan_tmp_size(n)))) {
+    tfree();
+    return rc;
+  }
+  dem::StepArgs a = step_args(h, h->cur_st, h->tri_cur);
+  a.shearupdate = 0;
+  hipError_t e = dem::launch_pair_local_count(a, o, h->stream);
+  if (e == hipSuccess) e = dem::launch_exclusive_scan(o.count, o.offset, scan, n, h->stream);
+  int total = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&total, o.offset + n, sizeof(int), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) {
+    tfree();
+    return fail(h, DEM_ERR_HIP, hipGetErrorString(e));
+  }
+  *npairs = total;
+  if (cap_out <= 0 || total == 0) {
+    tfree();
+    return DEM_OK;
+  }
+  if ((rc = talloc((void **)&o.tags, sizeof(int) * 2 * (size_t)total)) ||
+      (rc = talloc((void **)&o.rec, sizeof(double) * (size_t)nc * (size_t)total))) {
+    tfree();
+    return rc;
+  }
+  e = dem::launch_pair_local_fill(a, o, h->stream);
+  const int64_t m = std::min<int64_t>(cap_out, total);
+  if (e == hipSuccess) e = hipMemcpyAsync(tags, o.tags, sizeof(int) * 2 * (size_t)m, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(data, o.rec, sizeof(double) * (size_t)nc * (size_t)m, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  tfree();
+  if (e != hipSuccess) return fail(h, DEM_ERR_HIP, hipGetErrorString(e));
+  // Domain::remap of both positions (compute_pair_gran_local.cpp:448-451)
+  for (int64_t q = 0; q < m; q++)
+    for (int side = 0; side < 2; side++)
+      for (int d = 0; d < 3; d++) {
+        if (!h->p.periodic[d]) continue;
+        double &x = data[q * nc + 3 * side + d];
+        const double lo = h->p.boxlo[d], L = h->p.boxhi[d] - h->p.boxlo[d];
+        while (x < lo) x += L;
+        while (x >= lo + L) x -= L;
+      }
+  return DEM_OK;
+}
+
 int dem_get_pair_rolling(dem_handle *h, int64_t cap_out, double *rhist, int64_t *npairs) {
   if (!h || !npairs) return DEM_ERR_ARG;
   if (h->hw != 6) return fail(h, DEM_ERR_STATE, "no rolling history: no contact model uses epsd2");

@@ -65,6 +65,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <set>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -188,6 +189,13 @@ struct Interp {
   };
   std::map<std::string, int> mesh_ids;  // library mesh id of each fix mesh/surface
   std::map<std::string, std::string> computes;  // compute id -> style (erotate/sphere, ke)
+  // compute pair/gran/local (compute_pair_gran_local.cpp:71-124): the value
+  // groups of a row, in the reference's column order
+  struct LocalCompute {
+    std::vector<std::string> keys;
+    int ncols = 0;
+  };
+  std::map<std::string, LocalCompute> local_computes;
   std::map<std::string, MeshDef> mesh_defs;
   std::vector<std::string> mesh_order;  // meshes in fix wall/gran order
   int precision = DEM_FP64;
@@ -212,6 +220,7 @@ struct Interp {
     bool binary = false, ghost = false;
     std::vector<std::string> meshes;  // mesh/stl: the meshes dumped, in order
     std::string region;               // mesh/stl: only triangles whose centre is inside
+    bool local = false;               // dump local: rows of a local compute
   };
   std::vector<std::string> mesh_def_order;  // fix mesh/surface in definition order
   // thermo `time` (Update::atime / atimestep, update.cpp:434-446) and `cpu`
@@ -677,6 +686,194 @@ struct Interp {
     fclose(fp);
   }
 
+  // history values per pair the pair style registers (dnum): the tangential
+  // shear (3, tangential history) and the epsd2 rolling torque (3)
+  int pair_dnum() const {
+    return (pair_m.tangential_model == 0 ? 3 : 0) + (pair_m.rolling_epsd2 ? 3 : 0);
+  }
+
+  // compute ID group pair/gran/local [keywords] (compute_pair_gran_local.cpp:71-124,
+  // the column count of :281-283). Rows come from dem_compute_pair_local.
+  void local_compute_command(const std::vector<std::string> &t) {
+    if (t[3] == "wall/gran/local")
+      die("compute wall/gran/local is not available on this path (compute pair/gran/local is)");
+    if (step > 0) die("Compute pair/gran/local: Need to define this compute before first run");
+    if (t[2] != "all") die("compute pair/gran/local: only group all is available on this path");
+    static const char *order[] = {"pos", "vel", "id", "force", "force_normal", "force_tangential", "torque",
+                                  "torque_normal", "torque_tangential", "history", "contactArea", "delta",
+                                  "contactPoint"};
+    std::set<std::string> want;
+    if (t.size() == 4) {
+      want = {"pos", "vel", "id", "force", "torque", "history", "contactArea"};
+    } else {
+      for (size_t k = 4; k < t.size(); k++) {
+        const std::string &w = t[k];
+        if (w == "verbose") continue;
+        if (w == "extraSurfDistance")
+          die("this keyword is deprecated; neighbor->contactDistanceFactor is now used directly");
+        if (w == "heatFlux")
+          die("Compute pair/gran/local can not calculate heat flux values since no fix heat/gran/conduction "
+              "not compute them");
+        if (w == "ms_id") die("compute pair/gran/local ms_id needs fix multisphere, not available on this path");
+        bool known = false;
+        for (const char *o : order) known = known || w == o;
+        if (!known) die("Compute pair/gran/local: illegal/unrecognized keyword " + w);
+        want.insert(w);
+      }
+    }
+    LocalCompute lc;
+    for (const char *o : order)
+      if (want.count(o)) lc.keys.push_back(o);
+    const int dnum = pair_dnum();
+    if (want.count("history") && dnum == 0)
+      die("Compute pair/gran/local or wall/gran/local can not calculate history values since pair or wall "
+          "style does not compute them");
+    for (const std::string &k : lc.keys) {
+      if (k == "pos" || k == "vel") lc.ncols += 6;
+      else if (k == "history") lc.ncols += dnum;
+      else if (k == "contactArea" || k == "delta") lc.ncols += 1;
+      else lc.ncols += 3;
+    }
+    local_computes[t[1]] = lc;
+  }
+
+  // dump ID group local N file c_ID[k] ... | index (dump_local.cpp)
+  void dump_local_command(const std::vector<std::string> &t) {
+    if (dumps.count(t[1])) die("Reuse of dump ID " + t[1]);
+    Dump d;
+    d.local = true;
+    d.every = std::stoll(t[4]);
+    if (d.every <= 0) die("Illegal dump command");
+    d.file = t[5];
+    for (size_t k = 6; k < t.size(); k++) {
+      const std::string &c = t[k];
+      if (c != "index") {
+        const size_t lb = c.find('[');
+        if (c.compare(0, 2, "c_") != 0 || lb == std::string::npos || c.back() != ']')
+          die("Invalid attribute " + c + " in dump local command");
+        const std::string id = c.substr(2, lb - 2);
+        auto it = local_computes.find(id);
+        if (it == local_computes.end()) die("Could not find dump local compute ID " + id);
+        const int k1 = std::atoi(c.substr(lb + 1, c.size() - lb - 2).c_str());
+        if (k1 < 1 || k1 > it->second.ncols) die("Dump local compute does not calculate local array column " + c);
+      }
+      d.cols.push_back(c);
+    }
+    if (d.cols.empty()) die("Illegal dump command");
+    if (d.file.find('*') == std::string::npos) {
+      d.fp = fopen(d.file.c_str(), "w");
+      if (!d.fp) die("Cannot open dump file " + d.file);
+    }
+    dumps[t[1]] = d;
+  }
+
+  // The rows of one local compute at this step: ComputePairGranLocal::add_pair
+  // (compute_pair_gran_local.cpp:420-633) from the library's per-pair record.
+  // Normal parts use the unit vector from i to j (x_j - x_i with the minimum
+  // image), the contact point (x_i r_j + x_j r_i) / (r_i + r_j), the area the
+  // sphere-sphere intersection circle's.
+  std::vector<std::vector<double>> local_rows(const LocalCompute &lc) {
+    int64_t np = 0;
+    int nc = 0;
+    if (dem_compute_pair_local(h, 0, nullptr, nullptr, &np, &nc)) die(dem_last_error(h));
+    std::vector<int> tg(2 * std::max<int64_t>(np, 1));
+    std::vector<double> rec((size_t)nc * std::max<int64_t>(np, 1));
+    if (np > 0 && dem_compute_pair_local(h, np, tg.data(), rec.data(), &np, &nc)) die(dem_last_error(h));
+    const int dnum = pair_dnum();
+    double L[3];
+    for (int d = 0; d < 3; d++) L[d] = hi[d] - lo[d];
+    std::vector<std::vector<double>> rows((size_t)np);
+    for (int64_t q = 0; q < np; q++) {
+      const double *rc = &rec[(size_t)q * nc];
+      const double *xi = rc, *xj = rc + 3, *vi = rc + 6, *vj = rc + 9, *fc = rc + 12, *tc = rc + 15;
+      const double ri = rc[19], rj = rc[20];
+      double del[3];  // x_i - x_j, minimum image
+      for (int d = 0; d < 3; d++) {
+        del[d] = xi[d] - xj[d];
+        if (face[d][0] == 'p') {
+          if (del[d] > 0.5 * L[d]) del[d] -= L[d];
+          else if (del[d] < -0.5 * L[d]) del[d] += L[d];
+        }
+      }
+      const double rsq = del[0] * del[0] + del[1] * del[1] + del[2] * del[2];
+      const double dist = std::sqrt(rsq);
+      const double nrm[3] = {-del[0] / dist, -del[1] / dist, -del[2] / dist};
+      std::vector<double> &row = rows[(size_t)q];
+      for (const std::string &k : lc.keys) {
+        if (k == "pos") row.insert(row.end(), {xi[0], xi[1], xi[2], xj[0], xj[1], xj[2]});
+        else if (k == "vel") row.insert(row.end(), {vi[0], vi[1], vi[2], vj[0], vj[1], vj[2]});
+        else if (k == "id") row.insert(row.end(), {(double)tg[2 * q], (double)tg[2 * q + 1], rc[18]});
+        else if (k == "force") row.insert(row.end(), {fc[0], fc[1], fc[2]});
+        else if (k == "torque") row.insert(row.end(), {tc[0], tc[1], tc[2]});
+        else if (k == "force_normal" || k == "force_tangential" || k == "torque_normal" ||
+                 k == "torque_tangential") {
+          const double *v = k[0] == 'f' ? fc : tc;
+          const double mag = v[0] * nrm[0] + v[1] * nrm[1] + v[2] * nrm[2];
+          const bool normal = k.find("normal") != std::string::npos;
+          for (int d = 0; d < 3; d++) row.push_back(normal ? nrm[d] * mag : v[d] - nrm[d] * mag);
+        } else if (k == "history") {
+          // the record holds the shear (3) and, with epsd2, the rolling torque (3)
+          const int off = pair_m.tangential_model == 0 ? 0 : 3;
+          for (int d = 0; d < dnum; d++) row.push_back(rc[DEM_PAIR_LOCAL_FIXED_COLS + off + d]);
+        } else if (k == "contactArea") {
+          row.push_back(-3.14159265358979323846 / 4 * ((dist - ri - rj) * (dist + ri - rj) * (dist - ri + rj) * (dist + ri + rj)) / rsq);
+        } else if (k == "delta") {
+          row.push_back(ri + rj - dist);
+        } else if (k == "contactPoint") {
+          for (int d = 0; d < 3; d++) row.push_back((xi[d] * rj + (xi[d] - del[d]) * ri) / (ri + rj));
+        }
+      }
+    }
+    return rows;
+  }
+
+  // one snapshot of a dump local (DumpLocal::header_item / write_lines,
+  // dump_local.cpp:239-254, 350-363: "%g " per value)
+  void write_local_dump(Dump &d, const char *bstr, const double *blo, const double *bhi) {
+    std::map<std::string, std::vector<std::vector<double>>> data;
+    size_t nrows = 0;
+    bool first = true;
+    for (const std::string &c : d.cols) {
+      if (c == "index") continue;
+      const std::string id = c.substr(2, c.find('[') - 2);
+      if (!data.count(id)) {
+        if (h && !tag.empty()) data[id] = local_rows(local_computes[id]);
+        else data[id] = {};
+        if (!first && data[id].size() != nrows) die("Dump local count is not consistent across input fields");
+        nrows = data[id].size();
+        first = false;
+      }
+    }
+    FILE *fp = d.fp;
+    if (!fp) {
+      std::string name = d.file;
+      name.replace(name.find('*'), 1, std::to_string((long long)step));
+      fp = fopen(name.c_str(), "w");
+      if (!fp) die("Cannot open dump file " + name);
+    }
+    std::string cols;
+    for (const std::string &c : d.cols) cols += c + " ";
+    fprintf(fp, "ITEM: TIMESTEP\n%lld\nITEM: NUMBER OF ENTRIES\n%zu\nITEM: BOX BOUNDS %s\n", (long long)step,
+            nrows, bstr);
+    for (int k = 0; k < 3; k++) fprintf(fp, "%g %g\n", blo[k], bhi[k]);
+    fprintf(fp, "ITEM: ENTRIES %s\n", cols.c_str());
+    for (size_t q = 0; q < nrows; q++) {
+      for (const std::string &c : d.cols) {
+        if (c == "index") {
+          fprintf(fp, "%d ", (int)q + 1);
+          continue;
+        }
+        const size_t lb = c.find('[');
+        const std::string id = c.substr(2, lb - 2);
+        const int k1 = std::atoi(c.substr(lb + 1, c.size() - lb - 2).c_str());
+        fprintf(fp, "%g ", data[id][q][(size_t)k1 - 1]);
+      }
+      fprintf(fp, "\n");
+    }
+    if (d.fp) fflush(d.fp);
+    else fclose(fp);
+  }
+
   void dump_command(const std::vector<std::string> &t) {
     // dump id group custom N file col1 col2 ...
     if (t.size() < 6) die("Illegal dump command");
@@ -685,7 +882,11 @@ struct Interp {
       dump_stl_command(t);
       return;
     }
-    if (t[3] != "custom") die("dump style " + t[3] + " is not available on this path (custom, mesh/stl)");
+    if (t[3] == "local") {
+      dump_local_command(t);
+      return;
+    }
+    if (t[3] != "custom") die("dump style " + t[3] + " is not available on this path (custom, local, mesh/stl)");
     if (t[2] != "all") die("dump: only group all is available on this path");
     if (sq) die("dump custom for superquadrics is not available on this path");
     if (dumps.count(t[1])) die("Reuse of dump ID " + t[1]);
@@ -734,6 +935,11 @@ struct Interp {
       if (d.last == step || !(force_all || step % d.every == 0)) continue;
       if (d.stl) {
         write_stl(d);
+        d.last = step;
+        continue;
+      }
+      if (d.local) {
+        write_local_dump(d, bstr, blo, bhi);
         d.last = step;
         continue;
       }
@@ -1859,7 +2065,8 @@ struct Interp {
         die("Only atom_style granular/sphere/superquadric is supported on this path");
       }
     } else if (c == "atom_modify" || c == "processors" || c == "echo" || c == "log" ||
-               (c == "compute" && !(t.size() >= 4 && (t[3] == "erotate/sphere" || t[3] == "ke"))) ||
+               (c == "compute" && !(t.size() >= 4 && (t[3] == "erotate/sphere" || t[3] == "ke" ||
+                                                       t[3] == "pair/gran/local" || t[3] == "wall/gran/local"))) ||
                c == "uncompute" || c == "restart" ||
                c == "write_dump" || c == "soft_particles" ||
                c == "hard_particles") {
@@ -1907,6 +2114,8 @@ struct Interp {
       if (it == dumps.end()) die("Could not find undump ID");
       if (it->second.fp) fclose(it->second.fp);
       dumps.erase(it);
+    } else if (c == "compute" && t.size() >= 4 && (t[3] == "pair/gran/local" || t[3] == "wall/gran/local")) {
+      local_compute_command(t);
     } else if (c == "compute") {
       // compute id group erotate/sphere | ke: read by thermo as c_id
       computes[t[1]] = t[3];

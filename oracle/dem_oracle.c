@@ -1253,6 +1253,65 @@ int64_t orc_get_pairs(const orc_sys *s, int64_t cap, int *tags, int *flags, doub
   return np;
 }
 
+/* ComputePairGranLocal::add_pair (compute_pair_gran_local.cpp:420-633) for
+ * the default columns (pos, vel, id, force, torque, history), called from
+ * PairGran::compute_pgl (pair_gran.cpp:676-708): the pair loop with
+ * shearupdate 0 (pair_gran_base.h:275-447), which calls cpl_add_pair for every
+ * pair of the half list with the force and torque the contact puts on i
+ * (i_forces, zero when the pair does not touch; pair_gran.cpp:798-813). The
+ * history is the pair's stored one. Rows in the order of orc_get_pairs; each
+ * pair is oriented from its lower tag (the reference orients it from its
+ * first local index): for a pair whose list owner has the higher tag the row
+ * takes the partner's view, force -dFi and torque dTj, negated history.
+ * Columns: x_lo[3], x_hi[3] (unwrapped, as stored), v_lo[3], v_hi[3], F[3],
+ * T[3], periodic flag (1 if the minimum-image shift moved the pair), r_lo,
+ * r_hi, shear history[3], and with epsd2 the rolling-torque history[3]. Nothing in the
+ * system changes. Returns the number of pairs (negated when above cap). */
+int64_t orc_pair_local(const orc_sys *s, int64_t cap, int *tags, double *data) {
+  if (!s->built) return 0;
+  const int64_t np = s->first[s->n];
+  if (np > cap) return -np;
+  const int ncol = 21 + (s->p.model.rolling_epsd2 ? 6 : 3);
+  for (int i = 0; i < s->n; i++)
+    for (int64_t e = s->first[i]; e < s->first[i + 1]; e++) {
+      const int j = s->nbr[e];
+      double del[3], dFi[3] = {0, 0, 0}, dTi[3] = {0, 0, 0}, dTj[3] = {0, 0, 0};
+      double sh[3], rh[3];
+      for (int d = 0; d < 3; d++) {
+        sh[d] = s->hist[3 * e + d];
+        rh[d] = s->rhist ? s->rhist[3 * e + d] : 0.0;
+      }
+      min_image_del(&s->p, &s->x[3 * i], &s->x[3 * j], del);
+      orc_contact_pair_r(&s->p.model, &s->m, s->type[i], s->type[j], del, s->r[i], s->r[j], &s->v[3 * i],
+                         &s->v[3 * j], &s->w[3 * i], &s->w[3 * j], s->rm[i], s->rm[j], s->p.dt, 0, sh, rh, dFi,
+                         dTi, dTj);
+      const int lo_first = s->tag[i] < s->tag[j];
+      const int a = lo_first ? i : j, b = lo_first ? j : i;
+      const double sg = lo_first ? 1.0 : -1.0;
+      double *row = data + e * ncol;
+      tags[2 * e] = s->tag[a];
+      tags[2 * e + 1] = s->tag[b];
+      int periodic = 0;
+      for (int d = 0; d < 3; d++) {
+        row[d] = s->x[3 * a + d];
+        row[3 + d] = s->x[3 * b + d];
+        row[6 + d] = s->v[3 * a + d];
+        row[9 + d] = s->v[3 * b + d];
+        row[12 + d] = lo_first ? dFi[d] : -dFi[d];
+        row[15 + d] = lo_first ? dTi[d] : dTj[d];
+        if (del[d] != s->x[3 * i + d] - s->x[3 * j + d]) periodic = 1;
+      }
+      row[18] = periodic;
+      row[19] = s->r[a];
+      row[20] = s->r[b];
+      const int fl = s->flag[e] != 0;
+      for (int d = 0; d < 3; d++) row[21 + d] = fl ? sg * s->hist[3 * e + d] : 0.0;
+      if (s->p.model.rolling_epsd2)
+        for (int d = 0; d < 3; d++) row[24 + d] = fl ? sg * s->rhist[3 * e + d] : 0.0;
+    }
+  return np;
+}
+
 void orc_get_wall_hist(const orc_sys *s, double *hist) {
   memcpy(hist, s->whist, sizeof(double) * 3 * (size_t)s->n * (size_t)(s->p.nplanes > 0 ? s->p.nplanes : 1));
 }

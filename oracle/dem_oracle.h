@@ -163,6 +163,9 @@ void orc_get_state(const orc_sys *s, double *x, double *v, double *omega, double
 /* pairs of the current half list, as (tag_a, tag_b) with tag_a < tag_b,
  * contact flag, and the shear history oriented from tag_a towards tag_b */
 int64_t orc_get_pairs(const orc_sys *s, int64_t cap, int *tags, int *flags, double *hist);
+/* compute pair/gran/local's default columns per half-list pair, oriented
+ * from the lower tag (see dem_oracle.c); data [pairs][21 + 3 or 6] */
+int64_t orc_pair_local(const orc_sys *s, int64_t cap, int *tags, double *data);
 void orc_get_wall_hist(const orc_sys *s, double *hist /* n*nplanes*3 */);
 /* rolling-torque histories (epsd2), in the orders of orc_get_pairs / orc_get_wall_hist */
 int64_t orc_get_pair_rolling(const orc_sys *s, int64_t cap, double *rhist);

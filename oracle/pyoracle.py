@@ -119,6 +119,8 @@ def load():
     lib.orc_get_state.argtypes = [ctypes.c_void_p, dp, dp, dp, dp, dp]
     lib.orc_get_pairs.argtypes = [ctypes.c_void_p, ctypes.c_int64, ip, ip, dp]
     lib.orc_get_pairs.restype = ctypes.c_int64
+    lib.orc_pair_local.argtypes = [ctypes.c_void_p, ctypes.c_int64, ip, dp]
+    lib.orc_pair_local.restype = ctypes.c_int64
     lib.orc_get_wall_hist.argtypes = [ctypes.c_void_p, dp]
     lib.orc_lost_atoms.argtypes = [ctypes.c_void_p, P(ctypes.c_int64)]
     lib.orc_lost_atoms.restype = ctypes.c_int64
@@ -452,6 +454,18 @@ class OracleSystem:
         self.lib.orc_get_state(self.s, _dp(out["x"]), _dp(out["v"]), _dp(out["omega"]),
                                _dp(out["f"]), _dp(out["torque"]))
         return out
+
+    def pair_local(self):
+        """compute pair/gran/local's default columns per half-list pair, in
+        the order of pairs() (orc_pair_local): tags (P,2), data (P,ncol)."""
+        ncol = 21 + (6 if self.p.model.rolling_epsd2 else 3)
+        cap = max(-int(self.lib.orc_pair_local(self.s, 0, None, None)), 1)
+        tags = np.zeros((cap, 2), dtype=np.int32)
+        data = np.zeros((cap, ncol))
+        n = self.lib.orc_pair_local(self.s, cap, _ip(tags), _dp(data))
+        if n < 0:
+            raise RuntimeError("orc_pair_local: pair count changed")
+        return tags[:n], data[:n]
 
     def pairs(self):
         # ask for the count first (a negative return is minus the pairs held)

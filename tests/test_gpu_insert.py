@@ -118,9 +118,9 @@ def test_insert_mid_run_with_history_fp64(graphs):
     the empty top of its box (the present particles count and block), the new
     particles join both systems, and the arrays grow past their capacity."""
     params, mats, P, walls = wl.dense_pack(seed=3, n=1500, rolling=True, periodic=(0, 0, 0))
-    # side faces shrink-wrapped (boundary s s f): the compressed pack spreads
+    # side faces shrink-wrapped (boundary m m f): the compressed pack spreads
     # sideways, and an f face would lose those spheres
-    params = dict(params, shrink_faces=0b1111)
+    params = dict(params, shrink_faces=0b1111, shrink_min=0b1111)
     lo, hi = params["boxlo"], params["boxhi"]
     g = pydem.GpuSystem(params, mats, P, walls)
     if graphs:

@@ -523,7 +523,7 @@ def test_mesh_floor_contact_distance_factor_fp64():
     say("[gpu] mesh floor, contact_distance_factor 1.2")
     case = _mesh_floor_pack(seed=43, n=1500)
     # side faces shrink-wrapped: the compressed pack ejects a few spheres sideways
-    params = dict(case[0], skin=0.00005, cdf=1.2, shrink_faces=0b1111)
+    params = dict(case[0], skin=0.00005, cdf=1.2, shrink_faces=0b1111, shrink_min=0b1111)
     P = dict(case[2])
     P["v"] = P["v"] + np.array([0.0, 0.0, 0.05])  # lift off the floor
     case = (params, case[1], P) + case[3:]
