@@ -495,6 +495,20 @@ int dem_sq_net_forward(dem_sq_handle *h, int64_t n, const float *X, float *Y);
 /* Gradient of the mean squared error over n samples (no update). */
 int dem_sq_net_gradient(dem_sq_handle *h, int64_t n, const float *X, const float *T, float *grad,
                         float *loss);
+/* dem_sq_net_gradient on device memory, allocating nothing. No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (tight
+ * training or serving loops, harnesses that forbid hipMalloc). X [n][18] and
+ * T [n][3] are device pointers on the handle's device; grad (5571 floats) and
+ * loss (1 float, or NULL) receive the results in device memory; the call
+ * returns once they are written. The per-block partial sums go to a fixed
+ * workspace, one per device, so n must be 1 .. DEM_SQ_GRAD_NOALLOC_MAX
+ * (DEM_ERR_ARG otherwise) and the network must be initialised (DEM_ERR_STATE
+ * otherwise). Every call on a device shares that device's workspace: calls
+ * that overlap on one device (several host threads, or handles on different
+ * streams of the same device) race and must be serialised by the caller. */
+#define DEM_SQ_GRAD_NOALLOC_MAX 16384
+int dem_sq_net_gradient_noalloc(dem_sq_handle *h, int64_t n, const float *X, const float *T, float *grad,
+                                float *loss);
 /* `steps` Adam steps on batches drawn from the handle's pairs, the targets
  * being the points of the last DEM_SQ_FRESH solve; loss_curve (steps floats)
  * and train_ms may be NULL. */

@@ -95,11 +95,19 @@ hipError_t launch_to_target(const double *pa, const double *pb, int64_t n, const
                             hipStream_t s);
 hipError_t launch_mlp_forward(const NetDev &net, const float *X, int64_t n, float *Y, hipStream_t s);
 int grad_blocks(int64_t batch);
+// idx: the batch's rows of X / T, or null for rows 0 .. batch-1
 hipError_t launch_mlp_grad(const NetDev &net, const float *X, const float *T, const int64_t *idx,
                            int64_t batch, float *part, float *loss_part, hipStream_t s);
 hipError_t launch_mlp_adam(const NetDev &net, const float *part, int nparts, float *grad, const AdamArgs &ad,
                            hipStream_t s);
 hipError_t launch_mlp_pack(const NetDev &net, hipStream_t s);
 hipError_t launch_sample(int64_t *idx, int64_t batch, int64_t n, uint64_t seed, hipStream_t s);
+
+// the allocation-free gradient's workspace (a file-scope device array in
+// sq_kernels.hip): batches up to kGradScratchMaxBatch samples, devices
+// 0 .. kMaxScratchDevices-1; resolve with the device current
+constexpr int64_t kGradScratchMaxBatch = 16384;
+constexpr int kMaxScratchDevices = 64;
+hipError_t grad_scratch(int device, float **part, float **loss_part);
 
 }  // namespace sqk

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "sq_device.h"
 #include "sq_kernels.h"
 
@@ -392,7 +394,7 @@ __global__ __launch_bounds__(kBlk) void k_mlp_grad(NetDev net, const float *X, c
   const int64_t t0 = ((int64_t)blockIdx.x * (kBlk / 64) + wv) * 16;
   const int nvalid = (int)(batch - t0 < 0 ? 0 : (batch - t0 > 16 ? 16 : batch - t0));
   int64_t rows[16];
-  for (int i = 0; i < 16; i++) rows[i] = i < nvalid ? idx[t0 + i] : 0;
+  for (int i = 0; i < 16; i++) rows[i] = i < nvalid ? (idx ? idx[t0 + i] : t0 + i) : 0;
   Hidden hid;
   const floatx4 y = forward_tile(s, w, X, rows, nvalid, hid);
   const float scale = 2.0f / (float)batch;
@@ -594,6 +596,32 @@ hipError_t launch_mlp_forward(const NetDev &net, const float *X, int64_t n, floa
 }
 
 int grad_blocks(int64_t batch) { return (int)nblk(batch, kBlk / 4); }
+
+// Workspace of the allocation-free gradient entry point: k_mlp_grad's
+// per-block gradient partials ([blocks][kNumParams]) followed by its per-block
+// loss partials, for batches of up to kGradScratchMaxBatch samples. Every
+// block writes its whole partial, so nothing needs clearing between calls.
+// Each device holds its own copy until the process ends.
+constexpr int kGradScratchBlocks = (int)((kGradScratchMaxBatch + kBlk / 4 - 1) / (kBlk / 4));
+static __device__ __align__(256) float scratch[(size_t)kGradScratchBlocks * (kNumParams + 1)];
+
+// Device address of this device's scratch copy (the array's name is only a
+// host-side stand-in). Resolved once per device with that device current.
+hipError_t grad_scratch(int device, float **part, float **loss_part) {
+  static std::mutex mu;
+  static float *addr[kMaxScratchDevices] = {};
+  if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    float *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+  }
+  *part = addr[device];
+  *loss_part = addr[device] + (size_t)kGradScratchBlocks * kNumParams;
+  return hipSuccess;
+}
 
 hipError_t launch_mlp_grad(const NetDev &net, const float *X, const float *T, const int64_t *idx,
                            int64_t batch, float *part, float *loss_part, hipStream_t s) {

@@ -40,6 +40,8 @@ struct dem_sq_handle {
   int64_t idx_cap = 0, loss_cap = 0;
   int64_t adam_t = 0;
   bool net_ready = false;
+  // this device's copy of the allocation-free gradient's workspace
+  float *scr_part = nullptr, *scr_loss = nullptr;
 };
 
 namespace {
@@ -130,6 +132,11 @@ int dem_sq_create(int device, dem_sq_handle **out) {
       delete h;
       return sq_fail(nullptr, DEM_ERR_HIP, "hipEventCreate failed");
     }
+  const hipError_t se = sqk::grad_scratch(device, &h->scr_part, &h->scr_loss);
+  if (se != hipSuccess) {
+    dem_sq_destroy(h);
+    return sq_fail(nullptr, DEM_ERR_HIP, std::string("hipGetSymbolAddress: ") + hipGetErrorString(se));
+  }
   *out = h;
   return DEM_OK;
 }
@@ -442,6 +449,31 @@ int dem_sq_net_gradient(dem_sq_handle *h, int64_t n, const float *X, const float
   free_list(tmp);
   if (e != hipSuccess) return sq_fail(h, DEM_ERR_HIP, hipGetErrorString(e));
   if (loss) *loss = l;
+  return DEM_OK;
+}
+
+static_assert(sqk::kGradScratchMaxBatch == DEM_SQ_GRAD_NOALLOC_MAX, "workspace limit and the header disagree");
+
+int dem_sq_net_gradient_noalloc(dem_sq_handle *h, int64_t n, const float *X, const float *T, float *grad,
+                                float *loss) {
+  if (!h) return DEM_ERR_ARG;
+  if (!X || !T || !grad) return sq_fail(h, DEM_ERR_ARG, "null device pointer");
+  if (n <= 0 || n > sqk::kGradScratchMaxBatch)
+    return sq_fail(h, DEM_ERR_ARG,
+                   "batch of " + std::to_string(n) + " samples outside 1.." +
+                       std::to_string(sqk::kGradScratchMaxBatch) + " (the fixed workspace's limit)");
+  int rc = ensure_net(h);
+  if (rc) return rc;
+  if (!h->scr_part) return sq_fail(h, DEM_ERR_STATE, "workspace address not resolved");
+  SQCHK(h, hipSetDevice(h->device));
+  sqk::AdamArgs ad{};
+  ad.apply = 0;
+  ad.loss_part = h->scr_loss;
+  ad.loss_out = loss;
+  ad.batch = n;
+  SQCHK(h, sqk::launch_mlp_grad(h->net, X, T, nullptr, n, h->scr_part, h->scr_loss, h->stream));
+  SQCHK(h, sqk::launch_mlp_adam(h->net, h->scr_part, sqk::grad_blocks(n), grad, ad, h->stream));
+  SQCHK(h, hipStreamSynchronize(h->stream));
   return DEM_OK;
 }
 

@@ -32,7 +32,7 @@ EXPORTED = [
     "dem_sq_create", "dem_sq_destroy",
     "dem_sq_last_error", "dem_sq_set_pairs", "dem_sq_solve", "dem_sq_check", "dem_sq_get_results",
     "dem_sq_get_features", "dem_sq_net_init", "dem_sq_net_set_params", "dem_sq_net_get_params",
-    "dem_sq_net_forward", "dem_sq_net_gradient", "dem_sq_net_train",
+    "dem_sq_net_forward", "dem_sq_net_gradient", "dem_sq_net_gradient_noalloc", "dem_sq_net_train",
     "dem_sqdem_create", "dem_sqdem_destroy", "dem_sqdem_last_error", "dem_sqdem_set_materials",
     "dem_sqdem_set_particles", "dem_sqdem_set_warm_start", "dem_sqdem_setup", "dem_sqdem_run",
     "dem_sqdem_get_state",
@@ -99,6 +99,8 @@ def insert_params(**kw):
     return p
 SQ_HIST_DOUBLES = 7  # state, first OBB inequality, contact point[3], alpha_i, alpha_j
 SQ_NUM_PARAMS = 5571
+DEM_SQ_GRAD_NOALLOC_MAX = 16384  # include/liggghts_amd.h
+DEM_ERR_ARG, DEM_ERR_HIP, DEM_ERR_OVERFLOW, DEM_ERR_STATE = -1, -2, -3, -4
 
 
 class DemParams(ctypes.Structure):
@@ -275,6 +277,8 @@ def load_library(path=LIB_PATH):
     lib.dem_sq_net_get_params.argtypes = [h, fp]
     lib.dem_sq_net_forward.argtypes = [h, ctypes.c_int64, fp, fp]
     lib.dem_sq_net_gradient.argtypes = [h, ctypes.c_int64, fp, fp, fp, fp]
+    vp = ctypes.c_void_p
+    lib.dem_sq_net_gradient_noalloc.argtypes = [h, ctypes.c_int64, vp, vp, vp, vp]
     lib.dem_sq_net_train.argtypes = [h, ctypes.c_int64, ctypes.c_int64, ctypes.c_float, ctypes.c_uint64,
                                      fp, dp]
     lib.dem_get_step_shape.argtypes = [h, ip, ip, ctypes.c_int, ip, ip, ip, dp]
@@ -771,6 +775,12 @@ class SqSolver:
         loss = np.zeros(1, dtype=np.float32)
         self._check(self.lib.dem_sq_net_gradient(self.h, X.shape[0], _fp(X), _fp(T), _fp(g), _fp(loss)))
         return g, float(loss[0])
+
+    def net_gradient_noalloc(self, n, X_ptr, T_ptr, grad_ptr, loss_ptr):
+        """dem_sq_net_gradient_noalloc on device pointers (integers): X [n][18],
+        T [n][3] in, grad [5571] and loss [1] (or 0) out. Returns the status
+        code (raises only for a missing library)."""
+        return self.lib.dem_sq_net_gradient_noalloc(self.h, int(n), X_ptr, T_ptr, grad_ptr, loss_ptr or None)
 
     def net_train(self, steps, batch, lr, seed=0):
         curve = np.zeros(steps, dtype=np.float32)
