@@ -1607,16 +1607,24 @@ __global__ __launch_bounds__(kBlock) void k_gather(RebuildArgs a) {
 // slot beside it. A row with more flagged entries falls back to scanning the
 // old row in global memory.
 constexpr int kCarrySlots = 16;
+// List entries per particle that the build stages in LDS before writing
+// them row by row; entries past this many go straight to their slot.
+constexpr int kStageSlots = 16;
 
-struct CarryLds {
+struct BuildLds {
   uint32_t key[kCarrySlots][kBlock];
   uint16_t slot[kCarrySlots][kBlock];
+  // staged entries in stencil order
+  uint32_t entry[kStageSlots][kBlock];
+  // per touching staged entry, by touching rank: its old slot + 1, or 0
+  // when it carries no history
+  uint16_t carried[kCarrySlots][kBlock];
 };
 
 // Loads the flagged entries of old row io (oldcnt entries) into this lane's
 // sorted LDS column; returns their number, or -1 when they do not fit.
 __device__ __forceinline__ int carry_table(const RebuildArgs &a, const size_t cap, const int io,
-                                           const int oldcnt, CarryLds &L, const int t) {
+                                           const int oldcnt, BuildLds &L, const int t) {
   int nf = 0;
   for (int m0 = 0; m0 < oldcnt; m0 += 4) {
     uint32_t oe[4];
@@ -1645,7 +1653,7 @@ __device__ __forceinline__ int carry_table(const RebuildArgs &a, const size_t ca
 // Old list slot of flagged partner jo, or -1: binary search of the LDS column
 // (nf >= 0) or a scan of the old row (nf < 0).
 __device__ __forceinline__ int carry_find(const RebuildArgs &a, const size_t cap, const int io, const int oldcnt,
-                                          const CarryLds &L, const int t, const int nf, const uint32_t jo) {
+                                          const BuildLds &L, const int t, const int nf, const uint32_t jo) {
   if (nf >= 0) {
     int lo = 0, hi = nf;
     while (lo < hi) {
@@ -1664,19 +1672,40 @@ __device__ __forceinline__ int carry_find(const RebuildArgs &a, const size_t cap
   return -1;
 }
 
+template <typename T>
+__device__ __forceinline__ void copy_history(const RebuildArgs &a, const size_t cap, const size_t hw,
+                                             const int io, const int m, const int i, const int k) {
+  const T *ohist = (const T *)a.oldl.hist;
+  T *nhist = (T *)a.newl.hist;
+  const size_t ob = (size_t)m * hw * cap + io;
+  const size_t nb = (size_t)k * hw * cap + i;
+  for (size_t d = 0; d < hw; d++) nhist[nb + d * cap] = ohist[ob + d * cap];
+}
+
 // Neighbor::granular_bin_no_newton (neigh_gran.cpp:485-639) as a full list,
 // one thread per particle: the reference's predicate rsq <= ((ri+rj)*cdf +
-// skin)^2 over the 27-cell stencil in (z, y, x) cell order, j ascending within
-// a cell; history carry-in for pairs within (ri+rj)*cdf (neigh_gran.cpp:
-// 585-607: a flagged partner of the old row passes its history and flag 1).
-// The 27 cell ranges are loaded together up front, the candidates of each
-// stencil row (three cells in x) are tested in batches of four independent
-// position loads, and old partners are found by binary search in LDS.
+// skin)^2 over the 27-cell stencil; history carry-in for pairs within
+// (ri+rj)*cdf (neigh_gran.cpp:585-607: a flagged partner of the old row
+// passes its history and flag 1). The candidates of each stencil row (three
+// cells in x) are tested in batches of four independent position loads, and
+// old partners are found by binary search in LDS.
+//
+// Row order: a particle's entries within (ri+rj)*cdf (the touching ones)
+// come first, then the others, each in stencil order ((z, y, x) cells, j
+// ascending within a cell); entries past kStageSlots keep their stencil
+// slot. The reference's pair order only sets the order of the force sums.
+// Touching-first rows put the step's history reads and writes of a
+// wavefront's particles in the first few list rows, side by side.
+//
+// The first kStageSlots entries and the old slots of the touching ones wait
+// in LDS and are written row by row at the end, so the lanes of a wavefront
+// write the same list row (and history row) together. Workgroups walk the
+// particles XCD by XCD (xcd_block), so a neighborhood is fetched into one L2.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_build(RebuildArgs a) {
-  __shared__ CarryLds carry;
+  __shared__ BuildLds lds;
   const int t = threadIdx.x;
-  const int i = blockIdx.x * kBlock + t;
+  const int i = xcd_block(gridDim.x) * kBlock + t;
   const bool active = i < a.n;
   int cnt = 0;
   if (active) {
@@ -1718,10 +1747,11 @@ __global__ __launch_bounds__(kBlock) void k_build(RebuildArgs a) {
       cs[c] = s0;
       cn[c] = rk[c] < 0 ? 0 : s1 - s0;
     }
-    const int nf = oldcnt > 0 ? carry_table(a, cap, io, oldcnt, carry, t) : 0;
-    const T *ohist = (const T *)a.oldl.hist;
-    T *nhist = (T *)a.newl.hist;
+    const int nf = oldcnt > 0 ? carry_table(a, cap, io, oldcnt, lds, t) : 0;
     const size_t hw = (size_t)p.hist_w;
+    const int stage = a.W < kStageSlots ? a.W : kStageSlots;
+    uint32_t tmask = 0u;  // staged entries that touch
+    int nts = 0;          // their number
 #pragma unroll
     for (int row = 0; row < 9; row++) {
       const int s0 = cs[3 * row], s1 = cs[3 * row + 1], s2 = cs[3 * row + 2];
@@ -1749,20 +1779,50 @@ __global__ __launch_bounds__(kBlock) void k_build(RebuildArgs a) {
           if (!(rsq <= cutsq)) continue;
           if (cnt < a.W) {
             uint32_t entry = (uint32_t)j;
-            if (rsq < radsum * radsum && oldcnt > 0) {
-              const int m = carry_find(a, cap, io, oldcnt, carry, t, nf, (uint32_t)a.perm[j]);
-              if (m >= 0) {
-                const size_t ob = (size_t)m * hw * cap + io;
-                const size_t nb = (size_t)cnt * hw * cap + i;
-                for (size_t d = 0; d < hw; d++) nhist[nb + d * cap] = ohist[ob + d * cap];
-                entry |= kFlagBit | kCarryBit;
-              }
+            const bool touch = rsq < radsum * radsum;
+            int m = -1;
+            if (touch && oldcnt > 0) {
+              m = carry_find(a, cap, io, oldcnt, lds, t, nf, (uint32_t)a.perm[j]);
+              if (m >= 0) entry |= kFlagBit | kCarryBit;
             }
-            a.newl.nbr[(size_t)cnt * cap + i] = entry;
+            if (cnt < stage) {
+              lds.entry[cnt][t] = entry;
+              if (touch) {
+                if (nts < kCarrySlots)
+                  lds.carried[nts][t] = (uint16_t)(m + 1);
+                else if (m >= 0)
+                  copy_history<T>(a, cap, hw, io, m, i, nts);
+                tmask |= 1u << cnt;
+                nts++;
+              }
+            } else {
+              a.newl.nbr[(size_t)cnt * cap + i] = entry;
+              if (m >= 0) copy_history<T>(a, cap, hw, io, m, i, cnt);
+            }
           }
           cnt++;
         }
       }
+    }
+    // staged entries, touching ones first, one list row at a time
+    const int staged = cnt < stage ? cnt : stage;
+    uint32_t mt = tmask, mn = ~tmask & (staged < 32 ? (1u << staged) - 1u : ~0u);
+    for (int s = 0; s < staged; s++) {
+      int k;
+      if (mt) {
+        k = __builtin_ctz(mt);
+        mt &= mt - 1u;
+      } else {
+        k = __builtin_ctz(mn);
+        mn &= mn - 1u;
+      }
+      a.newl.nbr[(size_t)s * cap + i] = lds.entry[k][t];
+    }
+    // histories of the carried staged entries, at their touching rank
+    const int nq = nts < kCarrySlots ? nts : kCarrySlots;
+    for (int q = 0; q < nq; q++) {
+      const int m1 = lds.carried[q][t];
+      if (m1) copy_history<T>(a, cap, hw, io, m1 - 1, i, q);
     }
     const int stored = cnt < a.W ? cnt : a.W;
     a.aalt.meta[i] = (a.aalt.meta[i] & ~kMetaCountMask) | stored;
