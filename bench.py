@@ -374,9 +374,15 @@ def main_sqsettle(args):
             g.set_warm_start(solver)
         g.run(args.warmup)
         s0 = g.stats()
-        g.run(args.steps)
+        ms = 0.0
+        done = 0
+        while done < args.steps:
+            k = min(50, args.steps - done)
+            g.run(k)
+            done += k
+            ms += g.stats()["run_ms"]
+            print(f"# {mode}: {done}/{args.steps} steps, {ms:.0f} ms", file=sys.stderr, flush=True)
         s1 = g.stats()
-        ms = s1["run_ms"]
         fresh = s1["fresh"] - s0["fresh"]
         res[mode] = {"ms_per_step": ms / args.steps,
                      "particle_steps_per_s": n * args.steps / (ms * 1e-3),

@@ -358,6 +358,27 @@ int dem_get_wall_history(dem_handle *h, double *hist);
 int dem_compute_pair_local(dem_handle *h, int64_t cap, int *tags, double *data, int64_t *npairs,
                            int *ncol);
 
+/* compute wall/gran/local (compute_pair_gran_local.cpp:143-144, 247-270,
+ * 648-870) as evaluated through FixWallGran::post_force_pgl
+ * (fix_wall_gran.cpp:752-765): one pass over the mesh contacts with
+ * shearupdate off, on the GPU, changing nothing. A row per contact that the
+ * mesh walls' contact pass evaluates (within contact, or within the close band
+ * of contact_distance_factor, and accepted by the contact-history bookkeeping;
+ * fix_wall_gran.cpp:955-1006), mesh by mesh, the rows of one particle
+ * consecutive: ids[3k..3k+2] = the mesh (in dem_add_mesh order), the
+ * triangle's index within its mesh (the reference's element id), the
+ * particle's tag; data[k * ncol ...] = the contact point x + delta[3], x[3],
+ * the wall's velocity at the contact[3] (zero for a resting mesh), v[3], the
+ * force and the torque on the particle[3 + 3] (zero in the close band), the
+ * unit normal from the particle's centre to the contact point[3] (minus the
+ * contact's en, fix_wall_gran_base.h:332-334), the squared distance from the
+ * centre to the contact point (sidata.rsq), the particle's radius, then the
+ * slot's shear history[3]. *ncol receives 26; *nrows the number of rows,
+ * min(cap, *nrows) are written (cap 0: count only). Primitive walls give no
+ * rows, as in the reference. */
+#define DEM_WALL_LOCAL_FIXED_COLS 23
+int dem_compute_wall_local(dem_handle *h, int64_t cap, int *ids, double *data, int64_t *nrows, int *ncol);
+
 /* The rolling-torque history of rolling_friction epsd2 (the r_torque{x,y,z}_old
  * values RollingModel<EPSD2> registers, rolling_model_epsd2.h:75-77), for the
  * pairs of dem_get_pairs in the same order and orientation (rhist[3k..3k+2]),

@@ -267,12 +267,30 @@ struct PairLocalArgs {
   int ncol;
 };
 
+// compute wall/gran/local: per particle the number of its mesh contact rows,
+// their offsets, and per row the mesh, the triangle's index within its mesh
+// and the particle's tag, and a record of ncol doubles: contact point[3],
+// x_i[3], wall velocity at the contact[3], v_i[3], force on i[3], torque on
+// i[3], unit normal from the particle centre to the contact point[3], the
+// squared distance from the centre to the contact point, the radius,
+// history[3]
+constexpr int kWallLocalFixedCols = 23;
+struct WallLocalArgs {
+  int *count;      // [n]
+  int *offset;     // [n + 1]
+  int *ids;        // [nrows][3]
+  double *rec;     // [nrows][ncol]
+  int ncol;
+};
+
 // launchers (dem_kernels.hip)
 hipError_t launch_rebuild(const RebuildArgs &a, hipStream_t s);
 hipError_t launch_step(const StepArgs &a, hipStream_t s);
 hipError_t launch_initial(const StepArgs &a, hipStream_t s);
 hipError_t launch_pair_local_count(const StepArgs &a, const PairLocalArgs &o, hipStream_t s);
 hipError_t launch_pair_local_fill(const StepArgs &a, const PairLocalArgs &o, hipStream_t s);
+hipError_t launch_wall_local_count(const StepArgs &a, const WallLocalArgs &o, hipStream_t s);
+hipError_t launch_wall_local_fill(const StepArgs &a, const WallLocalArgs &o, hipStream_t s);
 // stop_at = v, run counters cleared (and the per-instance ones, nstat entries)
 hipError_t launch_set_stop(Ctrl *ctrl, long long v, hipStream_t s,
                            unsigned long long *inst_stats = nullptr, int nstat = 0);

@@ -1353,6 +1353,198 @@ __global__ __launch_bounds__(kBlock) void k_pair_local_fill(StepArgs a, PairLoca
   }
 }
 
+// compute wall/gran/local rows of particle i: the contacts that
+// FixWallGran::post_force_pgl (fix_wall_gran.cpp:752-765, computeflag 0,
+// shearupdate 0) hands to ComputePairGranLocal::add_wall_1 / add_wall_2
+// (fix_wall_gran_base.h:209-219, 325-337): every candidate triangle within
+// contact or the close band that handleContact accepts, mesh by mesh. The
+// slot table is evaluated on a private copy, so nothing is written; at the
+// end of a step it reproduces that step's own contact pass. FILL = false
+// only counts the rows.
+template <typename T, bool GEN, bool FILL>
+__device__ int wall_local_rows(const StepArgs &a, const int i, const WallLocalArgs &o, int64_t r) {
+  typedef typename OmgT<T>::type omg_t;
+  const DevParams &p = a.prm;
+  const MeshStep &M = a.mesh;
+  const int nc = M.ncand[i];
+  if (nc == 0) return 0;
+  const size_t cap = (size_t)a.cap;
+  const double4 pi = a.in.pos[i];
+  const double4 vi = a.in.vel[i];
+  const omg_t wi = ((const omg_t *)a.in.omg)[i];
+  const int wbits = (int)wi.w;
+  const int irow = (wbits >> kTypeBits) * p.mat_stride + (wbits & kTypeMask) * p.ntp1;
+  const double px[3] = {pi.x, pi.y, pi.z};
+  int rows = 0;
+  int c = 0;
+  for (int m = 0; m < p.nmesh; m++) {
+    int slot[kMeshSlots];
+    const int *gslot = a.aux.mslot + (size_t)m * kMeshSlots * cap + i;
+    const T *hist = (const T *)a.aux.mhist + (size_t)m * kMeshSlots * 3 * cap + i;
+    for (int k = 0; k < kMeshSlots; k++) slot[k] = gslot[(size_t)k * cap];
+    const int tend = p.mesh_tri0[m] + p.mesh_ntri[m];
+    const Model mdl = model_from_bits(p.mesh_model[m]);
+    const int mt = irow + p.mesh_wtype[m];
+    unsigned keep = 0u;
+    // slots whose history the step's pass would rewrite, with their values
+    unsigned over = 0u;
+    T lh[kMeshSlots][3];
+    for (; c < nc; c++) {
+      const int t = M.cand[(size_t)c * cap + i];
+      if (t >= tend) break;
+      const TriGeom &tg = M.tri[t];
+      Vec3 delta;
+      Bary bary;
+      int bs;
+      const double deltan = tri_sphere(tg, p.mesh_precision, pi.w, px, delta, bary, bs);
+      if (deltan > p.cutneighmax) continue;
+      if (!(deltan <= 0 || deltan < (p.cdf - 1.0) * pi.w)) continue;
+      // handleContact on the private slot table
+      int s = -1;
+      for (int k = 0; k < kMeshSlots; k++)
+        if (slot[k] == t) {
+          s = k;
+          break;
+        }
+      bool fresh = false;
+      if (s >= 0) {
+        keep |= 1u << s;
+      } else {
+        bool skip = false;
+        if (bs == 7)
+          for (int k = 0; k < kMeshSlots; k++)
+            if (slot[k] >= 0 && slot[k] != t && ((keep >> k) & 1u) && tri_coplanar(M, slot[k], t)) skip = true;
+        if (skip) continue;
+        for (int k = 0; k < kMeshSlots; k++)
+          if (slot[k] == -1) {
+            s = k;
+            break;
+          }
+        if (s < 0) continue;  // the step's own pass reported the overflow
+        slot[s] = t;
+        keep |= 1u << s;
+        fresh = true;
+      }
+      rows++;
+      if (!FILL) continue;
+      T sh[3];
+      if (fresh) {
+        sh[0] = sh[1] = sh[2] = T(0);
+        if (bs == 7)  // checkCoplanarContactHistory: the last coplanar slot wins
+          for (int k = 0; k < kMeshSlots; k++)
+            if (slot[k] >= 0 && slot[k] != t && tri_coplanar(M, slot[k], t))
+              for (int d = 0; d < 3; d++) sh[d] = ((over >> k) & 1u) ? lh[k][d] : hist[(size_t)(k * 3 + d) * cap];
+      } else {
+        for (int d = 0; d < 3; d++) sh[d] = ((over >> s) & 1u) ? lh[s][d] : hist[(size_t)(s * 3 + d) * cap];
+      }
+      const bool band = !(deltan <= 0);
+      if (band && !mdl.nohist) sh[0] = sh[1] = sh[2] = T(0);
+      // the step's pass rewrites a fresh or close-band slot with these values
+      if (fresh || band) {
+        for (int d = 0; d < 3; d++) lh[s][d] = sh[d];
+        over |= 1u << s;
+      }
+      double vw[3] = {0., 0., 0.};
+      if (p.mesh_moving[m])
+        for (int k = 0; k < 3; k++)
+          vw[k] = (bary.b0 * tg.vn[k] + bary.b1 * tg.vn[3 + k] + bary.b2 * tg.vn[6 + k]);
+      // Walls::Granular::compute_force (fix_wall_gran_base.h:148-337) with
+      // sidata.deltan = -deltan, sidata.delta = -delta
+      const double sdn = -deltan;
+      const double rr = pi.w - sdn;
+      const double rinv = 1.0 / rr;
+      const double sd0 = -delta.x, sd1 = -delta.y, sd2 = -delta.z;
+      T dF[3] = {T(0), T(0), T(0)}, dTi[3] = {T(0), T(0), T(0)};
+      if (!band) {
+        CIn<T> cc;
+        cc.radi = (T)pi.w;
+        cc.radj = T(0);
+        cc.rinv = (T)rinv;
+        cc.meff = (T)vi.w;
+        cc.deltan = (T)(pi.w - rr);
+        cc.deltan_in = (T)sdn;
+        cc.del[0] = (T)sd0;
+        cc.del[1] = (T)sd1;
+        cc.del[2] = (T)sd2;
+        cc.en[0] = (T)(sd0 * rinv);
+        cc.en[1] = (T)(sd1 * rinv);
+        cc.en[2] = (T)(sd2 * rinv);
+        cc.vr[0] = (T)(vi.x - vw[0]);
+        cc.vr[1] = (T)(vi.y - vw[1]);
+        cc.vr[2] = (T)(vi.z - vw[2]);
+        cc.wi[0] = (T)wi.x;
+        cc.wi[1] = (T)wi.y;
+        cc.wi[2] = (T)wi.z;
+        cc.wj[0] = cc.wj[1] = cc.wj[2] = T(0);
+        if (sizeof(T) == 8) {
+          cc.Y = (T)a.mat.Yeff[mt];
+          cc.G = (T)a.mat.Geff[mt];
+          cc.beta = (T)a.mat.beta[mt];
+          cc.mu = (T)a.mat.mu[mt];
+          cc.mur = (T)a.mat.mur[mt];
+        } else {
+          cc.Y = (T)a.mat.fYeff[mt];
+          cc.G = (T)a.mat.fGeff[mt];
+          cc.beta = (T)a.mat.fbeta[mt];
+          cc.mu = (T)a.mat.fmu[mt];
+          cc.mur = (T)a.mat.fmur[mt];
+        }
+        if (GEN) {
+          load_model_coeffs(a.mat, p, mdl, mt, cc);
+          cc.acont = (T)((pi.w * pi.w - rr * rr) * kPi);
+        }
+        T dTj[3];
+        T rh[3] = {T(0), T(0), T(0)};
+        contact_chain<T, true, GEN>(mdl, cc, (T)p.dt, 0, sh, rh, dF, dTi, dTj);
+      }
+      double *rec = o.rec + (size_t)(r + rows - 1) * (size_t)o.ncol;
+      // contact point x - sidata.delta (fix_wall_gran_base.h:211-212)
+      rec[0] = px[0] + delta.x;
+      rec[1] = px[1] + delta.y;
+      rec[2] = px[2] + delta.z;
+      rec[3] = px[0];
+      rec[4] = px[1];
+      rec[5] = px[2];
+      rec[6] = vw[0];
+      rec[7] = vw[1];
+      rec[8] = vw[2];
+      rec[9] = vi.x;
+      rec[10] = vi.y;
+      rec[11] = vi.z;
+      for (int d = 0; d < 3; d++) {
+        rec[12 + d] = (double)dF[d];
+        rec[15 + d] = (double)dTi[d];
+      }
+      // normal = -sidata.en = delta / r (fix_wall_gran_base.h:332-334), in FP64
+      rec[18] = -sd0 * rinv;
+      rec[19] = -sd1 * rinv;
+      rec[20] = -sd2 * rinv;
+      rec[21] = rr * rr;
+      rec[22] = pi.w;
+      for (int d = 0; d < 3; d++) rec[kWallLocalFixedCols + d] = (double)sh[d];
+      int *id = o.ids + 3 * (size_t)(r + rows - 1);
+      id[0] = m;
+      id[1] = t - p.mesh_tri0[m];
+      id[2] = a.aux.tag[i];
+    }
+  }
+  return rows;
+}
+
+template <typename T, bool GEN>
+__global__ __launch_bounds__(kBlock) void k_wall_local_count(StepArgs a, WallLocalArgs o) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  o.count[i] = wall_local_rows<T, GEN, false>(a, i, o, 0);
+}
+
+template <typename T, bool GEN>
+__global__ __launch_bounds__(kBlock) void k_wall_local_fill(StepArgs a, WallLocalArgs o) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  wall_local_rows<T, GEN, true>(a, i, o, o.offset[i]);
+}
+
 // initial integrate of step 1 after setup (the setup force pass has no
 // final integrate), plus the displacement check for step 1. In place.
 template <typename T>
@@ -2198,6 +2390,35 @@ hipError_t launch_pair_local_fill(const StepArgs &a, const PairLocalArgs &o, hip
       hipLaunchKernelGGL((k_pair_local_fill<double, false>), g, b, 0, s, a, o);
   }
   return hipGetLastError();
+}
+
+template <bool FILL>
+static hipError_t launch_wall_local(const StepArgs &a, const WallLocalArgs &o, hipStream_t s) {
+  if (a.n <= 0) return hipSuccess;
+  const dim3 g(nblk(a.n, kBlock)), b(kBlock);
+  if (a.mixed) {
+    if (a.prm.generic)
+      hipLaunchKernelGGL((FILL ? k_wall_local_fill<float, true> : k_wall_local_count<float, true>), g, b, 0, s, a, o);
+    else
+      hipLaunchKernelGGL((FILL ? k_wall_local_fill<float, false> : k_wall_local_count<float, false>), g, b, 0, s, a,
+                         o);
+  } else {
+    if (a.prm.generic)
+      hipLaunchKernelGGL((FILL ? k_wall_local_fill<double, true> : k_wall_local_count<double, true>), g, b, 0, s, a,
+                         o);
+    else
+      hipLaunchKernelGGL((FILL ? k_wall_local_fill<double, false> : k_wall_local_count<double, false>), g, b, 0, s,
+                         a, o);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_wall_local_count(const StepArgs &a, const WallLocalArgs &o, hipStream_t s) {
+  return launch_wall_local<false>(a, o, s);
+}
+
+hipError_t launch_wall_local_fill(const StepArgs &a, const WallLocalArgs &o, hipStream_t s) {
+  return launch_wall_local<true>(a, o, s);
 }
 
 hipError_t launch_initial(const StepArgs &a, hipStream_t s) {

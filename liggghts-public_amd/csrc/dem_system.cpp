@@ -2438,6 +2438,68 @@ int dem_compute_pair_local(dem_handle *h, int64_t cap_out, int *tags, double *da
   return DEM_OK;
 }
 
+static_assert(dem::kWallLocalFixedCols == DEM_WALL_LOCAL_FIXED_COLS, "wall-local record layout and the header disagree");
+
+int dem_compute_wall_local(dem_handle *h, int64_t cap_out, int *ids, double *data, int64_t *nrows, int *ncol) {
+  if (!h || !nrows) return DEM_ERR_ARG;
+  *nrows = 0;
+  const int nc = dem::kWallLocalFixedCols + 3;
+  if (ncol) *ncol = nc;
+  if (h->n <= 0 || !h->have_list || h->meshes.empty()) return DEM_OK;
+  if (cap_out > 0 && (!ids || !data)) return fail(h, DEM_ERR_ARG, "null output arrays");
+  HIPCHK(h, hipSetDevice(h->p.device));
+  int rc = check_ready(h);
+  if (rc) return rc;
+  const int n = (int)h->n;
+  std::vector<void *> tmp;
+  auto tfree = [&]() {
+    for (void *q : tmp) hipFree(q);
+  };
+  auto talloc = [&](void **q, size_t bytes) -> int {
+    if (hipMalloc(q, bytes ? bytes : 16) != hipSuccess) return fail(h, DEM_ERR_HIP, "hipMalloc failed");
+    tmp.push_back(*q);
+    return DEM_OK;
+  };
+  dem::WallLocalArgs o{};
+  o.ncol = nc;
+  int *scan = nullptr;
+  if ((rc = talloc((void **)&o.count, sizeof(int) * n)) || (rc = talloc((void **)&o.offset, sizeof(int) * (n + 1))) ||
+      (rc = talloc((void **)&scan, sizeof(int) * dem::scan_tmp_size(n)))) {
+    tfree();
+    return rc;
+  }
+  dem::StepArgs a = step_args(h, h->cur_st, h->tri_cur);
+  a.shearupdate = 0;
+  hipError_t e = dem::launch_wall_local_count(a, o, h->stream);
+  if (e == hipSuccess) e = dem::launch_exclusive_scan(o.count, o.offset, scan, n, h->stream);
+  int total = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&total, o.offset + n, sizeof(int), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) {
+    tfree();
+    return fail(h, DEM_ERR_HIP, hipGetErrorString(e));
+  }
+  *nrows = total;
+  if (cap_out <= 0 || total == 0) {
+    tfree();
+    return DEM_OK;
+  }
+  if ((rc = talloc((void **)&o.ids, sizeof(int) * 3 * (size_t)total)) ||
+      (rc = talloc((void **)&o.rec, sizeof(double) * (size_t)nc * (size_t)total))) {
+    tfree();
+    return rc;
+  }
+  e = dem::launch_wall_local_fill(a, o, h->stream);
+  const int64_t m = std::min<int64_t>(cap_out, total);
+  if (e == hipSuccess) e = hipMemcpyAsync(ids, o.ids, sizeof(int) * 3 * (size_t)m, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(data, o.rec, sizeof(double) * (size_t)nc * (size_t)m, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  tfree();
+  if (e != hipSuccess) return fail(h, DEM_ERR_HIP, hipGetErrorString(e));
+  return DEM_OK;
+}
+
 int dem_get_pair_rolling(dem_handle *h, int64_t cap_out, double *rhist, int64_t *npairs) {
   if (!h || !npairs) return DEM_ERR_ARG;
   if (h->hw != 6) return fail(h, DEM_ERR_STATE, "no rolling history: no contact model uses epsd2");

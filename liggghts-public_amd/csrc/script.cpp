@@ -194,10 +194,16 @@ struct Interp {
   struct LocalCompute {
     std::vector<std::string> keys;
     int ncols = 0;
+    bool wall = false;      // wall/gran/local
+    std::string wall_fix;   // its fix wall/gran (the last one with meshes)
+    int dnum = 0;           // history values per row
   };
   std::map<std::string, LocalCompute> local_computes;
   std::map<std::string, MeshDef> mesh_defs;
   std::vector<std::string> mesh_order;  // meshes in fix wall/gran order
+  // per mesh: its fix wall/gran and its index among that fix's meshes
+  std::map<std::string, std::pair<std::string, int>> mesh_fix;
+  std::string last_mesh_wall_fix;
   int precision = DEM_FP64;
   // thermo
   int thermo_every = 0;
@@ -692,11 +698,13 @@ struct Interp {
     return (pair_m.tangential_model == 0 ? 3 : 0) + (pair_m.rolling_epsd2 ? 3 : 0);
   }
 
-  // compute ID group pair/gran/local [keywords] (compute_pair_gran_local.cpp:71-124,
-  // the column count of :281-283). Rows come from dem_compute_pair_local.
+  // compute ID group pair/gran/local|wall/gran/local [keywords]
+  // (compute_pair_gran_local.cpp:71-124, the column count of :281-283). Rows
+  // come from dem_compute_pair_local / dem_compute_wall_local. wall/gran/local
+  // takes the mesh contacts of the last fix wall/gran with meshes defined
+  // before it (:247-270).
   void local_compute_command(const std::vector<std::string> &t) {
-    if (t[3] == "wall/gran/local")
-      die("compute wall/gran/local is not available on this path (compute pair/gran/local is)");
+    const bool wall = t[3] == "wall/gran/local";
     if (step > 0) die("Compute pair/gran/local: Need to define this compute before first run");
     if (t[2] != "all") die("compute pair/gran/local: only group all is available on this path");
     static const char *order[] = {"pos", "vel", "id", "force", "force_normal", "force_tangential", "torque",
@@ -724,7 +732,20 @@ struct Interp {
     LocalCompute lc;
     for (const char *o : order)
       if (want.count(o)) lc.keys.push_back(o);
-    const int dnum = pair_dnum();
+    int dnum = pair_dnum();
+    if (wall) {
+      if (last_mesh_wall_fix.empty())
+        die("Compute wall/gran/local requires a fix of type wall/gran using one or more mesh walls. This fix "
+            "has to come before the compute in the script");
+      lc.wall = true;
+      lc.wall_fix = last_mesh_wall_fix;
+      // the shear of tangential history; no rolling history on meshes here
+      dnum = 0;
+      for (const auto &mf : mesh_fix)
+        if (mf.second.first == last_mesh_wall_fix)
+          dnum = mesh_defs[mf.first].model.tangential_model == 0 ? 3 : 0;
+    }
+    lc.dnum = dnum;
     if (want.count("history") && dnum == 0)
       die("Compute pair/gran/local or wall/gran/local can not calculate history values since pair or wall "
           "style does not compute them");
@@ -773,6 +794,7 @@ struct Interp {
   // image), the contact point (x_i r_j + x_j r_i) / (r_i + r_j), the area the
   // sphere-sphere intersection circle's.
   std::vector<std::vector<double>> local_rows(const LocalCompute &lc) {
+    if (lc.wall) return wall_local_rows(lc);
     int64_t np = 0;
     int nc = 0;
     if (dem_compute_pair_local(h, 0, nullptr, nullptr, &np, &nc)) die(dem_last_error(h));
@@ -821,6 +843,61 @@ struct Interp {
           row.push_back(ri + rj - dist);
         } else if (k == "contactPoint") {
           for (int d = 0; d < 3; d++) row.push_back((xi[d] * rj + (xi[d] - del[d]) * ri) / (ri + rj));
+        }
+      }
+    }
+    return rows;
+  }
+
+  // The rows of a wall/gran/local compute: ComputePairGranLocal::add_wall_1 /
+  // add_wall_2 (compute_pair_gran_local.cpp:648-870) from the library's
+  // per-contact record, for the meshes of the compute's fix. Normal parts use
+  // the unit vector from the particle's centre to the contact point; the
+  // reference writes no contactPoint columns for walls (zeros here).
+  // Deviation: rows come particle by particle, the reference's triangle by
+  // triangle.
+  std::vector<std::vector<double>> wall_local_rows(const LocalCompute &lc) {
+    int64_t nr = 0;
+    int nc = 0;
+    if (dem_compute_wall_local(h, 0, nullptr, nullptr, &nr, &nc)) die(dem_last_error(h));
+    std::vector<int> ids(3 * std::max<int64_t>(nr, 1));
+    std::vector<double> rec((size_t)nc * std::max<int64_t>(nr, 1));
+    if (nr > 0 && dem_compute_wall_local(h, nr, ids.data(), rec.data(), &nr, &nc)) die(dem_last_error(h));
+    // library mesh index -> index among the compute's fix's meshes, or -1
+    std::vector<int> fix_index(mesh_order.size(), -1);
+    for (size_t m = 0; m < mesh_order.size(); m++) {
+      const auto &mf = mesh_fix[mesh_order[m]];
+      if (mf.first == lc.wall_fix) fix_index[m] = mf.second;
+    }
+    std::vector<std::vector<double>> rows;
+    for (int64_t q = 0; q < nr; q++) {
+      const int m = ids[3 * q];
+      if (m < 0 || m >= (int)fix_index.size() || fix_index[m] < 0) continue;
+      const double *rc = &rec[(size_t)q * nc];
+      const double *cp = rc, *x = rc + 3, *vw = rc + 6, *v = rc + 9, *fc = rc + 12, *tc = rc + 15, *nrm = rc + 18;
+      const double rsq = rc[21], rad = rc[22];
+      rows.emplace_back();
+      std::vector<double> &row = rows.back();
+      for (const std::string &k : lc.keys) {
+        if (k == "pos") row.insert(row.end(), {cp[0], cp[1], cp[2], x[0], x[1], x[2]});
+        else if (k == "vel") row.insert(row.end(), {vw[0], vw[1], vw[2], v[0], v[1], v[2]});
+        else if (k == "id") row.insert(row.end(), {(double)fix_index[m], (double)ids[3 * q + 1], (double)ids[3 * q + 2]});
+        else if (k == "force") row.insert(row.end(), {fc[0], fc[1], fc[2]});
+        else if (k == "torque") row.insert(row.end(), {tc[0], tc[1], tc[2]});
+        else if (k == "force_normal" || k == "force_tangential" || k == "torque_normal" ||
+                 k == "torque_tangential") {
+          const double *w = k[0] == 'f' ? fc : tc;
+          const double mag = w[0] * nrm[0] + w[1] * nrm[1] + w[2] * nrm[2];
+          const bool normal = k.find("normal") != std::string::npos;
+          for (int d = 0; d < 3; d++) row.push_back(normal ? nrm[d] * mag : w[d] - nrm[d] * mag);
+        } else if (k == "history") {
+          for (int d = 0; d < lc.dnum; d++) row.push_back(rc[DEM_WALL_LOCAL_FIXED_COLS + d]);
+        } else if (k == "contactArea") {
+          row.push_back((rad * rad - rsq) * 3.14159265358979323846);
+        } else if (k == "delta") {
+          row.push_back(rad - std::sqrt(rsq));
+        } else if (k == "contactPoint") {
+          row.insert(row.end(), {0.0, 0.0, 0.0});
         }
       }
     }
@@ -2360,7 +2437,9 @@ struct Interp {
           it->second.used = true;
           it->second.model = wm;
           mesh_order.push_back(t[k + 4 + q]);
+          mesh_fix[t[k + 4 + q]] = {t[1], q};
         }
+        last_mesh_wall_fix = t[1];
         if (!gravity) gravity_first = false;
         wall_seen = true;
         dirty = true;

@@ -25,7 +25,7 @@ EXPORTED = [
     "dem_set_kernel_timing",
     "dem_run_script", "dem_add_mesh", "dem_mesh_rotate", "dem_get_mesh_nodes", "dem_mesh_linear", "dem_mesh_wiggle", "dem_mesh_stress", "dem_get_mesh_force", "dem_mesh_viblin", "dem_mesh_vibrot",
     "dem_write_restart", "dem_read_restart", "dem_restart_info", "dem_restart_particles",
-    "dem_get_mesh_topology", "dem_get_mesh_history", "dem_set_instances", "dem_get_box", "dem_compute_pair_local",
+    "dem_get_mesh_topology", "dem_get_mesh_history", "dem_set_instances", "dem_get_box", "dem_compute_pair_local", "dem_compute_wall_local",
     "dem_mesh_generate",
     "dem_set_instance_materials", "dem_get_instance_stats", "dem_set_step_shape",
     "dem_get_step_shape", "dem_set_graph_capture", "dem_set_pair_model", "dem_set_model_coefficients",
@@ -268,6 +268,7 @@ def load_library(path=LIB_PATH):
                                       P(ctypes.c_int64)]
     lib.dem_get_box.argtypes = [h, dp, dp]
     lib.dem_compute_pair_local.argtypes = [h, ctypes.c_int64, ip, dp, P(ctypes.c_int64), ip]
+    lib.dem_compute_wall_local.argtypes = [h, ctypes.c_int64, ip, dp, P(ctypes.c_int64), ip]
     lib.dem_sq_solve.argtypes = [h, ctypes.c_int, dp, ctypes.c_int]
     lib.dem_sq_get_results.argtypes = [h, dp, dp, dp, ip, ip, dp, dp]
     lib.dem_sq_check.argtypes = [h, ctypes.c_int, ctypes.c_double, dp, dp, dp, ip, dp, dp, dp, dp]
@@ -548,6 +549,22 @@ class GpuSystem:
                                                     ctypes.byref(nc)))
         npairs = int(total.value)
         return tags[:npairs], data[:npairs]
+
+    def wall_local(self):
+        """compute wall/gran/local on the current state: (ids (R,3) = mesh,
+        triangle, tag; data (R,26)), data columns contact point, x, wall
+        velocity, v, force, torque, unit normal, r^2, radius, history (see
+        dem_compute_wall_local)."""
+        total = ctypes.c_int64(0)
+        nc = ctypes.c_int(0)
+        self._check(self.lib.dem_compute_wall_local(self.h, 0, None, None, ctypes.byref(total), ctypes.byref(nc)))
+        cap = max(int(total.value), 1)
+        ids = np.zeros((cap, 3), dtype=np.int32)
+        data = np.zeros((cap, nc.value))
+        self._check(self.lib.dem_compute_wall_local(self.h, cap, _ip(ids), _dp(data), ctypes.byref(total),
+                                                    ctypes.byref(nc)))
+        nrows = int(total.value)
+        return ids[:nrows], data[:nrows]
 
     def box(self):
         """(boxlo, boxhi) at the last neighbor build (shrink-wrapped faces

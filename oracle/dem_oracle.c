@@ -2330,3 +2330,104 @@ static void mesh_wall_compute(orc_sys *s, int im, int shearupdate) {
       }
   }
 }
+
+/* compute wall/gran/local (compute_pair_gran_local.cpp:648-870) through
+ * FixWallGran::post_force_pgl (fix_wall_gran.cpp:752-765: computeflag 0,
+ * shearupdate 0): the contact pass of mesh_wall_compute on private copies of
+ * each particle's slot table and histories, so nothing changes. A row per
+ * evaluated contact (touching, or in the close band, and accepted by
+ * handleContact), mesh by mesh and particle by particle: ids = mesh, triangle,
+ * tag; data = contact point x + delta[3], x[3], wall velocity at the contact[3],
+ * v[3], force[3] and torque[3] on the particle (zero in the close band), unit
+ * normal delta / r[3] (minus the contact's en, fix_wall_gran_base.h:332-334),
+ * r^2 with r = radius + deltan (sidata.rsq, fix_wall_gran_base.h:181-182),
+ * the radius, shear history[3]. Returns the number of rows (negated when above cap). */
+int64_t orc_wall_local(const orc_sys *s, int64_t cap, int *ids, double *data) {
+  enum { NC = 26 };
+  int64_t nr = 0;
+  for (int im = 0; im < s->nmesh; im++) {
+    const orc_mesh *m = &s->mesh[im];
+    for (int i = 0; i < s->n; i++) {
+      int slot[MESH_MAXSLOT];
+      unsigned char keep[MESH_MAXSLOT];
+      double hist[MESH_MAXSLOT][3];
+      for (int k = 0; k < MESH_MAXSLOT; k++) {
+        slot[k] = SLOT(m, i, k);
+        keep[k] = 0;
+        memcpy(hist[k], SHIST(m, i, k), sizeof hist[k]);
+      }
+      for (int c = 0; c < m->ncand[i]; c++) {
+        const int t = m->cand[(size_t)i * MESH_MAXCAND + c];
+        double delta[3], bary[3];
+        int barysign;
+        const double deltan = resolve_tri_sphere(m, t, s->r[i], &s->x[3 * i], delta, bary, &barysign);
+        if (deltan > s->cutneighmax) continue;
+        if (!(deltan <= 0 || deltan < (s->p.cdf - 1.0) * s->r[i])) continue;
+        /* mesh_handle_contact on the copies */
+        int ic = -1;
+        for (int k = 0; k < MESH_MAXSLOT; k++)
+          if (slot[k] == t) {
+            ic = k;
+            break;
+          }
+        if (ic >= 0) {
+          keep[ic] = 1;
+        } else {
+          int skip = 0;
+          if (7 == barysign)
+            for (int k = 0; k < MESH_MAXSLOT; k++)
+              if (slot[k] >= 0 && slot[k] != t && coplanar_node_neighs(m, slot[k], t) && keep[k]) skip = 1;
+          if (skip) continue;
+          for (int k = 0; k < MESH_MAXSLOT; k++)
+            if (slot[k] == -1) {
+              ic = k;
+              break;
+            }
+          if (ic < 0) abort();
+          slot[ic] = t;
+          keep[ic] = 1;
+          hist[ic][0] = hist[ic][1] = hist[ic][2] = 0.0;
+          if (7 == barysign)
+            for (int k = 0; k < MESH_MAXSLOT; k++)
+              if (slot[k] >= 0 && slot[k] != t && coplanar_node_neighs(m, slot[k], t))
+                memcpy(hist[ic], hist[k], sizeof hist[ic]);
+        }
+        double *h = hist[ic];
+        double vwall[3] = {0., 0., 0.};
+        if (m->moving)
+          for (int k = 0; k < 3; k++)
+            vwall[k] = (bary[0] * m->vnode[9 * t + k] + bary[1] * m->vnode[9 * t + 3 + k] +
+                        bary[2] * m->vnode[9 * t + 6 + k]);
+        double dF[3] = {0., 0., 0.}, dT[3] = {0., 0., 0.};
+        if (!(deltan <= 0)) {
+          if (m->model.tangential_model == 0) h[0] = h[1] = h[2] = 0.0;
+        } else {
+          const double sdelta[3] = {-delta[0], -delta[1], -delta[2]};
+          orc_contact_wall(&m->model, &s->m, s->type[i], m->wall_type, -deltan, sdelta, s->r[i],
+                           &s->v[3 * i], &s->w[3 * i], vwall, s->rm[i], s->p.dt, 0, h, dF, dT);
+        }
+        if (nr < cap) {
+          double *row = data + nr * NC;
+          const double rr = s->r[i] + deltan;
+          for (int d = 0; d < 3; d++) {
+            row[d] = s->x[3 * i + d] + delta[d];
+            row[3 + d] = s->x[3 * i + d];
+            row[6 + d] = vwall[d];
+            row[9 + d] = s->v[3 * i + d];
+            row[12 + d] = dF[d];
+            row[15 + d] = dT[d];
+            row[18 + d] = delta[d] * (1.0 / rr);
+            row[23 + d] = h[d];
+          }
+          row[21] = rr * rr;
+          row[22] = s->r[i];
+          ids[3 * nr] = im;
+          ids[3 * nr + 1] = t;
+          ids[3 * nr + 2] = s->tag[i];
+        }
+        nr++;
+      }
+    }
+  }
+  return nr > cap ? -nr : nr;
+}

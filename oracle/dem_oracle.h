@@ -166,6 +166,9 @@ int64_t orc_get_pairs(const orc_sys *s, int64_t cap, int *tags, int *flags, doub
 /* compute pair/gran/local's default columns per half-list pair, oriented
  * from the lower tag (see dem_oracle.c); data [pairs][21 + 3 or 6] */
 int64_t orc_pair_local(const orc_sys *s, int64_t cap, int *tags, double *data);
+/* compute wall/gran/local rows of the mesh contacts (26 columns), changing
+ * nothing; negative: -(rows) when above cap */
+int64_t orc_wall_local(const orc_sys *s, int64_t cap, int *ids, double *data);
 void orc_get_wall_hist(const orc_sys *s, double *hist /* n*nplanes*3 */);
 /* rolling-torque histories (epsd2), in the orders of orc_get_pairs / orc_get_wall_hist */
 int64_t orc_get_pair_rolling(const orc_sys *s, int64_t cap, double *rhist);

@@ -121,6 +121,8 @@ def load():
     lib.orc_get_pairs.restype = ctypes.c_int64
     lib.orc_pair_local.argtypes = [ctypes.c_void_p, ctypes.c_int64, ip, dp]
     lib.orc_pair_local.restype = ctypes.c_int64
+    lib.orc_wall_local.argtypes = [ctypes.c_void_p, ctypes.c_int64, ip, dp]
+    lib.orc_wall_local.restype = ctypes.c_int64
     lib.orc_get_wall_hist.argtypes = [ctypes.c_void_p, dp]
     lib.orc_lost_atoms.argtypes = [ctypes.c_void_p, P(ctypes.c_int64)]
     lib.orc_lost_atoms.restype = ctypes.c_int64
@@ -466,6 +468,17 @@ class OracleSystem:
         if n < 0:
             raise RuntimeError("orc_pair_local: pair count changed")
         return tags[:n], data[:n]
+
+    def wall_local(self):
+        """compute wall/gran/local's rows of the mesh contacts
+        (orc_wall_local): ids (R,3) = mesh, triangle, tag; data (R,26)."""
+        cap = max(-int(self.lib.orc_wall_local(self.s, 0, None, None)), 1)
+        ids = np.zeros((cap, 3), dtype=np.int32)
+        data = np.zeros((cap, 26))
+        n = self.lib.orc_wall_local(self.s, cap, _ip(ids), _dp(data))
+        if n < 0:
+            raise RuntimeError("orc_wall_local: row count changed")
+        return ids[:n], data[:n]
 
     def pairs(self):
         # ask for the count first (a negative return is minus the pairs held)
