@@ -96,7 +96,7 @@ def measured_traffic(workload, mode):
     return None
 
 
-TUNE_STEPS = 600  # covers every candidate tile shape (8 steps each, up to 3 tries)
+TUNE_STEPS = 800  # covers every candidate tile shape (8 steps each, up to 3 tries)
 
 
 def tune_shape(params, mats, P, walls, precision, meshes=(), instances=None, instance_materials=None):
@@ -123,6 +123,8 @@ def run_mode(params, mats, P, walls, precision, steps, warmup, timing_steps, mes
                         instances=instances, instance_materials=instance_materials)
     if tuned is not None:
         g.set_step_shape(tuned[0], tuned[1])
+        print("# tuned shape (variant, persist): " + ", ".join(
+            f"({c[0]},{c[1]}) {c[2]:.1f} us" for c in tuned[2] if c[2] > 0), file=sys.stderr, flush=True)
     if graphs:
         g.set_graph_capture(True)
     g.run(warmup)

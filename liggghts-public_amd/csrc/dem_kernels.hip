@@ -41,14 +41,11 @@ __device__ __forceinline__ int cell_coord(const DevParams &p, int d, double x) {
   return c;
 }
 
-// x_i - x_j with the minimum-image shift applied to the difference; exactly
+// The minimum-image shift of a difference x_i - x_j: a component beyond half
+// the period of a periodic dimension moves by one period. Exactly
 // antisymmetric under i <-> j.
-__device__ __forceinline__ void min_image(const DevParams &p, double xi, double yi, double zi,
-                                          double xj, double yj, double zj, double &dx,
-                                          double &dy, double &dz) {
-  dx = xi - xj;
-  dy = yi - yj;
-  dz = zi - zj;
+__device__ __forceinline__ void apply_image(const DevParams &p, double &dx, double &dy,
+                                            double &dz) {
   if (p.per[0]) {
     if (dx > p.halfL[0]) dx -= p.L[0];
     else if (dx < -p.halfL[0]) dx += p.L[0];
@@ -61,6 +58,25 @@ __device__ __forceinline__ void min_image(const DevParams &p, double xi, double 
     if (dz > p.halfL[2]) dz -= p.L[2];
     else if (dz < -p.halfL[2]) dz += p.L[2];
   }
+}
+
+// Whether apply_image would change the difference. The callers apply the
+// shift only on lanes that need it, so a wavefront with no periodic-image
+// partner skips that code; apply_image is the identity elsewhere, so the
+// results are the same bits either way.
+__device__ __forceinline__ bool needs_image(const DevParams &p, double dx, double dy, double dz) {
+  return (p.per[0] & (fabs(dx) > p.halfL[0])) | (p.per[1] & (fabs(dy) > p.halfL[1])) |
+         (p.per[2] & (fabs(dz) > p.halfL[2]));
+}
+
+// x_i - x_j with the minimum-image shift
+__device__ __forceinline__ void min_image(const DevParams &p, double xi, double yi, double zi,
+                                          double xj, double yj, double zj, double &dx,
+                                          double &dy, double &dz) {
+  dx = xi - xj;
+  dy = yi - yj;
+  dz = zi - zj;
+  apply_image(p, dx, dy, dz);
 }
 
 
@@ -829,12 +845,24 @@ __device__ __forceinline__ uint32_t broad_phase(const StepArgs &a, const Model &
   double4 pj[GROUP];
 #pragma unroll
   for (int u = 0; u < GROUP; u++) pj[u] = a.in.pos[ent[u] & kIndexMask];
+  double dx[GROUP], dy[GROUP], dz[GROUP];
+  bool wrap = false;
+#pragma unroll
+  for (int u = 0; u < GROUP; u++) {
+    dx[u] = pi.x - pj[u].x;
+    dy[u] = pi.y - pj[u].y;
+    dz[u] = pi.z - pj[u].z;
+    wrap |= needs_image(p, dx[u], dy[u], dz[u]);
+  }
+  // the minimum-image shift only on lanes with a periodic-image partner
+  if (wrap) {
+#pragma unroll
+    for (int u = 0; u < GROUP; u++) apply_image(p, dx[u], dy[u], dz[u]);
+  }
   uint32_t bits = 0u, close = 0u;
 #pragma unroll
   for (int u = 0; u < GROUP; u++) {
-    double dx, dy, dz;
-    min_image(p, pi.x, pi.y, pi.z, pj[u].x, pj[u].y, pj[u].z, dx, dy, dz);
-    const double rsq = dx * dx + dy * dy + dz * dz;
+    const double rsq = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
     const double radsum = pi.w + pj[u].w;
     if (k0 + u < cnt) {
       if (rsq < radsum * radsum)
@@ -953,17 +981,131 @@ __device__ __forceinline__ void pair_forces_wave(const StepArgs &a, const Model 
   }
 }
 
+// LDS of one wavefront for the row-queued pair phase: the touching entries of
+// a round of list slots of the 64 particles, particle by particle (owner lane
+// << 16 | list slot), and the contributions of one pass of the contact law.
+constexpr int kRowQueue = 256;
+template <typename T>
+struct RowLds {
+  uint32_t queue[kRowQueue];
+  T contrib[6][64];
+};
+
+template <typename T, int GROUP, int WAVE>
+struct PairLdsT {
+  typedef WaveLds<T, GROUP> type;
+};
+template <typename T, int GROUP>
+struct PairLdsT<T, GROUP, 2> {
+  typedef RowLds<T> type;
+};
+
+// Pair forces of the wavefront's 64 particles with the touching entries of
+// whole rows queued: each lane broad-phases rounds of 32 list slots of its
+// particle (GROUP entries in flight at a time) and keeps the touching ones as
+// bits; an exclusive scan of the counts gives each particle a contiguous range
+// of the queue, its entries in slot order; the wavefront then runs the
+// contact law on 64 queued entries at a time, and each particle adds the
+// contributions of its range in that order, so its sums are bitwise those of
+// the per-lane path. A round of touching entries longer than the queue is
+// taken in windows of kRowQueue. Every lane of the wavefront takes part;
+// lanes past the last particle have cnt = 0.
+template <typename T, bool GEN, int GROUP>
+__device__ __forceinline__ void pair_forces_rowq(const StepArgs &a, const Model &pm, const int i,
+                                                 const int pad, const int cnt, const double4 &pi,
+                                                 T &fx, T &fy, T &fz, T &tx, T &ty, T &tz,
+                                                 unsigned long long &my_contacts, RowLds<T> &L) {
+  static_assert(32 % GROUP == 0, "a round holds whole groups");
+  typedef typename OmgT<T>::type omg_t;
+  const DevParams &p = a.prm;
+  const size_t cap = (size_t)a.cap;
+  const int lane = threadIdx.x & 63;
+  const int ibase = i - lane;  // particle of lane 0
+  int kmax = cnt;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(kmax, off, 64);
+    kmax = o > kmax ? o : kmax;
+  }
+  for (int r0 = 0; r0 < kmax; r0 += 32) {
+    const int rend = kmax < r0 + 32 ? kmax : r0 + 32;
+    uint32_t tmask = 0u;
+    for (int k0 = r0; k0 < rend; k0 += GROUP) {
+      uint32_t ent[GROUP];
+      tmask |= broad_phase<T, GEN, GROUP>(a, pm, i, pad, cnt, k0, pi, ent) << (k0 - r0);
+    }
+    const int c = __popc(tmask);
+    my_contacts += (unsigned long long)c;
+    int incl = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += o;
+    }
+    const int total = __shfl(incl, 63, 64);
+    const int first = incl - c;
+    for (int w0 = 0; w0 < total; w0 += kRowQueue) {
+      const int wend = total < w0 + kRowQueue ? total : w0 + kRowQueue;
+      uint32_t m = tmask;
+      for (int q = first; m != 0u; q++) {
+        const int u = __builtin_ctz(m);
+        m &= m - 1u;
+        if (q >= w0 && q < wend) L.queue[q - w0] = ((uint32_t)lane << 16) | (uint32_t)(r0 + u);
+      }
+      wave_sync();
+      for (int c0 = w0; c0 < wend; c0 += 64) {
+        const int q = c0 + lane;
+        T dF[3] = {T(0), T(0), T(0)}, dTi[3] = {T(0), T(0), T(0)};
+        if (q < wend) {
+          const uint32_t qe = L.queue[q - w0];
+          const int ol = (int)(qe >> 16);
+          const int io = ibase + ol;
+          const int k = (int)(qe & 0xffffu);
+          const uint32_t e = a.list.nbr[(size_t)k * cap + io];
+          const double4 po = a.in.pos[io];
+          const double4 vo = a.in.vel[io];
+          const omg_t wo = ((const omg_t *)a.in.omg)[io];
+          const int wb = (int)wo.w;
+          const int orow = (wb >> kTypeBits) * p.mat_stride + (wb & kTypeMask) * p.ntp1;
+          pair_contact<T, GEN>(a, pm, io, k, e, po, vo, wo, orow, dF, dTi);
+        }
+        L.contrib[0][lane] = dF[0];
+        L.contrib[1][lane] = dF[1];
+        L.contrib[2][lane] = dF[2];
+        L.contrib[3][lane] = dTi[0];
+        L.contrib[4][lane] = dTi[1];
+        L.contrib[5][lane] = dTi[2];
+        wave_sync();
+        // this particle's entries of the pass, in slot order
+        const int lo = first > c0 ? first : c0;
+        const int hi = first + c < c0 + 64 ? first + c : c0 + 64;
+        for (int qq = lo; qq < hi; qq++) {
+          const int l = qq - c0;
+          fx += L.contrib[0][l];
+          fy += L.contrib[1][l];
+          fz += L.contrib[2][l];
+          tx += L.contrib[3][l];
+          ty += L.contrib[4][l];
+          tz += L.contrib[5][l];
+        }
+        wave_sync();
+      }
+    }
+  }
+}
+
 // One particle's step: contacts over its list, walls, meshes, gravity,
 // integration and the displacement test. Accumulates its touching pair and
-// wall contacts. WAVE: the pair phase is compacted over the wavefront
-// (pair_forces_wave; every lane calls, active says whether it holds a
-// particle); otherwise each lane resolves its own touching entries.
-template <typename T, bool MESH, int GROUP, bool GEN, bool WAVE>
+// wall contacts. WAVE 1: the pair phase is compacted over the wavefront
+// group by group (pair_forces_wave), WAVE 2: row by row (pair_forces_rowq);
+// there every lane calls, active says whether it holds a particle.
+// WAVE 0: each lane resolves its own touching entries.
+template <typename T, bool MESH, int GROUP, bool GEN, int WAVE>
 __device__ __forceinline__ void step_particle(const StepArgs &a, const int i, const bool active,
                                               unsigned long long &my_contacts,
                                               unsigned long long &my_wall, int &my_inst,
                                               bool &displaced, const long long ago_next,
-                                              WaveLds<T, GROUP> *lds) {
+                                              typename PairLdsT<T, GROUP, WAVE>::type *lds) {
   typedef typename OmgT<T>::type omg_t;
   const DevParams &p = a.prm;
 
@@ -983,19 +1125,24 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i, co
   T fx = T(0), fy = T(0), fz = T(0), tx = T(0), ty = T(0), tz = T(0);
 
   if (WAVE) {
-    pair_forces_wave<T, GEN, GROUP>(a, pm, i, ii, cnt, pi, fx, fy, fz, tx, ty, tz, my_contacts, *lds);
+    if constexpr (WAVE == 2)
+      pair_forces_rowq<T, GEN, GROUP>(a, pm, i, ii, cnt, pi, fx, fy, fz, tx, ty, tz, my_contacts, *lds);
+    else
+      pair_forces_wave<T, GEN, GROUP>(a, pm, i, ii, cnt, pi, fx, fy, fz, tx, ty, tz, my_contacts, *lds);
     if (!active) return;
   } else {
-    // Touching entries of a group are resolved one by one, reloading their
-    // (cache-resident) index so the group's registers are free during the
-    // contact law.
+    // Touching entries of a group are resolved one by one, their list
+    // entries selected from the group's registers.
     for (int k0 = 0; k0 < cnt; k0 += GROUP) {
       uint32_t ent[GROUP];
       uint32_t bits = broad_phase<T, GEN, GROUP>(a, pm, i, i, cnt, k0, pi, ent);
       while (bits) {
-        const int k = k0 + __builtin_ctz(bits);
+        const int u = __builtin_ctz(bits);
+        const int k = k0 + u;
         bits &= bits - 1u;
-        const uint32_t e = a.list.nbr[(size_t)k * a.cap + i];
+        uint32_t e = ent[0];
+#pragma unroll
+        for (int v = 1; v < GROUP; v++) e = u == v ? ent[v] : e;
         T dF[3], dTi[3];
         pair_contact<T, GEN>(a, pm, i, k, e, pi, vi, wi, irow, dF, dTi);
         fx += dF[0];
@@ -1205,7 +1352,7 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i, co
 // meshes into the next step's geometry buffer. Every tile shape computes
 // the same arithmetic in the same order, so results do not depend on it.
 // ---------------------------------------------------------------------------
-template <typename T, bool MESH, int BLOCK, int GROUP, bool GEN = false, bool WAVE = false>
+template <typename T, bool MESH, int BLOCK, int GROUP, bool GEN = false, int WAVE = 0>
 __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a) {
   // in a captured graph the step index is the control block's base plus this
   // launch's offset (a.step)
@@ -1220,8 +1367,9 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a) {
     return;
   }
   const DevParams &p = a.prm;
-  __shared__ WaveLds<T, GROUP> lds_all[WAVE ? BLOCK / 64 : 1];
-  WaveLds<T, GROUP> *lds = &lds_all[WAVE ? (threadIdx.x >> 6) : 0];
+  typedef typename PairLdsT<T, GROUP, WAVE>::type Lds;
+  __shared__ Lds lds_all[WAVE ? BLOCK / 64 : 1];
+  Lds *lds = &lds_all[WAVE ? (threadIdx.x >> 6) : 0];
   unsigned long long my_contacts = 0, my_wall = 0;
   bool displaced = false;
   const int ntiles = (a.n + BLOCK - 1) / BLOCK;
@@ -1959,13 +2107,24 @@ __global__ __launch_bounds__(kBlock) void k_build(RebuildArgs a) {
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) pj[u] = a.alt.pos[jj[u]];
+        double ddx[4], ddy[4], ddz[4];
+        bool wrap = false;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          ddx[u] = pi.x - pj[u].x;
+          ddy[u] = pi.y - pj[u].y;
+          ddz[u] = pi.z - pj[u].z;
+          wrap |= needs_image(p, ddx[u], ddy[u], ddz[u]);
+        }
+        if (wrap) {
+#pragma unroll
+          for (int u = 0; u < 4; u++) apply_image(p, ddx[u], ddy[u], ddz[u]);
+        }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
           const int j = jj[u];
           if (j == i) continue;  // i itself, or past the row's candidates
-          double ddx, ddy, ddz;
-          min_image(p, pi.x, pi.y, pi.z, pj[u].x, pj[u].y, pj[u].z, ddx, ddy, ddz);
-          const double rsq = ddx * ddx + ddy * ddy + ddz * ddz;
+          const double rsq = ddx[u] * ddx[u] + ddy[u] * ddy[u] + ddz[u] * ddz[u];
           const double radsum = (pi.w + pj[u].w) * p.cdf;
           const double cutsq = (radsum + p.skin) * (radsum + p.skin);
           if (!(rsq <= cutsq)) continue;
@@ -2317,7 +2476,9 @@ static void launch_step_variant(const StepArgs &a, int nb, hipStream_t s) {
     // the sibling contact models run in the default tile shape or its
     // wave-compacted form
     if (a.variant == 5)
-      hipLaunchKernelGGL((k_step<T, MESH, 256, 4, true, true>), dim3(nb), dim3(B), 0, s, a);
+      hipLaunchKernelGGL((k_step<T, MESH, 256, 4, true, 1>), dim3(nb), dim3(B), 0, s, a);
+    else if (a.variant == 7)
+      hipLaunchKernelGGL((k_step<T, MESH, 256, 4, true, 2>), dim3(nb), dim3(B), 0, s, a);
     else
       hipLaunchKernelGGL((k_step<T, MESH, 256, 4, true>), dim3(nb), dim3(B), 0, s, a);
     return;
@@ -2336,10 +2497,16 @@ static void launch_step_variant(const StepArgs &a, int nb, hipStream_t s) {
       hipLaunchKernelGGL((k_step<T, MESH, 64, 4>), dim3(nb), dim3(B), 0, s, a);
       break;
     case 5:
-      hipLaunchKernelGGL((k_step<T, MESH, 256, 4, false, true>), dim3(nb), dim3(B), 0, s, a);
+      hipLaunchKernelGGL((k_step<T, MESH, 256, 4, false, 1>), dim3(nb), dim3(B), 0, s, a);
       break;
     case 6:
-      hipLaunchKernelGGL((k_step<T, MESH, 256, 8, false, true>), dim3(nb), dim3(B), 0, s, a);
+      hipLaunchKernelGGL((k_step<T, MESH, 256, 8, false, 1>), dim3(nb), dim3(B), 0, s, a);
+      break;
+    case 7:
+      hipLaunchKernelGGL((k_step<T, MESH, 256, 4, false, 2>), dim3(nb), dim3(B), 0, s, a);
+      break;
+    case 8:
+      hipLaunchKernelGGL((k_step<T, MESH, 256, 8, false, 2>), dim3(nb), dim3(B), 0, s, a);
       break;
     default:
       hipLaunchKernelGGL((k_step<T, MESH, 256, 4>), dim3(nb), dim3(B), 0, s, a);
@@ -2349,7 +2516,7 @@ static void launch_step_variant(const StepArgs &a, int nb, hipStream_t s) {
 
 hipError_t launch_step(const StepArgs &a, hipStream_t s) {
   if (a.variant < 0 || a.variant >= kNumStepVariants) return hipErrorInvalidValue;
-  if (a.prm.generic && a.variant != 0 && a.variant != 5) return hipErrorInvalidValue;
+  if (a.prm.generic && a.variant != 0 && a.variant != 5 && a.variant != 7) return hipErrorInvalidValue;
   const bool mesh = a.prm.nmesh > 0;
   // pblocks (tiles walked by the particle blocks) is set by the caller;
   // trailing blocks move the meshes when asked to

@@ -1654,7 +1654,7 @@ int dem_set_step_shape(dem_handle *h, int variant, int persist) {
     tu.active = true;
     const bool gen = uses_generic_models(h);
     for (int v = 0; v < dem::kNumStepVariants; v++) {
-      if (gen && v != 0 && v != 5) continue;  // the sibling models' shapes
+      if (gen && v != 0 && v != 5 && v != 7) continue;  // the sibling models' shapes
       for (int f : {0, 4, 8}) tu.cand.push_back({v, f});
     }
     tu.us.assign(tu.cand.size(), 0.0);

@@ -229,10 +229,10 @@ def test_step_shapes_are_bitwise_equal():
     state: they only change which thread walks which particles."""
     say("[gpu] step kernel shapes")
     params, mats, P, walls = wl.dense_pack(seed=33, n=3000, rolling=True, speed=0.3)
-    ref = None
-    shapes = [(v, f) for v in range(7) for f in (0, 4)] + [(-1, 0)]
+    ref = {}
+    shapes = [(v, f) for v in range(9) for f in (0, 4)] + [(-1, 0)]
     for v, f in shapes:
-        for prec in (pydem.DEM_FP64,):
+        for prec in (pydem.DEM_FP64, pydem.DEM_MIXED):
             g = pydem.GpuSystem(params, mats, P, walls, precision=prec)
             g.set_step_shape(v, f)
             g.run(160 if v == -1 else 40)
@@ -243,10 +243,9 @@ def test_step_shapes_are_bitwise_equal():
                 g.close()
                 continue
             st = g.state()
-            if ref is None:
-                ref = st
+            ref.setdefault(prec, st)
             for k in ("x", "v", "omega", "f", "torque"):
-                assert np.array_equal(st[k], ref[k]), (v, f, k)
+                assert np.array_equal(st[k], ref[prec][k]), (v, f, prec, k)
             g.close()
     # the autotuned run against the default shape over the same 160 steps
     a = pydem.GpuSystem(params, mats, P, walls)
@@ -264,7 +263,7 @@ def test_step_shapes_are_bitwise_equal():
 @pytest.mark.parametrize("prec", [pydem.DEM_FP64, pydem.DEM_MIXED])
 def test_sibling_models_wave_compaction_is_bitwise(prec):
     """The sibling contact models' kernel (hooke + no_history + sjkr pairs,
-    a hertz/stiffness + epsd2 floor) in its per-lane and wavefront-compacted
+    a hertz/stiffness + epsd2 floor) in its per-lane, wavefront-compacted and row-queued
     forms: bitwise the same state and pair histories after 60 steps."""
     say("[gpu] sibling models: per-lane vs wave-compacted step kernel")
     params, mats, P, walls = wl.dense_pack(seed=34, n=2500, rolling=False, speed=0.3)
@@ -273,17 +272,18 @@ def test_sibling_models_wave_compaction_is_bitwise(prec):
                 rolling_friction=[[0.1]], **STIFF)
     walls = [(2, params["boxlo"][2], 1, {"normal": "hertz/stiffness", "rolling_epsd2": 1})]
     out = []
-    for v in (0, 5):
+    for v in (0, 5, 7):
         g = pydem.GpuSystem(params, mats, P, walls, precision=prec)
         g.set_step_shape(v, 0)
         g.run(60)
         out.append((g.state(), g.pairs(), g.wall_history()))
         g.close()
-    for k in ("x", "v", "omega", "f", "torque"):
-        assert np.array_equal(out[0][0][k], out[1][0][k]), k
-    for a, b in zip(out[0][1], out[1][1]):
-        assert np.array_equal(a, b)
-    assert np.array_equal(out[0][2], out[1][2])
+    for o in out[1:]:
+        for k in ("x", "v", "omega", "f", "torque"):
+            assert np.array_equal(out[0][0][k], o[0][k]), k
+        for a, b in zip(out[0][1], o[1]):
+            assert np.array_equal(a, b)
+        assert np.array_equal(out[0][2], o[2])
 
 
 def test_run_continue_is_bitwise_one_run():
