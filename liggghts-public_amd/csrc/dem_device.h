@@ -167,8 +167,20 @@ struct MeshStep {
 
 struct ListBuf {
   uint32_t *nbr;  // [W][cap] neighbor index | contact flag
-  void *hist;     // [W][hist_w][cap] tangential (+ rolling) history, double or float
+  void *hist;     // [W][cap][hist_rec] tangential (+ rolling) history, double or float
 };
+
+// The history values of one list entry sit together in a record of
+// hist_rec(hist_w) values (4 for 3, 8 for 6: the shear in values 0-2, the
+// epsd2 rolling torque in 4-6), the records of one list row consecutive, so
+// one entry's history is one aligned vector load and store.
+__host__ __device__ constexpr int hist_rec(int hist_w) { return hist_w <= 4 ? 4 : 8; }
+// offset of history value d (0-2 shear, 3-5 rolling torque) in its record
+__host__ __device__ constexpr int hist_off(int d) { return d < 3 ? d : d + 1; }
+// first value of the record of list entry (slot k, particle i)
+__host__ __device__ inline size_t hist_at(size_t k, size_t i, size_t cap, size_t rec) {
+  return (k * cap + i) * rec;
+}
 
 // Rigid rotation of the moving meshes for one step.
 // fix move/mesh movement types

@@ -84,6 +84,27 @@ template <typename T> struct OmgT;
 template <> struct OmgT<double> { typedef double4 type; };
 template <> struct OmgT<float> { typedef float4 type; };
 
+// three history values of a record (a 4-vector load / store; the fourth
+// value of a record is padding and is written as zero)
+template <typename T>
+__device__ __forceinline__ void load_rec3(const T *p, T v[3]) {
+  typedef typename OmgT<T>::type V4;
+  const V4 r = *(const V4 *)p;
+  v[0] = r.x;
+  v[1] = r.y;
+  v[2] = r.z;
+}
+template <typename T>
+__device__ __forceinline__ void store_rec3(T *p, const T v[3]) {
+  typedef typename OmgT<T>::type V4;
+  V4 r;
+  r.x = v[0];
+  r.y = v[1];
+  r.z = v[2];
+  r.w = T(0);
+  *(V4 *)p = r;
+}
+
 // ---------------------------------------------------------------------------
 // contact law, in the order of ContactModel::surfacesIntersect
 // (contact_models.h:228-238): SurfaceModel<DEFAULT> (surface_model_default.h:
@@ -716,8 +737,10 @@ __device__ __forceinline__ void close_entries(const StepArgs &a, const Model &pm
     const int k = k0 + u;
     const uint32_t e = ent[u];
     if (keeps) {
-      const size_t hb = (size_t)k * hw * cap + i;
-      for (size_t d = 0; d < hw; d++) hist[hb + d * cap] = T(0);
+      const T z[3] = {T(0), T(0), T(0)};
+      const size_t hb = hist_at(k, i, cap, hist_rec((int)hw));
+      store_rec3(hist + hb, z);
+      if (hw == 6) store_rec3(hist + hb + 4, z);
     }
     const uint32_t ne = (normal_bit || (e & kCarryBit)) ? (e | kFlagBit) : (e & ~kFlagBit);
     if (ne != e) a.list.nbr[(size_t)k * cap + i] = ne;
@@ -797,16 +820,10 @@ __device__ __forceinline__ void pair_contact(const StepArgs &a, const Model &pm,
   // written at the first touch and are neither read nor rewritten
   const bool keeps = !(GEN && pm.nohist);
   T sh[3], rh[3] = {T(0), T(0), T(0)};
-  const size_t hb = (size_t)k * hw * cap + i;
+  const size_t hb = hist_at(k, i, cap, hist_rec((int)hw));
   if (flagged && keeps) {
-    sh[0] = hist[hb];
-    sh[1] = hist[hb + cap];
-    sh[2] = hist[hb + 2 * cap];
-    if (GEN && pm.epsd2) {
-      rh[0] = hist[hb + 3 * cap];
-      rh[1] = hist[hb + 4 * cap];
-      rh[2] = hist[hb + 5 * cap];
-    }
+    load_rec3(hist + hb, sh);
+    if (GEN && pm.epsd2) load_rec3(hist + hb + 4, rh);
   } else {
     sh[0] = sh[1] = sh[2] = T(0);
   }
@@ -814,14 +831,8 @@ __device__ __forceinline__ void pair_contact(const StepArgs &a, const Model &pm,
   contact_chain<T, false, GEN>(pm, c, (T)p.dt, STORE ? a.shearupdate : 0, sh, rh, dF, dTi, dTj);
   if (!STORE) return;
   if ((a.shearupdate && keeps) || !flagged) {
-    hist[hb] = sh[0];
-    hist[hb + cap] = sh[1];
-    hist[hb + 2 * cap] = sh[2];
-    if (GEN && hw == 6) {
-      hist[hb + 3 * cap] = rh[0];
-      hist[hb + 4 * cap] = rh[1];
-      hist[hb + 5 * cap] = rh[2];
-    }
+    store_rec3(hist + hb, sh);
+    if (GEN && hw == 6) store_rec3(hist + hb + 4, rh);
   }
   if (!flagged) a.list.nbr[(size_t)k * cap + i] = e | kFlagBit;
 }
@@ -981,12 +992,19 @@ __device__ __forceinline__ void pair_forces_wave(const StepArgs &a, const Model 
   }
 }
 
-// LDS of one wavefront for the row-queued pair phase: the touching entries of
-// a round of list slots of the 64 particles, particle by particle (owner lane
-// << 16 | list slot), and the contributions of one pass of the contact law.
+// LDS of one wavefront for the row-queued pair phase: the records of the
+// wavefront's 64 particles, the first kRowStash touching list entries of each
+// particle in the current round of list slots, the queue of the round's
+// touching entries particle by particle (owner lane << 21 | touching rank << 16
+// | list slot), and the contributions of one pass of the contact law.
 constexpr int kRowQueue = 256;
+constexpr int kRowStash = 8;
 template <typename T>
 struct RowLds {
+  double4 pos[64];
+  double4 vel[64];
+  typename OmgT<T>::type omg[64];
+  uint32_t stash[kRowStash][64];
   uint32_t queue[kRowQueue];
   T contrib[6][64];
 };
@@ -1001,19 +1019,21 @@ struct PairLdsT<T, GROUP, 2> {
 };
 
 // Pair forces of the wavefront's 64 particles with the touching entries of
-// whole rows queued: each lane broad-phases rounds of 32 list slots of its
-// particle (GROUP entries in flight at a time) and keeps the touching ones as
-// bits; an exclusive scan of the counts gives each particle a contiguous range
-// of the queue, its entries in slot order; the wavefront then runs the
-// contact law on 64 queued entries at a time, and each particle adds the
-// contributions of its range in that order, so its sums are bitwise those of
-// the per-lane path. A round of touching entries longer than the queue is
-// taken in windows of kRowQueue. Every lane of the wavefront takes part;
-// lanes past the last particle have cnt = 0.
+// whole rows queued. Each lane broad-phases rounds of 32 list slots of its
+// particle (GROUP entries in flight at a time), keeps the touching ones as
+// bits and stashes their list entries in LDS; an exclusive scan of the
+// counts gives each particle a contiguous range of the queue, its entries in
+// slot order; the wavefront then runs the contact law on 64 queued entries at
+// a time, with the owners' records and list entries read from LDS, and each
+// particle adds the contributions of its range in that order, so its sums are
+// bitwise those of the per-lane path. A round of touching entries longer than
+// the queue is taken in windows of kRowQueue. Every lane of the wavefront
+// takes part; lanes past the last particle have cnt = 0. The particle records
+// are in L.pos / L.vel / L.omg, staged by the caller.
 template <typename T, bool GEN, int GROUP>
 __device__ __forceinline__ void pair_forces_rowq(const StepArgs &a, const Model &pm, const int i,
-                                                 const int pad, const int cnt, const double4 &pi,
-                                                 T &fx, T &fy, T &fz, T &tx, T &ty, T &tz,
+                                                 const int pad, const int cnt, T &fx, T &fy,
+                                                 T &fz, T &tx, T &ty, T &tz,
                                                  unsigned long long &my_contacts, RowLds<T> &L) {
   static_assert(32 % GROUP == 0, "a round holds whole groups");
   typedef typename OmgT<T>::type omg_t;
@@ -1030,11 +1050,22 @@ __device__ __forceinline__ void pair_forces_rowq(const StepArgs &a, const Model 
   for (int r0 = 0; r0 < kmax; r0 += 32) {
     const int rend = kmax < r0 + 32 ? kmax : r0 + 32;
     uint32_t tmask = 0u;
+    int c = 0;
     for (int k0 = r0; k0 < rend; k0 += GROUP) {
       uint32_t ent[GROUP];
-      tmask |= broad_phase<T, GEN, GROUP>(a, pm, i, pad, cnt, k0, pi, ent) << (k0 - r0);
+      const double4 pi = L.pos[lane];
+      uint32_t bits = broad_phase<T, GEN, GROUP>(a, pm, i, pad, cnt, k0, pi, ent);
+      tmask |= bits << (k0 - r0);
+      while (bits) {
+        const int u = __builtin_ctz(bits);
+        bits &= bits - 1u;
+        uint32_t e = ent[0];
+#pragma unroll
+        for (int v = 1; v < GROUP; v++) e = u == v ? ent[v] : e;
+        if (c < kRowStash) L.stash[c][lane] = e;
+        c++;
+      }
     }
-    const int c = __popc(tmask);
     my_contacts += (unsigned long long)c;
     int incl = c;
 #pragma unroll
@@ -1050,7 +1081,8 @@ __device__ __forceinline__ void pair_forces_rowq(const StepArgs &a, const Model 
       for (int q = first; m != 0u; q++) {
         const int u = __builtin_ctz(m);
         m &= m - 1u;
-        if (q >= w0 && q < wend) L.queue[q - w0] = ((uint32_t)lane << 16) | (uint32_t)(r0 + u);
+        if (q >= w0 && q < wend)
+          L.queue[q - w0] = ((uint32_t)lane << 21) | ((uint32_t)(q - first) << 16) | (uint32_t)(r0 + u);
       }
       wave_sync();
       for (int c0 = w0; c0 < wend; c0 += 64) {
@@ -1058,13 +1090,14 @@ __device__ __forceinline__ void pair_forces_rowq(const StepArgs &a, const Model 
         T dF[3] = {T(0), T(0), T(0)}, dTi[3] = {T(0), T(0), T(0)};
         if (q < wend) {
           const uint32_t qe = L.queue[q - w0];
-          const int ol = (int)(qe >> 16);
+          const int ol = (int)(qe >> 21);
+          const int rank = (int)((qe >> 16) & 31u);
           const int io = ibase + ol;
           const int k = (int)(qe & 0xffffu);
-          const uint32_t e = a.list.nbr[(size_t)k * cap + io];
-          const double4 po = a.in.pos[io];
-          const double4 vo = a.in.vel[io];
-          const omg_t wo = ((const omg_t *)a.in.omg)[io];
+          const uint32_t e = rank < kRowStash ? L.stash[rank][ol] : a.list.nbr[(size_t)k * cap + io];
+          const double4 po = L.pos[ol];
+          const double4 vo = L.vel[ol];
+          const omg_t wo = L.omg[ol];
           const int wb = (int)wo.w;
           const int orow = (wb >> kTypeBits) * p.mat_stride + (wb & kTypeMask) * p.ntp1;
           pair_contact<T, GEN>(a, pm, io, k, e, po, vo, wo, orow, dF, dTi);
@@ -1110,9 +1143,18 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i, co
   const DevParams &p = a.prm;
 
   const int ii = active ? i : 0;
-  const double4 pi = a.in.pos[ii];
-  const double4 vi = a.in.vel[ii];
-  const omg_t wi = ((const omg_t *)a.in.omg)[ii];
+  double4 pi = a.in.pos[ii];
+  double4 vi = a.in.vel[ii];
+  omg_t wi = ((const omg_t *)a.in.omg)[ii];
+  if constexpr (WAVE == 2) {
+    // the records the row-queued pair phase reads; reloaded from LDS after it
+    const int lane = threadIdx.x & 63;
+    wave_sync();
+    lds->pos[lane] = pi;
+    lds->vel[lane] = vi;
+    lds->omg[lane] = wi;
+    wave_sync();
+  }
   const int wbits = (int)wi.w;
   // material row of this particle: its instance's block, its type's row
   const int irow = (wbits >> kTypeBits) * p.mat_stride + (wbits & kTypeMask) * p.ntp1;
@@ -1125,9 +1167,13 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i, co
   T fx = T(0), fy = T(0), fz = T(0), tx = T(0), ty = T(0), tz = T(0);
 
   if (WAVE) {
-    if constexpr (WAVE == 2)
-      pair_forces_rowq<T, GEN, GROUP>(a, pm, i, ii, cnt, pi, fx, fy, fz, tx, ty, tz, my_contacts, *lds);
-    else
+    if constexpr (WAVE == 2) {
+      pair_forces_rowq<T, GEN, GROUP>(a, pm, i, ii, cnt, fx, fy, fz, tx, ty, tz, my_contacts, *lds);
+      const int lane = threadIdx.x & 63;
+      pi = lds->pos[lane];
+      vi = lds->vel[lane];
+      wi = lds->omg[lane];
+    } else
       pair_forces_wave<T, GEN, GROUP>(a, pm, i, ii, cnt, pi, fx, fy, fz, tx, ty, tz, my_contacts, *lds);
     if (!active) return;
   } else {
@@ -1494,7 +1540,8 @@ __global__ __launch_bounds__(kBlock) void k_pair_local_fill(StepArgs a, PairLoca
     rec[20] = pj.w;
     const bool flagged = (e & kFlagBit) != 0;
     for (size_t d = 0; d < hw; d++)
-      rec[kPairLocalFixedCols + d] = flagged ? (double)hist[((size_t)k * hw + d) * cap + i] : 0.0;
+      rec[kPairLocalFixedCols + d] =
+          flagged ? (double)hist[hist_at(k, i, cap, hist_rec((int)hw)) + hist_off((int)d)] : 0.0;
     o.tags[2 * r] = ti;
     o.tags[2 * r + 1] = tj;
     r++;
@@ -2017,9 +2064,16 @@ __device__ __forceinline__ void copy_history(const RebuildArgs &a, const size_t 
                                              const int io, const int m, const int i, const int k) {
   const T *ohist = (const T *)a.oldl.hist;
   T *nhist = (T *)a.newl.hist;
-  const size_t ob = (size_t)m * hw * cap + io;
-  const size_t nb = (size_t)k * hw * cap + i;
-  for (size_t d = 0; d < hw; d++) nhist[nb + d * cap] = ohist[ob + d * cap];
+  const size_t rec = hist_rec((int)hw);
+  const size_t ob = hist_at(m, io, cap, rec);
+  const size_t nb = hist_at(k, i, cap, rec);
+  T v[3];
+  load_rec3(ohist + ob, v);
+  store_rec3(nhist + nb, v);
+  if (hw == 6) {
+    load_rec3(ohist + ob + 4, v);
+    store_rec3(nhist + nb + 4, v);
+  }
 }
 
 // Neighbor::granular_bin_no_newton (neigh_gran.cpp:485-639) as a full list,

@@ -888,7 +888,7 @@ int grow_particles(dem_handle *h, int newcap) {
     cols.push_back({(void **)&h->aux[b].mslot, sizeof(int), nm * dem::kMeshSlots, 0xff, true});
     cols.push_back({&h->aux[b].mhist, h->fsz, 3 * nm * dem::kMeshSlots, 0, true});
     cols.push_back({(void **)&h->lst[b].nbr, sizeof(uint32_t), h->W, 0, true});
-    cols.push_back({&h->lst[b].hist, h->fsz, h->hw * h->W, 0, true});
+    cols.push_back({&h->lst[b].hist, h->fsz * dem::hist_rec(h->hw), h->W, 0, true});
   }
   cols.push_back({(void **)&h->xhold, sizeof(double), 3, 0, true});
   cols.push_back({(void **)&h->fout, sizeof(double), 6, 0, true});
@@ -1486,7 +1486,7 @@ int dem_set_particles(dem_handle *h, int64_t n, const double *x, const double *v
     if ((rc = dalloc(h, (void **)&h->aux[b].meta, sizeof(int) * cap))) return rc;
     if ((rc = dalloc(h, &h->aux[b].whist, h->fsz * h->hw * nw * cap))) return rc;
     if ((rc = dalloc(h, (void **)&h->lst[b].nbr, sizeof(uint32_t) * h->W * cap))) return rc;
-    if ((rc = dalloc(h, &h->lst[b].hist, h->fsz * h->hw * h->W * cap))) return rc;
+    if ((rc = dalloc(h, &h->lst[b].hist, h->fsz * dem::hist_rec(h->hw) * h->W * cap))) return rc;
   }
   if ((rc = dalloc(h, (void **)&h->xhold, sizeof(double) * 3 * cap))) return rc;
   if ((rc = dalloc(h, (void **)&h->fout, sizeof(double) * 6 * cap))) return rc;
@@ -2052,9 +2052,12 @@ struct RestartHeader {
   double mesh_time, dt, boxlo[3], boxhi[3];
 };
 constexpr char kRestartMagic[8] = {'L', 'G', 'A', 'M', 'D', 'R', 'S', 'T'};
+// 2: pair histories stored as one record per list entry
+constexpr int32_t kRestartVersion = 2;
 
 bool read_header(FILE *f, RestartHeader &hd) {
-  return fread(&hd, sizeof hd, 1, f) == 1 && std::memcmp(hd.magic, kRestartMagic, 8) == 0 && hd.version == 1;
+  return fread(&hd, sizeof hd, 1, f) == 1 && std::memcmp(hd.magic, kRestartMagic, 8) == 0 &&
+         hd.version == kRestartVersion;
 }
 
 // device blobs of a restart, in file order
@@ -2075,7 +2078,7 @@ std::vector<std::pair<void *, size_t>> restart_blobs(dem_handle *h) {
       {ax.mslot, sizeof(int) * nm * dem::kMeshSlots * cap},
       {ax.mhist, h->fsz * 3 * nm * dem::kMeshSlots * cap},
       {ls.nbr, sizeof(uint32_t) * h->W * cap},
-      {ls.hist, h->fsz * h->hw * h->W * cap},
+      {ls.hist, h->fsz * dem::hist_rec(h->hw) * h->W * cap},
       {h->xhold, sizeof(double) * 3 * cap},
       {h->fout, sizeof(double) * 6 * cap}};
   if (h->ntri > 0) {
@@ -2125,7 +2128,7 @@ int dem_write_restart(dem_handle *h, const char *path) {
   if (!f) return fail(h, DEM_ERR_ARG, std::string("Cannot open restart file ") + path);
   RestartHeader hd{};
   std::memcpy(hd.magic, kRestartMagic, 8);
-  hd.version = 1;
+  hd.version = kRestartVersion;
   hd.mixed = h->mixed ? 1 : 0;
   hd.ntypes = h->ntypes;
   hd.cap = h->cap;
@@ -2295,7 +2298,8 @@ int get_pairs(dem_handle *h, int64_t cap_out, int *tags, int *flags, double *his
   const int W = h->W;
   std::vector<uint32_t> nbr((size_t)W * cap);
   std::vector<int> meta(n), tag(n);
-  std::vector<char> hraw(h->fsz * h->hw * W * cap);
+  const size_t rec = dem::hist_rec(h->hw);
+  std::vector<char> hraw(h->fsz * rec * W * cap);
   const ListBuf &l = h->lst[h->cur_l];
   HIPCHK(h, hipMemcpy(nbr.data(), l.nbr, sizeof(uint32_t) * W * cap, hipMemcpyDeviceToHost));
   HIPCHK(h, hipMemcpy(hraw.data(), l.hist, hraw.size(), hipMemcpyDeviceToHost));
@@ -2315,7 +2319,7 @@ int get_pairs(dem_handle *h, int64_t cap_out, int *tags, int *flags, double *his
         const bool fl = (e & dem::kFlagBit) != 0;
         flags[np] = fl ? 1 : 0;
         for (int d = 0; d < 3; d++) {
-          const size_t idx = ((size_t)k * h->hw + comp0 + d) * cap + i;
+          const size_t idx = dem::hist_at(k, i, cap, rec) + dem::hist_off(comp0 + d);
           double val = h->mixed ? (double)((float *)hraw.data())[idx] : ((double *)hraw.data())[idx];
           hist[3 * np + d] = fl ? val : 0.0;
         }
