@@ -752,11 +752,12 @@ __device__ __forceinline__ void close_entries(const StepArgs &a, const Model &pm
 // Returns the contribution to i's force and torque.
 // STORE = false: the evaluation of compute pair/gran/local, which writes
 // neither history nor flag.
+// pre: the entry's shear history when the caller has loaded it already.
 template <typename T, bool GEN, typename W, bool STORE = true>
 __device__ __forceinline__ void pair_contact(const StepArgs &a, const Model &pm, const int i,
                                              const int k, const uint32_t e, const double4 &pi,
                                              const double4 &vi, const W &wi, const int irow,
-                                             T dF[3], T dTi[3]) {
+                                             T dF[3], T dTi[3], const T *pre = nullptr) {
   typedef typename OmgT<T>::type omg_t;
   const DevParams &p = a.prm;
   const size_t cap = (size_t)a.cap;
@@ -822,7 +823,13 @@ __device__ __forceinline__ void pair_contact(const StepArgs &a, const Model &pm,
   T sh[3], rh[3] = {T(0), T(0), T(0)};
   const size_t hb = hist_at(k, i, cap, hist_rec((int)hw));
   if (flagged && keeps) {
-    load_rec3(hist + hb, sh);
+    if (pre) {
+      sh[0] = pre[0];
+      sh[1] = pre[1];
+      sh[2] = pre[2];
+    } else {
+      load_rec3(hist + hb, sh);
+    }
     if (GEN && pm.epsd2) load_rec3(hist + hb + 4, rh);
   } else {
     sh[0] = sh[1] = sh[2] = T(0);
@@ -844,10 +851,15 @@ __device__ __forceinline__ void pair_contact(const StepArgs &a, const Model &pm,
 // index: i itself, or 0 for a lane past the last particle) and are ignored.
 // Returns the touching entries as bits; entries in the surfacesClose band are
 // handled here.
+// With hpre (Hertz / history kernels), the shear histories of the group's
+// entries flagged in contact at the last step are loaded beside the position
+// gathers: a history record is read once per step, so it comes from HBM, and
+// loading it here keeps that latency out of the contact law's chain.
 template <typename T, bool GEN, int GROUP>
 __device__ __forceinline__ uint32_t broad_phase(const StepArgs &a, const Model &pm, const int i,
                                                 const int pad, const int cnt, const int k0,
-                                                const double4 &pi, uint32_t (&ent)[GROUP]) {
+                                                const double4 &pi, uint32_t (&ent)[GROUP],
+                                                T (*hpre)[3] = nullptr) {
   const DevParams &p = a.prm;
   const size_t cap = (size_t)a.cap;
 #pragma unroll
@@ -856,6 +868,14 @@ __device__ __forceinline__ uint32_t broad_phase(const StepArgs &a, const Model &
   double4 pj[GROUP];
 #pragma unroll
   for (int u = 0; u < GROUP; u++) pj[u] = a.in.pos[ent[u] & kIndexMask];
+  if (hpre) {
+    const T *hist = (const T *)a.list.hist;
+#pragma unroll
+    for (int u = 0; u < GROUP; u++) {
+      hpre[u][0] = hpre[u][1] = hpre[u][2] = T(0);
+      if (k0 + u < cnt && (ent[u] & kFlagBit)) load_rec3(hist + hist_at(k0 + u, i, cap, 4), hpre[u]);
+    }
+  }
   double dx[GROUP], dy[GROUP], dz[GROUP];
   bool wrap = false;
 #pragma unroll
@@ -1181,16 +1201,22 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i, co
     // entries selected from the group's registers.
     for (int k0 = 0; k0 < cnt; k0 += GROUP) {
       uint32_t ent[GROUP];
-      uint32_t bits = broad_phase<T, GEN, GROUP>(a, pm, i, i, cnt, k0, pi, ent);
+      T hpre[GROUP][3];
+      uint32_t bits = broad_phase<T, GEN, GROUP>(a, pm, i, i, cnt, k0, pi, ent, GEN ? nullptr : hpre);
       while (bits) {
         const int u = __builtin_ctz(bits);
         const int k = k0 + u;
         bits &= bits - 1u;
         uint32_t e = ent[0];
+        T sh[3] = {hpre[0][0], hpre[0][1], hpre[0][2]};
 #pragma unroll
-        for (int v = 1; v < GROUP; v++) e = u == v ? ent[v] : e;
+        for (int v = 1; v < GROUP; v++) {
+          e = u == v ? ent[v] : e;
+          if (!GEN)
+            for (int d = 0; d < 3; d++) sh[d] = u == v ? hpre[v][d] : sh[d];
+        }
         T dF[3], dTi[3];
-        pair_contact<T, GEN>(a, pm, i, k, e, pi, vi, wi, irow, dF, dTi);
+        pair_contact<T, GEN>(a, pm, i, k, e, pi, vi, wi, irow, dF, dTi, GEN ? nullptr : sh);
         fx += dF[0];
         fy += dF[1];
         fz += dF[2];
