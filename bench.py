@@ -96,16 +96,25 @@ def measured_traffic(workload, mode):
     return None
 
 
-TUNE_STEPS = 800  # covers every candidate tile shape (8 steps each, up to 3 tries)
+TUNE_STEPS = 2400  # covers three passes over every candidate tile shape (8 steps each, up to 3 tries)
 
 
-def tune_shape(params, mats, P, walls, precision, meshes=(), instances=None, instance_materials=None):
+def tune_shape(params, mats, P, walls, precision, meshes=(), instances=None, instance_materials=None,
+               warmup=0):
     """Autotunes the step kernel's tile shape on a separate system of the same
-    workload, so that the timed system starts with its shape fixed and no
-    timed step runs a candidate shape. Returns (variant, persist, candidates)."""
+    workload, after the same warmup steps as the timed system, so that the
+    timed system starts with its shape fixed and no timed step runs a candidate
+    shape. Returns (variant, persist, candidates)."""
     import pydem
     g = pydem.GpuSystem(params, mats, P, walls, precision=precision, meshes=meshes,
                         instances=instances, instance_materials=instance_materials)
+    if warmup:
+        g.run(warmup)
+        g.set_step_shape(-1)
+        g.run_continue(TUNE_STEPS)
+        v, f, cand = g.step_shape()
+        g.close()
+        return v, f, cand
     g.set_step_shape(-1)
     g.run(TUNE_STEPS)
     v, f, cand = g.step_shape()
@@ -118,7 +127,7 @@ def run_mode(params, mats, P, walls, precision, steps, warmup, timing_steps, mes
     import pydem
     tuned = None
     if tune:
-        tuned = tune_shape(params, mats, P, walls, precision, meshes, instances, instance_materials)
+        tuned = tune_shape(params, mats, P, walls, precision, meshes, instances, instance_materials, warmup)
     g = pydem.GpuSystem(params, mats, P, walls, precision=precision, meshes=meshes,
                         instances=instances, instance_materials=instance_materials)
     if tuned is not None:

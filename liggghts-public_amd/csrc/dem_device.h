@@ -32,12 +32,13 @@ constexpr int kMaxInstances = 1 << 16;
 // Tile shapes the step kernel is compiled for (threads per block, neighbor
 // entries whose loads are issued together, and whether the touching entries
 // are compacted over the wavefront before the contact law: 1 group by group,
-// 2 over whole list rows); variant 0 is the default. The sibling contact
-// models run in variants 0, 5 and 7.
-constexpr int kNumStepVariants = 9;
-constexpr int kStepVariantBlock[kNumStepVariants] = {256, 256, 256, 128, 64, 256, 256, 256, 256};
-constexpr int kStepVariantGroup[kNumStepVariants] = {4, 2, 8, 4, 4, 4, 8, 4, 8};
-constexpr int kStepVariantWave[kNumStepVariants] = {0, 0, 0, 0, 0, 1, 1, 2, 2};
+// 2 over whole list rows; 3: per lane, with the shear histories loaded in the
+// broad phase); variant 0 is the default. The sibling contact models run in
+// variants 0, 5 and 7.
+constexpr int kNumStepVariants = 11;
+constexpr int kStepVariantBlock[kNumStepVariants] = {256, 256, 256, 128, 64, 256, 256, 256, 256, 256, 128};
+constexpr int kStepVariantGroup[kNumStepVariants] = {4, 2, 8, 4, 4, 4, 8, 4, 8, 4, 4};
+constexpr int kStepVariantWave[kNumStepVariants] = {0, 0, 0, 0, 0, 1, 1, 2, 2, 3, 3};
 
 // One triangle of a fix mesh/surface, FP64 (TriMesh element properties).
 struct TriGeom {

@@ -1151,6 +1151,8 @@ __device__ __forceinline__ void pair_forces_rowq(const StepArgs &a, const Model 
 // integration and the displacement test. Accumulates its touching pair and
 // wall contacts. WAVE 1: the pair phase is compacted over the wavefront
 // group by group (pair_forces_wave), WAVE 2: row by row (pair_forces_rowq);
+// WAVE 3: per lane as WAVE 0, with the shear histories of entries flagged in
+// contact loaded in the broad phase (Hertz / history kernels);
 // there every lane calls, active says whether it holds a particle.
 // WAVE 0: each lane resolves its own touching entries.
 template <typename T, bool MESH, int GROUP, bool GEN, int WAVE>
@@ -1186,7 +1188,7 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i, co
   const Model pm = model_from_bits(p.pair_model);
   T fx = T(0), fy = T(0), fz = T(0), tx = T(0), ty = T(0), tz = T(0);
 
-  if (WAVE) {
+  if (WAVE == 1 || WAVE == 2) {
     if constexpr (WAVE == 2) {
       pair_forces_rowq<T, GEN, GROUP>(a, pm, i, ii, cnt, fx, fy, fz, tx, ty, tz, my_contacts, *lds);
       const int lane = threadIdx.x & 63;
@@ -1201,8 +1203,9 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i, co
     // entries selected from the group's registers.
     for (int k0 = 0; k0 < cnt; k0 += GROUP) {
       uint32_t ent[GROUP];
+      constexpr bool PRE = WAVE == 3 && !GEN;
       T hpre[GROUP][3];
-      uint32_t bits = broad_phase<T, GEN, GROUP>(a, pm, i, i, cnt, k0, pi, ent, GEN ? nullptr : hpre);
+      uint32_t bits = broad_phase<T, GEN, GROUP>(a, pm, i, i, cnt, k0, pi, ent, PRE ? hpre : nullptr);
       while (bits) {
         const int u = __builtin_ctz(bits);
         const int k = k0 + u;
@@ -1212,11 +1215,11 @@ __device__ __forceinline__ void step_particle(const StepArgs &a, const int i, co
 #pragma unroll
         for (int v = 1; v < GROUP; v++) {
           e = u == v ? ent[v] : e;
-          if (!GEN)
+          if (PRE)
             for (int d = 0; d < 3; d++) sh[d] = u == v ? hpre[v][d] : sh[d];
         }
         T dF[3], dTi[3];
-        pair_contact<T, GEN>(a, pm, i, k, e, pi, vi, wi, irow, dF, dTi, GEN ? nullptr : sh);
+        pair_contact<T, GEN>(a, pm, i, k, e, pi, vi, wi, irow, dF, dTi, PRE ? sh : nullptr);
         fx += dF[0];
         fy += dF[1];
         fz += dF[2];
@@ -1440,15 +1443,16 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a) {
   }
   const DevParams &p = a.prm;
   typedef typename PairLdsT<T, GROUP, WAVE>::type Lds;
-  __shared__ Lds lds_all[WAVE ? BLOCK / 64 : 1];
-  Lds *lds = &lds_all[WAVE ? (threadIdx.x >> 6) : 0];
+  constexpr bool kWaveLds = WAVE == 1 || WAVE == 2;
+  __shared__ Lds lds_all[kWaveLds ? BLOCK / 64 : 1];
+  Lds *lds = &lds_all[kWaveLds ? (threadIdx.x >> 6) : 0];
   unsigned long long my_contacts = 0, my_wall = 0;
   bool displaced = false;
   const int ntiles = (a.n + BLOCK - 1) / BLOCK;
   for (int tile = xcd_block(a.pblocks); tile < ntiles; tile += a.pblocks) {
     const int i = tile * BLOCK + threadIdx.x;
     const bool active = i < a.n;
-    if (WAVE || active) {
+    if (kWaveLds || active) {
       unsigned long long c = 0, w = 0;
       int inst = 0;
       step_particle<T, MESH, GROUP, GEN, WAVE>(a, i, active, c, w, inst, displaced, ago_next, lds);
@@ -2587,6 +2591,12 @@ static void launch_step_variant(const StepArgs &a, int nb, hipStream_t s) {
       break;
     case 8:
       hipLaunchKernelGGL((k_step<T, MESH, 256, 8, false, 2>), dim3(nb), dim3(B), 0, s, a);
+      break;
+    case 9:
+      hipLaunchKernelGGL((k_step<T, MESH, 256, 4, false, 3>), dim3(nb), dim3(B), 0, s, a);
+      break;
+    case 10:
+      hipLaunchKernelGGL((k_step<T, MESH, 128, 4, false, 3>), dim3(nb), dim3(B), 0, s, a);
       break;
     default:
       hipLaunchKernelGGL((k_step<T, MESH, 256, 4>), dim3(nb), dim3(B), 0, s, a);

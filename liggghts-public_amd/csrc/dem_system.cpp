@@ -163,8 +163,9 @@ struct dem_handle {
   struct Tuner {
     bool active = false;
     std::vector<std::pair<int, int>> cand;  // (variant, persist)
-    std::vector<double> us;                 // measured microseconds per step
+    std::vector<double> us;                 // fastest measured microseconds per step
     int idx = 0, tries = 0;
+    int round = 0;                          // passes over the candidates so far
   } tuner;
   double kernel_ms = 0.0;
   int64_t kernel_launches = 0;
@@ -1736,6 +1737,11 @@ int step_graph(dem_handle *h, int st, int tc, bool moving, hipGraphExec_t *out) 
   return DEM_OK;
 }
 
+// passes over the candidate tile shapes: each keeps its fastest batch, so a
+// shape measured while the state changed (contacts forming, a rebuild near)
+// is measured again later
+constexpr int kTuneRounds = 3;
+
 // picks the fastest measured tile shape
 void finish_tuning(dem_handle *h) {
   auto &tu = h->tuner;
@@ -1876,15 +1882,21 @@ int run_steps_piece(dem_handle *h, int64_t nsteps) {
         h->kernel_launches += nb - 1;
       }
       if (tuning) {
-        tu.us[tu.idx] = 1e3 * ms / (nb - 1);
+        const double us = 1e3 * ms / (nb - 1);
+        if (tu.us[tu.idx] <= 0.0 || us < tu.us[tu.idx]) tu.us[tu.idx] = us;
         tu.idx++;
         tu.tries = 0;
       }
     } else if (tuning && ++tu.tries >= 3) {
-      tu.idx++;  // interrupted by rebuilds three times: leave this shape out
+      tu.idx++;  // interrupted by rebuilds three times: leave this shape out of the pass
       tu.tries = 0;
     }
-    if (tuning && tu.idx == (int)tu.cand.size()) finish_tuning(h);
+    if (tuning && tu.idx == (int)tu.cand.size()) {
+      if (++tu.round < kTuneRounds)
+        tu.idx = 0;
+      else
+        finish_tuning(h);
+    }
     const bool ran_last = t + executed - 1 == nsteps;  // the last step moves no mesh
     h->cur_st = (h->cur_st + executed) & 1;
     h->ntimestep += executed;

@@ -230,7 +230,7 @@ def test_step_shapes_are_bitwise_equal():
     say("[gpu] step kernel shapes")
     params, mats, P, walls = wl.dense_pack(seed=33, n=3000, rolling=True, speed=0.3)
     ref = {}
-    shapes = [(v, f) for v in range(9) for f in (0, 4)] + [(-1, 0)]
+    shapes = [(v, f) for v in range(11) for f in (0, 4)] + [(-1, 0)]
     for v, f in shapes:
         for prec in (pydem.DEM_FP64, pydem.DEM_MIXED):
             g = pydem.GpuSystem(params, mats, P, walls, precision=prec)
