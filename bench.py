@@ -82,10 +82,22 @@ def algorithmic_bytes(n, npairs, ncontacts, nwall, mixed, ntri=0, ncand=0, histo
     return B
 
 
+# workloads with a committed FETCH_SIZE / WRITE_SIZE profile of the step kernel
+PROFILED = {"settling": "1M_settling", "packed": "packed_1M"}
+
+
+def traffic_source(name):
+    return (f"profiles/k_step_{name}_pmc.md: FETCH_SIZE + WRITE_SIZE per launch of this kernel, from "
+            "rocprofv3 --pmc runs of this workload, scaled by a plain copy's counters (not measured "
+            "in this run)")
+
+
 def measured_traffic(workload, mode):
     """HBM bytes per launch of the step kernel from the committed FETCH_SIZE /
     WRITE_SIZE profile of this workload (calibrated against a plain copy), or
     None when no such profile exists."""
+    if workload is None:
+        return None
     path = os.path.join(ROOT, "profiles", f"k_step_{workload}_pmc.md")
     if not os.path.exists(path):
         return None
@@ -639,12 +651,10 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": (head["achieved_gbs"] / HBM_PEAK_GBS) if head["achieved_gbs"] else None,
-        "traffic": (measured_traffic("1M_settling", head_mode)
-                    if args.workload == "settling" and args.contact_model == "hertz" else None),
-        "traffic_source": ("profiles/k_step_1M_settling_pmc.md: FETCH_SIZE + WRITE_SIZE per launch of "
-                           "this kernel, from rocprofv3 --pmc runs of this workload, scaled by a plain "
-                           "copy's counters (not measured in this run)"
-                           if args.workload == "settling" and args.contact_model == "hertz" else None),
+        "traffic": (measured_traffic(PROFILED.get(args.workload), head_mode)
+                    if args.contact_model == "hertz" else None),
+        "traffic_source": (traffic_source(PROFILED[args.workload])
+                           if args.contact_model == "hertz" and args.workload in PROFILED else None),
         "bytes_per_launch": head["bytes_per_launch"],
         "kernel_ms": head["kernel_ms"],
     }
@@ -659,6 +669,8 @@ def main():
             packed[mode] = {"value": r["throughput"], "ms_per_step": r["ms_per_step"],
                             "kernel_ms": r["kernel_ms"], "achieved_gbs": r["achieved_gbs"],
                             "frac": r["achieved_gbs"] / HBM_PEAK_GBS if r["achieved_gbs"] else None,
+                            "traffic": measured_traffic("packed_1M", mode),
+                            "traffic_source": traffic_source("packed_1M"),
                             "npairs": r["npairs"], "ncontacts": r["ncontacts"], "nwall": r["nwall"],
                             "rebuilds_in_timed_steps": r["nrebuilds_timed"], "step_shape": r["step_shape"]}
             print(f"# packed {mode}: {packed[mode]}", file=sys.stderr, flush=True)
