@@ -227,6 +227,7 @@ struct Interp {
     std::vector<std::string> meshes;  // mesh/stl: the meshes dumped, in order
     std::string region;               // mesh/stl: only triangles whose centre is inside
     bool local = false;               // dump local: rows of a local compute
+    bool vtk = false;                 // dump custom/vtk: legacy VTK polydata
   };
   std::vector<std::string> mesh_def_order;  // fix mesh/surface in definition order
   // thermo `time` (Update::atime / atimestep, update.cpp:434-446) and `cpu`
@@ -963,24 +964,139 @@ struct Interp {
       dump_local_command(t);
       return;
     }
-    if (t[3] != "custom") die("dump style " + t[3] + " is not available on this path (custom, local, mesh/stl)");
+    if (t[3] != "custom" && t[3] != "custom/vtk")
+      die("dump style " + t[3] + " is not available on this path (custom, custom/vtk, local, mesh/stl)");
     if (t[2] != "all") die("dump: only group all is available on this path");
-    if (sq) die("dump custom for superquadrics is not available on this path");
+    if (sq) die("dump " + t[3] + " for superquadrics is not available on this path");
     if (dumps.count(t[1])) die("Reuse of dump ID " + t[1]);
     Dump d;
+    d.vtk = t[3] == "custom/vtk";
     d.every = std::stoll(t[4]);
     if (d.every <= 0) die("Illegal dump command");
     d.file = t[5];
     for (size_t k = 6; k < t.size(); k++) {
-      if (!dump_column_known(t[k])) die("Invalid attribute " + t[k] + " in dump custom command");
+      const bool image = t[k] == "ix" || t[k] == "iy" || t[k] == "iz";
+      if (!dump_column_known(t[k]) && !(d.vtk && image))
+        die("Invalid attribute " + t[k] + " in dump " + t[3] + " command");
       d.cols.push_back(t[k]);
     }
     if (d.cols.empty()) die("Illegal dump command");
+    if (d.vtk) {
+      // DumpVTK's file formats: the legacy .vtk is written here; .vtp / .vtu
+      // and their parallel forms go through the VTK library's XML writers
+      if (d.file.size() < 4 || d.file.compare(d.file.size() - 4, 4, ".vtk") != 0)
+        die("dump custom/vtk: only the legacy .vtk file format is available on this path");
+      dumps[t[1]] = d;
+      return;
+    }
     if (d.file.find('*') == std::string::npos) {
       d.fp = fopen(d.file.c_str(), "w");
       if (!d.fp) die("Cannot open dump file " + d.file);
     }
     dumps[t[1]] = d;
+  }
+
+  // dump custom/vtk (DumpCustomVTK, dump_custom_vtk.cpp:228-322, with
+  // DumpParticle's arrays, dump_particle.cpp:1372-1620, 1735-1800): a legacy VTK
+  // polydata file per snapshot, the atoms as points with one vertex cell each
+  // and the dump's columns as point-data arrays, in the order of DumpParticle's
+  // attribute list (not the order of the command). x, y, z are the points; three
+  // components of one vector (ix iy iz, vx vy vz, fx fy fz, omegax omegay omegaz,
+  // tqx tqy tqz) become one 3-component array named by the text before its 'x'
+  // ("i", "v", "f", "omega", "tq"); ids, types and image counts are int arrays,
+  // the rest double. The text follows the legacy VTK format as
+  // vtkPolyDataWriter writes it in ASCII (header "Generated by LIGGGHTS",
+  // values "%.11lg " nine to a line). Beside each snapshot the simulation box is
+  // written as a 2x2x2 rectilinear grid (DumpCustomVTK::setFileCurrent:
+  // <prefix>boundingBox_<step>.vtk for a '*' file name, <name>_boundingBox.vtk
+  // otherwise). Atoms are in ID order, as dump_modify sort id would give.
+  void write_vtk_dump(Dump &d, const std::vector<size_t> &order, const std::vector<double> &f,
+                      const std::vector<double> &tq, const double *blo, const double *bhi) {
+    static const char *attrs[] = {"id", "type", "mass", "ix", "iy", "iz", "vx", "vy", "vz", "fx", "fy", "fz",
+                                  "radius", "diameter", "omegax", "omegay", "omegaz", "tqx", "tqy", "tqz"};
+    std::set<std::string> have(d.cols.begin(), d.cols.end());
+    if (have.count("ix") || have.count("iy") || have.count("iz"))
+      for (int k = 0; k < 3; k++)
+        if (periodic[k]) die("dump custom/vtk: image flags of periodic boundaries are not available on this path");
+    const size_t n = order.size();
+    auto value = [&](const std::string &c, size_t i) -> double {
+      if (c == "vx" || c == "vy" || c == "vz") return v[3 * i + (c[1] - 'x')];
+      if (c == "fx" || c == "fy" || c == "fz") return f[3 * i + (c[1] - 'x')];
+      if (c == "omegax" || c == "omegay" || c == "omegaz") return w[3 * i + (c[5] - 'x')];
+      if (c == "tqx" || c == "tqy" || c == "tqz") return tq[3 * i + (c[2] - 'x')];
+      if (c == "radius") return r[i];
+      if (c == "diameter") return 2.0 * r[i];
+      if (c == "mass") return m[i];
+      return 0.0;  // image counts: no periodic boundary, so always 0
+    };
+    struct Arr {
+      std::string name;
+      std::vector<std::string> comps;
+      bool integer;
+    };
+    std::vector<Arr> arrs;
+    const int na = (int)(sizeof attrs / sizeof attrs[0]);
+    for (int a = 0; a < na; a++) {
+      const std::string c = attrs[a];
+      if (!have.count(c)) continue;
+      const bool vec_start = c.back() == 'x' && c != "id" && a + 2 < na && have.count(attrs[a + 1]) &&
+                             have.count(attrs[a + 2]) && std::string(attrs[a + 1]).back() == 'y';
+      const bool integer = c == "id" || c == "type" || c[0] == 'i';
+      if (vec_start) {
+        arrs.push_back({c.substr(0, c.find('x')), {c, attrs[a + 1], attrs[a + 2]}, integer});
+        a += 2;
+      } else {
+        arrs.push_back({c, {c}, integer});
+      }
+    }
+    std::string name = d.file, bbox;
+    const size_t star = name.find('*');
+    if (star != std::string::npos) {
+      const std::string st = std::to_string((long long)step);
+      bbox = name.substr(0, star) + "boundingBox_" + st + name.substr(star + 1);
+      name.replace(star, 1, st);
+    } else {
+      bbox = name.substr(0, name.size() - 4) + "_boundingBox.vtk";
+    }
+    FILE *fp = fopen(name.c_str(), "w");
+    if (!fp) die("Cannot open dump file " + name);
+    fprintf(fp, "# vtk DataFile Version 3.0\nGenerated by LIGGGHTS\nASCII\nDATASET POLYDATA\nPOINTS %zu double\n", n);
+    size_t col = 0;
+    auto put = [&](double val, bool integer) {
+      if (integer) fprintf(fp, "%d ", (int)val);
+      else fprintf(fp, "%.11lg ", val);
+      if (++col % 9 == 0) fprintf(fp, "\n");
+    };
+    auto end_block = [&]() {
+      if (col % 9 != 0) fprintf(fp, "\n");
+      col = 0;
+    };
+    for (size_t oi = 0; oi < n; oi++)
+      for (int k = 0; k < 3; k++) put(x[3 * order[oi] + k], false);
+    end_block();
+    fprintf(fp, "VERTICES %zu %zu\n", n, 2 * n);
+    for (size_t oi = 0; oi < n; oi++) fprintf(fp, "1 %zu\n", oi);
+    fprintf(fp, "\nPOINT_DATA %zu\nFIELD FieldData %zu\n", n, arrs.size());
+    for (const Arr &A : arrs) {
+      fprintf(fp, "%s %zu %zu %s\n", A.name.c_str(), A.comps.size(), n, A.integer ? "int" : "double");
+      for (size_t oi = 0; oi < n; oi++) {
+        const size_t i = order[oi];
+        for (const std::string &c : A.comps) {
+          if (c == "id") put(tag[i], true);
+          else if (c == "type") put(type[i], true);
+          else put(value(c, i), A.integer);
+        }
+      }
+      end_block();
+    }
+    fclose(fp);
+    FILE *fb = fopen(bbox.c_str(), "w");
+    if (!fb) die("Cannot open dump file " + bbox);
+    fprintf(fb, "# vtk DataFile Version 3.0\nGenerated by LIGGGHTS\nASCII\nDATASET RECTILINEAR_GRID\n"
+                "DIMENSIONS 2 2 2\n");
+    for (int k = 0; k < 3; k++)
+      fprintf(fb, "%c_COORDINATES 2 double\n%.11lg %.11lg \n", 'X' + k, blo[k], bhi[k]);
+    fclose(fb);
   }
 
   // one snapshot in the reference's text layout (DumpCustom::header_item,
@@ -1017,6 +1133,11 @@ struct Interp {
       }
       if (d.local) {
         write_local_dump(d, bstr, blo, bhi);
+        d.last = step;
+        continue;
+      }
+      if (d.vtk) {
+        write_vtk_dump(d, order, f, tq, blo, bhi);
         d.last = step;
         continue;
       }

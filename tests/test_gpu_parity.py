@@ -931,6 +931,108 @@ run 200
     g.close()
 
 
+def _read_vtk_polydata(path):
+    """A legacy ASCII VTK polydata file: (points (n,3), vertex cells, {name:
+    (ncomp, dtype, values)}) in file order."""
+    tok = path.read_text().split("\n")
+    assert tok[0] == "# vtk DataFile Version 3.0" and tok[2] == "ASCII" and tok[3] == "DATASET POLYDATA"
+    words = " ".join(tok[4:]).split()
+    assert words[0] == "POINTS" and words[2] == "double"
+    n = int(words[1])
+    pts = np.array(words[3:3 + 3 * n], dtype=float).reshape(n, 3)
+    k = 3 + 3 * n
+    assert words[k] == "VERTICES" and int(words[k + 1]) == n and int(words[k + 2]) == 2 * n
+    cells = np.array(words[k + 3:k + 3 + 2 * n], dtype=int).reshape(n, 2)
+    k += 3 + 2 * n
+    assert words[k] == "POINT_DATA" and int(words[k + 1]) == n and words[k + 2:k + 4] == ["FIELD", "FieldData"]
+    narr = int(words[k + 4])
+    k += 5
+    arrays = {}
+    for _ in range(narr):
+        name, ncomp, nt, typ = words[k], int(words[k + 1]), int(words[k + 2]), words[k + 3]
+        assert nt == n and typ in ("int", "double")
+        vals = np.array(words[k + 4:k + 4 + ncomp * n], dtype=float).reshape(n, ncomp)
+        arrays[name] = (ncomp, typ, vals)
+        k += 4 + ncomp * n
+    assert k == len(words)
+    return pts, cells, arrays
+
+
+def test_script_dump_custom_vtk(tmp_path):
+    """dump custom/vtk (the packing tutorial's column list without image
+    counts, a periodic box) beside dump custom at the same steps: the legacy
+    VTK polydata holds the atoms by ID as points with one vertex each and the
+    columns as point-data arrays in the reference's attribute order (id, type,
+    mass, v, f, radius, omega; three components of a vector in one array), the
+    same numbers as the text dump, and a bounding-box grid file with the box."""
+    say("[gpu] script dump custom/vtk")
+    params, mats, P, walls = wl.settling_4k(seed=8, n=400)
+    data = tmp_path / "d.data"
+    wl.write_data_file(str(data), params, P)
+    script = tmp_path / "in.vtk"
+    script.write_text(f"""atom_style granular
+atom_modify map array
+boundary p p f
+newton off
+communicate single vel yes
+units si
+region reg block 0 0.1 0 0.1 0 0.2 units box
+create_box 1 reg
+neighbor 0.001 bin
+neigh_modify delay 0
+fix m1 all property/global youngsModulus peratomtype 5.e6
+fix m2 all property/global poissonsRatio peratomtype 0.45
+fix m3 all property/global coefficientRestitution peratomtypepair 1 0.3
+fix m4 all property/global coefficientFriction peratomtypepair 1 0.5
+pair_style gran model hertz tangential history
+pair_coeff * *
+timestep 0.00001
+fix gravi all gravity 9.81 vector 0.0 0.0 -1.0
+fix zwall all wall/gran model hertz tangential history primitive type 1 zplane 0.0
+read_data {data}
+fix integr all nve/sphere
+dump dmp all custom/vtk 150 {tmp_path}/post/packing_*.vtk id type type x y z vx vy vz fx fy fz omegax omegay omegaz radius mass
+dump txt all custom 150 {tmp_path}/post/t*.txt id type x y z vx vy vz fx fy fz omegax omegay omegaz radius mass
+thermo 1000
+run 300
+""")
+    (tmp_path / "post").mkdir()
+    rc = pydem.run_script(str(script), str(tmp_path / "log"))
+    assert rc == 0, (tmp_path / "log").read_text()[-800:]
+    names = sorted(p.name for p in (tmp_path / "post").glob("packing_*.vtk"))
+    assert names == ["packing_0.vtk", "packing_150.vtk", "packing_300.vtk",
+                     "packing_boundingBox_0.vtk", "packing_boundingBox_150.vtk", "packing_boundingBox_300.vtk"]
+    for st in (0, 150, 300):
+        pts, cells, arrays = _read_vtk_polydata(tmp_path / "post" / f"packing_{st}.vtk")
+        assert list(arrays) == ["id", "type", "mass", "v", "f", "radius", "omega"]
+        assert [arrays[k][0] for k in arrays] == [1, 1, 1, 3, 3, 1, 3]
+        assert [arrays[k][1] for k in arrays] == ["int", "int", "double", "double", "double", "double", "double"]
+        assert np.array_equal(cells, np.stack([np.ones(400, int), np.arange(400)], axis=1))
+        _, cols, rows, _ = _read_dump(tmp_path / "post" / f"t{st}.txt")[0]
+        assert np.array_equal(arrays["id"][2][:, 0], rows[:, 0]) and np.array_equal(arrays["type"][2][:, 0], rows[:, 1])
+
+        def close(a, b):  # %g keeps six significant digits, %.11lg eleven
+            return np.allclose(a, b, rtol=1e-5, atol=1e-5 * np.abs(b).max() + 1e-300)
+
+        assert close(pts, rows[:, 2:5])
+        assert close(arrays["v"][2], rows[:, 5:8]) and close(arrays["f"][2], rows[:, 8:11])
+        assert close(arrays["omega"][2], rows[:, 11:14])
+        assert close(arrays["radius"][2][:, 0], rows[:, 14]) and close(arrays["mass"][2][:, 0], rows[:, 15])
+        box = (tmp_path / "post" / f"packing_boundingBox_{st}.vtk").read_text().split("\n")
+        assert box[3] == "DATASET RECTILINEAR_GRID" and box[4] == "DIMENSIONS 2 2 2"
+        got = [float(v) for line in (box[6], box[8], box[10]) for v in line.split()]
+        assert np.allclose(got[:4], [0.0, 0.1, 0.0, 0.1])
+    # the vtk file's own numbers keep eleven digits: the last snapshot against the API state
+    g = pydem.GpuSystem(params, mats, P, walls)
+    g.run(300)
+    st = g.state()
+    order = np.argsort(P["tag"])
+    pts, _, arrays = _read_vtk_polydata(tmp_path / "post" / "packing_300.vtk")
+    assert np.allclose(pts, st["x"][order], rtol=1e-10, atol=1e-13)
+    assert np.allclose(arrays["v"][2], st["v"][order], rtol=1e-9, atol=1e-12)
+    g.close()
+
+
 @pytest.mark.parametrize("case", ["pack_walls", "drum"])
 def test_restart_continues_bitwise(tmp_path, case):
     """write_restart after 200 steps, read_restart into a fresh system built

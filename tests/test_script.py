@@ -192,6 +192,17 @@ def test_dump_commands_parse(tmp_path):
     assert out.exists()  # opened at the dump command, as the reference does
 
 
+def test_dump_custom_vtk_parse(tmp_path):
+    text = HEADER + (f"dump v1 all custom/vtk 100 {tmp_path}/post_*.vtk id type type x y z ix iy iz vx vy vz "
+                     "fx fy fz omegax omegay omegaz radius\n")
+    rc, log = run_script(tmp_path, text)
+    assert rc == 0, log
+    rc, log = run_script(tmp_path, HEADER + f"dump v1 all custom/vtk 100 {tmp_path}/post_*.vtp id x y z\n", "in.vtp")
+    assert rc != 0 and "only the legacy .vtk file format" in log
+    rc, log = run_script(tmp_path, HEADER + f"dump v1 all custom 100 {tmp_path}/a.txt id ix\n", "in.ix")
+    assert rc != 0 and "Invalid attribute ix" in log
+
+
 def test_read_restart_errors(tmp_path):
     bogus = tmp_path / "bogus.restart"
     bogus.write_bytes(b"not a restart file")
