@@ -434,9 +434,12 @@ __device__ __forceinline__ void mesh_contacts(const StepArgs &a, int i, const do
   int c = 0;
   for (int m = 0; m < p.nmesh; m++) {
     int *slot = a.aux.mslot + (size_t)m * kMeshSlots * cap + i;
-    T *hist = (T *)a.aux.mhist + (size_t)m * kMeshSlots * 3 * cap + i;
+    // a slot's history: the shear, then (hist_w 6) the epsd2 rolling torque
+    const size_t mhw = GEN ? (size_t)p.hist_w : 3;
+    T *hist = (T *)a.aux.mhist + (size_t)m * kMeshSlots * mhw * cap + i;
     const int tend = p.mesh_tri0[m] + p.mesh_ntri[m];
     const Model mdl = model_from_bits(p.mesh_model[m]);
+    const bool roll = GEN && mdl.epsd2;
     const int mt = irow + p.mesh_wtype[m];
     unsigned keep = 0u;
     for (; c < nc; c++) {
@@ -480,28 +483,31 @@ __device__ __forceinline__ void mesh_contacts(const StepArgs &a, int i, const do
         keep |= 1u << s;
         fresh = true;
       }
-      T sh[3];
+      T sh[3], rh[3] = {T(0), T(0), T(0)};
       if (fresh) {
         sh[0] = sh[1] = sh[2] = T(0);
         if (bs == 7)  // checkCoplanarContactHistory: the last coplanar slot wins
           for (int k = 0; k < kMeshSlots; k++) {
             const int st = slot[(size_t)k * cap];
-            if (st >= 0 && st != t && tri_coplanar(M, st, t)) {
-              sh[0] = hist[(size_t)(k * 3) * cap];
-              sh[1] = hist[(size_t)(k * 3 + 1) * cap];
-              sh[2] = hist[(size_t)(k * 3 + 2) * cap];
-            }
+            if (st >= 0 && st != t && tri_coplanar(M, st, t))
+              for (int d = 0; d < 3; d++) {
+                sh[d] = hist[(size_t)(k * mhw + d) * cap];
+                if (roll) rh[d] = hist[(size_t)(k * mhw + 3 + d) * cap];
+              }
           }
       } else {
-        sh[0] = hist[(size_t)(s * 3) * cap];
-        sh[1] = hist[(size_t)(s * 3 + 1) * cap];
-        sh[2] = hist[(size_t)(s * 3 + 2) * cap];
+        for (int d = 0; d < 3; d++) {
+          sh[d] = hist[(size_t)(s * mhw + d) * cap];
+          if (roll) rh[d] = hist[(size_t)(s * mhw + 3 + d) * cap];
+        }
       }
       // within (cdf - 1) r of the triangle but not touching: compute_force with
       // intersectflag false, whose surfacesClose zeroes the slot's shear
-      // (fix_wall_gran_base.h:283-289, tangential_model_history.h:358-369)
+      // (fix_wall_gran_base.h:283-289, tangential_model_history.h:358-369) and
+      // the epsd2 rolling torque (rolling_model_epsd2.h:144-151)
       const bool band = !(deltan <= 0);
       if (band && !mdl.nohist) sh[0] = sh[1] = sh[2] = T(0);
+      if (band && roll) rh[0] = rh[1] = rh[2] = T(0);
       double vw[3] = {0., 0., 0.};
       if (p.mesh_moving[m])
         for (int k = 0; k < 3; k++)
@@ -551,7 +557,6 @@ __device__ __forceinline__ void mesh_contacts(const StepArgs &a, int i, const do
           cc.acont = (T)((pi.w * pi.w - r * r) * kPi);
         }
         T dF[3], dTi[3], dTj[3];
-        T rh[3] = {T(0), T(0), T(0)};  // epsd2 is not taken on meshes
         contact_chain<T, true, GEN>(mdl, cc, (T)p.dt, a.shearupdate, sh, rh, dF, dTi, dTj);
         if (a.count && ((p.mesh_stress >> m) & 1)) {
           // MeshModuleStress::add_particle_contribution (mesh_module_stress.cpp:
@@ -576,11 +581,11 @@ __device__ __forceinline__ void mesh_contacts(const StepArgs &a, int i, const do
         tz += dTi[2];
         nwall++;
       }
-      if (a.shearupdate || fresh || band) {
-        hist[(size_t)(s * 3) * cap] = sh[0];
-        hist[(size_t)(s * 3 + 1) * cap] = sh[1];
-        hist[(size_t)(s * 3 + 2) * cap] = sh[2];
-      }
+      if (a.shearupdate || fresh || band)
+        for (int d = 0; d < 3; d++) {
+          hist[(size_t)(s * mhw + d) * cap] = sh[d];
+          if (mhw == 6) hist[(size_t)(s * mhw + 3 + d) * cap] = rh[d];
+        }
     }
     // cleanUpContacts: slots not touched this pass are freed
     for (int k = 0; k < kMeshSlots; k++)
@@ -1605,7 +1610,8 @@ __device__ int wall_local_rows(const StepArgs &a, const int i, const WallLocalAr
   for (int m = 0; m < p.nmesh; m++) {
     int slot[kMeshSlots];
     const int *gslot = a.aux.mslot + (size_t)m * kMeshSlots * cap + i;
-    const T *hist = (const T *)a.aux.mhist + (size_t)m * kMeshSlots * 3 * cap + i;
+    const size_t mhw = GEN ? (size_t)p.hist_w : 3;
+    const T *hist = (const T *)a.aux.mhist + (size_t)m * kMeshSlots * mhw * cap + i;
     for (int k = 0; k < kMeshSlots; k++) slot[k] = gslot[(size_t)k * cap];
     const int tend = p.mesh_tri0[m] + p.mesh_ntri[m];
     const Model mdl = model_from_bits(p.mesh_model[m]);
@@ -1613,7 +1619,7 @@ __device__ int wall_local_rows(const StepArgs &a, const int i, const WallLocalAr
     unsigned keep = 0u;
     // slots whose history the step's pass would rewrite, with their values
     unsigned over = 0u;
-    T lh[kMeshSlots][3];
+    T lh[kMeshSlots][GEN ? 6 : 3];
     for (; c < nc; c++) {
       const int t = M.cand[(size_t)c * cap + i];
       if (t >= tend) break;
@@ -1652,21 +1658,24 @@ __device__ int wall_local_rows(const StepArgs &a, const int i, const WallLocalAr
       }
       rows++;
       if (!FILL) continue;
-      T sh[3];
+      // the shear, then the epsd2 rolling torque
+      const int nh = (GEN && mdl.epsd2) ? 6 : 3;
+      T sh[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
       if (fresh) {
-        sh[0] = sh[1] = sh[2] = T(0);
         if (bs == 7)  // checkCoplanarContactHistory: the last coplanar slot wins
           for (int k = 0; k < kMeshSlots; k++)
             if (slot[k] >= 0 && slot[k] != t && tri_coplanar(M, slot[k], t))
-              for (int d = 0; d < 3; d++) sh[d] = ((over >> k) & 1u) ? lh[k][d] : hist[(size_t)(k * 3 + d) * cap];
+              for (int d = 0; d < nh; d++)
+                sh[d] = ((over >> k) & 1u) ? lh[k][d] : hist[(size_t)(k * mhw + d) * cap];
       } else {
-        for (int d = 0; d < 3; d++) sh[d] = ((over >> s) & 1u) ? lh[s][d] : hist[(size_t)(s * 3 + d) * cap];
+        for (int d = 0; d < nh; d++) sh[d] = ((over >> s) & 1u) ? lh[s][d] : hist[(size_t)(s * mhw + d) * cap];
       }
       const bool band = !(deltan <= 0);
       if (band && !mdl.nohist) sh[0] = sh[1] = sh[2] = T(0);
+      if (band) sh[3] = sh[4] = sh[5] = T(0);
       // the step's pass rewrites a fresh or close-band slot with these values
       if (fresh || band) {
-        for (int d = 0; d < 3; d++) lh[s][d] = sh[d];
+        for (int d = 0; d < nh; d++) lh[s][d] = sh[d];
         over |= 1u << s;
       }
       double vw[3] = {0., 0., 0.};
@@ -1719,8 +1728,7 @@ __device__ int wall_local_rows(const StepArgs &a, const int i, const WallLocalAr
           cc.acont = (T)((pi.w * pi.w - rr * rr) * kPi);
         }
         T dTj[3];
-        T rh[3] = {T(0), T(0), T(0)};
-        contact_chain<T, true, GEN>(mdl, cc, (T)p.dt, 0, sh, rh, dF, dTi, dTj);
+        contact_chain<T, true, GEN>(mdl, cc, (T)p.dt, 0, sh, sh + 3, dF, dTi, dTj);
       }
       double *rec = o.rec + (size_t)(r + rows - 1) * (size_t)o.ncol;
       // contact point x - sidata.delta (fix_wall_gran_base.h:211-212)
@@ -2011,8 +2019,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(RebuildArgs a) {
       const int st = a.acur.mslot[sb + o];
       a.aalt.mslot[sb + i] = st;
       if (st >= 0)
-        for (int d = 0; d < 3; d++) {
-          const size_t hb = (size_t)((m * kMeshSlots + k) * 3 + d) * cap;
+        for (int d = 0; d < a.prm.hist_w; d++) {
+          const size_t hb = (size_t)((m * kMeshSlots + k) * a.prm.hist_w + d) * cap;
           ho[hb + i] = hi[hb + o];
         }
     }
@@ -2421,14 +2429,14 @@ __global__ __launch_bounds__(kBlock) void k_tcell_scatter(RebuildArgs a, int npa
 }
 
 template <typename T>
-__device__ __forceinline__ void slot_swap(int *slot, T *hist, size_t cap, int x, int y) {
+__device__ __forceinline__ void slot_swap(int *slot, T *hist, size_t cap, int hw, int x, int y) {
   const int ts = slot[(size_t)x * cap];
   slot[(size_t)x * cap] = slot[(size_t)y * cap];
   slot[(size_t)y * cap] = ts;
-  for (int d = 0; d < 3; d++) {
-    const T h = hist[(size_t)(x * 3 + d) * cap];
-    hist[(size_t)(x * 3 + d) * cap] = hist[(size_t)(y * 3 + d) * cap];
-    hist[(size_t)(y * 3 + d) * cap] = h;
+  for (int d = 0; d < hw; d++) {
+    const T h = hist[(size_t)(x * hw + d) * cap];
+    hist[(size_t)(x * hw + d) * cap] = hist[(size_t)(y * hw + d) * cap];
+    hist[(size_t)(y * hw + d) * cap] = h;
   }
 }
 
@@ -2470,7 +2478,8 @@ __device__ int mesh_cand_one(const RebuildArgs &a, const int i) {
   // slots, mesh by mesh
   for (int m = 0; m < p.nmesh; m++) {
     int *slot = a.aalt.mslot + (size_t)m * kMeshSlots * cap + i;
-    T *hist = (T *)a.aalt.mhist + (size_t)m * kMeshSlots * 3 * cap + i;
+    const int hw = p.hist_w;
+    T *hist = (T *)a.aalt.mhist + (size_t)m * kMeshSlots * hw * cap + i;
     int first_empty, last_filled;
     do {
       first_empty = last_filled = -1;
@@ -2480,7 +2489,7 @@ __device__ int mesh_cand_one(const RebuildArgs &a, const int i) {
         if (st >= 0) last_filled = j;
       }
       if (first_empty > -1 && last_filled > -1 && first_empty < last_filled)
-        slot_swap(slot, hist, cap, first_empty, last_filled);
+        slot_swap(slot, hist, cap, hw, first_empty, last_filled);
     } while (first_empty > -1 && last_filled > -1 && first_empty < last_filled);
     int np = 0;
     for (int j = 0; j < kMeshSlots; j++) np += slot[(size_t)j * cap] >= 0;
@@ -2492,8 +2501,8 @@ __device__ int mesh_cand_one(const RebuildArgs &a, const int i) {
         if (a.cand[(size_t)k * cap + i] == st) inlist = true;
       if (!inlist) {
         slot[(size_t)ip * cap] = -1;
-        for (int d = 0; d < 3; d++) hist[(size_t)(ip * 3 + d) * cap] = T(0);
-        slot_swap(slot, hist, cap, ip, np - 1);
+        for (int d = 0; d < hw; d++) hist[(size_t)(ip * hw + d) * cap] = T(0);
+        slot_swap(slot, hist, cap, hw, ip, np - 1);
         np--;
       } else {
         ip++;

@@ -782,12 +782,13 @@ int type_bits(const int *type, const std::vector<int> &inst_of, int64_t i) {
   return type[i] + (inst_of.empty() ? 0 : (inst_of[i] << dem::kTypeBits));
 }
 
-// history values per pair / plane-wall contact: the shear (3), and the
+// history values per pair / plane-wall / mesh contact: the shear (3), and the
 // rolling torque beside it (3 more) when some model is epsd2. Models, walls
 // and meshes are fixed before the particles, which allocate the histories.
 int needed_hist_width(const dem_handle *h) {
   bool epsd2 = (model_bits(h->pair_model) & dem::kModelEpsd2) != 0;
   for (const Wall &w : h->walls) epsd2 = epsd2 || (w.model & dem::kModelEpsd2);
+  for (const auto &m : h->meshes) epsd2 = epsd2 || (m.model & dem::kModelEpsd2);
   return epsd2 ? 6 : 3;
 }
 
@@ -829,9 +830,6 @@ int check_ready(dem_handle *h, bool need_particles = true) {
     if ((b & dem::kModelEpsd2) && (b & 1))
       return fail(h, DEM_ERR_ARG, "rolling_friction: cdt and epsd2 are exclusive");
   }
-  for (const auto &m : h->meshes)
-    if (m.model & dem::kModelEpsd2)
-      return fail(h, DEM_ERR_ARG, "rolling_friction epsd2 is not available on mesh walls on this path");
   if (h->n > 0 && needed_hist_width(h) != h->hw)
     return fail(h, DEM_ERR_STATE, "contact history width out of date");
   return DEM_OK;
@@ -887,7 +885,7 @@ int grow_particles(dem_handle *h, int newcap) {
     cols.push_back({(void **)&h->aux[b].meta, sizeof(int), 1, 0, true});
     cols.push_back({&h->aux[b].whist, h->fsz, h->hw * nw, 0, true});
     cols.push_back({(void **)&h->aux[b].mslot, sizeof(int), nm * dem::kMeshSlots, 0xff, true});
-    cols.push_back({&h->aux[b].mhist, h->fsz, 3 * nm * dem::kMeshSlots, 0, true});
+    cols.push_back({&h->aux[b].mhist, h->fsz, h->hw * nm * dem::kMeshSlots, 0, true});
     cols.push_back({(void **)&h->lst[b].nbr, sizeof(uint32_t), h->W, 0, true});
     cols.push_back({&h->lst[b].hist, h->fsz * dem::hist_rec(h->hw), h->W, 0, true});
   }
@@ -1414,10 +1412,10 @@ int dem_get_mesh_history(dem_handle *h, int mesh_id, int maxslot, int *tri, doub
   for (int m = 0; m < mesh_id; m++) t0 += h->meshes[m].ntri;
   const dem::AuxBuf &a = h->aux[h->cur_aux];
   std::vector<int> slot((size_t)dem::kMeshSlots * cap), orig(n);
-  std::vector<char> raw(h->fsz * 3 * dem::kMeshSlots * cap);
+  std::vector<char> raw(h->fsz * h->hw * dem::kMeshSlots * cap);
   HIPCHK(h, hipMemcpy(slot.data(), a.mslot + (size_t)mesh_id * dem::kMeshSlots * cap,
                       sizeof(int) * slot.size(), hipMemcpyDeviceToHost));
-  HIPCHK(h, hipMemcpy(raw.data(), (char *)a.mhist + h->fsz * 3 * dem::kMeshSlots * cap * mesh_id,
+  HIPCHK(h, hipMemcpy(raw.data(), (char *)a.mhist + h->fsz * h->hw * dem::kMeshSlots * cap * mesh_id,
                       raw.size(), hipMemcpyDeviceToHost));
   HIPCHK(h, hipMemcpy(orig.data(), a.orig, sizeof(int) * n, hipMemcpyDeviceToHost));
   for (int64_t i = 0; i < n; i++) {
@@ -1429,7 +1427,7 @@ int dem_get_mesh_history(dem_handle *h, int mesh_id, int maxslot, int *tri, doub
       for (int d = 0; d < 3; d++) {
         double val = 0.0;
         if (st >= 0) {
-          const size_t idx = (size_t)(k * 3 + d) * cap + i;
+          const size_t idx = (size_t)(k * h->hw + d) * cap + i;
           val = h->mixed ? (double)((float *)raw.data())[idx] : ((double *)raw.data())[idx];
         }
         hist[((size_t)o * maxslot + k) * 3 + d] = val;
@@ -1497,7 +1495,7 @@ int dem_set_particles(dem_handle *h, int64_t n, const double *x, const double *v
   for (int b = 0; b < 2; b++) {
     if ((rc = dalloc(h, (void **)&h->aux[b].mslot, sizeof(int) * nm * dem::kMeshSlots * cap)))
       return rc;
-    if ((rc = dalloc(h, &h->aux[b].mhist, h->fsz * 3 * nm * dem::kMeshSlots * cap))) return rc;
+    if ((rc = dalloc(h, &h->aux[b].mhist, h->fsz * h->hw * nm * dem::kMeshSlots * cap))) return rc;
     HIPCHK(h, hipMemset(h->aux[b].mslot, 0xff, sizeof(int) * nm * dem::kMeshSlots * cap));
   }
   if (h->ntri > 0) {
@@ -2088,7 +2086,7 @@ std::vector<std::pair<void *, size_t>> restart_blobs(dem_handle *h) {
       {ax.meta, sizeof(int) * n},
       {ax.whist, h->fsz * h->hw * nw * cap},
       {ax.mslot, sizeof(int) * nm * dem::kMeshSlots * cap},
-      {ax.mhist, h->fsz * 3 * nm * dem::kMeshSlots * cap},
+      {ax.mhist, h->fsz * h->hw * nm * dem::kMeshSlots * cap},
       {ls.nbr, sizeof(uint32_t) * h->W * cap},
       {ls.hist, h->fsz * dem::hist_rec(h->hw) * h->W * cap},
       {h->xhold, sizeof(double) * 3 * cap},

@@ -485,6 +485,8 @@ int64_t orc_pairs_bruteforce(int n, const double *x, const double *radius,
  * ---------------------------------------------------------------------- */
 #define MESH_MAXNEIGH 5   /* SurfaceMesh<3,5> (tri_mesh.h) */
 #define MESH_MAXSLOT 16   /* history slots per particle */
+/* values per mesh history slot: the shear, then the epsd2 rolling torque */
+#define MESH_HW 6
 #define MESH_MAXCAND 128  /* candidate triangles per particle */
 #define SMALL_TRIMESH (1.e-10)  /* tri_mesh_I.h:47-50 */
 #define LARGE_TRIMESH 1000000
@@ -677,7 +679,7 @@ static void insert_pending(orc_sys *s) {
     m->ptri = (int *)grow_zero(m->ptri, sizeof(int) * MESH_MAXSLOT * n0, sizeof(int) * MESH_MAXSLOT * n1);
     for (size_t q = MESH_MAXSLOT * n0; q < MESH_MAXSLOT * n1; q++) m->ptri[q] = -1;
     m->keep = (unsigned char *)grow_zero(m->keep, MESH_MAXSLOT * n0, MESH_MAXSLOT * n1);
-    m->phist = (double *)grow_zero(m->phist, 3 * D * MESH_MAXSLOT * n0, 3 * D * MESH_MAXSLOT * n1);
+    m->phist = (double *)grow_zero(m->phist, MESH_HW * D * MESH_MAXSLOT * n0, MESH_HW * D * MESH_MAXSLOT * n1);
   }
   double rmax = s->rmax_future;
   for (size_t k = 0; k < (size_t)s->npend; k++) {
@@ -1674,8 +1676,6 @@ static void mesh_free(orc_mesh *m) {
 int orc_add_mesh(orc_sys *s, int ntri, const double *nodes, int wall_type,
                  const orc_model *model) {
   if (s->nmesh >= ORC_MAXMESH || ntri <= 0) return -1;
-  /* the mesh contact history here holds the shear only: epsd2 is not restated on meshes */
-  if (model->rolling_epsd2) return -1;
   orc_mesh *m = &s->mesh[s->nmesh];
   memset(m, 0, sizeof(*m));
   m->curvature = 1. - EPSILON_CURVATURE;
@@ -1729,7 +1729,7 @@ int orc_add_mesh(orc_sys *s, int ntri, const double *nodes, int wall_type,
   m->ptri = (int *)xcalloc(n * MESH_MAXSLOT, sizeof(int));
   for (size_t k = 0; k < n * MESH_MAXSLOT; k++) m->ptri[k] = -1;
   m->keep = (unsigned char *)xcalloc(n * MESH_MAXSLOT, 1);
-  m->phist = (double *)xcalloc(n * MESH_MAXSLOT * 3, sizeof(double));
+  m->phist = (double *)xcalloc(n * MESH_MAXSLOT * MESH_HW, sizeof(double));
   return s->nmesh++;
 }
 
@@ -1830,7 +1830,7 @@ int orc_mesh_history(const orc_sys *s, int imesh, int maxslot, int *tri, double 
       tri[(size_t)i * maxslot + k] = tt;
       for (int d = 0; d < 3; d++)
         hist[((size_t)i * maxslot + k) * 3 + d] =
-            (inside && tt >= 0) ? m->phist[((size_t)i * MESH_MAXSLOT + k) * 3 + d] : 0.0;
+            (inside && tt >= 0) ? m->phist[((size_t)i * MESH_MAXSLOT + k) * MESH_HW + d] : 0.0;
     }
   return MESH_MAXSLOT;
 }
@@ -2064,13 +2064,13 @@ static double tri_point_dist2(const double *a, const double *b, const double *c,
 /* per-particle slot arrays */
 #define SLOT(m, i, k) ((m)->ptri[(size_t)(i) * MESH_MAXSLOT + (k)])
 #define SKEEP(m, i, k) ((m)->keep[(size_t)(i) * MESH_MAXSLOT + (k)])
-#define SHIST(m, i, k) (&(m)->phist[((size_t)(i) * MESH_MAXSLOT + (k)) * 3])
+#define SHIST(m, i, k) (&(m)->phist[((size_t)(i) * MESH_MAXSLOT + (k)) * MESH_HW])
 
 static void slot_swap(orc_mesh *m, int i, int a, int b, int keepflag) {
   int t = SLOT(m, i, a);
   SLOT(m, i, a) = SLOT(m, i, b);
   SLOT(m, i, b) = t;
-  double h[3];
+  double h[MESH_HW];
   memcpy(h, SHIST(m, i, a), sizeof h);
   memcpy(SHIST(m, i, a), SHIST(m, i, b), sizeof h);
   memcpy(SHIST(m, i, b), h, sizeof h);
@@ -2124,7 +2124,7 @@ static void mesh_rebuild(orc_sys *s) {
           if (m->cand[(size_t)i * MESH_MAXCAND + k] == tt) inlist = 1;
         if (!inlist) {
           SLOT(m, i, ip) = -1;
-          memset(SHIST(m, i, ip), 0, 3 * sizeof(double));
+          memset(SHIST(m, i, ip), 0, MESH_HW * sizeof(double));
           slot_swap(m, i, ip, m->npartner[i] - 1, 0);
           m->npartner[i]--;
         } else {
@@ -2253,12 +2253,12 @@ static int mesh_handle_contact(orc_mesh *m, int i, int tri, double **hist, int f
   SKEEP(m, i, ic) = 1;
   m->npartner[i]++;
   *hist = SHIST(m, i, ic);
-  (*hist)[0] = (*hist)[1] = (*hist)[2] = 0.0;
+  memset(*hist, 0, MESH_HW * sizeof(double));
   if (faceflag)
     for (int k = 0; k < MESH_MAXSLOT; k++) {
       const int pt = SLOT(m, i, k);
       if (pt >= 0 && pt != tri && coplanar_node_neighs(m, pt, tri))
-        memcpy(*hist, SHIST(m, i, k), 3 * sizeof(double));
+        memcpy(*hist, SHIST(m, i, k), MESH_HW * sizeof(double));
     }
   return 1;
 }
@@ -2288,15 +2288,17 @@ static void mesh_wall_compute(orc_sys *s, int im, int shearupdate) {
         if (!(deltan <= 0)) {
           /* compute_force with intersectflag false (fix_wall_gran.cpp:987-1006,
            * fix_wall_gran_base.h:283-289): surfacesClose zeroes the slot's shear
-           * (tangential_model_history.h:358-369); no force */
+           * (tangential_model_history.h:358-369) and the epsd2 rolling torque
+           * (rolling_model_epsd2.h:144-151); no force */
           if (m->model.tangential_model == 0) h[0] = h[1] = h[2] = 0.0;
+          if (m->model.rolling_epsd2) h[3] = h[4] = h[5] = 0.0;
           continue;
         }
         const double sdelta[3] = {-delta[0], -delta[1], -delta[2]};
         double dF[3], dT[3];
-        orc_contact_wall(&m->model, &s->m, s->type[i], m->wall_type, -deltan, sdelta, s->r[i],
-                         &s->v[3 * i], &s->w[3 * i], vwall, s->rm[i], s->p.dt, shearupdate, h, dF,
-                         dT);
+        orc_contact_wall_r(&m->model, &s->m, s->type[i], m->wall_type, -deltan, sdelta, s->r[i],
+                           &s->v[3 * i], &s->w[3 * i], vwall, s->rm[i], s->p.dt, shearupdate, h, h + 3,
+                           dF, dT);
         double fold[3];
         for (int d = 0; d < 3; d++) {
           fold[d] = s->f[3 * i + d];
@@ -2326,7 +2328,7 @@ static void mesh_wall_compute(orc_sys *s, int im, int shearupdate) {
       if (!SKEEP(m, i, k)) {
         if (SLOT(m, i, k) > -1) m->npartner[i]--;
         SLOT(m, i, k) = -1;
-        memset(SHIST(m, i, k), 0, 3 * sizeof(double));
+        memset(SHIST(m, i, k), 0, MESH_HW * sizeof(double));
       }
   }
 }
@@ -2350,7 +2352,7 @@ int64_t orc_wall_local(const orc_sys *s, int64_t cap, int *ids, double *data) {
     for (int i = 0; i < s->n; i++) {
       int slot[MESH_MAXSLOT];
       unsigned char keep[MESH_MAXSLOT];
-      double hist[MESH_MAXSLOT][3];
+      double hist[MESH_MAXSLOT][MESH_HW];
       for (int k = 0; k < MESH_MAXSLOT; k++) {
         slot[k] = SLOT(m, i, k);
         keep[k] = 0;
@@ -2386,7 +2388,7 @@ int64_t orc_wall_local(const orc_sys *s, int64_t cap, int *ids, double *data) {
           if (ic < 0) abort();
           slot[ic] = t;
           keep[ic] = 1;
-          hist[ic][0] = hist[ic][1] = hist[ic][2] = 0.0;
+          memset(hist[ic], 0, sizeof hist[ic]);
           if (7 == barysign)
             for (int k = 0; k < MESH_MAXSLOT; k++)
               if (slot[k] >= 0 && slot[k] != t && coplanar_node_neighs(m, slot[k], t))
@@ -2401,10 +2403,11 @@ int64_t orc_wall_local(const orc_sys *s, int64_t cap, int *ids, double *data) {
         double dF[3] = {0., 0., 0.}, dT[3] = {0., 0., 0.};
         if (!(deltan <= 0)) {
           if (m->model.tangential_model == 0) h[0] = h[1] = h[2] = 0.0;
+          if (m->model.rolling_epsd2) h[3] = h[4] = h[5] = 0.0;
         } else {
           const double sdelta[3] = {-delta[0], -delta[1], -delta[2]};
-          orc_contact_wall(&m->model, &s->m, s->type[i], m->wall_type, -deltan, sdelta, s->r[i],
-                           &s->v[3 * i], &s->w[3 * i], vwall, s->rm[i], s->p.dt, 0, h, dF, dT);
+          orc_contact_wall_r(&m->model, &s->m, s->type[i], m->wall_type, -deltan, sdelta, s->r[i],
+                             &s->v[3 * i], &s->w[3 * i], vwall, s->rm[i], s->p.dt, 0, h, h + 3, dF, dT);
         }
         if (nr < cap) {
           double *row = data + nr * NC;

@@ -204,6 +204,37 @@ def test_mesh_wall_with_its_own_model_fp64():
     o.close()
 
 
+@pytest.mark.parametrize("prec", [pydem.DEM_FP64, pydem.DEM_MIXED])
+def test_mesh_wall_epsd2(prec):
+    """rolling_friction epsd2 on a triangle-mesh floor (the rolling torque kept
+    in the mesh contact slots beside the shear) under an epsd2 pair style: 40
+    steps against the oracle, state and mesh shear histories 1e-9 relative in
+    FP64, one step at the mixed tolerance."""
+    say(f"[gpu] epsd2 on a mesh floor, precision {prec}")
+    params, mats, P, _ = wl.dense_pack(seed=42, n=2000, rolling=False, periodic=(0, 0, 0), speed=0.2)
+    params = dict(params, rolling_epsd2=1)
+    lo, hi = params["boxlo"], params["boxhi"]
+    floor = msh.grid(lo[0] - 0.002, hi[0] + 0.002, lo[1] - 0.002, hi[1] + 0.002, lo[2], 7, 6)
+    mats = wl.materials_two(rolling_friction=0.2)
+    meshes = [(floor, 2, {"rolling_epsd2": 1}, None)]
+    g, o = build(params, mats, P, [], precision=prec, meshes=meshes)
+    fp64 = prec == pydem.DEM_FP64
+    nsteps = 40 if fp64 else 1
+    g.run(nsteps)
+    o.run(nsteps)
+    tol = 1e-9 if fp64 else MIXED_TOL
+    gs, os_ = g.state(), o.state()
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert_close(k, gs[k], os_[k], tol)
+    gt, gh = g.mesh_history()
+    ot, oh = o.mesh_history()
+    assert np.array_equal(gt[:, :8], ot[:, :8])
+    assert (ot >= 0).sum() > 50
+    assert np.abs(gh[:, :8] - oh[:, :8]).max() <= tol * max(np.abs(oh).max(), 1e-300)
+    g.close()
+    o.close()
+
+
 @pytest.mark.parametrize("model,mats_fn,msg", [
     ({"normal": "hertz/stiffness"}, lambda m: m, "needs kn"),
     ({"normal": "hooke"}, lambda m: m, "characteristicVelocity"),

@@ -12,6 +12,8 @@ against what the model's mechanics fixes independently of the oracle's code:
               contact (newtonflag 1) negates it and gets the opposite torque
   wall        the plane contact spins at (R - delta/2)/(R - delta) omega, the
               surface rate of the sphere, with R* = R
+  mesh        a tessellated flat floor gives the plane's state and histories
+              while the spheres roll across its triangles' edges
 """
 import math
 
@@ -123,12 +125,42 @@ def test_wall_rolls_at_the_surface_rate():
     np.testing.assert_allclose(dT - dT0, -rt, rtol=1e-9, atol=1e-25)
 
 
-def test_epsd2_refused_on_meshes():
-    tri = np.array([[[0, 0, 0], [1, 0, 0], [0, 1, 0]]], dtype=float)
-    P = {"x": np.array([[0.5, 0.5, 0.5]]), "v": np.zeros((1, 3)), "omega": np.zeros((1, 3)),
-         "radius": np.array([R]), "rmass": np.array([MASS]), "type": np.array([1]),
-         "tag": np.array([1])}
-    params = {"dt": 1e-6, "skin": 1e-4, "boxlo": (-1, -1, -1), "boxhi": (2, 2, 2),
-              "periodic": (0, 0, 0)}
-    with pytest.raises(Exception):
-        orc.OracleSystem(params, MATS, P, meshes=[(tri, 1, EPSD2, None)])
+def test_epsd2_on_a_flat_mesh_equals_the_plane():
+    """A tessellated flat floor with rolling_friction epsd2 is the zplane wall
+    at the same height: spheres pressed onto it and rolling across triangle
+    edges (the slot history handed to the coplanar neighbour) keep the same
+    state, shear and rolling torque as on the plane, to rounding."""
+    import meshes as msh
+    rng = np.random.default_rng(5)
+    n = 9
+    xy = np.stack(np.meshgrid(np.linspace(0.004, 0.016, 3), np.linspace(0.004, 0.016, 3)), -1).reshape(-1, 2)
+    P = {"x": np.column_stack([xy, np.full(n, R - 2.7e-6)]),  # the static overlap under gravity
+         "v": np.column_stack([rng.normal(0, 1.0, (n, 2)), np.zeros(n)]),
+         "omega": rng.normal(0, 200.0, (n, 3)),
+         "radius": np.full(n, R), "rmass": np.full(n, MASS), "type": np.ones(n, dtype=np.int32),
+         "tag": np.arange(1, n + 1, dtype=np.int32)}
+    params = {"dt": 1e-6, "skin": 2e-4, "boxlo": (-0.01, -0.01, -0.01), "boxhi": (0.03, 0.03, 0.03),
+              "periodic": (0, 0, 0), "grav_acc": (0.0, 0.0, -9.81), "gravity_first": 1}
+    floor = msh.grid(-0.01, 0.03, -0.01, 0.03, 0.0, 8, 8)
+    a = orc.OracleSystem(params, MATS, P, walls=[(2, 0.0, 1, EPSD2)])
+    b = orc.OracleSystem(params, MATS, P, meshes=[(floor, 1, EPSD2, None)])
+    a.run(2000)
+    b.run(2000)
+    sa, sb = a.state(), b.state()
+    for k in ("x", "v", "omega", "f", "torque"):
+        scale = np.abs(sa[k]).max()
+        assert np.abs(sa[k] - sb[k]).max() <= 1e-10 * scale, k
+    # the spheres crossed triangle edges, and the rolling torque is not small
+    assert np.abs(sa["x"][:, :2] - P["x"][:, :2]).max() > 0.001
+    tri, hist = b.mesh_history(0)
+    wh = a.wall_history()
+    touching = 0
+    for i in range(n):
+        occ = tri[i] >= 0
+        assert occ.sum() <= 1
+        if occ.sum():
+            touching += 1
+            np.testing.assert_allclose(hist[i][occ][0], wh[i, 0], rtol=1e-8, atol=1e-14)
+    assert touching >= 5
+    a.close()
+    b.close()
