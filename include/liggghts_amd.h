@@ -419,12 +419,15 @@ int dem_set_instance_materials(dem_handle *h, int inst, int ntypes, const double
  * wall (plane or mesh) contacts at the last step of the last run. */
 int dem_get_instance_stats(dem_handle *h, int ninst, int64_t *ncontacts, int64_t *nwall);
 
-/* Tile shape of the fused step kernel: variant 0..4 (threads per block /
- * neighbor entries loaded together: 256/4, 256/2, 256/8, 128/4, 64/4) and a
- * persistent grid of `persist` blocks per compute unit (0: one block per
- * tile). variant = -1 asks for autotuning: the next run spends its first
- * batches timing every (variant, persist in {0, 4, 8}) candidate, 8 steps
- * each, and keeps the fastest. Every shape gives bitwise the same results.
+/* Tile shape of the fused step kernel: variant 0..10 (threads per block /
+ * neighbor entries loaded together: 0-4 per-lane pair phase 256/4, 256/2,
+ * 256/8, 128/4, 64/4; 5-6 compacted group by group over the wavefront 256/4,
+ * 256/8; 7-8 queued over whole list rows 256/4, 256/8; 9-10 per lane with the
+ * histories loaded in the broad phase 256/4, 128/4) and a persistent grid of
+ * `persist` blocks per compute unit (0: one block per tile). variant = -1
+ * asks for autotuning: the next runs spend their first batches timing every
+ * (variant, persist in {0, 4, 8}) candidate, 8 steps each, in three passes,
+ * and keep the fastest. Every shape gives bitwise the same results.
  * No reference counterpart (the reference has one CPU loop). */
 int dem_set_step_shape(dem_handle *h, int variant, int persist);
 

@@ -155,7 +155,7 @@ struct AuxBuf {
   int *meta;      // list length | wall-history bits
   void *whist;    // wall history, [wall][hist_w][cap], double or float
   int *mslot;     // mesh history slots: triangle index or -1, [mesh][slot][cap]
-  void *mhist;    // mesh shear history, [mesh][slot][3][cap], double or float
+  void *mhist;    // mesh history, [mesh][slot][hist_w][cap] (shear, then the epsd2 rolling torque)
 };
 
 // Mesh data used by the step kernel.
